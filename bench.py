@@ -1,0 +1,171 @@
+#!/usr/bin/env python3
+"""Benchmark of the overlap-force path on one MI355X.
+
+Workload (N=1): N=4096 polygons, 16 vertices each, bidisperse circumradii 1.0/1.4,
++-5% radial noise, packing fraction 0.85, periodic unit box, cell side = largest
+polygon diameter, FP64.  One "step" = one force evaluation through the C ABI
+(pcp_force_eval: geometry, cell list, edge intersections, records + pairing,
+exterior/interior forces, projection) -- the unit behind "force evals/sec".
+Positions stay resident in HBM; the timed region contains only device work.
+
+Alongside: a live roofline of one kernel (HIP events on the context's stream around
+every launch of that kernel inside the timed region), the CPU oracle timed on the
+host cores on a bounded sample of the same workload, and a bounded FIRE leg that
+reports FIRE steps/s and, when the packing converges to max|F| <= 1e-10, packings/s.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "packings minimized/sec to |F|max<1e-10 (learned vs plain); force evals/sec"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+# FP64 operations of one segment test in k_detect (2 minimum-image wraps of 5 ops,
+# cross product 3, |denom| test 1, two parameters 4+4, four range compares 4)
+FLOPS_PER_PAIR_TEST = 21
+# bytes k_detect must move from HBM per launch: staged edge data of every edge
+# (double2 position, double2 direction, int4 ids = 48 B), its count (4 B), and per
+# crossing one slot (key 8 B, float2 t/u 8 B, inside flag 1 B)
+BYTES_PER_EDGE = 52
+BYTES_PER_CROSSING = 17
+
+
+def pair_tests(cell_counts, B):
+    """Segment tests with nid > id1 over the 3x3 stencil (cell-count estimate: each
+    unordered pair within the stencil once)."""
+    n = cell_counts.reshape(B, B).astype(np.int64)
+    S = np.zeros_like(n)
+    shifts = sorted({(dx % B, dy % B) for dx in (-1, 0, 1) for dy in (-1, 0, 1)})
+    for dx, dy in shifts:
+        S += np.roll(np.roll(n, dx, axis=0), dy, axis=1)
+    return int((n * S).sum() - n.sum()) // 2
+
+
+def cpu_baseline(p, B, budget_s):
+    from oracle import oracle as O
+    sysO = O.System(p["startIndices"], B, "normal", p["targetEdgeV"], p["targetAreas"])
+    sysO.force_eval(p["positions"])   # warm-up (page-in, thread pool)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        sysO.force_eval(p["positions"])
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or n >= 1000:
+            break
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=n / el, unit="force evals/s", cores=threads, kind="port",
+                sample=f"{n} force evaluations of the same N=4096 workload ({el:.1f} s), "
+                       f"oracle/pcp_oracle.c with OpenMP ({threads} threads)")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cells", choices=["diameter", "edge"], default="diameter")
+    ap.add_argument("--roofline-kernel", default="k_detect")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--fire-seconds", type=float, default=20.0)
+    ap.add_argument("--no-fire", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; the multi-GPU design is described in DESIGN.md "
+              "(--gpus must be 1)", file=sys.stderr)
+        sys.exit(2)
+
+    from pycudapolygon_amd import capi
+    from pycudapolygon_amd.generate import make_packing, box_size_for
+
+    p = make_packing(4096, 16, 0.85, seed=args.seed)
+    B = box_size_for(p["diameter"] if args.cells == "diameter" else p["maxEdgeLength"])
+    s = capi.System([p], [B])
+    print(f"[bench] N=4096 n=16 phi=0.85 B={B} V={s.V}", flush=True)
+
+    for _ in range(args.warmup):
+        s.force_eval()
+    s.sync()
+    nI = s.get_records(0)["numIntersections"]
+    cl, ni, cpb = s.get_cells(0)
+    counts = np.bincount(cl, minlength=B * B)
+    tests = pair_tests(counts, B)
+
+    s.ctx.call("pcp_profile_kernel", args.roofline_kernel.encode())
+    s.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.force_eval()
+    s.sync()
+    el = time.perf_counter() - t0
+    ms = ctypes.c_double(0.0)
+    nl = ctypes.c_int(0)
+    s.ctx.call("pcp_profile_read", ctypes.byref(ms), ctypes.byref(nl))
+    s.ctx.call("pcp_profile_kernel", None)
+    evals_per_s = args.steps / el
+    kavg_s = (ms.value / max(nl.value, 1)) * 1e-3
+    print(f"[bench] {args.steps} evals in {el:.3f} s; {args.roofline_kernel}: {nl.value} launches, "
+          f"avg {kavg_s * 1e6:.2f} us; records {nI}; pair tests {tests}", flush=True)
+
+    if args.roofline_kernel == "k_detect":
+        flops = FLOPS_PER_PAIR_TEST * tests
+        nbytes = BYTES_PER_EDGE * s.V + BYTES_PER_CROSSING * nI
+        achieved = flops / kavg_s / 1e12
+        roof = dict(bound="mfma", compute="fp64 VALU (no MFMA in the geometry)", achieved=achieved,
+                    peak=FP64_PEAK_TFLOPS, unit="TFLOP/s", frac=achieved / FP64_PEAK_TFLOPS, traffic=None,
+                    kernel="k_detect", flops_per_launch=flops, algorithmic_bytes_per_launch=nbytes,
+                    avg_launch_us=kavg_s * 1e6, hbm_gbs_algorithmic=nbytes / kavg_s / 1e9)
+    else:
+        roof = dict(kernel=args.roofline_kernel, avg_launch_us=kavg_s * 1e6, achieved=None, peak=None,
+                    unit=None, frac=None, traffic=None, bound=None)
+
+    fire = None
+    if not args.no_fire:
+        # FIRE leg (bounded): area energy (the reference convention's energy is not the
+        # potential of its forces and FIRE stalls)
+        sf = capi.System([p], [B], model=capi.NORMAL | 0x100)
+        prm = sf.fire_params(fTol=1e-10)
+        sf.fire_init(prm, 1e-3)
+        t1, steps, st = time.perf_counter(), 0, None
+        while time.perf_counter() - t1 < args.fire_seconds:
+            st = sf.fire_step(prm)[0]
+            steps += 1
+            if st.converged:
+                break
+        fel = time.perf_counter() - t1
+        fire = dict(energy="area", steps=steps, seconds=fel, steps_per_s=steps / fel,
+                    converged=bool(st.converged), final_energy=st.energy, final_max_force=st.maxForce,
+                    packings_per_s=(1.0 / fel) if st.converged else 0.0)
+        print(f"[bench] FIRE: {steps} steps in {fel:.1f} s, converged={bool(st.converged)}, "
+              f"E={st.energy:.3e}, max|F|={st.maxForce:.3e}", flush=True)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(p, B, args.cpu_seconds)
+        print(f"[bench] CPU oracle: {cpu['value']:.2f} evals/s on {cpu['cores']} threads", flush=True)
+
+    out = {
+        "metric": METRIC, "value": evals_per_s, "unit": "force evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "N=4096 polygons x 16 vertices, bidisperse 1.0/1.4, +-5% radial noise, "
+                               "phi=0.85, periodic box, cell side = max polygon diameter, FP64; "
+                               "step = one force evaluation", "seed": args.seed, "box_cells": B,
+                   "records": int(nI)},
+        "roofline": roof, "cpu_baseline": cpu, "fire": fire,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,220 @@
+/*
+ * pcp.h -- C ABI of the MI355X (gfx950) overlap-force-minimization library
+ * (libpcp_hip.so).  Plain pointers and sizes; no torch or HIP types in the
+ * signatures.  Device pointers are marked "device", host pointers "host".
+ *
+ * Each entry point replaces one or more of the reference's extern "C"
+ * launchers (camdennis/pyCudaPolygon src/model.cpp:17-56 <-> src/model.cu);
+ * the replaced interface is cited at each declaration.  All entry points
+ * return 0 on success and a nonzero status otherwise (text from
+ * pcp_last_error()); the reference instead prints and exit()s
+ * (src/cuda_check.h:9-38).
+ *
+ * Batching: one context holds P independent packings, each in its own
+ * periodic unit box.  Polygons are numbered globally (packing-major) and
+ * vertices globally (polygon-major); record keys use packing-local ids exactly
+ * like the reference's single packing (src/kernels.cuh:860-874), so a context
+ * with P = 1 is the reference's Model.  Positions and forces are AoS
+ * double[2V] ([x0,y0,x1,y1,...], src/model.cpp:522-528).
+ *
+ * Ordering/threading: all work of a context is queued on that context's own
+ * HIP stream; entry points that return host values synchronize it.  A
+ * context is not thread-safe (as the reference Model, src/model.hpp:18).
+ */
+#ifndef PCP_H
+#define PCP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PCP_OK 0
+#define PCP_ERR_HIP 1
+#define PCP_ERR_ARG 2
+#define PCP_ERR_CAPACITY 3
+
+/* model types, ordinal order of src/enumTypes.h:6 */
+enum { PCP_NORMAL = 0, PCP_EDGE_ONLY = 1, PCP_AREA_ONLY = 2, PCP_SOFT_BODY = 3,
+       PCP_ABNORMAL = 4, PCP_HYBRID = 5 };
+/* Flag OR-ed into a model type (no counterpart in the reference): "area energy".
+ * The reference measures each record's chain from its own entry point with float-
+ * rounded crossing parameters (kernels.cuh:1117-1120, 857-858), so its energy is
+ * overlap area + sum (Qx-Px)(Qy-Py) over lenses and its force is not the gradient of
+ * that energy; FIRE's energy test (model.cpp:453) then stalls.  With this flag the
+ * crossing parameters are recomputed exactly at the current positions (pairing
+ * included), every record of a closed partner cycle measures from one origin per
+ * polygon pair, and forces are halved: energy = overlap area, force = -grad(energy). */
+#define PCP_AREA_ENERGY 0x100
+
+typedef struct pcp_ctx pcp_ctx;
+
+const char* pcp_last_error(void);
+int pcp_device_count(int* count);
+
+/* ---- device memory plumbing (replaces the cudaMalloc/cudaMemcpy calls in
+ *      src/model.cpp:60-101, 522-628 and the getters :680-889) ---- */
+int pcp_malloc(void** ptr, size_t bytes);
+int pcp_free(void* ptr);
+int pcp_memcpy_h2d(void* dst_device, const void* src_host, size_t bytes);
+int pcp_memcpy_d2h(void* dst_host, const void* src_device, size_t bytes);
+int pcp_memcpy_d2d(void* dst_device, const void* src_device, size_t bytes);
+int pcp_memset(void* ptr_device, int value, size_t bytes);
+
+/* ---- context ---- */
+int pcp_create(pcp_ctx** ctx);
+int pcp_destroy(pcp_ctx* ctx);
+int pcp_synchronize(pcp_ctx* ctx);
+
+/* Topology.  packPolyStart host[P+1] (polygon offsets per packing),
+ * polyStart host[N+1] (global vertex offsets per polygon; packing p owns
+ * vertices polyStart[packPolyStart[p]] .. polyStart[packPolyStart[p+1]]).
+ * Replaces setStartIndices model.cpp:537-598 + updateShapeIdCUDA model.cu:387-391
+ * + computeNextPrevCUDA model.cu:77-81 (shapeId/next/prev built on the device). */
+int pcp_set_topology(pcp_ctx* ctx, int numPackings, const int* packPolyStart, const int* polyStart);
+/* Cells per side for each packing, host[P] (setMaxEdgeLength model.cpp:513-516 +
+ * the cell-list allocations of initializeNeighborCells model.cpp:176-229). */
+int pcp_set_box_sizes(pcp_ctx* ctx, const int* boxSize);
+/* Reference neighbour-slot count M used by the neighbour-list getters
+ * (model.hpp:129 maxNeighbors; grown like model.cpp:264-305). */
+int pcp_set_max_neighbors(pcp_ctx* ctx, int maxNeighbors);
+int pcp_get_max_neighbors(pcp_ctx* ctx, int* maxNeighbors);
+
+/* ---- geometry: updatePolygonGeometryCUDA model.cu:113-163 ----
+ * edgeLengths device[V], areas/comX/comY device[N], constraints device[6V] or NULL,
+ * maxEdgeLength device[P] or NULL. */
+int pcp_update_geometry(pcp_ctx* ctx, const double* pos, double* edgeLengths, double* areas,
+                        double* comX, double* comY, double* constraints, double* maxEdgeLength);
+
+/* ---- cell list: updateNeighborCellsCUDA model.cu:308-385 (counting sort by the
+ *      cell hash of each edge midpoint; ascending vertex order inside a cell) ---- */
+int pcp_update_cells(pcp_ctx* ctx, const double* pos);
+
+/* ---- edge intersections: updateNeighborsCUDA model.cu:393-411 + maxReduceKernel.
+ *      Writes per-edge crossing counts and the crossing records.  maxCount host[1]
+ *      (optional; synchronizes) receives the largest per-edge count. ---- */
+int pcp_update_neighbors(pcp_ctx* ctx, const double* pos, int* maxCount);
+
+/* ---- records: updateValidAndCountsCUDA model.cu:413-450,
+ *      updateCompactedIntersectionsCUDA :452-464, sortKeysCUDA :50-75 (64 then 48
+ *      bits), applyPermutationCUDA_* :32-48, updateOutersectionsCUDA :509-517,
+ *      as orchestrated by Model::updateOutersections model.cpp:351-361.
+ *      numIntersections host[P] (optional; synchronizes). ---- */
+int pcp_update_records(pcp_ctx* ctx, const double* pairPos, int* numIntersections);
+/* pairPos: NULL = the reference's float-rounded pairing; device[2V] positions = exact
+ * crossing parameters at those positions (area energy) */
+
+/* ---- forces: Model::updateForceEnergy model.cpp:363-395 (exterior model.cu:519-527,
+ *      shape ranges :529-541, interior :543-551, edge :553-559, area :561-567).
+ *      targetEdgeV device[V]: target length of edge m -> next(m);
+ *      force device[2V] (overwritten), energy device[P]. ---- */
+int pcp_update_force_energy(pcp_ctx* ctx, int model, const double* pos, const double* edgeLengths,
+                            const double* areas, const double* targetEdgeV,
+                            const double* targetAreas, double stiffness, double compressibility,
+                            double* force, double* energy);
+
+/* ---- projection: projectForceCUDA model.cu:165-206 (removes from each polygon's
+ *      force the components along its n edge-length gradients and area gradient) ---- */
+int pcp_project_force(pcp_ctx* ctx, const double* pos, double* force);
+
+/* ---- SHAKE: shakeProjectCUDA model.cu:619-636 (kernel absent from the reference).
+ *      iters host[P] (optional; synchronizes): largest iteration count. ---- */
+int pcp_shake(pcp_ctx* ctx, double* pos, const double* targetEdgeV, const double* targetAreas,
+              int maxIter, double tol, int* iters);
+
+/* ---- XPBD: xpbdProjectCUDA model.cu:209-278.  targetEdgeP device[N] per polygon.
+ *      Returns the iteration count in iters host[1]. ---- */
+int pcp_xpbd(pcp_ctx* ctx, double* pos, const double* targetEdgeP, const double* targetAreas,
+             int nIter, double tol, int* iters);
+
+/* ---- reductions / small updates ---- */
+/* getMaxUnbalancedForceCUDA model.cu:612-614: max |component| per packing, out device[P] */
+int pcp_max_abs(pcp_ctx* ctx, const double* x, double* out);
+/* updatePositionsCUDA model.cu:569-574: x <- frac(x + dt*F) */
+int pcp_update_positions(pcp_ctx* ctx, double* pos, const double* force, double dt);
+/* resetAreasCUDA model.cu:639-645 */
+int pcp_reset_areas(pcp_ctx* ctx, double* pos, const double* areas, const double* targetAreas,
+                    const double* comX, const double* comY);
+/* saveTentativePositionsCUDA + getMaxEffectiveForceCUDA model.cu:280-306: out host[P] */
+int pcp_max_effective_force(pcp_ctx* ctx, const double* pos, const double* tentPos,
+                            const double* force, double scale, double* out);
+/* updateOverlapAreaCUDA model.cu:466-507 (grid estimator): counter device[D*D] or NULL,
+ * sum host[P] = sum of k(k-1)/2 over the D*D points of each packing */
+int pcp_overlap_count(pcp_ctx* ctx, const double* pos, int pointDensity, int* counter, long long* sum);
+
+/* ---- read-back of the context's internal arrays for one packing (the getters
+ *      getNeighborCells/getNeighborIndices/getBoxCounts/getNumNeighbors/
+ *      getNeighbors/getInsideFlag/getIntersections/getTU/getOutersections/getUT/
+ *      getShapeCounts, model.cpp:704-873).  All outputs are host arrays; any may
+ *      be NULL.  Vertex/polygon ids are packing-local, as in the reference. ---- */
+int pcp_get_cells(pcp_ctx* ctx, int packing, int* cellLocation, int* neighborIndices, int* countPerBox);
+int pcp_get_num_intersections(pcp_ctx* ctx, int packing, int* count);
+int pcp_get_records(pcp_ctx* ctx, int packing, uint64_t* intersections, double* tu,
+                    uint64_t* outersections, double* ut, int* shapeCounts);
+/* neighbors host[V_p*M], inside host[V_p*M], numNeighbors host[V_p] in the
+ * reference's slot order (unused slots: -1 / 0) */
+int pcp_get_neighbor_lists(pcp_ctx* ctx, int packing, int* neighbors, unsigned char* inside,
+                           int* numNeighbors);
+/* records in compaction order (edge-major, slot order) as left by
+ * updateCompactedIntersections before sorting (model.cpp:347-349) */
+int pcp_get_compacted_records(pcp_ctx* ctx, int packing, uint64_t* intersections, double* tu);
+/* stable sort of n keys by bits [0, endBit) on the device: sortKeysCUDA model.cu:50-75
+ * (host in/out: keys host[n], sorted host[n], perm host[n]) */
+int pcp_sort_keys(const uint64_t* keys, int n, int endBit, uint64_t* sorted, uint32_t* perm);
+
+/* ---- one force evaluation (the unit of "force evals/s"): geometry, cell list,
+ *      intersections, records, forces, projection (normal model) -- the sequence of
+ *      Model::minimizeFIREStep model.cpp:414-443 without the integrator. ---- */
+typedef struct {
+    double* pos;                /* device [2V] */
+    double* vel;                /* device [2V] */
+    double* force;              /* device [2V] */
+    const double* targetEdgeV;  /* device [V]  */
+    const double* targetAreas;  /* device [N]  */
+    double* edgeLengths;        /* device [V]  */
+    double* areas;              /* device [N]  */
+    double* energy;             /* device [P]  */
+    double stiffness, compressibility;
+    int model;
+} pcp_state;
+int pcp_force_eval(pcp_ctx* ctx, const pcp_state* st);
+
+/* ---- FIRE: Model::minimizeFIREStep / Model::minimizeFIRE model.cpp:405-504 and the
+ *      launchers model.cu:576-608 (kernels absent from the reference).  State per
+ *      packing (dt, alpha, nPos, energy, steps, convergence) stays on the device. ---- */
+typedef struct {
+    double dtMax, alphaStart, fAlpha, fInc, fDec;
+    int nMin, shakeIter;
+    double fTol;        /* convergence: max |force component| <= fTol */
+} pcp_fire_params;
+typedef struct {
+    double dt, alpha, energy, maxForce;
+    int nPos, steps, converged, lastShakeIters, shakeFailed, rolledBack;
+} pcp_fire_status;
+/* resetVelocities + fresh force evaluation (minimizeFIRE prologue model.cpp:489-496);
+ * dt0 host[P] or NULL (all = dt0s), alpha = alphaStart */
+int pcp_fire_init(pcp_ctx* ctx, const pcp_state* st, const pcp_fire_params* prm, double dt0);
+/* one FIRE step for every not-yet-converged packing; status host[P] (optional) */
+int pcp_fire_step(pcp_ctx* ctx, const pcp_state* st, const pcp_fire_params* prm, pcp_fire_status* status);
+/* FIRE until every packing has max|F| <= fTol or maxSteps steps; status host[P] */
+int pcp_fire_minimize(pcp_ctx* ctx, const pcp_state* st, const pcp_fire_params* prm, int maxSteps,
+                      pcp_fire_status* status);
+/* set the per-packing FIRE scalars (host[P] arrays, NULL = leave) and mark every packing
+ * active with a zero step count; read them back */
+int pcp_fire_set(pcp_ctx* ctx, const double* dt, const double* alpha, const int* nPos);
+int pcp_fire_get(pcp_ctx* ctx, pcp_fire_status* status);
+
+/* ---- measurement support (no counterpart in the reference) ----
+ * pcp_profile_kernel: bracket every later launch of the kernel called `name` (e.g.
+ * "k_detect") with HIP events on the context's stream; NULL stops.  pcp_profile_read:
+ * total milliseconds and launch count since the last pcp_profile_kernel call.
+ * pcp_stream_copy: plain device copy kernel of n doubles (HBM counter calibration). */
+int pcp_profile_kernel(pcp_ctx* ctx, const char* name);
+int pcp_profile_read(pcp_ctx* ctx, double* totalMs, int* launches);
+int pcp_stream_copy(pcp_ctx* ctx, double* dst, const double* src, long long n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,265 @@
+"""numpy front end of the CPU oracle (oracle/pcp_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.  Each function
+forwards to the C restatement of the reference kernel it names.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+
+NORMAL, EDGE_ONLY, AREA_ONLY, SOFT_BODY, ABNORMAL, HYBRID = range(6)
+MODEL_IDS = {"normal": NORMAL, "edgeOnly": EDGE_ONLY, "areaOnly": AREA_ONLY,
+             "softBody": SOFT_BODY, "abnormal": ABNORMAL, "hybrid": HYBRID}
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_i = ctypes.c_int
+_d = ctypes.c_double
+_p = ctypes.c_void_p
+
+
+class _System(ctypes.Structure):
+    _fields_ = [("V", _i), ("N", _i), ("boxSize", _i), ("model", _i), ("maxNeighbors", _i),
+                ("start", _p), ("targetEdgeV", _p), ("targetAreas", _p),
+                ("stiffness", _d), ("compressibility", _d)]
+
+
+class FireState(ctypes.Structure):
+    _fields_ = [("dt", _d), ("alpha", _d), ("nPos", _i), ("energy", _d),
+                ("lastShakeIters", _i), ("shakeFailed", _i), ("rolledBack", _i)]
+
+
+def _declare(L):
+    L.orc_wrap.restype = _d
+    L.orc_wrap.argtypes = [_d]
+    L.orc_topology.argtypes = [_i, _p, _i, _p, _p, _p]
+    L.orc_geometry.argtypes = [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p]
+    L.orc_cells.argtypes = [_i, _i, _p, _p, _i, _p, _p, _p]
+    L.orc_neighbors.restype = _i
+    L.orc_neighbors.argtypes = [_i, _i, _p, _p, _i, _p, _p, _p, _i, _p, _p, _p, _p]
+    L.orc_outersections.restype = _i
+    L.orc_outersections.argtypes = [_i, _i, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p]
+    L.orc_force_energy.restype = _d
+    L.orc_force_energy.argtypes = [_i, _i, _i, _p, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _d, _d, _p]
+    L.orc_project_force.argtypes = [_i, _i, _p, _p, _p]
+    L.orc_shake.restype = _i
+    L.orc_shake.argtypes = [_i, _i, _p, _p, _p, _p, _i, _d]
+    L.orc_xpbd.restype = _i
+    L.orc_xpbd.argtypes = [_i, _i, _p, _p, _p, _p, _i, _d]
+    L.orc_det_sum.restype = _d
+    L.orc_det_sum.argtypes = [_p, _i]
+    L.orc_overlap_count.restype = ctypes.c_longlong
+    L.orc_overlap_count.argtypes = [_i, _i, _p, _p, _i, _p, _p, _i, _p]
+    L.orc_force_eval.restype = _d
+    L.orc_force_eval.argtypes = [ctypes.POINTER(_System), _p, _p, _p]
+    L.orc_fire_init.argtypes = [ctypes.POINTER(_System), _p, _p, _p, ctypes.POINTER(FireState)]
+    L.orc_fire_step.argtypes = [ctypes.POINTER(_System), _p, _p, _p, ctypes.POINTER(FireState),
+                                _d, _d, _d, _d, _d, _i, _i]
+    L.orc_fire_minimize.restype = _i
+    L.orc_fire_minimize.argtypes = [ctypes.POINTER(_System), _p, _p, _p, ctypes.POINTER(FireState),
+                                    _d, _i, _d, _d, _d, _d, _d, _i, _i]
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_p) if a is not None else None
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def wrap(x):
+    return lib().orc_wrap(float(x))
+
+
+def topology(start):
+    start = _i32(start)
+    N = len(start) - 1
+    V = int(start[-1])
+    shapeId = np.zeros(V, np.int32); nxt = np.zeros(V, np.int32); prv = np.zeros(V, np.int32)
+    lib().orc_topology(N, _ptr(start), V, _ptr(shapeId), _ptr(nxt), _ptr(prv))
+    return shapeId, nxt, prv
+
+
+def geometry(pos, start):
+    """updatePolygonGeometry: dict of edgeLengths, areas, comX, comY, constraints, maxEdgeLength."""
+    pos = _f64(pos); start = _i32(start)
+    N = len(start) - 1; V = int(start[-1])
+    out = dict(edgeLengths=np.zeros(V), areas=np.zeros(N), comX=np.zeros(N), comY=np.zeros(N),
+               constraints=np.zeros(6 * V), constraintNormSq=np.zeros(3 * N))
+    mx = np.zeros(1)
+    lib().orc_geometry(V, N, _ptr(pos), _ptr(start), _ptr(out["edgeLengths"]), _ptr(out["areas"]),
+                       _ptr(out["comX"]), _ptr(out["comY"]), _ptr(out["constraints"]),
+                       _ptr(out["constraintNormSq"]), _ptr(mx))
+    out["maxEdgeLength"] = float(mx[0])
+    return out
+
+
+def cells(pos, start, boxSize):
+    pos = _f64(pos); start = _i32(start)
+    N = len(start) - 1; V = int(start[-1])
+    cl = np.zeros(V, np.int32); ni = np.zeros(V, np.int32); cpb = np.zeros(boxSize * boxSize, np.int32)
+    lib().orc_cells(V, N, _ptr(pos), _ptr(start), int(boxSize), _ptr(cl), _ptr(ni), _ptr(cpb))
+    return cl, ni, cpb
+
+
+def neighbors(pos, start, boxSize, maxNeighbors=100, cell=None):
+    """updateNeighbors incl. the overflow resize-and-retry of model.cpp:264-305."""
+    pos = _f64(pos); start = _i32(start)
+    N = len(start) - 1; V = int(start[-1])
+    cl, ni, cpb = cell if cell is not None else cells(pos, start, boxSize)
+    M = maxNeighbors
+    while True:
+        nb = np.full(V * M, -1, np.int32); ins = np.zeros(V * M, np.uint8)
+        tu = np.zeros(2 * V * M); num = np.zeros(V, np.int32)
+        mx = lib().orc_neighbors(V, N, _ptr(pos), _ptr(start), int(boxSize), _ptr(cl), _ptr(ni), _ptr(cpb),
+                                 M, _ptr(nb), _ptr(ins), _ptr(tu), _ptr(num))
+        if mx <= M:
+            break
+        M = max(2 * M + 1, mx)
+    return dict(neighbors=nb, inside=ins, tuSlots=tu, numNeighbors=num, maxNeighbors=M, maxCount=mx)
+
+
+def outersections(start, nbr):
+    start = _i32(start)
+    N = len(start) - 1; V = int(start[-1])
+    M = nbr["maxNeighbors"]
+    cap = max(1, int(np.minimum(nbr["numNeighbors"], M).sum()))
+    inter = np.zeros(cap, np.uint64); outer = np.zeros(cap, np.uint64)
+    tu = np.zeros(2 * cap); ut = np.zeros(2 * cap); sc = np.zeros(max(N, 1), np.int32)
+    I = lib().orc_outersections(V, N, _ptr(start), M, _ptr(nbr["neighbors"]), _ptr(nbr["inside"]),
+                                _ptr(nbr["tuSlots"]), _ptr(nbr["numNeighbors"]), _ptr(inter), _ptr(tu),
+                                _ptr(outer), _ptr(ut), _ptr(sc))
+    return dict(intersections=inter[:I], tu=tu[:2 * I], outersections=outer[:I], ut=ut[:2 * I],
+                shapeCounts=sc[:N], numIntersections=I)
+
+
+def per_vertex_targets(start, targetEdgeLengths):
+    start = np.asarray(start)
+    return np.repeat(np.asarray(targetEdgeLengths, np.float64), np.diff(start))
+
+
+def force_energy(model, pos, start, rec, geom, targetEdgeV=None, targetAreas=None,
+                 stiffness=0.0, compressibility=0.0):
+    pos = _f64(pos); start = _i32(start)
+    N = len(start) - 1; V = int(start[-1])
+    mid = MODEL_IDS[model] if isinstance(model, str) else int(model)
+    tev = _f64(targetEdgeV if targetEdgeV is not None else np.zeros(V))
+    ta = _f64(targetAreas if targetAreas is not None else np.zeros(N))
+    force = np.zeros(2 * V)
+    I = rec["numIntersections"] if rec is not None else 0
+    inter = _as_u64(rec["intersections"]) if rec is not None else np.zeros(1, np.uint64)
+    outer = _as_u64(rec["outersections"]) if rec is not None else np.zeros(1, np.uint64)
+    tu = _f64(rec["tu"]) if rec is not None else np.zeros(2)
+    ut = _f64(rec["ut"]) if rec is not None else np.zeros(2)
+    E = lib().orc_force_energy(mid, V, N, _ptr(start), _ptr(pos), I, _ptr(inter), _ptr(outer),
+                               _ptr(tu), _ptr(ut), _ptr(_f64(geom["edgeLengths"])),
+                               _ptr(_f64(geom["areas"])), _ptr(tev), _ptr(ta), float(stiffness),
+                               float(compressibility), _ptr(force))
+    return E, force
+
+
+def _as_u64(a):
+    return np.ascontiguousarray(a, dtype=np.uint64)
+
+
+def project_force(pos, start, force):
+    pos = _f64(pos); start = _i32(start); f = _f64(force).copy()
+    lib().orc_project_force(int(start[-1]), len(start) - 1, _ptr(start), _ptr(pos), _ptr(f))
+    return f
+
+
+def shake(pos, start, targetEdgeV, targetAreas, maxIter, tol=1e-15):
+    pos = _f64(pos).copy(); start = _i32(start)
+    it = lib().orc_shake(int(start[-1]), len(start) - 1, _ptr(start), _ptr(pos), _ptr(_f64(targetEdgeV)),
+                         _ptr(_f64(targetAreas)), int(maxIter), float(tol))
+    return pos, it
+
+
+def xpbd(pos, start, targetEdgeLengths, targetAreas, nIter, tol=0.0):
+    pos = _f64(pos).copy(); start = _i32(start)
+    it = lib().orc_xpbd(int(start[-1]), len(start) - 1, _ptr(start), _ptr(pos),
+                        _ptr(_f64(targetEdgeLengths)), _ptr(_f64(targetAreas)), int(nIter), float(tol))
+    return pos, it
+
+
+def det_sum(x):
+    x = _f64(x)
+    return lib().orc_det_sum(_ptr(x), len(x))
+
+
+def overlap_count(pos, start, boxSize, pointDensity):
+    pos = _f64(pos); start = _i32(start)
+    cl, ni, cpb = cells(pos, start, boxSize)
+    cnt = np.zeros(pointDensity * pointDensity, np.int32)
+    s = lib().orc_overlap_count(int(start[-1]), len(start) - 1, _ptr(start), _ptr(pos), int(boxSize),
+                                _ptr(ni), _ptr(cpb), int(pointDensity), _ptr(cnt))
+    return s, cnt
+
+
+class System:
+    """Holds the arrays behind an orc_system so they outlive the ctypes struct."""
+
+    def __init__(self, start, boxSize, model="normal", targetEdgeV=None, targetAreas=None,
+                 stiffness=0.0, compressibility=0.0, maxNeighbors=100):
+        self.start = _i32(start)
+        N = len(self.start) - 1; V = int(self.start[-1])
+        self.tev = _f64(targetEdgeV if targetEdgeV is not None else np.zeros(V))
+        self.ta = _f64(targetAreas if targetAreas is not None else np.zeros(N))
+        self.s = _System(V, N, int(boxSize), MODEL_IDS[model] if isinstance(model, str) else int(model),
+                         int(maxNeighbors), _ptr(self.start), _ptr(self.tev), _ptr(self.ta),
+                         float(stiffness), float(compressibility))
+
+    @property
+    def V(self):
+        return self.s.V
+
+    def force_eval(self, pos):
+        pos = _f64(pos); f = np.zeros(2 * self.V); nI = ctypes.c_int(0)
+        E = lib().orc_force_eval(ctypes.byref(self.s), _ptr(pos), _ptr(f), ctypes.byref(nI))
+        return E, f, nI.value
+
+    def fire_init(self, pos, dt, alpha):
+        st = FireState(dt, alpha, 0, 0.0, 0, 0, 0)
+        vel = np.zeros(2 * self.V); f = np.zeros(2 * self.V)
+        lib().orc_fire_init(ctypes.byref(self.s), _ptr(pos), _ptr(vel), _ptr(f), ctypes.byref(st))
+        return st, vel, f
+
+    def fire_step(self, pos, vel, force, st, dtMax=0.1, alphaStart=0.1, fAlpha=0.99, fInc=1.1,
+                  fDec=0.5, nMin=5, shakeIter=5):
+        lib().orc_fire_step(ctypes.byref(self.s), _ptr(pos), _ptr(vel), _ptr(force), ctypes.byref(st),
+                            dtMax, alphaStart, fAlpha, fInc, fDec, nMin, shakeIter)
+
+    def fire_minimize(self, pos, fTol=1e-10, dt=1e-3, maxSteps=100000, dtMax=0.1, alphaStart=0.1,
+                      fAlpha=0.99, fInc=1.1, fDec=0.5, nMin=5, shakeIter=5):
+        pos = _f64(pos).copy(); vel = np.zeros(2 * self.V); f = np.zeros(2 * self.V)
+        st = FireState(dt, alphaStart, 0, 0.0, 0, 0, 0)
+        steps = lib().orc_fire_minimize(ctypes.byref(self.s), _ptr(pos), _ptr(vel), _ptr(f),
+                                        ctypes.byref(st), fTol, maxSteps, dtMax, alphaStart, fAlpha,
+                                        fInc, fDec, nMin, shakeIter)
+        return pos, f, st, steps

@@ -1,0 +1,1290 @@
+/*
+ * CPU oracle: plain-C restatement of the reference overlap-force-minimization
+ * path (camdennis/pyCudaPolygon: src/kernels.cuh, src/model.cu, src/model.cpp,
+ * pyCudaPolygon/pyCudaPolygon.py).
+ *
+ * TEST INFRASTRUCTURE ONLY -- this is the checker, never the product.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ *
+ * Build: oracle/Makefile (gcc -O3 -ffp-contract=off -fno-fast-math -fopenmp).
+ * No FMA contraction: every expression is evaluated operation by operation in
+ * the order written in the reference source, so crossing decisions and
+ * float-rounded intersection parameters are reproduced exactly.
+ *
+ * Where the reference accumulates with double atomicAdd (order unspecified,
+ * kernels.cuh:1143-1144,1271,1330-1350,1415-1422) this oracle fixes ONE valid
+ * serialization, the same one the HIP kernels use:
+ *   force[m] = 0
+ *            + exterior contributions of records whose si owns m (record order,
+ *              slots i, zi, l, zl)
+ *            + exterior contributions of records whose sj owns m (record order,
+ *              slots j, zj, k, zk)
+ *            + interior local sum of thread m  (kernels.cuh:1416-1419)
+ *            + interior local sum of thread prev(m) for next(prev(m)) = m
+ *            + edge spring term, + area spring term.
+ * Energies: per polygon, sequential; packings reduce polygon energies with a
+ * fixed 256-wide pairwise tree (orc_det_sum).  Parity of these sums against the
+ * reference is therefore exact up to the reference's own atomic ordering.
+ *
+ * Functions whose reference kernels are missing from the reference tree
+ * (FIRE.h, shake.h; src/model.cu:15-16) are written from the host contract in
+ * src/model.cpp:405-504 and src/model.cu:576-636; see the notes at each.
+ */
+#include "pcp_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+
+typedef struct { double x, y; } d2;
+
+/* kernels.cuh:115-119 */
+double orc_wrap(double x) {
+    double y = x + 1.5;
+    y = y - floor(y);
+    return y - 0.5;
+}
+static d2 wrap2(d2 v) { v.x = orc_wrap(v.x); v.y = orc_wrap(v.y); return v; }
+static d2 P(const double* pos, int v) { d2 r = { pos[2 * v], pos[2 * v + 1] }; return r; }
+
+/* while-loop minimum image used by kernels.cuh:670-699 and :1303-1316.  Loops are
+ * bounded; for |x| beyond 2^20 (never produced by positions in the box) floor is used. */
+static double shift_into_unit(double x) {   /* while(x<0)x+=1; while(x>=1)x-=1 */
+    if (!(fabs(x) < 1048576.0)) return (x == x) ? x - floor(x) : x;
+    while (x < 0.0) x += 1.0;
+    while (x >= 1.0) x -= 1.0;
+    return x;
+}
+
+/* kernels.cuh:127-135, 652-659 */
+void orc_topology(int N, const int* start, int V, int* shapeId, int* next, int* prev) {
+    for (int s = 0; s < N; ++s)
+        for (int i = start[s]; i < start[s + 1]; ++i) shapeId[i] = s;
+    for (int i = 0; i < V; ++i) {
+        int s = shapeId[i];
+        next[i] = (i + 1 == start[s + 1]) ? start[s] : i + 1;
+        prev[i] = (i == start[s]) ? start[s + 1] - 1 : i - 1;
+    }
+}
+
+/* updatePolygonGeometryKernel kernels.cuh:295-333, segmented sums model.cu:123-140
+ * (sequential per polygon), normalizeKernel :635-650, normalizeConstraintsKernel
+ * :335-345, max edge length model.cu:144-150. */
+void orc_geometry(int V, int N, const double* pos, const int* start,
+                  double* edgeLengths, double* areas, double* comX, double* comY,
+                  double* constraints, double* constraintNormSq, double* maxEdgeLength) {
+    double mx = -DBL_MAX;
+    for (int s = 0; s < N; ++s) {
+        int st = start[s], en = start[s + 1], n = en - st;
+        double area = 0.0, cx = 0.0, cy = 0.0, ns0 = 0.0, ns1 = 0.0, ns2 = 0.0;
+        for (int k = st; k < en; ++k) {
+            int nx = (k + 1 == en) ? st : k + 1;
+            int pv = (k == st) ? en - 1 : k - 1;
+            for (int alpha = 0; alpha < 2; ++alpha) {
+                int idx = 2 * k + alpha;
+                double x1 = orc_wrap(pos[idx] - pos[st * 2 + alpha]);
+                double x4 = orc_wrap(pos[pv * 2 + 1 - alpha] - pos[nx * 2 + 1 - alpha]);
+                double dxPrev = orc_wrap(pos[2 * k] - pos[pv * 2]);
+                double dyPrev = orc_wrap(pos[2 * k + 1] - pos[pv * 2 + 1]);
+                double dxNext = orc_wrap(pos[2 * k] - pos[nx * 2]);
+                double dyNext = orc_wrap(pos[2 * k + 1] - pos[nx * 2 + 1]);
+                if (alpha == 0) cx += x1; else cy += x1;
+                double lenPrev = sqrt(dxPrev * dxPrev + dyPrev * dyPrev);
+                double lenNext = sqrt(dxNext * dxNext + dyNext * dyNext);
+                double areaComp = (2 * alpha - 1) * x4;
+                double edge2Comp = alpha ? (dyNext + dyPrev) : (dxNext + dxPrev);
+                double edgeComp = alpha ? (dyNext / lenNext) + (dyPrev / lenPrev)
+                                        : (dxNext / lenNext) + (dxPrev / lenPrev);
+                constraints[6 * k + alpha] = areaComp;
+                constraints[6 * k + 2 + alpha] = edge2Comp;
+                constraints[6 * k + 4 + alpha] = edgeComp;
+                ns0 += areaComp * areaComp;
+                ns1 += edge2Comp * edge2Comp;
+                ns2 += edgeComp * edgeComp;
+                if (alpha == 0) {
+                    double dx = orc_wrap(pos[2 * nx] - pos[2 * k]);
+                    double dy = orc_wrap(pos[2 * nx + 1] - pos[2 * k + 1]);
+                    double el = sqrt(dx * dx + dy * dy);
+                    edgeLengths[k] = el;
+                    if (el > mx) mx = el;
+                    double dy2 = orc_wrap(pos[nx * 2 + 1] - pos[idx + 1]);
+                    double x2 = orc_wrap(pos[nx * 2 + alpha] - pos[st * 2 + alpha]);
+                    area += dy2 * (x2 + x1) / 2.0;
+                }
+            }
+        }
+        areas[s] = area;
+        double c = cx;
+        c /= ((double)n);
+        c += pos[2 * st];
+        c -= floor(c);
+        comX[s] = c;
+        c = cy;
+        c /= ((double)n);
+        c += pos[2 * st + 1];
+        c -= floor(c);
+        comY[s] = c;
+        if (constraintNormSq) {
+            constraintNormSq[3 * s] = ns0; constraintNormSq[3 * s + 1] = ns1; constraintNormSq[3 * s + 2] = ns2;
+        }
+        double nsv[3] = { ns0, ns1, ns2 };
+        for (int k = st; k < en; ++k)
+            for (int alpha = 0; alpha < 2; ++alpha)
+                for (int cc = 0; cc < 3; ++cc)
+                    if (nsv[cc] > 0.0) constraints[6 * k + 2 * cc + alpha] /= sqrt(nsv[cc]);
+    }
+    if (maxEdgeLength) *maxEdgeLength = (V > 0) ? mx : 0.0;
+}
+
+/* cell of an edge midpoint: updateNeighborCellsKernel kernels.cuh:661-707 */
+static int cell_of_edge(const double* pos, int idx, int v2, int boxSize) {
+    double x = pos[v2 * 2] - pos[idx * 2] + 0.5;
+    double y = pos[v2 * 2 + 1] - pos[idx * 2 + 1] + 0.5;
+    x = shift_into_unit(x);
+    y = shift_into_unit(y);
+    x -= 0.5; y -= 0.5;
+    x /= 2.0; y /= 2.0;
+    x += pos[idx * 2];
+    y += pos[idx * 2 + 1];
+    x = shift_into_unit(x);
+    y = shift_into_unit(y);
+    double fx = floor(x * boxSize), fy = floor(y * boxSize);
+    int ix = (fx == fx && fx >= 0.0) ? (fx < 2147483647.0 ? (int)fx : boxSize - 1) : 0;
+    int iy = (fy == fy && fy >= 0.0) ? (fy < 2147483647.0 ? (int)fy : boxSize - 1) : 0;
+    if (ix < 0) ix = 0; else if (ix >= boxSize) ix = boxSize - 1;
+    if (iy < 0) iy = 0; else if (iy >= boxSize) iy = boxSize - 1;
+    return iy * boxSize + ix;
+}
+
+/* updateNeighborCellsCUDA model.cu:308-385: hash, stable sort of vertex ids by cell
+ * (CUB radix SortPairs over iota values), lowerBoundKernel kernels.cuh:45-63. */
+void orc_cells(int V, int N, const double* pos, const int* start, int boxSize,
+               int* cellLocation, int* neighborIndices, int* countPerBox) {
+    int nb = boxSize * boxSize;
+    int* shapeId = (int*)malloc(sizeof(int) * (V > 0 ? V : 1));
+    for (int s = 0; s < N; ++s) for (int i = start[s]; i < start[s + 1]; ++i) shapeId[i] = s;
+    for (int idx = 0; idx < V; ++idx) {
+        int v2 = idx + 1;
+        if (idx == V - 1 || shapeId[idx] != shapeId[idx + 1]) v2 = start[shapeId[idx]];
+        cellLocation[idx] = cell_of_edge(pos, idx, v2, boxSize);
+    }
+    int* cnt = (int*)calloc(nb + 1, sizeof(int));
+    for (int i = 0; i < V; ++i) cnt[cellLocation[i] + 1]++;
+    for (int c = 0; c < nb; ++c) cnt[c + 1] += cnt[c];
+    for (int c = 0; c < nb; ++c) countPerBox[c] = cnt[c];   /* lower bound of c */
+    for (int i = 0; i < V; ++i) neighborIndices[cnt[cellLocation[i]]++] = i;  /* stable */
+    free(cnt);
+    free(shapeId);
+}
+
+/* updateNeighborsKernel kernels.cuh:709-812.  tu holds double2 per slot [2*V*M].
+ * Returns max(numNeighbors) (maxReduceKernel :65-92). */
+int orc_neighbors(int V, int N, const double* pos, const int* start, int boxSize,
+                  const int* cellLocation, const int* neighborIndices, const int* countPerBox,
+                  int maxNeighbors, int* neighbors, unsigned char* inside, double* tu,
+                  int* numNeighbors) {
+    const double eps = 1e-12;
+    int* shapeId = (int*)malloc(sizeof(int) * (V > 0 ? V : 1));
+    for (int s = 0; s < N; ++s) for (int i = start[s]; i < start[s + 1]; ++i) shapeId[i] = s;
+    int boxCount = boxSize * boxSize;
+    int mx = (V > 0) ? -2147483647 - 1 : 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(max : mx)
+    for (int id1 = 0; id1 < V; ++id1) {
+        int shape = shapeId[id1];
+        int st = start[shape + 1] - 1;
+        int id2 = (id1 == st) ? start[shape] : id1 + 1;
+        const double px = pos[2 * id1], py = pos[2 * id1 + 1];
+        double rx = orc_wrap(pos[2 * id2] - px);
+        double ry = orc_wrap(pos[2 * id2 + 1] - py);
+        int box = cellLocation[id1];
+        int bx = box % boxSize, by = box / boxSize;
+        int count = 0;
+        int processed[16], np = 0;
+        for (int dx = -1; dx <= 1; dx++) {
+            int nx = bx + dx;
+            if (nx < 0) nx += boxSize; else if (nx >= boxSize) nx -= boxSize;
+            for (int dy = -1; dy <= 1; dy++) {
+                int ny = by + dy;
+                if (ny < 0) ny += boxSize; else if (ny >= boxSize) ny -= boxSize;
+                int nbx = ny * boxSize + nx;
+                int seen = 0;
+                for (int b = 0; b < np; b++) if (processed[b] == nbx) { seen = 1; break; }
+                if (seen) continue;
+                if (np < 16) processed[np++] = nbx;
+                int si = countPerBox[nbx];
+                int sf = (nbx + 1 < boxCount) ? countPerBox[nbx + 1] : V;
+                if (si >= sf) continue;
+                for (int q = si; q < sf; q++) {
+                    int nid = neighborIndices[q];
+                    if (nid <= id1) continue;
+                    int shape2 = shapeId[nid];
+                    int st2 = start[shape2 + 1] - 1;
+                    int nid2 = (nid == st2) ? start[shape2] : nid + 1;
+                    if (nid == id1 || nid == id2 || nid2 == id1 || nid2 == id2) continue;
+                    double qx = pos[2 * nid], qy = pos[2 * nid + 1];
+                    double sx = orc_wrap(pos[2 * nid2] - qx);
+                    double sy = orc_wrap(pos[2 * nid2 + 1] - qy);
+                    double gx = orc_wrap(qx - px);
+                    double gy = orc_wrap(qy - py);
+                    double denom = rx * sy - ry * sx;
+                    if (fabs(denom) < eps) continue;
+                    double tt = (gx * sy - gy * sx) / denom;
+                    double uu = (gx * ry - gy * rx) / denom;
+                    if (tt > 0.0 && tt < 1.0 && uu > 0.0 && uu < 1.0) {
+                        if (count < maxNeighbors) {
+                            long slot = (long)id1 * maxNeighbors + count;
+                            neighbors[slot] = nid;
+                            if (denom > 0) { inside[slot] = 1; tu[2 * slot] = tt; tu[2 * slot + 1] = uu; }
+                            else { inside[slot] = 0; tu[2 * slot] = uu; tu[2 * slot + 1] = tt; }
+                        }
+                        count++;
+                    }
+                }
+            }
+        }
+        numNeighbors[id1] = count;
+        if (count > mx) mx = count;
+    }
+    free(shapeId);
+    return mx;
+}
+
+typedef struct { uint64_t key; int idx; } keyidx;
+static int cmp_keyidx(const void* a, const void* b) {
+    const keyidx* x = (const keyidx*)a; const keyidx* y = (const keyidx*)b;
+    if (x->key < y->key) return -1;
+    if (x->key > y->key) return 1;
+    return (x->idx < y->idx) ? -1 : (x->idx > y->idx);   /* stable */
+}
+
+/* Model::updateOutersections model.cpp:351-361:
+ *   updateValidAndCountsKernel kernels.cuh:814-836 + exclusive scan model.cu:413-450,
+ *   updateCompactedIntersectionsKernel :838-877 (tu rounded to float),
+ *   sortKeysCUDA(0,64) model.cu:50-75 + gather, updateOutersectionsKernel :981-1065,
+ *   sortKeysCUDA(0,48) + gathers.
+ * tuSlots is the V*M double2 array written by orc_neighbors.  Returns I. */
+static double cross_param(d2 vi, d2 vzi, d2 vj, d2 vzj, double fallback);
+/* exact parameter along si's edge `ei` of its crossing with sj's edge `ej` */
+static double exact_param(const double* pos, const int* start, int si, int ei, int sj, int ej,
+                          double fallback) {
+    int ni = start[si + 1] - start[si], nj = start[sj + 1] - start[sj];
+    int a = start[si] + ei, b = start[si] + (ei + 1) % ni;
+    int c = start[sj] + ej, d = start[sj] + (ej + 1) % nj;
+    return cross_param(P(pos, a), P(pos, b), P(pos, c), P(pos, d), fallback);
+}
+
+int orc_outersections(int V, int N, const int* start, int maxNeighbors,
+                      const int* neighbors, const unsigned char* inside, const double* tuSlots,
+                      const int* numNeighbors, uint64_t* intersections, double* tu,
+                      uint64_t* outersections, double* ut, int* shapeCounts) {
+    return orc_outersections_ex(V, N, start, maxNeighbors, neighbors, inside, tuSlots, numNeighbors,
+                                intersections, tu, outersections, ut, shapeCounts, NULL);
+}
+
+/* pairExactPos != NULL (area energy): the pairing compares exact double parameters
+ * recomputed at these positions instead of the float-rounded ones. */
+int orc_outersections_ex(int V, int N, const int* start, int maxNeighbors,
+                         const int* neighbors, const unsigned char* inside, const double* tuSlots,
+                         const int* numNeighbors, uint64_t* intersections, double* tu,
+                         uint64_t* outersections, double* ut, int* shapeCounts,
+                         const double* pairExactPos) {
+    int* shapeId = (int*)malloc(sizeof(int) * (V > 0 ? V : 1));
+    for (int s = 0; s < N; ++s) for (int i = start[s]; i < start[s + 1]; ++i) shapeId[i] = s;
+    for (int s = 0; s < N; ++s) shapeCounts[s] = 0;
+    /* compaction in (id1, slot) order */
+    int I = 0;
+    for (int n1 = 0; n1 < V; ++n1) {
+        int c = numNeighbors[n1] < maxNeighbors ? numNeighbors[n1] : maxNeighbors;
+        for (int k = 0; k < c; ++k) {
+            long slot = (long)n1 * maxNeighbors + k;
+            int n2 = neighbors[slot];
+            int s1 = shapeId[n1], s2 = shapeId[n2];
+            int in = inside[slot];
+            shapeCounts[in ? s2 : s1]++;
+            int l1 = n1 - start[s1], l2 = n2 - start[s2];
+            float tVal = (float)tuSlots[2 * slot];
+            float uVal = (float)tuSlots[2 * slot + 1];
+            uint64_t packed;
+            if (in) packed = ((uint64_t)(uint16_t)s1 << 48) | ((uint64_t)(uint16_t)s2 << 32) |
+                             ((uint64_t)(uint16_t)l2 << 16) | (uint64_t)(uint16_t)l1;
+            else packed = ((uint64_t)(uint16_t)s2 << 48) | ((uint64_t)(uint16_t)s1 << 32) |
+                          ((uint64_t)(uint16_t)l1 << 16) | (uint64_t)(uint16_t)l2;
+            intersections[I] = packed;
+            tu[2 * I] = (double)tVal;
+            tu[2 * I + 1] = (double)uVal;
+            I++;
+        }
+    }
+    if (I == 0) { free(shapeId); return 0; }
+    /* sort by 64 bits (order A), permute tu */
+    keyidx* ka = (keyidx*)malloc(sizeof(keyidx) * I);
+    for (int r = 0; r < I; ++r) { ka[r].key = intersections[r]; ka[r].idx = r; }
+    qsort(ka, I, sizeof(keyidx), cmp_keyidx);
+    uint64_t* keyA = (uint64_t*)malloc(sizeof(uint64_t) * I);
+    double* tuA = (double*)malloc(sizeof(double) * 2 * I);
+    for (int r = 0; r < I; ++r) {
+        keyA[r] = ka[r].key;
+        tuA[2 * r] = tu[2 * ka[r].idx];
+        tuA[2 * r + 1] = tu[2 * ka[r].idx + 1];
+    }
+    /* pairing in order A */
+    uint64_t* outA = (uint64_t*)malloc(sizeof(uint64_t) * I);
+    double* utA = (double*)malloc(sizeof(double) * 2 * I);
+    for (int idx = 0; idx < I; ++idx) {
+        uint64_t inter = keyA[idx];
+        int sj = (int)((inter >> 48) & 0xFFFF);
+        int si = (int)((inter >> 32) & 0xFFFF);
+        int i = (int)((inter >> 16) & 0xFFFF);
+        int ni = start[si + 1] - start[si];
+        uint64_t sij = ((uint64_t)si << 48) | ((uint64_t)sj << 32);
+        double tValD = (float)tuA[2 * idx + 1];
+        if (pairExactPos)
+            tValD = exact_param(pairExactPos, start, si, i, sj, (int)(inter & 0xFFFF), tValD);
+        int lo = 0, hi = I - 1, mid;
+        while (hi > lo) {
+            mid = (hi + lo) / 2;
+            if (keyA[mid] < sij) lo = mid + 1; else hi = mid;
+        }
+        uint64_t ub = ((uint64_t)si << 48) | ((uint64_t)(sj + 1) << 32);
+        int bestDist = -1; double bestU = FLT_MAX; int bestIdx = -1;
+        double fallbackU = FLT_MAX; int fallbackIdx = -1;
+        int k = lo;
+        while (k < I && keyA[k] < ub) {
+            int l = (int)(keyA[k] & 0xFFFF);
+            double uVal = (float)tuA[2 * k];
+            if (pairExactPos && (keyA[k] >> 32) == (sij >> 32))   /* candidate of the proper group */
+                uVal = exact_param(pairExactPos, start, si, l, sj, (int)((keyA[k] >> 16) & 0xFFFF), uVal);
+            double tVal = tValD;
+            int d = (l - i + ni) % ni;
+            if (l == i && tVal >= uVal) { k++; continue; }
+            if (bestDist == -1 || d < bestDist) {
+                bestDist = d; bestU = FLT_MAX; bestIdx = -1; fallbackU = FLT_MAX; fallbackIdx = -1;
+            }
+            if (d == bestDist) {
+                if (d == 0) {
+                    if (uVal >= tVal && uVal < bestU) { bestU = uVal; bestIdx = k; }
+                    if (uVal < fallbackU) { fallbackU = uVal; fallbackIdx = k; }
+                } else {
+                    if (uVal < bestU) { bestU = uVal; bestIdx = k; }
+                }
+            }
+            k++;
+        }
+        int player = (bestIdx != -1) ? bestIdx : fallbackIdx;
+        if (player != -1) {   /* partner edges outside the two polygons: no partner (the
+                                 reference would read vertices of other polygons) */
+            int nj = start[sj + 1] - start[sj];
+            if ((int)((keyA[player] >> 16) & 0xFFFF) >= nj || (int)(keyA[player] & 0xFFFF) >= ni) player = -1;
+        }
+        if (player == -1) {
+            outA[idx] = keyA[idx];
+            utA[2 * idx] = tuA[2 * idx]; utA[2 * idx + 1] = tuA[2 * idx + 1];
+        } else {
+            outA[idx] = keyA[player];
+            utA[2 * idx] = tuA[2 * player]; utA[2 * idx + 1] = tuA[2 * player + 1];
+        }
+    }
+    /* stable sort by bits [0,48) (order B) */
+    const uint64_t m48 = (((uint64_t)1) << 48) - 1;
+    for (int r = 0; r < I; ++r) { ka[r].key = keyA[r] & m48; ka[r].idx = r; }
+    qsort(ka, I, sizeof(keyidx), cmp_keyidx);
+    for (int r = 0; r < I; ++r) {
+        int a = ka[r].idx;
+        intersections[r] = keyA[a];
+        tu[2 * r] = tuA[2 * a]; tu[2 * r + 1] = tuA[2 * a + 1];
+        ut[2 * r] = utA[2 * a]; ut[2 * r + 1] = utA[2 * a + 1];
+        outersections[r] = outA[a];
+    }
+    free(ka); free(keyA); free(tuA); free(outA); free(utA); free(shapeId);
+    return I;
+}
+
+/* h, g12 kernels.cuh:139-159 */
+static double hfun(d2 p1, d2 p2, d2 sp) {
+    d2 r1 = { p1.x - sp.x, p1.y - sp.y }, r2 = { p2.x - sp.x, p2.y - sp.y };
+    r1 = wrap2(r1); r2 = wrap2(r2);
+    d2 dd = { r2.x - r1.x, r2.y - r1.y };
+    d2 d = wrap2(dd);
+    return (r1.x + r2.x) * d.y;
+}
+static void g12(d2 p1, d2 p2, d2 sp, d2* g1, d2* g2) {
+    d2 r1 = { p1.x - sp.x, p1.y - sp.y }, r2 = { p2.x - sp.x, p2.y - sp.y };
+    r1 = wrap2(r1); r2 = wrap2(r2);
+    d2 d = { p2.x - p1.x, p2.y - p1.y };
+    d = wrap2(d);
+    g1->x = d.y; g1->y = -(r1.x + r2.x);
+    g2->x = d.y; g2->y = (r1.x + r2.x);
+}
+/* getDf kernels.cuh:161-222 (statement for statement) */
+static void getDf(d2 vi, d2 vzi, d2 vj, d2 vzj, double* df) {
+    d2 t;
+    t.x = vzj.x - vj.x; t.y = vzj.y - vj.y; d2 dj = wrap2(t);
+    t.x = vzi.x - vi.x; t.y = vzi.y - vi.y; d2 di = wrap2(t);
+    t.x = vj.x - vi.x; t.y = vj.y - vi.y; d2 dij = wrap2(t);
+    double w = dj.x * di.y - dj.y * di.x;
+    double diNorm2 = di.x * di.x + di.y * di.y;
+    double djNorm2 = dj.x * dj.x + dj.y * dj.y;
+    if (w * w < 1e-12 * diNorm2 * djNorm2 || diNorm2 < 1e-28 || djNorm2 < 1e-28) {
+        for (int i = 0; i < 16; i++) df[i] = 0.0;
+        return;
+    }
+    double k = dj.x * dij.y - dj.y * dij.x;
+    double u = k / w;
+    double dk[8] = { 0 };
+    dk[0] = dj.y; dk[1] = -dj.x;
+    dk[4] = -dij.y - dj.y; dk[5] = dj.x + dij.x;
+    dk[6] = dij.y; dk[7] = -dij.x;
+    double dw[8];
+    dw[0] = dj.y; dw[1] = -dj.x; dw[2] = -dj.y; dw[3] = dj.x;
+    dw[4] = -di.y; dw[5] = di.x; dw[6] = di.y; dw[7] = -di.x;
+    for (int i = 0; i < 16; ++i) df[i] = 0.0;
+    for (int alpha = 0; alpha < 2; ++alpha) {
+        for (int beta = 0; beta < 2; ++beta) {
+            for (int p = 0; p < 4; ++p) {
+                double du = dk[p * 2 + beta] / w - u * dw[p * 2 + beta] / w;
+                int col = 2 * p + beta;
+                df[col * 2 + alpha] += (alpha == 0 ? di.x : di.y) * du;
+            }
+            if (alpha == beta) {
+                df[beta * 2 + alpha] += 1.0 - u;
+                df[(2 + beta) * 2 + alpha] += u;
+            }
+        }
+    }
+}
+/* parameter along edge (vi -> vzi) of its crossing with the line (vj -> vzj): the u of
+ * getDf kernels.cuh:163-175; fallback for (near) parallel edges */
+static double cross_param(d2 vi, d2 vzi, d2 vj, d2 vzj, double fallback) {
+    d2 t;
+    t.x = vzj.x - vj.x; t.y = vzj.y - vj.y; d2 dj = wrap2(t);
+    t.x = vzi.x - vi.x; t.y = vzi.y - vi.y; d2 di = wrap2(t);
+    t.x = vj.x - vi.x; t.y = vj.y - vi.y; d2 dij = wrap2(t);
+    double w = dj.x * di.y - dj.y * di.x;
+    double diNorm2 = di.x * di.x + di.y * di.y;
+    double djNorm2 = dj.x * dj.x + dj.y * dj.y;
+    if (w * w < 1e-12 * diNorm2 * djNorm2 || diNorm2 < 1e-28 || djNorm2 < 1e-28) return fallback;
+    double k = dj.x * dij.y - dj.y * dij.x;
+    return k / w;
+}
+/* negated g . df[cols c0, c0+1] */
+static d2 contrib(const double* df, int c0, d2 g) {
+    d2 inc;
+    inc.x = g.x * df[c0 * 2 + 0] + g.y * df[c0 * 2 + 1];
+    inc.y = g.x * df[(c0 + 1) * 2 + 0] + g.y * df[(c0 + 1) * 2 + 1];
+    return inc;
+}
+
+/* Exterior record contributions: updateForceEnergyExteriorKernel kernels.cuh:1067-1256.
+ * Slots: 0:i 1:zi 2:l 3:zl (owned by si), 4:j 5:zj 6:k 7:zk (owned by sj).
+ * c[slot] is the value the reference atomically adds (-inc). */
+static double exterior_record(uint64_t inter, uint64_t outer, const double* tuR, const double* utR,
+                              const double* pos, const int* start, const int* next,
+                              int tgt[8], d2 c[8], int areaMode, d2* spOut) {
+    int s1 = (int)((inter >> 32) & 0xFFFF), s2 = (int)((inter >> 48) & 0xFFFF);
+    int iLoc = (int)((inter >> 16) & 0xFFFF), jLoc = (int)(inter & 0xFFFF);
+    int kLoc = (int)((outer >> 16) & 0xFFFF), lLoc = (int)(outer & 0xFFFF);
+    int i = start[s1] + iLoc, j = start[s2] + jLoc, k = start[s2] + kLoc, l = start[s1] + lLoc;
+    d2 pi = P(pos, i), pj = P(pos, j), pk = P(pos, k), pl = P(pos, l);
+    int zi = next[i], zj = next[j], zk = next[k], zl = next[l];
+    d2 pzi = P(pos, zi), pzj = P(pos, zj), pzk = P(pos, zk), pzl = P(pos, zl);
+    d2 t;
+    t.x = pzi.x - pi.x; t.y = pzi.y - pi.y; d2 r1 = wrap2(t);
+    t.x = pzk.x - pk.x; t.y = pzk.y - pk.y; d2 r2 = wrap2(t);
+    double t1 = tuR[1], t2 = utR[1];
+    if (areaMode) {   /* exact crossing parameters at the current positions */
+        t1 = cross_param(pi, pzi, pj, pzj, t1);
+        t2 = cross_param(pk, pzk, pl, pzl, t2);
+    }
+    t.x = pi.x + t1 * r1.x; t.y = pi.y + t1 * r1.y; d2 fij = wrap2(t);
+    t.x = pk.x + t2 * r2.x; t.y = pk.y + t2 * r2.y; d2 fkl = wrap2(t);
+    d2 sp = fij;
+    if (areaMode == 1) {   /* closed loop: one origin per polygon pair (first vertex of the
+                                 lower-numbered polygon) */
+        int so = s1 < s2 ? s1 : s2;
+        sp = P(pos, start[so]);
+    }
+    *spOut = sp;
+    double e = 0.0;
+    double dfa[16], dfb[16];
+    tgt[0] = i; tgt[1] = zi; tgt[2] = l; tgt[3] = zl; tgt[4] = j; tgt[5] = zj; tgt[6] = k; tgt[7] = zk;
+    d2 inc[8];
+    if (i == l) {
+        e += hfun(fij, fkl, sp);
+        d2 ga, gb;
+        g12(fij, fkl, sp, &ga, &gb);
+        getDf(pi, pzi, pj, pzj, dfa);
+        getDf(pk, pzk, pi, pzi, dfb);
+        inc[0] = contrib(dfa, 0, ga);
+        inc[1] = contrib(dfa, 2, ga);
+        inc[4] = contrib(dfa, 4, ga);
+        inc[5] = contrib(dfa, 6, ga);
+        inc[6] = contrib(dfb, 0, gb);
+        inc[7] = contrib(dfb, 2, gb);
+        inc[2] = contrib(dfb, 4, gb);
+        inc[3] = contrib(dfb, 6, gb);
+    } else {
+        e += hfun(fij, pzi, sp);
+        e += hfun(pl, fkl, sp);
+        d2 g1a, g2a;
+        g12(fij, pzi, sp, &g1a, &g2a);
+        getDf(pi, pzi, pj, pzj, dfa);
+        getDf(pk, pzk, pl, pzl, dfb);
+        inc[0] = contrib(dfa, 0, g1a);
+        inc[4] = contrib(dfa, 4, g1a);
+        {
+            d2 dd = contrib(dfa, 2, g1a);
+            inc[1].x = dd.x + g2a.x;
+            inc[1].y = dd.y + g2a.y;
+        }
+        inc[5] = contrib(dfa, 6, g1a);
+        d2 g1b, g2b;
+        g12(pl, fkl, sp, &g1b, &g2b);
+        {
+            d2 dd = contrib(dfb, 4, g2b);
+            inc[2].x = g1b.x; inc[2].y = g1b.y;
+            inc[2].x += dd.x; inc[2].y += dd.y;
+        }
+        inc[6] = contrib(dfb, 0, g2b);
+        inc[3] = contrib(dfb, 6, g2b);
+        inc[7] = contrib(dfb, 2, g2b);
+    }
+    double sc = areaMode ? 0.5 : 1.0;   /* areaMode 2: broken loop, own entry point */
+    for (int s = 0; s < 8; ++s) { c[s].x = -inc[s].x * sc; c[s].y = -inc[s].y * sc; }
+    return e;
+}
+
+/* Pairwise 256-wide tree sum (fixed order), recursive over partials. */
+static double tree256(const double* x, int n) {
+    double v[256];
+    for (int t = 0; t < 256; ++t) v[t] = (t < n) ? x[t] : 0.0;
+    for (int s = 128; s >= 1; s >>= 1)
+        for (int t = 0; t < s; ++t) v[t] += v[t + s];
+    return v[0];
+}
+double orc_det_sum(const double* x, int n) {
+    if (n <= 0) return 0.0;
+    if (n <= 256) return tree256(x, n);
+    int m = (n + 255) / 256;
+    double* p = (double*)malloc(sizeof(double) * m);
+    for (int b = 0; b < m; ++b) {
+        int len = n - 256 * b; if (len > 256) len = 256;
+        p[b] = tree256(x + 256 * b, len);
+    }
+    double r = orc_det_sum(p, m);
+    free(p);
+    return r;
+}
+double orc_max_abs(const double* x, int n) {
+    double m = 0.0;
+    for (int i = 0; i < n; ++i) { double a = fabs(x[i]); if (a > m) m = a; }
+    return m;
+}
+
+/* Model::updateForceEnergy model.cpp:363-395 with kernels
+ *   exterior kernels.cuh:1067-1273, updateShapeRangesKernel :1275-1282,
+ *   interior :1354-1424, edge :1284-1335, area :1337-1352.
+ * targetEdgeV: per-vertex target edge length (reference: targetEdgeLengths[shapeId[m]]).
+ * Returns the energy; force[2V] is overwritten. */
+double orc_force_energy(int model, int V, int N, const int* start, const double* pos,
+                        int I, const uint64_t* intersections, const uint64_t* outersections,
+                        const double* tu, const double* ut, const double* edgeLengths,
+                        const double* areas, const double* targetEdgeV, const double* targetAreas,
+                        double stiffness, double compressibility, double* force) {
+    int* shapeId = (int*)malloc(sizeof(int) * (V > 0 ? V : 1));
+    int* next = (int*)malloc(sizeof(int) * (V > 0 ? V : 1));
+    int* prev = (int*)malloc(sizeof(int) * (V > 0 ? V : 1));
+    orc_topology(N, start, V, shapeId, next, prev);
+    int areaMode = (model & ORC_AREA_ENERGY) != 0;
+    model &= 0xFF;
+    int overlap = (model == ORC_NORMAL || model == ORC_HYBRID);
+    int edgeOn = (model == ORC_SOFT_BODY || model == ORC_EDGE_ONLY || model == ORC_HYBRID);
+    int areaOn = (model == ORC_SOFT_BODY || model == ORC_AREA_ONLY || model == ORC_HYBRID);
+    if (!overlap) I = 0;
+    int Ialloc = I > 0 ? I : 1;
+    int* tgt = (int*)malloc(sizeof(int) * 8 * Ialloc);
+    d2* cc = (d2*)malloc(sizeof(d2) * 8 * Ialloc);
+    double* eext = (double*)malloc(sizeof(double) * Ialloc);
+    unsigned char* self = (unsigned char*)malloc(Ialloc);
+    d2* spr = (d2*)malloc(sizeof(d2) * Ialloc);
+    int* shapeStart = (int*)malloc(sizeof(int) * (N > 0 ? N : 1));
+    int* shapeEnd = (int*)malloc(sizeof(int) * (N > 0 ? N : 1));
+    for (int s = 0; s < N; ++s) { shapeStart[s] = I; shapeEnd[s] = -1; }
+    /* area energy: records on a closed partner cycle (<= 8 hops) share the pair origin */
+    int* loopMode = (int*)malloc(sizeof(int) * Ialloc);
+    for (int r = 0; r < I; ++r) loopMode[r] = areaMode ? 2 : 0;
+    if (areaMode && I > 0) {
+        int* partner = (int*)malloc(sizeof(int) * I);
+        for (int r = 0; r < I; ++r) {
+            partner[r] = -1;
+            if (intersections[r] == outersections[r]) continue;
+            int lo = 0, hi = I;   /* records are sorted by (si,i,j,sj): binary search */
+            uint64_t o = outersections[r];
+            uint64_t ko = ((o & 0xFFFFFFFFull) << 16) | (o >> 48);
+            uint64_t kp = (o >> 32) & 0xFFFF;
+            while (lo < hi) {
+                int mid = (lo + hi) / 2;
+                uint64_t km = intersections[mid];
+                uint64_t sm = (km >> 32) & 0xFFFF;
+                uint64_t om = ((km & 0xFFFFFFFFull) << 16) | (km >> 48);
+                if (sm < kp || (sm == kp && om < ko)) lo = mid + 1; else hi = mid;
+            }
+            if (lo < I && intersections[lo] == o) partner[r] = lo;
+        }
+        for (int r = 0; r < I; ++r) {
+            int q = partner[r];
+            for (int hop = 0; hop < 8 && q >= 0 && q != r; ++hop) q = partner[q];
+            if (q == r) loopMode[r] = 1;
+        }
+        free(partner);
+    }
+#pragma omp parallel for schedule(static)
+    for (int r = 0; r < I; ++r) {
+        self[r] = (intersections[r] == outersections[r]);
+        if (self[r]) { eext[r] = 0.0; continue; }
+        eext[r] = exterior_record(intersections[r], outersections[r], tu + 2 * r, ut + 2 * r,
+                                  pos, start, next, tgt + 8 * r, cc + 8 * r, loopMode[r], spr + r);
+    }
+    for (int r = 0; r < I; ++r) {
+        int s = (int)((intersections[r] >> 32) & 0xFFFF);
+        if (r < shapeStart[s]) shapeStart[s] = r;
+        if (r > shapeEnd[s]) shapeEnd[s] = r;
+    }
+    /* records owned through sj, in record order */
+    int* sjCount = (int*)calloc(N + 1, sizeof(int));
+    for (int r = 0; r < I; ++r) sjCount[((intersections[r] >> 48) & 0xFFFF) + 1]++;
+    for (int s = 0; s < N; ++s) sjCount[s + 1] += sjCount[s];
+    int* sjList = (int*)malloc(sizeof(int) * Ialloc);
+    int* fill = (int*)malloc(sizeof(int) * (N > 0 ? N : 1));
+    for (int s = 0; s < N; ++s) fill[s] = sjCount[s];
+    for (int r = 0; r < I; ++r) sjList[fill[(intersections[r] >> 48) & 0xFFFF]++] = r;
+    /* interior: thread-local sums (kernels.cuh:1354-1411) */
+    d2* LM = (d2*)malloc(sizeof(d2) * (V > 0 ? V : 1));
+    d2* LN = (d2*)malloc(sizeof(d2) * (V > 0 ? V : 1));
+    double* eint = (double*)malloc(sizeof(double) * (V > 0 ? V : 1));
+#pragma omp parallel for schedule(static)
+    for (int m = 0; m < V; ++m) {
+        d2 lm = { 0.0, 0.0 }, ln = { 0.0, 0.0 };
+        double le = 0.0;
+        int s = shapeId[m];
+        if (overlap && shapeStart[s] <= shapeEnd[s]) {
+            int n = start[s + 1] - start[s];
+            int mLoc = m - start[s];
+            d2 pm = P(pos, m);
+            int nxt = next[m];
+            d2 pn = P(pos, nxt);
+            for (int k = shapeStart[s]; k <= shapeEnd[s]; ++k) {
+                uint64_t inter = intersections[k], outer = outersections[k];
+                if (inter == outer) continue;
+                int si = (int)((inter >> 32) & 0xFFFF);
+                int iLoc = (int)((inter >> 16) & 0xFFFF);
+                int lLoc = (int)(outer & 0xFFFF);
+                int iDist = (mLoc - iLoc + n) % n;
+                int lDist = (lLoc - iLoc + n) % n;
+                if (iDist > 0 && iDist < lDist) {
+                    int ik = start[si] + iLoc;
+                    d2 pik = P(pos, ik), pzik = P(pos, next[ik]);
+                    d2 t = { pzik.x - pik.x, pzik.y - pik.y };
+                    d2 rik = wrap2(t);
+                    double t1k = tu[2 * k + 1];
+                    t.x = pik.x + t1k * rik.x; t.y = pik.y + t1k * rik.y;
+                    d2 sp = wrap2(t);
+                    if (areaMode) sp = spr[k];
+                    le += hfun(pm, pn, sp);
+                    d2 g1, g2;
+                    g12(pm, pn, sp, &g1, &g2);
+                    if (areaMode) { g1.x *= 0.5; g1.y *= 0.5; g2.x *= 0.5; g2.y *= 0.5; }
+                    lm.x -= g1.x; lm.y -= g1.y;
+                    ln.x -= g2.x; ln.y -= g2.y;
+                }
+            }
+        }
+        LM[m] = lm; LN[m] = ln; eint[m] = le;
+    }
+    /* canonical per-vertex accumulation */
+    double* eedge = (double*)calloc(V > 0 ? V : 1, sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (int m = 0; m < V; ++m) {
+        int s = shapeId[m];
+        d2 F = { 0.0, 0.0 };
+        if (overlap) {
+            if (shapeStart[s] <= shapeEnd[s])
+                for (int r = shapeStart[s]; r <= shapeEnd[s]; ++r) {
+                    if (self[r]) continue;
+                    for (int q = 0; q < 4; ++q)
+                        if (tgt[8 * r + q] == m) { F.x += cc[8 * r + q].x; F.y += cc[8 * r + q].y; }
+                }
+            for (int e = sjCount[s]; e < sjCount[s + 1]; ++e) {
+                int r = sjList[e];
+                if (self[r]) continue;
+                for (int q = 4; q < 8; ++q)
+                    if (tgt[8 * r + q] == m) { F.x += cc[8 * r + q].x; F.y += cc[8 * r + q].y; }
+            }
+            if (LM[m].x != 0.0 || LM[m].y != 0.0) { F.x += LM[m].x; F.y += LM[m].y; }
+            int pm = prev[m];
+            if (LN[pm].x != 0.0 || LN[pm].y != 0.0) { F.x += LN[pm].x; F.y += LN[pm].y; }
+        }
+        if (edgeOn) {
+            int prv = prev[m], nxt = next[m];
+            double l0 = targetEdgeV[m], l0prv = targetEdgeV[prv];
+            double l = edgeLengths[m], prvl = edgeLengths[prv];
+            d2 a = { pos[2 * m] - pos[2 * nxt], pos[2 * m + 1] - pos[2 * nxt + 1] };
+            d2 b = { pos[2 * prv] - pos[2 * m], pos[2 * prv + 1] - pos[2 * m + 1] };
+            a.x += 1.5; a.y += 1.5; b.x += 1.5; b.y += 1.5;
+            a.x = shift_into_unit(a.x); a.y = shift_into_unit(a.y);
+            b.x = shift_into_unit(b.x); b.y = shift_into_unit(b.y);
+            a.x -= 0.5; a.y -= 0.5; b.x -= 0.5; b.y -= 0.5;
+            double c1 = (l > 1e-14) ? (1.0 - l0 / l) : 0.0;
+            double c2 = (prvl > 1e-14) ? (1.0 - l0prv / prvl) : 0.0;
+            eedge[m] = stiffness * (l - l0) * (l - l0) / 2.0;
+            d2 lf;
+            lf.x = -(c1 * a.x - c2 * b.x) * stiffness;
+            lf.y = -(c1 * a.y - c2 * b.y) * stiffness;
+            if (lf.x != 0.0 || lf.y != 0.0) { F.x += lf.x; F.y += lf.y; }
+        }
+        if (areaOn) {
+            double A = areas[s], A0 = targetAreas[s];
+            if (!(A0 < 1e-14)) {
+                double coeff = 0.5 * compressibility * (A - A0);
+                double fx = -coeff * orc_wrap(pos[2 * next[m] + 1] - pos[2 * prev[m] + 1]);
+                double fy = -coeff * orc_wrap(pos[2 * prev[m]] - pos[2 * next[m]]);
+                F.x += fx; F.y += fy;
+            }
+        }
+        force[2 * m] = F.x; force[2 * m + 1] = F.y;
+    }
+    /* energies per polygon, then tree over polygons */
+    double* ep = (double*)malloc(sizeof(double) * (N > 0 ? N : 1));
+    for (int s = 0; s < N; ++s) {
+        double acc = 0.0;
+        if (overlap) {
+            if (shapeStart[s] <= shapeEnd[s])
+                for (int r = shapeStart[s]; r <= shapeEnd[s]; ++r) acc += eext[r];
+            for (int m = start[s]; m < start[s + 1]; ++m) acc += eint[m];
+        }
+        double e = 0.5 * acc;
+        if (edgeOn) for (int m = start[s]; m < start[s + 1]; ++m) e += eedge[m];
+        if (areaOn) {
+            double A = areas[s], A0 = targetAreas[s];
+            if (!(A0 < 1e-14)) e += compressibility * (A - A0) * (A - A0) / 2.0;
+        }
+        ep[s] = e;
+    }
+    double E = orc_det_sum(ep, N);
+    free(ep); free(eedge); free(LM); free(LN); free(eint); free(sjCount); free(sjList); free(fill);
+    free(shapeStart); free(shapeEnd); free(tgt); free(cc); free(eext); free(self); free(spr);
+    free(loopMode);
+    free(shapeId); free(next); free(prev);
+    return E;
+}
+
+/* ---------- force projection: getConstrainedForcePython pyCudaPolygon.py:441-487 ----------
+ * Per polygon: Householder QR of M = [edge-length gradients | area gradient]
+ * (2n x (n+1)), f <- f - Q Q^T f.  Equal (up to rounding) to the reference's
+ * projectForceCUDA model.cu:165-206 (SVD basis of the edge gradients + Gram-Schmidt
+ * of the normalized area gradient, kernels.cuh:389-523), which is basis-independent. */
+void orc_project_force(int V, int N, const int* start, const double* pos, double* force) {
+    (void)V;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int s = 0; s < N; ++s) {
+        int st = start[s], n = start[s + 1] - st;
+        int rows = 2 * n, cols = n + 1;
+        double* M = (double*)calloc((size_t)rows * cols, sizeof(double));
+        for (int e = 0; e < n; ++e) {
+            int a = st + e, b = st + (e + 1) % n;
+            double dx = orc_wrap(pos[2 * b] - pos[2 * a]);
+            double dy = orc_wrap(pos[2 * b + 1] - pos[2 * a + 1]);
+            double len = sqrt(dx * dx + dy * dy);
+            if (len < 1e-14) continue;
+            double tx = dx / len, ty = dy / len;
+            M[(2 * e) * cols + e] = -tx;
+            M[(2 * e + 1) * cols + e] = -ty;
+            M[(2 * ((e + 1) % n)) * cols + e] = tx;
+            M[(2 * ((e + 1) % n) + 1) * cols + e] = ty;
+        }
+        for (int k = 0; k < n; ++k) {
+            int nx = st + (k + 1) % n, pv = st + (k + n - 1) % n;
+            M[(2 * k) * cols + n] = orc_wrap(pos[2 * nx + 1] - pos[2 * pv + 1]) / 2;
+            M[(2 * k + 1) * cols + n] = -orc_wrap(pos[2 * nx] - pos[2 * pv]) / 2;
+        }
+        /* Householder: store reflectors, apply Q^T to f, zero the first cols entries, apply Q */
+        double* f = (double*)malloc(sizeof(double) * rows);
+        for (int r = 0; r < rows; ++r) f[r] = force[2 * st + r];
+        double* vk = (double*)malloc(sizeof(double) * rows * cols);
+        double* beta = (double*)malloc(sizeof(double) * cols);
+        for (int k = 0; k < cols; ++k) {
+            double nrm = 0.0;
+            for (int r = k; r < rows; ++r) nrm += M[r * cols + k] * M[r * cols + k];
+            nrm = sqrt(nrm);
+            double* v = vk + (size_t)k * rows;
+            for (int r = 0; r < rows; ++r) v[r] = 0.0;
+            if (nrm == 0.0) { beta[k] = 0.0; continue; }
+            double alpha = (M[k * cols + k] > 0) ? -nrm : nrm;
+            for (int r = k; r < rows; ++r) v[r] = M[r * cols + k];
+            v[k] -= alpha;
+            double vv = 0.0;
+            for (int r = k; r < rows; ++r) vv += v[r] * v[r];
+            beta[k] = (vv > 0.0) ? 2.0 / vv : 0.0;
+            for (int c = k; c < cols; ++c) {
+                double d = 0.0;
+                for (int r = k; r < rows; ++r) d += v[r] * M[r * cols + c];
+                d *= beta[k];
+                for (int r = k; r < rows; ++r) M[r * cols + c] -= d * v[r];
+            }
+        }
+        /* g = Q^T f */
+        for (int k = 0; k < cols; ++k) {
+            double* v = vk + (size_t)k * rows;
+            double d = 0.0;
+            for (int r = k; r < rows; ++r) d += v[r] * f[r];
+            d *= beta[k];
+            for (int r = k; r < rows; ++r) f[r] -= d * v[r];
+        }
+        for (int k = 0; k < cols; ++k) f[k] = 0.0;   /* drop the span components */
+        for (int k = cols - 1; k >= 0; --k) {       /* back to the original basis */
+            double* v = vk + (size_t)k * rows;
+            double d = 0.0;
+            for (int r = k; r < rows; ++r) d += v[r] * f[r];
+            d *= beta[k];
+            for (int r = k; r < rows; ++r) f[r] -= d * v[r];
+        }
+        for (int r = 0; r < rows; ++r) force[2 * st + r] = f[r];
+        free(M); free(f); free(vk); free(beta);
+    }
+}
+
+/* ---------- SHAKE (shake.h is absent: src/model.cu:16) ----------
+ * Contract from shakeProjectCUDA model.cu:619-636 and Model::minimizeFIREStep
+ * model.cpp:418-438: per polygon, project positions onto {|edge e| = L_e for all e,
+ * area = A0} (n+1 constraints), Newton iterations with the (n+1)x(n+1) system
+ * J J^T lambda = C, x -= J^T lambda, until max|C| < tol; returns the largest
+ * iteration count (maxIter when some polygon did not converge).
+ * This oracle solves the system densely (Gaussian elimination, partial pivoting),
+ * independently of the structured solver in the HIP kernel. */
+static double poly_area(const double* pos, int st, int n) {   /* kernels.cuh:302-332 */
+    double area = 0.0;
+    for (int k = 0; k < n; ++k) {
+        int a = st + k, b = st + (k + 1) % n;
+        double x1 = orc_wrap(pos[2 * a] - pos[2 * st]);
+        double dy2 = orc_wrap(pos[2 * b + 1] - pos[2 * a + 1]);
+        double x2 = orc_wrap(pos[2 * b] - pos[2 * st]);
+        area += dy2 * (x2 + x1) / 2.0;
+    }
+    return area;
+}
+static int solve_dense(double* A, double* b, int n) {
+    for (int c = 0; c < n; ++c) {
+        int p = c; double best = fabs(A[c * n + c]);
+        for (int r = c + 1; r < n; ++r) if (fabs(A[r * n + c]) > best) { best = fabs(A[r * n + c]); p = r; }
+        if (best == 0.0) return -1;
+        if (p != c) {
+            for (int k = 0; k < n; ++k) { double t = A[c * n + k]; A[c * n + k] = A[p * n + k]; A[p * n + k] = t; }
+            double t = b[c]; b[c] = b[p]; b[p] = t;
+        }
+        for (int r = c + 1; r < n; ++r) {
+            double f = A[r * n + c] / A[c * n + c];
+            for (int k = c; k < n; ++k) A[r * n + k] -= f * A[c * n + k];
+            b[r] -= f * b[c];
+        }
+    }
+    for (int c = n - 1; c >= 0; --c) {
+        double s = b[c];
+        for (int k = c + 1; k < n; ++k) s -= A[c * n + k] * b[k];
+        b[c] = s / A[c * n + c];
+    }
+    return 0;
+}
+int orc_shake(int V, int N, const int* start, double* pos, const double* targetEdgeV,
+              const double* targetAreas, int maxIter, double tol) {
+    (void)V;
+    if (N == 0 || maxIter == 0) return 0;
+    int worst = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(max : worst)
+    for (int s = 0; s < N; ++s) {
+        int st = start[s], n = start[s + 1] - st, nc = n + 1, rows = 2 * n;
+        double* J = (double*)malloc(sizeof(double) * nc * rows);
+        double* A = (double*)malloc(sizeof(double) * nc * nc);
+        double* C = (double*)malloc(sizeof(double) * nc);
+        int it = 0;
+        for (; it < maxIter; ++it) {
+            for (int q = 0; q < nc * rows; ++q) J[q] = 0.0;
+            double cmax = 0.0;
+            for (int e = 0; e < n; ++e) {
+                int a = st + e, b = st + (e + 1) % n;
+                double dx = orc_wrap(pos[2 * b] - pos[2 * a]);
+                double dy = orc_wrap(pos[2 * b + 1] - pos[2 * a + 1]);
+                double len = sqrt(dx * dx + dy * dy);
+                C[e] = len - targetEdgeV[a];
+                double tx = len > 0 ? dx / len : 0.0, ty = len > 0 ? dy / len : 0.0;
+                J[e * rows + 2 * e] -= tx; J[e * rows + 2 * e + 1] -= ty;
+                J[e * rows + 2 * ((e + 1) % n)] += tx; J[e * rows + 2 * ((e + 1) % n) + 1] += ty;
+                if (fabs(C[e]) > cmax) cmax = fabs(C[e]);
+            }
+            C[n] = poly_area(pos, st, n) - targetAreas[s];
+            if (fabs(C[n]) > cmax) cmax = fabs(C[n]);
+            for (int k = 0; k < n; ++k) {
+                int nx = st + (k + 1) % n, pv = st + (k + n - 1) % n;
+                J[n * rows + 2 * k] = orc_wrap(pos[2 * nx + 1] - pos[2 * pv + 1]) / 2;
+                J[n * rows + 2 * k + 1] = orc_wrap(pos[2 * pv] - pos[2 * nx]) / 2;
+            }
+            if (cmax < tol) break;
+            for (int a = 0; a < nc; ++a)
+                for (int b = 0; b < nc; ++b) {
+                    double d = 0.0;
+                    for (int r = 0; r < rows; ++r) d += J[a * rows + r] * J[b * rows + r];
+                    A[a * nc + b] = d;
+                }
+            if (solve_dense(A, C, nc) != 0) { it = maxIter; break; }
+            for (int r = 0; r < rows; ++r) {
+                double d = 0.0;
+                for (int a = 0; a < nc; ++a) d += J[a * rows + r] * C[a];
+                double p = pos[2 * st + r] - d;
+                pos[2 * st + r] = p - floor(p);
+            }
+        }
+        if (it > worst) worst = it;
+        free(J); free(A); free(C);
+    }
+    return worst;
+}
+
+/* ---------- XPBD: xpbdProjectCUDA model.cu:209-278, kernels.cuh:532-623 ----------
+ * Colour 0 then colour 1 edge projections (disjoint within a colour for even n;
+ * here applied in ascending edge order, which is the reference's result for even n),
+ * then the area correction from the area/gradient reductions. */
+int orc_xpbd(int V, int N, const int* start, double* pos, const double* targetEdgeP,
+             const double* targetAreas, int nIter, double tol) {
+    (void)V;
+    int iter = 0;
+    for (; iter < nIter; ++iter) {
+        for (int color = 0; color < 2; ++color)
+            for (int s = 0; s < N; ++s) {
+                int st = start[s], n = start[s + 1] - st;
+                for (int e = color; e < n; e += 2) {
+                    int k = st + e, nk = st + (e + 1) % n;
+                    double ex = orc_wrap(pos[2 * nk] - pos[2 * k]);
+                    double ey = orc_wrap(pos[2 * nk + 1] - pos[2 * k + 1]);
+                    double len = sqrt(ex * ex + ey * ey);
+                    if (len < 1e-14) continue;
+                    double c = 0.5 * (len - targetEdgeP[s]) / len;
+                    pos[2 * k] += c * ex; pos[2 * k + 1] += c * ey;
+                    pos[2 * nk] -= c * ex; pos[2 * nk + 1] -= c * ey;
+                }
+            }
+        double maxA = 0.0, maxE = 0.0;
+        for (int s = 0; s < N; ++s) {
+            int st = start[s], n = start[s + 1] - st;
+            double xs = pos[2 * st], ys = pos[2 * st + 1];
+            double area = 0.0, g2 = 0.0;
+            for (int e = 0; e < n; ++e) {
+                int k = st + e, nk = st + (e + 1) % n, pk = st + (e + n - 1) % n;
+                double xk = orc_wrap(pos[2 * k] - xs), yk = orc_wrap(pos[2 * k + 1] - ys);
+                double xnk = orc_wrap(pos[2 * nk] - xs), ynk = orc_wrap(pos[2 * nk + 1] - ys);
+                double xpk = orc_wrap(pos[2 * pk] - xs), ypk = orc_wrap(pos[2 * pk + 1] - ys);
+                area += 0.5 * (xk * ynk - xnk * yk);
+                double gx = 0.5 * (ynk - ypk), gy = 0.5 * (xpk - xnk);
+                g2 += gx * gx + gy * gy;
+            }
+            double alpha = (area - targetAreas[s]) / fmax(g2, 1e-24);
+            double* old = (double*)malloc(sizeof(double) * 2 * n);
+            for (int q = 0; q < 2 * n; ++q) old[q] = pos[2 * st + q];
+            for (int e = 0; e < n; ++e) {
+                int nk = (e + 1) % n, pk = (e + n - 1) % n;
+                double xnk = orc_wrap(old[2 * nk] - xs), ynk = orc_wrap(old[2 * nk + 1] - ys);
+                double xpk = orc_wrap(old[2 * pk] - xs), ypk = orc_wrap(old[2 * pk + 1] - ys);
+                double gx = 0.5 * (ynk - ypk), gy = 0.5 * (xpk - xnk);
+                pos[2 * (st + e)] -= alpha * gx;
+                pos[2 * (st + e) + 1] -= alpha * gy;
+            }
+            free(old);
+            double dA = fabs(area - targetAreas[s]);
+            if (dA > maxA) maxA = dA;
+        }
+        if (tol > 0) {
+            for (int s = 0; s < N; ++s) {
+                int st = start[s], n = start[s + 1] - st;
+                for (int e = 0; e < n; ++e) {
+                    int k = st + e, nk = st + (e + 1) % n;
+                    double ex = orc_wrap(pos[2 * nk] - pos[2 * k]);
+                    double ey = orc_wrap(pos[2 * nk + 1] - pos[2 * k + 1]);
+                    double d = fabs(sqrt(ex * ex + ey * ey) - targetEdgeP[s]);
+                    if (d > maxE) maxE = d;
+                }
+            }
+            if (maxE < tol && maxA < tol) break;
+        }
+    }
+    return iter;
+}
+
+/* ---------- Monte-Carlo overlap counter: updateOverlapAreaKernel kernels.cuh:879-979 ---------- */
+static double wrapPeriodic(double x) {   /* kernels.cuh:108-113 */
+    x += 1.5;
+    while (x > 1.0) x -= 1.0;
+    return x - 0.5;
+}
+long long orc_overlap_count(int V, int N, const int* start, const double* pos, int boxSize,
+                            const int* neighborIndices, const int* countPerBox,
+                            int pointDensity, int* counter) {
+    const double eps = 1e-12, pi = 3.141592653589793238462643383279;
+    int* shapeId = (int*)malloc(sizeof(int) * (V > 0 ? V : 1));
+    for (int s = 0; s < N; ++s) for (int i = start[s]; i < start[s + 1]; ++i) shapeId[i] = s;
+    int total = pointDensity * pointDensity, boxCount = boxSize * boxSize;
+    long long sum = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : sum)
+    for (int idx = 0; idx < total; ++idx) {
+        int ix = idx % pointDensity, iy = idx / pointDensity;
+        double px = (ix + 0.5) / pointDensity, py = (iy + 0.5) / pointDensity;
+        int num = 0;
+        int cellX = (int)(px * boxSize), cellY = (int)(py * boxSize);
+        int tracked[512], nt = 0, full = 0;
+        for (int dx = -1; dx <= 1 && !full; dx++) {
+            int nx = cellX + dx;
+            if (nx < 0) nx += boxSize; else if (nx >= boxSize) nx -= boxSize;
+            for (int dy = -1; dy <= 1 && !full; dy++) {
+                int ny = cellY + dy;
+                if (ny < 0) ny += boxSize; else if (ny >= boxSize) ny -= boxSize;
+                int boxId = ny * boxSize + nx;
+                int si = countPerBox[boxId];
+                int sf = (boxId + 1 < boxCount) ? countPerBox[boxId + 1] : V;
+                for (int q = si; q < sf && !full; q++) {
+                    int poly = shapeId[neighborIndices[q]];
+                    int seen = 0;
+                    for (int t = 0; t < nt; t++) if (tracked[t] == poly) { seen = 1; break; }
+                    if (seen) continue;
+                    int st = start[poly], en = start[poly + 1];
+                    if (en - st < 3) continue;
+                    double angleSum = 0.0;
+                    double vxPrev = wrapPeriodic(pos[2 * st] - px);
+                    double vyPrev = wrapPeriodic(pos[2 * st + 1] - py);
+                    double vx0 = vxPrev + 0.0, vy0 = vyPrev + 0.0;
+                    for (int e = st + 1; e < en; e++) {
+                        double vx = vxPrev + wrapPeriodic(pos[2 * e] - pos[2 * e - 2]);
+                        double vy = vyPrev + wrapPeriodic(pos[2 * e + 1] - pos[2 * e - 1]);
+                        double cross = vxPrev * vy - vyPrev * vx;
+                        double dot = vxPrev * vx + vyPrev * vy;
+                        angleSum += atan2(cross, dot);
+                        vxPrev = vx; vyPrev = vy;
+                    }
+                    angleSum += atan2(vxPrev * vy0 - vyPrev * vx0, vxPrev * vx0 + vyPrev * vy0);
+                    if (fabs(angleSum - 2.0 * pi) < eps) num++;
+                    if (nt < 512) tracked[nt++] = poly; else full = 1;
+                }
+            }
+        }
+        int c = num * (num - 1) / 2;
+        if (counter) counter[idx] = c;
+        sum += c;
+    }
+    free(shapeId);
+    return sum;
+}
+
+/* ---------- one force evaluation + FIRE ---------- */
+typedef struct {
+    int* cellLocation; int* neighborIndices; int* countPerBox;
+    int M; int* neighbors; unsigned char* inside; double* tuSlots; int* numNeighbors;
+    uint64_t* inter; uint64_t* outer; double* tu; double* ut; int* shapeCounts;
+    double* edgeLengths; double* areas; double* comX; double* comY; double* constraints;
+    int I;
+} orc_ws;
+
+static void ws_free(orc_ws* w) {
+    free(w->cellLocation); free(w->neighborIndices); free(w->countPerBox);
+    free(w->neighbors); free(w->inside); free(w->tuSlots); free(w->numNeighbors);
+    free(w->inter); free(w->outer); free(w->tu); free(w->ut); free(w->shapeCounts);
+    free(w->edgeLengths); free(w->areas); free(w->comX); free(w->comY); free(w->constraints);
+}
+static void ws_alloc(orc_ws* w, int V, int N, int B, int M) {
+    memset(w, 0, sizeof(*w));
+    int Vp = V > 0 ? V : 1, Np = N > 0 ? N : 1;
+    w->cellLocation = (int*)malloc(sizeof(int) * Vp);
+    w->neighborIndices = (int*)malloc(sizeof(int) * Vp);
+    w->countPerBox = (int*)malloc(sizeof(int) * B * B);
+    w->M = M;
+    w->neighbors = (int*)malloc(sizeof(int) * (size_t)Vp * M);
+    w->inside = (unsigned char*)malloc((size_t)Vp * M);
+    w->tuSlots = (double*)malloc(sizeof(double) * 2 * (size_t)Vp * M);
+    w->numNeighbors = (int*)malloc(sizeof(int) * Vp);
+    w->inter = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)Vp * M);
+    w->outer = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)Vp * M);
+    w->tu = (double*)malloc(sizeof(double) * 2 * (size_t)Vp * M);
+    w->ut = (double*)malloc(sizeof(double) * 2 * (size_t)Vp * M);
+    w->shapeCounts = (int*)malloc(sizeof(int) * Np);
+    w->edgeLengths = (double*)malloc(sizeof(double) * Vp);
+    w->areas = (double*)malloc(sizeof(double) * Np);
+    w->comX = (double*)malloc(sizeof(double) * Np);
+    w->comY = (double*)malloc(sizeof(double) * Np);
+    w->constraints = (double*)malloc(sizeof(double) * 6 * Vp);
+}
+
+static void detect(const orc_system* s, const double* pos, orc_ws* w) {
+    orc_cells(s->V, s->N, pos, s->start, s->boxSize, w->cellLocation, w->neighborIndices, w->countPerBox);
+    int mx = orc_neighbors(s->V, s->N, pos, s->start, s->boxSize, w->cellLocation, w->neighborIndices,
+                           w->countPerBox, w->M, w->neighbors, w->inside, w->tuSlots, w->numNeighbors);
+    if (mx > w->M) {   /* model.cpp:264-305: grow and retry once */
+        int newM = (2 * w->M + 1 > mx) ? 2 * w->M + 1 : mx;
+        ws_free(w);
+        ws_alloc(w, s->V, s->N, s->boxSize, newM);
+        orc_cells(s->V, s->N, pos, s->start, s->boxSize, w->cellLocation, w->neighborIndices, w->countPerBox);
+        orc_neighbors(s->V, s->N, pos, s->start, s->boxSize, w->cellLocation, w->neighborIndices,
+                      w->countPerBox, w->M, w->neighbors, w->inside, w->tuSlots, w->numNeighbors);
+    }
+}
+static void records(const orc_system* s, orc_ws* w, const double* pos) {
+    const double* ex = (s->model & ORC_AREA_ENERGY) ? pos : NULL;
+    w->I = orc_outersections_ex(s->V, s->N, s->start, w->M, w->neighbors, w->inside, w->tuSlots,
+                                w->numNeighbors, w->inter, w->tu, w->outer, w->ut, w->shapeCounts, ex);
+}
+static void geometry(const orc_system* s, const double* pos, orc_ws* w) {
+    orc_geometry(s->V, s->N, pos, s->start, w->edgeLengths, w->areas, w->comX, w->comY,
+                 w->constraints, NULL, NULL);
+}
+static double forces(const orc_system* s, const double* pos, orc_ws* w, double* force) {
+    double E = orc_force_energy(s->model, s->V, s->N, s->start, pos, w->I, w->inter, w->outer,
+                                w->tu, w->ut, w->edgeLengths, w->areas, s->targetEdgeV, s->targetAreas,
+                                s->stiffness, s->compressibility, force);
+    if ((s->model & 0xFF) == ORC_NORMAL) orc_project_force(s->V, s->N, s->start, pos, force);
+    return E;
+}
+
+double orc_force_eval(const orc_system* s, const double* pos, double* force, int* numIntersections) {
+    orc_ws w;
+    ws_alloc(&w, s->V, s->N, s->boxSize, s->maxNeighbors > 0 ? s->maxNeighbors : 100);
+    geometry(s, pos, &w);
+    int overlap = ((s->model & 0xFF) == ORC_NORMAL || (s->model & 0xFF) == ORC_HYBRID);
+    if (overlap) { detect(s, pos, &w); records(s, &w, pos); }
+    double E = forces(s, pos, &w, force);
+    if (numIntersections) *numIntersections = w.I;
+    ws_free(&w);
+    return E;
+}
+
+/* per-polygon sequential dot, then tree over polygons */
+static double fire_dot(const orc_system* s, const double* a, const double* b) {
+    double* pp = (double*)malloc(sizeof(double) * (s->N > 0 ? s->N : 1));
+    for (int q = 0; q < s->N; ++q) {
+        double acc = 0.0;
+        for (int m = s->start[q]; m < s->start[q + 1]; ++m) {
+            acc += a[2 * m] * b[2 * m];
+            acc += a[2 * m + 1] * b[2 * m + 1];
+        }
+        pp[q] = acc;
+    }
+    double r = orc_det_sum(pp, s->N);
+    free(pp);
+    return r;
+}
+
+/* minimizeFIRE's prologue model.cpp:489-496 */
+void orc_fire_init(const orc_system* s, double* pos, double* vel, double* force, orc_fire_state* st) {
+    for (int q = 0; q < 2 * s->V; ++q) vel[q] = 0.0;
+    int nI = 0;
+    st->energy = orc_force_eval(s, pos, force, &nI);
+    st->lastShakeIters = 0;
+    st->shakeFailed = 0;
+    st->rolledBack = 0;
+}
+
+/* Model::minimizeFIREStep model.cpp:405-486.  The FIRE kernels (FIRE.h) are absent
+ * from the reference; their semantics are fixed here from the launch signatures
+ * model.cu:576-608 and Bitzek et al., PRL 97, 170201 (2006):
+ *   updatePositionAndVelocity:  vh = v + (0.5*dt)*F; x = frac(x + dt*vh); v = vh
+ *   updateVelocity:             v = v + (0.5*dt)*F
+ *   bendVelocityTowardsForce:   P = F.v, |v|, |F|; v = (1-a)*v + ((a*|v|)/|F|)*F
+ *   rederiveVelocity:           v = v + wrap(x_shaken - x_unshaken)/dt            */
+void orc_fire_step(const orc_system* s, double* pos, double* vel, double* force, orc_fire_state* st,
+                   double dtMax, double alphaStart, double fAlpha, double fInc, double fDec,
+                   int nMin, int shakeIter) {
+    int V2 = 2 * s->V;
+    int needsInter = ((s->model & 0xFF) == ORC_NORMAL || (s->model & 0xFF) == ORC_HYBRID);
+    int rigid = ((s->model & 0xFF) == ORC_NORMAL);
+    double dt = st->dt;
+    double* bak = (double*)malloc(sizeof(double) * V2);
+    double* tmp = (double*)malloc(sizeof(double) * V2);
+    memcpy(bak, pos, sizeof(double) * V2);
+    double ePre = st->energy;
+    for (int q = 0; q < V2; ++q) {
+        double vh = vel[q] + (0.5 * dt) * force[q];
+        double p = pos[q] + dt * vh;
+        pos[q] = p - floor(p);
+        vel[q] = vh;
+    }
+    orc_ws w;
+    ws_alloc(&w, s->V, s->N, s->boxSize, s->maxNeighbors > 0 ? s->maxNeighbors : 100);
+    int shakeFailed = 0;
+    st->rolledBack = 0;
+    int areaMode = (s->model & ORC_AREA_ENERGY) != 0;
+    if (needsInter) {
+        if (!areaMode) detect(s, pos, &w);   /* reference order: detect, SHAKE, records */
+        if (rigid && shakeIter > 0) {
+            memcpy(tmp, pos, sizeof(double) * V2);
+            int it = orc_shake(s->V, s->N, s->start, pos, s->targetEdgeV, s->targetAreas, shakeIter, 1e-15);
+            st->lastShakeIters = it;
+            if (it >= shakeIter) {
+                it = orc_shake(s->V, s->N, s->start, pos, s->targetEdgeV, s->targetAreas, shakeIter * 10, 1e-15);
+                st->lastShakeIters = it;
+                if (it >= shakeIter * 10) shakeFailed = 1;
+            }
+            if (shakeFailed) {
+                memcpy(pos, bak, sizeof(double) * V2);
+                for (int q = 0; q < V2; ++q) vel[q] = 0.0;
+            } else {
+                for (int q = 0; q < V2; ++q) vel[q] = vel[q] + orc_wrap(pos[q] - tmp[q]) / dt;
+            }
+        }
+        if (areaMode) detect(s, pos, &w);    /* area energy: crossings of the final positions */
+        records(s, &w, pos);
+    }
+    geometry(s, pos, &w);
+    double ePost = forces(s, pos, &w, force);
+    st->energy = ePost;
+    st->shakeFailed = shakeFailed;
+    if (shakeFailed) {
+        st->dt = dt * fDec; st->alpha = alphaStart; st->nPos = 0;
+    } else if (ePost > ePre + fmax(fabs(ePre) * 1e-10, 1e-14)) {
+        memcpy(pos, bak, sizeof(double) * V2);
+        geometry(s, pos, &w);
+        if (needsInter) { detect(s, pos, &w); records(s, &w, pos); }
+        st->energy = forces(s, pos, &w, force);
+        for (int q = 0; q < V2; ++q) vel[q] = 0.0;
+        st->dt = dt * fDec; st->alpha = alphaStart; st->nPos = 0;
+        st->rolledBack = 1;
+    } else {
+        for (int q = 0; q < V2; ++q) vel[q] = vel[q] + (0.5 * dt) * force[q];
+        double Pw = fire_dot(s, force, vel);
+        double vnorm = sqrt(fire_dot(s, vel, vel));
+        double fnorm = sqrt(fire_dot(s, force, force));
+        double a = st->alpha;
+        double scale = (fnorm > 0.0) ? (a * vnorm) / fnorm : 0.0;
+        for (int q = 0; q < V2; ++q) vel[q] = (1.0 - a) * vel[q] + scale * force[q];
+        if (Pw > 0.0) {
+            if (++st->nPos >= nMin) {
+                double newDt = fmin(dt * fInc, dtMax);
+                if (newDt > dt) st->alpha = st->alpha * fAlpha;
+                dt = newDt;
+            }
+            st->dt = dt;
+        } else {
+            for (int q = 0; q < V2; ++q) vel[q] = 0.0;
+            st->dt = dt * fDec; st->alpha = alphaStart; st->nPos = 0;
+        }
+    }
+    ws_free(&w);
+    free(bak); free(tmp);
+}
+
+/* Model::minimizeFIRE model.cpp:488-504 (convergence on max |force component|,
+ * model.cu:612-614).  Returns the number of steps taken. */
+int orc_fire_minimize(const orc_system* s, double* pos, double* vel, double* force,
+                      orc_fire_state* st, double fTol, int maxSteps, double dtMax,
+                      double alphaStart, double fAlpha, double fInc, double fDec, int nMin,
+                      int shakeIter) {
+    st->alpha = alphaStart;
+    st->nPos = 0;
+    orc_fire_init(s, pos, vel, force, st);
+    for (int step = 0; step < maxSteps; ++step) {
+        orc_fire_step(s, pos, vel, force, st, dtMax, alphaStart, fAlpha, fInc, fDec, nMin, shakeIter);
+        if (orc_max_abs(force, 2 * s->V) <= fTol) return step + 1;
+    }
+    return maxSteps;
+}

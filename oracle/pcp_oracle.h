@@ -1,0 +1,98 @@
+/*
+ * CPU oracle for the overlap-force-minimization path of camdennis/pyCudaPolygon.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: tests/, the smoke()
+ * entry point and bench.py's cpu_baseline leg may load it; the product path
+ * (pycudapolygon_amd) never does.  Every function restates the reference's
+ * algorithm (file:line cited at each definition in pcp_oracle.c) in plain C,
+ * compiled with -ffp-contract=off so that each IEEE operation happens exactly
+ * as written.
+ *
+ * Layouts follow the reference: positions/forces are AoS double[2V]
+ * ([x0,y0,x1,y1,...]), startIndices int[N+1], one packing per call.
+ */
+#ifndef PCP_ORACLE_H
+#define PCP_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* reference enumTypes.h:6 ordinal order */
+enum { ORC_NORMAL = 0, ORC_EDGE_ONLY = 1, ORC_AREA_ONLY = 2, ORC_SOFT_BODY = 3,
+       ORC_ABNORMAL = 4, ORC_HYBRID = 5 };
+/* model flag: overlap energy measured from one origin per polygon pair (the first
+ * vertex of the lower-numbered polygon) instead of each record's entry point, with
+ * forces halved -- energy = exact overlap area and force = -grad(energy). */
+#define ORC_AREA_ENERGY 0x100
+
+double orc_wrap(double x);
+void orc_topology(int N, const int* start, int V, int* shapeId, int* next, int* prev);
+void orc_geometry(int V, int N, const double* pos, const int* start,
+                  double* edgeLengths, double* areas, double* comX, double* comY,
+                  double* constraints, double* constraintNormSq, double* maxEdgeLength);
+void orc_cells(int V, int N, const double* pos, const int* start, int boxSize,
+               int* cellLocation, int* neighborIndices, int* countPerBox);
+int orc_neighbors(int V, int N, const double* pos, const int* start, int boxSize,
+                  const int* cellLocation, const int* neighborIndices, const int* countPerBox,
+                  int maxNeighbors, int* neighbors, unsigned char* inside, double* tu,
+                  int* numNeighbors);
+int orc_outersections(int V, int N, const int* start, int maxNeighbors,
+                      const int* neighbors, const unsigned char* inside, const double* tuSlots,
+                      const int* numNeighbors, uint64_t* intersections, double* tu,
+                      uint64_t* outersections, double* ut, int* shapeCounts);
+int orc_outersections_ex(int V, int N, const int* start, int maxNeighbors,
+                         const int* neighbors, const unsigned char* inside, const double* tuSlots,
+                         const int* numNeighbors, uint64_t* intersections, double* tu,
+                         uint64_t* outersections, double* ut, int* shapeCounts,
+                         const double* pairExactPos);
+double orc_force_energy(int model, int V, int N, const int* start, const double* pos,
+                        int I, const uint64_t* intersections, const uint64_t* outersections,
+                        const double* tu, const double* ut, const double* edgeLengths,
+                        const double* areas, const double* targetEdgeV, const double* targetAreas,
+                        double stiffness, double compressibility, double* force);
+void orc_project_force(int V, int N, const int* start, const double* pos, double* force);
+int orc_shake(int V, int N, const int* start, double* pos, const double* targetEdgeV,
+              const double* targetAreas, int maxIter, double tol);
+int orc_xpbd(int V, int N, const int* start, double* pos, const double* targetEdgeP,
+             const double* targetAreas, int nIter, double tol);
+double orc_det_sum(const double* x, int n);
+double orc_max_abs(const double* x, int n);
+long long orc_overlap_count(int V, int N, const int* start, const double* pos, int boxSize,
+                            const int* neighborIndices, const int* countPerBox,
+                            int pointDensity, int* counter);
+
+/* One force evaluation as Model::minimizeFIREStep performs it: geometry, cell list,
+ * neighbours, records, force/energy (+projection for the normal model).
+ * Workspace is allocated internally.  Returns the energy. */
+typedef struct {
+    int V, N, boxSize, model, maxNeighbors;
+    const int* start;
+    const double* targetEdgeV;   /* [V] per-edge target length (edge m -> next(m)) */
+    const double* targetAreas;   /* [N] */
+    double stiffness, compressibility;
+} orc_system;
+double orc_force_eval(const orc_system* s, const double* pos, double* force, int* numIntersections);
+
+/* FIRE (host contract model.cpp:405-504) */
+typedef struct {
+    double dt, alpha;
+    int nPos;
+    double energy;
+    int lastShakeIters;
+    int shakeFailed, rolledBack;
+} orc_fire_state;
+void orc_fire_init(const orc_system* s, double* pos, double* vel, double* force, orc_fire_state* st);
+void orc_fire_step(const orc_system* s, double* pos, double* vel, double* force, orc_fire_state* st,
+                   double dtMax, double alphaStart, double fAlpha, double fInc, double fDec,
+                   int nMin, int shakeIter);
+int orc_fire_minimize(const orc_system* s, double* pos, double* vel, double* force,
+                      orc_fire_state* st, double fTol, int maxSteps, double dtMax,
+                      double alphaStart, double fAlpha, double fInc, double fDec, int nMin,
+                      int shakeIter);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,450 @@
+// FIRE minimization on the device (Model::minimizeFIREStep / minimizeFIRE,
+// src/model.cpp:405-504).  The FIRE kernels of FIRE.h are absent from the reference
+// (src/model.cu:15); they are written here from the launch contract model.cu:576-608
+// and Bitzek et al., PRL 97, 170201 (2006):
+//   updatePositionAndVelocity:  vh = v + (0.5*dt)*F;  x = frac(x + dt*vh);  v = vh
+//   updateVelocity:             v = v + (0.5*dt)*F
+//   bendVelocityTowardsForce:   P = F.v, |v|, |F|;  v = (1-a)*v + ((a*|v|)/|F|)*F
+//   rederiveVelocity:           v = v + wrap(x_shaken - x_unshaken)/dt
+// All per-packing state (dt, alpha, nPos, energies, flags) lives on the device; the
+// kernels skip packings that have converged, so P independent packings advance in the
+// same launches.  The host reads one small status block per step.
+//
+// This file is compiled as part of pcp_kernels.hip (one translation unit).
+
+namespace pcp {
+
+enum { FL_FAIL = 0, FL_ROLLBACK = 1, FL_ACCEPT = 2, FL_ZEROV = 3, FL_LASTSHAKE = 4, FL_ACTIVEBAK = 5,
+       FL_NF = 8 };
+enum { FS_SCALE = 0, FS_ONEMA = 1, FS_MAXF = 2, FS_DTB = 3, FS_ALB = 4, FS_NPB = 5, FS_STEPB = 6,
+       FS_NF = 16 };
+
+__global__ void k_fire_init(int P, double dtv, double alpha0, double* dt, double* alpha, int* nPos,
+                            int* steps, int* active, int* flags, double* fs) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    dt[p] = dtv;
+    alpha[p] = alpha0;
+    nPos[p] = 0;
+    steps[p] = 0;
+    active[p] = 1;
+    for (int k = 0; k < FL_NF; ++k) flags[FL_NF * p + k] = 0;
+    fs[FS_NF * p + FS_MAXF] = 0.0;
+}
+
+// per packing: backups for rollback / retry, flag reset, E_pre (model.cpp:411-412)
+__global__ void k_fire_prologue(int P, const int* __restrict__ active, const double* __restrict__ dt,
+                                const double* __restrict__ alpha, const int* __restrict__ nPos,
+                                const int* __restrict__ steps, const double* __restrict__ energy,
+                                double* __restrict__ epre, int* __restrict__ flags, double* __restrict__ fs,
+                                int* __restrict__ shakeIt, int* __restrict__ summary) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < 4 && p < 4) summary[p] = 0;
+    if (p >= P) return;
+    int* fl = flags + FL_NF * p;
+    fl[FL_FAIL] = 0;
+    fl[FL_ROLLBACK] = 0;
+    fl[FL_ACCEPT] = 0;
+    fl[FL_ZEROV] = 0;
+    fl[FL_ACTIVEBAK] = active[p];
+    shakeIt[p] = 0;
+    shakeIt[P + p] = 0;
+    if (!active[p]) return;
+    epre[p] = energy[p];
+    fs[FS_NF * p + FS_DTB] = dt[p];
+    fs[FS_NF * p + FS_ALB] = alpha[p];
+    fs[FS_NF * p + FS_NPB] = (double)nPos[p];
+    fs[FS_NF * p + FS_STEPB] = (double)steps[p];
+}
+
+// velocity-Verlet position update with backups of x and v
+__global__ void k_fire_begin(int V, const int* __restrict__ shapeId, const int* __restrict__ polyPack,
+                             const int* __restrict__ active, const double* __restrict__ dt,
+                             double2* __restrict__ x, double2* __restrict__ v, const double2* __restrict__ F,
+                             double2* __restrict__ xbak, double2* __restrict__ vbak,
+                             double2* __restrict__ xpre) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    int p = polyPack[shapeId[i]];
+    if (!active[p]) return;
+    double h = dt[p];
+    double2 xi = x[i], vi = v[i], fi = F[i];
+    xbak[i] = xi;
+    vbak[i] = vi;
+    double vhx = vi.x + (0.5 * h) * fi.x;
+    double vhy = vi.y + (0.5 * h) * fi.y;
+    double px = xi.x + h * vhx, py = xi.y + h * vhy;
+    px = px - floor(px);
+    py = py - floor(py);
+    x[i] = make_double2(px, py);
+    xpre[i] = make_double2(px, py);
+    v[i] = make_double2(vhx, vhy);
+}
+
+// after SHAKE: restore on failure (model.cpp:431-435) else rederive (model.cpp:437)
+__global__ void k_fire_post_shake(int V, int P, int shakeIter, const int* __restrict__ shapeId,
+                                  const int* __restrict__ polyPack, const int* __restrict__ packVert,
+                                  const int* __restrict__ active, const double* __restrict__ dt,
+                                  const int* __restrict__ shakeIt, double2* __restrict__ x,
+                                  double2* __restrict__ v, const double2* __restrict__ xbak,
+                                  const double2* __restrict__ xpre, int* __restrict__ flags) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    int p = polyPack[shapeId[i]];
+    if (!active[p]) return;
+    int it1 = shakeIt[p], it2 = shakeIt[P + p];
+    bool retried = it1 >= shakeIter;
+    bool fail = retried && it2 >= shakeIter * 10;
+    if (fail) {
+        x[i] = xbak[i];
+        v[i] = make_double2(0.0, 0.0);
+    } else {
+        double h = dt[p];
+        double2 xi = x[i], xp = xpre[i], vi = v[i];
+        v[i] = make_double2(vi.x + wrap(xi.x - xp.x) / h, vi.y + wrap(xi.y - xp.y) / h);
+    }
+    if (i == packVert[p]) {
+        flags[FL_NF * p + FL_FAIL] = fail ? 1 : 0;
+        flags[FL_NF * p + FL_LASTSHAKE] = retried ? it2 : it1;
+    }
+}
+
+// energy test (model.cpp:444-470)
+__global__ void k_fire_control(int P, const int* __restrict__ active, const double* __restrict__ energy,
+                               const double* __restrict__ epre, double alphaStart, double fDec, double* dt,
+                               double* alpha, int* nPos, int* flags) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P || !active[p]) return;
+    int* fl = flags + FL_NF * p;
+    double ePost = energy[p], ePre = epre[p];
+    if (fl[FL_FAIL]) {
+        dt[p] = dt[p] * fDec;
+        alpha[p] = alphaStart;
+        nPos[p] = 0;
+        fl[FL_ACCEPT] = 0;
+    } else if (ePost > ePre + fmax(fabs(ePre) * 1e-10, 1e-14)) {
+        fl[FL_ROLLBACK] = 1;
+        fl[FL_ACCEPT] = 0;
+        dt[p] = dt[p] * fDec;
+        alpha[p] = alphaStart;
+        nPos[p] = 0;
+    } else {
+        fl[FL_ACCEPT] = 1;
+    }
+}
+
+// second half-kick and per-polygon partial sums of F.v, v.v, F.F (vertex order)
+__global__ void k_fire_kick(int N, const int* __restrict__ polyStart, const int* __restrict__ polyPack,
+                            const int* __restrict__ active, const int* __restrict__ flags,
+                            const double* __restrict__ dt, double2* __restrict__ v,
+                            const double2* __restrict__ F, double* __restrict__ red) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N) return;
+    int p = polyPack[s];
+    double a = 0.0, b = 0.0, c = 0.0;
+    if (active[p] && flags[FL_NF * p + FL_ACCEPT]) {
+        double h = dt[p];
+        for (int m = polyStart[s]; m < polyStart[s + 1]; ++m) {
+            double2 vi = v[m], fi = F[m];
+            double vx = vi.x + (0.5 * h) * fi.x;
+            double vy = vi.y + (0.5 * h) * fi.y;
+            v[m] = make_double2(vx, vy);
+            a += fi.x * vx;
+            a += fi.y * vy;
+            b += vx * vx;
+            b += vy * vy;
+            c += fi.x * fi.x;
+            c += fi.y * fi.y;
+        }
+    }
+    red[s] = a;
+    red[N + s] = b;
+    red[2LL * N + s] = c;
+}
+
+// bend + P>0 logic (model.cpp:471-485)
+__global__ void k_fire_control2(int P, const int* __restrict__ active, const double* __restrict__ sums,
+                                int nMin, double dtMax, double alphaStart, double fAlpha, double fInc,
+                                double fDec, double* dt, double* alpha, int* nPos, int* flags, double* fs) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P || !active[p]) return;
+    int* fl = flags + FL_NF * p;
+    if (!fl[FL_ACCEPT]) return;
+    double Pw = sums[p], vv = sums[P + p], ff = sums[2 * P + p];
+    double vnorm = sqrt(vv), fnorm = sqrt(ff);
+    double a = alpha[p];
+    fs[FS_NF * p + FS_SCALE] = (fnorm > 0.0) ? (a * vnorm) / fnorm : 0.0;
+    fs[FS_NF * p + FS_ONEMA] = 1.0 - a;
+    double h = dt[p];
+    if (Pw > 0.0) {
+        fl[FL_ZEROV] = 0;
+        int np = nPos[p] + 1;
+        nPos[p] = np;
+        if (np >= nMin) {
+            double newDt = fmin(h * fInc, dtMax);
+            if (newDt > h) alpha[p] = alpha[p] * fAlpha;
+            dt[p] = newDt;
+        }
+    } else {
+        fl[FL_ZEROV] = 1;
+        dt[p] = h * fDec;
+        alpha[p] = alphaStart;
+        nPos[p] = 0;
+    }
+}
+
+__global__ void k_fire_bend(int V, const int* __restrict__ shapeId, const int* __restrict__ polyPack,
+                            const int* __restrict__ active, const int* __restrict__ flags,
+                            const double* __restrict__ fs, double2* __restrict__ x, double2* __restrict__ v,
+                            const double2* __restrict__ F, const double2* __restrict__ xbak) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    int p = polyPack[shapeId[i]];
+    if (!active[p]) return;
+    const int* fl = flags + FL_NF * p;
+    if (fl[FL_ACCEPT]) {
+        if (fl[FL_ZEROV]) {
+            v[i] = make_double2(0.0, 0.0);
+        } else {
+            double sc = fs[FS_NF * p + FS_SCALE], om = fs[FS_NF * p + FS_ONEMA];
+            double2 vi = v[i], fi = F[i];
+            v[i] = make_double2(om * vi.x + sc * fi.x, om * vi.y + sc * fi.y);
+        }
+    } else if (fl[FL_ROLLBACK]) {
+        x[i] = xbak[i];
+        v[i] = make_double2(0.0, 0.0);
+    }
+}
+
+// convergence on max |force component| (model.cpp:500), step count, status block.
+// staleRollback: a rolled-back packing's forces are not yet recomputed, so it cannot
+// converge in this pass.
+__global__ void k_fire_finish(int P, int* __restrict__ active, const int* __restrict__ flags,
+                              const double* __restrict__ maxF, double fTol, int* __restrict__ steps,
+                              const double* __restrict__ dt, const double* __restrict__ alpha,
+                              const int* __restrict__ nPos, const double* __restrict__ energy,
+                              double* __restrict__ fs, pcp_fire_status* __restrict__ status,
+                              int* __restrict__ summary, int countStep, int staleRollback) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const int* fl = flags + FL_NF * p;
+    if (active[p]) {
+        double mf = maxF[p];
+        fs[FS_NF * p + FS_MAXF] = mf;
+        if (countStep) steps[p] = steps[p] + 1;
+        bool conv = (mf <= fTol) && !(staleRollback && fl[FL_ROLLBACK]);
+        if (conv) active[p] = 0;
+        if (fl[FL_ROLLBACK]) atomicOr(&summary[0], 1);
+        if (!conv) atomicOr(&summary[1], 1);
+    }
+    pcp_fire_status s;
+    s.dt = dt[p];
+    s.alpha = alpha[p];
+    s.energy = energy[p];
+    s.maxForce = fs[FS_NF * p + FS_MAXF];
+    s.nPos = nPos[p];
+    s.steps = steps[p];
+    s.converged = active[p] ? 0 : 1;
+    s.lastShakeIters = fl[FL_LASTSHAKE];
+    s.shakeFailed = fl[FL_FAIL];
+    s.rolledBack = fl[FL_ROLLBACK];
+    status[p] = s;
+}
+
+// undo an attempt whose buffers overflowed: x, v, FIRE scalars, active flags, steps
+__global__ void k_fire_undo(int V, int P, const int* __restrict__ shapeId, const int* __restrict__ polyPack,
+                            int* __restrict__ active, const int* __restrict__ flags, double2* __restrict__ x,
+                            double2* __restrict__ v, const double2* __restrict__ xbak,
+                            const double2* __restrict__ vbak, double* dt, double* alpha, int* nPos,
+                            int* steps, const double* __restrict__ fs) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < V) {
+        int p = polyPack[shapeId[i]];
+        if (flags[FL_NF * p + FL_ACTIVEBAK]) { x[i] = xbak[i]; v[i] = vbak[i]; }
+    }
+    if (i < P && flags[FL_NF * i + FL_ACTIVEBAK]) {
+        active[i] = 1;
+        dt[i] = fs[FS_NF * i + FS_DTB];
+        alpha[i] = fs[FS_NF * i + FS_ALB];
+        nPos[i] = (int)fs[FS_NF * i + FS_NPB];
+        steps[i] = (int)fs[FS_NF * i + FS_STEPB];
+    }
+}
+
+__global__ void k_zero2(long long n, double2* v) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = make_double2(0.0, 0.0);
+}
+
+// the evaluation part of one step (positions already moved by k_fire_begin)
+static int fire_eval(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm) {
+    double2* x = (double2*)st->pos;
+    double2* v = (double2*)st->vel;
+    int mt = st->model & 0xFF;
+    bool area = (st->model & PCP_AREA_ENERGY) != 0;
+    bool overlap = (mt == PCP_NORMAL || mt == PCP_HYBRID);
+    bool rigid = (mt == PCP_NORMAL);
+    if (overlap) {
+        if (!area) {   // reference order (model.cpp:416-440): detect, SHAKE, records
+            PCP_TRY(run_cells(c, x));
+            PCP_TRY(run_detect(c, x));
+        }
+        if (rigid && prm->shakeIter > 0) {
+            PCP_TRY(run_shake(c, x, st->targetEdgeV, st->targetAreas, prm->shakeIter, 1e-15,
+                              10 * prm->shakeIter, c->dActive, c->dShakeIt));
+            PCP_LAUNCH(c, k_fire_post_shake, dim3(nblk(c->V)), dim3(BLK), c->V, c->P,
+                               prm->shakeIter, c->dShapeId, c->dPolyPack, c->dPackVert, c->dActive, c->dDt,
+                               c->dShakeIt, x, v, (const double2*)c->dXbak, (const double2*)c->dXpre,
+                               c->dFlags);
+        }
+        if (area) {   // area energy: crossings of the final positions
+            PCP_TRY(run_cells(c, x));
+            PCP_TRY(run_detect(c, x));
+        }
+        PCP_TRY(run_records(c, area ? x : nullptr));
+    }
+    PCP_TRY(run_geometry(c, x, st->edgeLengths, st->areas, c->dCom, c->dCom + c->N, nullptr, nullptr));
+    PCP_TRY(run_forces(c, st->model, x, st->edgeLengths, st->areas, st->targetEdgeV, st->targetAreas,
+                       st->stiffness, st->compressibility, (double2*)st->force, st->energy));
+    if (rigid) PCP_TRY(run_project(c, x, (double2*)st->force));
+    return PCP_OK;
+}
+
+static int fire_finish(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm, int countStep,
+                       int staleRollback) {
+    PCP_LAUNCH(c, k_max_abs_poly, dim3(nblk(c->N)), dim3(BLK), c->N, c->dPolyStart,
+                       (const double2*)st->force, c->dRed);
+    PCP_TRY(launch_det_reduce(c, c->dRed, 1, c->dRedOut + 3LL * c->P, 1));
+    PCP_LAUNCH(c, k_fire_finish, dim3(nblk(c->P)), dim3(BLK), c->P, c->dActive, c->dFlags,
+                       c->dRedOut + 3LL * c->P, prm->fTol, c->dSteps, c->dDt, c->dAlpha, c->dNPos, st->energy,
+                       c->dFireScal, c->dStatus, c->dSummary, countStep, staleRollback);
+    return PCP_OK;
+}
+
+// one step; *anyActive receives whether some packing is still unconverged
+static int fire_step_impl(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm, int* anyActive) {
+    int mt0 = st->model & 0xFF;
+    bool overlap = (mt0 == PCP_NORMAL || mt0 == PCP_HYBRID);
+    for (int attempt = 0; attempt < 6; ++attempt) {
+        hipLaunchKernelGGL(k_fire_prologue, dim3(nblk(std::max(c->P, 4))), dim3(BLK), 0, c->stream, c->P,
+                           c->dActive, c->dDt, c->dAlpha, c->dNPos, c->dSteps, st->energy, c->dEpre, c->dFlags,
+                           c->dFireScal, c->dShakeIt, c->dSummary);
+        PCP_LAUNCHED();
+        PCP_LAUNCH(c, k_fire_begin, dim3(nblk(c->V)), dim3(BLK), c->V, c->dShapeId,
+                           c->dPolyPack, c->dActive, c->dDt, (double2*)st->pos, (double2*)st->vel,
+                           (const double2*)st->force, (double2*)c->dXbak, (double2*)c->dVbak,
+                           (double2*)c->dXpre);
+        PCP_TRY(fire_eval(c, st, prm));
+        PCP_LAUNCH(c, k_fire_control, dim3(nblk(c->P)), dim3(BLK), c->P, c->dActive,
+                           st->energy, c->dEpre, prm->alphaStart, prm->fDec, c->dDt, c->dAlpha, c->dNPos,
+                           c->dFlags);
+        PCP_LAUNCH(c, k_fire_kick, dim3(nblk(c->N)), dim3(BLK), c->N, c->dPolyStart,
+                           c->dPolyPack, c->dActive, c->dFlags, c->dDt, (double2*)st->vel,
+                           (const double2*)st->force, c->dRed);
+        PCP_TRY(launch_det_reduce(c, c->dRed, 3, c->dRedOut, 0));
+        PCP_LAUNCH(c, k_fire_control2, dim3(nblk(c->P)), dim3(BLK), c->P, c->dActive,
+                           c->dRedOut, prm->nMin, prm->dtMax, prm->alphaStart, prm->fAlpha, prm->fInc,
+                           prm->fDec, c->dDt, c->dAlpha, c->dNPos, c->dFlags, c->dFireScal);
+        PCP_LAUNCH(c, k_fire_bend, dim3(nblk(c->V)), dim3(BLK), c->V, c->dShapeId,
+                           c->dPolyPack, c->dActive, c->dFlags, c->dFireScal, (double2*)st->pos,
+                           (double2*)st->vel, (const double2*)st->force, (const double2*)c->dXbak);
+        PCP_TRY(fire_finish(c, st, prm, 1, 1));
+        int summary[4] = {0, 0, 0, 0};
+        int scal[SC_NSCAL];
+        int I = 0;
+        PCP_HIP(hipMemcpyAsync(summary, c->dSummary, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
+        PCP_HIP(hipMemcpyAsync(scal, c->dScal, sizeof(int) * SC_NSCAL, hipMemcpyDeviceToHost, c->stream));
+        PCP_HIP(hipMemcpyAsync(&I, c->dBStart + c->N, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        PCP_HIP(hipStreamSynchronize(c->stream));
+        if (overlap) c->lastI = I;
+        if (overlap && (scal[SC_MAXCOUNT] > c->K || I > c->Icap)) {
+            // the step ran with too few crossing slots or records: grow, undo, retry
+            hipLaunchKernelGGL(k_fire_undo, dim3(nblk(std::max(c->V, c->P))), dim3(BLK), 0, c->stream, c->V,
+                               c->P, c->dShapeId, c->dPolyPack, c->dActive, c->dFlags, (double2*)st->pos,
+                               (double2*)st->vel, (const double2*)c->dXbak, (const double2*)c->dVbak, c->dDt,
+                               c->dAlpha, c->dNPos, c->dSteps, c->dFireScal);
+            PCP_LAUNCHED();
+            if (scal[SC_MAXCOUNT] > c->K) PCP_TRY(grow_slots(c, std::max(scal[SC_MAXCOUNT], 2 * c->K)));
+            if (I > c->Icap) PCP_TRY(ensure_records(c, (long long)I + I / 4));
+            PCP_TRY(pcp_force_eval(c, st));
+            continue;
+        }
+        if (summary[0]) {
+            // energy rose for some packing: x was restored there; recompute its forces
+            // (model.cpp:454-464) -- deterministic, so unchanged packings reproduce theirs
+            PCP_TRY(pcp_force_eval(c, st));
+            PCP_TRY(fire_finish(c, st, prm, 0, 0));
+            PCP_HIP(hipMemcpyAsync(summary, c->dSummary, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
+            PCP_HIP(hipStreamSynchronize(c->stream));
+        }
+        if (anyActive) *anyActive = summary[1];
+        return PCP_OK;
+    }
+    set_error("pcp_fire_step: crossing buffers kept overflowing");
+    return PCP_ERR_CAPACITY;
+}
+
+}  // namespace pcp
+
+extern "C" {
+
+int pcp_fire_init(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm, double dt0) {
+    if (!c->haveBoxes && ((st->model & 0xFF) == PCP_NORMAL || (st->model & 0xFF) == PCP_HYBRID)) {
+        set_error("pcp_fire_init: set the box sizes first");
+        return PCP_ERR_ARG;
+    }
+    hipLaunchKernelGGL(k_zero2, dim3(nblk(c->V)), dim3(BLK), 0, c->stream, (long long)c->V, (double2*)st->vel);
+    PCP_LAUNCHED();
+    hipLaunchKernelGGL(k_fire_init, dim3(nblk(c->P)), dim3(BLK), 0, c->stream, c->P, dt0, prm->alphaStart,
+                       c->dDt, c->dAlpha, c->dNPos, c->dSteps, c->dActive, c->dFlags, c->dFireScal);
+    PCP_LAUNCHED();
+    PCP_TRY(pcp_force_eval(c, st));
+    PCP_HIP(hipStreamSynchronize(c->stream));
+    return PCP_OK;
+}
+
+int pcp_fire_step(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm, pcp_fire_status* status) {
+    PCP_TRY(fire_step_impl(c, st, prm, nullptr));
+    if (status) {
+        PCP_HIP(hipMemcpyAsync(status, c->dStatus, sizeof(pcp_fire_status) * c->P, hipMemcpyDeviceToHost,
+                               c->stream));
+        PCP_HIP(hipStreamSynchronize(c->stream));
+    }
+    return PCP_OK;
+}
+
+int pcp_fire_minimize(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm, int maxSteps,
+                      pcp_fire_status* status) {
+    for (int step = 0; step < maxSteps; ++step) {
+        int any = 1;
+        PCP_TRY(fire_step_impl(c, st, prm, &any));
+        if (!any) break;
+    }
+    if (status) {
+        PCP_HIP(hipMemcpyAsync(status, c->dStatus, sizeof(pcp_fire_status) * c->P, hipMemcpyDeviceToHost,
+                               c->stream));
+        PCP_HIP(hipStreamSynchronize(c->stream));
+    }
+    return PCP_OK;
+}
+
+int pcp_fire_set(pcp_ctx* c, const double* dt, const double* alpha, const int* nPos) {
+    PCP_HIP(hipStreamSynchronize(c->stream));
+    if (dt) PCP_HIP(hipMemcpy(c->dDt, dt, sizeof(double) * c->P, hipMemcpyHostToDevice));
+    if (alpha) PCP_HIP(hipMemcpy(c->dAlpha, alpha, sizeof(double) * c->P, hipMemcpyHostToDevice));
+    if (nPos) PCP_HIP(hipMemcpy(c->dNPos, nPos, sizeof(int) * c->P, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_fill_int, dim3(nblk(c->P)), dim3(BLK), 0, c->stream, c->dActive, (long long)c->P, 1);
+    PCP_LAUNCHED();
+    hipLaunchKernelGGL(k_fill_int, dim3(nblk(c->P)), dim3(BLK), 0, c->stream, c->dSteps, (long long)c->P, 0);
+    PCP_LAUNCHED();
+    PCP_HIP(hipStreamSynchronize(c->stream));
+    return PCP_OK;
+}
+
+int pcp_fire_get(pcp_ctx* c, pcp_fire_status* status) {
+    PCP_HIP(hipStreamSynchronize(c->stream));
+    PCP_HIP(hipMemcpy(status, c->dStatus, sizeof(pcp_fire_status) * c->P, hipMemcpyDeviceToHost));
+    return PCP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,151 @@
+// Internal context of libpcp_hip.so (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include "pcp_device.hpp"
+#include "../../include/pcp.h"
+
+namespace pcp {
+
+void set_error(const std::string& msg);
+int fail_hip(hipError_t e, const char* what, const char* file, int line);
+
+#define PCP_HIP(call)                                                         \
+    do {                                                                      \
+        hipError_t _e = (call);                                               \
+        if (_e != hipSuccess) return ::pcp::fail_hip(_e, #call, __FILE__, __LINE__); \
+    } while (0)
+#define PCP_LAUNCHED()                                                        \
+    do {                                                                      \
+        hipError_t _e = hipGetLastError();                                    \
+        if (_e != hipSuccess) return ::pcp::fail_hip(_e, "kernel launch", __FILE__, __LINE__); \
+    } while (0)
+#define PCP_TRY(call)                                                         \
+    do {                                                                      \
+        int _s = (call);                                                      \
+        if (_s != PCP_OK) return _s;                                          \
+    } while (0)
+
+constexpr int BLK = 256;
+inline int nblk(long long n, int b = BLK) { return (int)((n + b - 1) / b); }
+
+// scalar slots in ctx->dScal
+enum { SC_MAXCOUNT = 0, SC_OVERFLOW = 1, SC_NSCAL = 8 };
+
+template <class T> int dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    PCP_HIP(hipMalloc((void**)p, count * sizeof(T)));
+    return PCP_OK;
+}
+template <class T> void dfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+}  // namespace pcp
+
+struct pcp_ctx {
+    hipStream_t stream = nullptr;
+    int P = 0, N = 0, V = 0;
+    int K = 32;          // crossing slots per edge (grown on demand)
+    int M = 100;         // reference maxNeighbors for the neighbour-list getters
+    long long Icap = 0;  // record capacity
+    long long Ctot = 0;  // total cells over all packings
+    long long lastI = 0; // last observed record total (grid sizing only)
+    bool haveBoxes = false;
+
+    std::vector<int> hPackPoly, hPackVert, hPolyStart, hBox;
+    std::vector<long long> hCellOff;
+
+    // topology
+    int *dPackPoly = nullptr, *dPackVert = nullptr, *dPolyStart = nullptr, *dShapeId = nullptr;
+    int *dNext = nullptr, *dPrev = nullptr, *dPolyPack = nullptr, *dBox = nullptr;
+    long long* dCellOff = nullptr;
+
+    // cell list
+    int *dCellOf = nullptr, *dCellCount = nullptr, *dCellStart = nullptr;
+    int *dCellTmp = nullptr, *dCellSorted = nullptr, *dCellInv = nullptr;
+    double2 *dSP = nullptr, *dSR = nullptr;
+    int4* dSI = nullptr;
+
+    // detection
+    uint64_t* dSlotKey = nullptr;
+    float2* dSlotTU = nullptr;
+    unsigned char* dSlotIn = nullptr;
+    int* dNum = nullptr;
+    int* dCounts = nullptr;     // 2N: si buckets, then sj buckets
+    int* dBStart = nullptr;     // 2N+1 exclusive scan of dCounts
+    unsigned long long* dPackMaxKey = nullptr;
+    int* dScal = nullptr;
+
+    // records (capacity Icap)
+    uint64_t* dKeys2 = nullptr;    // 2*Icap: si-bucketed keys, then sj-bucketed keys
+    float2* dBTU = nullptr;
+    int* dBPoly = nullptr;
+    uint64_t *dRKey = nullptr, *dROuter = nullptr;
+    float2 *dRTU = nullptr, *dRUT = nullptr;
+    int* dRPoly = nullptr;
+    int* dSjList = nullptr;
+    double2* dContrib = nullptr;
+    int* dTgt = nullptr;
+    double* dEext = nullptr;
+    double2* dSp = nullptr;
+
+    // per-vertex / per-polygon scratch
+    double *dEint = nullptr, *dEedge = nullptr, *dEpoly = nullptr;
+    double* dWork = nullptr;       // 8*V doubles (projection / SHAKE solver)
+    double* dRed = nullptr;        // 4*N per-polygon partials
+    double* dRedOut = nullptr;     // 4*P per-packing reductions
+    int* dScanPart = nullptr;
+    int scanPartCap = 0;
+
+    // FIRE (per packing unless noted)
+    double *dXbak = nullptr, *dXpre = nullptr, *dVbak = nullptr;   // 2V
+    double *dDt = nullptr, *dAlpha = nullptr, *dEpre = nullptr;
+    int *dNPos = nullptr, *dSteps = nullptr, *dActive = nullptr;
+    int* dFlags = nullptr;         // per packing: see FL_* in pcp_fire.hip
+    double* dFireScal = nullptr;   // per packing: bend scale, 1-alpha, maxF, backups
+    int* dShakeIt = nullptr;       // per packing: [0] call-1 max, [1] call-2 max
+    double* dCom = nullptr;        // 2N scratch (COM of internal geometry passes)
+    pcp_fire_status* dStatus = nullptr;
+    int* dSummary = nullptr;       // [0] any rollback, [1] any unconverged
+    std::string profName;          // kernel bracketed with events (measurement)
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> profEvents;
+    size_t profUsed = 0;
+    int* dPartner = nullptr;       // Icap: index of each record's exit record (area energy)
+    unsigned char* dClosed = nullptr;  // Icap: record lies on a closed partner cycle
+};
+
+namespace pcp {
+int ensure_records(pcp_ctx* c, long long need);
+int scan_exclusive(pcp_ctx* c, const int* in, int n, int* out);
+int launch_det_reduce(pcp_ctx* c, const double* partials, int nArrays, double* outSums, int nMax);
+int run_cells(pcp_ctx* c, const double2* pos);
+int run_detect(pcp_ctx* c, const double2* pos);
+int run_records(pcp_ctx* c, const double2* pairPos);
+int run_forces(pcp_ctx* c, int model, const double2* pos, const double* edgeLengths,
+               const double* areas, const double* targetEdgeV, const double* targetAreas,
+               double stiffness, double compressibility, double2* force, double* energy);
+int run_project(pcp_ctx* c, const double2* pos, double2* force);
+int run_geometry(pcp_ctx* c, const double2* pos, double* edgeLengths, double* areas, double* comX,
+                 double* comY, double* constraints, double* maxEdge);
+int run_shake(pcp_ctx* c, double2* pos, const double* targetEdgeV, const double* targetAreas,
+              int maxIter, double tol, int retryIter, const int* active, int* itOut);
+int record_grid(pcp_ctx* c);
+int grow_slots(pcp_ctx* c, int K);
+int prof_begin(pcp_ctx* c, const char* name, int* slot);
+int prof_end(pcp_ctx* c, int slot);
+}  // namespace pcp
+
+// launch `kern` on the context stream, bracketed with events when it is the profiled kernel
+#define PCP_LAUNCH(c, kern, grid, block, ...)                                          \
+    do {                                                                               \
+        int _slot = -1;                                                                \
+        PCP_TRY(::pcp::prof_begin((c), #kern, &_slot));                                \
+        hipLaunchKernelGGL(kern, grid, block, 0, (c)->stream, __VA_ARGS__);            \
+        PCP_LAUNCHED();                                                                \
+        PCP_TRY(::pcp::prof_end((c), _slot));                                          \
+    } while (0)

@@ -1,0 +1,2060 @@
+// HIP kernels + C ABI for the overlap-force path (gfx950).  See include/pcp.h.
+//
+// Pipeline of one force evaluation (one HIP stream, no host round trips):
+//   geometry      thread / polygon     edge lengths, shoelace areas, COM, constraints
+//   cells         hash (thread / vertex, LDS-free atomics on cell counts), exclusive
+//                 scan, scatter, in-cell rank (ascending vertex id) + SoA edge staging
+//   detect        thread / cell-sorted edge: 3x3 cell stencil, segment test, records
+//                 written to per-edge slots, per-polygon bucket counts
+//   records       scan of bucket counts, scatter into si- and sj-buckets, one pass that
+//                 ranks each record in its si bucket (final order (si,i,j,sj)) and pairs
+//                 it with its exit record from the sj bucket
+//   forces        exterior contributions per record -> deterministic per-vertex gather
+//                 (exterior + interior chain + penalty terms), per-polygon energies,
+//                 fixed 256-wide tree per packing
+//   projection    thread / polygon bordered cyclic-tridiagonal solve
+// Nothing is accumulated with floating-point atomics, so every result is
+// bit-reproducible run to run.
+#include "pcp_internal.hpp"
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+using namespace pcp;
+
+namespace pcp {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+int fail_hip(hipError_t e, const char* what, const char* file, int line) {
+    char buf[512];
+    snprintf(buf, sizeof buf, "HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what,
+             file, line);
+    g_err = buf;
+    return PCP_ERR_HIP;
+}
+static int fail_arg(const std::string& m) {
+    g_err = m;
+    return PCP_ERR_ARG;
+}
+
+// ------------------------------------------------------------------ wave helpers
+__device__ __forceinline__ int wave_max_int(int v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ int block_sum_int(int v, int* lds) {
+    // BLK threads, wave64
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) lds[w] = v;
+    __syncthreads();
+    int s = 0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) s += lds[q];
+    return s;
+}
+__device__ __forceinline__ int block_excl_scan_int(int v, int* lds, int* total) {
+    int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o);
+        if (l >= o) x += y;
+    }
+    __syncthreads();
+    if (l == 63) lds[w] = x;
+    __syncthreads();
+    int off = 0, tot = 0;
+    int nw = blockDim.x >> 6;
+    for (int q = 0; q < nw; ++q) {
+        if (q < w) off += lds[q];
+        tot += lds[q];
+    }
+    *total = tot;
+    return off + x - v;
+}
+
+// fixed pairwise tree over 256 values held one per thread (thread t has v[t]);
+// result valid in thread 0.  Same pairing as orc_det_sum (s = 128, 64, ..., 1).
+__device__ __forceinline__ double tree256(double v, double* lds) {
+    int t = threadIdx.x;
+    __syncthreads();
+    lds[t] = v;
+    __syncthreads();
+    if (t < 128) lds[t] = lds[t] + lds[t + 128];
+    __syncthreads();
+    double a = 0.0;
+    if (t < 64) {
+        a = lds[t] + lds[t + 64];
+#pragma unroll
+        for (int s = 32; s >= 1; s >>= 1) {
+            double b = __shfl_down(a, s);
+            a = a + b;
+        }
+    }
+    return a;
+}
+
+// ------------------------------------------------------------------ small kernels
+__global__ void k_fill_int(int* p, long long n, int v) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_double(double* p, long long n, double v) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// topology: shapeId/next/prev per vertex (updateShapeIdKernel kernels.cuh:652-659,
+// computeNextPrevKernel :127-135), packing of each polygon
+__global__ void k_topology(int V, int N, int P, const int* __restrict__ polyStart,
+                           const int* __restrict__ packPoly, int* shapeId, int* next, int* prev,
+                           int* polyPack) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V) {
+        int lo = 0, hi = N - 1;
+        while (lo < hi) {   // last polygon with polyStart <= v
+            int mid = (lo + hi + 1) >> 1;
+            if (polyStart[mid] <= v) lo = mid; else hi = mid - 1;
+        }
+        int s = lo, st = polyStart[s], en = polyStart[s + 1];
+        shapeId[v] = s;
+        next[v] = (v + 1 == en) ? st : v + 1;
+        prev[v] = (v == st) ? en - 1 : v - 1;
+    }
+    if (v < N) {
+        int lo = 0, hi = P - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (packPoly[mid] <= v) lo = mid; else hi = mid - 1;
+        }
+        polyPack[v] = lo;
+    }
+}
+
+// ------------------------------------------------------------------ scan
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = BLK * SCAN_ITEMS;
+
+__global__ void k_scan_partials(const int* __restrict__ in, int n, int* __restrict__ part) {
+    __shared__ int lds[16];
+    long long base = (long long)blockIdx.x * SCAN_TILE;
+    int s = 0;
+    for (int q = threadIdx.x; q < SCAN_TILE; q += BLK) {
+        long long i = base + q;
+        if (i < n) s += in[i];
+    }
+    s = block_sum_int(s, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+__global__ void k_scan_final(const int* __restrict__ in, int n, const int* __restrict__ part,
+                             int* __restrict__ out) {
+    __shared__ int lds[16];
+    int b = blockIdx.x;
+    int off = 0;
+    for (int q = threadIdx.x; q < b; q += BLK) off += part[q];
+    off = block_sum_int(off, lds);
+    long long base = (long long)b * SCAN_TILE + (long long)threadIdx.x * SCAN_ITEMS;
+    int v[SCAN_ITEMS];
+    int loc = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        v[k] = (base + k < n) ? in[base + k] : 0;
+        loc += v[k];
+    }
+    int tot;
+    int run = off + block_excl_scan_int(loc, lds, &tot);
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (b == (int)gridDim.x - 1 && threadIdx.x == BLK - 1) out[n] = run;
+}
+
+int scan_exclusive(pcp_ctx* c, const int* in, int n, int* out) {
+    int nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nb < 1) nb = 1;
+    if (nb > c->scanPartCap) {
+        dfree(c->dScanPart);
+        PCP_TRY(dalloc(&c->dScanPart, nb));
+        c->scanPartCap = nb;
+    }
+    PCP_LAUNCH(c, k_scan_partials, dim3(nb), dim3(BLK), in, n, c->dScanPart);
+    PCP_LAUNCH(c, k_scan_final, dim3(nb), dim3(BLK), in, n, c->dScanPart, out);
+    return PCP_OK;
+}
+
+// ------------------------------------------------------------------ geometry
+// updatePolygonGeometryKernel kernels.cuh:295-333 + segmented sums model.cu:123-140
+// (sequential per polygon) + normalizeKernel :635-650 + normalizeConstraintsKernel
+// :335-345; max edge length per packing (model.cu:144-150).
+__global__ void k_geometry(int N, const int* __restrict__ polyStart, const int* __restrict__ polyPack,
+                           const double2* __restrict__ pos, double* __restrict__ edgeLengths,
+                           double* __restrict__ areas, double* __restrict__ comX,
+                           double* __restrict__ comY, double* __restrict__ constraints,
+                           unsigned long long* __restrict__ maxEdgeBits) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N) return;
+    int st = polyStart[s], en = polyStart[s + 1], n = en - st;
+    d2 p0 = ldpos(pos, st);
+    double area = 0.0, cx = 0.0, cy = 0.0, ns0 = 0.0, ns1 = 0.0, ns2 = 0.0, mx = 0.0;
+    d2 pprev = ldpos(pos, en - 1), pk = p0;
+    for (int k = st; k < en; ++k) {
+        int nx = (k + 1 == en) ? st : k + 1;
+        d2 pn = ldpos(pos, nx);
+        double dxPrev = wrap(pk.x - pprev.x);
+        double dyPrev = wrap(pk.y - pprev.y);
+        double dxNext = wrap(pk.x - pn.x);
+        double dyNext = wrap(pk.y - pn.y);
+        double lenPrev = sqrt(dxPrev * dxPrev + dyPrev * dyPrev);
+        double lenNext = sqrt(dxNext * dxNext + dyNext * dyNext);
+        // alpha = 0
+        double x1a = wrap(pk.x - p0.x);
+        double x4a = wrap(pprev.y - pn.y);
+        cx += x1a;
+        double a0 = (double)(2 * 0 - 1) * x4a;
+        double e20 = dxNext + dxPrev;
+        double e0 = (dxNext / lenNext) + (dxPrev / lenPrev);
+        ns0 += a0 * a0;
+        ns1 += e20 * e20;
+        ns2 += e0 * e0;
+        double dx = wrap(pn.x - pk.x);
+        double dy = wrap(pn.y - pk.y);
+        double el = sqrt(dx * dx + dy * dy);
+        edgeLengths[k] = el;
+        if (el > mx) mx = el;
+        double dy2 = wrap(pn.y - pk.y);
+        double x2 = wrap(pn.x - p0.x);
+        area += dy2 * (x2 + x1a) / 2.0;
+        // alpha = 1
+        double x1b = wrap(pk.y - p0.y);
+        double x4b = wrap(pprev.x - pn.x);
+        cy += x1b;
+        double a1 = (double)(2 * 1 - 1) * x4b;
+        double e21 = dyNext + dyPrev;
+        double e1 = (dyNext / lenNext) + (dyPrev / lenPrev);
+        ns0 += a1 * a1;
+        ns1 += e21 * e21;
+        ns2 += e1 * e1;
+        if (constraints) {
+            double* cc = constraints + 6 * (long long)k;
+            cc[0] = a0; cc[1] = a1; cc[2] = e20; cc[3] = e21; cc[4] = e0; cc[5] = e1;
+        }
+        pprev = pk;
+        pk = pn;
+    }
+    areas[s] = area;
+    double c = cx;
+    c /= ((double)n);
+    c += p0.x;
+    c -= floor(c);
+    comX[s] = c;
+    c = cy;
+    c /= ((double)n);
+    c += p0.y;
+    c -= floor(c);
+    comY[s] = c;
+    if (constraints) {
+        double r0 = sqrt(ns0), r1 = sqrt(ns1), r2 = sqrt(ns2);
+        for (int k = st; k < en; ++k) {
+            double* cc = constraints + 6 * (long long)k;
+            if (ns0 > 0.0) { cc[0] /= r0; cc[1] /= r0; }
+            if (ns1 > 0.0) { cc[2] /= r1; cc[3] /= r1; }
+            if (ns2 > 0.0) { cc[4] /= r2; cc[5] /= r2; }
+        }
+    }
+    if (maxEdgeBits) atomicMax(&maxEdgeBits[polyPack[s]], (unsigned long long)__double_as_longlong(mx));
+}
+
+int run_geometry(pcp_ctx* c, const double2* pos, double* edgeLengths, double* areas, double* comX,
+                 double* comY, double* constraints, double* maxEdge) {
+    if (c->N == 0) return PCP_OK;
+    if (maxEdge) {
+        hipLaunchKernelGGL(k_fill_double, dim3(nblk(c->P)), dim3(BLK), 0, c->stream, maxEdge,
+                           (long long)c->P, 0.0);
+        PCP_LAUNCHED();
+    }
+    PCP_LAUNCH(c, k_geometry, dim3(nblk(c->N)), dim3(BLK), c->N, c->dPolyStart,
+                       c->dPolyPack, pos, edgeLengths, areas, comX, comY, constraints,
+                       (unsigned long long*)maxEdge);
+    return PCP_OK;
+}
+
+// ------------------------------------------------------------------ cell list
+// updateNeighborCellsKernel kernels.cuh:661-707 (cell of the edge midpoint)
+__device__ __forceinline__ int edge_cell(d2 p, d2 q, int B) {
+    double x = q.x - p.x + 0.5;
+    double y = q.y - p.y + 0.5;
+    x = shift_into_unit(x);
+    y = shift_into_unit(y);
+    x -= 0.5;
+    y -= 0.5;
+    x /= 2.0;
+    y /= 2.0;
+    x += p.x;
+    y += p.y;
+    x = shift_into_unit(x);
+    y = shift_into_unit(y);
+    double fx = floor(x * B), fy = floor(y * B);
+    int ix = (fx == fx && fx >= 0.0) ? (fx < 2147483647.0 ? (int)fx : B - 1) : 0;
+    int iy = (fy == fy && fy >= 0.0) ? (fy < 2147483647.0 ? (int)fy : B - 1) : 0;
+    if (ix < 0) ix = 0; else if (ix >= B) ix = B - 1;
+    if (iy < 0) iy = 0; else if (iy >= B) iy = B - 1;
+    return iy * B + ix;
+}
+
+__global__ void k_cell_hash(int V, const double2* __restrict__ pos, const int* __restrict__ shapeId,
+                            const int* __restrict__ next, const int* __restrict__ polyPack,
+                            const int* __restrict__ box, const long long* __restrict__ cellOff,
+                            int* __restrict__ cellOf, int* __restrict__ cellCount) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    int p = polyPack[shapeId[v]];
+    int B = box[p];
+    int cl = edge_cell(ldpos(pos, v), ldpos(pos, next[v]), B);
+    cellOf[v] = cl;
+    atomicAdd(&cellCount[cellOff[p] + cl], 1);
+}
+
+__global__ void k_cell_scatter(int V, const int* __restrict__ shapeId, const int* __restrict__ polyPack,
+                               const long long* __restrict__ cellOff, const int* __restrict__ cellOf,
+                               int* __restrict__ cellCount, const int* __restrict__ cellStart,
+                               int* __restrict__ cellTmp) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    long long gc = cellOff[polyPack[shapeId[v]]] + cellOf[v];
+    int slot = atomicSub(&cellCount[gc], 1) - 1;   // leaves the counts at zero
+    cellTmp[cellStart[gc] + slot] = v;
+}
+
+// stable order inside each cell = ascending vertex id (CUB radix sort of iota values,
+// model.cu:322-349); stage the edge data of every sorted position for the stencil scan
+__global__ void k_cell_sort(int V, const double2* __restrict__ pos, const int* __restrict__ shapeId,
+                            const int* __restrict__ next, const int* __restrict__ polyStart,
+                            const int* __restrict__ polyPack, const long long* __restrict__ cellOff,
+                            const int* __restrict__ cellOf, const int* __restrict__ cellStart,
+                            const int* __restrict__ cellTmp, int* __restrict__ cellSorted,
+                            int* __restrict__ cellInv, double2* __restrict__ SP, double2* __restrict__ SR,
+                            int4* __restrict__ SI) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= V) return;
+    int v = cellTmp[t];
+    int s = shapeId[v];
+    long long gc = cellOff[polyPack[s]] + cellOf[v];
+    int lo = cellStart[gc], hi = cellStart[gc + 1];
+    int rank = 0;
+    for (int q = lo; q < hi; ++q) rank += (cellTmp[q] < v);
+    int dst = lo + rank;
+    cellSorted[dst] = v;
+    cellInv[v] = dst;
+    int nv = next[v];
+    d2 p = ldpos(pos, v), pn = ldpos(pos, nv);
+    SP[dst] = make_double2(p.x, p.y);
+    SR[dst] = make_double2(wrap(pn.x - p.x), wrap(pn.y - p.y));
+    SI[dst] = make_int4(v, nv, s, v - polyStart[s]);
+}
+
+int run_cells(pcp_ctx* c, const double2* pos) {
+    if (c->V == 0) return PCP_OK;
+    PCP_LAUNCH(c, k_cell_hash, dim3(nblk(c->V)), dim3(BLK), c->V, pos, c->dShapeId,
+                       c->dNext, c->dPolyPack, c->dBox, c->dCellOff, c->dCellOf, c->dCellCount);
+    PCP_TRY(scan_exclusive(c, c->dCellCount, (int)c->Ctot, c->dCellStart));
+    PCP_LAUNCH(c, k_cell_scatter, dim3(nblk(c->V)), dim3(BLK), c->V, c->dShapeId,
+                       c->dPolyPack, c->dCellOff, c->dCellOf, c->dCellCount, c->dCellStart, c->dCellTmp);
+    PCP_LAUNCH(c, k_cell_sort, dim3(nblk(c->V)), dim3(BLK), c->V, pos, c->dShapeId,
+                       c->dNext, c->dPolyStart, c->dPolyPack, c->dCellOff, c->dCellOf, c->dCellStart,
+                       c->dCellTmp, c->dCellSorted, c->dCellInv, c->dSP, c->dSR, c->dSI);
+    return PCP_OK;
+}
+
+// ------------------------------------------------------------------ edge intersections
+// updateNeighborsKernel kernels.cuh:709-812.  One thread per cell-sorted edge, so the
+// lanes of a wavefront share stencil cells and read the same staged edges.
+
+__global__ __launch_bounds__(BLK) void k_detect(
+    int V, int N, int K, const int* __restrict__ polyPack, const int* __restrict__ packPoly,
+    const int* __restrict__ box, const long long* __restrict__ cellOff, const int* __restrict__ cellOf,
+    const int* __restrict__ cellStart, const double2* __restrict__ SP, const double2* __restrict__ SR,
+    const int4* __restrict__ SI, uint64_t* __restrict__ slotKey, float2* __restrict__ slotTU,
+    unsigned char* __restrict__ slotIn, int* __restrict__ numNeighbors, int* __restrict__ scal) {
+    const double eps = 1e-12;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int count = 0;
+    if (t < V) {
+        int4 me = SI[t];
+        int id1 = me.x, id2 = me.y, s1g = me.z, n1 = me.w;
+        int p = polyPack[s1g];
+        int B = box[p];
+        long long cbase = cellOff[p];
+        int pbase = packPoly[p];
+        int s1l = s1g - pbase;
+        double2 P1 = SP[t], R1 = SR[t];
+        int cl = cellOf[id1];
+        int bx = cl % B, by = cl / B;
+        for (int kc = 0; kc < 9; ++kc) {
+            int dx = kc / 3 - 1, dy = kc % 3 - 1;
+            int nx = bx + dx;
+            if (nx < 0) nx += B; else if (nx >= B) nx -= B;
+            int ny = by + dy;
+            if (ny < 0) ny += B; else if (ny >= B) ny -= B;
+            int nb = ny * B + nx;
+            bool dup = false;   // skip cells already visited (kernels.cuh:746-757)
+            for (int kp = 0; kp < kc; ++kp) {
+                int ex = bx + kp / 3 - 1;
+                if (ex < 0) ex += B; else if (ex >= B) ex -= B;
+                int ey = by + kp % 3 - 1;
+                if (ey < 0) ey += B; else if (ey >= B) ey -= B;
+                dup |= (ey * B + ex == nb);
+            }
+            if (dup) continue;
+            int q0 = cellStart[cbase + nb], q1 = cellStart[cbase + nb + 1];
+            for (int q = q0; q < q1; ++q) {
+                int4 ot = SI[q];
+                int nid = ot.x, nid2 = ot.y;
+                if (nid <= id1) continue;
+                if (nid == id1 || nid == id2 || nid2 == id1 || nid2 == id2) continue;
+                double2 Q = SP[q], S = SR[q];
+                double gx = wrap(Q.x - P1.x);
+                double gy = wrap(Q.y - P1.y);
+                double denom = R1.x * S.y - R1.y * S.x;
+                if (fabs(denom) < eps) continue;
+                double tt = (gx * S.y - gy * S.x) / denom;
+                double uu = (gx * R1.y - gy * R1.x) / denom;
+                if (tt > 0.0 && tt < 1.0 && uu > 0.0 && uu < 1.0) {
+                    if (count < K) {
+                        int s2g = ot.z, l2 = ot.w, s2l = s2g - pbase;
+                        uint64_t key;
+                        float2 tu;
+                        unsigned char in;
+                        // updateCompactedIntersectionsKernel kernels.cuh:857-876
+                        if (denom > 0) {
+                            key = ((uint64_t)(uint16_t)s1l << 48) | ((uint64_t)(uint16_t)s2l << 32) |
+                                  ((uint64_t)(uint16_t)l2 << 16) | (uint64_t)(uint16_t)n1;
+                            tu = make_float2((float)tt, (float)uu);
+                            in = 1;
+                        } else {
+                            key = ((uint64_t)(uint16_t)s2l << 48) | ((uint64_t)(uint16_t)s1l << 32) |
+                                  ((uint64_t)(uint16_t)n1 << 16) | (uint64_t)(uint16_t)l2;
+                            tu = make_float2((float)uu, (float)tt);
+                            in = 0;
+                        }
+                        long long slot = (long long)id1 * K + count;
+                        slotKey[slot] = key;
+                        slotTU[slot] = tu;
+                        slotIn[slot] = in;
+                    }
+                    count++;
+                }
+            }
+        }
+        numNeighbors[id1] = count;
+    }
+    int m = wave_max_int(count);
+    if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(&scal[SC_MAXCOUNT], m);
+}
+
+int run_detect(pcp_ctx* c, const double2* pos) {
+    (void)pos;
+    if (c->V == 0) return PCP_OK;
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(BLK), 0, c->stream, c->dScal, (long long)SC_NSCAL, 0);
+    PCP_LAUNCHED();
+    PCP_LAUNCH(c, k_detect, dim3(nblk(c->V)), dim3(BLK), c->V, c->N, c->K,
+                       c->dPolyPack, c->dPackPoly, c->dBox, c->dCellOff, c->dCellOf, c->dCellStart,
+                       c->dSP, c->dSR, c->dSI, c->dSlotKey, c->dSlotTU, c->dSlotIn, c->dNum, c->dScal);
+    return PCP_OK;
+}
+
+// restage the sorted edge data from the live positions (updateNeighbors reads the
+// positions at call time, kernels.cuh:721-725; the cells stay those of the last
+// updateNeighborCells)
+__global__ void k_restage(int V, const double2* __restrict__ pos, const int* __restrict__ next,
+                          const int* __restrict__ cellSorted, double2* __restrict__ SP,
+                          double2* __restrict__ SR) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= V) return;
+    int v = cellSorted[t];
+    d2 p = ldpos(pos, v), pn = ldpos(pos, next[v]);
+    SP[t] = make_double2(p.x, p.y);
+    SR[t] = make_double2(wrap(pn.x - p.x), wrap(pn.y - p.y));
+}
+
+// stable rank sort of keys by their low endBit bits (sortKeysCUDA model.cu:50-75)
+__global__ void k_sort_keys(int n, uint64_t mask, const uint64_t* __restrict__ in,
+                            uint64_t* __restrict__ out, uint32_t* __restrict__ perm) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    uint64_t k = in[r] & mask;
+    int rank = 0;
+    for (int q = 0; q < n; ++q) {
+        uint64_t o = in[q] & mask;
+        rank += (o < k) || (o == k && q < r);
+    }
+    out[rank] = in[r];
+    perm[rank] = (uint32_t)r;
+}
+
+// ------------------------------------------------------------------ records
+// bucket sizes from the stored crossings (updateValidAndCountsKernel kernels.cuh:814-836
+// counts per si; the sj counts index the records owned through sj).  Recounting here
+// keeps updateOutersections idempotent; the scatter below returns the counts to zero.
+__global__ void k_count(int V, int N, int P, int K, const int* __restrict__ shapeId,
+                        const int* __restrict__ polyPack, const int* __restrict__ packPoly,
+                        const int* __restrict__ numNeighbors, const uint64_t* __restrict__ slotKey,
+                        int* __restrict__ counts, unsigned long long* __restrict__ packMaxKey) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < P) packMaxKey[v] = 0ull;
+    if (v >= V) return;
+    int cnt = min(numNeighbors[v], K);
+    if (cnt == 0) return;
+    int pbase = packPoly[polyPack[shapeId[v]]];
+    for (int c = 0; c < cnt; ++c) {
+        uint64_t key = slotKey[(long long)v * K + c];
+        atomicAdd(&counts[pbase + key_si(key)], 1);
+        atomicAdd(&counts[N + pbase + key_sj(key)], 1);
+    }
+}
+
+// scatter each crossing into its si bucket and its sj bucket (unordered)
+__global__ void k_bucket_scatter(int V, int N, int K, long long Icap, const int* __restrict__ shapeId,
+                                 const int* __restrict__ polyPack, const int* __restrict__ packPoly,
+                                 const int* __restrict__ numNeighbors, const uint64_t* __restrict__ slotKey,
+                                 const float2* __restrict__ slotTU, int* __restrict__ counts,
+                                 const int* __restrict__ bStart, uint64_t* __restrict__ keys2,
+                                 float2* __restrict__ bTU, int* __restrict__ bPoly,
+                                 unsigned long long* __restrict__ packMaxKey, int* __restrict__ scal) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long mk = 0;
+    int p = -1;
+    if (v < V) {
+        int cnt = min(numNeighbors[v], K);
+        p = polyPack[shapeId[v]];
+        int pbase = packPoly[p];
+        long long I = bStart[N];
+        if (I > Icap) scal[SC_OVERFLOW] = 1;
+        for (int c = 0; c < cnt; ++c) {
+            long long slot = (long long)v * K + c;
+            uint64_t key = slotKey[slot];
+            float2 tu = slotTU[slot];
+            int sig = pbase + key_si(key), sjg = pbase + key_sj(key);
+            int a = atomicSub(&counts[sig], 1) - 1;
+            long long p1 = (long long)bStart[sig] + a;
+            if (p1 < Icap) {
+                keys2[p1] = key;
+                bTU[p1] = tu;
+                bPoly[p1] = sig;
+            }
+            int b = atomicSub(&counts[N + sjg], 1) - 1;
+            long long p2 = (long long)bStart[N + sjg] + b;   // in [I, 2I)
+            if (p2 < 2 * Icap && I <= Icap) keys2[p2] = key;
+            if (key > mk) mk = key;
+        }
+    }
+    // per-packing maximum key (needed by the pairing rule); one atomic per wave
+    // when the whole wavefront belongs to one packing
+    int p0 = __shfl(p, 0);
+    bool uni = __all(p == p0 || p == -1);
+    if (uni) {
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            unsigned long long y = __shfl_xor(mk, o);
+            mk = (y > mk) ? y : mk;
+        }
+        if ((threadIdx.x & 63) == 0 && mk > 0 && p0 >= 0) atomicMax(&packMaxKey[p0], mk);
+    } else if (mk > 0) {
+        atomicMax(&packMaxKey[p], mk);
+    }
+}
+
+// rank each record inside its si bucket (final order = (si, i, j, sj)) and pair it
+// with its exit record: updateOutersectionsKernel kernels.cuh:981-1065.
+// The reference scans the group of keys with prefix (si<<48 | sj<<32) in 64-bit key
+// order; that group is the si'=sj bucket filtered on sj'=si, and its key order is
+// bucket_order order.  The choice (min cyclic distance d, then min float u, ties to
+// the first in key order; edge l==i only when u > t) does not depend on scan order.
+// The reference's binary search starts with end = I-1, so when every key lies below
+// (si<<48 | sj<<32) the last record of the packing is the only candidate; kept.
+__global__ __launch_bounds__(BLK) void k_rank_pair(
+    int N, long long Icap, const int* __restrict__ polyStart, const int* __restrict__ polyPack,
+    const int* __restrict__ packPoly, const int* __restrict__ bStart, const uint64_t* __restrict__ keys2,
+    const float2* __restrict__ bTU, const int* __restrict__ bPoly,
+    const unsigned long long* __restrict__ packMaxKey, uint64_t* __restrict__ rKey,
+    float2* __restrict__ rTU, uint64_t* __restrict__ rOuter, float2* __restrict__ rUT,
+    int* __restrict__ rPoly, int* __restrict__ sjList, const double2* __restrict__ pairPos) {
+    long long I = bStart[N];
+    if (I > Icap) return;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < I;
+         e += (long long)gridDim.x * blockDim.x) {
+        uint64_t key = keys2[e];
+        float2 tu = bTU[e];
+        int s = bPoly[e];
+        int p = polyPack[s];
+        int pbase = packPoly[p];
+        int lo = bStart[s], hi = bStart[s + 1];
+        uint64_t ord = bucket_order(key);
+        int rank = 0;
+        for (int q = lo; q < hi; ++q) rank += (bucket_order(keys2[q]) < ord);
+        long long f = lo + rank;
+
+        int si_l = key_si(key), sj_l = key_sj(key), i = key_i(key);
+        int ni = polyStart[s + 1] - polyStart[s];
+        int sjg0 = pbase + sj_l;
+        int nj = polyStart[sjg0 + 1] - polyStart[sjg0];
+        // exact parameter along si's edge ei of its crossing with sj's edge ej (area energy)
+        auto exact = [&](int ei, int ej, double fb) -> double {
+            int a = polyStart[s] + ei, b = polyStart[s] + ((ei + 1 == ni) ? 0 : ei + 1);
+            int c2 = polyStart[sjg0] + ej, d = polyStart[sjg0] + ((ej + 1 == nj) ? 0 : ej + 1);
+            return cross_param(ldpos(pairPos, a), ldpos(pairPos, b), ldpos(pairPos, c2), ldpos(pairPos, d), fb);
+        };
+        double tVal = (double)tu.y;
+        if (pairPos) tVal = exact(i, key_j(key), tVal);
+        uint64_t sij = ((uint64_t)si_l << 48) | ((uint64_t)sj_l << 32);
+        uint64_t mxk = packMaxKey[p];
+        int bestDist = -1;
+        bool found = false;
+        double bestU = 0.0;
+        uint64_t bestOrd = 0, bestKey = 0;
+        float2 bestTU = make_float2(0.f, 0.f);
+        auto consider = [&](uint64_t k2, float2 tu2) {
+            int l = key_j(k2);
+            double uVal = (double)tu2.x;
+            if (pairPos && mxk >= sij) uVal = exact(l, key_i(k2), uVal);
+            int d = (l - i + ni) % ni;
+            if (l == i && tVal >= uVal) return;
+            if (bestDist == -1 || d < bestDist) { bestDist = d; found = false; }
+            if (d == bestDist && uVal == uVal) {
+                uint64_t o2 = bucket_order(k2);
+                if (!found || uVal < bestU || (uVal == bestU && o2 < bestOrd)) {
+                    found = true; bestU = uVal; bestOrd = o2; bestKey = k2; bestTU = tu2;
+                }
+            }
+        };
+        if (mxk < sij) {
+            int s3 = pbase + key_si(mxk);
+            for (int q = bStart[s3]; q < bStart[s3 + 1]; ++q)
+                if (keys2[q] == mxk) { consider(mxk, bTU[q]); break; }
+        } else {
+            int sjg = pbase + sj_l;
+            for (int q = bStart[sjg]; q < bStart[sjg + 1]; ++q) {
+                uint64_t k2 = keys2[q];
+                if (key_sj(k2) != si_l) continue;
+                consider(k2, bTU[q]);
+            }
+        }
+        // a partner whose edge indices fall outside the two polygons (possible only in the
+        // last-record case above) is treated as no partner: the reference would read
+        // vertices of other polygons there
+        if (found && (key_i(bestKey) >= nj || key_j(bestKey) >= ni)) found = false;
+        rKey[f] = key;
+        rTU[f] = tu;
+        rOuter[f] = found ? bestKey : key;
+        rUT[f] = found ? bestTU : tu;
+        rPoly[f] = s;
+        // position among the records whose sj is this record's sj (key order)
+        int sjg = pbase + sj_l;
+        int lo2 = bStart[N + sjg], hi2 = bStart[N + sjg + 1];
+        int rA = 0;
+        for (int q = lo2; q < hi2; ++q) rA += (keys2[q] < key);
+        sjList[lo2 - I + rA] = (int)f;
+    }
+}
+
+int record_grid(pcp_ctx* c) {
+    long long est = std::max<long long>(c->lastI + c->lastI / 2, 4096);
+    est = std::min<long long>(est, c->Icap);
+    int g = nblk(est);
+    return std::max(1, std::min(g, 8192));
+}
+
+int run_records(pcp_ctx* c, const double2* pairPos) {
+    if (c->V == 0) return PCP_OK;
+    hipLaunchKernelGGL(k_count, dim3(nblk(std::max(c->V, c->P))), dim3(BLK), 0, c->stream, c->V, c->N, c->P,
+                       c->K, c->dShapeId, c->dPolyPack, c->dPackPoly, c->dNum, c->dSlotKey, c->dCounts,
+                       c->dPackMaxKey);
+    PCP_LAUNCHED();
+    PCP_TRY(scan_exclusive(c, c->dCounts, 2 * c->N, c->dBStart));
+    PCP_LAUNCH(c, k_bucket_scatter, dim3(nblk(c->V)), dim3(BLK), c->V, c->N, c->K,
+                       c->Icap, c->dShapeId, c->dPolyPack, c->dPackPoly, c->dNum, c->dSlotKey,
+                       c->dSlotTU, c->dCounts, c->dBStart, c->dKeys2, c->dBTU, c->dBPoly,
+                       c->dPackMaxKey, c->dScal);
+    PCP_LAUNCH(c, k_rank_pair, dim3(record_grid(c)), dim3(BLK), c->N, c->Icap,
+                       c->dPolyStart, c->dPolyPack, c->dPackPoly, c->dBStart, c->dKeys2, c->dBTU,
+                       c->dBPoly, c->dPackMaxKey, c->dRKey, c->dRTU, c->dROuter, c->dRUT, c->dRPoly,
+                       c->dSjList, pairPos);
+    return PCP_OK;
+}
+
+// ------------------------------------------------------------------ forces
+// updateForceEnergyExteriorKernel kernels.cuh:1067-1256: the 8 contributions of one
+// record (slots i, zi, l, zl owned by si; j, zj, k, zk owned by sj), its energy and
+// its start point fij (also the start point of the interior chain terms, :1390-1396).
+__global__ __launch_bounds__(BLK) void k_exterior(
+    int N, long long Icap, const int* __restrict__ bStart, const int* __restrict__ polyStart,
+    const int* __restrict__ polyPack, const int* __restrict__ packPoly, const int* __restrict__ next,
+    const double2* __restrict__ pos, const uint64_t* __restrict__ rKey,
+    const uint64_t* __restrict__ rOuter, const float2* __restrict__ rTU, const float2* __restrict__ rUT,
+    const int* __restrict__ rPoly, double2* __restrict__ contrib, int* __restrict__ tgt,
+    double* __restrict__ eext, double2* __restrict__ spOut, int area,
+    const unsigned char* __restrict__ closed) {
+    long long I = bStart[N];
+    if (I > Icap) return;
+    for (long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x; f < I;
+         f += (long long)gridDim.x * blockDim.x) {
+        uint64_t inter = rKey[f], outer = rOuter[f];
+        if (inter == outer) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) tgt[f * 8 + q] = -1;
+            eext[f] = 0.0;
+            continue;
+        }
+        int sig = rPoly[f];
+        int pbase = packPoly[polyPack[sig]];
+        int sjg = pbase + key_sj(inter);
+        int st1 = polyStart[sig], st2 = polyStart[sjg];
+        int i = st1 + key_i(inter), j = st2 + key_j(inter);
+        int k = st2 + key_i(outer), l = st1 + key_j(outer);
+        d2 pi = ldpos(pos, i), pj = ldpos(pos, j), pk = ldpos(pos, k), pl = ldpos(pos, l);
+        int zi = next[i], zj = next[j], zk = next[k], zl = next[l];
+        d2 pzi = ldpos(pos, zi), pzj = ldpos(pos, zj), pzk = ldpos(pos, zk), pzl = ldpos(pos, zl);
+        d2 r1 = wrap2(mk(pzi.x - pi.x, pzi.y - pi.y));
+        d2 r2 = wrap2(mk(pzk.x - pk.x, pzk.y - pk.y));
+        double t1 = (double)rTU[f].y, t2 = (double)rUT[f].y;
+        if (area) {   // exact crossing parameters at the current positions
+            t1 = cross_param(pi, pzi, pj, pzj, t1);
+            t2 = cross_param(pk, pzk, pl, pzl, t2);
+        }
+        d2 fij = wrap2(mk(pi.x + t1 * r1.x, pi.y + t1 * r1.y));
+        d2 fkl = wrap2(mk(pk.x + t2 * r2.x, pk.y + t2 * r2.y));
+        d2 sp = fij;
+        if (area && closed[f]) sp = ldpos(pos, polyStart[min(sig, sjg)]);   // one origin per pair
+        double e = 0.0;
+        double dfa[16], dfb[16];
+        d2 inc[8];
+        if (i == l) {
+            e += hfun(fij, fkl, sp);
+            d2 ga, gb;
+            g12(fij, fkl, sp, ga, gb);
+            getDf(pi, pzi, pj, pzj, dfa);
+            getDf(pk, pzk, pi, pzi, dfb);
+            inc[0] = dfdot(dfa, 0, ga);
+            inc[1] = dfdot(dfa, 2, ga);
+            inc[4] = dfdot(dfa, 4, ga);
+            inc[5] = dfdot(dfa, 6, ga);
+            inc[6] = dfdot(dfb, 0, gb);
+            inc[7] = dfdot(dfb, 2, gb);
+            inc[2] = dfdot(dfb, 4, gb);
+            inc[3] = dfdot(dfb, 6, gb);
+        } else {
+            e += hfun(fij, pzi, sp);
+            e += hfun(pl, fkl, sp);
+            d2 g1a, g2a;
+            g12(fij, pzi, sp, g1a, g2a);
+            getDf(pi, pzi, pj, pzj, dfa);
+            getDf(pk, pzk, pl, pzl, dfb);
+            inc[0] = dfdot(dfa, 0, g1a);
+            inc[4] = dfdot(dfa, 4, g1a);
+            d2 dd = dfdot(dfa, 2, g1a);
+            inc[1] = mk(dd.x + g2a.x, dd.y + g2a.y);
+            inc[5] = dfdot(dfa, 6, g1a);
+            d2 g1b, g2b;
+            g12(pl, fkl, sp, g1b, g2b);
+            d2 de = dfdot(dfb, 4, g2b);
+            inc[2] = mk(g1b.x + de.x, g1b.y + de.y);
+            inc[6] = dfdot(dfb, 0, g2b);
+            inc[3] = dfdot(dfb, 6, g2b);
+            inc[7] = dfdot(dfb, 2, g2b);
+        }
+        int tg[8] = {i, zi, l, zl, j, zj, k, zk};
+        double sc = area ? 0.5 : 1.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            tgt[f * 8 + q] = tg[q];
+            contrib[f * 8 + q] = make_double2(-inc[q].x * sc, -inc[q].y * sc);
+        }
+        eext[f] = e;
+        spOut[f] = make_double2(sp.x, sp.y);
+    }
+}
+
+// area energy: index of each record's exit record, then whether the partner links
+// close into a cycle within 8 hops (a closed lens boundary)
+__global__ void k_partner(int N, long long Icap, const int* __restrict__ bStart,
+                          const int* __restrict__ polyPack, const int* __restrict__ packPoly,
+                          const uint64_t* __restrict__ rKey, const uint64_t* __restrict__ rOuter,
+                          const int* __restrict__ rPoly, int* __restrict__ partner) {
+    long long I = bStart[N];
+    if (I > Icap) return;
+    for (long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x; f < I;
+         f += (long long)gridDim.x * blockDim.x) {
+        uint64_t key = rKey[f], o = rOuter[f];
+        int q = -1;
+        if (o != key) {
+            int s2 = packPoly[polyPack[rPoly[f]]] + key_si(o);
+            for (int r = bStart[s2]; r < bStart[s2 + 1]; ++r)
+                if (rKey[r] == o) { q = r; break; }
+        }
+        partner[f] = q;
+    }
+}
+__global__ void k_loop(int N, long long Icap, const int* __restrict__ bStart, const int* __restrict__ partner,
+                       unsigned char* __restrict__ closed) {
+    long long I = bStart[N];
+    if (I > Icap) return;
+    for (long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x; f < I;
+         f += (long long)gridDim.x * blockDim.x) {
+        int q = partner[f];
+        for (int hop = 0; hop < 8 && q >= 0 && q != (int)f; ++hop) q = partner[q];
+        closed[f] = (q == (int)f) ? 1 : 0;
+    }
+}
+
+// per-vertex force: exterior contributions of the records owned through si, then
+// through sj (record order), then the interior chain sums of this vertex's edge and
+// of the previous edge (updateForceEnergyInteriorKernel kernels.cuh:1354-1424), then
+// the edge spring (:1284-1335) and area spring (:1337-1352) terms.
+__global__ __launch_bounds__(BLK) void k_gather(
+    int V, int N, long long Icap, int overlap, int edgeOn, int areaOn, const int* __restrict__ shapeId,
+    const int* __restrict__ polyStart, const int* __restrict__ next, const int* __restrict__ prev,
+    const double2* __restrict__ pos, const int* __restrict__ bStart, const uint64_t* __restrict__ rKey,
+    const uint64_t* __restrict__ rOuter, const int* __restrict__ sjList,
+    const double2* __restrict__ contrib, const int* __restrict__ tgt, const double2* __restrict__ sp,
+    const double* __restrict__ edgeLengths, const double* __restrict__ areas,
+    const double* __restrict__ targetEdgeV, const double* __restrict__ targetAreas, double stiffness,
+    double compressibility, double2* __restrict__ force, double* __restrict__ eint,
+    double* __restrict__ eedge, int area) {
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= V) return;
+    int s = shapeId[m];
+    int st = polyStart[s], n = polyStart[s + 1] - st;
+    int mLoc = m - st;
+    int nx = next[m], pm = prev[m];
+    d2 F = mk(0.0, 0.0);
+    double le = 0.0;
+    if (overlap && bStart[N] <= Icap) {
+        int I = bStart[N];
+        int lo = bStart[s], hi = bStart[s + 1];
+        for (int r = lo; r < hi; ++r) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (tgt[r * 8 + q] == m) {
+                    double2 cc = contrib[r * 8 + q];
+                    F.x += cc.x; F.y += cc.y;
+                }
+        }
+        int lo2 = bStart[N + s] - I, hi2 = bStart[N + s + 1] - I;
+        for (int e = lo2; e < hi2; ++e) {
+            int r = sjList[e];
+#pragma unroll
+            for (int q = 4; q < 8; ++q)
+                if (tgt[r * 8 + q] == m) {
+                    double2 cc = contrib[r * 8 + q];
+                    F.x += cc.x; F.y += cc.y;
+                }
+        }
+        d2 Pm = ldpos(pos, m), Pn = ldpos(pos, nx), Pp = ldpos(pos, pm);
+        int pmLoc = pm - st;
+        d2 LM = mk(0.0, 0.0), LN = mk(0.0, 0.0);
+        for (int r = lo; r < hi; ++r) {
+            uint64_t inter = rKey[r], outer = rOuter[r];
+            if (inter == outer) continue;
+            int iLoc = key_i(inter), lLoc = key_j(outer);
+            int lDist = (lLoc - iLoc + n) % n;
+            int dm = (mLoc - iLoc + n) % n;
+            int dp = (pmLoc - iLoc + n) % n;
+            bool cm = (dm > 0 && dm < lDist), cp = (dp > 0 && dp < lDist);
+            if (cm || cp) {
+                double2 s2 = sp[r];
+                d2 spr = mk(s2.x, s2.y);
+                d2 g1, g2;
+                if (cm) {
+                    le += hfun(Pm, Pn, spr);
+                    g12(Pm, Pn, spr, g1, g2);
+                    if (area) { g1.x *= 0.5; g1.y *= 0.5; }
+                    LM.x -= g1.x; LM.y -= g1.y;
+                }
+                if (cp) {
+                    g12(Pp, Pm, spr, g1, g2);
+                    if (area) { g2.x *= 0.5; g2.y *= 0.5; }
+                    LN.x -= g2.x; LN.y -= g2.y;
+                }
+            }
+        }
+        if (LM.x != 0.0 || LM.y != 0.0) { F.x += LM.x; F.y += LM.y; }
+        if (LN.x != 0.0 || LN.y != 0.0) { F.x += LN.x; F.y += LN.y; }
+    }
+    double ee = 0.0;
+    if (edgeOn) {
+        double l0 = targetEdgeV[m], l0prv = targetEdgeV[pm];
+        double l = edgeLengths[m], prvl = edgeLengths[pm];
+        d2 Pm = ldpos(pos, m), Pn = ldpos(pos, nx), Pp = ldpos(pos, pm);
+        d2 a = mk(Pm.x - Pn.x, Pm.y - Pn.y), b = mk(Pp.x - Pm.x, Pp.y - Pm.y);
+        a.x += 1.5; a.y += 1.5; b.x += 1.5; b.y += 1.5;
+        a.x = shift_into_unit(a.x); a.y = shift_into_unit(a.y);
+        b.x = shift_into_unit(b.x); b.y = shift_into_unit(b.y);
+        a.x -= 0.5; a.y -= 0.5; b.x -= 0.5; b.y -= 0.5;
+        double c1 = (l > 1e-14) ? (1.0 - l0 / l) : 0.0;
+        double c2 = (prvl > 1e-14) ? (1.0 - l0prv / prvl) : 0.0;
+        ee = stiffness * (l - l0) * (l - l0) / 2.0;
+        d2 lf;
+        lf.x = -(c1 * a.x - c2 * b.x) * stiffness;
+        lf.y = -(c1 * a.y - c2 * b.y) * stiffness;
+        if (lf.x != 0.0 || lf.y != 0.0) { F.x += lf.x; F.y += lf.y; }
+    }
+    if (areaOn) {
+        double A = areas[s], A0 = targetAreas[s];
+        if (!(A0 < 1e-14)) {
+            double coeff = 0.5 * compressibility * (A - A0);
+            d2 Pn = ldpos(pos, nx), Pp = ldpos(pos, pm);
+            double fx = -coeff * wrap(Pn.y - Pp.y);
+            double fy = -coeff * wrap(Pp.x - Pn.x);
+            F.x += fx; F.y += fy;
+        }
+    }
+    force[m] = make_double2(F.x, F.y);
+    eint[m] = le;
+    eedge[m] = ee;
+}
+
+// per-polygon energy: 0.5*(exterior of its records + interior of its edges)
+// + edge springs + area spring
+__global__ void k_poly_energy(int N, long long Icap, int overlap, int edgeOn, int areaOn,
+                              const int* __restrict__ polyStart,
+                              const int* __restrict__ bStart, const double* __restrict__ eext,
+                              const double* __restrict__ eint, const double* __restrict__ eedge,
+                              const double* __restrict__ areas, const double* __restrict__ targetAreas,
+                              double compressibility, double* __restrict__ epoly) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N) return;
+    int st = polyStart[s], en = polyStart[s + 1];
+    double acc = 0.0;
+    if (overlap && bStart[N] <= Icap) {
+        for (int r = bStart[s]; r < bStart[s + 1]; ++r) acc += eext[r];
+        for (int m = st; m < en; ++m) acc += eint[m];
+    }
+    double e = 0.5 * acc;
+    if (edgeOn)
+        for (int m = st; m < en; ++m) e += eedge[m];
+    if (areaOn) {
+        double A = areas[s], A0 = targetAreas[s];
+        if (!(A0 < 1e-14)) e += compressibility * (A - A0) * (A - A0) / 2.0;
+    }
+    epoly[s] = e;
+}
+
+// per-packing deterministic reductions over polygons: nArrays arrays of N values
+// (stride N) are tree-summed; if nMax > 0 the last nMax arrays are max-reduced.
+// One block of 256 threads per packing.
+__global__ __launch_bounds__(BLK) void k_det_reduce(const int* __restrict__ packPoly,
+                                                   const double* __restrict__ x, int N, int nArrays,
+                                                   int nMax, double* __restrict__ out, int P) {
+    __shared__ double lds[BLK];
+    __shared__ double part[BLK];
+    int p = blockIdx.x;
+    int a0 = packPoly[p], a1 = packPoly[p + 1];
+    int n = a1 - a0;
+    int t = threadIdx.x;
+    for (int a = 0; a < nArrays; ++a) {
+        const double* xa = x + (long long)a * N + a0;
+        bool isMax = (a >= nArrays - nMax);
+        double res;
+        if (isMax) {
+            double m = 0.0;
+            for (int q = t; q < n; q += BLK) m = fmax(m, xa[q]);
+            __syncthreads();
+            lds[t] = m;
+            __syncthreads();
+            for (int s = BLK / 2; s >= 1; s >>= 1) {
+                if (t < s) lds[t] = fmax(lds[t], lds[t + s]);
+                __syncthreads();
+            }
+            res = lds[0];
+        } else {
+            int chunks = (n + BLK - 1) / BLK;
+            if (chunks <= 1) {
+                res = tree256((t < n) ? xa[t] : 0.0, lds);
+                if (t == 0) part[0] = res;
+                __syncthreads();
+                res = part[0];
+            } else {
+                // level 1: one tree per 256-chunk (n <= 65536 polygons => <= 256 chunks)
+                for (int cch = 0; cch < chunks; ++cch) {
+                    int q = cch * BLK + t;
+                    double r = tree256((q < n) ? xa[q] : 0.0, lds);
+                    if (t == 0) part[cch] = r;
+                }
+                __syncthreads();
+                double v = (t < chunks) ? part[t] : 0.0;
+                res = tree256(v, lds);
+                __syncthreads();
+                if (t == 0) part[0] = res;
+                __syncthreads();
+                res = part[0];
+            }
+        }
+        if (t == 0) out[(long long)a * P + p] = res;
+        __syncthreads();
+    }
+}
+
+int launch_det_reduce(pcp_ctx* c, const double* partials, int nArrays, double* outSums, int nMax) {
+    if (c->P == 0) return PCP_OK;
+    PCP_LAUNCH(c, k_det_reduce, dim3(c->P), dim3(BLK), c->dPackPoly, partials, c->N,
+                       nArrays, nMax, outSums, c->P);
+    return PCP_OK;
+}
+
+int run_forces(pcp_ctx* c, int model, const double2* pos, const double* edgeLengths, const double* areas,
+               const double* targetEdgeV, const double* targetAreas, double stiffness,
+               double compressibility, double2* force, double* energy) {
+    if (c->V == 0) return PCP_OK;
+    int area = (model & PCP_AREA_ENERGY) ? 1 : 0;
+    model &= 0xFF;
+    int overlap = (model == PCP_NORMAL || model == PCP_HYBRID);
+    int edgeOn = (model == PCP_SOFT_BODY || model == PCP_EDGE_ONLY || model == PCP_HYBRID);
+    int areaOn = (model == PCP_SOFT_BODY || model == PCP_AREA_ONLY || model == PCP_HYBRID);
+    if (model == PCP_ABNORMAL) {   // model.cpp:390-391: forces and energy zeroed only
+        hipLaunchKernelGGL(k_fill_double, dim3(nblk(2LL * c->V)), dim3(BLK), 0, c->stream, (double*)force,
+                           2LL * c->V, 0.0);
+        PCP_LAUNCHED();
+        hipLaunchKernelGGL(k_fill_double, dim3(nblk(c->P)), dim3(BLK), 0, c->stream, energy,
+                           (long long)c->P, 0.0);
+        PCP_LAUNCHED();
+        return PCP_OK;
+    }
+    if (overlap) {
+        if (area) {
+            PCP_LAUNCH(c, k_partner, dim3(record_grid(c)), dim3(BLK), c->N, c->Icap,
+                               c->dBStart, c->dPolyPack, c->dPackPoly, c->dRKey, c->dROuter, c->dRPoly,
+                               c->dPartner);
+            PCP_LAUNCH(c, k_loop, dim3(record_grid(c)), dim3(BLK), c->N, c->Icap,
+                               c->dBStart, c->dPartner, c->dClosed);
+        }
+        PCP_LAUNCH(c, k_exterior, dim3(record_grid(c)), dim3(BLK), c->N, c->Icap,
+                           c->dBStart, c->dPolyStart, c->dPolyPack, c->dPackPoly, c->dNext, pos, c->dRKey,
+                           c->dROuter, c->dRTU, c->dRUT, c->dRPoly, c->dContrib, c->dTgt, c->dEext,
+                           c->dSp, area, c->dClosed);
+    }
+    PCP_LAUNCH(c, k_gather, dim3(nblk(c->V)), dim3(BLK), c->V, c->N, c->Icap, overlap, edgeOn,
+                       areaOn, c->dShapeId, c->dPolyStart, c->dNext, c->dPrev, pos, c->dBStart, c->dRKey,
+                       c->dROuter, c->dSjList, c->dContrib, c->dTgt, c->dSp, edgeLengths, areas,
+                       targetEdgeV, targetAreas, stiffness, compressibility, force, c->dEint, c->dEedge,
+                       area);
+    PCP_LAUNCH(c, k_poly_energy, dim3(nblk(c->N)), dim3(BLK), c->N, c->Icap, overlap, edgeOn,
+                       areaOn, c->dPolyStart, c->dBStart, c->dEext, c->dEint, c->dEedge, areas, targetAreas,
+                       compressibility, c->dEpoly);
+    return launch_det_reduce(c, c->dEpoly, 1, energy, 0);
+}
+
+// ------------------------------------------------------------------ projection
+// Per polygon, remove from f its component in span{grad l_e (n edges), grad A}
+// (projectForceCUDA model.cu:165-206, kernels.cuh:389-523; the reference builds an
+// SVD basis of the edge gradients and Gram-Schmidts the normalized area gradient; the
+// projector is basis independent).  Normal equations of [A a]: G = A^T A is cyclic
+// tridiagonal (G_ee = 2|t_e|^2, G_{e,e+1} = -t_e.t_{e+1}), bordered by b = A^T a and
+// c = a.a.  Solved by Thomas elimination on the leading (n-1)x(n-1) block with three
+// right-hand sides, then the 1x1 Schur complements of the cyclic corner and of the
+// area row.  The area direction is dropped when its residual after removing the edge
+// directions is below 1e-14 (normalized), as kernels.cuh:464.  Degenerate edges
+// (|e| < 1e-14, kernels.cuh:399) contribute no direction.
+struct PolySolver {
+    double* tx; double* ty; double* ax; double* ay; double* cp; double* x1; double* x2; double* x3;
+};
+
+// Solve [[G, b],[b^T, c]] [y; z] = [r; s]; y stored in S.x1[0..n), returns z.
+// rhs r in S.x1 on entry.  S.tx/ty/ax/ay hold tangents and area gradient.
+__device__ double bordered_solve(const PolySolver& S, int n, double c, double s, double cthresh) {
+    // off_e = G_{e,e+1 mod n}
+    auto off = [&](int e) -> double {
+        int f = (e + 1 == n) ? 0 : e + 1;
+        return -(S.tx[e] * S.tx[f] + S.ty[e] * S.ty[f]);
+    };
+    auto diag = [&](int e) -> double {
+        double d = S.tx[e] * S.tx[e] + S.ty[e] * S.ty[e];
+        return (d > 0.0) ? 2.0 * d : 1.0;
+    };
+    auto bvec = [&](int e) -> double {
+        int f = (e + 1 == n) ? 0 : e + 1;
+        return S.tx[e] * (S.ax[f] - S.ax[e]) + S.ty[e] * (S.ay[f] - S.ay[e]);
+    };
+    int m = n - 1;
+    // right-hand sides: x1 = r (given), x2 = b, x3 = w (border column of index n-1)
+    double r_last = S.x1[m];
+    for (int j = 0; j < m; ++j) {
+        S.x2[j] = bvec(j);
+        S.x3[j] = 0.0;
+    }
+    double b_last = bvec(m);
+    S.x3[0] = off(m);                       // G_{0,n-1} (cyclic corner)
+    S.x3[m - 1] += off(m - 1);              // G_{n-2,n-1}
+    double w0 = S.x3[0], wl = (m - 1 == 0) ? 0.0 : S.x3[m - 1];
+    // forward sweep
+    double cprev = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
+    for (int j = 0; j < m; ++j) {
+        double sub = (j > 0) ? off(j - 1) : 0.0;
+        double den = diag(j) - sub * cprev;
+        double sup = (j < m - 1) ? off(j) : 0.0;
+        double cj = sup / den;
+        d1 = (S.x1[j] - sub * d1) / den;
+        d2 = (S.x2[j] - sub * d2) / den;
+        d3 = (S.x3[j] - sub * d3) / den;
+        S.cp[j] = cj;
+        S.x1[j] = d1; S.x2[j] = d2; S.x3[j] = d3;
+        cprev = cj;
+    }
+    for (int j = m - 2; j >= 0; --j) {
+        S.x1[j] = S.x1[j] - S.cp[j] * S.x1[j + 1];
+        S.x2[j] = S.x2[j] - S.cp[j] * S.x2[j + 1];
+        S.x3[j] = S.x3[j] - S.cp[j] * S.x3[j + 1];
+    }
+    // w^T X for the three solutions (w nonzero at 0 and m-1 only)
+    double wX1, wX2, wX3;
+    if (m - 1 == 0) {
+        wX1 = w0 * S.x1[0]; wX2 = w0 * S.x2[0]; wX3 = w0 * S.x3[0];
+    } else {
+        wX1 = w0 * S.x1[0] + wl * S.x1[m - 1];
+        wX2 = w0 * S.x2[0] + wl * S.x2[m - 1];
+        wX3 = w0 * S.x3[0] + wl * S.x3[m - 1];
+    }
+    double schur = diag(m) - wX3;
+    double y1l = (r_last - wX1) / schur;
+    double ul = (b_last - wX2) / schur;
+    // y1 = G^{-1} r (into x1), u = G^{-1} b (into x2)
+    for (int j = 0; j < m; ++j) {
+        S.x1[j] = S.x1[j] - S.x3[j] * y1l;
+        S.x2[j] = S.x2[j] - S.x3[j] * ul;
+    }
+    S.x1[m] = y1l;
+    S.x2[m] = ul;
+    double bu = 0.0, by = 0.0;
+    for (int e = 0; e < n; ++e) {
+        double be = bvec(e);
+        bu += be * S.x2[e];
+        by += be * S.x1[e];
+    }
+    double sigma = c - bu;
+    double z = (sigma > cthresh * c) ? (s - by) / sigma : 0.0;
+    for (int e = 0; e < n; ++e) S.x1[e] = S.x1[e] - z * S.x2[e];
+    return z;
+}
+
+__global__ __launch_bounds__(BLK) void k_project(int N, int V, const int* __restrict__ polyStart,
+                                                const double2* __restrict__ pos, double2* __restrict__ force,
+                                                double* __restrict__ work) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N) return;
+    int st = polyStart[s], n = polyStart[s + 1] - st;
+    if (n < 3) return;
+    PolySolver S;
+    S.tx = work + st; S.ty = work + V + st; S.ax = work + 2LL * V + st; S.ay = work + 3LL * V + st;
+    S.cp = work + 4LL * V + st; S.x1 = work + 5LL * V + st; S.x2 = work + 6LL * V + st;
+    S.x3 = work + 7LL * V + st;
+    double c = 0.0, sdot = 0.0;
+    for (int e = 0; e < n; ++e) {
+        int f = (e + 1 == n) ? 0 : e + 1, g = (e == 0) ? n - 1 : e - 1;
+        d2 pe = ldpos(pos, st + e), pf = ldpos(pos, st + f), pg = ldpos(pos, st + g);
+        double dx = wrap(pf.x - pe.x), dy = wrap(pf.y - pe.y);
+        double len = sqrt(dx * dx + dy * dy);
+        if (len < 1e-14) { S.tx[e] = 0.0; S.ty[e] = 0.0; }
+        else { S.tx[e] = dx / len; S.ty[e] = dy / len; }
+        double ax = wrap(pf.y - pg.y), ay = wrap(pg.x - pf.x);
+        S.ax[e] = ax; S.ay[e] = ay;
+        double2 fe = force[st + e];
+        c += ax * ax + ay * ay;
+        sdot += ax * fe.x + ay * fe.y;
+    }
+    for (int e = 0; e < n; ++e) {
+        int f = (e + 1 == n) ? 0 : e + 1;
+        double2 fe = force[st + e], ff = force[st + f];
+        S.x1[e] = S.tx[e] * (ff.x - fe.x) + S.ty[e] * (ff.y - fe.y);
+    }
+    double z = bordered_solve(S, n, c, sdot, 1e-28);
+    for (int k = 0; k < n; ++k) {
+        int g = (k == 0) ? n - 1 : k - 1;
+        double px = -S.x1[k] * S.tx[k] + S.x1[g] * S.tx[g] + z * S.ax[k];
+        double py = -S.x1[k] * S.ty[k] + S.x1[g] * S.ty[g] + z * S.ay[k];
+        double2 fk = force[st + k];
+        force[st + k] = make_double2(fk.x - px, fk.y - py);
+    }
+}
+
+int run_project(pcp_ctx* c, const double2* pos, double2* force) {
+    if (c->N == 0) return PCP_OK;
+    PCP_LAUNCH(c, k_project, dim3(nblk(c->N)), dim3(BLK), c->N, c->V, c->dPolyStart, pos,
+                       force, c->dWork);
+    return PCP_OK;
+}
+
+// ------------------------------------------------------------------ SHAKE
+// shake.h is absent from the reference (model.cu:16); contract from shakeProjectCUDA
+// model.cu:619-636 and Model::minimizeFIREStep model.cpp:418-438.  Per polygon,
+// Gauss-Newton projection onto {|edge e| = L_e, area = A0}: J J^T lambda = C,
+// x -= J^T lambda (same bordered cyclic-tridiagonal structure as the projection, with
+// the exact area gradient), until max|C| < tol.  retryIter > 0 reproduces the second
+// call with 10x iterations that minimizeFIREStep makes when the first call hit its
+// limit: polygons that converged in the first call take 0 further iterations, so the
+// per-polygon continuation equals the reference's packing-wide retry.
+// itOut[2P]: [p] = max iterations of call 1, [P+p] = of call 2 (atomicMax).
+__device__ double poly_area_dev(const double2* pos, int st, int n) {
+    d2 p0 = ldpos(pos, st);
+    double area = 0.0;
+    for (int k = 0; k < n; ++k) {
+        int kn = (k + 1 == n) ? 0 : k + 1;
+        d2 pa = ldpos(pos, st + k), pb = ldpos(pos, st + kn);
+        double x1 = wrap(pa.x - p0.x);
+        double dy2 = wrap(pb.y - pa.y);
+        double x2 = wrap(pb.x - p0.x);
+        area += dy2 * (x2 + x1) / 2.0;
+    }
+    return area;
+}
+
+__device__ int shake_poly(double2* pos, int st, int n, const double* targetEdgeV, double A0, int maxIter,
+                          double tol, const PolySolver& S) {
+    int it = 0;
+    for (; it < maxIter; ++it) {
+        double cmax = 0.0, c = 0.0;
+        for (int e = 0; e < n; ++e) {
+            int f = (e + 1 == n) ? 0 : e + 1, g = (e == 0) ? n - 1 : e - 1;
+            d2 pe = ldpos(pos, st + e), pf = ldpos(pos, st + f), pg = ldpos(pos, st + g);
+            double dx = wrap(pf.x - pe.x), dy = wrap(pf.y - pe.y);
+            double len = sqrt(dx * dx + dy * dy);
+            double ce = len - targetEdgeV[st + e];
+            if (fabs(ce) > cmax) cmax = fabs(ce);
+            S.x1[e] = ce;
+            if (len > 0.0) { S.tx[e] = dx / len; S.ty[e] = dy / len; }
+            else { S.tx[e] = 0.0; S.ty[e] = 0.0; }
+            double ax = wrap(pf.y - pg.y) / 2, ay = wrap(pg.x - pf.x) / 2;
+            S.ax[e] = ax; S.ay[e] = ay;
+            c += ax * ax + ay * ay;
+        }
+        double cA = poly_area_dev(pos, st, n) - 0.0;
+        cA = cA - A0;
+        if (fabs(cA) > cmax) cmax = fabs(cA);
+        if (cmax < tol) break;
+        double z = bordered_solve(S, n, c, cA, 1e-28);
+        for (int k = 0; k < n; ++k) {
+            int g = (k == 0) ? n - 1 : k - 1;
+            double dx = -S.x1[k] * S.tx[k] + S.x1[g] * S.tx[g] + z * S.ax[k];
+            double dy = -S.x1[k] * S.ty[k] + S.x1[g] * S.ty[g] + z * S.ay[k];
+            double2 p = pos[st + k];
+            double qx = p.x - dx, qy = p.y - dy;
+            pos[st + k] = make_double2(qx - floor(qx), qy - floor(qy));
+        }
+    }
+    return it;
+}
+
+__global__ __launch_bounds__(BLK) void k_shake(int N, int V, int P, const int* __restrict__ polyStart,
+                                              const int* __restrict__ polyPack, double2* __restrict__ pos,
+                                              const double* __restrict__ targetEdgeV,
+                                              const double* __restrict__ targetAreas, int maxIter,
+                                              double tol, int retryIter, const int* __restrict__ active,
+                                              double* __restrict__ work, int* __restrict__ itOut) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N) return;
+    int p = polyPack[s];
+    if (active && !active[p]) return;
+    int st = polyStart[s], n = polyStart[s + 1] - st;
+    if (n < 3) return;
+    PolySolver S;
+    S.tx = work + st; S.ty = work + V + st; S.ax = work + 2LL * V + st; S.ay = work + 3LL * V + st;
+    S.cp = work + 4LL * V + st; S.x1 = work + 5LL * V + st; S.x2 = work + 6LL * V + st;
+    S.x3 = work + 7LL * V + st;
+    int it1 = shake_poly(pos, st, n, targetEdgeV, targetAreas[s], maxIter, tol, S);
+    atomicMax(&itOut[p], it1);
+    if (retryIter > 0 && it1 >= maxIter) {
+        int it2 = shake_poly(pos, st, n, targetEdgeV, targetAreas[s], retryIter, tol, S);
+        atomicMax(&itOut[P + p], it2);
+    }
+}
+
+int run_shake(pcp_ctx* c, double2* pos, const double* targetEdgeV, const double* targetAreas, int maxIter,
+              double tol, int retryIter, const int* active, int* itOut) {
+    if (c->N == 0 || maxIter == 0) return PCP_OK;
+    PCP_LAUNCH(c, k_shake, dim3(nblk(c->N)), dim3(BLK), c->N, c->V, c->P, c->dPolyStart,
+                       c->dPolyPack, (double2*)pos, targetEdgeV, targetAreas, maxIter, tol, retryIter,
+                       active, c->dWork, itOut);
+    return PCP_OK;
+}
+
+// ------------------------------------------------------------------ XPBD
+// xpbdProjectCUDA model.cu:209-278 with kernels.cuh:532-623: colour-0 then colour-1
+// edge projections, then the area correction; thread per polygon (edges of one
+// colour are applied in ascending order, which equals the reference's concurrent
+// update for even n and removes its race for odd n; the area correction reads the
+// positions before the correction, which the reference's concurrent kernel races on).
+__global__ void k_xpbd_iter(int N, const int* __restrict__ polyStart, double2* __restrict__ pos,
+                            const double* __restrict__ targetEdgeP, const double* __restrict__ targetAreas,
+                            double* __restrict__ devOut, double* __restrict__ work, int V) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N) return;
+    int st = polyStart[s], n = polyStart[s + 1] - st;
+    double L0 = targetEdgeP[s];
+    for (int color = 0; color < 2; ++color)
+        for (int e = color; e < n; e += 2) {
+            int k = st + e, nk = st + ((e + 1 == n) ? 0 : e + 1);
+            double2 a = pos[k], b = pos[nk];
+            double ex = wrap(b.x - a.x), ey = wrap(b.y - a.y);
+            double len = sqrt(ex * ex + ey * ey);
+            if (len < 1e-14) continue;
+            double cc = 0.5 * (len - L0) / len;
+            pos[k] = make_double2(a.x + cc * ex, a.y + cc * ey);
+            double2 b2 = pos[nk];
+            pos[nk] = make_double2(b2.x - cc * ex, b2.y - cc * ey);
+        }
+    double2 ps = pos[st];
+    double xs = ps.x, ys = ps.y;
+    double area = 0.0, g2 = 0.0;
+    double* old = work + 2LL * st;
+    for (int e = 0; e < n; ++e) {
+        double2 q = pos[st + e];
+        old[2 * e] = q.x; old[2 * e + 1] = q.y;
+    }
+    for (int e = 0; e < n; ++e) {
+        int nk = (e + 1 == n) ? 0 : e + 1, pk = (e == 0) ? n - 1 : e - 1;
+        double xk = wrap(old[2 * e] - xs), yk = wrap(old[2 * e + 1] - ys);
+        double xnk = wrap(old[2 * nk] - xs), ynk = wrap(old[2 * nk + 1] - ys);
+        double xpk = wrap(old[2 * pk] - xs), ypk = wrap(old[2 * pk + 1] - ys);
+        area += 0.5 * (xk * ynk - xnk * yk);
+        double gx = 0.5 * (ynk - ypk), gy = 0.5 * (xpk - xnk);
+        g2 += gx * gx + gy * gy;
+    }
+    double alpha = (area - targetAreas[s]) / fmax(g2, 1e-24);
+    for (int e = 0; e < n; ++e) {
+        int nk = (e + 1 == n) ? 0 : e + 1, pk = (e == 0) ? n - 1 : e - 1;
+        double xnk = wrap(old[2 * nk] - xs), ynk = wrap(old[2 * nk + 1] - ys);
+        double xpk = wrap(old[2 * pk] - xs), ypk = wrap(old[2 * pk + 1] - ys);
+        double gx = 0.5 * (ynk - ypk), gy = 0.5 * (xpk - xnk);
+        pos[st + e] = make_double2(old[2 * e] - alpha * gx, old[2 * e + 1] - alpha * gy);
+    }
+    double maxE = 0.0;
+    for (int e = 0; e < n; ++e) {
+        int k = st + e, nk = st + ((e + 1 == n) ? 0 : e + 1);
+        double2 a = pos[k], b = pos[nk];
+        double ex = wrap(b.x - a.x), ey = wrap(b.y - a.y);
+        double d = fabs(sqrt(ex * ex + ey * ey) - L0);
+        if (d > maxE) maxE = d;
+    }
+    devOut[s] = maxE;
+    devOut[N + s] = fabs(area - targetAreas[s]);
+}
+
+// ------------------------------------------------------------------ misc kernels
+__global__ void k_max_abs_poly(int N, const int* __restrict__ polyStart, const double2* __restrict__ x,
+                               double* __restrict__ out) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N) return;
+    double m = 0.0;
+    for (int v = polyStart[s]; v < polyStart[s + 1]; ++v) {
+        double2 a = x[v];
+        m = fmax(m, fmax(fabs(a.x), fabs(a.y)));
+    }
+    out[s] = m;
+}
+// updatePositionsKernel kernels.cuh:1433-1438
+__global__ void k_update_positions(long long n2, double* pos, const double* force, double dt) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n2) return;
+    double p = pos[i] + force[i] * dt;
+    pos[i] = p - floor(p);
+}
+// resetAreasKernel kernels.cuh:1440-1455
+__global__ void k_reset_areas(long long n2, const int* __restrict__ shapeId, double* pos,
+                              const double* areas, const double* targetAreas, const double* comX,
+                              const double* comY) {
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n2) return;
+    double x = pos[idx];
+    int s = shapeId[idx / 2];
+    double currentArea = areas[s];
+    double targetArea = targetAreas[s];
+    if (!(currentArea > 0.0)) return;
+    double rescale = sqrt(targetArea / currentArea);
+    double u = (idx % 2) ? wrap(x - comY[s]) : wrap(x - comX[s]);
+    u *= rescale;
+    double p = (idx % 2) ? u + comY[s] : u + comX[s];
+    pos[idx] = p - floor(p);
+}
+// effectiveForceMagKernel kernels.cuh:625-633, max per polygon
+__global__ void k_eff_force_poly(int N, const int* __restrict__ polyStart, const double2* __restrict__ pos,
+                                 const double2* __restrict__ tent, const double2* __restrict__ force,
+                                 double scale, double* __restrict__ out) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N) return;
+    double m = 0.0;
+    for (int k = polyStart[s]; k < polyStart[s + 1]; ++k) {
+        double fx = force[k].x + scale * (pos[k].x - tent[k].x);
+        double fy = force[k].y + scale * (pos[k].y - tent[k].y);
+        m = fmax(m, sqrt(fx * fx + fy * fy));
+    }
+    out[s] = m;
+}
+// wrapPeriodic kernels.cuh:108-113
+__device__ __forceinline__ double wrapPeriodic(double x) {
+    x += 1.5;
+    int guard = 0;
+    while (x > 1.0 && guard < 64) { x -= 1.0; ++guard; }
+    return x - 0.5;
+}
+// updateOverlapAreaKernel kernels.cuh:879-979 (grid points of one packing per launch)
+__global__ void k_overlap_count(int D, int B, int Vp, const int* __restrict__ shapeIdLocal0, int vbase,
+                                const int* __restrict__ shapeId, const int* __restrict__ polyStart,
+                                const int* __restrict__ cellStartP, const int* __restrict__ cellSorted,
+                                const double2* __restrict__ pos, int* __restrict__ counter,
+                                long long* __restrict__ sum) {
+    (void)shapeIdLocal0; (void)Vp;
+    const double eps = 1e-12, pi = 3.141592653589793238462643383279;
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    int total = D * D;
+    if (idx >= total) return;
+    int ix = idx % D, iy = idx / D;
+    double px = (ix + 0.5) / D, py = (iy + 0.5) / D;
+    int num = 0;
+    int cellX = (int)(px * B), cellY = (int)(py * B);
+    int tracked[512];
+    int nt = 0;
+    bool full = false;
+    for (int dx = -1; dx <= 1 && !full; dx++) {
+        int nx = cellX + dx;
+        if (nx < 0) nx += B; else if (nx >= B) nx -= B;
+        for (int dy = -1; dy <= 1 && !full; dy++) {
+            int ny = cellY + dy;
+            if (ny < 0) ny += B; else if (ny >= B) ny -= B;
+            int boxId = ny * B + nx;
+            int q0 = cellStartP[boxId], q1 = cellStartP[boxId + 1];
+            for (int q = q0; q < q1 && !full; q++) {
+                int poly = shapeId[cellSorted[q]];
+                bool seen = false;
+                for (int t = 0; t < nt; t++) if (tracked[t] == poly) { seen = true; break; }
+                if (seen) continue;
+                int st = polyStart[poly], en = polyStart[poly + 1];
+                if (en - st < 3) continue;
+                double angleSum = 0.0;
+                double vxPrev = wrapPeriodic(pos[st].x - px);
+                double vyPrev = wrapPeriodic(pos[st].y - py);
+                double vx0 = vxPrev + 0.0, vy0 = vyPrev + 0.0;
+                for (int e = st + 1; e < en; e++) {
+                    double vx = vxPrev + wrapPeriodic(pos[e].x - pos[e - 1].x);
+                    double vy = vyPrev + wrapPeriodic(pos[e].y - pos[e - 1].y);
+                    double cross = vxPrev * vy - vyPrev * vx;
+                    double dot = vxPrev * vx + vyPrev * vy;
+                    angleSum += atan2(cross, dot);
+                    vxPrev = vx; vyPrev = vy;
+                }
+                angleSum += atan2(vxPrev * vy0 - vyPrev * vx0, vxPrev * vx0 + vyPrev * vy0);
+                if (fabs(angleSum - 2.0 * pi) < eps) num++;
+                if (nt < 512) tracked[nt++] = poly; else full = true;
+            }
+        }
+    }
+    (void)vbase;
+    int c = num * (num - 1) / 2;
+    if (counter) counter[idx] = c;
+    if (c) atomicAdd((unsigned long long*)sum, (unsigned long long)c);
+}
+
+// ordered neighbour lists for the getters (reference slot order: 3x3 stencil in
+// (dx, dy) order with de-duplication, ascending vertex id inside a cell)
+__global__ void k_neighbor_lists(int v0, int v1, int M, const int* __restrict__ shapeId,
+                                 const int* __restrict__ polyPack, const int* __restrict__ box,
+                                 const long long* __restrict__ cellOff, const int* __restrict__ cellOf,
+                                 const int* __restrict__ cellStart, const int* __restrict__ next,
+                                 const double2* __restrict__ SP, const double2* __restrict__ SR,
+                                 const int4* __restrict__ SI, const int* __restrict__ cellInv,
+                                 int* __restrict__ outN, unsigned char* __restrict__ outIn,
+                                 int* __restrict__ outNum) {
+    int id1 = v0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (id1 >= v1) return;
+    int p = polyPack[shapeId[id1]];
+    int B = box[p];
+    long long cbase = cellOff[p];
+    int id2 = next[id1];
+    int me = cellInv[id1];
+    double2 P1 = SP[me];
+    double rx = SR[me].x, ry = SR[me].y;
+    int cl = cellOf[id1];
+    int bx = cl % B, by = cl / B;
+    int count = 0;
+    long long row = (long long)(id1 - v0) * M;
+    for (int kc = 0; kc < 9; ++kc) {
+        int nx = bx + kc / 3 - 1;
+        if (nx < 0) nx += B; else if (nx >= B) nx -= B;
+        int ny = by + kc % 3 - 1;
+        if (ny < 0) ny += B; else if (ny >= B) ny -= B;
+        int nb = ny * B + nx;
+        bool dup = false;
+        for (int kp = 0; kp < kc; ++kp) {
+            int ex = bx + kp / 3 - 1;
+            if (ex < 0) ex += B; else if (ex >= B) ex -= B;
+            int ey = by + kp % 3 - 1;
+            if (ey < 0) ey += B; else if (ey >= B) ey -= B;
+            dup |= (ey * B + ex == nb);
+        }
+        if (dup) continue;
+        for (int q = cellStart[cbase + nb]; q < cellStart[cbase + nb + 1]; ++q) {
+            int4 ot = SI[q];
+            int nid = ot.x, nid2 = ot.y;
+            if (nid <= id1) continue;
+            if (nid == id1 || nid == id2 || nid2 == id1 || nid2 == id2) continue;
+            double2 Q = SP[q], S = SR[q];
+            double gx = wrap(Q.x - P1.x), gy = wrap(Q.y - P1.y);
+            double denom = rx * S.y - ry * S.x;
+            if (fabs(denom) < 1e-12) continue;
+            double tt = (gx * S.y - gy * S.x) / denom;
+            double uu = (gx * ry - gy * rx) / denom;
+            if (tt > 0.0 && tt < 1.0 && uu > 0.0 && uu < 1.0) {
+                if (count < M) {
+                    outN[row + count] = nid - (v0);
+                    outIn[row + count] = (denom > 0) ? 1 : 0;
+                }
+                count++;
+            }
+        }
+    }
+    outNum[id1 - v0] = count;
+}
+
+int prof_begin(pcp_ctx* c, const char* name, int* slot) {
+    *slot = -1;
+    if (c->profName.empty() || c->profName != name) return PCP_OK;
+    if (c->profUsed == c->profEvents.size()) {
+        hipEvent_t a, b;
+        PCP_HIP(hipEventCreate(&a));
+        PCP_HIP(hipEventCreate(&b));
+        c->profEvents.push_back({a, b});
+    }
+    *slot = (int)c->profUsed++;
+    PCP_HIP(hipEventRecord(c->profEvents[*slot].first, c->stream));
+    return PCP_OK;
+}
+int prof_end(pcp_ctx* c, int slot) {
+    if (slot < 0) return PCP_OK;
+    PCP_HIP(hipEventRecord(c->profEvents[slot].second, c->stream));
+    return PCP_OK;
+}
+
+__global__ void k_copy(long long n, double* __restrict__ dst, const double* __restrict__ src) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+}  // namespace pcp
+
+// =================================================================== C ABI
+extern "C" {
+
+const char* pcp_last_error(void) { return pcp::g_err.c_str(); }
+
+int pcp_device_count(int* count) {
+    int n = 0;
+    PCP_HIP(hipGetDeviceCount(&n));
+    *count = n;
+    return PCP_OK;
+}
+int pcp_malloc(void** ptr, size_t bytes) {
+    PCP_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+    return PCP_OK;
+}
+int pcp_free(void* ptr) {
+    if (ptr) PCP_HIP(hipFree(ptr));
+    return PCP_OK;
+}
+int pcp_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+    if (bytes) PCP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return PCP_OK;
+}
+int pcp_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+    if (bytes) PCP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return PCP_OK;
+}
+int pcp_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+    if (bytes) PCP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return PCP_OK;
+}
+int pcp_memset(void* ptr, int value, size_t bytes) {
+    if (bytes) PCP_HIP(hipMemset(ptr, value, bytes));
+    return PCP_OK;
+}
+
+static void free_topology(pcp_ctx* c) {
+    dfree(c->dPackPoly); dfree(c->dPackVert); dfree(c->dPolyStart); dfree(c->dShapeId);
+    dfree(c->dNext); dfree(c->dPrev); dfree(c->dPolyPack); dfree(c->dBox); dfree(c->dCellOff);
+    dfree(c->dCellOf); dfree(c->dCellCount); dfree(c->dCellStart); dfree(c->dCellTmp);
+    dfree(c->dCellSorted); dfree(c->dSP); dfree(c->dSR); dfree(c->dSI);
+    dfree(c->dSlotKey); dfree(c->dSlotTU); dfree(c->dSlotIn); dfree(c->dNum); dfree(c->dCounts);
+    dfree(c->dBStart); dfree(c->dPackMaxKey); dfree(c->dScal);
+    dfree(c->dKeys2); dfree(c->dBTU); dfree(c->dBPoly); dfree(c->dRKey); dfree(c->dROuter);
+    dfree(c->dRTU); dfree(c->dRUT); dfree(c->dRPoly); dfree(c->dSjList); dfree(c->dContrib);
+    dfree(c->dTgt); dfree(c->dEext); dfree(c->dSp);
+    dfree(c->dEint); dfree(c->dEedge); dfree(c->dEpoly); dfree(c->dWork); dfree(c->dRed);
+    dfree(c->dRedOut); dfree(c->dScanPart); c->scanPartCap = 0;
+    dfree(c->dXbak); dfree(c->dXpre); dfree(c->dVbak); dfree(c->dDt); dfree(c->dAlpha);
+    dfree(c->dEpre); dfree(c->dNPos); dfree(c->dSteps); dfree(c->dActive); dfree(c->dFlags);
+    dfree(c->dFireScal); dfree(c->dShakeIt); dfree(c->dCellInv); dfree(c->dCom); dfree(c->dStatus);
+    dfree(c->dSummary); dfree(c->dPartner); dfree(c->dClosed);
+    c->Icap = 0;
+    c->haveBoxes = false;
+}
+
+int pcp_create(pcp_ctx** out) {
+    pcp_ctx* c = new pcp_ctx();
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return pcp::fail_hip(e, "hipStreamCreate", __FILE__, __LINE__);
+    }
+    *out = c;
+    return PCP_OK;
+}
+int pcp_destroy(pcp_ctx* c) {
+    if (!c) return PCP_OK;
+    (void)hipStreamSynchronize(c->stream);
+    free_topology(c);
+    for (auto& e : c->profEvents) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return PCP_OK;
+}
+int pcp_synchronize(pcp_ctx* c) {
+    PCP_HIP(hipStreamSynchronize(c->stream));
+    return PCP_OK;
+}
+
+}  // extern "C"
+
+namespace pcp {
+int ensure_records(pcp_ctx* c, long long need) {
+    if (need <= c->Icap) return PCP_OK;
+    long long cap = std::max<long long>(need, 2 * c->Icap);
+    dfree(c->dKeys2); dfree(c->dBTU); dfree(c->dBPoly); dfree(c->dRKey); dfree(c->dROuter);
+    dfree(c->dRTU); dfree(c->dRUT); dfree(c->dRPoly); dfree(c->dSjList); dfree(c->dContrib);
+    dfree(c->dTgt); dfree(c->dEext); dfree(c->dSp);
+    PCP_TRY(dalloc(&c->dKeys2, 2 * cap)); PCP_TRY(dalloc(&c->dBTU, cap)); PCP_TRY(dalloc(&c->dBPoly, cap));
+    PCP_TRY(dalloc(&c->dRKey, cap)); PCP_TRY(dalloc(&c->dROuter, cap)); PCP_TRY(dalloc(&c->dRTU, cap));
+    PCP_TRY(dalloc(&c->dRUT, cap)); PCP_TRY(dalloc(&c->dRPoly, cap)); PCP_TRY(dalloc(&c->dSjList, cap));
+    PCP_TRY(dalloc(&c->dContrib, 8 * cap)); PCP_TRY(dalloc(&c->dTgt, 8 * cap));
+    PCP_TRY(dalloc(&c->dEext, cap)); PCP_TRY(dalloc(&c->dSp, cap));
+    dfree(c->dPartner); dfree(c->dClosed);
+    PCP_TRY(dalloc(&c->dPartner, cap)); PCP_TRY(dalloc(&c->dClosed, cap));
+    c->Icap = cap;
+    return PCP_OK;
+}
+int grow_slots(pcp_ctx* c, int K) {
+    dfree(c->dSlotKey); dfree(c->dSlotTU); dfree(c->dSlotIn);
+    long long n = (long long)c->V * K;
+    PCP_TRY(dalloc(&c->dSlotKey, n)); PCP_TRY(dalloc(&c->dSlotTU, n)); PCP_TRY(dalloc(&c->dSlotIn, n));
+    c->K = K;
+    return PCP_OK;
+}
+}  // namespace pcp
+
+extern "C" {
+
+int pcp_set_topology(pcp_ctx* c, int P, const int* packPolyStart, const int* polyStart) {
+    if (P < 1) return fail_arg("pcp_set_topology: need at least one packing");
+    int N = packPolyStart[P];
+    if (packPolyStart[0] != 0) return fail_arg("pcp_set_topology: packPolyStart[0] must be 0");
+    for (int p = 0; p < P; ++p) {
+        if (packPolyStart[p + 1] < packPolyStart[p])
+            return fail_arg("pcp_set_topology: packPolyStart must be non-decreasing");
+        if (packPolyStart[p + 1] - packPolyStart[p] > 65535)
+            return fail_arg("pcp_set_topology: more than 65535 polygons in one packing (16-bit record ids)");
+    }
+    if (polyStart[0] != 0) return fail_arg("pcp_set_topology: polyStart[0] must be 0");
+    for (int s = 0; s < N; ++s) {
+        int n = polyStart[s + 1] - polyStart[s];
+        if (n < 1) return fail_arg("pcp_set_topology: every polygon needs at least one vertex");
+        if (n > 65535) return fail_arg("pcp_set_topology: more than 65535 vertices in one polygon");
+    }
+    PCP_HIP(hipStreamSynchronize(c->stream));
+    free_topology(c);
+    int V = polyStart[N];
+    c->P = P; c->N = N; c->V = V;
+    c->hPackPoly.assign(packPolyStart, packPolyStart + P + 1);
+    c->hPolyStart.assign(polyStart, polyStart + N + 1);
+    c->hPackVert.resize(P + 1);
+    for (int p = 0; p <= P; ++p) c->hPackVert[p] = polyStart[packPolyStart[p]];
+    c->lastI = 0;
+    PCP_TRY(dalloc(&c->dPackPoly, P + 1)); PCP_TRY(dalloc(&c->dPackVert, P + 1));
+    PCP_TRY(dalloc(&c->dPolyStart, N + 1)); PCP_TRY(dalloc(&c->dShapeId, V));
+    PCP_TRY(dalloc(&c->dNext, V)); PCP_TRY(dalloc(&c->dPrev, V)); PCP_TRY(dalloc(&c->dPolyPack, N));
+    PCP_TRY(dalloc(&c->dBox, P)); PCP_TRY(dalloc(&c->dCellOff, P + 1));
+    PCP_HIP(hipMemcpy(c->dPackPoly, packPolyStart, sizeof(int) * (P + 1), hipMemcpyHostToDevice));
+    PCP_HIP(hipMemcpy(c->dPackVert, c->hPackVert.data(), sizeof(int) * (P + 1), hipMemcpyHostToDevice));
+    PCP_HIP(hipMemcpy(c->dPolyStart, polyStart, sizeof(int) * (N + 1), hipMemcpyHostToDevice));
+    int nt = std::max(V, N);
+    if (nt > 0) {
+        hipLaunchKernelGGL(k_topology, dim3(nblk(nt)), dim3(BLK), 0, c->stream, V, N, P, c->dPolyStart,
+                           c->dPackPoly, c->dShapeId, c->dNext, c->dPrev, c->dPolyPack);
+        PCP_LAUNCHED();
+    }
+    PCP_TRY(dalloc(&c->dCellOf, V)); PCP_TRY(dalloc(&c->dCellTmp, V)); PCP_TRY(dalloc(&c->dCellSorted, V));
+    PCP_TRY(dalloc(&c->dSP, V)); PCP_TRY(dalloc(&c->dSR, V)); PCP_TRY(dalloc(&c->dSI, V));
+    PCP_TRY(dalloc(&c->dNum, V)); PCP_TRY(dalloc(&c->dCounts, 2LL * N)); PCP_TRY(dalloc(&c->dBStart, 2LL * N + 1));
+    PCP_TRY(dalloc(&c->dPackMaxKey, P)); PCP_TRY(dalloc(&c->dScal, SC_NSCAL));
+    PCP_HIP(hipMemsetAsync(c->dCounts, 0, sizeof(int) * 2 * (size_t)std::max(N, 1), c->stream));
+    PCP_HIP(hipMemsetAsync(c->dBStart, 0, sizeof(int) * (2 * (size_t)N + 1), c->stream));
+    PCP_HIP(hipMemsetAsync(c->dScal, 0, sizeof(int) * SC_NSCAL, c->stream));
+    PCP_HIP(hipMemsetAsync(c->dNum, 0, sizeof(int) * (size_t)std::max(V, 1), c->stream));
+    PCP_TRY(grow_slots(c, c->K));
+    PCP_TRY(ensure_records(c, std::max<long long>(8LL * V, 65536)));
+    PCP_TRY(dalloc(&c->dEint, V)); PCP_TRY(dalloc(&c->dEedge, V)); PCP_TRY(dalloc(&c->dEpoly, N));
+    PCP_TRY(dalloc(&c->dWork, 8LL * V)); PCP_TRY(dalloc(&c->dRed, 4LL * N)); PCP_TRY(dalloc(&c->dRedOut, 4LL * P));
+    PCP_TRY(dalloc(&c->dXbak, 2LL * V)); PCP_TRY(dalloc(&c->dXpre, 2LL * V)); PCP_TRY(dalloc(&c->dVbak, 2LL * V));
+    PCP_TRY(dalloc(&c->dDt, P)); PCP_TRY(dalloc(&c->dAlpha, P)); PCP_TRY(dalloc(&c->dEpre, P));
+    PCP_TRY(dalloc(&c->dNPos, P)); PCP_TRY(dalloc(&c->dSteps, P)); PCP_TRY(dalloc(&c->dActive, P));
+    PCP_TRY(dalloc(&c->dFlags, 8LL * P)); PCP_TRY(dalloc(&c->dFireScal, 16LL * P));
+    PCP_TRY(dalloc(&c->dShakeIt, 2LL * P));
+    PCP_TRY(dalloc(&c->dCellInv, V)); PCP_TRY(dalloc(&c->dCom, 2LL * N)); PCP_TRY(dalloc(&c->dStatus, P));
+    PCP_TRY(dalloc(&c->dSummary, 4));
+    PCP_HIP(hipMemsetAsync(c->dFlags, 0, sizeof(int) * 8 * (size_t)P, c->stream));
+    PCP_HIP(hipMemsetAsync(c->dActive, 0, sizeof(int) * (size_t)P, c->stream));
+    PCP_HIP(hipStreamSynchronize(c->stream));
+    return PCP_OK;
+}
+
+int pcp_set_box_sizes(pcp_ctx* c, const int* box) {
+    if (c->P == 0) return fail_arg("pcp_set_box_sizes: set the topology first");
+    for (int p = 0; p < c->P; ++p)
+        if (box[p] < 1) return fail_arg("pcp_set_box_sizes: box size must be >= 1");
+    PCP_HIP(hipStreamSynchronize(c->stream));
+    c->hBox.assign(box, box + c->P);
+    c->hCellOff.assign(c->P + 1, 0);
+    for (int p = 0; p < c->P; ++p) c->hCellOff[p + 1] = c->hCellOff[p] + (long long)box[p] * box[p];
+    long long Ctot = c->hCellOff[c->P];
+    if (Ctot > 2000000000LL) return fail_arg("pcp_set_box_sizes: too many cells");
+    dfree(c->dCellCount); dfree(c->dCellStart);
+    c->Ctot = Ctot;
+    PCP_TRY(dalloc(&c->dCellCount, Ctot)); PCP_TRY(dalloc(&c->dCellStart, Ctot + 1));
+    PCP_HIP(hipMemsetAsync(c->dCellCount, 0, sizeof(int) * (size_t)Ctot, c->stream));
+    PCP_HIP(hipMemcpy(c->dBox, box, sizeof(int) * c->P, hipMemcpyHostToDevice));
+    PCP_HIP(hipMemcpy(c->dCellOff, c->hCellOff.data(), sizeof(long long) * (c->P + 1), hipMemcpyHostToDevice));
+    PCP_HIP(hipStreamSynchronize(c->stream));
+    c->haveBoxes = true;
+    return PCP_OK;
+}
+int pcp_set_max_neighbors(pcp_ctx* c, int M) {
+    if (M < 1) return fail_arg("maxNeighbors must be >= 1");
+    c->M = M;
+    return PCP_OK;
+}
+int pcp_get_max_neighbors(pcp_ctx* c, int* M) {
+    *M = c->M;
+    return PCP_OK;
+}
+
+int pcp_update_geometry(pcp_ctx* c, const double* pos, double* edgeLengths, double* areas, double* comX,
+                        double* comY, double* constraints, double* maxEdgeLength) {
+    return run_geometry(c, (const double2*)pos, edgeLengths, areas, comX, comY, constraints, maxEdgeLength);
+}
+int pcp_update_cells(pcp_ctx* c, const double* pos) {
+    if (!c->haveBoxes) return fail_arg("pcp_update_cells: set the box sizes first");
+    return run_cells(c, (const double2*)pos);
+}
+
+// detect; if any edge has more crossings than its slots, grow the slots and repeat
+static int detect_checked(pcp_ctx* c, const double2* pos, int* maxCount) {
+    PCP_TRY(run_detect(c, pos));
+    int mx = 0;
+    PCP_HIP(hipMemcpyAsync(&mx, c->dScal + SC_MAXCOUNT, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    PCP_HIP(hipStreamSynchronize(c->stream));
+    if (mx > c->K) {
+        PCP_TRY(grow_slots(c, std::max(mx, 2 * c->K)));
+        PCP_TRY(run_detect(c, pos));
+    }
+    if (maxCount) *maxCount = mx;
+    return PCP_OK;
+}
+int pcp_update_neighbors(pcp_ctx* c, const double* pos, int* maxCount) {
+    if (!c->haveBoxes) return fail_arg("pcp_update_neighbors: set the box sizes first");
+    if (c->V > 0) {
+        hipLaunchKernelGGL(k_restage, dim3(nblk(c->V)), dim3(BLK), 0, c->stream, c->V, (const double2*)pos,
+                           c->dNext, c->dCellSorted, c->dSP, c->dSR);
+        PCP_LAUNCHED();
+    }
+    if (maxCount) return detect_checked(c, (const double2*)pos, maxCount);
+    return run_detect(c, (const double2*)pos);
+}
+
+// records; grow the record arrays when the crossings exceed their capacity
+static int records_checked(pcp_ctx* c, const double2* pairPos) {
+    PCP_TRY(run_records(c, pairPos));
+    int I = 0;
+    PCP_HIP(hipMemcpyAsync(&I, c->dBStart + c->N, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    PCP_HIP(hipStreamSynchronize(c->stream));
+    c->lastI = I;
+    if (I > c->Icap) {   // grow and rebuild (the counts are recomputed from the slots)
+        PCP_TRY(ensure_records(c, (long long)I + I / 4));
+        PCP_TRY(run_records(c, pairPos));
+    }
+    return PCP_OK;
+}
+int pcp_update_records(pcp_ctx* c, const double* pairPos, int* numIntersections) {
+    if (numIntersections) {
+        PCP_TRY(records_checked(c, (const double2*)pairPos));
+        std::vector<int> bs(c->N + 1);
+        PCP_HIP(hipMemcpy(bs.data(), c->dBStart, sizeof(int) * (c->N + 1), hipMemcpyDeviceToHost));
+        for (int p = 0; p < c->P; ++p) numIntersections[p] = bs[c->hPackPoly[p + 1]] - bs[c->hPackPoly[p]];
+        return PCP_OK;
+    }
+    return run_records(c, (const double2*)pairPos);
+}
+
+int pcp_update_force_energy(pcp_ctx* c, int model, const double* pos, const double* edgeLengths,
+                            const double* areas, const double* targetEdgeV, const double* targetAreas,
+                            double stiffness, double compressibility, double* force, double* energy) {
+    return run_forces(c, model, (const double2*)pos, edgeLengths, areas, targetEdgeV, targetAreas, stiffness,
+                      compressibility, (double2*)force, energy);
+}
+int pcp_project_force(pcp_ctx* c, const double* pos, double* force) {
+    return run_project(c, (const double2*)pos, (double2*)force);
+}
+int pcp_shake(pcp_ctx* c, double* pos, const double* targetEdgeV, const double* targetAreas, int maxIter,
+              double tol, int* iters) {
+    if (c->N == 0 || maxIter == 0) {
+        if (iters) for (int p = 0; p < c->P; ++p) iters[p] = 0;
+        return PCP_OK;
+    }
+    hipLaunchKernelGGL(k_fill_int, dim3(nblk(2LL * c->P)), dim3(BLK), 0, c->stream, c->dShakeIt,
+                       2LL * c->P, 0);
+    PCP_LAUNCHED();
+    PCP_TRY(run_shake(c, (double2*)pos, targetEdgeV, targetAreas, maxIter, tol, 0, nullptr, c->dShakeIt));
+    if (iters) {
+        PCP_HIP(hipMemcpyAsync(iters, c->dShakeIt, sizeof(int) * c->P, hipMemcpyDeviceToHost, c->stream));
+        PCP_HIP(hipStreamSynchronize(c->stream));
+    }
+    return PCP_OK;
+}
+int pcp_xpbd(pcp_ctx* c, double* pos, const double* targetEdgeP, const double* targetAreas, int nIter,
+             double tol, int* iters) {
+    int it = 0;
+    std::vector<double> dev(2 * (size_t)c->N);
+    for (; it < nIter; ++it) {
+        hipLaunchKernelGGL(k_xpbd_iter, dim3(nblk(c->N)), dim3(BLK), 0, c->stream, c->N, c->dPolyStart,
+                           (double2*)pos, targetEdgeP, targetAreas, c->dRed, c->dWork, c->V);
+        PCP_LAUNCHED();
+        if (tol > 0) {
+            PCP_HIP(hipMemcpyAsync(dev.data(), c->dRed, sizeof(double) * 2 * c->N, hipMemcpyDeviceToHost,
+                                   c->stream));
+            PCP_HIP(hipStreamSynchronize(c->stream));
+            double mE = 0.0, mA = 0.0;
+            for (int s = 0; s < c->N; ++s) { mE = std::max(mE, dev[s]); mA = std::max(mA, dev[c->N + s]); }
+            if (mE < tol && mA < tol) break;
+        }
+    }
+    if (iters) *iters = it;
+    PCP_HIP(hipStreamSynchronize(c->stream));
+    return PCP_OK;
+}
+int pcp_max_abs(pcp_ctx* c, const double* x, double* out) {
+    if (c->N == 0) return PCP_OK;
+    hipLaunchKernelGGL(k_max_abs_poly, dim3(nblk(c->N)), dim3(BLK), 0, c->stream, c->N, c->dPolyStart,
+                       (const double2*)x, c->dRed);
+    PCP_LAUNCHED();
+    return launch_det_reduce(c, c->dRed, 1, out, 1);
+}
+int pcp_update_positions(pcp_ctx* c, double* pos, const double* force, double dt) {
+    if (c->V == 0) return PCP_OK;
+    hipLaunchKernelGGL(k_update_positions, dim3(nblk(2LL * c->V)), dim3(BLK), 0, c->stream, 2LL * c->V, pos,
+                       force, dt);
+    PCP_LAUNCHED();
+    return PCP_OK;
+}
+int pcp_reset_areas(pcp_ctx* c, double* pos, const double* areas, const double* targetAreas,
+                    const double* comX, const double* comY) {
+    if (c->V == 0) return PCP_OK;
+    hipLaunchKernelGGL(k_reset_areas, dim3(nblk(2LL * c->V)), dim3(BLK), 0, c->stream, 2LL * c->V,
+                       c->dShapeId, pos, areas, targetAreas, comX, comY);
+    PCP_LAUNCHED();
+    return PCP_OK;
+}
+int pcp_max_effective_force(pcp_ctx* c, const double* pos, const double* tentPos, const double* force,
+                            double scale, double* out) {
+    if (c->N == 0) return PCP_OK;
+    hipLaunchKernelGGL(k_eff_force_poly, dim3(nblk(c->N)), dim3(BLK), 0, c->stream, c->N, c->dPolyStart,
+                       (const double2*)pos, (const double2*)tentPos, (const double2*)force, scale, c->dRed);
+    PCP_LAUNCHED();
+    PCP_TRY(launch_det_reduce(c, c->dRed, 1, c->dRedOut, 1));
+    PCP_HIP(hipMemcpyAsync(out, c->dRedOut, sizeof(double) * c->P, hipMemcpyDeviceToHost, c->stream));
+    PCP_HIP(hipStreamSynchronize(c->stream));
+    return PCP_OK;
+}
+int pcp_overlap_count(pcp_ctx* c, const double* pos, int D, int* counter, long long* sum) {
+    if (!c->haveBoxes) return fail_arg("pcp_overlap_count: set the box sizes first");
+    long long* dsum = nullptr;
+    PCP_TRY(dalloc(&dsum, c->P));
+    PCP_HIP(hipMemsetAsync(dsum, 0, sizeof(long long) * c->P, c->stream));
+    for (int p = 0; p < c->P; ++p) {
+        int total = D * D;
+        if (total == 0) continue;
+        hipLaunchKernelGGL(k_overlap_count, dim3(nblk(total)), dim3(BLK), 0, c->stream, D, c->hBox[p],
+                           c->hPackVert[p + 1] - c->hPackVert[p], (const int*)nullptr, c->hPackVert[p],
+                           c->dShapeId, c->dPolyStart, c->dCellStart + c->hCellOff[p], c->dCellSorted,
+                           (const double2*)pos, counter ? counter + (long long)p * total : nullptr, dsum + p);
+        PCP_LAUNCHED();
+    }
+    PCP_HIP(hipMemcpyAsync(sum, dsum, sizeof(long long) * c->P, hipMemcpyDeviceToHost, c->stream));
+    PCP_HIP(hipStreamSynchronize(c->stream));
+    dfree(dsum);
+    return PCP_OK;
+}
+
+// ---- getters
+int pcp_get_cells(pcp_ctx* c, int p, int* cellLocation, int* neighborIndices, int* countPerBox) {
+    if (p < 0 || p >= c->P) return fail_arg("packing out of range");
+    PCP_HIP(hipStreamSynchronize(c->stream));
+    int v0 = c->hPackVert[p], v1 = c->hPackVert[p + 1];
+    if (cellLocation) PCP_HIP(hipMemcpy(cellLocation, c->dCellOf + v0, sizeof(int) * (v1 - v0), hipMemcpyDeviceToHost));
+    if (neighborIndices) {
+        PCP_HIP(hipMemcpy(neighborIndices, c->dCellSorted + v0, sizeof(int) * (v1 - v0), hipMemcpyDeviceToHost));
+        for (int i = 0; i < v1 - v0; ++i) neighborIndices[i] -= v0;
+    }
+    if (countPerBox) {
+        long long nb = (long long)c->hBox[p] * c->hBox[p];
+        PCP_HIP(hipMemcpy(countPerBox, c->dCellStart + c->hCellOff[p], sizeof(int) * nb, hipMemcpyDeviceToHost));
+        for (long long i = 0; i < nb; ++i) countPerBox[i] -= v0;
+    }
+    return PCP_OK;
+}
+int pcp_get_num_intersections(pcp_ctx* c, int p, int* count) {
+    if (p < 0 || p >= c->P) return fail_arg("packing out of range");
+    PCP_HIP(hipStreamSynchronize(c->stream));
+    int a = 0, b = 0;
+    PCP_HIP(hipMemcpy(&a, c->dBStart + c->hPackPoly[p], sizeof(int), hipMemcpyDeviceToHost));
+    PCP_HIP(hipMemcpy(&b, c->dBStart + c->hPackPoly[p + 1], sizeof(int), hipMemcpyDeviceToHost));
+    *count = b - a;
+    return PCP_OK;
+}
+int pcp_get_records(pcp_ctx* c, int p, uint64_t* inter, double* tu, uint64_t* outer, double* ut,
+                    int* shapeCounts) {
+    if (p < 0 || p >= c->P) return fail_arg("packing out of range");
+    PCP_HIP(hipStreamSynchronize(c->stream));
+    int n0 = c->hPackPoly[p], n1 = c->hPackPoly[p + 1];
+    std::vector<int> bs(n1 - n0 + 1);
+    PCP_HIP(hipMemcpy(bs.data(), c->dBStart + n0, sizeof(int) * (n1 - n0 + 1), hipMemcpyDeviceToHost));
+    int a = bs[0], I = bs[n1 - n0] - a;
+    if (shapeCounts) for (int s = 0; s < n1 - n0; ++s) shapeCounts[s] = bs[s + 1] - bs[s];
+    if (I == 0) return PCP_OK;
+    if (inter) PCP_HIP(hipMemcpy(inter, c->dRKey + a, sizeof(uint64_t) * I, hipMemcpyDeviceToHost));
+    if (outer) PCP_HIP(hipMemcpy(outer, c->dROuter + a, sizeof(uint64_t) * I, hipMemcpyDeviceToHost));
+    std::vector<float2> tmp(I);
+    if (tu) {
+        PCP_HIP(hipMemcpy(tmp.data(), c->dRTU + a, sizeof(float2) * I, hipMemcpyDeviceToHost));
+        for (int r = 0; r < I; ++r) { tu[2 * r] = tmp[r].x; tu[2 * r + 1] = tmp[r].y; }
+    }
+    if (ut) {
+        PCP_HIP(hipMemcpy(tmp.data(), c->dRUT + a, sizeof(float2) * I, hipMemcpyDeviceToHost));
+        for (int r = 0; r < I; ++r) { ut[2 * r] = tmp[r].x; ut[2 * r + 1] = tmp[r].y; }
+    }
+    return PCP_OK;
+}
+int pcp_get_neighbor_lists(pcp_ctx* c, int p, int* neighbors, unsigned char* inside, int* numNeighbors) {
+    if (p < 0 || p >= c->P) return fail_arg("packing out of range");
+    int v0 = c->hPackVert[p], v1 = c->hPackVert[p + 1];
+    long long n = (long long)(v1 - v0) * c->M;
+    int* dN = nullptr; unsigned char* dI = nullptr; int* dC = nullptr;
+    PCP_TRY(dalloc(&dN, n)); PCP_TRY(dalloc(&dI, n)); PCP_TRY(dalloc(&dC, v1 - v0));
+    PCP_HIP(hipMemsetAsync(dN, 0xFF, sizeof(int) * n, c->stream));
+    PCP_HIP(hipMemsetAsync(dI, 0, n, c->stream));
+    if (v1 > v0) {
+        hipLaunchKernelGGL(k_neighbor_lists, dim3(nblk(v1 - v0)), dim3(BLK), 0, c->stream, v0, v1, c->M,
+                           c->dShapeId, c->dPolyPack, c->dBox, c->dCellOff, c->dCellOf, c->dCellStart,
+                           c->dNext, c->dSP, c->dSR, c->dSI, c->dCellInv, dN, dI, dC);
+        PCP_LAUNCHED();
+    }
+    PCP_HIP(hipStreamSynchronize(c->stream));
+    if (neighbors) PCP_HIP(hipMemcpy(neighbors, dN, sizeof(int) * n, hipMemcpyDeviceToHost));
+    if (inside) PCP_HIP(hipMemcpy(inside, dI, n, hipMemcpyDeviceToHost));
+    if (numNeighbors) PCP_HIP(hipMemcpy(numNeighbors, dC, sizeof(int) * (v1 - v0), hipMemcpyDeviceToHost));
+    dfree(dN); dfree(dI); dfree(dC);
+    return PCP_OK;
+}
+int pcp_get_compacted_records(pcp_ctx* c, int p, uint64_t* inter, double* tu) {
+    if (p < 0 || p >= c->P) return fail_arg("packing out of range");
+    PCP_HIP(hipStreamSynchronize(c->stream));
+    int v0 = c->hPackVert[p], v1 = c->hPackVert[p + 1];
+    std::vector<int> num(v1 - v0);
+    PCP_HIP(hipMemcpy(num.data(), c->dNum + v0, sizeof(int) * (v1 - v0), hipMemcpyDeviceToHost));
+    // the detection stores each edge's crossings in the reference's slot order; only
+    // that order needs to be read out, edge by edge
+    std::vector<uint64_t> keys;
+    std::vector<float2> tus;
+    long long w = 0;
+    for (int v = v0; v < v1; ++v) {
+        int cnt = std::min(num[v - v0], c->K);
+        if (cnt == 0) continue;
+        keys.resize(cnt); tus.resize(cnt);
+        PCP_HIP(hipMemcpy(keys.data(), c->dSlotKey + (long long)v * c->K, sizeof(uint64_t) * cnt, hipMemcpyDeviceToHost));
+        PCP_HIP(hipMemcpy(tus.data(), c->dSlotTU + (long long)v * c->K, sizeof(float2) * cnt, hipMemcpyDeviceToHost));
+        for (int q = 0; q < cnt; ++q) {
+            if (inter) inter[w] = keys[q];
+            if (tu) { tu[2 * w] = tus[q].x; tu[2 * w + 1] = tus[q].y; }
+            ++w;
+        }
+    }
+    return PCP_OK;
+}
+
+int pcp_profile_kernel(pcp_ctx* c, const char* name) {
+    PCP_HIP(hipStreamSynchronize(c->stream));
+    c->profName = name ? name : "";
+    c->profUsed = 0;
+    return PCP_OK;
+}
+int pcp_profile_read(pcp_ctx* c, double* totalMs, int* launches) {
+    PCP_HIP(hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    for (size_t k = 0; k < c->profUsed; ++k) {
+        float ms = 0.f;
+        PCP_HIP(hipEventElapsedTime(&ms, c->profEvents[k].first, c->profEvents[k].second));
+        tot += ms;
+    }
+    *totalMs = tot;
+    *launches = (int)c->profUsed;
+    return PCP_OK;
+}
+int pcp_stream_copy(pcp_ctx* c, double* dst, const double* src, long long n) {
+    int g = (int)std::min<long long>(nblk(n), 8192);
+    PCP_LAUNCH(c, k_copy, dim3(std::max(g, 1)), dim3(BLK), n, dst, src);
+    return PCP_OK;
+}
+
+int pcp_sort_keys(const uint64_t* keys, int n, int endBit, uint64_t* sorted, uint32_t* perm) {
+    if (n <= 0) return PCP_OK;
+    uint64_t mask = (endBit >= 64) ? ~0ull : ((endBit <= 0) ? 0ull : ((1ull << endBit) - 1));
+    uint64_t *din = nullptr, *dout = nullptr;
+    uint32_t* dp = nullptr;
+    PCP_TRY(dalloc(&din, n)); PCP_TRY(dalloc(&dout, n)); PCP_TRY(dalloc(&dp, n));
+    PCP_HIP(hipMemcpy(din, keys, sizeof(uint64_t) * n, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_sort_keys, dim3(nblk(n)), dim3(BLK), 0, 0, n, mask, din, dout, dp);
+    PCP_LAUNCHED();
+    PCP_HIP(hipMemcpy(sorted, dout, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+    PCP_HIP(hipMemcpy(perm, dp, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    dfree(din); dfree(dout); dfree(dp);
+    return PCP_OK;
+}
+
+int pcp_force_eval(pcp_ctx* c, const pcp_state* st) {
+    const double2* pos = (const double2*)st->pos;
+    PCP_TRY(run_geometry(c, pos, st->edgeLengths, st->areas, c->dCom, c->dCom + c->N, nullptr, nullptr));
+    int mt = st->model & 0xFF;
+    int overlap = (mt == PCP_NORMAL || mt == PCP_HYBRID);
+    if (overlap) {
+        PCP_TRY(run_cells(c, pos));
+        PCP_TRY(run_detect(c, pos));
+        PCP_TRY(run_records(c, (st->model & PCP_AREA_ENERGY) ? pos : nullptr));
+    }
+    PCP_TRY(run_forces(c, st->model, pos, st->edgeLengths, st->areas, st->targetEdgeV, st->targetAreas,
+                       st->stiffness, st->compressibility, (double2*)st->force, st->energy));
+    if (mt == PCP_NORMAL) PCP_TRY(run_project(c, pos, (double2*)st->force));
+    return PCP_OK;
+}
+
+}  // extern "C"
+
+#include "pcp_fire.hip"
