@@ -115,9 +115,10 @@ vector<uint32_t> Model::getKeys() const { return keys; }
 void Model::setNumVertices(int n) {
     size = n;
     freeVertexBuffers();
-    dalloc(&positions, 2 * (size_t)n); dalloc(&force, 2 * (size_t)n); dalloc(&velocities, 2 * (size_t)n);
-    dalloc(&positionsTMP, 2 * (size_t)n); dalloc(&edgeLengths, n); dalloc(&constraints, 6 * (size_t)n);
-    dalloc(&targetEdgeV, n); dalloc(&scratch, 2 * (size_t)n);
+    size_t m = (size_t)std::max(n, 1);
+    dalloc(&positions, 2 * m); dalloc(&force, 2 * m); dalloc(&velocities, 2 * m);
+    dalloc(&positionsTMP, 2 * m); dalloc(&edgeLengths, m); dalloc(&constraints, 6 * m);
+    dalloc(&targetEdgeV, m); dalloc(&scratch, 2 * m);
     check(pcp_memset(force, 0, 16 * (size_t)std::max(n, 1)));
     check(pcp_memset(velocities, 0, 16 * (size_t)std::max(n, 1)));
     check(pcp_memset(targetEdgeV, 0, 8 * (size_t)std::max(n, 1)));
@@ -146,8 +147,9 @@ void Model::setStartIndices(const vector<int>& s) {
     numPolygons = (int)s.size() - 1;
     hStart = s;
     freePolygonBuffers();
-    dalloc(&areas, numPolygons); dalloc(&comX, numPolygons); dalloc(&comY, numPolygons);
-    dalloc(&targetAreas, numPolygons);
+    size_t m = (size_t)std::max(numPolygons, 1);
+    dalloc(&areas, m); dalloc(&comX, m); dalloc(&comY, m);
+    dalloc(&targetAreas, m);
     check(pcp_memset(targetAreas, 0, 8 * (size_t)std::max(numPolygons, 1)));
     check(pcp_memset(areas, 0, 8 * (size_t)std::max(numPolygons, 1)));
     hTargetEdgeP.assign(numPolygons, 0.0);
