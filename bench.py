@@ -29,25 +29,16 @@ METRIC = "packings minimized/sec to |F|max<1e-10 (learned vs plain); force evals
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
-# FP64 operations of one segment test in k_detect (2 minimum-image wraps of 5 ops,
+# FP64 operations of one segment test in k_detect_poly (2 minimum-image wraps of 5 ops,
 # cross product 3, |denom| test 1, two parameters 4+4, four range compares 4)
-FLOPS_PER_PAIR_TEST = 21
-# bytes k_detect must move from HBM per launch: staged edge data of every edge
-# (double2 position, double2 direction, int4 ids = 48 B), its count (4 B), and per
-# crossing one slot (key 8 B, float2 t/u 8 B, inside flag 1 B)
-BYTES_PER_EDGE = 52
+FLOPS_PER_EDGE_PAIR = 21
+# bytes the detection must move from HBM per launch: every vertex position once (16 B)
+# and its crossing counter (4 B), every polygon's bounding circle and sorted id (36 B),
+# and per crossing one slot (key 8 B, float2 t/u 8 B, inside flag 1 B)
+BYTES_PER_VERTEX = 20
+BYTES_PER_POLYGON = 36
 BYTES_PER_CROSSING = 17
-
-
-def pair_tests(cell_counts, B):
-    """Segment tests with nid > id1 over the 3x3 stencil (cell-count estimate: each
-    unordered pair within the stencil once)."""
-    n = cell_counts.reshape(B, B).astype(np.int64)
-    S = np.zeros_like(n)
-    shifts = sorted({(dx % B, dy % B) for dx in (-1, 0, 1) for dy in (-1, 0, 1)})
-    for dx, dy in shifts:
-        S += np.roll(np.roll(n, dx, axis=0), dy, axis=1)
-    return int((n * S).sum() - n.sum()) // 2
+DETECT_KERNEL = "k_detect_poly<0>"
 
 
 def cpu_baseline(p, B, budget_s):
@@ -74,7 +65,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cells", choices=["diameter", "edge"], default="diameter")
-    ap.add_argument("--roofline-kernel", default="k_detect")
+    ap.add_argument("--roofline-kernel", default=DETECT_KERNEL)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fire-seconds", type=float, default=20.0)
@@ -97,9 +88,9 @@ def main():
         s.force_eval()
     s.sync()
     nI = s.get_records(0)["numIntersections"]
-    cl, ni, cpb = s.get_cells(0)
-    counts = np.bincount(cl, minlength=B * B)
-    tests = pair_tests(counts, B)
+    ppairs, epairs = ctypes.c_longlong(0), ctypes.c_longlong(0)
+    s.ctx.call("pcp_detect_stats", ctypes.byref(ppairs), ctypes.byref(epairs))
+    tests = epairs.value
 
     s.ctx.call("pcp_profile_kernel", args.roofline_kernel.encode())
     s.sync()
@@ -115,16 +106,18 @@ def main():
     evals_per_s = args.steps / el
     kavg_s = (ms.value / max(nl.value, 1)) * 1e-3
     print(f"[bench] {args.steps} evals in {el:.3f} s; {args.roofline_kernel}: {nl.value} launches, "
-          f"avg {kavg_s * 1e6:.2f} us; records {nI}; pair tests {tests}", flush=True)
+          f"avg {kavg_s * 1e6:.2f} us; records {nI}; polygon pairs {ppairs.value}; edge-pair tests {tests}",
+          flush=True)
 
-    if args.roofline_kernel == "k_detect":
-        flops = FLOPS_PER_PAIR_TEST * tests
-        nbytes = BYTES_PER_EDGE * s.V + BYTES_PER_CROSSING * nI
+    if args.roofline_kernel == DETECT_KERNEL and nl.value > 0:
+        flops = FLOPS_PER_EDGE_PAIR * tests
+        nbytes = BYTES_PER_VERTEX * s.V + BYTES_PER_POLYGON * s.N + BYTES_PER_CROSSING * nI
         achieved = flops / kavg_s / 1e12
-        roof = dict(bound="mfma", compute="fp64 VALU (no MFMA in the geometry)", achieved=achieved,
-                    peak=FP64_PEAK_TFLOPS, unit="TFLOP/s", frac=achieved / FP64_PEAK_TFLOPS, traffic=None,
-                    kernel="k_detect", flops_per_launch=flops, algorithmic_bytes_per_launch=nbytes,
-                    avg_launch_us=kavg_s * 1e6, hbm_gbs_algorithmic=nbytes / kavg_s / 1e9)
+        roof = dict(bound="mfma", compute="fp64 VALU (no MFMA in the geometry); peak = FP64 78.6 TFLOP/s",
+                    achieved=achieved, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s", frac=achieved / FP64_PEAK_TFLOPS,
+                    traffic=None, kernel=DETECT_KERNEL, flops_per_launch=flops, edge_pair_tests=tests,
+                    polygon_pairs=ppairs.value, algorithmic_bytes_per_launch=nbytes, avg_launch_us=kavg_s * 1e6,
+                    hbm_gbs_algorithmic=nbytes / kavg_s / 1e9)
     else:
         roof = dict(kernel=args.roofline_kernel, avg_launch_us=kavg_s * 1e6, achieved=None, peak=None,
                     unit=None, frac=None, traffic=None, bound=None)

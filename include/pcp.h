@@ -207,12 +207,15 @@ int pcp_fire_get(pcp_ctx* ctx, pcp_fire_status* status);
 
 /* ---- measurement support (no counterpart in the reference) ----
  * pcp_profile_kernel: bracket every later launch of the kernel called `name` (e.g.
- * "k_detect") with HIP events on the context's stream; NULL stops.  pcp_profile_read:
+ * "k_detect_poly<0>") with HIP events on the context's stream; NULL stops.  pcp_profile_read:
  * total milliseconds and launch count since the last pcp_profile_kernel call.
  * pcp_stream_copy: plain device copy kernel of n doubles (HBM counter calibration). */
 int pcp_profile_kernel(pcp_ctx* ctx, const char* name);
 int pcp_profile_read(pcp_ctx* ctx, double* totalMs, int* launches);
 int pcp_stream_copy(pcp_ctx* ctx, double* dst, const double* src, long long n);
+/* work of the last detection: candidate polygon pairs and the edge pairs they hold
+ * (nA*nB per pair of distinct polygons, n(n-3)/2 per polygon with itself) */
+int pcp_detect_stats(pcp_ctx* ctx, long long* polygonPairs, long long* edgePairs);
 
 #ifdef __cplusplus
 }

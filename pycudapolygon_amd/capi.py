@@ -82,6 +82,7 @@ SIGNATURES = {
     "pcp_profile_kernel": (_i, [_p, ctypes.c_char_p]),
     "pcp_profile_read": (_i, [_p, _p, _p]),
     "pcp_stream_copy": (_i, [_p, _p, _p, ctypes.c_longlong]),
+    "pcp_detect_stats": (_i, [_p, _p, _p]),
 }
 
 _lib = None
@@ -317,6 +318,17 @@ class System:
         self.ctx.call("pcp_get_records", int(p), _ptr(inter), _ptr(tu), _ptr(outer), _ptr(ut), _ptr(sc))
         return dict(intersections=inter[:I], tu=tu[:2 * I], outersections=outer[:I], ut=ut[:2 * I],
                     shapeCounts=sc[:nP], numIntersections=I)
+
+    def get_compacted_records(self, p):
+        """Crossings in compaction order (edge by edge, each edge's crossings in the
+        order of the reference's stencil scan), before the record sort."""
+        n = ctypes.c_int(0)
+        self.ctx.call("pcp_get_num_intersections", int(p), ctypes.byref(n))
+        I = n.value
+        inter = np.zeros(max(I, 1), np.uint64)
+        tu = np.zeros(max(2 * I, 2))
+        self.ctx.call("pcp_get_compacted_records", int(p), _ptr(inter), _ptr(tu))
+        return dict(intersections=inter[:I], tu=tu[:2 * I])
 
     def get_neighbor_lists(self, p):
         a, b = self.vertex_range(p)

@@ -285,10 +285,8 @@ static int fire_eval(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm
     bool overlap = (mt == PCP_NORMAL || mt == PCP_HYBRID);
     bool rigid = (mt == PCP_NORMAL);
     if (overlap) {
-        if (!area) {   // reference order (model.cpp:416-440): detect, SHAKE, records
-            PCP_TRY(run_cells(c, x));
-            PCP_TRY(run_detect(c, x));
-        }
+        if (!area)   // reference order (model.cpp:416-440): detect, SHAKE, records
+            PCP_TRY(run_detect(c, x, 0));
         if (rigid && prm->shakeIter > 0) {
             PCP_TRY(run_shake(c, x, st->targetEdgeV, st->targetAreas, prm->shakeIter, 1e-15,
                               10 * prm->shakeIter, c->dActive, c->dShakeIt));
@@ -297,10 +295,8 @@ static int fire_eval(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm
                                c->dShakeIt, x, v, (const double2*)c->dXbak, (const double2*)c->dXpre,
                                c->dFlags);
         }
-        if (area) {   // area energy: crossings of the final positions
-            PCP_TRY(run_cells(c, x));
-            PCP_TRY(run_detect(c, x));
-        }
+        if (area)   // area energy: crossings of the final positions
+            PCP_TRY(run_detect(c, x, 0));
         PCP_TRY(run_records(c, area ? x : nullptr));
     }
     PCP_TRY(run_geometry(c, x, st->edgeLengths, st->areas, c->dCom, c->dCom + c->N, nullptr, nullptr));

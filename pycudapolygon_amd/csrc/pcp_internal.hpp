@@ -71,7 +71,14 @@ struct pcp_ctx {
     double2 *dSP = nullptr, *dSR = nullptr;
     int4* dSI = nullptr;
 
-    // detection
+    // detection: polygon bounding circles and their per-packing cell sort
+    int seg = 16;                  // lanes per polygon in the segment kernels
+    int maxN = 1;                  // largest polygon (vertices)
+    double4* dBounds = nullptr;    // N: centre x, y, radius, big flag
+    int* dPcSorted = nullptr;      // N: polygons sorted by cell, per packing
+    int* dPcStart = nullptr;       // P x (PC_CELLS + 1): cell starts of each packing
+    int* dPackB = nullptr;         // P: polygon grid size | big << 30
+    int* dSticky = nullptr;        // [0] a fused evaluation overflowed a buffer
     uint64_t* dSlotKey = nullptr;
     float2* dSlotTU = nullptr;
     unsigned char* dSlotIn = nullptr;
@@ -124,7 +131,13 @@ int ensure_records(pcp_ctx* c, long long need);
 int scan_exclusive(pcp_ctx* c, const int* in, int n, int* out);
 int launch_det_reduce(pcp_ctx* c, const double* partials, int nArrays, double* outSums, int nMax);
 int run_cells(pcp_ctx* c, const double2* pos);
-int run_detect(pcp_ctx* c, const double2* pos);
+int run_detect(pcp_ctx* c, const double2* pos, int cellsFromList);
+int detect_alloc(pcp_ctx* c);
+int detect_stats(pcp_ctx* c, long long* polygonPairs, long long* edgePairs);
+int seg_width(int maxN);
+int run_project_seg(pcp_ctx* c, const double2* pos, double2* force);
+int run_shake_seg(pcp_ctx* c, double2* pos, const double* targetEdgeV, const double* targetAreas, int maxIter,
+                  double tol, int retryIter, const int* active, int* itOut);
 int run_records(pcp_ctx* c, const double2* pairPos);
 int run_forces(pcp_ctx* c, int model, const double2* pos, const double* edgeLengths,
                const double* areas, const double* targetEdgeV, const double* targetAreas,

@@ -2,17 +2,20 @@
 //
 // Pipeline of one force evaluation (one HIP stream, no host round trips):
 //   geometry      thread / polygon     edge lengths, shoelace areas, COM, constraints
-//   cells         hash (thread / vertex, LDS-free atomics on cell counts), exclusive
-//                 scan, scatter, in-cell rank (ascending vertex id) + SoA edge staging
-//   detect        thread / cell-sorted edge: 3x3 cell stencil, segment test, records
-//                 written to per-edge slots, per-polygon bucket counts
+//   detect        polygon bounding circles (lane segment / polygon), polygon cell sort
+//                 + candidate polygon pairs (workgroup / packing, LDS), edge-pair tiles
+//                 (wavefront / polygon pair, LDS-staged edges) with the reference's
+//                 cell-stencil test and segment test; crossings go to per-edge slots
+//   cells         (cell-list getters and updateNeighborCells only) hash, exclusive scan,
+//                 scatter, in-cell rank (ascending vertex id) + SoA edge staging
 //   records       scan of bucket counts, scatter into si- and sj-buckets, one pass that
 //                 ranks each record in its si bucket (final order (si,i,j,sj)) and pairs
 //                 it with its exit record from the sj bucket
 //   forces        exterior contributions per record -> deterministic per-vertex gather
 //                 (exterior + interior chain + penalty terms), per-polygon energies,
 //                 fixed 256-wide tree per packing
-//   projection    thread / polygon bordered cyclic-tridiagonal solve
+//   projection    lane segment / polygon: bordered cyclic-tridiagonal solve by parallel
+//                 cyclic reduction over shuffles (thread / polygon beyond 64 vertices)
 // Nothing is accumulated with floating-point atomics, so every result is
 // bit-reproducible run to run.
 #include "pcp_internal.hpp"
@@ -373,101 +376,9 @@ int run_cells(pcp_ctx* c, const double2* pos) {
 }
 
 // ------------------------------------------------------------------ edge intersections
-// updateNeighborsKernel kernels.cuh:709-812.  One thread per cell-sorted edge, so the
-// lanes of a wavefront share stencil cells and read the same staged edges.
-
-__global__ __launch_bounds__(BLK) void k_detect(
-    int V, int N, int K, const int* __restrict__ polyPack, const int* __restrict__ packPoly,
-    const int* __restrict__ box, const long long* __restrict__ cellOff, const int* __restrict__ cellOf,
-    const int* __restrict__ cellStart, const double2* __restrict__ SP, const double2* __restrict__ SR,
-    const int4* __restrict__ SI, uint64_t* __restrict__ slotKey, float2* __restrict__ slotTU,
-    unsigned char* __restrict__ slotIn, int* __restrict__ numNeighbors, int* __restrict__ scal) {
-    const double eps = 1e-12;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int count = 0;
-    if (t < V) {
-        int4 me = SI[t];
-        int id1 = me.x, id2 = me.y, s1g = me.z, n1 = me.w;
-        int p = polyPack[s1g];
-        int B = box[p];
-        long long cbase = cellOff[p];
-        int pbase = packPoly[p];
-        int s1l = s1g - pbase;
-        double2 P1 = SP[t], R1 = SR[t];
-        int cl = cellOf[id1];
-        int bx = cl % B, by = cl / B;
-        for (int kc = 0; kc < 9; ++kc) {
-            int dx = kc / 3 - 1, dy = kc % 3 - 1;
-            int nx = bx + dx;
-            if (nx < 0) nx += B; else if (nx >= B) nx -= B;
-            int ny = by + dy;
-            if (ny < 0) ny += B; else if (ny >= B) ny -= B;
-            int nb = ny * B + nx;
-            bool dup = false;   // skip cells already visited (kernels.cuh:746-757)
-            for (int kp = 0; kp < kc; ++kp) {
-                int ex = bx + kp / 3 - 1;
-                if (ex < 0) ex += B; else if (ex >= B) ex -= B;
-                int ey = by + kp % 3 - 1;
-                if (ey < 0) ey += B; else if (ey >= B) ey -= B;
-                dup |= (ey * B + ex == nb);
-            }
-            if (dup) continue;
-            int q0 = cellStart[cbase + nb], q1 = cellStart[cbase + nb + 1];
-            for (int q = q0; q < q1; ++q) {
-                int4 ot = SI[q];
-                int nid = ot.x, nid2 = ot.y;
-                if (nid <= id1) continue;
-                if (nid == id1 || nid == id2 || nid2 == id1 || nid2 == id2) continue;
-                double2 Q = SP[q], S = SR[q];
-                double gx = wrap(Q.x - P1.x);
-                double gy = wrap(Q.y - P1.y);
-                double denom = R1.x * S.y - R1.y * S.x;
-                if (fabs(denom) < eps) continue;
-                double tt = (gx * S.y - gy * S.x) / denom;
-                double uu = (gx * R1.y - gy * R1.x) / denom;
-                if (tt > 0.0 && tt < 1.0 && uu > 0.0 && uu < 1.0) {
-                    if (count < K) {
-                        int s2g = ot.z, l2 = ot.w, s2l = s2g - pbase;
-                        uint64_t key;
-                        float2 tu;
-                        unsigned char in;
-                        // updateCompactedIntersectionsKernel kernels.cuh:857-876
-                        if (denom > 0) {
-                            key = ((uint64_t)(uint16_t)s1l << 48) | ((uint64_t)(uint16_t)s2l << 32) |
-                                  ((uint64_t)(uint16_t)l2 << 16) | (uint64_t)(uint16_t)n1;
-                            tu = make_float2((float)tt, (float)uu);
-                            in = 1;
-                        } else {
-                            key = ((uint64_t)(uint16_t)s2l << 48) | ((uint64_t)(uint16_t)s1l << 32) |
-                                  ((uint64_t)(uint16_t)n1 << 16) | (uint64_t)(uint16_t)l2;
-                            tu = make_float2((float)uu, (float)tt);
-                            in = 0;
-                        }
-                        long long slot = (long long)id1 * K + count;
-                        slotKey[slot] = key;
-                        slotTU[slot] = tu;
-                        slotIn[slot] = in;
-                    }
-                    count++;
-                }
-            }
-        }
-        numNeighbors[id1] = count;
-    }
-    int m = wave_max_int(count);
-    if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(&scal[SC_MAXCOUNT], m);
-}
-
-int run_detect(pcp_ctx* c, const double2* pos) {
-    (void)pos;
-    if (c->V == 0) return PCP_OK;
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(BLK), 0, c->stream, c->dScal, (long long)SC_NSCAL, 0);
-    PCP_LAUNCHED();
-    PCP_LAUNCH(c, k_detect, dim3(nblk(c->V)), dim3(BLK), c->V, c->N, c->K,
-                       c->dPolyPack, c->dPackPoly, c->dBox, c->dCellOff, c->dCellOf, c->dCellStart,
-                       c->dSP, c->dSR, c->dSI, c->dSlotKey, c->dSlotTU, c->dSlotIn, c->dNum, c->dScal);
-    return PCP_OK;
-}
+}  // namespace pcp
+#include "pcp_detect.hip"
+namespace pcp {
 
 // restage the sorted edge data from the live positions (updateNeighbors reads the
 // positions at call time, kernels.cuh:721-725; the cells stay those of the last
@@ -1183,6 +1094,7 @@ __global__ __launch_bounds__(BLK) void k_project(int N, int V, const int* __rest
 
 int run_project(pcp_ctx* c, const double2* pos, double2* force) {
     if (c->N == 0) return PCP_OK;
+    if (c->maxN <= 64) return run_project_seg(c, pos, force);
     PCP_LAUNCH(c, k_project, dim3(nblk(c->N)), dim3(BLK), c->N, c->V, c->dPolyStart, pos,
                        force, c->dWork);
     return PCP_OK;
@@ -1275,11 +1187,17 @@ __global__ __launch_bounds__(BLK) void k_shake(int N, int V, int P, const int* _
 int run_shake(pcp_ctx* c, double2* pos, const double* targetEdgeV, const double* targetAreas, int maxIter,
               double tol, int retryIter, const int* active, int* itOut) {
     if (c->N == 0 || maxIter == 0) return PCP_OK;
+    if (c->maxN <= 64)
+        return run_shake_seg(c, pos, targetEdgeV, targetAreas, maxIter, tol, retryIter, active, itOut);
     PCP_LAUNCH(c, k_shake, dim3(nblk(c->N)), dim3(BLK), c->N, c->V, c->P, c->dPolyStart,
                        c->dPolyPack, (double2*)pos, targetEdgeV, targetAreas, maxIter, tol, retryIter,
                        active, c->dWork, itOut);
     return PCP_OK;
 }
+
+}  // namespace pcp
+#include "pcp_solve.hip"
+namespace pcp {
 
 // ------------------------------------------------------------------ XPBD
 // xpbdProjectCUDA model.cu:209-278 with kernels.cuh:532-623: colour-0 then colour-1
@@ -1537,6 +1455,13 @@ int prof_end(pcp_ctx* c, int slot) {
     return PCP_OK;
 }
 
+// records whether the evaluation just enqueued overflowed one of its buffers
+__global__ void k_sticky(const int* __restrict__ scal, const int* __restrict__ numRecords, int K,
+                         long long Icap, int* __restrict__ sticky) {
+    if (threadIdx.x == 0 && (scal[SC_MAXCOUNT] > K || *numRecords > Icap))
+        sticky[0] = 1;
+}
+
 __global__ void k_copy(long long n, double* __restrict__ dst, const double* __restrict__ src) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long stride = (long long)gridDim.x * blockDim.x;
@@ -1597,6 +1522,7 @@ static void free_topology(pcp_ctx* c) {
     dfree(c->dEpre); dfree(c->dNPos); dfree(c->dSteps); dfree(c->dActive); dfree(c->dFlags);
     dfree(c->dFireScal); dfree(c->dShakeIt); dfree(c->dCellInv); dfree(c->dCom); dfree(c->dStatus);
     dfree(c->dSummary); dfree(c->dPartner); dfree(c->dClosed);
+    dfree(c->dBounds); dfree(c->dPcSorted); dfree(c->dPcStart); dfree(c->dPackB); dfree(c->dSticky);
     c->Icap = 0;
     c->haveBoxes = false;
 }
@@ -1622,7 +1548,23 @@ int pcp_destroy(pcp_ctx* c) {
 }
 int pcp_synchronize(pcp_ctx* c) {
     PCP_HIP(hipStreamSynchronize(c->stream));
-    return PCP_OK;
+    if (!c->dSticky) return PCP_OK;
+    int sticky = 0;
+    PCP_HIP(hipMemcpy(&sticky, c->dSticky, sizeof(int), hipMemcpyDeviceToHost));
+    if (!sticky) return PCP_OK;
+    // a fused evaluation since the last check ran out of crossing slots or records: grow
+    // what overflowed and report it, so that the caller evaluates again
+    int scal[SC_NSCAL];
+    int I = 0;
+    PCP_HIP(hipMemcpy(scal, c->dScal, sizeof(int) * SC_NSCAL, hipMemcpyDeviceToHost));
+    PCP_HIP(hipMemcpy(&I, c->dBStart + c->N, sizeof(int), hipMemcpyDeviceToHost));
+    PCP_TRY(pcp::grow_slots(c, std::max(scal[SC_MAXCOUNT], 2 * c->K)));
+    PCP_TRY(pcp::ensure_records(c, std::max<long long>((long long)I + I / 4, c->Icap + 1)));
+    sticky = 0;
+    PCP_HIP(hipMemcpy(c->dSticky, &sticky, sizeof(int), hipMemcpyHostToDevice));
+    pcp::set_error("crossing buffers overflowed in an evaluation since the last synchronisation; "
+                   "they were enlarged: evaluate again");
+    return PCP_ERR_CAPACITY;
 }
 
 }  // extern "C"
@@ -1712,6 +1654,9 @@ int pcp_set_topology(pcp_ctx* c, int P, const int* packPolyStart, const int* pol
     PCP_TRY(dalloc(&c->dShakeIt, 2LL * P));
     PCP_TRY(dalloc(&c->dCellInv, V)); PCP_TRY(dalloc(&c->dCom, 2LL * N)); PCP_TRY(dalloc(&c->dStatus, P));
     PCP_TRY(dalloc(&c->dSummary, 4));
+    PCP_TRY(detect_alloc(c));
+    PCP_TRY(dalloc(&c->dSticky, 1));
+    PCP_HIP(hipMemsetAsync(c->dSticky, 0, sizeof(int), c->stream));
     PCP_HIP(hipMemsetAsync(c->dFlags, 0, sizeof(int) * 8 * (size_t)P, c->stream));
     PCP_HIP(hipMemsetAsync(c->dActive, 0, sizeof(int) * (size_t)P, c->stream));
     PCP_HIP(hipStreamSynchronize(c->stream));
@@ -1757,28 +1702,37 @@ int pcp_update_cells(pcp_ctx* c, const double* pos) {
     return run_cells(c, (const double2*)pos);
 }
 
-// detect; if any edge has more crossings than its slots, grow the slots and repeat
+// detect (cells of the last cell-list build); grow the slots when they overflowed and
+// repeat.  *maxCount = largest crossing count of one edge.
 static int detect_checked(pcp_ctx* c, const double2* pos, int* maxCount) {
-    PCP_TRY(run_detect(c, pos));
-    int mx = 0;
-    PCP_HIP(hipMemcpyAsync(&mx, c->dScal + SC_MAXCOUNT, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    PCP_HIP(hipStreamSynchronize(c->stream));
-    if (mx > c->K) {
-        PCP_TRY(grow_slots(c, std::max(mx, 2 * c->K)));
-        PCP_TRY(run_detect(c, pos));
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        PCP_TRY(run_detect(c, pos, 1));
+        if (c->V > 0) {
+            hipLaunchKernelGGL(k_num_max, dim3(nblk(c->V)), dim3(BLK), 0, c->stream, c->V, c->dNum,
+                               c->dScal + SC_MAXCOUNT);
+            PCP_LAUNCHED();
+        }
+        int scal[SC_NSCAL];
+        PCP_HIP(hipMemcpyAsync(scal, c->dScal, sizeof(int) * SC_NSCAL, hipMemcpyDeviceToHost, c->stream));
+        PCP_HIP(hipStreamSynchronize(c->stream));
+        bool again = false;
+        if (scal[SC_MAXCOUNT] > c->K) { PCP_TRY(grow_slots(c, std::max(scal[SC_MAXCOUNT], 2 * c->K))); again = true; }
+        if (!again) {
+            if (maxCount) *maxCount = scal[SC_MAXCOUNT];
+            return PCP_OK;
+        }
     }
-    if (maxCount) *maxCount = mx;
-    return PCP_OK;
+    set_error("pcp_update_neighbors: crossing buffers kept overflowing");
+    return PCP_ERR_CAPACITY;
 }
 int pcp_update_neighbors(pcp_ctx* c, const double* pos, int* maxCount) {
     if (!c->haveBoxes) return fail_arg("pcp_update_neighbors: set the box sizes first");
-    if (c->V > 0) {
+    if (c->V > 0) {   // staged edges for the neighbour-list getter
         hipLaunchKernelGGL(k_restage, dim3(nblk(c->V)), dim3(BLK), 0, c->stream, c->V, (const double2*)pos,
                            c->dNext, c->dCellSorted, c->dSP, c->dSR);
         PCP_LAUNCHED();
     }
-    if (maxCount) return detect_checked(c, (const double2*)pos, maxCount);
-    return run_detect(c, (const double2*)pos);
+    return detect_checked(c, (const double2*)pos, maxCount);
 }
 
 // records; grow the record arrays when the crossings exceed their capacity
@@ -1977,23 +1931,47 @@ int pcp_get_neighbor_lists(pcp_ctx* c, int p, int* neighbors, unsigned char* ins
 int pcp_get_compacted_records(pcp_ctx* c, int p, uint64_t* inter, double* tu) {
     if (p < 0 || p >= c->P) return fail_arg("packing out of range");
     PCP_HIP(hipStreamSynchronize(c->stream));
-    int v0 = c->hPackVert[p], v1 = c->hPackVert[p + 1];
-    std::vector<int> num(v1 - v0);
-    PCP_HIP(hipMemcpy(num.data(), c->dNum + v0, sizeof(int) * (v1 - v0), hipMemcpyDeviceToHost));
-    // the detection stores each edge's crossings in the reference's slot order; only
-    // that order needs to be read out, edge by edge
+    int v0 = c->hPackVert[p], v1 = c->hPackVert[p + 1], nv = v1 - v0;
+    int B = c->hBox[p], pbase = c->hPackPoly[p];
+    std::vector<int> num(nv), cell(nv);
+    PCP_HIP(hipMemcpy(num.data(), c->dNum + v0, sizeof(int) * nv, hipMemcpyDeviceToHost));
+    PCP_HIP(hipMemcpy(cell.data(), c->dCellOf + v0, sizeof(int) * nv, hipMemcpyDeviceToHost));
+    // compaction order of updateCompactedIntersectionsKernel (kernels.cuh:838-877): edge by
+    // edge, each edge's crossings in the order updateNeighborsKernel found them -- 3x3
+    // stencil cells in (dx, dy) order, a cell counted once, ascending vertex id inside a
+    // cell.  The detection stores them unordered; they are put in that order here.
     std::vector<uint64_t> keys;
     std::vector<float2> tus;
+    std::vector<unsigned char> ins;
+    std::vector<std::pair<long long, int>> ord;
     long long w = 0;
     for (int v = v0; v < v1; ++v) {
         int cnt = std::min(num[v - v0], c->K);
         if (cnt == 0) continue;
-        keys.resize(cnt); tus.resize(cnt);
-        PCP_HIP(hipMemcpy(keys.data(), c->dSlotKey + (long long)v * c->K, sizeof(uint64_t) * cnt, hipMemcpyDeviceToHost));
-        PCP_HIP(hipMemcpy(tus.data(), c->dSlotTU + (long long)v * c->K, sizeof(float2) * cnt, hipMemcpyDeviceToHost));
+        keys.resize(cnt); tus.resize(cnt); ins.resize(cnt);
+        long long base = (long long)v * c->K;
+        PCP_HIP(hipMemcpy(keys.data(), c->dSlotKey + base, sizeof(uint64_t) * cnt, hipMemcpyDeviceToHost));
+        PCP_HIP(hipMemcpy(tus.data(), c->dSlotTU + base, sizeof(float2) * cnt, hipMemcpyDeviceToHost));
+        PCP_HIP(hipMemcpy(ins.data(), c->dSlotIn + base, cnt, hipMemcpyDeviceToHost));
+        int cl = cell[v - v0], bx = cl % B, by = cl / B;
+        ord.clear();
         for (int q = 0; q < cnt; ++q) {
-            if (inter) inter[w] = keys[q];
-            if (tu) { tu[2 * w] = tus[q].x; tu[2 * w + 1] = tus[q].y; }
+            uint64_t k = keys[q];
+            int other = ins[q] ? (int)((k >> 32) & 0xFFFF) : (int)(k >> 48);
+            int oedge = ins[q] ? (int)((k >> 16) & 0xFFFF) : (int)(k & 0xFFFF);
+            int nid = c->hPolyStart[pbase + other] + oedge;
+            int oc = cell[nid - v0];
+            int first = 9;
+            for (int kc = 0; kc < 9 && first == 9; ++kc) {
+                int nx = (bx + kc / 3 - 1 + B) % B, ny = (by + kc % 3 - 1 + B) % B;
+                if (ny * B + nx == oc) first = kc;
+            }
+            ord.push_back({(long long)first * (1LL << 32) + nid, q});
+        }
+        std::sort(ord.begin(), ord.end());
+        for (auto& o : ord) {
+            if (inter) inter[w] = keys[o.second];
+            if (tu) { tu[2 * w] = tus[o.second].x; tu[2 * w + 1] = tus[o.second].y; }
             ++w;
         }
     }
@@ -2024,6 +2002,10 @@ int pcp_stream_copy(pcp_ctx* c, double* dst, const double* src, long long n) {
     return PCP_OK;
 }
 
+int pcp_detect_stats(pcp_ctx* c, long long* polygonPairs, long long* edgePairs) {
+    return detect_stats(c, polygonPairs, edgePairs);
+}
+
 int pcp_sort_keys(const uint64_t* keys, int n, int endBit, uint64_t* sorted, uint32_t* perm) {
     if (n <= 0) return PCP_OK;
     uint64_t mask = (endBit >= 64) ? ~0ull : ((endBit <= 0) ? 0ull : ((1ull << endBit) - 1));
@@ -2041,13 +2023,16 @@ int pcp_sort_keys(const uint64_t* keys, int n, int endBit, uint64_t* sorted, uin
 
 int pcp_force_eval(pcp_ctx* c, const pcp_state* st) {
     const double2* pos = (const double2*)st->pos;
+    if (!c->haveBoxes) return fail_arg("pcp_force_eval: set the box sizes first");
     PCP_TRY(run_geometry(c, pos, st->edgeLengths, st->areas, c->dCom, c->dCom + c->N, nullptr, nullptr));
     int mt = st->model & 0xFF;
     int overlap = (mt == PCP_NORMAL || mt == PCP_HYBRID);
     if (overlap) {
-        PCP_TRY(run_cells(c, pos));
-        PCP_TRY(run_detect(c, pos));
+        PCP_TRY(run_detect(c, pos, 0));
         PCP_TRY(run_records(c, (st->model & PCP_AREA_ENERGY) ? pos : nullptr));
+        hipLaunchKernelGGL(k_sticky, dim3(1), dim3(64), 0, c->stream, c->dScal, c->dBStart + c->N, c->K,
+                           c->Icap, c->dSticky);
+        PCP_LAUNCHED();
     }
     PCP_TRY(run_forces(c, st->model, pos, st->edgeLengths, st->areas, st->targetEdgeV, st->targetAreas,
                        st->stiffness, st->compressibility, (double2*)st->force, st->energy));

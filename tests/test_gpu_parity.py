@@ -68,6 +68,12 @@ def test_cells_neighbors_records_bit_exact(seed, which):
     assert nI[0] == rec["numIntersections"] == r["numIntersections"]
     for key in ("intersections", "outersections", "tu", "ut", "shapeCounts"):
         np.testing.assert_array_equal(r[key], rec[key], err_msg=key)
+    # compaction order: edge by edge, slots in the order of the stencil scan
+    comp = s.get_compacted_records(0)
+    want = np.concatenate([nb["tuSlots"][2 * i * M:2 * (i * M + nb["numNeighbors"][i])]
+                           for i in range(len(nb["numNeighbors"]))] + [np.zeros(0)])
+    np.testing.assert_array_equal(comp["tu"], want.astype(np.float32).astype(np.float64))
+    np.testing.assert_array_equal(np.sort(comp["intersections"]), np.sort(rec["intersections"]))
 
 
 @pytest.mark.parametrize("seed", range(6))
