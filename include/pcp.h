@@ -65,6 +65,8 @@ int pcp_memset(void* ptr_device, int value, size_t bytes);
 /* ---- context ---- */
 int pcp_create(pcp_ctx** ctx);
 int pcp_destroy(pcp_ctx* ctx);
+/* waits for the context's stream; PCP_ERR_CAPACITY if an asynchronous evaluation since
+ * the last call overflowed a crossing buffer (the buffers are enlarged by this call) */
 int pcp_synchronize(pcp_ctx* ctx);
 
 /* Topology.  packPolyStart host[P+1] (polygon offsets per packing),
@@ -163,9 +165,12 @@ int pcp_get_compacted_records(pcp_ctx* ctx, int packing, uint64_t* intersections
  * (host in/out: keys host[n], sorted host[n], perm host[n]) */
 int pcp_sort_keys(const uint64_t* keys, int n, int endBit, uint64_t* sorted, uint32_t* perm);
 
-/* ---- one force evaluation (the unit of "force evals/s"): geometry, cell list,
- *      intersections, records, forces, projection (normal model) -- the sequence of
- *      Model::minimizeFIREStep model.cpp:414-443 without the integrator. ---- */
+/* ---- one force evaluation (the unit of "force evals/s"): geometry, intersections,
+ *      records, forces, projection (normal model) -- the sequence of
+ *      Model::minimizeFIREStep model.cpp:414-443 without the integrator.  The edge cells
+ *      of the user grid are computed inline; the cell-list getters keep reflecting the
+ *      last pcp_update_cells.  Asynchronous: a buffer overflow is reported by the next
+ *      pcp_synchronize (PCP_ERR_CAPACITY, buffers enlarged; evaluate again). ---- */
 typedef struct {
     double* pos;                /* device [2V] */
     double* vel;                /* device [2V] */

@@ -305,6 +305,43 @@ def test_shake_satisfies_constraints():
     assert it2 == 0 and np.array_equal(out2, out)
 
 
+def test_soft_body_penalties_closed_form():
+    """softBody = edge springs + area springs only (kernels.cuh:1284-1352):
+    E = sum_m k (l_m - l0_m)^2 / 2 + sum_s c (A_s - A0_s)^2 / 2 (A0 >= 1e-14), computed here
+    from an independent numpy geometry, and F = -grad E by central differences."""
+    rng = np.random.default_rng(5)
+    polys = [G.regular(int(n), 0.05, (0.2 + 0.3 * k, 0.5), rng.uniform(0, 6)) for k, n in enumerate((5, 7, 9))]
+    polys = [P + rng.normal(scale=3e-3, size=P.shape) for P in polys]
+    start, pos = G.to_model(polys)
+    V = int(start[-1])
+    l0 = rng.uniform(0.02, 0.04, V)
+    A0 = np.array([G.shoelace(P) * f for P, f in zip(polys, (0.9, 1.1, 1.0))])
+    k, c = 3.0, 7.0
+
+    def energy_np(x):
+        e = 0.0
+        for s, P in enumerate(np.split(x.reshape(-1, 2), start[1:-1])):
+            d = np.roll(P, -1, axis=0) - P
+            e += (k * (np.linalg.norm(d, axis=1) - l0[start[s]:start[s + 1]]) ** 2 / 2).sum()
+            e += c * (G.shoelace(P) - A0[s]) ** 2 / 2
+        return e
+
+    def oracle(x):
+        g = O.geometry(x, start)
+        return O.force_energy("softBody", x, start, None, g, l0, A0, k, c)
+
+    E, f = oracle(pos)
+    np.testing.assert_allclose(E, energy_np(pos), rtol=1e-12)
+    h = 1e-7
+    fe = np.zeros_like(pos)
+    for q in range(len(pos)):
+        xp, xm = pos.copy(), pos.copy()
+        xp[q] += h
+        xm[q] -= h
+        fe[q] = -(energy_np(xp) - energy_np(xm)) / (2 * h)
+    np.testing.assert_allclose(f, fe, rtol=0, atol=1e-6 * np.abs(fe).max())
+
+
 def test_det_sum_is_a_fixed_tree():
     rng = np.random.default_rng(0)
     for n in (1, 7, 256, 257, 5000, 70000):
