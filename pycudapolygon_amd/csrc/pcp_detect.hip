@@ -14,13 +14,14 @@
 //                   kept polygon, its edges held in lanes and tested against A's.  An edge
 //                   pair is tested when the midpoint cells of the two edges in the user's
 //                   cell grid are neighbours (the reference's stencil), with the
-//                   reference's arithmetic, and a crossing goes to a slot of the lower edge
+//                   reference's arithmetic; a crossing is appended to the si bucket of the
+//                   polygon its record belongs to and to the sj bucket of the other one
 //
 // Two crossing edges lie in polygons whose bounding circles meet, so the candidates
 // contain every crossing the reference can find, and the cell test drops exactly the
 // edge pairs the reference never looks at: for every box size the crossings (keys,
-// float parameters, inside flags) are the reference's.  Only the order of the slots of
-// one edge differs; every consumer orders the crossings by key.
+// float parameters, inside flags) are the reference's.  Only the order inside a bucket
+// differs from the reference's slot order; every consumer orders the crossings by key.
 
 namespace pcp {
 
@@ -38,14 +39,20 @@ constexpr int PP_BLK = 1024;
 // centre = p_0 + mean u_k, radius = max |u_k - mean u|.  bounds[s] = (cx, cy, r, big),
 // big = 1 when the chain does not close (the polygon wraps the torus) or r is large.
 template <int SEG>
-__global__ __launch_bounds__(BLK) void k_bounds(int N, const int* __restrict__ polyStart,
+__global__ __launch_bounds__(BLK) void k_bounds(int N, int P, const int* __restrict__ polyStart,
                                                const double2* __restrict__ pos, double4* __restrict__ bounds,
-                                               int* __restrict__ num, int* __restrict__ scal) {
+                                               int* __restrict__ num, int* __restrict__ bcnt, int* __restrict__ jcnt,
+                                               unsigned long long* __restrict__ packMaxKey, int* __restrict__ scal) {
     if (blockIdx.x == 0 && threadIdx.x < SC_NSCAL) scal[threadIdx.x] = 0;
+    for (int q = blockIdx.x * BLK + threadIdx.x; q < P; q += gridDim.x * BLK) packMaxKey[q] = 0ull;
     int sub = threadIdx.x & (SEG - 1);
     int s = (blockIdx.x * BLK + threadIdx.x) / SEG;
     int st = 0, n = 0;
-    if (s < N) { st = polyStart[s]; n = polyStart[s + 1] - st; }
+    if (s < N) {
+        st = polyStart[s];
+        n = polyStart[s + 1] - st;
+        if (sub == 0) { bcnt[s] = 0; jcnt[s] = 0; }
+    }
     d2 p0 = (n > 0) ? ldpos(pos, st) : mk(0.0, 0.0);
     // pass 1: chain and its sum (chunks of SEG vertices, running carry)
     double carryX = 0.0, carryY = 0.0, sumX = 0.0, sumY = 0.0;
@@ -225,11 +232,12 @@ struct EdgeStage {
 // every edge of polygon Bq (held in lanes).  A <= Bq; A == Bq tests each pair of
 // non-adjacent edges once.
 template <int CELLS>
-__device__ void edge_tile(int lane, int A, int stA, int nA, int Bq, int sAl, int sBl, int Bu,
+__device__ void edge_tile(int lane, int A, int stA, int nA, int Bq, int sAl, int sBl, int Bu, int pbase,
                           const int* __restrict__ polyStart, const double2* __restrict__ pos,
-                          const int* __restrict__ cellOf, EdgeStage sg, int& staged, int K,
-                          uint64_t* __restrict__ slotKey, float2* __restrict__ slotTU,
-                          unsigned char* __restrict__ slotIn, int* __restrict__ num, int* __restrict__ scal) {
+                          const int* __restrict__ cellOf, EdgeStage sg, int& staged, int Kp,
+                          int* __restrict__ bcnt, uint64_t* __restrict__ bKey, float2* __restrict__ bTU,
+                          int* __restrict__ jcnt, uint64_t* __restrict__ jKey, int* __restrict__ num,
+                          int* __restrict__ scal, uint64_t& kmax) {
     const double eps = 1e-12;
     bool self = (A == Bq);
     int stB = polyStart[Bq], nB = polyStart[Bq + 1] - stB;
@@ -271,31 +279,31 @@ __device__ void edge_tile(int lane, int A, int stA, int nA, int Bq, int sAl, int
                 double tt = (gx * S.y - gy * S.x) / denom;
                 double uu = (gx * R1.y - gy * R1.x) / denom;
                 if (tt > 0.0 && tt < 1.0 && uu > 0.0 && uu < 1.0) {
-                    int id1 = stA + a;
-                    int cnt = atomicAdd(&num[id1], 1);
-                    if (cnt < K) {
-                        uint64_t key;
-                        float2 tu;
-                        unsigned char in;
-                        // updateCompactedIntersectionsKernel kernels.cuh:857-876
-                        if (denom > 0) {
-                            key = ((uint64_t)(uint16_t)sAl << 48) | ((uint64_t)(uint16_t)sBl << 32) |
-                                  ((uint64_t)(uint16_t)b << 16) | (uint64_t)(uint16_t)a;
-                            tu = make_float2((float)tt, (float)uu);
-                            in = 1;
-                        } else {
-                            key = ((uint64_t)(uint16_t)sBl << 48) | ((uint64_t)(uint16_t)sAl << 32) |
-                                  ((uint64_t)(uint16_t)a << 16) | (uint64_t)(uint16_t)b;
-                            tu = make_float2((float)uu, (float)tt);
-                            in = 0;
-                        }
-                        long long slot = (long long)id1 * K + cnt;
-                        slotKey[slot] = key;
-                        slotTU[slot] = tu;
-                        slotIn[slot] = in;
+                    uint64_t key;
+                    float2 tu;
+                    // updateCompactedIntersectionsKernel kernels.cuh:857-876
+                    if (denom > 0) {
+                        key = ((uint64_t)(uint16_t)sAl << 48) | ((uint64_t)(uint16_t)sBl << 32) |
+                              ((uint64_t)(uint16_t)b << 16) | (uint64_t)(uint16_t)a;
+                        tu = make_float2((float)tt, (float)uu);
                     } else {
-                        atomicMax(&scal[SC_MAXCOUNT], cnt + 1);
+                        key = ((uint64_t)(uint16_t)sBl << 48) | ((uint64_t)(uint16_t)sAl << 32) |
+                              ((uint64_t)(uint16_t)a << 16) | (uint64_t)(uint16_t)b;
+                        tu = make_float2((float)uu, (float)tt);
                     }
+                    kmax = (key > kmax) ? key : kmax;
+                    if (CELLS) atomicAdd(&num[stA + a], 1);   // per-edge count (updateNeighbors)
+                    int sig = pbase + key_si(key), sjg = pbase + key_sj(key);
+                    int b1 = atomicAdd(&bcnt[sig], 1);
+                    if (b1 < Kp) {
+                        bKey[(long long)sig * Kp + b1] = key;
+                        bTU[(long long)sig * Kp + b1] = tu;
+                    } else {
+                        atomicMax(&scal[SC_BUCKET], b1 + 1);
+                    }
+                    int b2 = atomicAdd(&jcnt[sjg], 1);
+                    if (b2 < Kp) jKey[(long long)sjg * Kp + b2] = key;
+                    else atomicMax(&scal[SC_BUCKET], b2 + 1);
                 }
             }
         }
@@ -315,9 +323,9 @@ __global__ __launch_bounds__(BLK) void k_detect_poly(
     int N, const int* __restrict__ polyStart, const int* __restrict__ polyPack,
     const int* __restrict__ packPoly, const int* __restrict__ box, const double2* __restrict__ pos,
     const int* __restrict__ cellOf, const double4* __restrict__ bounds, const int* __restrict__ pcStart,
-    const int* __restrict__ pcSorted, const int* __restrict__ packB, int K, uint64_t* __restrict__ slotKey,
-    float2* __restrict__ slotTU, unsigned char* __restrict__ slotIn, int* __restrict__ num,
-    int* __restrict__ scal) {
+    const int* __restrict__ pcSorted, const int* __restrict__ packB, int Kp, int* __restrict__ bcnt,
+    uint64_t* __restrict__ bKey, float2* __restrict__ bTU, int* __restrict__ jcnt, uint64_t* __restrict__ jKey,
+    unsigned long long* __restrict__ packMaxKey, int* __restrict__ num, int* __restrict__ scal) {
     __shared__ double2 sP[BLK / 64][64];
     __shared__ double2 sR[BLK / 64][64];
     __shared__ int sC[BLK / 64][64];
@@ -335,9 +343,10 @@ __global__ __launch_bounds__(BLK) void k_detect_poly(
     bool big = (pb >> 30) & 1;
     int sAl = A - s0;
     double4 bA = bounds[A];
+    uint64_t kmax = 0;
     if (nA >= 4)
-        edge_tile<CELLS>(lane, A, stA, nA, A, sAl, sAl, Bu, polyStart, pos, cellOf, sg, staged, K, slotKey,
-                         slotTU, slotIn, num, scal);
+        edge_tile<CELLS>(lane, A, stA, nA, A, sAl, sAl, Bu, s0, polyStart, pos, cellOf, sg, staged, Kp, bcnt,
+                         bKey, bTU, jcnt, jKey, num, scal, kmax);
     // candidate cells: lanes 0..8 hold the ranges of the 3x3 block (one cell when Bp == 1)
     long long gbase = (long long)pk * (PC_CELLS + 1);
     int ncell = (Bp == 1) ? 1 : 9;
@@ -378,10 +387,18 @@ __global__ __launch_bounds__(BLK) void k_detect_poly(
             int j = __ffsll((long long)keep) - 1;
             keep &= keep - 1;
             int Bsel = __shfl(Bq, j);
-            edge_tile<CELLS>(lane, A, stA, nA, Bsel, sAl, Bsel - s0, Bu, polyStart, pos, cellOf, sg, staged, K,
-                             slotKey, slotTU, slotIn, num, scal);
+            edge_tile<CELLS>(lane, A, stA, nA, Bsel, sAl, Bsel - s0, Bu, s0, polyStart, pos, cellOf, sg, staged,
+                             Kp, bcnt, bKey, bTU, jcnt, jKey, num, scal, kmax);
         }
     }
+    // largest key of the packing (the pairing's last-record case): one atomic per
+    // wavefront, only when it raises the current value
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        unsigned long long y = __shfl_xor((unsigned long long)kmax, o);
+        kmax = (y > kmax) ? y : kmax;
+    }
+    if (lane == 0 && kmax > 0 && kmax > packMaxKey[pk]) atomicMax(&packMaxKey[pk], (unsigned long long)kmax);
 }
 
 // exact maximum of the per-edge crossing counts (updateNeighbors' overflow check)
@@ -410,11 +427,11 @@ int launch_bounds(pcp_ctx* c, const double2* pos) {
     int grid = (c->N + polysPerBlock - 1) / polysPerBlock;
     if (grid < 1) grid = 1;
     switch (seg) {
-    case 4: PCP_LAUNCH(c, k_bounds<4>, dim3(grid), dim3(BLK), c->N, c->dPolyStart, pos, c->dBounds, c->dNum, c->dScal); break;
-    case 8: PCP_LAUNCH(c, k_bounds<8>, dim3(grid), dim3(BLK), c->N, c->dPolyStart, pos, c->dBounds, c->dNum, c->dScal); break;
-    case 16: PCP_LAUNCH(c, k_bounds<16>, dim3(grid), dim3(BLK), c->N, c->dPolyStart, pos, c->dBounds, c->dNum, c->dScal); break;
-    case 32: PCP_LAUNCH(c, k_bounds<32>, dim3(grid), dim3(BLK), c->N, c->dPolyStart, pos, c->dBounds, c->dNum, c->dScal); break;
-    default: PCP_LAUNCH(c, k_bounds<64>, dim3(grid), dim3(BLK), c->N, c->dPolyStart, pos, c->dBounds, c->dNum, c->dScal); break;
+    case 4: PCP_LAUNCH(c, k_bounds<4>, dim3(grid), dim3(BLK), c->N, c->P, c->dPolyStart, pos, c->dBounds, c->dNum, c->dBcnt, c->dJcnt, c->dPackMaxKey, c->dScal); break;
+    case 8: PCP_LAUNCH(c, k_bounds<8>, dim3(grid), dim3(BLK), c->N, c->P, c->dPolyStart, pos, c->dBounds, c->dNum, c->dBcnt, c->dJcnt, c->dPackMaxKey, c->dScal); break;
+    case 16: PCP_LAUNCH(c, k_bounds<16>, dim3(grid), dim3(BLK), c->N, c->P, c->dPolyStart, pos, c->dBounds, c->dNum, c->dBcnt, c->dJcnt, c->dPackMaxKey, c->dScal); break;
+    case 32: PCP_LAUNCH(c, k_bounds<32>, dim3(grid), dim3(BLK), c->N, c->P, c->dPolyStart, pos, c->dBounds, c->dNum, c->dBcnt, c->dJcnt, c->dPackMaxKey, c->dScal); break;
+    default: PCP_LAUNCH(c, k_bounds<64>, dim3(grid), dim3(BLK), c->N, c->P, c->dPolyStart, pos, c->dBounds, c->dNum, c->dBcnt, c->dJcnt, c->dPackMaxKey, c->dScal); break;
     }
     return PCP_OK;
 }
@@ -428,12 +445,12 @@ int run_detect(pcp_ctx* c, const double2* pos, int cellsFromList) {
     int grid = (c->N + BLK / 64 - 1) / (BLK / 64);
     if (cellsFromList)
         PCP_LAUNCH(c, k_detect_poly<1>, dim3(grid), dim3(BLK), c->N, c->dPolyStart, c->dPolyPack, c->dPackPoly,
-                   c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted, c->dPackB, c->K, c->dSlotKey,
-                   c->dSlotTU, c->dSlotIn, c->dNum, c->dScal);
+                   c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted, c->dPackB, c->Kp, c->dBcnt,
+                   c->dBKey, c->dBTU, c->dJcnt, c->dJKey, c->dPackMaxKey, c->dNum, c->dScal);
     else
         PCP_LAUNCH(c, k_detect_poly<0>, dim3(grid), dim3(BLK), c->N, c->dPolyStart, c->dPolyPack, c->dPackPoly,
-                   c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted, c->dPackB, c->K, c->dSlotKey,
-                   c->dSlotTU, c->dSlotIn, c->dNum, c->dScal);
+                   c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted, c->dPackB, c->Kp, c->dBcnt,
+                   c->dBKey, c->dBTU, c->dJcnt, c->dJKey, c->dPackMaxKey, c->dNum, c->dScal);
     return PCP_OK;
 }
 

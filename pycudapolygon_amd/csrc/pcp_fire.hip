@@ -299,10 +299,8 @@ static int fire_eval(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm
             PCP_TRY(run_detect(c, x, 0));
         PCP_TRY(run_records(c, area ? x : nullptr));
     }
-    PCP_TRY(run_geometry(c, x, st->edgeLengths, st->areas, c->dCom, c->dCom + c->N, nullptr, nullptr));
     PCP_TRY(run_forces(c, st->model, x, st->edgeLengths, st->areas, st->targetEdgeV, st->targetAreas,
-                       st->stiffness, st->compressibility, (double2*)st->force, st->energy));
-    if (rigid) PCP_TRY(run_project(c, x, (double2*)st->force));
+                       st->stiffness, st->compressibility, (double2*)st->force, st->energy, 1, rigid));
     return PCP_OK;
 }
 
@@ -347,22 +345,19 @@ static int fire_step_impl(pcp_ctx* c, const pcp_state* st, const pcp_fire_params
         PCP_TRY(fire_finish(c, st, prm, 1, 1));
         int summary[4] = {0, 0, 0, 0};
         int scal[SC_NSCAL];
-        int I = 0;
         PCP_HIP(hipMemcpyAsync(summary, c->dSummary, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
         PCP_HIP(hipMemcpyAsync(scal, c->dScal, sizeof(int) * SC_NSCAL, hipMemcpyDeviceToHost, c->stream));
-        PCP_HIP(hipMemcpyAsync(&I, c->dBStart + c->N, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         PCP_HIP(hipStreamSynchronize(c->stream));
-        if (overlap) c->lastI = I;
-        if (overlap && (scal[SC_MAXCOUNT] > c->K || I > c->Icap)) {
-            // the step ran with too few crossing slots or records: grow, undo, retry
+        if (overlap && scal[SC_BUCKET] > c->Kp) {
+            // the step ran with too small record buckets: grow, undo, retry
             hipLaunchKernelGGL(k_fire_undo, dim3(nblk(std::max(c->V, c->P))), dim3(BLK), 0, c->stream, c->V,
                                c->P, c->dShapeId, c->dPolyPack, c->dActive, c->dFlags, (double2*)st->pos,
                                (double2*)st->vel, (const double2*)c->dXbak, (const double2*)c->dVbak, c->dDt,
                                c->dAlpha, c->dNPos, c->dSteps, c->dFireScal);
             PCP_LAUNCHED();
-            if (scal[SC_MAXCOUNT] > c->K) PCP_TRY(grow_slots(c, std::max(scal[SC_MAXCOUNT], 2 * c->K)));
-            if (I > c->Icap) PCP_TRY(ensure_records(c, (long long)I + I / 4));
+            PCP_TRY(grow_buckets(c, scal[SC_BUCKET]));
             PCP_TRY(pcp_force_eval(c, st));
+            PCP_HIP(hipMemsetAsync(c->dSticky, 0, sizeof(int), c->stream));   // handled here
             continue;
         }
         if (summary[0]) {
@@ -376,7 +371,7 @@ static int fire_step_impl(pcp_ctx* c, const pcp_state* st, const pcp_fire_params
         if (anyActive) *anyActive = summary[1];
         return PCP_OK;
     }
-    set_error("pcp_fire_step: crossing buffers kept overflowing");
+    set_error("pcp_fire_step: record buckets kept overflowing");
     return PCP_ERR_CAPACITY;
 }
 

@@ -32,7 +32,9 @@ constexpr int BLK = 256;
 inline int nblk(long long n, int b = BLK) { return (int)((n + b - 1) / b); }
 
 // scalar slots in ctx->dScal
-enum { SC_MAXCOUNT = 0, SC_OVERFLOW = 1, SC_NSCAL = 8 };
+// SC_MAXCOUNT: largest per-edge crossing count (updateNeighbors); SC_BUCKET: largest
+// bucket count, set only when a bucket overflowed
+enum { SC_MAXCOUNT = 0, SC_BUCKET = 1, SC_NSCAL = 8 };
 
 template <class T> int dalloc(T** p, size_t count) {
     *p = nullptr;
@@ -50,11 +52,9 @@ template <class T> void dfree(T*& p) {
 struct pcp_ctx {
     hipStream_t stream = nullptr;
     int P = 0, N = 0, V = 0;
-    int K = 32;          // crossing slots per edge (grown on demand)
+    int Kp = 64;         // records per polygon bucket (grown on demand)
     int M = 100;         // reference maxNeighbors for the neighbour-list getters
-    long long Icap = 0;  // record capacity
     long long Ctot = 0;  // total cells over all packings
-    long long lastI = 0; // last observed record total (grid sizing only)
     bool haveBoxes = false;
 
     std::vector<int> hPackPoly, hPackVert, hPolyStart, hBox;
@@ -79,30 +79,24 @@ struct pcp_ctx {
     int* dPcStart = nullptr;       // P x (PC_CELLS + 1): cell starts of each packing
     int* dPackB = nullptr;         // P: polygon grid size | big << 30
     int* dSticky = nullptr;        // [0] a fused evaluation overflowed a buffer
-    uint64_t* dSlotKey = nullptr;
-    float2* dSlotTU = nullptr;
-    unsigned char* dSlotIn = nullptr;
-    int* dNum = nullptr;
-    int* dCounts = nullptr;     // 2N: si buckets, then sj buckets
-    int* dBStart = nullptr;     // 2N+1 exclusive scan of dCounts
-    unsigned long long* dPackMaxKey = nullptr;
+    int* dNum = nullptr;           // V: crossings per edge (updateNeighbors' count)
+    unsigned long long* dPackMaxKey = nullptr;   // P: largest record key of each packing
     int* dScal = nullptr;
 
-    // records (capacity Icap)
-    uint64_t* dKeys2 = nullptr;    // 2*Icap: si-bucketed keys, then sj-bucketed keys
+    // records, per polygon s at rows [s*Kp, s*Kp + count)
+    int *dBcnt = nullptr, *dJcnt = nullptr;      // N: si / sj bucket counts
+    uint64_t* dBKey = nullptr;     // si bucket as detected (key, float t/u)
     float2* dBTU = nullptr;
-    int* dBPoly = nullptr;
-    uint64_t *dRKey = nullptr, *dROuter = nullptr;
+    uint64_t* dJKey = nullptr;     // sj bucket keys as detected
+    uint64_t *dRKey = nullptr, *dROuter = nullptr;   // si bucket sorted (i, j, sj) + exit record
     float2 *dRTU = nullptr, *dRUT = nullptr;
-    int* dRPoly = nullptr;
-    int* dSjList = nullptr;
-    double2* dContrib = nullptr;
-    int* dTgt = nullptr;
-    double* dEext = nullptr;
-    double2* dSp = nullptr;
+    double2* dJC = nullptr;        // 4 per sj-list entry: contributions acting on the polygon
+    int* dJT = nullptr;            // their vertex (local index, -1 none)
+    double2 *dF1 = nullptr, *dLM = nullptr, *dLN = nullptr;   // V: per-vertex partial sums
+    double* dAccR = nullptr;       // N: record + interior energy sums
 
     // per-vertex / per-polygon scratch
-    double *dEint = nullptr, *dEedge = nullptr, *dEpoly = nullptr;
+    double* dEpoly = nullptr;
     double* dWork = nullptr;       // 8*V doubles (projection / SHAKE solver)
     double* dRed = nullptr;        // 4*N per-polygon partials
     double* dRedOut = nullptr;     // 4*P per-packing reductions
@@ -122,12 +116,9 @@ struct pcp_ctx {
     std::string profName;          // kernel bracketed with events (measurement)
     std::vector<std::pair<hipEvent_t, hipEvent_t>> profEvents;
     size_t profUsed = 0;
-    int* dPartner = nullptr;       // Icap: index of each record's exit record (area energy)
-    unsigned char* dClosed = nullptr;  // Icap: record lies on a closed partner cycle
 };
 
 namespace pcp {
-int ensure_records(pcp_ctx* c, long long need);
 int scan_exclusive(pcp_ctx* c, const int* in, int n, int* out);
 int launch_det_reduce(pcp_ctx* c, const double* partials, int nArrays, double* outSums, int nMax);
 int run_cells(pcp_ctx* c, const double2* pos);
@@ -135,30 +126,32 @@ int run_detect(pcp_ctx* c, const double2* pos, int cellsFromList);
 int detect_alloc(pcp_ctx* c);
 int detect_stats(pcp_ctx* c, long long* polygonPairs, long long* edgePairs);
 int seg_width(int maxN);
-int run_project_seg(pcp_ctx* c, const double2* pos, double2* force);
-int run_shake_seg(pcp_ctx* c, double2* pos, const double* targetEdgeV, const double* targetAreas, int maxIter,
-                  double tol, int retryIter, const int* active, int* itOut);
+int alloc_buckets(pcp_ctx* c, int Kp);
+int grow_buckets(pcp_ctx* c, int need);
+int packing_counts(pcp_ctx* c, int p, std::vector<int>& cnt);
 int run_records(pcp_ctx* c, const double2* pairPos);
-int run_forces(pcp_ctx* c, int model, const double2* pos, const double* edgeLengths,
-               const double* areas, const double* targetEdgeV, const double* targetAreas,
-               double stiffness, double compressibility, double2* force, double* energy);
+int run_forces(pcp_ctx* c, int model, const double2* pos, double* edgeLengths, double* areas,
+               const double* targetEdgeV, const double* targetAreas, double stiffness,
+               double compressibility, double2* force, double* energy, int geom, int project);
 int run_project(pcp_ctx* c, const double2* pos, double2* force);
+int run_project_seg(pcp_ctx* c, const double2* pos, double2* force);
 int run_geometry(pcp_ctx* c, const double2* pos, double* edgeLengths, double* areas, double* comX,
                  double* comY, double* constraints, double* maxEdge);
 int run_shake(pcp_ctx* c, double2* pos, const double* targetEdgeV, const double* targetAreas,
               int maxIter, double tol, int retryIter, const int* active, int* itOut);
-int record_grid(pcp_ctx* c);
-int grow_slots(pcp_ctx* c, int K);
+int run_shake_seg(pcp_ctx* c, double2* pos, const double* targetEdgeV, const double* targetAreas, int maxIter,
+                  double tol, int retryIter, const int* active, int* itOut);
 int prof_begin(pcp_ctx* c, const char* name, int* slot);
 int prof_end(pcp_ctx* c, int slot);
 }  // namespace pcp
 
 // launch `kern` on the context stream, bracketed with events when it is the profiled kernel
-#define PCP_LAUNCH(c, kern, grid, block, ...)                                          \
+#define PCP_LAUNCH(c, kern, grid, block, ...) PCP_LAUNCH_LDS(c, kern, grid, block, 0, __VA_ARGS__)
+#define PCP_LAUNCH_LDS(c, kern, grid, block, lds, ...)                                 \
     do {                                                                               \
         int _slot = -1;                                                                \
         PCP_TRY(::pcp::prof_begin((c), #kern, &_slot));                                \
-        hipLaunchKernelGGL(kern, grid, block, 0, (c)->stream, __VA_ARGS__);            \
+        hipLaunchKernelGGL(kern, grid, block, lds, (c)->stream, __VA_ARGS__);          \
         PCP_LAUNCHED();                                                                \
         PCP_TRY(::pcp::prof_end((c), _slot));                                          \
     } while (0)

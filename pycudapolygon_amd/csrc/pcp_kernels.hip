@@ -409,453 +409,6 @@ __global__ void k_sort_keys(int n, uint64_t mask, const uint64_t* __restrict__ i
     perm[rank] = (uint32_t)r;
 }
 
-// ------------------------------------------------------------------ records
-// bucket sizes from the stored crossings (updateValidAndCountsKernel kernels.cuh:814-836
-// counts per si; the sj counts index the records owned through sj).  Recounting here
-// keeps updateOutersections idempotent; the scatter below returns the counts to zero.
-__global__ void k_count(int V, int N, int P, int K, const int* __restrict__ shapeId,
-                        const int* __restrict__ polyPack, const int* __restrict__ packPoly,
-                        const int* __restrict__ numNeighbors, const uint64_t* __restrict__ slotKey,
-                        int* __restrict__ counts, unsigned long long* __restrict__ packMaxKey) {
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < P) packMaxKey[v] = 0ull;
-    if (v >= V) return;
-    int cnt = min(numNeighbors[v], K);
-    if (cnt == 0) return;
-    int pbase = packPoly[polyPack[shapeId[v]]];
-    for (int c = 0; c < cnt; ++c) {
-        uint64_t key = slotKey[(long long)v * K + c];
-        atomicAdd(&counts[pbase + key_si(key)], 1);
-        atomicAdd(&counts[N + pbase + key_sj(key)], 1);
-    }
-}
-
-// scatter each crossing into its si bucket and its sj bucket (unordered)
-__global__ void k_bucket_scatter(int V, int N, int K, long long Icap, const int* __restrict__ shapeId,
-                                 const int* __restrict__ polyPack, const int* __restrict__ packPoly,
-                                 const int* __restrict__ numNeighbors, const uint64_t* __restrict__ slotKey,
-                                 const float2* __restrict__ slotTU, int* __restrict__ counts,
-                                 const int* __restrict__ bStart, uint64_t* __restrict__ keys2,
-                                 float2* __restrict__ bTU, int* __restrict__ bPoly,
-                                 unsigned long long* __restrict__ packMaxKey, int* __restrict__ scal) {
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long mk = 0;
-    int p = -1;
-    if (v < V) {
-        int cnt = min(numNeighbors[v], K);
-        p = polyPack[shapeId[v]];
-        int pbase = packPoly[p];
-        long long I = bStart[N];
-        if (I > Icap) scal[SC_OVERFLOW] = 1;
-        for (int c = 0; c < cnt; ++c) {
-            long long slot = (long long)v * K + c;
-            uint64_t key = slotKey[slot];
-            float2 tu = slotTU[slot];
-            int sig = pbase + key_si(key), sjg = pbase + key_sj(key);
-            int a = atomicSub(&counts[sig], 1) - 1;
-            long long p1 = (long long)bStart[sig] + a;
-            if (p1 < Icap) {
-                keys2[p1] = key;
-                bTU[p1] = tu;
-                bPoly[p1] = sig;
-            }
-            int b = atomicSub(&counts[N + sjg], 1) - 1;
-            long long p2 = (long long)bStart[N + sjg] + b;   // in [I, 2I)
-            if (p2 < 2 * Icap && I <= Icap) keys2[p2] = key;
-            if (key > mk) mk = key;
-        }
-    }
-    // per-packing maximum key (needed by the pairing rule); one atomic per wave
-    // when the whole wavefront belongs to one packing
-    int p0 = __shfl(p, 0);
-    bool uni = __all(p == p0 || p == -1);
-    if (uni) {
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) {
-            unsigned long long y = __shfl_xor(mk, o);
-            mk = (y > mk) ? y : mk;
-        }
-        if ((threadIdx.x & 63) == 0 && mk > 0 && p0 >= 0) atomicMax(&packMaxKey[p0], mk);
-    } else if (mk > 0) {
-        atomicMax(&packMaxKey[p], mk);
-    }
-}
-
-// rank each record inside its si bucket (final order = (si, i, j, sj)) and pair it
-// with its exit record: updateOutersectionsKernel kernels.cuh:981-1065.
-// The reference scans the group of keys with prefix (si<<48 | sj<<32) in 64-bit key
-// order; that group is the si'=sj bucket filtered on sj'=si, and its key order is
-// bucket_order order.  The choice (min cyclic distance d, then min float u, ties to
-// the first in key order; edge l==i only when u > t) does not depend on scan order.
-// The reference's binary search starts with end = I-1, so when every key lies below
-// (si<<48 | sj<<32) the last record of the packing is the only candidate; kept.
-__global__ __launch_bounds__(BLK) void k_rank_pair(
-    int N, long long Icap, const int* __restrict__ polyStart, const int* __restrict__ polyPack,
-    const int* __restrict__ packPoly, const int* __restrict__ bStart, const uint64_t* __restrict__ keys2,
-    const float2* __restrict__ bTU, const int* __restrict__ bPoly,
-    const unsigned long long* __restrict__ packMaxKey, uint64_t* __restrict__ rKey,
-    float2* __restrict__ rTU, uint64_t* __restrict__ rOuter, float2* __restrict__ rUT,
-    int* __restrict__ rPoly, int* __restrict__ sjList, const double2* __restrict__ pairPos) {
-    long long I = bStart[N];
-    if (I > Icap) return;
-    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < I;
-         e += (long long)gridDim.x * blockDim.x) {
-        uint64_t key = keys2[e];
-        float2 tu = bTU[e];
-        int s = bPoly[e];
-        int p = polyPack[s];
-        int pbase = packPoly[p];
-        int lo = bStart[s], hi = bStart[s + 1];
-        uint64_t ord = bucket_order(key);
-        int rank = 0;
-        for (int q = lo; q < hi; ++q) rank += (bucket_order(keys2[q]) < ord);
-        long long f = lo + rank;
-
-        int si_l = key_si(key), sj_l = key_sj(key), i = key_i(key);
-        int ni = polyStart[s + 1] - polyStart[s];
-        int sjg0 = pbase + sj_l;
-        int nj = polyStart[sjg0 + 1] - polyStart[sjg0];
-        // exact parameter along si's edge ei of its crossing with sj's edge ej (area energy)
-        auto exact = [&](int ei, int ej, double fb) -> double {
-            int a = polyStart[s] + ei, b = polyStart[s] + ((ei + 1 == ni) ? 0 : ei + 1);
-            int c2 = polyStart[sjg0] + ej, d = polyStart[sjg0] + ((ej + 1 == nj) ? 0 : ej + 1);
-            return cross_param(ldpos(pairPos, a), ldpos(pairPos, b), ldpos(pairPos, c2), ldpos(pairPos, d), fb);
-        };
-        double tVal = (double)tu.y;
-        if (pairPos) tVal = exact(i, key_j(key), tVal);
-        uint64_t sij = ((uint64_t)si_l << 48) | ((uint64_t)sj_l << 32);
-        uint64_t mxk = packMaxKey[p];
-        int bestDist = -1;
-        bool found = false;
-        double bestU = 0.0;
-        uint64_t bestOrd = 0, bestKey = 0;
-        float2 bestTU = make_float2(0.f, 0.f);
-        auto consider = [&](uint64_t k2, float2 tu2) {
-            int l = key_j(k2);
-            double uVal = (double)tu2.x;
-            if (pairPos && mxk >= sij) uVal = exact(l, key_i(k2), uVal);
-            int d = (l - i + ni) % ni;
-            if (l == i && tVal >= uVal) return;
-            if (bestDist == -1 || d < bestDist) { bestDist = d; found = false; }
-            if (d == bestDist && uVal == uVal) {
-                uint64_t o2 = bucket_order(k2);
-                if (!found || uVal < bestU || (uVal == bestU && o2 < bestOrd)) {
-                    found = true; bestU = uVal; bestOrd = o2; bestKey = k2; bestTU = tu2;
-                }
-            }
-        };
-        if (mxk < sij) {
-            int s3 = pbase + key_si(mxk);
-            for (int q = bStart[s3]; q < bStart[s3 + 1]; ++q)
-                if (keys2[q] == mxk) { consider(mxk, bTU[q]); break; }
-        } else {
-            int sjg = pbase + sj_l;
-            for (int q = bStart[sjg]; q < bStart[sjg + 1]; ++q) {
-                uint64_t k2 = keys2[q];
-                if (key_sj(k2) != si_l) continue;
-                consider(k2, bTU[q]);
-            }
-        }
-        // a partner whose edge indices fall outside the two polygons (possible only in the
-        // last-record case above) is treated as no partner: the reference would read
-        // vertices of other polygons there
-        if (found && (key_i(bestKey) >= nj || key_j(bestKey) >= ni)) found = false;
-        rKey[f] = key;
-        rTU[f] = tu;
-        rOuter[f] = found ? bestKey : key;
-        rUT[f] = found ? bestTU : tu;
-        rPoly[f] = s;
-        // position among the records whose sj is this record's sj (key order)
-        int sjg = pbase + sj_l;
-        int lo2 = bStart[N + sjg], hi2 = bStart[N + sjg + 1];
-        int rA = 0;
-        for (int q = lo2; q < hi2; ++q) rA += (keys2[q] < key);
-        sjList[lo2 - I + rA] = (int)f;
-    }
-}
-
-int record_grid(pcp_ctx* c) {
-    long long est = std::max<long long>(c->lastI + c->lastI / 2, 4096);
-    est = std::min<long long>(est, c->Icap);
-    int g = nblk(est);
-    return std::max(1, std::min(g, 8192));
-}
-
-int run_records(pcp_ctx* c, const double2* pairPos) {
-    if (c->V == 0) return PCP_OK;
-    hipLaunchKernelGGL(k_count, dim3(nblk(std::max(c->V, c->P))), dim3(BLK), 0, c->stream, c->V, c->N, c->P,
-                       c->K, c->dShapeId, c->dPolyPack, c->dPackPoly, c->dNum, c->dSlotKey, c->dCounts,
-                       c->dPackMaxKey);
-    PCP_LAUNCHED();
-    PCP_TRY(scan_exclusive(c, c->dCounts, 2 * c->N, c->dBStart));
-    PCP_LAUNCH(c, k_bucket_scatter, dim3(nblk(c->V)), dim3(BLK), c->V, c->N, c->K,
-                       c->Icap, c->dShapeId, c->dPolyPack, c->dPackPoly, c->dNum, c->dSlotKey,
-                       c->dSlotTU, c->dCounts, c->dBStart, c->dKeys2, c->dBTU, c->dBPoly,
-                       c->dPackMaxKey, c->dScal);
-    PCP_LAUNCH(c, k_rank_pair, dim3(record_grid(c)), dim3(BLK), c->N, c->Icap,
-                       c->dPolyStart, c->dPolyPack, c->dPackPoly, c->dBStart, c->dKeys2, c->dBTU,
-                       c->dBPoly, c->dPackMaxKey, c->dRKey, c->dRTU, c->dROuter, c->dRUT, c->dRPoly,
-                       c->dSjList, pairPos);
-    return PCP_OK;
-}
-
-// ------------------------------------------------------------------ forces
-// updateForceEnergyExteriorKernel kernels.cuh:1067-1256: the 8 contributions of one
-// record (slots i, zi, l, zl owned by si; j, zj, k, zk owned by sj), its energy and
-// its start point fij (also the start point of the interior chain terms, :1390-1396).
-__global__ __launch_bounds__(BLK) void k_exterior(
-    int N, long long Icap, const int* __restrict__ bStart, const int* __restrict__ polyStart,
-    const int* __restrict__ polyPack, const int* __restrict__ packPoly, const int* __restrict__ next,
-    const double2* __restrict__ pos, const uint64_t* __restrict__ rKey,
-    const uint64_t* __restrict__ rOuter, const float2* __restrict__ rTU, const float2* __restrict__ rUT,
-    const int* __restrict__ rPoly, double2* __restrict__ contrib, int* __restrict__ tgt,
-    double* __restrict__ eext, double2* __restrict__ spOut, int area,
-    const unsigned char* __restrict__ closed) {
-    long long I = bStart[N];
-    if (I > Icap) return;
-    for (long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x; f < I;
-         f += (long long)gridDim.x * blockDim.x) {
-        uint64_t inter = rKey[f], outer = rOuter[f];
-        if (inter == outer) {
-#pragma unroll
-            for (int q = 0; q < 8; ++q) tgt[f * 8 + q] = -1;
-            eext[f] = 0.0;
-            continue;
-        }
-        int sig = rPoly[f];
-        int pbase = packPoly[polyPack[sig]];
-        int sjg = pbase + key_sj(inter);
-        int st1 = polyStart[sig], st2 = polyStart[sjg];
-        int i = st1 + key_i(inter), j = st2 + key_j(inter);
-        int k = st2 + key_i(outer), l = st1 + key_j(outer);
-        d2 pi = ldpos(pos, i), pj = ldpos(pos, j), pk = ldpos(pos, k), pl = ldpos(pos, l);
-        int zi = next[i], zj = next[j], zk = next[k], zl = next[l];
-        d2 pzi = ldpos(pos, zi), pzj = ldpos(pos, zj), pzk = ldpos(pos, zk), pzl = ldpos(pos, zl);
-        d2 r1 = wrap2(mk(pzi.x - pi.x, pzi.y - pi.y));
-        d2 r2 = wrap2(mk(pzk.x - pk.x, pzk.y - pk.y));
-        double t1 = (double)rTU[f].y, t2 = (double)rUT[f].y;
-        if (area) {   // exact crossing parameters at the current positions
-            t1 = cross_param(pi, pzi, pj, pzj, t1);
-            t2 = cross_param(pk, pzk, pl, pzl, t2);
-        }
-        d2 fij = wrap2(mk(pi.x + t1 * r1.x, pi.y + t1 * r1.y));
-        d2 fkl = wrap2(mk(pk.x + t2 * r2.x, pk.y + t2 * r2.y));
-        d2 sp = fij;
-        if (area && closed[f]) sp = ldpos(pos, polyStart[min(sig, sjg)]);   // one origin per pair
-        double e = 0.0;
-        double dfa[16], dfb[16];
-        d2 inc[8];
-        if (i == l) {
-            e += hfun(fij, fkl, sp);
-            d2 ga, gb;
-            g12(fij, fkl, sp, ga, gb);
-            getDf(pi, pzi, pj, pzj, dfa);
-            getDf(pk, pzk, pi, pzi, dfb);
-            inc[0] = dfdot(dfa, 0, ga);
-            inc[1] = dfdot(dfa, 2, ga);
-            inc[4] = dfdot(dfa, 4, ga);
-            inc[5] = dfdot(dfa, 6, ga);
-            inc[6] = dfdot(dfb, 0, gb);
-            inc[7] = dfdot(dfb, 2, gb);
-            inc[2] = dfdot(dfb, 4, gb);
-            inc[3] = dfdot(dfb, 6, gb);
-        } else {
-            e += hfun(fij, pzi, sp);
-            e += hfun(pl, fkl, sp);
-            d2 g1a, g2a;
-            g12(fij, pzi, sp, g1a, g2a);
-            getDf(pi, pzi, pj, pzj, dfa);
-            getDf(pk, pzk, pl, pzl, dfb);
-            inc[0] = dfdot(dfa, 0, g1a);
-            inc[4] = dfdot(dfa, 4, g1a);
-            d2 dd = dfdot(dfa, 2, g1a);
-            inc[1] = mk(dd.x + g2a.x, dd.y + g2a.y);
-            inc[5] = dfdot(dfa, 6, g1a);
-            d2 g1b, g2b;
-            g12(pl, fkl, sp, g1b, g2b);
-            d2 de = dfdot(dfb, 4, g2b);
-            inc[2] = mk(g1b.x + de.x, g1b.y + de.y);
-            inc[6] = dfdot(dfb, 0, g2b);
-            inc[3] = dfdot(dfb, 6, g2b);
-            inc[7] = dfdot(dfb, 2, g2b);
-        }
-        int tg[8] = {i, zi, l, zl, j, zj, k, zk};
-        double sc = area ? 0.5 : 1.0;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            tgt[f * 8 + q] = tg[q];
-            contrib[f * 8 + q] = make_double2(-inc[q].x * sc, -inc[q].y * sc);
-        }
-        eext[f] = e;
-        spOut[f] = make_double2(sp.x, sp.y);
-    }
-}
-
-// area energy: index of each record's exit record, then whether the partner links
-// close into a cycle within 8 hops (a closed lens boundary)
-__global__ void k_partner(int N, long long Icap, const int* __restrict__ bStart,
-                          const int* __restrict__ polyPack, const int* __restrict__ packPoly,
-                          const uint64_t* __restrict__ rKey, const uint64_t* __restrict__ rOuter,
-                          const int* __restrict__ rPoly, int* __restrict__ partner) {
-    long long I = bStart[N];
-    if (I > Icap) return;
-    for (long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x; f < I;
-         f += (long long)gridDim.x * blockDim.x) {
-        uint64_t key = rKey[f], o = rOuter[f];
-        int q = -1;
-        if (o != key) {
-            int s2 = packPoly[polyPack[rPoly[f]]] + key_si(o);
-            for (int r = bStart[s2]; r < bStart[s2 + 1]; ++r)
-                if (rKey[r] == o) { q = r; break; }
-        }
-        partner[f] = q;
-    }
-}
-__global__ void k_loop(int N, long long Icap, const int* __restrict__ bStart, const int* __restrict__ partner,
-                       unsigned char* __restrict__ closed) {
-    long long I = bStart[N];
-    if (I > Icap) return;
-    for (long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x; f < I;
-         f += (long long)gridDim.x * blockDim.x) {
-        int q = partner[f];
-        for (int hop = 0; hop < 8 && q >= 0 && q != (int)f; ++hop) q = partner[q];
-        closed[f] = (q == (int)f) ? 1 : 0;
-    }
-}
-
-// per-vertex force: exterior contributions of the records owned through si, then
-// through sj (record order), then the interior chain sums of this vertex's edge and
-// of the previous edge (updateForceEnergyInteriorKernel kernels.cuh:1354-1424), then
-// the edge spring (:1284-1335) and area spring (:1337-1352) terms.
-__global__ __launch_bounds__(BLK) void k_gather(
-    int V, int N, long long Icap, int overlap, int edgeOn, int areaOn, const int* __restrict__ shapeId,
-    const int* __restrict__ polyStart, const int* __restrict__ next, const int* __restrict__ prev,
-    const double2* __restrict__ pos, const int* __restrict__ bStart, const uint64_t* __restrict__ rKey,
-    const uint64_t* __restrict__ rOuter, const int* __restrict__ sjList,
-    const double2* __restrict__ contrib, const int* __restrict__ tgt, const double2* __restrict__ sp,
-    const double* __restrict__ edgeLengths, const double* __restrict__ areas,
-    const double* __restrict__ targetEdgeV, const double* __restrict__ targetAreas, double stiffness,
-    double compressibility, double2* __restrict__ force, double* __restrict__ eint,
-    double* __restrict__ eedge, int area) {
-    int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= V) return;
-    int s = shapeId[m];
-    int st = polyStart[s], n = polyStart[s + 1] - st;
-    int mLoc = m - st;
-    int nx = next[m], pm = prev[m];
-    d2 F = mk(0.0, 0.0);
-    double le = 0.0;
-    if (overlap && bStart[N] <= Icap) {
-        int I = bStart[N];
-        int lo = bStart[s], hi = bStart[s + 1];
-        for (int r = lo; r < hi; ++r) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (tgt[r * 8 + q] == m) {
-                    double2 cc = contrib[r * 8 + q];
-                    F.x += cc.x; F.y += cc.y;
-                }
-        }
-        int lo2 = bStart[N + s] - I, hi2 = bStart[N + s + 1] - I;
-        for (int e = lo2; e < hi2; ++e) {
-            int r = sjList[e];
-#pragma unroll
-            for (int q = 4; q < 8; ++q)
-                if (tgt[r * 8 + q] == m) {
-                    double2 cc = contrib[r * 8 + q];
-                    F.x += cc.x; F.y += cc.y;
-                }
-        }
-        d2 Pm = ldpos(pos, m), Pn = ldpos(pos, nx), Pp = ldpos(pos, pm);
-        int pmLoc = pm - st;
-        d2 LM = mk(0.0, 0.0), LN = mk(0.0, 0.0);
-        for (int r = lo; r < hi; ++r) {
-            uint64_t inter = rKey[r], outer = rOuter[r];
-            if (inter == outer) continue;
-            int iLoc = key_i(inter), lLoc = key_j(outer);
-            int lDist = (lLoc - iLoc + n) % n;
-            int dm = (mLoc - iLoc + n) % n;
-            int dp = (pmLoc - iLoc + n) % n;
-            bool cm = (dm > 0 && dm < lDist), cp = (dp > 0 && dp < lDist);
-            if (cm || cp) {
-                double2 s2 = sp[r];
-                d2 spr = mk(s2.x, s2.y);
-                d2 g1, g2;
-                if (cm) {
-                    le += hfun(Pm, Pn, spr);
-                    g12(Pm, Pn, spr, g1, g2);
-                    if (area) { g1.x *= 0.5; g1.y *= 0.5; }
-                    LM.x -= g1.x; LM.y -= g1.y;
-                }
-                if (cp) {
-                    g12(Pp, Pm, spr, g1, g2);
-                    if (area) { g2.x *= 0.5; g2.y *= 0.5; }
-                    LN.x -= g2.x; LN.y -= g2.y;
-                }
-            }
-        }
-        if (LM.x != 0.0 || LM.y != 0.0) { F.x += LM.x; F.y += LM.y; }
-        if (LN.x != 0.0 || LN.y != 0.0) { F.x += LN.x; F.y += LN.y; }
-    }
-    double ee = 0.0;
-    if (edgeOn) {
-        double l0 = targetEdgeV[m], l0prv = targetEdgeV[pm];
-        double l = edgeLengths[m], prvl = edgeLengths[pm];
-        d2 Pm = ldpos(pos, m), Pn = ldpos(pos, nx), Pp = ldpos(pos, pm);
-        d2 a = mk(Pm.x - Pn.x, Pm.y - Pn.y), b = mk(Pp.x - Pm.x, Pp.y - Pm.y);
-        a.x += 1.5; a.y += 1.5; b.x += 1.5; b.y += 1.5;
-        a.x = shift_into_unit(a.x); a.y = shift_into_unit(a.y);
-        b.x = shift_into_unit(b.x); b.y = shift_into_unit(b.y);
-        a.x -= 0.5; a.y -= 0.5; b.x -= 0.5; b.y -= 0.5;
-        double c1 = (l > 1e-14) ? (1.0 - l0 / l) : 0.0;
-        double c2 = (prvl > 1e-14) ? (1.0 - l0prv / prvl) : 0.0;
-        ee = stiffness * (l - l0) * (l - l0) / 2.0;
-        d2 lf;
-        lf.x = -(c1 * a.x - c2 * b.x) * stiffness;
-        lf.y = -(c1 * a.y - c2 * b.y) * stiffness;
-        if (lf.x != 0.0 || lf.y != 0.0) { F.x += lf.x; F.y += lf.y; }
-    }
-    if (areaOn) {
-        double A = areas[s], A0 = targetAreas[s];
-        if (!(A0 < 1e-14)) {
-            double coeff = 0.5 * compressibility * (A - A0);
-            d2 Pn = ldpos(pos, nx), Pp = ldpos(pos, pm);
-            double fx = -coeff * wrap(Pn.y - Pp.y);
-            double fy = -coeff * wrap(Pp.x - Pn.x);
-            F.x += fx; F.y += fy;
-        }
-    }
-    force[m] = make_double2(F.x, F.y);
-    eint[m] = le;
-    eedge[m] = ee;
-}
-
-// per-polygon energy: 0.5*(exterior of its records + interior of its edges)
-// + edge springs + area spring
-__global__ void k_poly_energy(int N, long long Icap, int overlap, int edgeOn, int areaOn,
-                              const int* __restrict__ polyStart,
-                              const int* __restrict__ bStart, const double* __restrict__ eext,
-                              const double* __restrict__ eint, const double* __restrict__ eedge,
-                              const double* __restrict__ areas, const double* __restrict__ targetAreas,
-                              double compressibility, double* __restrict__ epoly) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= N) return;
-    int st = polyStart[s], en = polyStart[s + 1];
-    double acc = 0.0;
-    if (overlap && bStart[N] <= Icap) {
-        for (int r = bStart[s]; r < bStart[s + 1]; ++r) acc += eext[r];
-        for (int m = st; m < en; ++m) acc += eint[m];
-    }
-    double e = 0.5 * acc;
-    if (edgeOn)
-        for (int m = st; m < en; ++m) e += eedge[m];
-    if (areaOn) {
-        double A = areas[s], A0 = targetAreas[s];
-        if (!(A0 < 1e-14)) e += compressibility * (A - A0) * (A - A0) / 2.0;
-    }
-    epoly[s] = e;
-}
-
 // per-packing deterministic reductions over polygons: nArrays arrays of N values
 // (stride N) are tree-summed; if nMax > 0 the last nMax arrays are max-reduced.
 // One block of 256 threads per packing.
@@ -916,48 +469,6 @@ int launch_det_reduce(pcp_ctx* c, const double* partials, int nArrays, double* o
     PCP_LAUNCH(c, k_det_reduce, dim3(c->P), dim3(BLK), c->dPackPoly, partials, c->N,
                        nArrays, nMax, outSums, c->P);
     return PCP_OK;
-}
-
-int run_forces(pcp_ctx* c, int model, const double2* pos, const double* edgeLengths, const double* areas,
-               const double* targetEdgeV, const double* targetAreas, double stiffness,
-               double compressibility, double2* force, double* energy) {
-    if (c->V == 0) return PCP_OK;
-    int area = (model & PCP_AREA_ENERGY) ? 1 : 0;
-    model &= 0xFF;
-    int overlap = (model == PCP_NORMAL || model == PCP_HYBRID);
-    int edgeOn = (model == PCP_SOFT_BODY || model == PCP_EDGE_ONLY || model == PCP_HYBRID);
-    int areaOn = (model == PCP_SOFT_BODY || model == PCP_AREA_ONLY || model == PCP_HYBRID);
-    if (model == PCP_ABNORMAL) {   // model.cpp:390-391: forces and energy zeroed only
-        hipLaunchKernelGGL(k_fill_double, dim3(nblk(2LL * c->V)), dim3(BLK), 0, c->stream, (double*)force,
-                           2LL * c->V, 0.0);
-        PCP_LAUNCHED();
-        hipLaunchKernelGGL(k_fill_double, dim3(nblk(c->P)), dim3(BLK), 0, c->stream, energy,
-                           (long long)c->P, 0.0);
-        PCP_LAUNCHED();
-        return PCP_OK;
-    }
-    if (overlap) {
-        if (area) {
-            PCP_LAUNCH(c, k_partner, dim3(record_grid(c)), dim3(BLK), c->N, c->Icap,
-                               c->dBStart, c->dPolyPack, c->dPackPoly, c->dRKey, c->dROuter, c->dRPoly,
-                               c->dPartner);
-            PCP_LAUNCH(c, k_loop, dim3(record_grid(c)), dim3(BLK), c->N, c->Icap,
-                               c->dBStart, c->dPartner, c->dClosed);
-        }
-        PCP_LAUNCH(c, k_exterior, dim3(record_grid(c)), dim3(BLK), c->N, c->Icap,
-                           c->dBStart, c->dPolyStart, c->dPolyPack, c->dPackPoly, c->dNext, pos, c->dRKey,
-                           c->dROuter, c->dRTU, c->dRUT, c->dRPoly, c->dContrib, c->dTgt, c->dEext,
-                           c->dSp, area, c->dClosed);
-    }
-    PCP_LAUNCH(c, k_gather, dim3(nblk(c->V)), dim3(BLK), c->V, c->N, c->Icap, overlap, edgeOn,
-                       areaOn, c->dShapeId, c->dPolyStart, c->dNext, c->dPrev, pos, c->dBStart, c->dRKey,
-                       c->dROuter, c->dSjList, c->dContrib, c->dTgt, c->dSp, edgeLengths, areas,
-                       targetEdgeV, targetAreas, stiffness, compressibility, force, c->dEint, c->dEedge,
-                       area);
-    PCP_LAUNCH(c, k_poly_energy, dim3(nblk(c->N)), dim3(BLK), c->N, c->Icap, overlap, edgeOn,
-                       areaOn, c->dPolyStart, c->dBStart, c->dEext, c->dEint, c->dEedge, areas, targetAreas,
-                       compressibility, c->dEpoly);
-    return launch_det_reduce(c, c->dEpoly, 1, energy, 0);
 }
 
 // ------------------------------------------------------------------ projection
@@ -1197,6 +708,7 @@ int run_shake(pcp_ctx* c, double2* pos, const double* targetEdgeV, const double*
 
 }  // namespace pcp
 #include "pcp_solve.hip"
+#include "pcp_records.hip"
 namespace pcp {
 
 // ------------------------------------------------------------------ XPBD
@@ -1455,11 +967,9 @@ int prof_end(pcp_ctx* c, int slot) {
     return PCP_OK;
 }
 
-// records whether the evaluation just enqueued overflowed one of its buffers
-__global__ void k_sticky(const int* __restrict__ scal, const int* __restrict__ numRecords, int K,
-                         long long Icap, int* __restrict__ sticky) {
-    if (threadIdx.x == 0 && (scal[SC_MAXCOUNT] > K || *numRecords > Icap))
-        sticky[0] = 1;
+// records whether the evaluation just enqueued overflowed a record bucket
+__global__ void k_sticky(const int* __restrict__ scal, int Kp, int* __restrict__ sticky) {
+    if (threadIdx.x == 0 && scal[SC_BUCKET] > Kp) sticky[0] = 1;
 }
 
 __global__ void k_copy(long long n, double* __restrict__ dst, const double* __restrict__ src) {
@@ -1511,19 +1021,17 @@ static void free_topology(pcp_ctx* c) {
     dfree(c->dNext); dfree(c->dPrev); dfree(c->dPolyPack); dfree(c->dBox); dfree(c->dCellOff);
     dfree(c->dCellOf); dfree(c->dCellCount); dfree(c->dCellStart); dfree(c->dCellTmp);
     dfree(c->dCellSorted); dfree(c->dSP); dfree(c->dSR); dfree(c->dSI);
-    dfree(c->dSlotKey); dfree(c->dSlotTU); dfree(c->dSlotIn); dfree(c->dNum); dfree(c->dCounts);
-    dfree(c->dBStart); dfree(c->dPackMaxKey); dfree(c->dScal);
-    dfree(c->dKeys2); dfree(c->dBTU); dfree(c->dBPoly); dfree(c->dRKey); dfree(c->dROuter);
-    dfree(c->dRTU); dfree(c->dRUT); dfree(c->dRPoly); dfree(c->dSjList); dfree(c->dContrib);
-    dfree(c->dTgt); dfree(c->dEext); dfree(c->dSp);
-    dfree(c->dEint); dfree(c->dEedge); dfree(c->dEpoly); dfree(c->dWork); dfree(c->dRed);
+    dfree(c->dNum); dfree(c->dPackMaxKey); dfree(c->dScal);
+    dfree(c->dBcnt); dfree(c->dJcnt); dfree(c->dBKey); dfree(c->dBTU); dfree(c->dJKey); dfree(c->dRKey);
+    dfree(c->dROuter); dfree(c->dRTU); dfree(c->dRUT); dfree(c->dJC); dfree(c->dJT);
+    dfree(c->dF1); dfree(c->dLM); dfree(c->dLN); dfree(c->dAccR);
+    dfree(c->dEpoly); dfree(c->dWork); dfree(c->dRed);
     dfree(c->dRedOut); dfree(c->dScanPart); c->scanPartCap = 0;
     dfree(c->dXbak); dfree(c->dXpre); dfree(c->dVbak); dfree(c->dDt); dfree(c->dAlpha);
     dfree(c->dEpre); dfree(c->dNPos); dfree(c->dSteps); dfree(c->dActive); dfree(c->dFlags);
     dfree(c->dFireScal); dfree(c->dShakeIt); dfree(c->dCellInv); dfree(c->dCom); dfree(c->dStatus);
-    dfree(c->dSummary); dfree(c->dPartner); dfree(c->dClosed);
+    dfree(c->dSummary);
     dfree(c->dBounds); dfree(c->dPcSorted); dfree(c->dPcStart); dfree(c->dPackB); dfree(c->dSticky);
-    c->Icap = 0;
     c->haveBoxes = false;
 }
 
@@ -1552,14 +1060,11 @@ int pcp_synchronize(pcp_ctx* c) {
     int sticky = 0;
     PCP_HIP(hipMemcpy(&sticky, c->dSticky, sizeof(int), hipMemcpyDeviceToHost));
     if (!sticky) return PCP_OK;
-    // a fused evaluation since the last check ran out of crossing slots or records: grow
-    // what overflowed and report it, so that the caller evaluates again
+    // a fused evaluation since the last check overflowed a record bucket: enlarge the
+    // buckets and report it, so that the caller evaluates again
     int scal[SC_NSCAL];
-    int I = 0;
     PCP_HIP(hipMemcpy(scal, c->dScal, sizeof(int) * SC_NSCAL, hipMemcpyDeviceToHost));
-    PCP_HIP(hipMemcpy(&I, c->dBStart + c->N, sizeof(int), hipMemcpyDeviceToHost));
-    PCP_TRY(pcp::grow_slots(c, std::max(scal[SC_MAXCOUNT], 2 * c->K)));
-    PCP_TRY(pcp::ensure_records(c, std::max<long long>((long long)I + I / 4, c->Icap + 1)));
+    PCP_TRY(pcp::grow_buckets(c, std::max(scal[SC_BUCKET], c->Kp + 1)));
     sticky = 0;
     PCP_HIP(hipMemcpy(c->dSticky, &sticky, sizeof(int), hipMemcpyHostToDevice));
     pcp::set_error("crossing buffers overflowed in an evaluation since the last synchronisation; "
@@ -1569,31 +1074,6 @@ int pcp_synchronize(pcp_ctx* c) {
 
 }  // extern "C"
 
-namespace pcp {
-int ensure_records(pcp_ctx* c, long long need) {
-    if (need <= c->Icap) return PCP_OK;
-    long long cap = std::max<long long>(need, 2 * c->Icap);
-    dfree(c->dKeys2); dfree(c->dBTU); dfree(c->dBPoly); dfree(c->dRKey); dfree(c->dROuter);
-    dfree(c->dRTU); dfree(c->dRUT); dfree(c->dRPoly); dfree(c->dSjList); dfree(c->dContrib);
-    dfree(c->dTgt); dfree(c->dEext); dfree(c->dSp);
-    PCP_TRY(dalloc(&c->dKeys2, 2 * cap)); PCP_TRY(dalloc(&c->dBTU, cap)); PCP_TRY(dalloc(&c->dBPoly, cap));
-    PCP_TRY(dalloc(&c->dRKey, cap)); PCP_TRY(dalloc(&c->dROuter, cap)); PCP_TRY(dalloc(&c->dRTU, cap));
-    PCP_TRY(dalloc(&c->dRUT, cap)); PCP_TRY(dalloc(&c->dRPoly, cap)); PCP_TRY(dalloc(&c->dSjList, cap));
-    PCP_TRY(dalloc(&c->dContrib, 8 * cap)); PCP_TRY(dalloc(&c->dTgt, 8 * cap));
-    PCP_TRY(dalloc(&c->dEext, cap)); PCP_TRY(dalloc(&c->dSp, cap));
-    dfree(c->dPartner); dfree(c->dClosed);
-    PCP_TRY(dalloc(&c->dPartner, cap)); PCP_TRY(dalloc(&c->dClosed, cap));
-    c->Icap = cap;
-    return PCP_OK;
-}
-int grow_slots(pcp_ctx* c, int K) {
-    dfree(c->dSlotKey); dfree(c->dSlotTU); dfree(c->dSlotIn);
-    long long n = (long long)c->V * K;
-    PCP_TRY(dalloc(&c->dSlotKey, n)); PCP_TRY(dalloc(&c->dSlotTU, n)); PCP_TRY(dalloc(&c->dSlotIn, n));
-    c->K = K;
-    return PCP_OK;
-}
-}  // namespace pcp
 
 extern "C" {
 
@@ -1621,7 +1101,6 @@ int pcp_set_topology(pcp_ctx* c, int P, const int* packPolyStart, const int* pol
     c->hPolyStart.assign(polyStart, polyStart + N + 1);
     c->hPackVert.resize(P + 1);
     for (int p = 0; p <= P; ++p) c->hPackVert[p] = polyStart[packPolyStart[p]];
-    c->lastI = 0;
     PCP_TRY(dalloc(&c->dPackPoly, P + 1)); PCP_TRY(dalloc(&c->dPackVert, P + 1));
     PCP_TRY(dalloc(&c->dPolyStart, N + 1)); PCP_TRY(dalloc(&c->dShapeId, V));
     PCP_TRY(dalloc(&c->dNext, V)); PCP_TRY(dalloc(&c->dPrev, V)); PCP_TRY(dalloc(&c->dPolyPack, N));
@@ -1637,15 +1116,16 @@ int pcp_set_topology(pcp_ctx* c, int P, const int* packPolyStart, const int* pol
     }
     PCP_TRY(dalloc(&c->dCellOf, V)); PCP_TRY(dalloc(&c->dCellTmp, V)); PCP_TRY(dalloc(&c->dCellSorted, V));
     PCP_TRY(dalloc(&c->dSP, V)); PCP_TRY(dalloc(&c->dSR, V)); PCP_TRY(dalloc(&c->dSI, V));
-    PCP_TRY(dalloc(&c->dNum, V)); PCP_TRY(dalloc(&c->dCounts, 2LL * N)); PCP_TRY(dalloc(&c->dBStart, 2LL * N + 1));
+    PCP_TRY(dalloc(&c->dNum, V));
     PCP_TRY(dalloc(&c->dPackMaxKey, P)); PCP_TRY(dalloc(&c->dScal, SC_NSCAL));
-    PCP_HIP(hipMemsetAsync(c->dCounts, 0, sizeof(int) * 2 * (size_t)std::max(N, 1), c->stream));
-    PCP_HIP(hipMemsetAsync(c->dBStart, 0, sizeof(int) * (2 * (size_t)N + 1), c->stream));
     PCP_HIP(hipMemsetAsync(c->dScal, 0, sizeof(int) * SC_NSCAL, c->stream));
     PCP_HIP(hipMemsetAsync(c->dNum, 0, sizeof(int) * (size_t)std::max(V, 1), c->stream));
-    PCP_TRY(grow_slots(c, c->K));
-    PCP_TRY(ensure_records(c, std::max<long long>(8LL * V, 65536)));
-    PCP_TRY(dalloc(&c->dEint, V)); PCP_TRY(dalloc(&c->dEedge, V)); PCP_TRY(dalloc(&c->dEpoly, N));
+    PCP_TRY(dalloc(&c->dBcnt, N)); PCP_TRY(dalloc(&c->dJcnt, N));
+    PCP_HIP(hipMemsetAsync(c->dBcnt, 0, sizeof(int) * (size_t)std::max(N, 1), c->stream));
+    PCP_HIP(hipMemsetAsync(c->dJcnt, 0, sizeof(int) * (size_t)std::max(N, 1), c->stream));
+    PCP_TRY(alloc_buckets(c, 64));
+    PCP_TRY(dalloc(&c->dF1, V)); PCP_TRY(dalloc(&c->dLM, V)); PCP_TRY(dalloc(&c->dLN, V));
+    PCP_TRY(dalloc(&c->dAccR, N)); PCP_TRY(dalloc(&c->dEpoly, N));
     PCP_TRY(dalloc(&c->dWork, 8LL * V)); PCP_TRY(dalloc(&c->dRed, 4LL * N)); PCP_TRY(dalloc(&c->dRedOut, 4LL * P));
     PCP_TRY(dalloc(&c->dXbak, 2LL * V)); PCP_TRY(dalloc(&c->dXpre, 2LL * V)); PCP_TRY(dalloc(&c->dVbak, 2LL * V));
     PCP_TRY(dalloc(&c->dDt, P)); PCP_TRY(dalloc(&c->dAlpha, P)); PCP_TRY(dalloc(&c->dEpre, P));
@@ -1702,10 +1182,10 @@ int pcp_update_cells(pcp_ctx* c, const double* pos) {
     return run_cells(c, (const double2*)pos);
 }
 
-// detect (cells of the last cell-list build); grow the slots when they overflowed and
-// repeat.  *maxCount = largest crossing count of one edge.
+// detect (cells of the last cell-list build); grow the record buckets when they
+// overflowed and repeat.  *maxCount = largest crossing count of one edge.
 static int detect_checked(pcp_ctx* c, const double2* pos, int* maxCount) {
-    for (int attempt = 0; attempt < 4; ++attempt) {
+    for (int attempt = 0; attempt < 6; ++attempt) {
         PCP_TRY(run_detect(c, pos, 1));
         if (c->V > 0) {
             hipLaunchKernelGGL(k_num_max, dim3(nblk(c->V)), dim3(BLK), 0, c->stream, c->V, c->dNum,
@@ -1715,14 +1195,14 @@ static int detect_checked(pcp_ctx* c, const double2* pos, int* maxCount) {
         int scal[SC_NSCAL];
         PCP_HIP(hipMemcpyAsync(scal, c->dScal, sizeof(int) * SC_NSCAL, hipMemcpyDeviceToHost, c->stream));
         PCP_HIP(hipStreamSynchronize(c->stream));
-        bool again = false;
-        if (scal[SC_MAXCOUNT] > c->K) { PCP_TRY(grow_slots(c, std::max(scal[SC_MAXCOUNT], 2 * c->K))); again = true; }
-        if (!again) {
-            if (maxCount) *maxCount = scal[SC_MAXCOUNT];
-            return PCP_OK;
+        if (scal[SC_BUCKET] > c->Kp) {
+            PCP_TRY(grow_buckets(c, scal[SC_BUCKET]));
+            continue;
         }
+        if (maxCount) *maxCount = scal[SC_MAXCOUNT];
+        return PCP_OK;
     }
-    set_error("pcp_update_neighbors: crossing buffers kept overflowing");
+    set_error("pcp_update_neighbors: record buckets kept overflowing");
     return PCP_ERR_CAPACITY;
 }
 int pcp_update_neighbors(pcp_ctx* c, const double* pos, int* maxCount) {
@@ -1735,35 +1215,26 @@ int pcp_update_neighbors(pcp_ctx* c, const double* pos, int* maxCount) {
     return detect_checked(c, (const double2*)pos, maxCount);
 }
 
-// records; grow the record arrays when the crossings exceed their capacity
-static int records_checked(pcp_ctx* c, const double2* pairPos) {
-    PCP_TRY(run_records(c, pairPos));
-    int I = 0;
-    PCP_HIP(hipMemcpyAsync(&I, c->dBStart + c->N, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    PCP_HIP(hipStreamSynchronize(c->stream));
-    c->lastI = I;
-    if (I > c->Icap) {   // grow and rebuild (the counts are recomputed from the slots)
-        PCP_TRY(ensure_records(c, (long long)I + I / 4));
-        PCP_TRY(run_records(c, pairPos));
+int pcp_update_records(pcp_ctx* c, const double* pairPos, int* numIntersections) {
+    PCP_TRY(run_records(c, (const double2*)pairPos));
+    if (numIntersections) {
+        PCP_HIP(hipStreamSynchronize(c->stream));
+        std::vector<int> cnt;
+        for (int p = 0; p < c->P; ++p) {
+            PCP_TRY(packing_counts(c, p, cnt));
+            long long I = 0;
+            for (int x : cnt) I += x;
+            numIntersections[p] = (int)I;
+        }
     }
     return PCP_OK;
-}
-int pcp_update_records(pcp_ctx* c, const double* pairPos, int* numIntersections) {
-    if (numIntersections) {
-        PCP_TRY(records_checked(c, (const double2*)pairPos));
-        std::vector<int> bs(c->N + 1);
-        PCP_HIP(hipMemcpy(bs.data(), c->dBStart, sizeof(int) * (c->N + 1), hipMemcpyDeviceToHost));
-        for (int p = 0; p < c->P; ++p) numIntersections[p] = bs[c->hPackPoly[p + 1]] - bs[c->hPackPoly[p]];
-        return PCP_OK;
-    }
-    return run_records(c, (const double2*)pairPos);
 }
 
 int pcp_update_force_energy(pcp_ctx* c, int model, const double* pos, const double* edgeLengths,
                             const double* areas, const double* targetEdgeV, const double* targetAreas,
                             double stiffness, double compressibility, double* force, double* energy) {
-    return run_forces(c, model, (const double2*)pos, edgeLengths, areas, targetEdgeV, targetAreas, stiffness,
-                      compressibility, (double2*)force, energy);
+    return run_forces(c, model, (const double2*)pos, (double*)edgeLengths, (double*)areas, targetEdgeV,
+                      targetAreas, stiffness, compressibility, (double2*)force, energy, 0, 0);
 }
 int pcp_project_force(pcp_ctx* c, const double* pos, double* force) {
     return run_project(c, (const double2*)pos, (double2*)force);
@@ -1878,33 +1349,40 @@ int pcp_get_cells(pcp_ctx* c, int p, int* cellLocation, int* neighborIndices, in
 int pcp_get_num_intersections(pcp_ctx* c, int p, int* count) {
     if (p < 0 || p >= c->P) return fail_arg("packing out of range");
     PCP_HIP(hipStreamSynchronize(c->stream));
-    int a = 0, b = 0;
-    PCP_HIP(hipMemcpy(&a, c->dBStart + c->hPackPoly[p], sizeof(int), hipMemcpyDeviceToHost));
-    PCP_HIP(hipMemcpy(&b, c->dBStart + c->hPackPoly[p + 1], sizeof(int), hipMemcpyDeviceToHost));
-    *count = b - a;
+    std::vector<int> cnt;
+    PCP_TRY(packing_counts(c, p, cnt));
+    long long I = 0;
+    for (int x : cnt) I += x;
+    *count = (int)I;
     return PCP_OK;
 }
+// records of packing p in the reference's final order: polygon by polygon (si), each
+// polygon's records by (i, j, sj)
 int pcp_get_records(pcp_ctx* c, int p, uint64_t* inter, double* tu, uint64_t* outer, double* ut,
                     int* shapeCounts) {
     if (p < 0 || p >= c->P) return fail_arg("packing out of range");
     PCP_HIP(hipStreamSynchronize(c->stream));
     int n0 = c->hPackPoly[p], n1 = c->hPackPoly[p + 1];
-    std::vector<int> bs(n1 - n0 + 1);
-    PCP_HIP(hipMemcpy(bs.data(), c->dBStart + n0, sizeof(int) * (n1 - n0 + 1), hipMemcpyDeviceToHost));
-    int a = bs[0], I = bs[n1 - n0] - a;
-    if (shapeCounts) for (int s = 0; s < n1 - n0; ++s) shapeCounts[s] = bs[s + 1] - bs[s];
-    if (I == 0) return PCP_OK;
-    if (inter) PCP_HIP(hipMemcpy(inter, c->dRKey + a, sizeof(uint64_t) * I, hipMemcpyDeviceToHost));
-    if (outer) PCP_HIP(hipMemcpy(outer, c->dROuter + a, sizeof(uint64_t) * I, hipMemcpyDeviceToHost));
-    std::vector<float2> tmp(I);
-    if (tu) {
-        PCP_HIP(hipMemcpy(tmp.data(), c->dRTU + a, sizeof(float2) * I, hipMemcpyDeviceToHost));
-        for (int r = 0; r < I; ++r) { tu[2 * r] = tmp[r].x; tu[2 * r + 1] = tmp[r].y; }
-    }
-    if (ut) {
-        PCP_HIP(hipMemcpy(tmp.data(), c->dRUT + a, sizeof(float2) * I, hipMemcpyDeviceToHost));
-        for (int r = 0; r < I; ++r) { ut[2 * r] = tmp[r].x; ut[2 * r + 1] = tmp[r].y; }
-    }
+    std::vector<int> cnt;
+    PCP_TRY(packing_counts(c, p, cnt));
+    if (shapeCounts) for (int s = 0; s < n1 - n0; ++s) shapeCounts[s] = cnt[s];
+    if (n1 == n0) return PCP_OK;
+    size_t rows = (size_t)(n1 - n0) * c->Kp, off = (size_t)n0 * c->Kp;
+    std::vector<uint64_t> k1(rows), k2(rows);
+    std::vector<float2> t1(rows), t2(rows);
+    PCP_HIP(hipMemcpy(k1.data(), c->dRKey + off, sizeof(uint64_t) * rows, hipMemcpyDeviceToHost));
+    PCP_HIP(hipMemcpy(k2.data(), c->dROuter + off, sizeof(uint64_t) * rows, hipMemcpyDeviceToHost));
+    PCP_HIP(hipMemcpy(t1.data(), c->dRTU + off, sizeof(float2) * rows, hipMemcpyDeviceToHost));
+    PCP_HIP(hipMemcpy(t2.data(), c->dRUT + off, sizeof(float2) * rows, hipMemcpyDeviceToHost));
+    long long w = 0;
+    for (int s = 0; s < n1 - n0; ++s)
+        for (int f = 0; f < cnt[s]; ++f, ++w) {
+            size_t r = (size_t)s * c->Kp + f;
+            if (inter) inter[w] = k1[r];
+            if (outer) outer[w] = k2[r];
+            if (tu) { tu[2 * w] = t1[r].x; tu[2 * w + 1] = t1[r].y; }
+            if (ut) { ut[2 * w] = t2[r].x; ut[2 * w + 1] = t2[r].y; }
+        }
     return PCP_OK;
 }
 int pcp_get_neighbor_lists(pcp_ctx* c, int p, int* neighbors, unsigned char* inside, int* numNeighbors) {
@@ -1932,48 +1410,46 @@ int pcp_get_compacted_records(pcp_ctx* c, int p, uint64_t* inter, double* tu) {
     if (p < 0 || p >= c->P) return fail_arg("packing out of range");
     PCP_HIP(hipStreamSynchronize(c->stream));
     int v0 = c->hPackVert[p], v1 = c->hPackVert[p + 1], nv = v1 - v0;
-    int B = c->hBox[p], pbase = c->hPackPoly[p];
-    std::vector<int> num(nv), cell(nv);
-    PCP_HIP(hipMemcpy(num.data(), c->dNum + v0, sizeof(int) * nv, hipMemcpyDeviceToHost));
-    PCP_HIP(hipMemcpy(cell.data(), c->dCellOf + v0, sizeof(int) * nv, hipMemcpyDeviceToHost));
+    int n0 = c->hPackPoly[p], n1 = c->hPackPoly[p + 1];
+    int B = c->hBox[p];
+    std::vector<int> cnt, cell(nv);
+    PCP_TRY(packing_counts(c, p, cnt));
+    if (nv) PCP_HIP(hipMemcpy(cell.data(), c->dCellOf + v0, sizeof(int) * nv, hipMemcpyDeviceToHost));
+    size_t rows = (size_t)(n1 - n0) * c->Kp, off = (size_t)n0 * c->Kp;
+    std::vector<uint64_t> keys(rows);
+    std::vector<float2> tus(rows);
+    if (rows) {
+        PCP_HIP(hipMemcpy(keys.data(), c->dBKey + off, sizeof(uint64_t) * rows, hipMemcpyDeviceToHost));
+        PCP_HIP(hipMemcpy(tus.data(), c->dBTU + off, sizeof(float2) * rows, hipMemcpyDeviceToHost));
+    }
     // compaction order of updateCompactedIntersectionsKernel (kernels.cuh:838-877): edge by
-    // edge, each edge's crossings in the order updateNeighborsKernel found them -- 3x3
-    // stencil cells in (dx, dy) order, a cell counted once, ascending vertex id inside a
-    // cell.  The detection stores them unordered; they are put in that order here.
-    std::vector<uint64_t> keys;
-    std::vector<float2> tus;
-    std::vector<unsigned char> ins;
-    std::vector<std::pair<long long, int>> ord;
-    long long w = 0;
-    for (int v = v0; v < v1; ++v) {
-        int cnt = std::min(num[v - v0], c->K);
-        if (cnt == 0) continue;
-        keys.resize(cnt); tus.resize(cnt); ins.resize(cnt);
-        long long base = (long long)v * c->K;
-        PCP_HIP(hipMemcpy(keys.data(), c->dSlotKey + base, sizeof(uint64_t) * cnt, hipMemcpyDeviceToHost));
-        PCP_HIP(hipMemcpy(tus.data(), c->dSlotTU + base, sizeof(float2) * cnt, hipMemcpyDeviceToHost));
-        PCP_HIP(hipMemcpy(ins.data(), c->dSlotIn + base, cnt, hipMemcpyDeviceToHost));
-        int cl = cell[v - v0], bx = cl % B, by = cl / B;
-        ord.clear();
-        for (int q = 0; q < cnt; ++q) {
-            uint64_t k = keys[q];
-            int other = ins[q] ? (int)((k >> 32) & 0xFFFF) : (int)(k >> 48);
-            int oedge = ins[q] ? (int)((k >> 16) & 0xFFFF) : (int)(k & 0xFFFF);
-            int nid = c->hPolyStart[pbase + other] + oedge;
-            int oc = cell[nid - v0];
+    // edge (the edge with the lower vertex id of each crossing), each edge's crossings in
+    // the order updateNeighborsKernel found them -- 3x3 stencil cells in (dx, dy) order, a
+    // cell counted once, ascending vertex id inside a cell.
+    struct Item { int id1; long long ord; uint64_t key; float2 tu; };
+    std::vector<Item> items;
+    const int* ps = c->hPolyStart.data();
+    for (int s = 0; s < n1 - n0; ++s)
+        for (int f = 0; f < cnt[s]; ++f) {
+            uint64_t k = keys[(size_t)s * c->Kp + f];
+            int sj = (int)(k >> 48), si = (int)((k >> 32) & 0xFFFF);
+            int vi = ps[n0 + si] + (int)((k >> 16) & 0xFFFF) - v0;   // edge i of si
+            int vj = ps[n0 + sj] + (int)(k & 0xFFFF) - v0;           // edge j of sj
+            int id1 = std::min(vi, vj), nid = std::max(vi, vj);
+            int bx = cell[id1] % B, by = cell[id1] / B, oc = cell[nid];
             int first = 9;
             for (int kc = 0; kc < 9 && first == 9; ++kc) {
                 int nx = (bx + kc / 3 - 1 + B) % B, ny = (by + kc % 3 - 1 + B) % B;
                 if (ny * B + nx == oc) first = kc;
             }
-            ord.push_back({(long long)first * (1LL << 32) + nid, q});
+            items.push_back({id1, (long long)first * (1LL << 32) + nid, k, tus[(size_t)s * c->Kp + f]});
         }
-        std::sort(ord.begin(), ord.end());
-        for (auto& o : ord) {
-            if (inter) inter[w] = keys[o.second];
-            if (tu) { tu[2 * w] = tus[o.second].x; tu[2 * w + 1] = tus[o.second].y; }
-            ++w;
-        }
+    std::sort(items.begin(), items.end(), [](const Item& a, const Item& b) {
+        return a.id1 != b.id1 ? a.id1 < b.id1 : a.ord < b.ord;
+    });
+    for (size_t w = 0; w < items.size(); ++w) {
+        if (inter) inter[w] = items[w].key;
+        if (tu) { tu[2 * w] = items[w].tu.x; tu[2 * w + 1] = items[w].tu.y; }
     }
     return PCP_OK;
 }
@@ -2024,19 +1500,19 @@ int pcp_sort_keys(const uint64_t* keys, int n, int endBit, uint64_t* sorted, uin
 int pcp_force_eval(pcp_ctx* c, const pcp_state* st) {
     const double2* pos = (const double2*)st->pos;
     if (!c->haveBoxes) return fail_arg("pcp_force_eval: set the box sizes first");
-    PCP_TRY(run_geometry(c, pos, st->edgeLengths, st->areas, c->dCom, c->dCom + c->N, nullptr, nullptr));
     int mt = st->model & 0xFF;
     int overlap = (mt == PCP_NORMAL || mt == PCP_HYBRID);
     if (overlap) {
         PCP_TRY(run_detect(c, pos, 0));
         PCP_TRY(run_records(c, (st->model & PCP_AREA_ENERGY) ? pos : nullptr));
-        hipLaunchKernelGGL(k_sticky, dim3(1), dim3(64), 0, c->stream, c->dScal, c->dBStart + c->N, c->K,
-                           c->Icap, c->dSticky);
-        PCP_LAUNCHED();
     }
     PCP_TRY(run_forces(c, st->model, pos, st->edgeLengths, st->areas, st->targetEdgeV, st->targetAreas,
-                       st->stiffness, st->compressibility, (double2*)st->force, st->energy));
-    if (mt == PCP_NORMAL) PCP_TRY(run_project(c, pos, (double2*)st->force));
+                       st->stiffness, st->compressibility, (double2*)st->force, st->energy, 1,
+                       mt == PCP_NORMAL));
+    if (overlap) {
+        hipLaunchKernelGGL(k_sticky, dim3(1), dim3(64), 0, c->stream, c->dScal, c->Kp, c->dSticky);
+        PCP_LAUNCHED();
+    }
     return PCP_OK;
 }
 

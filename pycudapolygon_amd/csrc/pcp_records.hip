@@ -1,0 +1,573 @@
+// Records, pairing and forces, one wavefront (or lane segment) per polygon (part of the
+// pcp_kernels.hip translation unit).
+//
+// The detection appends every crossing to two per-polygon buckets of capacity Kp: the
+// si bucket of the polygon the record belongs to (key, float t/u) and the sj bucket of
+// the other polygon (key only).  Then:
+//   k_pair_poly      wavefront / polygon s: rank of each record of s in the order
+//                    (i, j, sj) -- the order left by the reference's 64-bit then stable
+//                    48-bit radix sorts (model.cpp:354-360) -- and its exit record from
+//                    the si bucket of sj (updateOutersectionsKernel kernels.cuh:981-1065);
+//                    the sorted records of s are written to s's record rows
+//   k_exterior_poly  wavefront / polygon s: the 8 exterior contributions and the energy of
+//                    each record (updateForceEnergyExteriorKernel kernels.cuh:1067-1256);
+//                    the 4 acting on s are summed per vertex in record order, the 4 acting
+//                    on sj are written at the record's rank in sj's list (key order); the
+//                    interior chain terms (updateForceEnergyInteriorKernel
+//                    kernels.cuh:1354-1424) per vertex of s
+//   k_gather_poly    lane segment / polygon: per vertex the sum over s's records, then the
+//                    sj-side contributions in key order, the interior terms, the edge and
+//                    area springs (kernels.cuh:1284-1352); the polygon energy; in the fused
+//                    evaluation also edge lengths, area and the force projection
+// The order of every floating-point sum is fixed (the CPU oracle uses the same), so
+// forces and energies are bit-reproducible.
+
+namespace pcp {
+
+constexpr int REC_MAX_KP = 512;   // LDS of k_exterior_poly: Kp * 120 bytes per wavefront
+
+// ---------------------------------------------------------------- pairing
+// The reference's binary search starts with end = I-1, so when every key of the packing
+// lies below (si<<48 | sj<<32) the last record of the packing is the only candidate;
+// kept.  EXACT: crossing parameters recomputed from pairPos (area energy).
+template <int EXACT>
+__global__ __launch_bounds__(64) void k_pair_poly(int N, int Kp, const int* __restrict__ polyStart,
+                                                 const int* __restrict__ polyPack, const int* __restrict__ packPoly,
+                                                 const int* __restrict__ bcnt, const uint64_t* __restrict__ bKey,
+                                                 const float2* __restrict__ bTU,
+                                                 const unsigned long long* __restrict__ packMaxKey,
+                                                 const double2* __restrict__ pairPos, uint64_t* __restrict__ rKey,
+                                                 float2* __restrict__ rTU, uint64_t* __restrict__ rOuter,
+                                                 float2* __restrict__ rUT) {
+    int lane = threadIdx.x;
+    int s = blockIdx.x;
+    if (s >= N) return;
+    int cnt = min(bcnt[s], Kp);
+    if (cnt == 0) return;
+    int pk = polyPack[s];
+    int pbase = packPoly[pk];
+    int st1 = polyStart[s], ni = polyStart[s + 1] - st1;
+    uint64_t mxk = packMaxKey[pk];
+    long long base = (long long)s * Kp;
+    for (int f = lane; f < cnt; f += 64) {
+        uint64_t key = bKey[base + f];
+        float2 tu = bTU[base + f];
+        uint64_t ord = bucket_order(key);
+        int rank = 0;
+        for (int q = 0; q < cnt; ++q) rank += (bucket_order(bKey[base + q]) < ord);
+        int si_l = key_si(key), sj_l = key_sj(key), i = key_i(key);
+        int sjg = pbase + sj_l;
+        int st2 = polyStart[sjg], nj = polyStart[sjg + 1] - st2;
+        // exact parameter along s's edge ei of its crossing with sj's edge ej (area energy)
+        auto exact = [&](int ei, int ej, double fb) -> double {
+            int a = st1 + ei, b = st1 + ((ei + 1 == ni) ? 0 : ei + 1);
+            int c2 = st2 + ej, d = st2 + ((ej + 1 == nj) ? 0 : ej + 1);
+            return cross_param(ldpos(pairPos, a), ldpos(pairPos, b), ldpos(pairPos, c2), ldpos(pairPos, d), fb);
+        };
+        double tVal = (double)tu.y;
+        if (EXACT) tVal = exact(i, key_j(key), tVal);
+        uint64_t sij = ((uint64_t)si_l << 48) | ((uint64_t)sj_l << 32);
+        int bestDist = -1;
+        bool found = false;
+        double bestU = 0.0;
+        uint64_t bestOrd = 0, bestKey = 0;
+        float2 bestTU = make_float2(0.f, 0.f);
+        auto consider = [&](uint64_t k2, float2 tu2) {
+            int l = key_j(k2);
+            double uVal = (double)tu2.x;
+            if (EXACT && mxk >= sij) uVal = exact(l, key_i(k2), uVal);
+            int d = (l - i + ni) % ni;
+            if (l == i && tVal >= uVal) return;
+            if (bestDist == -1 || d < bestDist) { bestDist = d; found = false; }
+            if (d == bestDist && uVal == uVal) {
+                uint64_t o2 = bucket_order(k2);
+                if (!found || uVal < bestU || (uVal == bestU && o2 < bestOrd)) {
+                    found = true; bestU = uVal; bestOrd = o2; bestKey = k2; bestTU = tu2;
+                }
+            }
+        };
+        if (mxk < sij) {
+            int s3 = pbase + key_si(mxk);
+            long long b3 = (long long)s3 * Kp;
+            int c3 = min(bcnt[s3], Kp);
+            for (int q = 0; q < c3; ++q)
+                if (bKey[b3 + q] == mxk) { consider(mxk, bTU[b3 + q]); break; }
+        } else {
+            long long b2 = (long long)sjg * Kp;
+            int c2 = min(bcnt[sjg], Kp);
+            for (int q = 0; q < c2; ++q) {
+                uint64_t k2 = bKey[b2 + q];
+                if (key_sj(k2) != si_l) continue;
+                consider(k2, bTU[b2 + q]);
+            }
+        }
+        // a partner whose edge indices fall outside the two polygons (possible only in the
+        // last-record case above) is treated as no partner: the reference would read
+        // vertices of other polygons there
+        if (found && (key_i(bestKey) >= nj || key_j(bestKey) >= ni)) found = false;
+        rKey[base + rank] = key;
+        rTU[base + rank] = tu;
+        rOuter[base + rank] = found ? bestKey : key;
+        rUT[base + rank] = found ? bestTU : tu;
+    }
+}
+
+// ---------------------------------------------------------------- exterior + interior
+struct ExtLds {
+    double2* c;    // Kp*4 contributions acting on s
+    int* t;        // Kp*4 their vertex (local index in s, -1 none)
+    double2* sp;   // Kp start points
+    double* e;     // Kp record energies
+    int* il;       // Kp: iLoc | lLoc << 16, -1 for a record without partner
+};
+__device__ __forceinline__ ExtLds ext_lds(char* smem, int Kp) {
+    ExtLds L;
+    L.c = (double2*)smem;
+    L.sp = L.c + 4 * Kp;
+    L.e = (double*)(L.sp + Kp);
+    L.t = (int*)(L.e + Kp);
+    L.il = L.t + 4 * Kp;
+    return L;
+}
+size_t ext_lds_bytes(int Kp) { return (size_t)Kp * (64 + 16 + 8 + 16 + 4); }
+
+// the record of the packing whose key is `key`: its outer key (false if absent)
+__device__ __forceinline__ bool find_record(uint64_t key, int pbase, int Kp, const int* __restrict__ bcnt,
+                                            const uint64_t* __restrict__ rKey,
+                                            const uint64_t* __restrict__ rOuter, uint64_t* outer) {
+    int s2 = pbase + key_si(key);
+    long long b2 = (long long)s2 * Kp;
+    int c2 = min(bcnt[s2], Kp);
+    for (int q = 0; q < c2; ++q)
+        if (rKey[b2 + q] == key) { *outer = rOuter[b2 + q]; return true; }
+    return false;
+}
+
+template <int AREA>
+__global__ __launch_bounds__(64) void k_exterior_poly(
+    int N, int Kp, const int* __restrict__ polyStart, const int* __restrict__ polyPack,
+    const int* __restrict__ packPoly, const double2* __restrict__ pos, const int* __restrict__ bcnt,
+    const int* __restrict__ jcnt, const uint64_t* __restrict__ jKey, const uint64_t* __restrict__ rKey,
+    const uint64_t* __restrict__ rOuter, const float2* __restrict__ rTU, const float2* __restrict__ rUT,
+    double2* __restrict__ jC, int* __restrict__ jT, double2* __restrict__ F1, double2* __restrict__ LMo,
+    double2* __restrict__ LNo, double* __restrict__ accR) {
+    extern __shared__ char smem[];
+    __shared__ double leBuf[64];
+    ExtLds L = ext_lds(smem, Kp);
+    int lane = threadIdx.x;
+    int s = blockIdx.x;
+    if (s >= N) return;
+    int cnt = min(bcnt[s], Kp);
+    int pk = polyPack[s];
+    int pbase = packPoly[pk];
+    int st1 = polyStart[s], n1 = polyStart[s + 1] - st1;
+    long long base = (long long)s * Kp;
+    const double sc = AREA ? 0.5 : 1.0;
+    for (int f = lane; f < cnt; f += 64) {
+        uint64_t inter = rKey[base + f], outer = rOuter[base + f];
+        int sjg = pbase + key_sj(inter);
+        int st2 = polyStart[sjg], n2 = polyStart[sjg + 1] - st2;
+        d2 inc[8];
+        int tg[8];
+        double e = 0.0;
+        d2 sp = mk(0.0, 0.0);
+        bool valid = (inter != outer);
+        if (valid) {
+            int iL = key_i(inter), jL = key_j(inter), kL = key_i(outer), lL = key_j(outer);
+            int ziL = (iL + 1 == n1) ? 0 : iL + 1, zlL = (lL + 1 == n1) ? 0 : lL + 1;
+            int zjL = (jL + 1 == n2) ? 0 : jL + 1, zkL = (kL + 1 == n2) ? 0 : kL + 1;
+            d2 pi = ldpos(pos, st1 + iL), pj = ldpos(pos, st2 + jL), pkk = ldpos(pos, st2 + kL);
+            d2 pl = ldpos(pos, st1 + lL);
+            d2 pzi = ldpos(pos, st1 + ziL), pzj = ldpos(pos, st2 + zjL), pzk = ldpos(pos, st2 + zkL);
+            d2 pzl = ldpos(pos, st1 + zlL);
+            d2 r1 = wrap2(mk(pzi.x - pi.x, pzi.y - pi.y));
+            d2 r2 = wrap2(mk(pzk.x - pkk.x, pzk.y - pkk.y));
+            double t1 = (double)rTU[base + f].y, t2 = (double)rUT[base + f].y;
+            if (AREA) {   // exact crossing parameters at the current positions
+                t1 = cross_param(pi, pzi, pj, pzj, t1);
+                t2 = cross_param(pkk, pzk, pl, pzl, t2);
+            }
+            d2 fij = wrap2(mk(pi.x + t1 * r1.x, pi.y + t1 * r1.y));
+            d2 fkl = wrap2(mk(pkk.x + t2 * r2.x, pkk.y + t2 * r2.y));
+            sp = fij;
+            if (AREA) {
+                // a record on a closed cycle of exit links (a closed lens boundary) measures
+                // from one origin per polygon pair: the first vertex of the lower polygon
+                uint64_t cur = outer, nxt = 0;
+                bool ok = find_record(cur, pbase, Kp, bcnt, rKey, rOuter, &nxt);
+                for (int hop = 0; hop < 8 && ok && cur != inter; ++hop) {
+                    if (nxt == cur) { ok = false; break; }
+                    cur = nxt;
+                    ok = find_record(cur, pbase, Kp, bcnt, rKey, rOuter, &nxt);
+                }
+                if (ok && cur == inter) sp = ldpos(pos, polyStart[min(s, sjg)]);
+            }
+            double dfa[16], dfb[16];
+            if (iL == lL) {
+                e += hfun(fij, fkl, sp);
+                d2 ga, gb;
+                g12(fij, fkl, sp, ga, gb);
+                getDf(pi, pzi, pj, pzj, dfa);
+                getDf(pkk, pzk, pi, pzi, dfb);
+                inc[0] = dfdot(dfa, 0, ga);
+                inc[1] = dfdot(dfa, 2, ga);
+                inc[4] = dfdot(dfa, 4, ga);
+                inc[5] = dfdot(dfa, 6, ga);
+                inc[6] = dfdot(dfb, 0, gb);
+                inc[7] = dfdot(dfb, 2, gb);
+                inc[2] = dfdot(dfb, 4, gb);
+                inc[3] = dfdot(dfb, 6, gb);
+            } else {
+                e += hfun(fij, pzi, sp);
+                e += hfun(pl, fkl, sp);
+                d2 g1a, g2a;
+                g12(fij, pzi, sp, g1a, g2a);
+                getDf(pi, pzi, pj, pzj, dfa);
+                getDf(pkk, pzk, pl, pzl, dfb);
+                inc[0] = dfdot(dfa, 0, g1a);
+                inc[4] = dfdot(dfa, 4, g1a);
+                d2 dd = dfdot(dfa, 2, g1a);
+                inc[1] = mk(dd.x + g2a.x, dd.y + g2a.y);
+                inc[5] = dfdot(dfa, 6, g1a);
+                d2 g1b, g2b;
+                g12(pl, fkl, sp, g1b, g2b);
+                d2 de = dfdot(dfb, 4, g2b);
+                inc[2] = mk(g1b.x + de.x, g1b.y + de.y);
+                inc[6] = dfdot(dfb, 0, g2b);
+                inc[3] = dfdot(dfb, 6, g2b);
+                inc[7] = dfdot(dfb, 2, g2b);
+            }
+            tg[0] = iL; tg[1] = ziL; tg[2] = lL; tg[3] = zlL;
+            tg[4] = jL; tg[5] = zjL; tg[6] = kL; tg[7] = zkL;
+            L.il[f] = iL | (lL << 16);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) { tg[q] = -1; inc[q] = mk(0.0, 0.0); }
+            L.il[f] = -1;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            L.c[4 * f + q] = make_double2(-inc[q].x * sc, -inc[q].y * sc);
+            L.t[4 * f + q] = tg[q];
+        }
+        L.e[f] = e;
+        L.sp[f] = make_double2(sp.x, sp.y);
+        // sj side: at this record's rank among the records whose sj is sjg (key order)
+        long long bj = (long long)sjg * Kp;
+        int cj = min(jcnt[sjg], Kp);
+        int r2 = 0;
+        for (int q = 0; q < cj; ++q) r2 += (jKey[bj + q] < inter);
+        if (r2 < Kp) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                jC[(bj + r2) * 4 + q] = make_double2(-inc[4 + q].x * sc, -inc[4 + q].y * sc);
+                jT[(bj + r2) * 4 + q] = tg[4 + q];
+            }
+        }
+    }
+    __syncthreads();
+    // polygon energy, first part: record energies in record order
+    double acc = 0.0;
+    if (lane == 0)
+        for (int f = 0; f < cnt; ++f) acc += L.e[f];
+    // per vertex: contributions acting on s (record order), interior chain terms
+    for (int m0 = 0; m0 < n1; m0 += 64) {
+        int mLoc = m0 + lane;
+        double le = 0.0;
+        if (mLoc < n1) {
+            d2 F = mk(0.0, 0.0);
+            for (int f = 0; f < cnt; ++f) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (L.t[4 * f + q] == mLoc) {
+                        double2 cc = L.c[4 * f + q];
+                        F.x += cc.x; F.y += cc.y;
+                    }
+            }
+            int nxL = (mLoc + 1 == n1) ? 0 : mLoc + 1, pmLoc = (mLoc == 0) ? n1 - 1 : mLoc - 1;
+            d2 Pm = ldpos(pos, st1 + mLoc), Pn = ldpos(pos, st1 + nxL), Pp = ldpos(pos, st1 + pmLoc);
+            d2 LM = mk(0.0, 0.0), LN = mk(0.0, 0.0);
+            for (int f = 0; f < cnt; ++f) {
+                int il = L.il[f];
+                if (il < 0) continue;
+                int iLoc = il & 0xFFFF, lLoc = il >> 16;
+                int lDist = (lLoc - iLoc + n1) % n1;
+                int dm = (mLoc - iLoc + n1) % n1;
+                int dp = (pmLoc - iLoc + n1) % n1;
+                bool cm = (dm > 0 && dm < lDist), cp = (dp > 0 && dp < lDist);
+                if (cm || cp) {
+                    double2 s2 = L.sp[f];
+                    d2 spr = mk(s2.x, s2.y);
+                    d2 g1, g2;
+                    if (cm) {
+                        le += hfun(Pm, Pn, spr);
+                        g12(Pm, Pn, spr, g1, g2);
+                        if (AREA) { g1.x *= 0.5; g1.y *= 0.5; }
+                        LM.x -= g1.x; LM.y -= g1.y;
+                    }
+                    if (cp) {
+                        g12(Pp, Pm, spr, g1, g2);
+                        if (AREA) { g2.x *= 0.5; g2.y *= 0.5; }
+                        LN.x -= g2.x; LN.y -= g2.y;
+                    }
+                }
+            }
+            F1[st1 + mLoc] = make_double2(F.x, F.y);
+            LMo[st1 + mLoc] = make_double2(LM.x, LM.y);
+            LNo[st1 + mLoc] = make_double2(LN.x, LN.y);
+        }
+        leBuf[lane] = le;
+        __syncthreads();
+        if (lane == 0) {
+            int cl = min(64, n1 - m0);
+            for (int q = 0; q < cl; ++q) acc += leBuf[q];
+        }
+        __syncthreads();
+    }
+    if (lane == 0) accR[s] = acc;
+}
+
+// ---------------------------------------------------------------- per-vertex gather
+// GEOM = 1: edge lengths and the area are computed here (reference summation order,
+// kernels.cuh:295-333) and written out; GEOM = 0: they are read (updateForceEnergy uses
+// the arrays of the last updatePolygonGeometry).  PROJ = 1: the force of each polygon is
+// projected (n <= SEG) before it is stored.
+template <int SEG, int GEOM, int PROJ>
+__global__ __launch_bounds__(BLK) void k_gather_poly(
+    int N, int Kp, int overlap, int edgeOn, int areaOn, const int* __restrict__ polyStart,
+    const double2* __restrict__ pos, const int* __restrict__ jcnt, const double2* __restrict__ jC,
+    const int* __restrict__ jT, const double2* __restrict__ F1, const double2* __restrict__ LMi,
+    const double2* __restrict__ LNi, const double* __restrict__ accR, double* __restrict__ edgeLengths,
+    double* __restrict__ areas, const double* __restrict__ targetEdgeV, const double* __restrict__ targetAreas,
+    double stiffness, double compressibility, double2* __restrict__ force, double* __restrict__ epoly) {
+    int sub = threadIdx.x & (SEG - 1);
+    int s = (blockIdx.x * BLK + threadIdx.x) / SEG;
+    if (s >= N) return;   // whole segments leave together
+    int st = polyStart[s], n = polyStart[s + 1] - st;
+    // area in the reference's order (needed by the area spring and written when GEOM)
+    double A = 0.0;
+    if (GEOM) {
+        d2 p0 = ldpos(pos, st);
+        for (int k0 = 0; k0 < n; k0 += SEG) {
+            int k = k0 + sub;
+            double term = 0.0;
+            if (k < n) {
+                d2 pk = ldpos(pos, st + k), pn = ldpos(pos, st + ((k + 1 == n) ? 0 : k + 1));
+                double x1a = wrap(pk.x - p0.x);
+                double dy2 = wrap(pn.y - pk.y);
+                double x2 = wrap(pn.x - p0.x);
+                term = dy2 * (x2 + x1a) / 2.0;
+            }
+            int cl = min(SEG, n - k0);
+            for (int q = 0; q < cl; ++q) A += __shfl(term, q, SEG);
+        }
+        if (sub == 0) areas[s] = A;
+    } else if (areaOn) {
+        A = areas[s];
+    }
+    double e = 0.5 * ((overlap) ? accR[s] : 0.0);
+    int jn = overlap ? min(jcnt[s], Kp) : 0;
+    long long bj = (long long)s * Kp;
+    for (int m0 = 0; m0 < n; m0 += SEG) {
+        int mLoc = m0 + sub;
+        bool live = mLoc < n;
+        double ee = 0.0;
+        d2 F = mk(0.0, 0.0);
+        d2 Pm = mk(0.0, 0.0);
+        if (live) {
+            int m = st + mLoc;
+            int nx = st + ((mLoc + 1 == n) ? 0 : mLoc + 1), pm = st + ((mLoc == 0) ? n - 1 : mLoc - 1);
+            Pm = ldpos(pos, m);
+            if (overlap) {
+                double2 f1 = F1[m];
+                F = mk(f1.x, f1.y);
+                for (int r = 0; r < jn; ++r) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        if (jT[(bj + r) * 4 + q] == mLoc) {
+                            double2 cc = jC[(bj + r) * 4 + q];
+                            F.x += cc.x; F.y += cc.y;
+                        }
+                }
+                double2 LM = LMi[m], LN = LNi[m];
+                if (LM.x != 0.0 || LM.y != 0.0) { F.x += LM.x; F.y += LM.y; }
+                if (LN.x != 0.0 || LN.y != 0.0) { F.x += LN.x; F.y += LN.y; }
+            }
+            double l = 0.0, prvl = 0.0;
+            if (GEOM || edgeOn) {
+                if (GEOM) {
+                    d2 Pn = ldpos(pos, nx), Pp = ldpos(pos, pm);
+                    double dx = wrap(Pn.x - Pm.x), dy = wrap(Pn.y - Pm.y);
+                    l = sqrt(dx * dx + dy * dy);
+                    double dxp = wrap(Pm.x - Pp.x), dyp = wrap(Pm.y - Pp.y);
+                    prvl = sqrt(dxp * dxp + dyp * dyp);
+                    edgeLengths[m] = l;
+                } else {
+                    l = edgeLengths[m];
+                    prvl = edgeLengths[pm];
+                }
+            }
+            if (edgeOn) {
+                double l0 = targetEdgeV[m], l0prv = targetEdgeV[pm];
+                d2 Pn = ldpos(pos, nx), Pp = ldpos(pos, pm);
+                d2 a = mk(Pm.x - Pn.x, Pm.y - Pn.y), b = mk(Pp.x - Pm.x, Pp.y - Pm.y);
+                a.x += 1.5; a.y += 1.5; b.x += 1.5; b.y += 1.5;
+                a.x = shift_into_unit(a.x); a.y = shift_into_unit(a.y);
+                b.x = shift_into_unit(b.x); b.y = shift_into_unit(b.y);
+                a.x -= 0.5; a.y -= 0.5; b.x -= 0.5; b.y -= 0.5;
+                double c1 = (l > 1e-14) ? (1.0 - l0 / l) : 0.0;
+                double c2 = (prvl > 1e-14) ? (1.0 - l0prv / prvl) : 0.0;
+                ee = stiffness * (l - l0) * (l - l0) / 2.0;
+                d2 lf;
+                lf.x = -(c1 * a.x - c2 * b.x) * stiffness;
+                lf.y = -(c1 * a.y - c2 * b.y) * stiffness;
+                if (lf.x != 0.0 || lf.y != 0.0) { F.x += lf.x; F.y += lf.y; }
+            }
+            if (areaOn) {
+                double A0 = targetAreas[s];
+                if (!(A0 < 1e-14)) {
+                    double coeff = 0.5 * compressibility * (A - A0);
+                    d2 Pn = ldpos(pos, nx), Pp = ldpos(pos, pm);
+                    double fx = -coeff * wrap(Pn.y - Pp.y);
+                    double fy = -coeff * wrap(Pp.x - Pn.x);
+                    F.x += fx; F.y += fy;
+                }
+            }
+        }
+        if (edgeOn) {   // edge energies in vertex order
+            int cl = min(SEG, n - m0);
+            for (int q = 0; q < cl; ++q) e += __shfl(ee, q, SEG);
+        }
+        double2 fo = make_double2(F.x, F.y);
+        if (PROJ && n <= SEG && n >= 3) fo = seg_project_force<SEG>(sub, n, Pm, fo);
+        if (live) force[st + mLoc] = fo;
+    }
+    if (areaOn) {
+        double A0 = targetAreas[s];
+        if (!(A0 < 1e-14)) e += compressibility * (A - A0) * (A - A0) / 2.0;
+    }
+    if (sub == 0) epoly[s] = e;
+}
+
+// ---------------------------------------------------------------- host side
+int grow_buckets(pcp_ctx* c, int need) {
+    if (need > REC_MAX_KP) {
+        set_error("more than 512 crossings on one polygon");
+        return PCP_ERR_CAPACITY;
+    }
+    int Kp = std::min(std::max(need + need / 4, 2 * c->Kp), REC_MAX_KP);
+    return alloc_buckets(c, Kp);
+}
+
+int alloc_buckets(pcp_ctx* c, int Kp) {
+    dfree(c->dBKey); dfree(c->dBTU); dfree(c->dJKey); dfree(c->dRKey); dfree(c->dROuter);
+    dfree(c->dRTU); dfree(c->dRUT); dfree(c->dJC); dfree(c->dJT);
+    long long n = (long long)std::max(c->N, 1) * Kp;
+    PCP_TRY(dalloc(&c->dBKey, n)); PCP_TRY(dalloc(&c->dBTU, n)); PCP_TRY(dalloc(&c->dJKey, n));
+    PCP_TRY(dalloc(&c->dRKey, n)); PCP_TRY(dalloc(&c->dROuter, n)); PCP_TRY(dalloc(&c->dRTU, n));
+    PCP_TRY(dalloc(&c->dRUT, n)); PCP_TRY(dalloc(&c->dJC, 4 * n)); PCP_TRY(dalloc(&c->dJT, 4 * n));
+    c->Kp = Kp;
+    return PCP_OK;
+}
+
+int run_records(pcp_ctx* c, const double2* pairPos) {
+    if (c->N == 0) return PCP_OK;
+    if (pairPos)
+        PCP_LAUNCH(c, k_pair_poly<1>, dim3(c->N), dim3(64), c->N, c->Kp, c->dPolyStart, c->dPolyPack,
+                   c->dPackPoly, c->dBcnt, c->dBKey, c->dBTU, c->dPackMaxKey, pairPos, c->dRKey, c->dRTU,
+                   c->dROuter, c->dRUT);
+    else
+        PCP_LAUNCH(c, k_pair_poly<0>, dim3(c->N), dim3(64), c->N, c->Kp, c->dPolyStart, c->dPolyPack,
+                   c->dPackPoly, c->dBcnt, c->dBKey, c->dBTU, c->dPackMaxKey, pairPos, c->dRKey, c->dRTU,
+                   c->dROuter, c->dRUT);
+    return PCP_OK;
+}
+
+template <int SEG, int GEOM, int PROJ>
+static int launch_gather(pcp_ctx* c, int overlap, int edgeOn, int areaOn, const double2* pos, double* edgeLengths,
+                         double* areas, const double* targetEdgeV, const double* targetAreas, double stiffness,
+                         double compressibility, double2* force) {
+    int grid = (c->N + BLK / SEG - 1) / (BLK / SEG);
+    int slot = -1;
+    PCP_TRY(prof_begin(c, "k_gather_poly", &slot));
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gather_poly<SEG, GEOM, PROJ>), dim3(grid), dim3(BLK), 0, c->stream, c->N,
+                       c->Kp, overlap, edgeOn, areaOn, c->dPolyStart, pos, c->dJcnt, c->dJC, c->dJT, c->dF1, c->dLM,
+                       c->dLN, c->dAccR, edgeLengths, areas, targetEdgeV, targetAreas, stiffness, compressibility,
+                       force, c->dEpoly);
+    PCP_LAUNCHED();
+    PCP_TRY(prof_end(c, slot));
+    return PCP_OK;
+}
+
+template <int GEOM, int PROJ>
+static int dispatch_gather(pcp_ctx* c, int overlap, int edgeOn, int areaOn, const double2* pos,
+                           double* edgeLengths, double* areas, const double* targetEdgeV,
+                           const double* targetAreas, double stiffness, double compressibility, double2* force) {
+#define PCP_GATHER(W) \
+    return launch_gather<W, GEOM, PROJ>(c, overlap, edgeOn, areaOn, pos, edgeLengths, areas, targetEdgeV, \
+                                        targetAreas, stiffness, compressibility, force)
+    switch (c->seg) {
+    case 4: PCP_GATHER(4);
+    case 8: PCP_GATHER(8);
+    case 16: PCP_GATHER(16);
+    case 32: PCP_GATHER(32);
+    default: PCP_GATHER(64);
+    }
+#undef PCP_GATHER
+}
+
+// Forces and per-packing energies.  geom: compute edge lengths / areas here and write
+// them (fused evaluation); else read them.  project: project the force of every polygon.
+int run_forces(pcp_ctx* c, int model, const double2* pos, double* edgeLengths, double* areas,
+               const double* targetEdgeV, const double* targetAreas, double stiffness,
+               double compressibility, double2* force, double* energy, int geom, int project) {
+    if (c->V == 0) return PCP_OK;
+    int area = (model & PCP_AREA_ENERGY) ? 1 : 0;
+    model &= 0xFF;
+    int overlap = (model == PCP_NORMAL || model == PCP_HYBRID);
+    int edgeOn = (model == PCP_SOFT_BODY || model == PCP_EDGE_ONLY || model == PCP_HYBRID);
+    int areaOn = (model == PCP_SOFT_BODY || model == PCP_AREA_ONLY || model == PCP_HYBRID);
+    if (model == PCP_ABNORMAL) {   // model.cpp:390-391: forces and energy zeroed only
+        hipLaunchKernelGGL(k_fill_double, dim3(nblk(2LL * c->V)), dim3(BLK), 0, c->stream, (double*)force,
+                           2LL * c->V, 0.0);
+        PCP_LAUNCHED();
+        hipLaunchKernelGGL(k_fill_double, dim3(nblk(c->P)), dim3(BLK), 0, c->stream, energy, (long long)c->P, 0.0);
+        PCP_LAUNCHED();
+        if (geom) PCP_TRY(run_geometry(c, pos, edgeLengths, areas, c->dCom, c->dCom + c->N, nullptr, nullptr));
+        return PCP_OK;
+    }
+    if (overlap) {
+        size_t lds = ext_lds_bytes(c->Kp);
+        if (area)
+            PCP_LAUNCH_LDS(c, k_exterior_poly<1>, dim3(c->N), dim3(64), lds, c->N, c->Kp, c->dPolyStart,
+                           c->dPolyPack, c->dPackPoly, pos, c->dBcnt, c->dJcnt, c->dJKey, c->dRKey, c->dROuter,
+                           c->dRTU, c->dRUT, c->dJC, c->dJT, c->dF1, c->dLM, c->dLN, c->dAccR);
+        else
+            PCP_LAUNCH_LDS(c, k_exterior_poly<0>, dim3(c->N), dim3(64), lds, c->N, c->Kp, c->dPolyStart,
+                           c->dPolyPack, c->dPackPoly, pos, c->dBcnt, c->dJcnt, c->dJKey, c->dRKey, c->dROuter,
+                           c->dRTU, c->dRUT, c->dJC, c->dJT, c->dF1, c->dLM, c->dLN, c->dAccR);
+    }
+    bool proj = project && c->maxN <= 64;
+    if (geom && proj)
+        PCP_TRY((dispatch_gather<1, 1>(c, overlap, edgeOn, areaOn, pos, edgeLengths, areas, targetEdgeV,
+                                       targetAreas, stiffness, compressibility, force)));
+    else if (geom)
+        PCP_TRY((dispatch_gather<1, 0>(c, overlap, edgeOn, areaOn, pos, edgeLengths, areas, targetEdgeV,
+                                       targetAreas, stiffness, compressibility, force)));
+    else
+        PCP_TRY((dispatch_gather<0, 0>(c, overlap, edgeOn, areaOn, pos, edgeLengths, areas, targetEdgeV,
+                                       targetAreas, stiffness, compressibility, force)));
+    if (project && !proj) PCP_TRY(run_project(c, pos, force));
+    return launch_det_reduce(c, c->dEpoly, 1, energy, 0);
+}
+
+// counts of the si buckets of packing p (host)
+int packing_counts(pcp_ctx* c, int p, std::vector<int>& cnt) {
+    int n0 = c->hPackPoly[p], n1 = c->hPackPoly[p + 1];
+    cnt.assign(n1 - n0, 0);
+    if (n1 > n0) PCP_HIP(hipMemcpy(cnt.data(), c->dBcnt + n0, sizeof(int) * (n1 - n0), hipMemcpyDeviceToHost));
+    for (int& x : cnt) x = std::min(x, c->Kp);
+    return PCP_OK;
+}
+
+}  // namespace pcp

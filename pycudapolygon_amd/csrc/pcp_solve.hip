@@ -102,6 +102,37 @@ __device__ double seg_bordered_solve(int e, int n, double tx, double ty, double 
     return live ? y - z * u : 0.0;
 }
 
+// Force of vertex e (lane e of the segment, e < n) with the components along the n
+// edge-length gradients and the area gradient of its polygon removed.  Every lane of
+// the segment calls it; padding lanes (e >= n) pass anything and get it back.
+template <int SEG>
+__device__ double2 seg_project_force(int e, int n, d2 pe, double2 fe) {
+    bool live = e < n;
+    int ef = (e + 1 >= n) ? 0 : e + 1, eg = (e == 0) ? n - 1 : e - 1;
+    double pfx = seg_get<SEG>(pe.x, ef), pfy = seg_get<SEG>(pe.y, ef);
+    double pgx = seg_get<SEG>(pe.x, eg), pgy = seg_get<SEG>(pe.y, eg);
+    double tx = 0.0, ty = 0.0, ax = 0.0, ay = 0.0;
+    if (live) {
+        double dx = wrap(pfx - pe.x), dy = wrap(pfy - pe.y);
+        double len = sqrt(dx * dx + dy * dy);
+        if (!(len < 1e-14)) { tx = dx / len; ty = dy / len; }
+        ax = wrap(pfy - pgy);
+        ay = wrap(pgx - pfx);
+    }
+    double fx = live ? fe.x : 0.0, fy = live ? fe.y : 0.0;
+    double ffx = seg_get<SEG>(fx, ef), ffy = seg_get<SEG>(fy, ef);
+    double r = live ? tx * (ffx - fx) + ty * (ffy - fy) : 0.0;
+    double c = seg_sum<SEG>(ax * ax + ay * ay);
+    double sd = seg_sum<SEG>(ax * fx + ay * fy);
+    double z;
+    double y = seg_bordered_solve<SEG>(e, n, tx, ty, ax, ay, r, c, sd, 1e-28, &z);
+    double yg = seg_get<SEG>(y, eg), tgx = seg_get<SEG>(tx, eg), tgy = seg_get<SEG>(ty, eg);
+    if (!live) return fe;
+    double px = -y * tx + yg * tgx + z * ax;
+    double py = -y * ty + yg * tgy + z * ay;
+    return make_double2(fe.x - px, fe.y - py);
+}
+
 // projection of the force of every polygon onto the tangent space of its constraints
 template <int SEG>
 __global__ __launch_bounds__(BLK) void k_project_seg(int N, const int* __restrict__ polyStart,
@@ -113,31 +144,10 @@ __global__ __launch_bounds__(BLK) void k_project_seg(int N, const int* __restric
     int st = polyStart[s], n = polyStart[s + 1] - st;
     if (n < 3 || n > SEG) return;
     bool live = e < n;
-    int ef = (e + 1 >= n) ? 0 : e + 1, eg = (e == 0) ? n - 1 : e - 1;
     d2 pe = live ? ldpos(pos, st + e) : mk(0.0, 0.0);
     double2 fe = live ? force[st + e] : make_double2(0.0, 0.0);
-    double pfx = seg_get<SEG>(pe.x, ef), pfy = seg_get<SEG>(pe.y, ef);
-    double pgx = seg_get<SEG>(pe.x, eg), pgy = seg_get<SEG>(pe.y, eg);
-    double tx = 0.0, ty = 0.0, ax = 0.0, ay = 0.0;
-    if (live) {
-        double dx = wrap(pfx - pe.x), dy = wrap(pfy - pe.y);
-        double len = sqrt(dx * dx + dy * dy);
-        if (!(len < 1e-14)) { tx = dx / len; ty = dy / len; }
-        ax = wrap(pfy - pgy);
-        ay = wrap(pgx - pfx);
-    }
-    double ffx = seg_get<SEG>(fe.x, ef), ffy = seg_get<SEG>(fe.y, ef);
-    double r = live ? tx * (ffx - fe.x) + ty * (ffy - fe.y) : 0.0;
-    double c = seg_sum<SEG>(ax * ax + ay * ay);
-    double sd = seg_sum<SEG>(ax * fe.x + ay * fe.y);
-    double z;
-    double y = seg_bordered_solve<SEG>(e, n, tx, ty, ax, ay, r, c, sd, 1e-28, &z);
-    double yg = seg_get<SEG>(y, eg), tgx = seg_get<SEG>(tx, eg), tgy = seg_get<SEG>(ty, eg);
-    if (live) {
-        double px = -y * tx + yg * tgx + z * ax;
-        double py = -y * ty + yg * tgy + z * ay;
-        force[st + e] = make_double2(fe.x - px, fe.y - py);
-    }
+    double2 fo = seg_project_force<SEG>(e, n, pe, fe);
+    if (live) force[st + e] = fo;
 }
 
 // SHAKE iterations of one polygon held in the lanes of its segment: Gauss-Newton
