@@ -70,6 +70,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fire-seconds", type=float, default=20.0)
     ap.add_argument("--no-fire", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="N",
+                    help="only run the plain copy kernel on N doubles (10 launches) for HBM counter "
+                         "calibration, then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; the multi-GPU design is described in DESIGN.md "
@@ -78,6 +81,20 @@ def main():
 
     from pycudapolygon_amd import capi
     from pycudapolygon_amd.generate import make_packing, box_size_for
+
+    if args.calibrate_copy > 0:
+        n = args.calibrate_copy
+        src = capi.DeviceArray((n,), np.float64)
+        dst = capi.DeviceArray((n,), np.float64)
+        src.upload(np.arange(n, dtype=np.float64))
+        ctx = capi.Context()
+        for _ in range(10):
+            ctx.call("pcp_stream_copy", dst.ptr, src.ptr, ctypes.c_longlong(n))
+        ctx.call("pcp_synchronize")
+        assert np.array_equal(dst.get()[:16], np.arange(16, dtype=np.float64))
+        print(json.dumps({"calibration": "k_copy", "doubles": n, "bytes_read": 8 * n, "bytes_written": 8 * n,
+                          "launches": 10}), flush=True)
+        return
 
     p = make_packing(4096, 16, 0.85, seed=args.seed)
     B = box_size_for(p["diameter"] if args.cells == "diameter" else p["maxEdgeLength"])
@@ -124,23 +141,38 @@ def main():
 
     fire = None
     if not args.no_fire:
-        # FIRE leg (bounded): area energy (the reference convention's energy is not the
-        # potential of its forces and FIRE stalls)
-        sf = capi.System([p], [B], model=capi.NORMAL | 0x100)
+        # FIRE leg (bounded) on squared lens areas: the reference convention's energy is
+        # not the potential of its forces, and the overlap area itself has kinks at flush
+        # edge contacts, so FIRE stalls on both; sum A^2/2 is smooth there.  The packing
+        # has the same sizes, but no centre closer than (R_i + R_j)/2 to another, so no
+        # polygon starts inside another (which no edge-crossing energy can see).
+        # Stopping: max|F| <= 1e-10, or the reference driver's rule (pyCudaPolygon.py:
+        # 674-731) of an energy unchanged (|dE| < 1e-15) for more than 5 steps.
+        pf = make_packing(4096, 16, 0.85, seed=args.seed, min_center_frac=0.5)
+        Bf = box_size_for(pf["diameter"] if args.cells == "diameter" else pf["maxEdgeLength"])
+        sf = capi.System([pf], [Bf], model=capi.NORMAL | 0x200)
         prm = sf.fire_params(fTol=1e-10)
         sf.fire_init(prm, 1e-3)
-        t1, steps, st = time.perf_counter(), 0, None
+        t1, steps, st, flat, prevE, stop = time.perf_counter(), 0, None, 0, 1e9, "time"
         while time.perf_counter() - t1 < args.fire_seconds:
             st = sf.fire_step(prm)[0]
             steps += 1
             if st.converged:
+                stop = "force"
+                break
+            flat = flat + 1 if abs(st.energy - prevE) < 1e-15 else 0
+            prevE = st.energy
+            if flat > 5:
+                stop = "flat_energy"
                 break
         fel = time.perf_counter() - t1
-        fire = dict(energy="area", steps=steps, seconds=fel, steps_per_s=steps / fel,
+        fire = dict(energy="squared lens areas", packing="N=4096 x 16, phi=0.85, centres >= (Ri+Rj)/2 apart",
+                    steps=steps, seconds=fel, steps_per_s=steps / fel, stop=stop,
                     converged=bool(st.converged), final_energy=st.energy, final_max_force=st.maxForce,
-                    packings_per_s=(1.0 / fel) if st.converged else 0.0)
-        print(f"[bench] FIRE: {steps} steps in {fel:.1f} s, converged={bool(st.converged)}, "
-              f"E={st.energy:.3e}, max|F|={st.maxForce:.3e}", flush=True)
+                    packings_per_s=(1.0 / fel) if st.converged else 0.0,
+                    packings_per_s_reference_stop=(1.0 / fel) if stop != "time" else 0.0)
+        print(f"[bench] FIRE: {steps} steps in {fel:.1f} s, stop={stop}, E={st.energy:.3e}, "
+              f"max|F|={st.maxForce:.3e}", flush=True)
 
     cpu = None
     if not args.no_cpu_baseline:

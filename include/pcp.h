@@ -47,6 +47,13 @@ enum { PCP_NORMAL = 0, PCP_EDGE_ONLY = 1, PCP_AREA_ONLY = 2, PCP_SOFT_BODY = 3,
  * included), every record of a closed partner cycle measures from one origin per
  * polygon pair, and forces are halved: energy = overlap area, force = -grad(energy). */
 #define PCP_AREA_ENERGY 0x100
+/* Flag (no counterpart in the reference): squared lens areas.  Implies PCP_AREA_ENERGY's
+ * exact crossings and cycle origins; every overlap lens (a closed cycle of exit links, or
+ * a record off a cycle on its own) has area A, the energy is sum A^2/2 and the force
+ * -sum A grad A.  The area itself has kinks where polygons touch edge to edge, at which
+ * no force tolerance is reachable; the squared energy is smooth there and FIRE converges
+ * to max|F| <= 1e-10. */
+#define PCP_AREA_SQUARED 0x200
 
 typedef struct pcp_ctx pcp_ctx;
 

@@ -594,7 +594,8 @@ double orc_force_energy(int model, int V, int N, const int* start, const double*
     int* next = (int*)malloc(sizeof(int) * (V > 0 ? V : 1));
     int* prev = (int*)malloc(sizeof(int) * (V > 0 ? V : 1));
     orc_topology(N, start, V, shapeId, next, prev);
-    int areaMode = (model & ORC_AREA_ENERGY) != 0;
+    int sqMode = (model & ORC_AREA_SQUARED) != 0;
+    int areaMode = (model & (ORC_AREA_ENERGY | ORC_AREA_SQUARED)) != 0;
     model &= 0xFF;
     int overlap = (model == ORC_NORMAL || model == ORC_HYBRID);
     int edgeOn = (model == ORC_SOFT_BODY || model == ORC_EDGE_ONLY || model == ORC_HYBRID);
@@ -611,9 +612,9 @@ double orc_force_energy(int model, int V, int N, const int* start, const double*
     for (int s = 0; s < N; ++s) { shapeStart[s] = I; shapeEnd[s] = -1; }
     /* area energy: records on a closed partner cycle (<= 8 hops) share the pair origin */
     int* loopMode = (int*)malloc(sizeof(int) * Ialloc);
+    int* partner = (int*)malloc(sizeof(int) * Ialloc);
     for (int r = 0; r < I; ++r) loopMode[r] = areaMode ? 2 : 0;
     if (areaMode && I > 0) {
-        int* partner = (int*)malloc(sizeof(int) * I);
         for (int r = 0; r < I; ++r) {
             partner[r] = -1;
             if (intersections[r] == outersections[r]) continue;
@@ -635,7 +636,6 @@ double orc_force_energy(int model, int V, int N, const int* start, const double*
             for (int hop = 0; hop < 8 && q >= 0 && q != r; ++hop) q = partner[q];
             if (q == r) loopMode[r] = 1;
         }
-        free(partner);
     }
 #pragma omp parallel for schedule(static)
     for (int r = 0; r < I; ++r) {
@@ -648,6 +648,44 @@ double orc_force_energy(int model, int V, int N, const int* start, const double*
         int s = (int)((intersections[r] >> 32) & 0xFFFF);
         if (r < shapeStart[s]) shapeStart[s] = r;
         if (r > shapeEnd[s]) shapeEnd[s] = r;
+    }
+    /* squared lens areas: per record E_r = exterior + its chain (h sums, twice the area it
+     * covers); a lens is a closed partner cycle, A = 0.5 * sum of E_r over the cycle starting
+     * at its smallest key; a record off a cycle is its own lens.  Contributions scale by A. */
+    double* lensA = (double*)malloc(sizeof(double) * Ialloc);
+    int* lensN = (int*)malloc(sizeof(int) * Ialloc);
+    if (sqMode && I > 0) {
+        double* Er = (double*)malloc(sizeof(double) * I);
+        for (int r = 0; r < I; ++r) {
+            Er[r] = 0.0;
+            if (self[r]) continue;
+            uint64_t inter = intersections[r], outer = outersections[r];
+            int si = (int)((inter >> 32) & 0xFFFF), n = start[si + 1] - start[si];
+            int iLoc = (int)((inter >> 16) & 0xFFFF), lLoc = (int)(outer & 0xFFFF);
+            int lDist = (lLoc - iLoc + n) % n;
+            double chain = 0.0;
+            for (int d = 1; d < lDist; ++d) {
+                int m = start[si] + (iLoc + d) % n;
+                chain += hfun(P(pos, m), P(pos, next[m]), spr[r]);
+            }
+            Er[r] = eext[r] + chain;
+        }
+        for (int r = 0; r < I; ++r) {
+            lensA[r] = 0.0; lensN[r] = 1;
+            if (self[r]) continue;
+            if (loopMode[r] != 1) { lensA[r] = 0.5 * Er[r]; continue; }
+            int mem[16], cnt = 0, q = r;
+            do { mem[cnt++] = q; q = partner[q]; } while (q != r && cnt < 16);
+            int k0 = 0;
+            for (int k = 1; k < cnt; ++k) if (intersections[mem[k]] < intersections[mem[k0]]) k0 = k;
+            double sum = 0.0;
+            for (int k = 0; k < cnt; ++k) sum += Er[mem[(k0 + k) % cnt]];
+            lensA[r] = 0.5 * sum;
+            lensN[r] = cnt;
+        }
+        for (int r = 0; r < I; ++r)
+            for (int q = 0; q < 8; ++q) { cc[8 * r + q].x *= lensA[r]; cc[8 * r + q].y *= lensA[r]; }
+        free(Er);
     }
     /* records owned through sj, in record order */
     int* sjCount = (int*)calloc(N + 1, sizeof(int));
@@ -693,6 +731,7 @@ double orc_force_energy(int model, int V, int N, const int* start, const double*
                     d2 g1, g2;
                     g12(pm, pn, sp, &g1, &g2);
                     if (areaMode) { g1.x *= 0.5; g1.y *= 0.5; g2.x *= 0.5; g2.y *= 0.5; }
+                    if (sqMode) { g1.x *= lensA[k]; g1.y *= lensA[k]; g2.x *= lensA[k]; g2.y *= lensA[k]; }
                     lm.x -= g1.x; lm.y -= g1.y;
                     ln.x -= g2.x; ln.y -= g2.y;
                 }
@@ -756,12 +795,20 @@ double orc_force_energy(int model, int V, int N, const int* start, const double*
     double* ep = (double*)malloc(sizeof(double) * (N > 0 ? N : 1));
     for (int s = 0; s < N; ++s) {
         double acc = 0.0;
-        if (overlap) {
-            if (shapeStart[s] <= shapeEnd[s])
-                for (int r = shapeStart[s]; r <= shapeEnd[s]; ++r) acc += eext[r];
-            for (int m = start[s]; m < start[s + 1]; ++m) acc += eint[m];
+        double e;
+        if (sqMode) {   /* each lens's A^2/2 split evenly over its records */
+            if (overlap && shapeStart[s] <= shapeEnd[s])
+                for (int r = shapeStart[s]; r <= shapeEnd[s]; ++r)
+                    acc += (lensA[r] * lensA[r]) / (2.0 * lensN[r]);
+            e = acc;
+        } else {
+            if (overlap) {
+                if (shapeStart[s] <= shapeEnd[s])
+                    for (int r = shapeStart[s]; r <= shapeEnd[s]; ++r) acc += eext[r];
+                for (int m = start[s]; m < start[s + 1]; ++m) acc += eint[m];
+            }
+            e = 0.5 * acc;
         }
-        double e = 0.5 * acc;
         if (edgeOn) for (int m = start[s]; m < start[s + 1]; ++m) e += eedge[m];
         if (areaOn) {
             double A = areas[s], A0 = targetAreas[s];
@@ -772,7 +819,7 @@ double orc_force_energy(int model, int V, int N, const int* start, const double*
     double E = orc_det_sum(ep, N);
     free(ep); free(eedge); free(LM); free(LN); free(eint); free(sjCount); free(sjList); free(fill);
     free(shapeStart); free(shapeEnd); free(tgt); free(cc); free(eext); free(self); free(spr);
-    free(loopMode);
+    free(loopMode); free(partner); free(lensA); free(lensN);
     free(shapeId); free(next); free(prev);
     return E;
 }
@@ -1131,7 +1178,7 @@ static void detect(const orc_system* s, const double* pos, orc_ws* w) {
     }
 }
 static void records(const orc_system* s, orc_ws* w, const double* pos) {
-    const double* ex = (s->model & ORC_AREA_ENERGY) ? pos : NULL;
+    const double* ex = (s->model & (ORC_AREA_ENERGY | ORC_AREA_SQUARED)) ? pos : NULL;
     w->I = orc_outersections_ex(s->V, s->N, s->start, w->M, w->neighbors, w->inside, w->tuSlots,
                                 w->numNeighbors, w->inter, w->tu, w->outer, w->ut, w->shapeCounts, ex);
 }
@@ -1213,7 +1260,7 @@ void orc_fire_step(const orc_system* s, double* pos, double* vel, double* force,
     ws_alloc(&w, s->V, s->N, s->boxSize, s->maxNeighbors > 0 ? s->maxNeighbors : 100);
     int shakeFailed = 0;
     st->rolledBack = 0;
-    int areaMode = (s->model & ORC_AREA_ENERGY) != 0;
+    int areaMode = (s->model & (ORC_AREA_ENERGY | ORC_AREA_SQUARED)) != 0;
     if (needsInter) {
         if (!areaMode) detect(s, pos, &w);   /* reference order: detect, SHAKE, records */
         if (rigid && shakeIter > 0) {

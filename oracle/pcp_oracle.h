@@ -26,6 +26,9 @@ enum { ORC_NORMAL = 0, ORC_EDGE_ONLY = 1, ORC_AREA_ONLY = 2, ORC_SOFT_BODY = 3,
  * vertex of the lower-numbered polygon) instead of each record's entry point, with
  * forces halved -- energy = exact overlap area and force = -grad(energy). */
 #define ORC_AREA_ENERGY 0x100
+/* squared lens areas (implies ORC_AREA_ENERGY): energy sum over overlap lenses of A^2/2,
+ * force -sum A grad A (smooth where a lens closes, unlike the area itself) */
+#define ORC_AREA_SQUARED 0x200
 
 double orc_wrap(double x);
 void orc_topology(int N, const int* start, int V, int* shapeId, int* next, int* prev);

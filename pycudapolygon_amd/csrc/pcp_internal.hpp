@@ -94,6 +94,7 @@ struct pcp_ctx {
     int* dJT = nullptr;            // their vertex (local index, -1 none)
     double2 *dF1 = nullptr, *dLM = nullptr, *dLN = nullptr;   // V: per-vertex partial sums
     double* dAccR = nullptr;       // N: record + interior energy sums
+    double* dEr = nullptr;         // per record: its h sums (squared lens energy)
 
     // per-vertex / per-polygon scratch
     double* dEpoly = nullptr;

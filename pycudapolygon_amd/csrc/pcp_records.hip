@@ -24,7 +24,7 @@
 
 namespace pcp {
 
-constexpr int REC_MAX_KP = 512;   // LDS of k_exterior_poly: Kp * 120 bytes per wavefront
+constexpr int REC_MAX_KP = 512;   // LDS of k_exterior_poly: Kp * 116 bytes per wavefront
 
 // ---------------------------------------------------------------- pairing
 // The reference's binary search starts with end = I-1, so when every key of the packing
@@ -118,6 +118,7 @@ struct ExtLds {
     int* t;        // Kp*4 their vertex (local index in s, -1 none)
     double2* sp;   // Kp start points
     double* e;     // Kp record energies
+    double* la;    // Kp lens areas (squared lens energy)
     int* il;       // Kp: iLoc | lLoc << 16, -1 for a record without partner
 };
 __device__ __forceinline__ ExtLds ext_lds(char* smem, int Kp) {
@@ -125,32 +126,115 @@ __device__ __forceinline__ ExtLds ext_lds(char* smem, int Kp) {
     L.c = (double2*)smem;
     L.sp = L.c + 4 * Kp;
     L.e = (double*)(L.sp + Kp);
-    L.t = (int*)(L.e + Kp);
+    L.la = L.e + Kp;
+    L.t = (int*)(L.la + Kp);
     L.il = L.t + 4 * Kp;
     return L;
 }
-size_t ext_lds_bytes(int Kp) { return (size_t)Kp * (64 + 16 + 8 + 16 + 4); }
+size_t ext_lds_bytes(int Kp) { return (size_t)Kp * (64 + 16 + 8 + 8 + 16 + 4); }
 
-// the record of the packing whose key is `key`: its outer key (false if absent)
+// the record of the packing whose key is `key`: its outer key and its row (false if absent)
 __device__ __forceinline__ bool find_record(uint64_t key, int pbase, int Kp, const int* __restrict__ bcnt,
                                             const uint64_t* __restrict__ rKey,
-                                            const uint64_t* __restrict__ rOuter, uint64_t* outer) {
+                                            const uint64_t* __restrict__ rOuter, uint64_t* outer,
+                                            long long* row = nullptr) {
     int s2 = pbase + key_si(key);
     long long b2 = (long long)s2 * Kp;
     int c2 = min(bcnt[s2], Kp);
     for (int q = 0; q < c2; ++q)
-        if (rKey[b2 + q] == key) { *outer = rOuter[b2 + q]; return true; }
+        if (rKey[b2 + q] == key) {
+            *outer = rOuter[b2 + q];
+            if (row) *row = b2 + q;
+            return true;
+        }
     return false;
 }
 
-template <int AREA>
+// whether the exit links starting at record (inter, outer) close into a cycle back to it
+// within 8 hops (a closed lens boundary)
+__device__ __forceinline__ bool closed_cycle(uint64_t inter, uint64_t outer, int pbase, int Kp,
+                                             const int* __restrict__ bcnt, const uint64_t* __restrict__ rKey,
+                                             const uint64_t* __restrict__ rOuter) {
+    uint64_t cur = outer, nxt = 0;
+    bool ok = find_record(cur, pbase, Kp, bcnt, rKey, rOuter, &nxt);
+    for (int hop = 0; hop < 8 && ok && cur != inter; ++hop) {
+        if (nxt == cur) { ok = false; break; }
+        cur = nxt;
+        ok = find_record(cur, pbase, Kp, bcnt, rKey, rOuter, &nxt);
+    }
+    return ok && cur == inter;
+}
+
+// Squared lens areas, first pass (wavefront / polygon s, lane / record): the h sums of
+// each record -- its exterior terms plus its chain of s's edges, measured from the same
+// origin as in k_exterior_poly -- twice the area the record covers.
+__global__ __launch_bounds__(64) void k_lens_energy(int N, int Kp, const int* __restrict__ polyStart,
+                                                   const int* __restrict__ polyPack,
+                                                   const int* __restrict__ packPoly, const double2* __restrict__ pos,
+                                                   const int* __restrict__ bcnt, const uint64_t* __restrict__ rKey,
+                                                   const uint64_t* __restrict__ rOuter,
+                                                   const float2* __restrict__ rTU, const float2* __restrict__ rUT,
+                                                   double* __restrict__ Er) {
+    int lane = threadIdx.x;
+    int s = blockIdx.x;
+    if (s >= N) return;
+    int cnt = min(bcnt[s], Kp);
+    int pk = polyPack[s];
+    int pbase = packPoly[pk];
+    int st1 = polyStart[s], n1 = polyStart[s + 1] - st1;
+    long long base = (long long)s * Kp;
+    for (int f = lane; f < cnt; f += 64) {
+        uint64_t inter = rKey[base + f], outer = rOuter[base + f];
+        double E = 0.0;
+        if (inter != outer) {
+            int sjg = pbase + key_sj(inter);
+            int st2 = polyStart[sjg], n2 = polyStart[sjg + 1] - st2;
+            int iL = key_i(inter), jL = key_j(inter), kL = key_i(outer), lL = key_j(outer);
+            int ziL = (iL + 1 == n1) ? 0 : iL + 1;
+            int zjL = (jL + 1 == n2) ? 0 : jL + 1, zkL = (kL + 1 == n2) ? 0 : kL + 1;
+            int zlL = (lL + 1 == n1) ? 0 : lL + 1;
+            d2 pi = ldpos(pos, st1 + iL), pj = ldpos(pos, st2 + jL), pkk = ldpos(pos, st2 + kL);
+            d2 pl = ldpos(pos, st1 + lL);
+            d2 pzi = ldpos(pos, st1 + ziL), pzj = ldpos(pos, st2 + zjL), pzk = ldpos(pos, st2 + zkL);
+            d2 pzl = ldpos(pos, st1 + zlL);
+            d2 r1 = wrap2(mk(pzi.x - pi.x, pzi.y - pi.y));
+            d2 r2 = wrap2(mk(pzk.x - pkk.x, pzk.y - pkk.y));
+            double t1 = cross_param(pi, pzi, pj, pzj, (double)rTU[base + f].y);
+            double t2 = cross_param(pkk, pzk, pl, pzl, (double)rUT[base + f].y);
+            d2 fij = wrap2(mk(pi.x + t1 * r1.x, pi.y + t1 * r1.y));
+            d2 fkl = wrap2(mk(pkk.x + t2 * r2.x, pkk.y + t2 * r2.y));
+            d2 sp = fij;
+            if (closed_cycle(inter, outer, pbase, Kp, bcnt, rKey, rOuter)) sp = ldpos(pos, polyStart[min(s, sjg)]);
+            double e = 0.0;
+            if (iL == lL) {
+                e += hfun(fij, fkl, sp);
+            } else {
+                e += hfun(fij, pzi, sp);
+                e += hfun(pl, fkl, sp);
+            }
+            double chain = 0.0;
+            int lDist = (lL - iL + n1) % n1;
+            for (int d = 1; d < lDist; ++d) {
+                int m = (iL + d) % n1;
+                int mn = (m + 1 == n1) ? 0 : m + 1;
+                chain += hfun(ldpos(pos, st1 + m), ldpos(pos, st1 + mn), sp);
+            }
+            E = e + chain;
+        }
+        Er[base + f] = E;
+    }
+}
+
+// MODE 0: reference energy; 1: area energy; 2: squared lens areas
+template <int MODE>
 __global__ __launch_bounds__(64) void k_exterior_poly(
     int N, int Kp, const int* __restrict__ polyStart, const int* __restrict__ polyPack,
     const int* __restrict__ packPoly, const double2* __restrict__ pos, const int* __restrict__ bcnt,
     const int* __restrict__ jcnt, const uint64_t* __restrict__ jKey, const uint64_t* __restrict__ rKey,
     const uint64_t* __restrict__ rOuter, const float2* __restrict__ rTU, const float2* __restrict__ rUT,
     double2* __restrict__ jC, int* __restrict__ jT, double2* __restrict__ F1, double2* __restrict__ LMo,
-    double2* __restrict__ LNo, double* __restrict__ accR) {
+    double2* __restrict__ LNo, double* __restrict__ accR, const double* __restrict__ Er) {
+    constexpr int AREA = (MODE >= 1), SQ = (MODE == 2);
     extern __shared__ char smem[];
     __shared__ double leBuf[64];
     ExtLds L = ext_lds(smem, Kp);
@@ -171,6 +255,8 @@ __global__ __launch_bounds__(64) void k_exterior_poly(
         int tg[8];
         double e = 0.0;
         d2 sp = mk(0.0, 0.0);
+        double lensA = 0.0;
+        int lensN = 1;
         bool valid = (inter != outer);
         if (valid) {
             int iL = key_i(inter), jL = key_j(inter), kL = key_i(outer), lL = key_j(outer);
@@ -190,17 +276,39 @@ __global__ __launch_bounds__(64) void k_exterior_poly(
             d2 fij = wrap2(mk(pi.x + t1 * r1.x, pi.y + t1 * r1.y));
             d2 fkl = wrap2(mk(pkk.x + t2 * r2.x, pkk.y + t2 * r2.y));
             sp = fij;
+            bool closed = false;
             if (AREA) {
                 // a record on a closed cycle of exit links (a closed lens boundary) measures
                 // from one origin per polygon pair: the first vertex of the lower polygon
-                uint64_t cur = outer, nxt = 0;
-                bool ok = find_record(cur, pbase, Kp, bcnt, rKey, rOuter, &nxt);
-                for (int hop = 0; hop < 8 && ok && cur != inter; ++hop) {
-                    if (nxt == cur) { ok = false; break; }
-                    cur = nxt;
-                    ok = find_record(cur, pbase, Kp, bcnt, rKey, rOuter, &nxt);
+                closed = closed_cycle(inter, outer, pbase, Kp, bcnt, rKey, rOuter);
+                if (closed) sp = ldpos(pos, polyStart[min(s, sjg)]);
+            }
+            if (SQ) {
+                // lens area: half the h sums of the records of the cycle, added in cycle
+                // order from the record with the smallest key; off a cycle, the record alone
+                if (closed) {
+                    uint64_t mk2[16];
+                    long long mr[16];
+                    int cntL = 0;
+                    uint64_t cur = inter, nxt = 0;
+                    long long row = base + f;
+                    do {
+                        mk2[cntL] = cur;
+                        mr[cntL] = row;
+                        ++cntL;
+                        uint64_t o2 = (cntL == 1) ? outer : nxt;
+                        find_record(o2, pbase, Kp, bcnt, rKey, rOuter, &nxt, &row);
+                        cur = o2;
+                    } while (cur != inter && cntL < 16);
+                    int k0 = 0;
+                    for (int k = 1; k < cntL; ++k) if (mk2[k] < mk2[k0]) k0 = k;
+                    double sum = 0.0;
+                    for (int k = 0; k < cntL; ++k) sum += Er[mr[(k0 + k) % cntL]];
+                    lensA = 0.5 * sum;
+                    lensN = cntL;
+                } else {
+                    lensA = 0.5 * Er[base + f];
                 }
-                if (ok && cur == inter) sp = ldpos(pos, polyStart[min(s, sjg)]);
             }
             double dfa[16], dfb[16];
             if (iL == lL) {
@@ -246,11 +354,21 @@ __global__ __launch_bounds__(64) void k_exterior_poly(
             L.il[f] = -1;
         }
 #pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            double cx = -inc[q].x * sc, cy = -inc[q].y * sc;
+            if (SQ) { cx *= lensA; cy *= lensA; }
+            inc[q] = mk(cx, cy);
+        }
+#pragma unroll
         for (int q = 0; q < 4; ++q) {
-            L.c[4 * f + q] = make_double2(-inc[q].x * sc, -inc[q].y * sc);
+            L.c[4 * f + q] = make_double2(inc[q].x, inc[q].y);
             L.t[4 * f + q] = tg[q];
         }
-        L.e[f] = e;
+        // energy of the record: its h terms (the polygon sums them with its interior
+        // terms and halves), or with SQ its share of the lens energy A^2/2, doubled so
+        // that the same halving gives it exactly
+        L.e[f] = SQ ? 2.0 * ((lensA * lensA) / (2.0 * lensN)) : e;
+        L.la[f] = lensA;
         L.sp[f] = make_double2(sp.x, sp.y);
         // sj side: at this record's rank among the records whose sj is sjg (key order)
         long long bj = (long long)sjg * Kp;
@@ -260,7 +378,7 @@ __global__ __launch_bounds__(64) void k_exterior_poly(
         if (r2 < Kp) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                jC[(bj + r2) * 4 + q] = make_double2(-inc[4 + q].x * sc, -inc[4 + q].y * sc);
+                jC[(bj + r2) * 4 + q] = make_double2(inc[4 + q].x, inc[4 + q].y);
                 jT[(bj + r2) * 4 + q] = tg[4 + q];
             }
         }
@@ -300,14 +418,16 @@ __global__ __launch_bounds__(64) void k_exterior_poly(
                     d2 spr = mk(s2.x, s2.y);
                     d2 g1, g2;
                     if (cm) {
-                        le += hfun(Pm, Pn, spr);
+                        if (!SQ) le += hfun(Pm, Pn, spr);
                         g12(Pm, Pn, spr, g1, g2);
                         if (AREA) { g1.x *= 0.5; g1.y *= 0.5; }
+                        if (SQ) { g1.x *= L.la[f]; g1.y *= L.la[f]; }
                         LM.x -= g1.x; LM.y -= g1.y;
                     }
                     if (cp) {
                         g12(Pp, Pm, spr, g1, g2);
                         if (AREA) { g2.x *= 0.5; g2.y *= 0.5; }
+                        if (SQ) { g2.x *= L.la[f]; g2.y *= L.la[f]; }
                         LN.x -= g2.x; LN.y -= g2.y;
                     }
                 }
@@ -461,11 +581,12 @@ int grow_buckets(pcp_ctx* c, int need) {
 
 int alloc_buckets(pcp_ctx* c, int Kp) {
     dfree(c->dBKey); dfree(c->dBTU); dfree(c->dJKey); dfree(c->dRKey); dfree(c->dROuter);
-    dfree(c->dRTU); dfree(c->dRUT); dfree(c->dJC); dfree(c->dJT);
+    dfree(c->dRTU); dfree(c->dRUT); dfree(c->dJC); dfree(c->dJT); dfree(c->dEr);
     long long n = (long long)std::max(c->N, 1) * Kp;
     PCP_TRY(dalloc(&c->dBKey, n)); PCP_TRY(dalloc(&c->dBTU, n)); PCP_TRY(dalloc(&c->dJKey, n));
     PCP_TRY(dalloc(&c->dRKey, n)); PCP_TRY(dalloc(&c->dROuter, n)); PCP_TRY(dalloc(&c->dRTU, n));
     PCP_TRY(dalloc(&c->dRUT, n)); PCP_TRY(dalloc(&c->dJC, 4 * n)); PCP_TRY(dalloc(&c->dJT, 4 * n));
+    PCP_TRY(dalloc(&c->dEr, n));
     c->Kp = Kp;
     return PCP_OK;
 }
@@ -522,7 +643,8 @@ int run_forces(pcp_ctx* c, int model, const double2* pos, double* edgeLengths, d
                const double* targetEdgeV, const double* targetAreas, double stiffness,
                double compressibility, double2* force, double* energy, int geom, int project) {
     if (c->V == 0) return PCP_OK;
-    int area = (model & PCP_AREA_ENERGY) ? 1 : 0;
+    int sq = (model & PCP_AREA_SQUARED) ? 1 : 0;
+    int area = (model & (PCP_AREA_ENERGY | PCP_AREA_SQUARED)) ? 1 : 0;
     model &= 0xFF;
     int overlap = (model == PCP_NORMAL || model == PCP_HYBRID);
     int edgeOn = (model == PCP_SOFT_BODY || model == PCP_EDGE_ONLY || model == PCP_HYBRID);
@@ -538,14 +660,21 @@ int run_forces(pcp_ctx* c, int model, const double2* pos, double* edgeLengths, d
     }
     if (overlap) {
         size_t lds = ext_lds_bytes(c->Kp);
-        if (area)
-            PCP_LAUNCH_LDS(c, k_exterior_poly<1>, dim3(c->N), dim3(64), lds, c->N, c->Kp, c->dPolyStart,
-                           c->dPolyPack, c->dPackPoly, pos, c->dBcnt, c->dJcnt, c->dJKey, c->dRKey, c->dROuter,
-                           c->dRTU, c->dRUT, c->dJC, c->dJT, c->dF1, c->dLM, c->dLN, c->dAccR);
-        else
-            PCP_LAUNCH_LDS(c, k_exterior_poly<0>, dim3(c->N), dim3(64), lds, c->N, c->Kp, c->dPolyStart,
-                           c->dPolyPack, c->dPackPoly, pos, c->dBcnt, c->dJcnt, c->dJKey, c->dRKey, c->dROuter,
-                           c->dRTU, c->dRUT, c->dJC, c->dJT, c->dF1, c->dLM, c->dLN, c->dAccR);
+        if (sq) {
+            PCP_LAUNCH(c, k_lens_energy, dim3(c->N), dim3(64), c->N, c->Kp, c->dPolyStart, c->dPolyPack,
+                       c->dPackPoly, pos, c->dBcnt, c->dRKey, c->dROuter, c->dRTU, c->dRUT, c->dEr);
+            PCP_LAUNCH_LDS(c, k_exterior_poly<2>, dim3(c->N), dim3(64), lds, c->N, c->Kp,
+                           c->dPolyStart, c->dPolyPack, c->dPackPoly, pos, c->dBcnt, c->dJcnt, c->dJKey, c->dRKey,
+                           c->dROuter, c->dRTU, c->dRUT, c->dJC, c->dJT, c->dF1, c->dLM, c->dLN, c->dAccR, c->dEr);
+        } else if (area) {
+            PCP_LAUNCH_LDS(c, k_exterior_poly<1>, dim3(c->N), dim3(64), lds, c->N, c->Kp,
+                           c->dPolyStart, c->dPolyPack, c->dPackPoly, pos, c->dBcnt, c->dJcnt, c->dJKey, c->dRKey,
+                           c->dROuter, c->dRTU, c->dRUT, c->dJC, c->dJT, c->dF1, c->dLM, c->dLN, c->dAccR, c->dEr);
+        } else {
+            PCP_LAUNCH_LDS(c, k_exterior_poly<0>, dim3(c->N), dim3(64), lds, c->N, c->Kp,
+                           c->dPolyStart, c->dPolyPack, c->dPackPoly, pos, c->dBcnt, c->dJcnt, c->dJKey, c->dRKey,
+                           c->dROuter, c->dRTU, c->dRUT, c->dJC, c->dJT, c->dF1, c->dLM, c->dLN, c->dAccR, c->dEr);
+        }
     }
     bool proj = project && c->maxN <= 64;
     if (geom && proj)

@@ -20,8 +20,32 @@ def polygon_areas(verts):
     return 0.5 * (x * np.roll(y, -1, axis=-1) - np.roll(x, -1, axis=-1) * y).sum(axis=-1)
 
 
-def make_packing(N, n, phi, seed, radii=(1.0, 1.4), noise=0.05):
+def separated_centers(rng, R, L, frac, tries=10000):
+    """Centres uniform in the unit box, placed one by one and re-drawn while closer than
+    frac * (R_i + R_j) / L (minimum image) to a centre already placed."""
+    N = len(R)
+    out = np.empty((N, 2))
+    for i in range(N):
+        for _ in range(tries):
+            c = rng.uniform(0.0, 1.0, 2)
+            if i == 0:
+                break
+            d = out[:i] - c
+            d -= np.round(d)
+            if np.all((d ** 2).sum(1) >= (frac * (R[:i] + R[i]) / L) ** 2):
+                break
+        else:
+            raise RuntimeError("separated_centers: no room; lower frac or phi")
+        out[i] = c
+    return out
+
+
+def make_packing(N, n, phi, seed, radii=(1.0, 1.4), noise=0.05, min_center_frac=None):
     """One packing of N polygons with n vertices each.
+
+    min_center_frac: None = centres uniform; f = centres uniform but no two closer than
+    f * (R_i + R_j) (f = 0.5 keeps every polygon's centre outside the others, so no
+    polygon starts inside another -- a state the edge-crossing overlap cannot see).
 
     Returns dict(nArray, startIndices, positions (AoS flat, 2V), targetAreas[N],
     targetEdgeLengths[N], targetEdgeV[V], boxLength, maxEdgeLength, diameter).
@@ -37,6 +61,8 @@ def make_packing(N, n, phi, seed, radii=(1.0, 1.4), noise=0.05):
     verts = np.stack([r * np.cos(ang), r * np.sin(ang)], axis=-1)      # (N, n, 2), CCW
     areas = polygon_areas(verts)
     L = np.sqrt(areas.sum() / phi)
+    if min_center_frac is not None:
+        centers = separated_centers(rng, R, L, min_center_frac)
     local = verts / L
     pos = (centers[:, None, :] + local) % 1.0
     pos = np.where(pos >= 1.0, 0.0, pos)

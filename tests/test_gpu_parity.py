@@ -13,6 +13,7 @@ from pycudapolygon_amd.generate import make_packing, make_batch, box_size_for
 pytestmark = pytest.mark.gpu
 
 AREA = 0x100
+SQUARED = 0x200
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -112,19 +113,37 @@ def test_force_energy_parity(seed, model):
 
 
 @pytest.mark.parametrize("seed", range(6))
-def test_area_energy_parity(seed):
+@pytest.mark.parametrize("flag", [AREA, SQUARED])
+def test_area_energy_parity(seed, flag):
+    """Area energy and squared lens areas: GPU vs oracle, bit for bit (same exact
+    crossings, cycle origins and accumulation order)."""
     p, B = c1(seed)
     s, mx, nI = run_gpu(p, B, exact=True)
-    s.forces(capi.NORMAL | AREA)
+    s.forces(capi.NORMAL | flag)
     F, E = s.get_forces(0), s.get_energy()[0]
     start, pos = p["startIndices"], p["positions"]
-    nb = O.neighbors(pos, start, B)
-    # oracle records with exact pairing
-    sysO = O.System(start, B, AREA | 5, p["targetEdgeV"], p["targetAreas"])   # hybrid: no projection
+    sysO = O.System(start, B, flag | 5, p["targetEdgeV"], p["targetAreas"])   # hybrid: no projection
     Eo, Fo, nIo = sysO.force_eval(pos)
     assert nIo == nI[0]
-    assert np.abs(F - Fo).max() <= 1e-12 * np.abs(Fo).max()
-    assert abs(E - Eo) <= 1e-12 * abs(Eo)
+    assert np.array_equal(F, Fo), np.abs(F - Fo).max()
+    assert E == Eo
+
+
+def test_fire_squared_lens_areas_converges_c1():
+    """FIRE on squared lens areas (N=64 12-gons, phi=0.80, no polygon starting inside
+    another) reaches max|F| <= 1e-10; the oracle's FIRE ends at the same energy scale."""
+    p = make_packing(64, 12, 0.80, seed=0, min_center_frac=0.5)
+    B = box_size_for(p["maxEdgeLength"])
+    s = capi.System([p], [B], model=capi.NORMAL | SQUARED)
+    prm = s.fire_params(fTol=1e-10)
+    s.fire_init(prm, 1e-3)
+    st = s.fire_minimize(prm, 20000)[0]
+    print(f"[fire] steps={st.steps} E={st.energy:.3e} maxF={st.maxForce:.3e}", flush=True)
+    assert st.converged and st.maxForce <= 1e-10
+    sysO = O.System(p["startIndices"], B, SQUARED, p["targetEdgeV"], p["targetAreas"])
+    xo, fo, sto, steps = sysO.fire_minimize(p["positions"], fTol=1e-10, maxSteps=20000)
+    assert np.abs(fo).max() <= 1e-10
+    assert st.energy < 1e-10 and sto.energy < 1e-10
 
 
 @pytest.mark.parametrize("seed", range(4))

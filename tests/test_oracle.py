@@ -305,6 +305,53 @@ def test_shake_satisfies_constraints():
     assert it2 == 0 and np.array_equal(out2, out)
 
 
+@pytest.mark.parametrize("seed", range(4))
+def test_squared_lens_energy_closed_form(seed):
+    """Squared lens areas on one lens between convex polygons: E = A^2/2 with A the
+    overlap area of an independent clipping geometry, F = -grad E by central
+    differences of that geometry."""
+    SQUARED = 0x200
+    rng = np.random.default_rng(seed)
+    n1, n2 = rng.integers(5, 12), rng.integers(5, 12)
+    ang = rng.uniform(0, 2 * np.pi)
+    d = rng.uniform(0.06, 0.09)
+    p1 = G.regular(n1, 0.06, (0.4, 0.4), rng.uniform(0, 6))
+    p2 = G.regular(n2, 0.05, (0.4 + d * np.cos(ang), 0.4 + d * np.sin(ang)), rng.uniform(0, 6))
+    assert len(G.crossings(p1, p2)) == 2
+    start, pos = G.to_model([p1, p2])
+    sysO = O.System(start, 2, SQUARED | 5, np.zeros(n1 + n2), np.zeros(2))   # hybrid: no projection
+    E, f, nI = sysO.force_eval(pos)
+    A = G.overlap_area(p1, p2)
+    np.testing.assert_allclose(E, A * A / 2, rtol=1e-10)
+    k1 = 2 * n1
+    h = 1e-7
+    fe = np.zeros_like(pos)
+    for q in range(len(pos)):
+        xp, xm = pos.copy(), pos.copy()
+        xp[q] += h
+        xm[q] -= h
+        Ap = G.overlap_area(xp[:k1].reshape(-1, 2), xp[k1:].reshape(-1, 2))
+        Am = G.overlap_area(xm[:k1].reshape(-1, 2), xm[k1:].reshape(-1, 2))
+        fe[q] = -(Ap * Ap / 2 - Am * Am / 2) / (2 * h)
+    np.testing.assert_allclose(f, fe, rtol=0, atol=1e-6 * np.abs(fe).max())
+
+
+def test_squared_lens_fire_converges_where_area_energy_stalls():
+    """C1 (N=64, phi=0.80, no polygon starting inside another): FIRE on the area energy
+    stalls at a kink (flush contacts, rollbacks shrink dt); on squared lens areas it
+    reaches max|F| <= 1e-10."""
+    from pycudapolygon_amd.generate import make_packing, box_size_for
+    p = make_packing(64, 12, 0.80, seed=0, min_center_frac=0.5)
+    B = box_size_for(p["maxEdgeLength"])
+    res = {}
+    for flag in (0x100, 0x200):
+        sysO = O.System(p["startIndices"], B, flag, p["targetEdgeV"], p["targetAreas"])
+        x, f, st, steps = sysO.fire_minimize(p["positions"], fTol=1e-10, maxSteps=3000)
+        res[flag] = (np.abs(f).max(), st.dt)
+    assert res[0x200][0] <= 1e-10
+    assert res[0x100][0] > 1e-6 and res[0x100][1] < 1e-4
+
+
 def test_soft_body_penalties_closed_form():
     """softBody = edge springs + area springs only (kernels.cuh:1284-1352):
     E = sum_m k (l_m - l0_m)^2 / 2 + sum_s c (A_s - A0_s)^2 / 2 (A0 >= 1e-14), computed here
