@@ -225,8 +225,9 @@ int pcp_fire_get(pcp_ctx* ctx, pcp_fire_status* status);
 int pcp_profile_kernel(pcp_ctx* ctx, const char* name);
 int pcp_profile_read(pcp_ctx* ctx, double* totalMs, int* launches);
 int pcp_stream_copy(pcp_ctx* ctx, double* dst, const double* src, long long n);
-/* work of the last detection: candidate polygon pairs and the edge pairs they hold
- * (nA*nB per pair of distinct polygons, n(n-3)/2 per polygon with itself) */
+/* work of the last detection: tiles tested (each polygon with itself, each pair of
+ * distinct polygons whose bounding circles meet once from each side) and the edge pairs
+ * they hold (nA*nB per tile of distinct polygons, n(n-3)/2 per polygon with itself) */
 int pcp_detect_stats(pcp_ctx* ctx, long long* polygonPairs, long long* edgePairs);
 
 #ifdef __cplusplus

@@ -10,12 +10,13 @@
 //                   cell grid whose side is at least the largest bounding diameter
 //   k_detect_poly   one wavefront per polygon A: its edges staged in LDS; the polygons of
 //                   the 3x3 cells around it gathered by the lanes in parallel, kept when
-//                   their id is larger and their circle meets A's; for A itself and each
-//                   kept polygon, its edges held in lanes and tested against A's.  An edge
-//                   pair is tested when the midpoint cells of the two edges in the user's
-//                   cell grid are neighbours (the reference's stencil), with the
-//                   reference's arithmetic; a crossing is appended to the si bucket of the
-//                   polygon its record belongs to and to the sj bucket of the other one
+//                   their circle meets A's; for A itself and each kept polygon, its edges
+//                   held in lanes and tested against A's.  An edge pair is tested when the
+//                   midpoint cells of the two edges in the user's cell grid are neighbours
+//                   (the reference's stencil), with the reference's arithmetic (edge of
+//                   lower vertex id first).  Each pair of polygons is tested by both of
+//                   their wavefronts, and each keeps the records that belong to its own
+//                   buckets (si or sj), so every bucket has one writer and needs no atomics
 //
 // Two crossing edges lie in polygons whose bounding circles meet, so the candidates
 // contain every crossing the reference can find, and the cell test drops exactly the
@@ -132,11 +133,12 @@ __device__ __forceinline__ bool circles_meet(double4 a, double4 b) {
 // centre in a B x B grid with side >= (1 + 1e-5) x the largest circle diameter, so that
 // circles that meet have centres in neighbouring cells.  B = 1 (one cell) when B would be
 // below 3 or a polygon is big.  Writes pcStart[p*(PC_CELLS+1) + 0..B*B], the sorted
-// polygon ids into pcSorted[packPoly[p]..], and packB[p] = B | big << 30.
+// polygon ids into pcSorted[packPoly[p]..] (their circles into sBounds[..]), and
+// packB[p] = B | big << 30.
 __global__ __launch_bounds__(PP_BLK) void k_poly_cells(const int* __restrict__ packPoly,
                                                       const double4* __restrict__ bounds,
                                                       int* __restrict__ pcStart, int* __restrict__ pcSorted,
-                                                      int* __restrict__ packB) {
+                                                      int* __restrict__ packB, double4* __restrict__ sBounds) {
     __shared__ int cStart[PC_CELLS + 1];
     __shared__ int cCur[PC_CELLS];
     __shared__ double redR[PP_BLK / 64];
@@ -206,19 +208,23 @@ __global__ __launch_bounds__(PP_BLK) void k_poly_cells(const int* __restrict__ p
         double4 b = bounds[s];
         int slot = atomicAdd(&cCur[pc_coord(b.y, B) * B + pc_coord(b.x, B)], 1);
         pcSorted[s0 + slot] = s;
+        sBounds[s0 + slot] = b;
     }
 }
 
 // ---------------------------------------------------------------- edge pairs
+// LDS hand-off between the lanes of one wavefront: the LDS executes a wavefront's
+// instructions in order, so only compiler reordering has to be prevented (no wait for
+// outstanding global loads is needed here)
 __device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __asm__ __volatile__("" ::: "memory");
     __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// cell of an edge in the user grid as x | y << 16
+__device__ __forceinline__ int cell_xy(int flat, int B) { return (flat % B) | ((flat / B) << 16); }
 __device__ __forceinline__ bool cells_adjacent(int ca, int cb, int B) {
-    int ax = ca % B, ay = ca / B, bx = cb % B, by = cb / B;
-    int dx = abs(ax - bx), dy = abs(ay - by);
+    int dx = abs((ca & 0xFFFF) - (cb & 0xFFFF)), dy = abs((ca >> 16) - (cb >> 16));
     return (dx <= 1 || dx == B - 1) && (dy <= 1 || dy == B - 1);
 }
 
@@ -228,19 +234,48 @@ struct EdgeStage {
     int* C;
 };
 
-// Test the edges of polygon A (staged in LDS, chunk a0 of at most 64 edges) against
-// every edge of polygon Bq (held in lanes).  A <= Bq; A == Bq tests each pair of
-// non-adjacent edges once.
+// Crossings found by the wavefront of polygon A, appended to A's own buckets: a record
+// whose si is A to the si bucket, one whose sj is A to the sj bucket (a crossing of A
+// with itself to both).  The wavefront is the only writer of A's buckets, so slots come
+// from ballots, without atomics.
+struct Appender {
+    long long base;   // A * Kp
+    int Kp;
+    int nSi, nSj;     // wavefront-uniform counts so far
+    uint64_t* bKey;
+    float2* bTU;
+    uint64_t* jKey;
+};
+__device__ __forceinline__ void append_records(Appender& ap, int lane, bool toSi, bool toSj, uint64_t key,
+                                               float2 tu) {
+    unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    unsigned long long mi = __ballot(toSi), mj = __ballot(toSj);
+    if (toSi) {
+        int slot = ap.nSi + __popcll(mi & below);
+        if (slot < ap.Kp) { ap.bKey[ap.base + slot] = key; ap.bTU[ap.base + slot] = tu; }
+    }
+    if (toSj) {
+        int slot = ap.nSj + __popcll(mj & below);
+        if (slot < ap.Kp) ap.jKey[ap.base + slot] = key;
+    }
+    ap.nSi += __popcll(mi);
+    ap.nSj += __popcll(mj);
+}
+
+constexpr int TILE_UNROLL = 4;
+
+// Test the edges of polygon A (staged in LDS, chunks of 64) against every edge of polygon
+// Bq (held in lanes).  Each edge pair is computed with the edge of lower vertex id as the
+// reference's edge 1 (updateNeighborsKernel tests id1 against nid > id1), so the tests
+// from A's and from Bq's wavefront are the same arithmetic.  A == Bq tests each pair of
+// non-adjacent edges once.  Every lane runs the same number of iterations (ballots).
 template <int CELLS>
-__device__ void edge_tile(int lane, int A, int stA, int nA, int Bq, int sAl, int sBl, int Bu, int pbase,
-                          const int* __restrict__ polyStart, const double2* __restrict__ pos,
-                          const int* __restrict__ cellOf, EdgeStage sg, int& staged, int Kp,
-                          int* __restrict__ bcnt, uint64_t* __restrict__ bKey, float2* __restrict__ bTU,
-                          int* __restrict__ jcnt, uint64_t* __restrict__ jKey, int* __restrict__ num,
-                          int* __restrict__ scal, uint64_t& kmax) {
+__device__ void edge_tile(int lane, int A, int stA, int nA, int Bq, int stB, int nB, int sAl, int sBl, int Bu,
+                          const double2* __restrict__ pos, const int* __restrict__ cellOf, EdgeStage sg,
+                          int& staged, Appender& ap, int* __restrict__ num, uint64_t& kmax) {
     const double eps = 1e-12;
     bool self = (A == Bq);
-    int stB = polyStart[Bq], nB = polyStart[Bq + 1] - stB;
+    bool aFirst = (A < Bq);
     for (int a0 = 0; a0 < nA; a0 += 64) {
         int na = min(64, nA - a0);
         if (staged != a0) {
@@ -250,7 +285,7 @@ __device__ void edge_tile(int lane, int A, int stA, int nA, int Bq, int sAl, int
                 d2 p0 = ldpos(pos, stA + a), p1 = ldpos(pos, stA + ((a + 1 == nA) ? 0 : a + 1));
                 sg.P[lane] = make_double2(p0.x, p0.y);
                 sg.R[lane] = make_double2(wrap(p1.x - p0.x), wrap(p1.y - p0.y));
-                sg.C[lane] = CELLS ? cellOf[stA + a] : edge_cell(p0, p1, Bu);
+                sg.C[lane] = cell_xy(CELLS ? cellOf[stA + a] : edge_cell(p0, p1, Bu), Bu);
             }
             wave_lds_sync();
             staged = a0;
@@ -259,61 +294,88 @@ __device__ void edge_tile(int lane, int A, int stA, int nA, int Bq, int sAl, int
             int nb = min(64, nB - b0);
             int G = 64 / nb;
             int g = lane / nb, bl = lane - g * nb;
-            if (g >= G) continue;
+            bool act = g < G;
             int b = b0 + bl;
-            d2 q0 = ldpos(pos, stB + b), q1 = ldpos(pos, stB + ((b + 1 == nB) ? 0 : b + 1));
-            d2 Q = q0, S = mk(wrap(q1.x - q0.x), wrap(q1.y - q0.y));
-            int cb = CELLS ? cellOf[stB + b] : edge_cell(q0, q1, Bu);
-            for (int al = g; al < na; al += G) {
-                int a = a0 + al;
-                if (self) {
-                    if (b <= a) continue;
-                    if (b == a + 1 || (a == 0 && b == nA - 1)) continue;   // shared vertex
+            d2 Q = mk(0.0, 0.0), S = mk(0.0, 0.0);
+            int cb = 0;
+            if (act) {
+                d2 q0 = ldpos(pos, stB + b), q1 = ldpos(pos, stB + ((b + 1 == nB) ? 0 : b + 1));
+                Q = q0;
+                S = mk(wrap(q1.x - q0.x), wrap(q1.y - q0.y));
+                cb = cell_xy(CELLS ? cellOf[stB + b] : edge_cell(q0, q1, Bu), Bu);
+            }
+            int iters = (na + TILE_UNROLL * G - 1) / (TILE_UNROLL * G);
+            for (int it = 0; it < iters; ++it) {
+                double2 PA[TILE_UNROLL], RA[TILE_UNROLL];
+                int CA[TILE_UNROLL];
+                bool ok[TILE_UNROLL];
+#pragma unroll
+                for (int u = 0; u < TILE_UNROLL; ++u) {   // LDS loads of the unrolled pairs first
+                    int al = g + (it * TILE_UNROLL + u) * G;
+                    int a = a0 + al;
+                    ok[u] = act && al < na;
+                    if (self) ok[u] = ok[u] && b > a && b != a + 1 && !(a == 0 && b == nA - 1);
+                    int ls = ok[u] ? al : 0;
+                    PA[u] = sg.P[ls];
+                    RA[u] = sg.R[ls];
+                    CA[u] = sg.C[ls];
                 }
-                if (!cells_adjacent(sg.C[al], cb, Bu)) continue;
-                double2 P1 = sg.P[al], R1 = sg.R[al];
-                double gx = wrap(Q.x - P1.x);
-                double gy = wrap(Q.y - P1.y);
-                double denom = R1.x * S.y - R1.y * S.x;
-                if (fabs(denom) < eps) continue;
-                double tt = (gx * S.y - gy * S.x) / denom;
-                double uu = (gx * R1.y - gy * R1.x) / denom;
-                if (tt > 0.0 && tt < 1.0 && uu > 0.0 && uu < 1.0) {
-                    uint64_t key;
-                    float2 tu;
-                    // updateCompactedIntersectionsKernel kernels.cuh:857-876
-                    if (denom > 0) {
-                        key = ((uint64_t)(uint16_t)sAl << 48) | ((uint64_t)(uint16_t)sBl << 32) |
-                              ((uint64_t)(uint16_t)b << 16) | (uint64_t)(uint16_t)a;
-                        tu = make_float2((float)tt, (float)uu);
-                    } else {
-                        key = ((uint64_t)(uint16_t)sBl << 48) | ((uint64_t)(uint16_t)sAl << 32) |
-                              ((uint64_t)(uint16_t)a << 16) | (uint64_t)(uint16_t)b;
-                        tu = make_float2((float)uu, (float)tt);
+#pragma unroll
+                for (int u = 0; u < TILE_UNROLL; ++u) {
+                    int al = g + (it * TILE_UNROLL + u) * G;
+                    int a = a0 + al;
+                    bool hit = ok[u] && cells_adjacent(CA[u], cb, Bu);
+                    // edge 1 = the edge of lower vertex id
+                    bool af = self || aFirst;
+                    d2 P1 = af ? mk(PA[u].x, PA[u].y) : Q, R1 = af ? mk(RA[u].x, RA[u].y) : S;
+                    d2 P2 = af ? Q : mk(PA[u].x, PA[u].y), R2 = af ? S : mk(RA[u].x, RA[u].y);
+                    double denom = R1.x * R2.y - R1.y * R2.x;
+                    hit = hit && !(fabs(denom) < eps);
+                    double gx = wrap(P2.x - P1.x);
+                    double gy = wrap(P2.y - P1.y);
+                    double nt = gx * R2.y - gy * R2.x;
+                    double nu = gx * R1.y - gy * R1.x;
+                    // t = nt/denom and u = nu/denom lie in (0,1) only if nt and nu lie strictly
+                    // between 0 and denom (a correctly rounded quotient keeps its sign and is
+                    // >= 1 when |numerator| >= |denom|); only those pairs need the divisions
+                    hit = hit && ((denom > 0.0) ? (nt > 0.0 && nt < denom && nu > 0.0 && nu < denom)
+                                                : (nt < 0.0 && nt > denom && nu < 0.0 && nu > denom));
+                    uint64_t key = 0;
+                    float2 tu = make_float2(0.f, 0.f);
+                    bool toSi = false, toSj = false;
+                    if (hit) {
+                        double tt = nt / denom;
+                        double uu = nu / denom;
+                        if (tt > 0.0 && tt < 1.0 && uu > 0.0 && uu < 1.0) {
+                            int s1l = af ? sAl : sBl, s2l = af ? sBl : sAl;
+                            int n1 = af ? a : b, l2 = af ? b : a;
+                            // updateCompactedIntersectionsKernel kernels.cuh:857-876
+                            if (denom > 0) {
+                                key = ((uint64_t)(uint16_t)s1l << 48) | ((uint64_t)(uint16_t)s2l << 32) |
+                                      ((uint64_t)(uint16_t)l2 << 16) | (uint64_t)(uint16_t)n1;
+                                tu = make_float2((float)tt, (float)uu);
+                            } else {
+                                key = ((uint64_t)(uint16_t)s2l << 48) | ((uint64_t)(uint16_t)s1l << 32) |
+                                      ((uint64_t)(uint16_t)n1 << 16) | (uint64_t)(uint16_t)l2;
+                                tu = make_float2((float)uu, (float)tt);
+                            }
+                            toSi = (key_si(key) == sAl);
+                            toSj = (key_sj(key) == sAl);
+                            if (toSi) kmax = (key > kmax) ? key : kmax;
+                            if (CELLS && af) atomicAdd(&num[stA + a], 1);   // per-edge count
+                        }
                     }
-                    kmax = (key > kmax) ? key : kmax;
-                    if (CELLS) atomicAdd(&num[stA + a], 1);   // per-edge count (updateNeighbors)
-                    int sig = pbase + key_si(key), sjg = pbase + key_sj(key);
-                    int b1 = atomicAdd(&bcnt[sig], 1);
-                    if (b1 < Kp) {
-                        bKey[(long long)sig * Kp + b1] = key;
-                        bTU[(long long)sig * Kp + b1] = tu;
-                    } else {
-                        atomicMax(&scal[SC_BUCKET], b1 + 1);
-                    }
-                    int b2 = atomicAdd(&jcnt[sjg], 1);
-                    if (b2 < Kp) jKey[(long long)sjg * Kp + b2] = key;
-                    else atomicMax(&scal[SC_BUCKET], b2 + 1);
+                    append_records(ap, lane, toSi, toSj, key, tu);
                 }
             }
         }
     }
 }
 
-// One wavefront per polygon A: the polygons of the 3x3 block of cells around A's cell
-// are loaded by the lanes in parallel and kept when their id is larger than A's and
-// their circle meets A's; A's edges are staged in LDS once and tested against A itself
-// and against every kept polygon.
+// One wavefront per polygon A (in cell order): A's edges staged in LDS; the polygons of
+// the 3x3 block of cells around A's cell gathered by the lanes in parallel, kept when
+// their circle meets A's; A tested against itself and against every kept polygon, and
+// every crossing involving A appended to A's buckets.
 // CELLS = 0: the user-grid cell of every edge is computed from the live positions (the
 // fused evaluation, where cells and crossings come from the same positions);
 // CELLS = 1: the cells of the last cell-list build are used (updateNeighbors after
@@ -323,15 +385,17 @@ __global__ __launch_bounds__(BLK) void k_detect_poly(
     int N, const int* __restrict__ polyStart, const int* __restrict__ polyPack,
     const int* __restrict__ packPoly, const int* __restrict__ box, const double2* __restrict__ pos,
     const int* __restrict__ cellOf, const double4* __restrict__ bounds, const int* __restrict__ pcStart,
-    const int* __restrict__ pcSorted, const int* __restrict__ packB, int Kp, int* __restrict__ bcnt,
-    uint64_t* __restrict__ bKey, float2* __restrict__ bTU, int* __restrict__ jcnt, uint64_t* __restrict__ jKey,
-    unsigned long long* __restrict__ packMaxKey, int* __restrict__ num, int* __restrict__ scal) {
+    const int* __restrict__ pcSorted, const double4* __restrict__ sBounds, const int* __restrict__ packB, int Kp,
+    int* __restrict__ bcnt, uint64_t* __restrict__ bKey, float2* __restrict__ bTU, int* __restrict__ jcnt,
+    uint64_t* __restrict__ jKey, unsigned long long* __restrict__ packMaxKey, int* __restrict__ num,
+    int* __restrict__ scal) {
     __shared__ double2 sP[BLK / 64][64];
     __shared__ double2 sR[BLK / 64][64];
     __shared__ int sC[BLK / 64][64];
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int A = blockIdx.x * (BLK / 64) + w;
-    if (A >= N) return;   // whole wavefront
+    int idx = xcd_block(blockIdx.x, gridDim.x) * (BLK / 64) + w;
+    if (idx >= N) return;   // whole wavefront
+    int A = pcSorted[idx];   // polygons in cell order
     EdgeStage sg{sP[w], sR[w], sC[w]};
     int staged = -1;
     int stA = polyStart[A], nA = polyStart[A + 1] - stA;
@@ -344,10 +408,9 @@ __global__ __launch_bounds__(BLK) void k_detect_poly(
     int sAl = A - s0;
     double4 bA = bounds[A];
     uint64_t kmax = 0;
-    if (nA >= 4)
-        edge_tile<CELLS>(lane, A, stA, nA, A, sAl, sAl, Bu, s0, polyStart, pos, cellOf, sg, staged, Kp, bcnt,
-                         bKey, bTU, jcnt, jKey, num, scal, kmax);
-    // candidate cells: lanes 0..8 hold the ranges of the 3x3 block (one cell when Bp == 1)
+    Appender ap{(long long)A * Kp, Kp, 0, 0, bKey, bTU, jKey};
+    // candidate cells: lanes 0..8 hold the ranges of the 3x3 block (one cell when Bp == 1);
+    // loaded before the polygon's own tile so that the two latencies overlap
     long long gbase = (long long)pk * (PC_CELLS + 1);
     int ncell = (Bp == 1) ? 1 : 9;
     int lo = 0, cnt = 0;
@@ -362,6 +425,7 @@ __global__ __launch_bounds__(BLK) void k_detect_poly(
         lo = pcStart[gbase + cell];
         cnt = pcStart[gbase + cell + 1] - lo;
     }
+    if (nA >= 4) edge_tile<CELLS>(lane, A, stA, nA, A, stA, nA, sAl, sAl, Bu, pos, cellOf, sg, staged, ap, num, kmax);
     int incl = cnt;
 #pragma unroll
     for (int o = 1; o < 16; o <<= 1) {
@@ -370,35 +434,46 @@ __global__ __launch_bounds__(BLK) void k_detect_poly(
     }
     int T = __shfl(incl, ncell - 1);
     for (int c0 = 0; c0 < T; c0 += 64) {
-        int idx = c0 + lane;
+        int ci = c0 + lane;
         int q = 0;
         bool found = false;
         for (int k = 0; k < ncell; ++k) {   // every lane takes part in the shuffles
             int hiK = __shfl(incl, k), cK = __shfl(cnt, k), loK = __shfl(lo, k);
-            if (!found && idx < hiK) { q = loK + idx - (hiK - cK); found = true; }
+            if (!found && ci < hiK) { q = loK + ci - (hiK - cK); found = true; }
         }
-        int Bq = -1;
-        if (idx < T) {
+        int Bq = -1, stq = 0, enq = 0;
+        if (ci < T) {
             int cand = pcSorted[s0 + q];
-            if (cand > A && (big || circles_meet(bA, bounds[cand]))) Bq = cand;
+            double4 bc = sBounds[s0 + q];
+            if (cand != A && (big || circles_meet(bA, bc))) {
+                Bq = cand;
+                stq = polyStart[cand];
+                enq = polyStart[cand + 1];
+            }
         }
         unsigned long long keep = __ballot(Bq >= 0);
         while (keep) {
             int j = __ffsll((long long)keep) - 1;
             keep &= keep - 1;
             int Bsel = __shfl(Bq, j);
-            edge_tile<CELLS>(lane, A, stA, nA, Bsel, sAl, Bsel - s0, Bu, s0, polyStart, pos, cellOf, sg, staged,
-                             Kp, bcnt, bKey, bTU, jcnt, jKey, num, scal, kmax);
+            int stB = __shfl(stq, j), nB = __shfl(enq, j) - stB;
+            edge_tile<CELLS>(lane, A, stA, nA, Bsel, stB, nB, sAl, Bsel - s0, Bu, pos, cellOf, sg, staged, ap, num,
+                             kmax);
         }
     }
-    // largest key of the packing (the pairing's last-record case): one atomic per
-    // wavefront, only when it raises the current value
+    // A's bucket counts; the largest key of the packing (the pairing's last-record case)
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
         unsigned long long y = __shfl_xor((unsigned long long)kmax, o);
         kmax = (y > kmax) ? y : kmax;
     }
-    if (lane == 0 && kmax > 0 && kmax > packMaxKey[pk]) atomicMax(&packMaxKey[pk], (unsigned long long)kmax);
+    if (lane == 0) {
+        bcnt[A] = ap.nSi;
+        jcnt[A] = ap.nSj;
+        int mx = max(ap.nSi, ap.nSj);
+        if (mx > Kp) atomicMax(&scal[SC_BUCKET], mx);
+        if (kmax > 0 && kmax > packMaxKey[pk]) atomicMax(&packMaxKey[pk], (unsigned long long)kmax);
+    }
 }
 
 // exact maximum of the per-edge crossing counts (updateNeighbors' overflow check)
@@ -441,15 +516,17 @@ int run_detect(pcp_ctx* c, const double2* pos, int cellsFromList) {
     if (c->V == 0) return PCP_OK;
     PCP_TRY(launch_bounds(c, pos));
     PCP_LAUNCH(c, k_poly_cells, dim3(c->P), dim3(PP_BLK), c->dPackPoly, c->dBounds, c->dPcStart, c->dPcSorted,
-               c->dPackB);
+               c->dPackB, c->dSBounds);
     int grid = (c->N + BLK / 64 - 1) / (BLK / 64);
     if (cellsFromList)
         PCP_LAUNCH(c, k_detect_poly<1>, dim3(grid), dim3(BLK), c->N, c->dPolyStart, c->dPolyPack, c->dPackPoly,
-                   c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted, c->dPackB, c->Kp, c->dBcnt,
+                   c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted, c->dSBounds, c->dPackB, c->Kp,
+                   c->dBcnt,
                    c->dBKey, c->dBTU, c->dJcnt, c->dJKey, c->dPackMaxKey, c->dNum, c->dScal);
     else
         PCP_LAUNCH(c, k_detect_poly<0>, dim3(grid), dim3(BLK), c->N, c->dPolyStart, c->dPolyPack, c->dPackPoly,
-                   c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted, c->dPackB, c->Kp, c->dBcnt,
+                   c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted, c->dSBounds, c->dPackB, c->Kp,
+                   c->dBcnt,
                    c->dBKey, c->dBTU, c->dJcnt, c->dJKey, c->dPackMaxKey, c->dNum, c->dScal);
     return PCP_OK;
 }
@@ -458,14 +535,22 @@ int detect_alloc(pcp_ctx* c) {
     c->maxN = max_poly_vertices(c);
     c->seg = seg_width(c->maxN);
     PCP_TRY(dalloc(&c->dBounds, std::max(c->N, 1)));
+    PCP_TRY(dalloc(&c->dSBounds, std::max(c->N, 1)));
     PCP_TRY(dalloc(&c->dPcSorted, std::max(c->N, 1)));
+    {   // identity until the first detection sorts the polygons by cell (kernels that visit
+        // polygons in cell order also run for models without overlap)
+        std::vector<int> iota(std::max(c->N, 1));
+        for (int s = 0; s < (int)iota.size(); ++s) iota[s] = s;
+        PCP_HIP(hipMemcpy(c->dPcSorted, iota.data(), sizeof(int) * iota.size(), hipMemcpyHostToDevice));
+    }
     PCP_TRY(dalloc(&c->dPcStart, (long long)c->P * (PC_CELLS + 1)));
     PCP_TRY(dalloc(&c->dPackB, std::max(c->P, 1)));
     return PCP_OK;
 }
 
 // Work of the last detection, recomputed on the host from the device's polygon cells
-// with the kernel's candidate rule: polygon pairs tested and the edge pairs they hold.
+// with the kernel's candidate rule: tiles tested (a polygon with itself, and each pair of
+// polygons once from each side) and the edge pairs they hold.
 int detect_stats(pcp_ctx* c, long long* polygonPairs, long long* edgePairs) {
     PCP_HIP(hipStreamSynchronize(c->stream));
     std::vector<double4> bd(c->N);
@@ -493,7 +578,7 @@ int detect_stats(pcp_ctx* c, long long* polygonPairs, long long* edgePairs) {
                 int cell = cy * B + cx;
                 for (int q = st[cell]; q < st[cell + 1]; ++q) {
                     int Bq = sorted[s0 + q];
-                    if (Bq <= A) continue;
+                    if (Bq == A) continue;
                     if (!big) {
                         double dx = wrapH(bd[Bq].x - bd[A].x), dy = wrapH(bd[Bq].y - bd[A].y);
                         double rr = (bd[A].z + bd[Bq].z) * (1.0 + 1e-6) + 1e-12;

@@ -124,4 +124,13 @@ __device__ __forceinline__ uint64_t bucket_order(uint64_t k) {
     return ((k & 0xFFFFFFFFull) << 16) | (k >> 48);
 }
 
+// Workgroups are dispatched to the 8 XCDs in turn (workgroup b on XCD b % 8), each XCD
+// with its own L2.  Renumber the G workgroups of a launch so that XCD x runs a contiguous
+// range of logical indices; with polygons in cell order, each XCD then works on one
+// region of the box and finds its neighbours' data in its own L2.
+__device__ __forceinline__ int xcd_block(int b, int G) {
+    int q = G / 8, r = G % 8, x = b % 8, k = b / 8;
+    return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
 }  // namespace pcp

@@ -32,38 +32,37 @@ __global__ void k_fire_init(int P, double dtv, double alpha0, double* dt, double
     fs[FS_NF * p + FS_MAXF] = 0.0;
 }
 
-// per packing: backups for rollback / retry, flag reset, E_pre (model.cpp:411-412)
-__global__ void k_fire_prologue(int P, const int* __restrict__ active, const double* __restrict__ dt,
-                                const double* __restrict__ alpha, const int* __restrict__ nPos,
-                                const int* __restrict__ steps, const double* __restrict__ energy,
-                                double* __restrict__ epre, int* __restrict__ flags, double* __restrict__ fs,
-                                int* __restrict__ shakeIt, int* __restrict__ summary) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < 4 && p < 4) summary[p] = 0;
-    if (p >= P) return;
-    int* fl = flags + FL_NF * p;
-    fl[FL_FAIL] = 0;
-    fl[FL_ROLLBACK] = 0;
-    fl[FL_ACCEPT] = 0;
-    fl[FL_ZEROV] = 0;
-    fl[FL_ACTIVEBAK] = active[p];
-    shakeIt[p] = 0;
-    shakeIt[P + p] = 0;
-    if (!active[p]) return;
-    epre[p] = energy[p];
-    fs[FS_NF * p + FS_DTB] = dt[p];
-    fs[FS_NF * p + FS_ALB] = alpha[p];
-    fs[FS_NF * p + FS_NPB] = (double)nPos[p];
-    fs[FS_NF * p + FS_STEPB] = (double)steps[p];
-}
-
-// velocity-Verlet position update with backups of x and v
-__global__ void k_fire_begin(int V, const int* __restrict__ shapeId, const int* __restrict__ polyPack,
+// Step start.  Per packing (thread p < P): backups for rollback / retry, flag reset,
+// E_pre (model.cpp:411-412).  Per vertex (thread i < V): velocity-Verlet half kick and
+// drift, with backups of x and v.
+__global__ void k_fire_begin(int V, int P, const int* __restrict__ shapeId, const int* __restrict__ polyPack,
                              const int* __restrict__ active, const double* __restrict__ dt,
-                             double2* __restrict__ x, double2* __restrict__ v, const double2* __restrict__ F,
-                             double2* __restrict__ xbak, double2* __restrict__ vbak,
-                             double2* __restrict__ xpre) {
+                             const double* __restrict__ alpha, const int* __restrict__ nPos,
+                             const int* __restrict__ steps, const double* __restrict__ energy,
+                             double* __restrict__ epre, int* __restrict__ flags, double* __restrict__ fs,
+                             int* __restrict__ shakeIt, int* __restrict__ summary, double2* __restrict__ x,
+                             double2* __restrict__ v, const double2* __restrict__ F, double2* __restrict__ xbak,
+                             double2* __restrict__ vbak, double2* __restrict__ xpre) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 4) summary[i] = 0;
+    if (i < P) {
+        int p = i;
+        int* fl = flags + FL_NF * p;
+        fl[FL_FAIL] = 0;
+        fl[FL_ROLLBACK] = 0;
+        fl[FL_ACCEPT] = 0;
+        fl[FL_ZEROV] = 0;
+        fl[FL_ACTIVEBAK] = active[p];
+        shakeIt[p] = 0;
+        shakeIt[P + p] = 0;
+        if (active[p]) {
+            epre[p] = energy[p];
+            fs[FS_NF * p + FS_DTB] = dt[p];
+            fs[FS_NF * p + FS_ALB] = alpha[p];
+            fs[FS_NF * p + FS_NPB] = (double)nPos[p];
+            fs[FS_NF * p + FS_STEPB] = (double)steps[p];
+        }
+    }
     if (i >= V) return;
     int p = polyPack[shapeId[i]];
     if (!active[p]) return;
@@ -110,11 +109,10 @@ __global__ void k_fire_post_shake(int V, int P, int shakeIter, const int* __rest
 }
 
 // energy test (model.cpp:444-470)
-__global__ void k_fire_control(int P, const int* __restrict__ active, const double* __restrict__ energy,
-                               const double* __restrict__ epre, double alphaStart, double fDec, double* dt,
-                               double* alpha, int* nPos, int* flags) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P || !active[p]) return;
+__device__ void fire_control(int p, const int* __restrict__ active, const double* __restrict__ energy,
+                             const double* __restrict__ epre, double alphaStart, double fDec, double* dt,
+                             double* alpha, int* nPos, int* flags) {
+    if (!active[p]) return;
     int* fl = flags + FL_NF * p;
     double ePost = energy[p], ePre = epre[p];
     if (fl[FL_FAIL]) {
@@ -133,7 +131,24 @@ __global__ void k_fire_control(int P, const int* __restrict__ active, const doub
     }
 }
 
-// second half-kick and per-polygon partial sums of F.v, v.v, F.F (vertex order)
+// block per packing: energy = fixed-order sum of the polygon energies, then the test
+__global__ __launch_bounds__(BLK) void k_fire_energy(int N, const int* __restrict__ packPoly,
+                                                    const double* __restrict__ epoly, double* __restrict__ energy,
+                                                    const int* __restrict__ active, const double* __restrict__ epre,
+                                                    double alphaStart, double fDec, double* dt, double* alpha,
+                                                    int* nPos, int* flags) {
+    __shared__ double L[DR_CHUNKS * BLK];
+    __shared__ double part[BLK];
+    int p = blockIdx.x;
+    int a0 = packPoly[p], n = packPoly[p + 1] - a0;
+    double e = det_sum_block(epoly + a0, n, L, part);
+    if (threadIdx.x == 0) {
+        energy[p] = e;
+        fire_control(p, active, energy, epre, alphaStart, fDec, dt, alpha, nPos, flags);
+    }
+}
+
+// second half-kick and per-polygon partial sums of F.v, v.v, F.F (vertex order); max |F|
 __global__ void k_fire_kick(int N, const int* __restrict__ polyStart, const int* __restrict__ polyPack,
                             const int* __restrict__ active, const int* __restrict__ flags,
                             const double* __restrict__ dt, double2* __restrict__ v,
@@ -141,7 +156,11 @@ __global__ void k_fire_kick(int N, const int* __restrict__ polyStart, const int*
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= N) return;
     int p = polyPack[s];
-    double a = 0.0, b = 0.0, c = 0.0;
+    double a = 0.0, b = 0.0, c = 0.0, mf = 0.0;
+    for (int m = polyStart[s]; m < polyStart[s + 1]; ++m) {   // max |force component|
+        double2 fi = F[m];
+        mf = fmax(mf, fmax(fabs(fi.x), fabs(fi.y)));
+    }
     if (active[p] && flags[FL_NF * p + FL_ACCEPT]) {
         double h = dt[p];
         for (int m = polyStart[s]; m < polyStart[s + 1]; ++m) {
@@ -160,17 +179,16 @@ __global__ void k_fire_kick(int N, const int* __restrict__ polyStart, const int*
     red[s] = a;
     red[N + s] = b;
     red[2LL * N + s] = c;
+    red[3LL * N + s] = mf;
 }
 
 // bend + P>0 logic (model.cpp:471-485)
-__global__ void k_fire_control2(int P, const int* __restrict__ active, const double* __restrict__ sums,
-                                int nMin, double dtMax, double alphaStart, double fAlpha, double fInc,
-                                double fDec, double* dt, double* alpha, int* nPos, int* flags, double* fs) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P || !active[p]) return;
+__device__ void fire_control2(int p, const int* __restrict__ active, double Pw, double vv, double ff,
+                              int nMin, double dtMax, double alphaStart, double fAlpha, double fInc,
+                              double fDec, double* dt, double* alpha, int* nPos, int* flags, double* fs) {
+    if (!active[p]) return;
     int* fl = flags + FL_NF * p;
     if (!fl[FL_ACCEPT]) return;
-    double Pw = sums[p], vv = sums[P + p], ff = sums[2 * P + p];
     double vnorm = sqrt(vv), fnorm = sqrt(ff);
     double a = alpha[p];
     fs[FS_NF * p + FS_SCALE] = (fnorm > 0.0) ? (a * vnorm) / fnorm : 0.0;
@@ -200,8 +218,8 @@ __global__ void k_fire_bend(int V, const int* __restrict__ shapeId, const int* _
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= V) return;
     int p = polyPack[shapeId[i]];
-    if (!active[p]) return;
     const int* fl = flags + FL_NF * p;
+    if (!fl[FL_ACTIVEBAK]) return;   // active when the step began
     if (fl[FL_ACCEPT]) {
         if (fl[FL_ZEROV]) {
             v[i] = make_double2(0.0, 0.0);
@@ -219,17 +237,14 @@ __global__ void k_fire_bend(int V, const int* __restrict__ shapeId, const int* _
 // convergence on max |force component| (model.cpp:500), step count, status block.
 // staleRollback: a rolled-back packing's forces are not yet recomputed, so it cannot
 // converge in this pass.
-__global__ void k_fire_finish(int P, int* __restrict__ active, const int* __restrict__ flags,
-                              const double* __restrict__ maxF, double fTol, int* __restrict__ steps,
-                              const double* __restrict__ dt, const double* __restrict__ alpha,
-                              const int* __restrict__ nPos, const double* __restrict__ energy,
-                              double* __restrict__ fs, pcp_fire_status* __restrict__ status,
-                              int* __restrict__ summary, int countStep, int staleRollback) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
+__device__ void fire_finish_p(int p, int* __restrict__ active, const int* __restrict__ flags, double mf,
+                              double fTol, int* __restrict__ steps, const double* __restrict__ dt,
+                              const double* __restrict__ alpha, const int* __restrict__ nPos,
+                              const double* __restrict__ energy, double* __restrict__ fs,
+                              pcp_fire_status* __restrict__ status, int* __restrict__ summary, int countStep,
+                              int staleRollback) {
     const int* fl = flags + FL_NF * p;
     if (active[p]) {
-        double mf = maxF[p];
         fs[FS_NF * p + FS_MAXF] = mf;
         if (countStep) steps[p] = steps[p] + 1;
         bool conv = (mf <= fTol) && !(staleRollback && fl[FL_ROLLBACK]);
@@ -249,6 +264,38 @@ __global__ void k_fire_finish(int P, int* __restrict__ active, const int* __rest
     s.shakeFailed = fl[FL_FAIL];
     s.rolledBack = fl[FL_ROLLBACK];
     status[p] = s;
+}
+
+// Block per packing, after the kick: fixed-order sums of F.v, v.v, F.F and the max
+// |force component|, then (full) the dt / alpha update (model.cpp:471-485) and the
+// convergence test, or (full = 0, after a rollback's fresh forces) the test only.
+// Block 0 also reports the largest record bucket count of the step in summary[2].
+__global__ __launch_bounds__(BLK) void k_fire_stats(int N, const int* __restrict__ packPoly,
+                                                   const double* __restrict__ red, int full, int nMin,
+                                                   double dtMax, double alphaStart, double fAlpha, double fInc,
+                                                   double fDec, double fTol, int* __restrict__ active,
+                                                   int* __restrict__ flags, double* dt, double* alpha, int* nPos,
+                                                   int* __restrict__ steps, const double* __restrict__ energy,
+                                                   double* __restrict__ fs, pcp_fire_status* __restrict__ status,
+                                                   int* __restrict__ summary, const int* __restrict__ scal) {
+    __shared__ double L[DR_CHUNKS * BLK];
+    __shared__ double part[BLK];
+    int p = blockIdx.x;
+    int a0 = packPoly[p], n = packPoly[p + 1] - a0;
+    double Pw = 0.0, vv = 0.0, ff = 0.0;
+    if (full) {
+        Pw = det_sum_block(red + a0, n, L, part);
+        vv = det_sum_block(red + N + a0, n, L, part);
+        ff = det_sum_block(red + 2LL * N + a0, n, L, part);
+    }
+    double mf = max_block(red + 3LL * N + a0, n, L);
+    if (threadIdx.x == 0) {
+        if (full)
+            fire_control2(p, active, Pw, vv, ff, nMin, dtMax, alphaStart, fAlpha, fInc, fDec, dt, alpha, nPos,
+                          flags, fs);
+        fire_finish_p(p, active, flags, mf, fTol, steps, dt, alpha, nPos, energy, fs, status, summary, full, full);
+        if (p == 0) summary[2] = scal[SC_BUCKET];
+    }
 }
 
 // undo an attempt whose buffers overflowed: x, v, FIRE scalars, active flags, steps
@@ -300,62 +347,52 @@ static int fire_eval(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm
         PCP_TRY(run_records(c, area ? x : nullptr));
     }
     PCP_TRY(run_forces(c, st->model, x, st->edgeLengths, st->areas, st->targetEdgeV, st->targetAreas,
-                       st->stiffness, st->compressibility, (double2*)st->force, st->energy, 1, rigid));
+                       st->stiffness, st->compressibility, (double2*)st->force, nullptr, 1, rigid));
+    PCP_LAUNCH(c, k_fire_energy, dim3(c->P), dim3(BLK), c->N, c->dPackPoly, c->dEpoly, st->energy, c->dActive,
+               c->dEpre, prm->alphaStart, prm->fDec, c->dDt, c->dAlpha, c->dNPos, c->dFlags);
     return PCP_OK;
 }
 
-static int fire_finish(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm, int countStep,
-                       int staleRollback) {
-    PCP_LAUNCH(c, k_max_abs_poly, dim3(nblk(c->N)), dim3(BLK), c->N, c->dPolyStart,
-                       (const double2*)st->force, c->dRed);
-    PCP_TRY(launch_det_reduce(c, c->dRed, 1, c->dRedOut + 3LL * c->P, 1));
-    PCP_LAUNCH(c, k_fire_finish, dim3(nblk(c->P)), dim3(BLK), c->P, c->dActive, c->dFlags,
-                       c->dRedOut + 3LL * c->P, prm->fTol, c->dSteps, c->dDt, c->dAlpha, c->dNPos, st->energy,
-                       c->dFireScal, c->dStatus, c->dSummary, countStep, staleRollback);
+// read the status block (summary[4] then the per-packing status) back to the host
+static int fire_read_status(pcp_ctx* c) {
+    size_t bytes = 16 + sizeof(pcp_fire_status) * (size_t)c->P;
+    c->hStatusBlock.resize(bytes);
+    PCP_HIP(hipMemcpyAsync(c->hStatusBlock.data(), c->dStatusBlock, bytes, hipMemcpyDeviceToHost, c->stream));
+    PCP_HIP(hipStreamSynchronize(c->stream));
     return PCP_OK;
 }
+static const int* host_summary(pcp_ctx* c) { return (const int*)c->hStatusBlock.data(); }
 
 // one step; *anyActive receives whether some packing is still unconverged
 static int fire_step_impl(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm, int* anyActive) {
     int mt0 = st->model & 0xFF;
     bool overlap = (mt0 == PCP_NORMAL || mt0 == PCP_HYBRID);
     for (int attempt = 0; attempt < 6; ++attempt) {
-        hipLaunchKernelGGL(k_fire_prologue, dim3(nblk(std::max(c->P, 4))), dim3(BLK), 0, c->stream, c->P,
-                           c->dActive, c->dDt, c->dAlpha, c->dNPos, c->dSteps, st->energy, c->dEpre, c->dFlags,
-                           c->dFireScal, c->dShakeIt, c->dSummary);
-        PCP_LAUNCHED();
-        PCP_LAUNCH(c, k_fire_begin, dim3(nblk(c->V)), dim3(BLK), c->V, c->dShapeId,
-                           c->dPolyPack, c->dActive, c->dDt, (double2*)st->pos, (double2*)st->vel,
-                           (const double2*)st->force, (double2*)c->dXbak, (double2*)c->dVbak,
-                           (double2*)c->dXpre);
+        PCP_LAUNCH(c, k_fire_begin, dim3(nblk(std::max(std::max(c->V, c->P), 4))), dim3(BLK), c->V, c->P,
+                   c->dShapeId, c->dPolyPack, c->dActive, c->dDt, c->dAlpha, c->dNPos, c->dSteps, st->energy,
+                   c->dEpre, c->dFlags, c->dFireScal, c->dShakeIt, c->dSummary, (double2*)st->pos,
+                   (double2*)st->vel, (const double2*)st->force, (double2*)c->dXbak, (double2*)c->dVbak,
+                   (double2*)c->dXpre);
         PCP_TRY(fire_eval(c, st, prm));
-        PCP_LAUNCH(c, k_fire_control, dim3(nblk(c->P)), dim3(BLK), c->P, c->dActive,
-                           st->energy, c->dEpre, prm->alphaStart, prm->fDec, c->dDt, c->dAlpha, c->dNPos,
-                           c->dFlags);
-        PCP_LAUNCH(c, k_fire_kick, dim3(nblk(c->N)), dim3(BLK), c->N, c->dPolyStart,
-                           c->dPolyPack, c->dActive, c->dFlags, c->dDt, (double2*)st->vel,
-                           (const double2*)st->force, c->dRed);
-        PCP_TRY(launch_det_reduce(c, c->dRed, 3, c->dRedOut, 0));
-        PCP_LAUNCH(c, k_fire_control2, dim3(nblk(c->P)), dim3(BLK), c->P, c->dActive,
-                           c->dRedOut, prm->nMin, prm->dtMax, prm->alphaStart, prm->fAlpha, prm->fInc,
-                           prm->fDec, c->dDt, c->dAlpha, c->dNPos, c->dFlags, c->dFireScal);
-        PCP_LAUNCH(c, k_fire_bend, dim3(nblk(c->V)), dim3(BLK), c->V, c->dShapeId,
-                           c->dPolyPack, c->dActive, c->dFlags, c->dFireScal, (double2*)st->pos,
-                           (double2*)st->vel, (const double2*)st->force, (const double2*)c->dXbak);
-        PCP_TRY(fire_finish(c, st, prm, 1, 1));
-        int summary[4] = {0, 0, 0, 0};
-        int scal[SC_NSCAL];
-        PCP_HIP(hipMemcpyAsync(summary, c->dSummary, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
-        PCP_HIP(hipMemcpyAsync(scal, c->dScal, sizeof(int) * SC_NSCAL, hipMemcpyDeviceToHost, c->stream));
-        PCP_HIP(hipStreamSynchronize(c->stream));
-        if (overlap && scal[SC_BUCKET] > c->Kp) {
+        PCP_LAUNCH(c, k_fire_kick, dim3(nblk(c->N)), dim3(BLK), c->N, c->dPolyStart, c->dPolyPack, c->dActive,
+                   c->dFlags, c->dDt, (double2*)st->vel, (const double2*)st->force, c->dRed);
+        PCP_LAUNCH(c, k_fire_stats, dim3(c->P), dim3(BLK), c->N, c->dPackPoly, c->dRed, 1, prm->nMin, prm->dtMax,
+                   prm->alphaStart, prm->fAlpha, prm->fInc, prm->fDec, prm->fTol, c->dActive, c->dFlags, c->dDt,
+                   c->dAlpha, c->dNPos, c->dSteps, st->energy, c->dFireScal, c->dStatus, c->dSummary, c->dScal);
+        PCP_LAUNCH(c, k_fire_bend, dim3(nblk(c->V)), dim3(BLK), c->V, c->dShapeId, c->dPolyPack, c->dActive,
+                   c->dFlags, c->dFireScal, (double2*)st->pos, (double2*)st->vel, (const double2*)st->force,
+                   (const double2*)c->dXbak);
+        PCP_TRY(fire_read_status(c));
+        const int* summary = host_summary(c);
+        if (overlap && summary[2] > c->Kp) {
             // the step ran with too small record buckets: grow, undo, retry
+            int need = summary[2];
             hipLaunchKernelGGL(k_fire_undo, dim3(nblk(std::max(c->V, c->P))), dim3(BLK), 0, c->stream, c->V,
                                c->P, c->dShapeId, c->dPolyPack, c->dActive, c->dFlags, (double2*)st->pos,
                                (double2*)st->vel, (const double2*)c->dXbak, (const double2*)c->dVbak, c->dDt,
                                c->dAlpha, c->dNPos, c->dSteps, c->dFireScal);
             PCP_LAUNCHED();
-            PCP_TRY(grow_buckets(c, scal[SC_BUCKET]));
+            PCP_TRY(grow_buckets(c, need));
             PCP_TRY(pcp_force_eval(c, st));
             PCP_HIP(hipMemsetAsync(c->dSticky, 0, sizeof(int), c->stream));   // handled here
             continue;
@@ -364,9 +401,14 @@ static int fire_step_impl(pcp_ctx* c, const pcp_state* st, const pcp_fire_params
             // energy rose for some packing: x was restored there; recompute its forces
             // (model.cpp:454-464) -- deterministic, so unchanged packings reproduce theirs
             PCP_TRY(pcp_force_eval(c, st));
-            PCP_TRY(fire_finish(c, st, prm, 0, 0));
-            PCP_HIP(hipMemcpyAsync(summary, c->dSummary, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
-            PCP_HIP(hipStreamSynchronize(c->stream));
+            PCP_LAUNCH(c, k_max_abs_poly, dim3(nblk(c->N)), dim3(BLK), c->N, c->dPolyStart,
+                       (const double2*)st->force, c->dRed + 3LL * c->N);
+            PCP_LAUNCH(c, k_fire_stats, dim3(c->P), dim3(BLK), c->N, c->dPackPoly, c->dRed, 0, prm->nMin,
+                       prm->dtMax, prm->alphaStart, prm->fAlpha, prm->fInc, prm->fDec, prm->fTol, c->dActive,
+                       c->dFlags, c->dDt, c->dAlpha, c->dNPos, c->dSteps, st->energy, c->dFireScal, c->dStatus,
+                       c->dSummary, c->dScal);
+            PCP_TRY(fire_read_status(c));
+            summary = host_summary(c);
         }
         if (anyActive) *anyActive = summary[1];
         return PCP_OK;
@@ -396,11 +438,7 @@ int pcp_fire_init(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm, d
 
 int pcp_fire_step(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm, pcp_fire_status* status) {
     PCP_TRY(fire_step_impl(c, st, prm, nullptr));
-    if (status) {
-        PCP_HIP(hipMemcpyAsync(status, c->dStatus, sizeof(pcp_fire_status) * c->P, hipMemcpyDeviceToHost,
-                               c->stream));
-        PCP_HIP(hipStreamSynchronize(c->stream));
-    }
+    if (status) memcpy(status, c->hStatusBlock.data() + 16, sizeof(pcp_fire_status) * c->P);
     return PCP_OK;
 }
 
@@ -412,9 +450,8 @@ int pcp_fire_minimize(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* pr
         if (!any) break;
     }
     if (status) {
-        PCP_HIP(hipMemcpyAsync(status, c->dStatus, sizeof(pcp_fire_status) * c->P, hipMemcpyDeviceToHost,
-                               c->stream));
-        PCP_HIP(hipStreamSynchronize(c->stream));
+        PCP_TRY(fire_read_status(c));
+        memcpy(status, c->hStatusBlock.data() + 16, sizeof(pcp_fire_status) * c->P);
     }
     return PCP_OK;
 }

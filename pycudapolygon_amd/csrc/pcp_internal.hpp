@@ -76,6 +76,7 @@ struct pcp_ctx {
     int maxN = 1;                  // largest polygon (vertices)
     double4* dBounds = nullptr;    // N: centre x, y, radius, big flag
     int* dPcSorted = nullptr;      // N: polygons sorted by cell, per packing
+    double4* dSBounds = nullptr;   // N: their circles in the same order
     int* dPcStart = nullptr;       // P x (PC_CELLS + 1): cell starts of each packing
     int* dPackB = nullptr;         // P: polygon grid size | big << 30
     int* dSticky = nullptr;        // [0] a fused evaluation overflowed a buffer
@@ -112,8 +113,11 @@ struct pcp_ctx {
     double* dFireScal = nullptr;   // per packing: bend scale, 1-alpha, maxF, backups
     int* dShakeIt = nullptr;       // per packing: [0] call-1 max, [1] call-2 max
     double* dCom = nullptr;        // 2N scratch (COM of internal geometry passes)
-    pcp_fire_status* dStatus = nullptr;
-    int* dSummary = nullptr;       // [0] any rollback, [1] any unconverged
+    char* dStatusBlock = nullptr;  // summary (16 bytes) then status[P], read back in one copy
+    std::vector<char> hStatusBlock;
+    pcp_fire_status* dStatus = nullptr;   // inside dStatusBlock
+    int* dSummary = nullptr;       // inside dStatusBlock: [0] any rollback, [1] any unconverged,
+                                   // [2] largest record bucket count of the step
     std::string profName;          // kernel bracketed with events (measurement)
     std::vector<std::pair<hipEvent_t, hipEvent_t>> profEvents;
     size_t profUsed = 0;

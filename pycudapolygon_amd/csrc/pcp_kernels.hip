@@ -81,27 +81,6 @@ __device__ __forceinline__ int block_excl_scan_int(int v, int* lds, int* total) 
     return off + x - v;
 }
 
-// fixed pairwise tree over 256 values held one per thread (thread t has v[t]);
-// result valid in thread 0.  Same pairing as orc_det_sum (s = 128, 64, ..., 1).
-__device__ __forceinline__ double tree256(double v, double* lds) {
-    int t = threadIdx.x;
-    __syncthreads();
-    lds[t] = v;
-    __syncthreads();
-    if (t < 128) lds[t] = lds[t] + lds[t + 128];
-    __syncthreads();
-    double a = 0.0;
-    if (t < 64) {
-        a = lds[t] + lds[t + 64];
-#pragma unroll
-        for (int s = 32; s >= 1; s >>= 1) {
-            double b = __shfl_down(a, s);
-            a = a + b;
-        }
-    }
-    return a;
-}
-
 // ------------------------------------------------------------------ small kernels
 __global__ void k_fill_int(int* p, long long n, int v) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -409,58 +388,77 @@ __global__ void k_sort_keys(int n, uint64_t mask, const uint64_t* __restrict__ i
     perm[rank] = (uint32_t)r;
 }
 
-// per-packing deterministic reductions over polygons: nArrays arrays of N values
-// (stride N) are tree-summed; if nMax > 0 the last nMax arrays are max-reduced.
-// One block of 256 threads per packing.
+// Per-packing deterministic sum of n values (fixed order, the same as orc_det_sum):
+// pairwise 256-wide trees over consecutive 256-value chunks (v[t] += v[t+s], s = 128..1),
+// then one such tree over the chunk sums (n <= 65536).  One block of BLK threads per
+// packing; the values of up to DR_CHUNKS chunks are loaded at once, every load issued
+// before the first tree level.  Result valid in every thread.
+constexpr int DR_CHUNKS = 16;
+__device__ double det_sum_block(const double* __restrict__ xa, int n, double* L, double* part) {
+    int t = threadIdx.x;
+    int chunks = (n + BLK - 1) / BLK;
+    for (int c0 = 0; c0 < chunks; c0 += DR_CHUNKS) {
+        int nc = min(DR_CHUNKS, chunks - c0);
+        double v[DR_CHUNKS];
+#pragma unroll
+        for (int c = 0; c < DR_CHUNKS; ++c) {
+            int q = (c0 + c) * BLK + t;
+            v[c] = (c < nc && q < n) ? xa[q] : 0.0;
+        }
+#pragma unroll
+        for (int c = 0; c < DR_CHUNKS; ++c) L[c * BLK + t] = v[c];
+        __syncthreads();
+        for (int st = BLK / 2; st >= 1; st >>= 1) {
+            if (t < st)
+                for (int c = 0; c < nc; ++c) L[c * BLK + t] += L[c * BLK + t + st];
+            __syncthreads();
+        }
+        if (t < nc) part[c0 + t] = L[t * BLK];
+        __syncthreads();
+    }
+    double res;
+    if (chunks <= 1) {
+        res = (chunks == 1) ? part[0] : 0.0;
+    } else {
+        L[t] = (t < chunks) ? part[t] : 0.0;
+        __syncthreads();
+        for (int st = BLK / 2; st >= 1; st >>= 1) {
+            if (t < st) L[t] += L[t + st];
+            __syncthreads();
+        }
+        res = L[0];
+    }
+    __syncthreads();
+    return res;
+}
+// maximum of n values (exact in any order); valid in every thread
+__device__ double max_block(const double* __restrict__ xa, int n, double* L) {
+    int t = threadIdx.x;
+    double m = 0.0;
+    for (int q = t; q < n; q += BLK) m = fmax(m, xa[q]);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((t & 63) == 0) L[t >> 6] = m;
+    __syncthreads();
+    double r = 0.0;
+    for (int q = 0; q < BLK / 64; ++q) r = fmax(r, L[q]);
+    __syncthreads();
+    return r;
+}
+
+// per-packing reductions over polygons: nArrays arrays of N values (stride N) are summed
+// with det_sum_block; the last nMax of them are max-reduced instead.  Block per packing.
 __global__ __launch_bounds__(BLK) void k_det_reduce(const int* __restrict__ packPoly,
                                                    const double* __restrict__ x, int N, int nArrays,
                                                    int nMax, double* __restrict__ out, int P) {
-    __shared__ double lds[BLK];
+    __shared__ double L[DR_CHUNKS * BLK];
     __shared__ double part[BLK];
     int p = blockIdx.x;
-    int a0 = packPoly[p], a1 = packPoly[p + 1];
-    int n = a1 - a0;
-    int t = threadIdx.x;
+    int a0 = packPoly[p], n = packPoly[p + 1] - a0;
     for (int a = 0; a < nArrays; ++a) {
         const double* xa = x + (long long)a * N + a0;
-        bool isMax = (a >= nArrays - nMax);
-        double res;
-        if (isMax) {
-            double m = 0.0;
-            for (int q = t; q < n; q += BLK) m = fmax(m, xa[q]);
-            __syncthreads();
-            lds[t] = m;
-            __syncthreads();
-            for (int s = BLK / 2; s >= 1; s >>= 1) {
-                if (t < s) lds[t] = fmax(lds[t], lds[t + s]);
-                __syncthreads();
-            }
-            res = lds[0];
-        } else {
-            int chunks = (n + BLK - 1) / BLK;
-            if (chunks <= 1) {
-                res = tree256((t < n) ? xa[t] : 0.0, lds);
-                if (t == 0) part[0] = res;
-                __syncthreads();
-                res = part[0];
-            } else {
-                // level 1: one tree per 256-chunk (n <= 65536 polygons => <= 256 chunks)
-                for (int cch = 0; cch < chunks; ++cch) {
-                    int q = cch * BLK + t;
-                    double r = tree256((q < n) ? xa[q] : 0.0, lds);
-                    if (t == 0) part[cch] = r;
-                }
-                __syncthreads();
-                double v = (t < chunks) ? part[t] : 0.0;
-                res = tree256(v, lds);
-                __syncthreads();
-                if (t == 0) part[0] = res;
-                __syncthreads();
-                res = part[0];
-            }
-        }
-        if (t == 0) out[(long long)a * P + p] = res;
-        __syncthreads();
+        double res = (a >= nArrays - nMax) ? max_block(xa, n, L) : det_sum_block(xa, n, L, part);
+        if (threadIdx.x == 0) out[(long long)a * P + p] = res;
     }
 }
 
@@ -1029,9 +1027,9 @@ static void free_topology(pcp_ctx* c) {
     dfree(c->dRedOut); dfree(c->dScanPart); c->scanPartCap = 0;
     dfree(c->dXbak); dfree(c->dXpre); dfree(c->dVbak); dfree(c->dDt); dfree(c->dAlpha);
     dfree(c->dEpre); dfree(c->dNPos); dfree(c->dSteps); dfree(c->dActive); dfree(c->dFlags);
-    dfree(c->dFireScal); dfree(c->dShakeIt); dfree(c->dCellInv); dfree(c->dCom); dfree(c->dStatus);
-    dfree(c->dSummary);
-    dfree(c->dBounds); dfree(c->dPcSorted); dfree(c->dPcStart); dfree(c->dPackB); dfree(c->dSticky);
+    dfree(c->dFireScal); dfree(c->dShakeIt); dfree(c->dCellInv); dfree(c->dCom); dfree(c->dStatusBlock);
+    c->dStatus = nullptr; c->dSummary = nullptr;
+    dfree(c->dBounds); dfree(c->dSBounds); dfree(c->dPcSorted); dfree(c->dPcStart); dfree(c->dPackB); dfree(c->dSticky);
     c->haveBoxes = false;
 }
 
@@ -1132,8 +1130,10 @@ int pcp_set_topology(pcp_ctx* c, int P, const int* packPolyStart, const int* pol
     PCP_TRY(dalloc(&c->dNPos, P)); PCP_TRY(dalloc(&c->dSteps, P)); PCP_TRY(dalloc(&c->dActive, P));
     PCP_TRY(dalloc(&c->dFlags, 8LL * P)); PCP_TRY(dalloc(&c->dFireScal, 16LL * P));
     PCP_TRY(dalloc(&c->dShakeIt, 2LL * P));
-    PCP_TRY(dalloc(&c->dCellInv, V)); PCP_TRY(dalloc(&c->dCom, 2LL * N)); PCP_TRY(dalloc(&c->dStatus, P));
-    PCP_TRY(dalloc(&c->dSummary, 4));
+    PCP_TRY(dalloc(&c->dCellInv, V)); PCP_TRY(dalloc(&c->dCom, 2LL * N));
+    PCP_TRY(dalloc(&c->dStatusBlock, 16 + sizeof(pcp_fire_status) * (size_t)P));
+    c->dSummary = (int*)c->dStatusBlock;
+    c->dStatus = (pcp_fire_status*)(c->dStatusBlock + 16);
     PCP_TRY(detect_alloc(c));
     PCP_TRY(dalloc(&c->dSticky, 1));
     PCP_HIP(hipMemsetAsync(c->dSticky, 0, sizeof(int), c->stream));

@@ -38,10 +38,9 @@ __global__ __launch_bounds__(64) void k_pair_poly(int N, int Kp, const int* __re
                                                  const unsigned long long* __restrict__ packMaxKey,
                                                  const double2* __restrict__ pairPos, uint64_t* __restrict__ rKey,
                                                  float2* __restrict__ rTU, uint64_t* __restrict__ rOuter,
-                                                 float2* __restrict__ rUT) {
+                                                 float2* __restrict__ rUT, const int* __restrict__ order) {
     int lane = threadIdx.x;
-    int s = blockIdx.x;
-    if (s >= N) return;
+    int s = order[xcd_block(blockIdx.x, gridDim.x)];
     int cnt = min(bcnt[s], Kp);
     if (cnt == 0) return;
     int pk = polyPack[s];
@@ -174,10 +173,9 @@ __global__ __launch_bounds__(64) void k_lens_energy(int N, int Kp, const int* __
                                                    const int* __restrict__ bcnt, const uint64_t* __restrict__ rKey,
                                                    const uint64_t* __restrict__ rOuter,
                                                    const float2* __restrict__ rTU, const float2* __restrict__ rUT,
-                                                   double* __restrict__ Er) {
+                                                   double* __restrict__ Er, const int* __restrict__ order) {
     int lane = threadIdx.x;
-    int s = blockIdx.x;
-    if (s >= N) return;
+    int s = order[xcd_block(blockIdx.x, gridDim.x)];
     int cnt = min(bcnt[s], Kp);
     int pk = polyPack[s];
     int pbase = packPoly[pk];
@@ -233,14 +231,14 @@ __global__ __launch_bounds__(64) void k_exterior_poly(
     const int* __restrict__ jcnt, const uint64_t* __restrict__ jKey, const uint64_t* __restrict__ rKey,
     const uint64_t* __restrict__ rOuter, const float2* __restrict__ rTU, const float2* __restrict__ rUT,
     double2* __restrict__ jC, int* __restrict__ jT, double2* __restrict__ F1, double2* __restrict__ LMo,
-    double2* __restrict__ LNo, double* __restrict__ accR, const double* __restrict__ Er) {
+    double2* __restrict__ LNo, double* __restrict__ accR, const double* __restrict__ Er,
+    const int* __restrict__ order) {
     constexpr int AREA = (MODE >= 1), SQ = (MODE == 2);
     extern __shared__ char smem[];
     __shared__ double leBuf[64];
     ExtLds L = ext_lds(smem, Kp);
     int lane = threadIdx.x;
-    int s = blockIdx.x;
-    if (s >= N) return;
+    int s = order[xcd_block(blockIdx.x, gridDim.x)];
     int cnt = min(bcnt[s], Kp);
     int pk = polyPack[s];
     int pbase = packPoly[pk];
@@ -459,10 +457,12 @@ __global__ __launch_bounds__(BLK) void k_gather_poly(
     const int* __restrict__ jT, const double2* __restrict__ F1, const double2* __restrict__ LMi,
     const double2* __restrict__ LNi, const double* __restrict__ accR, double* __restrict__ edgeLengths,
     double* __restrict__ areas, const double* __restrict__ targetEdgeV, const double* __restrict__ targetAreas,
-    double stiffness, double compressibility, double2* __restrict__ force, double* __restrict__ epoly) {
+    double stiffness, double compressibility, double2* __restrict__ force, double* __restrict__ epoly,
+    const int* __restrict__ order) {
     int sub = threadIdx.x & (SEG - 1);
-    int s = (blockIdx.x * BLK + threadIdx.x) / SEG;
-    if (s >= N) return;   // whole segments leave together
+    int si = (xcd_block(blockIdx.x, gridDim.x) * BLK + threadIdx.x) / SEG;
+    if (si >= N) return;   // whole segments leave together
+    int s = order[si];
     int st = polyStart[s], n = polyStart[s + 1] - st;
     // area in the reference's order (needed by the area spring and written when GEOM)
     double A = 0.0;
@@ -596,11 +596,11 @@ int run_records(pcp_ctx* c, const double2* pairPos) {
     if (pairPos)
         PCP_LAUNCH(c, k_pair_poly<1>, dim3(c->N), dim3(64), c->N, c->Kp, c->dPolyStart, c->dPolyPack,
                    c->dPackPoly, c->dBcnt, c->dBKey, c->dBTU, c->dPackMaxKey, pairPos, c->dRKey, c->dRTU,
-                   c->dROuter, c->dRUT);
+                   c->dROuter, c->dRUT, c->dPcSorted);
     else
         PCP_LAUNCH(c, k_pair_poly<0>, dim3(c->N), dim3(64), c->N, c->Kp, c->dPolyStart, c->dPolyPack,
                    c->dPackPoly, c->dBcnt, c->dBKey, c->dBTU, c->dPackMaxKey, pairPos, c->dRKey, c->dRTU,
-                   c->dROuter, c->dRUT);
+                   c->dROuter, c->dRUT, c->dPcSorted);
     return PCP_OK;
 }
 
@@ -614,7 +614,7 @@ static int launch_gather(pcp_ctx* c, int overlap, int edgeOn, int areaOn, const 
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gather_poly<SEG, GEOM, PROJ>), dim3(grid), dim3(BLK), 0, c->stream, c->N,
                        c->Kp, overlap, edgeOn, areaOn, c->dPolyStart, pos, c->dJcnt, c->dJC, c->dJT, c->dF1, c->dLM,
                        c->dLN, c->dAccR, edgeLengths, areas, targetEdgeV, targetAreas, stiffness, compressibility,
-                       force, c->dEpoly);
+                       force, c->dEpoly, c->dPcSorted);
     PCP_LAUNCHED();
     PCP_TRY(prof_end(c, slot));
     return PCP_OK;
@@ -653,8 +653,15 @@ int run_forces(pcp_ctx* c, int model, const double2* pos, double* edgeLengths, d
         hipLaunchKernelGGL(k_fill_double, dim3(nblk(2LL * c->V)), dim3(BLK), 0, c->stream, (double*)force,
                            2LL * c->V, 0.0);
         PCP_LAUNCHED();
-        hipLaunchKernelGGL(k_fill_double, dim3(nblk(c->P)), dim3(BLK), 0, c->stream, energy, (long long)c->P, 0.0);
-        PCP_LAUNCHED();
+        if (energy) {
+            hipLaunchKernelGGL(k_fill_double, dim3(nblk(c->P)), dim3(BLK), 0, c->stream, energy, (long long)c->P,
+                               0.0);
+            PCP_LAUNCHED();
+        } else {
+            hipLaunchKernelGGL(k_fill_double, dim3(nblk(c->N)), dim3(BLK), 0, c->stream, c->dEpoly,
+                               (long long)c->N, 0.0);
+            PCP_LAUNCHED();
+        }
         if (geom) PCP_TRY(run_geometry(c, pos, edgeLengths, areas, c->dCom, c->dCom + c->N, nullptr, nullptr));
         return PCP_OK;
     }
@@ -662,18 +669,21 @@ int run_forces(pcp_ctx* c, int model, const double2* pos, double* edgeLengths, d
         size_t lds = ext_lds_bytes(c->Kp);
         if (sq) {
             PCP_LAUNCH(c, k_lens_energy, dim3(c->N), dim3(64), c->N, c->Kp, c->dPolyStart, c->dPolyPack,
-                       c->dPackPoly, pos, c->dBcnt, c->dRKey, c->dROuter, c->dRTU, c->dRUT, c->dEr);
+                       c->dPackPoly, pos, c->dBcnt, c->dRKey, c->dROuter, c->dRTU, c->dRUT, c->dEr, c->dPcSorted);
             PCP_LAUNCH_LDS(c, k_exterior_poly<2>, dim3(c->N), dim3(64), lds, c->N, c->Kp,
                            c->dPolyStart, c->dPolyPack, c->dPackPoly, pos, c->dBcnt, c->dJcnt, c->dJKey, c->dRKey,
-                           c->dROuter, c->dRTU, c->dRUT, c->dJC, c->dJT, c->dF1, c->dLM, c->dLN, c->dAccR, c->dEr);
+                           c->dROuter, c->dRTU, c->dRUT, c->dJC, c->dJT, c->dF1, c->dLM, c->dLN, c->dAccR, c->dEr,
+                           c->dPcSorted);
         } else if (area) {
             PCP_LAUNCH_LDS(c, k_exterior_poly<1>, dim3(c->N), dim3(64), lds, c->N, c->Kp,
                            c->dPolyStart, c->dPolyPack, c->dPackPoly, pos, c->dBcnt, c->dJcnt, c->dJKey, c->dRKey,
-                           c->dROuter, c->dRTU, c->dRUT, c->dJC, c->dJT, c->dF1, c->dLM, c->dLN, c->dAccR, c->dEr);
+                           c->dROuter, c->dRTU, c->dRUT, c->dJC, c->dJT, c->dF1, c->dLM, c->dLN, c->dAccR, c->dEr,
+                           c->dPcSorted);
         } else {
             PCP_LAUNCH_LDS(c, k_exterior_poly<0>, dim3(c->N), dim3(64), lds, c->N, c->Kp,
                            c->dPolyStart, c->dPolyPack, c->dPackPoly, pos, c->dBcnt, c->dJcnt, c->dJKey, c->dRKey,
-                           c->dROuter, c->dRTU, c->dRUT, c->dJC, c->dJT, c->dF1, c->dLM, c->dLN, c->dAccR, c->dEr);
+                           c->dROuter, c->dRTU, c->dRUT, c->dJC, c->dJT, c->dF1, c->dLM, c->dLN, c->dAccR, c->dEr,
+                           c->dPcSorted);
         }
     }
     bool proj = project && c->maxN <= 64;
@@ -687,6 +697,7 @@ int run_forces(pcp_ctx* c, int model, const double2* pos, double* edgeLengths, d
         PCP_TRY((dispatch_gather<0, 0>(c, overlap, edgeOn, areaOn, pos, edgeLengths, areas, targetEdgeV,
                                        targetAreas, stiffness, compressibility, force)));
     if (project && !proj) PCP_TRY(run_project(c, pos, force));
+    if (!energy) return PCP_OK;   // the caller reduces the polygon energies (FIRE)
     return launch_det_reduce(c, c->dEpoly, 1, energy, 0);
 }
 
