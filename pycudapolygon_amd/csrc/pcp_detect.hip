@@ -221,183 +221,261 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-// cell of an edge in the user grid as x | y << 16
+// cell of an edge in the user grid as x | y << 16 (box sizes are below 2^16: at most 2e9 cells)
 __device__ __forceinline__ int cell_xy(int flat, int B) { return (flat % B) | ((flat / B) << 16); }
 __device__ __forceinline__ bool cells_adjacent(int ca, int cb, int B) {
     int dx = abs((ca & 0xFFFF) - (cb & 0xFFFF)), dy = abs((ca >> 16) - (cb >> 16));
     return (dx <= 1 || dx == B - 1) && (dy <= 1 || dy == B - 1);
 }
 
-struct EdgeStage {
-    double2* P;
-    double2* R;
-    int* C;
-};
+// bits [0, n) and [lo, hi) of a 64-bit mask (bounds clamped to 0..64)
+__device__ __forceinline__ unsigned long long bits_below(int n) {
+    return n <= 0 ? 0ull : (n >= 64 ? ~0ull : ((1ull << n) - 1ull));
+}
+__device__ __forceinline__ unsigned long long bit_range(int lo, int hi) { return bits_below(hi) & ~bits_below(lo); }
 
-// Crossings found by the wavefront of polygon A, appended to A's own buckets: a record
-// whose si is A to the si bucket, one whose sj is A to the sj bucket (a crossing of A
-// with itself to both).  The wavefront is the only writer of A's buckets, so slots come
-// from ballots, without atomics.
-struct Appender {
-    long long base;   // A * Kp
-    int Kp;
-    int nSi, nSj;     // wavefront-uniform counts so far
-    uint64_t* bKey;
-    float2* bTU;
-    uint64_t* jKey;
-};
-__device__ __forceinline__ void append_records(Appender& ap, int lane, bool toSi, bool toSj, uint64_t key,
-                                               float2 tu) {
-    unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    unsigned long long mi = __ballot(toSi), mj = __ballot(toSj);
-    if (toSi) {
-        int slot = ap.nSi + __popcll(mi & below);
-        if (slot < ap.Kp) { ap.bKey[ap.base + slot] = key; ap.bTU[ap.base + slot] = tu; }
-    }
-    if (toSj) {
-        int slot = ap.nSj + __popcll(mj & below);
-        if (slot < ap.Kp) ap.jKey[ap.base + slot] = key;
-    }
-    ap.nSi += __popcll(mi);
-    ap.nSj += __popcll(mj);
+// floor(x / d) for x >= 0, d >= 1 and x * d < 2^16 with a multiply and shift:
+// m = ceil(2^16 / d) exceeds 2^16 / d by less than 1/d, which keeps x * m / 2^16 below
+// the next integer while x * d < 2^16
+__device__ __forceinline__ int small_div(int x, int d) {
+    int m = (65536 + d - 1) / d;
+    return (x * m) >> 16;
 }
 
-constexpr int TILE_UNROLL = 4;
+// Where the detection writes: per-polygon buckets of Kp records, the si bucket (keys and
+// float parameters) and the sj bucket (keys), with their counters
+struct DetOut {
+    int Kp;
+    int* bcnt;
+    uint64_t* bKey;
+    float2* bTU;
+    int* jcnt;
+    uint64_t* jKey;
+    int* scal;
+};
 
-// Test the edges of polygon A (staged in LDS, chunks of 64) against every edge of polygon
-// Bq (held in lanes).  Each edge pair is computed with the edge of lower vertex id as the
-// reference's edge 1 (updateNeighborsKernel tests id1 against nid > id1), so the tests
-// from A's and from Bq's wavefront are the same arithmetic.  A == Bq tests each pair of
-// non-adjacent edges once.  Every lane runs the same number of iterations (ballots).
-template <int CELLS>
-__device__ void edge_tile(int lane, int A, int stA, int nA, int Bq, int stB, int nB, int sAl, int sBl, int Bu,
-                          const double2* __restrict__ pos, const int* __restrict__ cellOf, EdgeStage sg,
-                          int& staged, Appender& ap, int* __restrict__ num, uint64_t& kmax) {
-    const double eps = 1e-12;
-    bool self = (A == Bq);
-    bool aFirst = (A < Bq);
+// A crossing record appended to the buckets of its two polygons (global ids s0 + si and
+// s0 + sj; both the same for a self-crossing).  Slots come from atomic counters, since a
+// bucket receives records from the wavefronts of several polygons; a slot beyond the
+// capacity is not written and raises the overflow maximum, after which the caller grows
+// the buckets and detects again.  The order inside a bucket is the order of arrival;
+// every consumer orders the records by key.
+__device__ __forceinline__ void emit_record(uint64_t key, float2 tu, int s0, const DetOut& o) {
+    int si = s0 + key_si(key), sj = s0 + key_sj(key);
+    int a = atomicAdd(&o.bcnt[si], 1);
+    if (a < o.Kp) {
+        o.bKey[(long long)si * o.Kp + a] = key;
+        o.bTU[(long long)si * o.Kp + a] = tu;
+    } else {
+        atomicMax(&o.scal[SC_BUCKET], a + 1);
+    }
+    int b = atomicAdd(&o.jcnt[sj], 1);
+    if (b < o.Kp) o.jKey[(long long)sj * o.Kp + b] = key;
+    else atomicMax(&o.scal[SC_BUCKET], b + 1);
+}
+
+// Segment test of updateNeighborsKernel (kernels.cuh:760-790) for edge 1 = (P1, P1 + R1)
+// and edge 2 = (P2, P2 + R2): denom = R1 x R2, g = wrap(P2 - P1), t = (g x R2) / denom,
+// u = (g x R1) / denom; a crossing when |denom| >= 1e-12 and t, u lie in (0, 1).
+struct SegNum {
+    double denom, nt, nu;
+};
+__device__ __forceinline__ SegNum seg_numerators(d2 P1, d2 R1, d2 P2, d2 R2) {
+    SegNum s;
+    s.denom = R1.x * R2.y - R1.y * R2.x;
+    double gx = wrap(P2.x - P1.x);
+    double gy = wrap(P2.y - P1.y);
+    s.nt = gx * R2.y - gy * R2.x;
+    s.nu = gx * R1.y - gy * R1.x;
+    return s;
+}
+// Necessary condition for a crossing without the divisions: t = nt/denom and u = nu/denom
+// lie in (0,1) only if nt and nu lie strictly between 0 and denom (a correctly rounded
+// quotient keeps its sign and is >= 1 when |numerator| >= |denom|).  The sign of denom is
+// moved onto the numerators by flipping their sign bits, which is exact, so one set of
+// comparisons against |denom| covers both signs (NaNs fail every comparison, as in the
+// two-sided form).
+__device__ __forceinline__ bool seg_may_cross(const SegNum& s) {
+    unsigned long long sg = (unsigned long long)__double_as_longlong(s.denom) & 0x8000000000000000ull;
+    double at = __longlong_as_double((long long)((unsigned long long)__double_as_longlong(s.nt) ^ sg));
+    double au = __longlong_as_double((long long)((unsigned long long)__double_as_longlong(s.nu) ^ sg));
+    double ad = fabs(s.denom);
+    return !(ad < 1e-12) && at > 0.0 && at < ad && au > 0.0 && au < ad;
+}
+// The exact test and the record of a pair that passed seg_may_cross: edge 1 is vertex n1
+// of local polygon s1l, edge 2 vertex l2 of local polygon s2l.  The key and the inside
+// order follow updateCompactedIntersectionsKernel (kernels.cuh:857-876); t and u are
+// rounded to float as the reference stores them.
+__device__ __forceinline__ bool exact_record(const SegNum& s, int s1l, int n1, int s2l, int l2, uint64_t& key,
+                                             float2& tu) {
+    double tt = s.nt / s.denom;
+    double uu = s.nu / s.denom;
+    if (!(tt > 0.0 && tt < 1.0 && uu > 0.0 && uu < 1.0)) return false;
+    if (s.denom > 0) {
+        key = ((uint64_t)(uint16_t)s1l << 48) | ((uint64_t)(uint16_t)s2l << 32) | ((uint64_t)(uint16_t)l2 << 16) |
+              (uint64_t)(uint16_t)n1;
+        tu = make_float2((float)tt, (float)uu);
+    } else {
+        key = ((uint64_t)(uint16_t)s2l << 48) | ((uint64_t)(uint16_t)s1l << 32) | ((uint64_t)(uint16_t)n1 << 16) |
+              (uint64_t)(uint16_t)l2;
+        tu = make_float2((float)uu, (float)tt);
+    }
+    return true;
+}
+
+// An edge (start P, vector R) that crosses an edge of a polygon whose vertices lie in the
+// circle (c, r) meets that circle (the crossing point lies on both edges, and the circle
+// contains the polygon's edges).  Conservative test with the edge's own bounding circle:
+// centre P + R/2, radius (|Rx| + |Ry|) / 2 >= |R| / 2, minimum image of the centres, a
+// relative margin for rounding.  Valid while both polygons are small against the box
+// (radii below CULL_RADIUS): then the minimum images of every vertex, centre and edge
+// involved are consistent with the polygons' own chains.
+constexpr double CULL_RADIUS = 0.1;
+__device__ __forceinline__ bool edge_near_circle(d2 P, d2 R, double cx, double cy, double r) {
+    double dx = wrap(cx - (P.x + 0.5 * R.x));
+    double dy = wrap(cy - (P.y + 0.5 * R.y));
+    double rr = (r + 0.5 * (fabs(R.x) + fabs(R.y))) * (1.0 + 1e-6) + 1e-12;
+    return dx * dx + dy * dy <= rr * rr;
+}
+__device__ __forceinline__ bool cullable(double4 b) { return b.w == 0.0 && b.z < CULL_RADIUS; }
+
+constexpr int TILE_UNROLL = 4;
+// vertices and polygons of one batch of candidates in a wavefront's LDS
+constexpr int DET_BATCH = 256;
+constexpr int DET_KMAX = 16;
+
+// A wavefront's LDS: A's edges (chunk of 64), the vertices of a batch of candidate
+// polygons with the lane that loaded each, and the culled edge lists of the batch
+struct DetLds {
+    double2 P[64], R[64];         // A's edges: start and wrapped edge vector
+    int C[64];                    // their user-grid cells (x | y << 16)
+    double2 V[DET_BATCH];         // candidate vertices, polygon after polygon
+    uint8_t own[DET_BATCH];       // lane of the candidate a vertex belongs to
+    uint8_t listB[DET_BATCH];     // candidate edges near A's circle (local ids), per candidate
+    uint8_t listA[DET_KMAX * 64]; // A's edges near each candidate's circle
+    uint8_t klane[DET_KMAX];      // lane of the batch's k-th candidate
+};
+
+// Where the lanes take the vertices of polygon B from in edge_tile: global memory, or (A
+// against itself, one chunk) A's own LDS stage
+enum { SRC_GLOBAL = 0, SRC_STAGE = 2 };
+// Tile roles: A against itself, or A against a polygon of larger id (A's edges are then
+// the reference's edge 1: updateNeighborsKernel tests edge id1 against larger ids)
+enum { TILE_SELF = 0, TILE_A_FIRST = 1 };
+
+__device__ __forceinline__ void stage_a(DetLds& L, int lane, int stA, int nA, int a0, int Bu,
+                                        const double2* __restrict__ pos, const int* __restrict__ cellOf, bool cells) {
+    int na = min(64, nA - a0);
+    wave_lds_sync();
+    if (lane < na) {
+        int a = a0 + lane;
+        d2 p0 = ldpos(pos, stA + a), p1 = ldpos(pos, stA + ((a + 1 == nA) ? 0 : a + 1));
+        L.P[lane] = make_double2(p0.x, p0.y);
+        L.R[lane] = make_double2(wrap(p1.x - p0.x), wrap(p1.y - p0.y));
+        L.C[lane] = cells ? cell_xy(cellOf[stA + a], Bu) : edge_cell_xy(p0, p1, Bu);
+    }
+    wave_lds_sync();
+}
+
+// Test every edge of polygon A (staged in LDS, chunks of 64) against every edge of
+// polygon B (held in lanes; ids wavefront-uniform; vertices from global memory at
+// pos + stB, or for TILE_SELF with nA <= 64 from A's stage).  TILE_SELF tests each pair of
+// non-adjacent edges of A once.  The hot loop evaluates the division-free necessary
+// condition; iterations where some lane passes it take the exact test, the user-grid cell
+// filter and the record.  Used for A itself and for polygons of more than 64 vertices.
+template <int CELLS, int ROLE, int SRC>
+__device__ void edge_tile(DetLds& L, int lane, int stA, int nA, int stB, int nB, int sAl, int sBl, int Bu, int s0,
+                          const double2* __restrict__ pos, const int* __restrict__ cellOf, int& staged,
+                          const DetOut& out, int* __restrict__ num, uint64_t& kmax) {
     for (int a0 = 0; a0 < nA; a0 += 64) {
         int na = min(64, nA - a0);
         if (staged != a0) {
-            wave_lds_sync();
-            if (lane < na) {
-                int a = a0 + lane;
-                d2 p0 = ldpos(pos, stA + a), p1 = ldpos(pos, stA + ((a + 1 == nA) ? 0 : a + 1));
-                sg.P[lane] = make_double2(p0.x, p0.y);
-                sg.R[lane] = make_double2(wrap(p1.x - p0.x), wrap(p1.y - p0.y));
-                sg.C[lane] = cell_xy(CELLS ? cellOf[stA + a] : edge_cell(p0, p1, Bu), Bu);
-            }
-            wave_lds_sync();
+            stage_a(L, lane, stA, nA, a0, Bu, pos, cellOf, CELLS != 0);
             staged = a0;
         }
         for (int b0 = 0; b0 < nB; b0 += 64) {
             int nb = min(64, nB - b0);
             int G = 64 / nb;
-            int g = lane / nb, bl = lane - g * nb;
+            int g = small_div(lane, nb), bl = lane - g * nb;
             bool act = g < G;
             int b = b0 + bl;
+            int bn = (b + 1 == nB) ? 0 : b + 1;
             d2 Q = mk(0.0, 0.0), S = mk(0.0, 0.0);
-            int cb = 0;
             if (act) {
-                d2 q0 = ldpos(pos, stB + b), q1 = ldpos(pos, stB + ((b + 1 == nB) ? 0 : b + 1));
-                Q = q0;
-                S = mk(wrap(q1.x - q0.x), wrap(q1.y - q0.y));
-                cb = cell_xy(CELLS ? cellOf[stB + b] : edge_cell(q0, q1, Bu), Bu);
+                if (SRC == SRC_STAGE) {   // A itself, all of it in the stage (nA <= 64)
+                    double2 q = L.P[b], r = L.R[b];
+                    Q = mk(q.x, q.y);
+                    S = mk(r.x, r.y);
+                } else {
+                    d2 q0 = ldpos(pos, stB + b), q1 = ldpos(pos, stB + bn);
+                    Q = q0;
+                    S = mk(wrap(q1.x - q0.x), wrap(q1.y - q0.y));
+                }
             }
             int iters = (na + TILE_UNROLL * G - 1) / (TILE_UNROLL * G);
             for (int it = 0; it < iters; ++it) {
-                double2 PA[TILE_UNROLL], RA[TILE_UNROLL];
-                int CA[TILE_UNROLL];
-                bool ok[TILE_UNROLL];
+                bool may[TILE_UNROLL];
+                bool anyMay = false;
 #pragma unroll
-                for (int u = 0; u < TILE_UNROLL; ++u) {   // LDS loads of the unrolled pairs first
+                for (int u = 0; u < TILE_UNROLL; ++u) {
                     int al = g + (it * TILE_UNROLL + u) * G;
                     int a = a0 + al;
-                    ok[u] = act && al < na;
-                    if (self) ok[u] = ok[u] && b > a && b != a + 1 && !(a == 0 && b == nA - 1);
-                    int ls = ok[u] ? al : 0;
-                    PA[u] = sg.P[ls];
-                    RA[u] = sg.R[ls];
-                    CA[u] = sg.C[ls];
+                    bool ok = act && al < na;
+                    if (ROLE == TILE_SELF) ok = ok && b > a && b != a + 1 && !(a == 0 && b == nA - 1);
+                    double2 pa = L.P[ok ? al : 0], ra = L.R[ok ? al : 0];
+                    SegNum s = seg_numerators(mk(pa.x, pa.y), mk(ra.x, ra.y), Q, S);
+                    may[u] = ok && seg_may_cross(s);
+                    anyMay = anyMay || may[u];
+                }
+                if (__ballot(anyMay) == 0ull) continue;   // wavefront-uniform: no crossing here
+                int cb = 0;   // the user-grid cell of B's edge, only needed here
+                if (act) {
+                    if (SRC == SRC_STAGE) cb = L.C[b];
+                    else if (CELLS) cb = cell_xy(cellOf[stB + b], Bu);
+                    else cb = edge_cell_xy(ldpos(pos, stB + b), ldpos(pos, stB + bn), Bu);
                 }
 #pragma unroll
                 for (int u = 0; u < TILE_UNROLL; ++u) {
                     int al = g + (it * TILE_UNROLL + u) * G;
                     int a = a0 + al;
-                    bool hit = ok[u] && cells_adjacent(CA[u], cb, Bu);
-                    // edge 1 = the edge of lower vertex id
-                    bool af = self || aFirst;
-                    d2 P1 = af ? mk(PA[u].x, PA[u].y) : Q, R1 = af ? mk(RA[u].x, RA[u].y) : S;
-                    d2 P2 = af ? Q : mk(PA[u].x, PA[u].y), R2 = af ? S : mk(RA[u].x, RA[u].y);
-                    double denom = R1.x * R2.y - R1.y * R2.x;
-                    hit = hit && !(fabs(denom) < eps);
-                    double gx = wrap(P2.x - P1.x);
-                    double gy = wrap(P2.y - P1.y);
-                    double nt = gx * R2.y - gy * R2.x;
-                    double nu = gx * R1.y - gy * R1.x;
-                    // t = nt/denom and u = nu/denom lie in (0,1) only if nt and nu lie strictly
-                    // between 0 and denom (a correctly rounded quotient keeps its sign and is
-                    // >= 1 when |numerator| >= |denom|); only those pairs need the divisions
-                    hit = hit && ((denom > 0.0) ? (nt > 0.0 && nt < denom && nu > 0.0 && nu < denom)
-                                                : (nt < 0.0 && nt > denom && nu < 0.0 && nu > denom));
                     uint64_t key = 0;
-                    float2 tu = make_float2(0.f, 0.f);
-                    bool toSi = false, toSj = false;
-                    if (hit) {
-                        double tt = nt / denom;
-                        double uu = nu / denom;
-                        if (tt > 0.0 && tt < 1.0 && uu > 0.0 && uu < 1.0) {
-                            int s1l = af ? sAl : sBl, s2l = af ? sBl : sAl;
-                            int n1 = af ? a : b, l2 = af ? b : a;
-                            // updateCompactedIntersectionsKernel kernels.cuh:857-876
-                            if (denom > 0) {
-                                key = ((uint64_t)(uint16_t)s1l << 48) | ((uint64_t)(uint16_t)s2l << 32) |
-                                      ((uint64_t)(uint16_t)l2 << 16) | (uint64_t)(uint16_t)n1;
-                                tu = make_float2((float)tt, (float)uu);
-                            } else {
-                                key = ((uint64_t)(uint16_t)s2l << 48) | ((uint64_t)(uint16_t)s1l << 32) |
-                                      ((uint64_t)(uint16_t)n1 << 16) | (uint64_t)(uint16_t)l2;
-                                tu = make_float2((float)uu, (float)tt);
-                            }
-                            toSi = (key_si(key) == sAl);
-                            toSj = (key_sj(key) == sAl);
-                            if (toSi) kmax = (key > kmax) ? key : kmax;
-                            if (CELLS && af) atomicAdd(&num[stA + a], 1);   // per-edge count
-                        }
+                    float2 tu;
+                    bool rec = false;
+                    if (may[u] && cells_adjacent(L.C[al], cb, Bu)) {
+                        double2 pa = L.P[al], ra = L.R[al];
+                        SegNum s = seg_numerators(mk(pa.x, pa.y), mk(ra.x, ra.y), Q, S);
+                        rec = exact_record(s, sAl, a, sBl, b, key, tu);
                     }
-                    append_records(ap, lane, toSi, toSj, key, tu);
+                    if (rec) {
+                        kmax = (key > kmax) ? key : kmax;
+                        if (CELLS) atomicAdd(&num[stA + a], 1);   // per-edge count of edge 1
+                        emit_record(key, tu, s0, out);
+                    }
                 }
             }
         }
     }
 }
 
-// One wavefront per polygon A (in cell order): A's edges staged in LDS; the polygons of
-// the 3x3 block of cells around A's cell gathered by the lanes in parallel, kept when
-// their circle meets A's; A tested against itself and against every kept polygon, and
-// every crossing involving A appended to A's buckets.
-// CELLS = 0: the user-grid cell of every edge is computed from the live positions (the
-// fused evaluation, where cells and crossings come from the same positions);
-// CELLS = 1: the cells of the last cell-list build are used (updateNeighbors after
-// updateNeighborCells, which may have been called at other positions).
+// Crossings of polygon A with itself and with every polygon of larger id in the 3x3 block
+// of polygon cells around it whose circle meets A's (each pair of polygons is tested once,
+// from its lower polygon, whose edges are then the reference's edge 1).
+//   1. A's edges staged in LDS; A against itself (edge_tile).
+//   2. Candidates gathered by the lanes in parallel, in chunks of 64.
+//   3. Candidates of at most 64 vertices (A too) in batches of at most DET_KMAX polygons
+//      and DET_BATCH vertices: all their vertices copied to LDS with one memory latency;
+//      the candidate edges near A's circle and A's edges near each candidate's circle
+//      listed (the cull, skipped for large polygons); the products of the lists, i.e. all
+//      edge pairs that can cross, enumerated over the batch and tested 64 per pass, every
+//      lane busy whatever the polygons' sizes.
+//   4. Other candidates: one edge_tile each.
 template <int CELLS>
-__global__ __launch_bounds__(BLK) void k_detect_poly(
-    int N, const int* __restrict__ polyStart, const int* __restrict__ polyPack,
+__device__ void detect_polygon(
+    int A, int lane, DetLds& L, const int* __restrict__ polyStart, const int* __restrict__ polyPack,
     const int* __restrict__ packPoly, const int* __restrict__ box, const double2* __restrict__ pos,
     const int* __restrict__ cellOf, const double4* __restrict__ bounds, const int* __restrict__ pcStart,
-    const int* __restrict__ pcSorted, const double4* __restrict__ sBounds, const int* __restrict__ packB, int Kp,
-    int* __restrict__ bcnt, uint64_t* __restrict__ bKey, float2* __restrict__ bTU, int* __restrict__ jcnt,
-    uint64_t* __restrict__ jKey, unsigned long long* __restrict__ packMaxKey, int* __restrict__ num,
-    int* __restrict__ scal) {
-    __shared__ double2 sP[BLK / 64][64];
-    __shared__ double2 sR[BLK / 64][64];
-    __shared__ int sC[BLK / 64][64];
-    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int idx = xcd_block(blockIdx.x, gridDim.x) * (BLK / 64) + w;
-    if (idx >= N) return;   // whole wavefront
-    int A = pcSorted[idx];   // polygons in cell order
-    EdgeStage sg{sP[w], sR[w], sC[w]};
-    int staged = -1;
+    const int* __restrict__ pcSorted, const double4* __restrict__ sBounds, const int* __restrict__ packB,
+    const DetOut& out, unsigned long long* __restrict__ packMaxKey, int* __restrict__ num) {
     int stA = polyStart[A], nA = polyStart[A + 1] - stA;
     int pk = polyPack[A];
     int Bu = box[pk];
@@ -407,8 +485,8 @@ __global__ __launch_bounds__(BLK) void k_detect_poly(
     bool big = (pb >> 30) & 1;
     int sAl = A - s0;
     double4 bA = bounds[A];
+    bool cullA = cullable(bA);
     uint64_t kmax = 0;
-    Appender ap{(long long)A * Kp, Kp, 0, 0, bKey, bTU, jKey};
     // candidate cells: lanes 0..8 hold the ranges of the 3x3 block (one cell when Bp == 1);
     // loaded before the polygon's own tile so that the two latencies overlap
     long long gbase = (long long)pk * (PC_CELLS + 1);
@@ -425,7 +503,17 @@ __global__ __launch_bounds__(BLK) void k_detect_poly(
         lo = pcStart[gbase + cell];
         cnt = pcStart[gbase + cell + 1] - lo;
     }
-    if (nA >= 4) edge_tile<CELLS>(lane, A, stA, nA, A, stA, nA, sAl, sAl, Bu, pos, cellOf, sg, staged, ap, num, kmax);
+    int staged = -1;
+    if (nA <= 64) {
+        stage_a(L, lane, stA, nA, 0, Bu, pos, cellOf, CELLS != 0);
+        staged = 0;
+        if (nA >= 4)
+            edge_tile<CELLS, TILE_SELF, SRC_STAGE>(L, lane, stA, nA, stA, nA, sAl, sAl, Bu, s0, pos, cellOf, staged,
+                                                   out, num, kmax);
+    } else {
+        edge_tile<CELLS, TILE_SELF, SRC_GLOBAL>(L, lane, stA, nA, stA, nA, sAl, sAl, Bu, s0, pos, cellOf, staged, out,
+                                                num, kmax);
+    }
     int incl = cnt;
 #pragma unroll
     for (int o = 1; o < 16; o <<= 1) {
@@ -441,39 +529,206 @@ __global__ __launch_bounds__(BLK) void k_detect_poly(
             int hiK = __shfl(incl, k), cK = __shfl(cnt, k), loK = __shfl(lo, k);
             if (!found && ci < hiK) { q = loK + ci - (hiK - cK); found = true; }
         }
-        int Bq = -1, stq = 0, enq = 0;
+        int Bq = -1, stq = 0, nq = 0;
+        double4 bc = make_double4(0.0, 0.0, 0.0, 1.0);
         if (ci < T) {
             int cand = pcSorted[s0 + q];
-            double4 bc = sBounds[s0 + q];
-            if (cand != A && (big || circles_meet(bA, bc))) {
+            double4 c4 = sBounds[s0 + q];
+            if (cand > A && (big || circles_meet(bA, c4))) {
                 Bq = cand;
+                bc = c4;
                 stq = polyStart[cand];
-                enq = polyStart[cand + 1];
+                nq = polyStart[cand + 1] - stq;
             }
         }
-        unsigned long long keep = __ballot(Bq >= 0);
-        while (keep) {
-            int j = __ffsll((long long)keep) - 1;
-            keep &= keep - 1;
-            int Bsel = __shfl(Bq, j);
-            int stB = __shfl(stq, j), nB = __shfl(enq, j) - stB;
-            edge_tile<CELLS>(lane, A, stA, nA, Bsel, stB, nB, sAl, Bsel - s0, Bu, pos, cellOf, sg, staged, ap, num,
-                             kmax);
+        bool packed = Bq >= 0 && nA <= 64 && nq <= 64;
+        // candidates outside the packed scheme: one tile each
+        unsigned long long tiles = __ballot(Bq >= 0 && !packed);
+        while (tiles) {
+            int j = __ffsll((long long)tiles) - 1;
+            tiles &= tiles - 1;
+            int Bsel = __builtin_amdgcn_readlane(Bq, j);
+            int stB = __builtin_amdgcn_readlane(stq, j), nB = __builtin_amdgcn_readlane(nq, j);
+            edge_tile<CELLS, TILE_A_FIRST, SRC_GLOBAL>(L, lane, stA, nA, stB, nB, sAl, Bsel - s0, Bu, s0, pos, cellOf,
+                                                       staged, out, num, kmax);
+        }
+        // packed candidates, in lane order, cut into batches
+        unsigned long long pmask = __ballot(packed);
+        int nqp = packed ? nq : 0;
+        int inc = nqp;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            int y = __shfl_up(inc, o);
+            if (lane >= o) inc += y;
+        }
+        int exc = inc - nqp;
+        int prank = __popcll(pmask & bits_below(lane));
+        bool cullB = packed && cullA && cullable(bc);
+        unsigned long long rest = pmask;
+        while (rest) {
+            int j0 = __ffsll((long long)rest) - 1;
+            int base = __builtin_amdgcn_readlane(exc, j0), r0 = __builtin_amdgcn_readlane(prank, j0);
+            bool inB = ((rest >> lane) & 1ull) && inc - base <= DET_BATCH && prank - r0 < DET_KMAX;
+            unsigned long long batch = __ballot(inB);   // holds lane j0 at least (nq <= 64)
+            rest &= ~batch;
+            int K = __popcll(batch);
+            int kr = inB ? prank - r0 : 0;   // the candidate's place in the batch
+            int off = inB ? exc - base : 0;  // its first vertex in L.V
+            int nqb = inB ? nq : 0;
+            int Vb = nqb, m = nqb;
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) {
+                Vb += __shfl_xor(Vb, o);
+                m = max(m, __shfl_xor(m, o));
+            }
+            Vb = __builtin_amdgcn_readfirstlane(Vb);
+            m = __builtin_amdgcn_readfirstlane(m);
+            // the batch's vertices to LDS, each candidate by its own lane, all loads in flight
+            wave_lds_sync();   // the previous batch's reads are issued
+            for (int v0 = 0; v0 < m; v0 += 8) {
+                double2 t[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    t[k] = (v0 + k < nqb) ? pos[stq + v0 + k] : make_double2(0.0, 0.0);
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (v0 + k < nqb) {
+                        L.V[off + v0 + k] = t[k];
+                        L.own[off + v0 + k] = (uint8_t)lane;
+                    }
+            }
+            if (inB) L.klane[kr] = (uint8_t)lane;
+            wave_lds_sync();
+            // candidate edges near A's circle, listed per candidate in edge order
+            int cntB = 0;
+            for (int e0 = 0; e0 < Vb; e0 += 64) {
+                int e = e0 + lane;
+                bool valid = e < Vb;
+                int k = valid ? L.own[e] : 0;
+                int offk = __shfl(off, k), nk = __shfl(nqb, k), carry = __shfl(cntB, k);
+                bool cu = __shfl((int)cullB, k) != 0;
+                bool near = false;
+                int v = e - offk;
+                if (valid) {
+                    double2 q0 = L.V[e], q1 = L.V[(v + 1 == nk) ? offk : e + 1];
+                    d2 R = mk(wrap(q1.x - q0.x), wrap(q1.y - q0.y));
+                    near = !cu || edge_near_circle(mk(q0.x, q0.y), R, bA.x, bA.y, bA.z);
+                }
+                unsigned long long nm = __ballot(near);
+                if (near) L.listB[offk + carry + __popcll(nm & bit_range(offk - e0, lane))] = (uint8_t)v;
+                cntB += __popcll(nm & bit_range(off - e0, off + nqb - e0));
+            }
+            // A's edges near each candidate's circle, listed per candidate in edge order
+            int cntA = 0;
+            int KA = K * nA;   // < DET_KMAX * 64 = 1024, so small_div applies (KA * nA < 2^16)
+            for (int i0 = 0; i0 < KA; i0 += 64) {
+                int i = i0 + lane;
+                bool valid = i < KA;
+                int kb = valid ? small_div(i, nA) : 0;
+                int a = i - kb * nA;
+                int kl = L.klane[kb];
+                double cx = __shfl(bc.x, kl), cy = __shfl(bc.y, kl), cr = __shfl(bc.z, kl);
+                int carry = __shfl(cntA, kl);
+                bool cu = __shfl((int)cullB, kl) != 0;
+                bool near = false;
+                if (valid) {
+                    double2 p = L.P[a], r = L.R[a];
+                    near = !cu || edge_near_circle(mk(p.x, p.y), mk(r.x, r.y), cx, cy, cr);
+                }
+                unsigned long long nm = __ballot(near);
+                if (near) L.listA[kb * 64 + carry + __popcll(nm & bit_range(kb * nA - i0, lane))] = (uint8_t)a;
+                if (inB) cntA += __popcll(nm & bit_range(kr * nA - i0, (kr + 1) * nA - i0));
+            }
+            wave_lds_sync();
+            // all pairs (A edge, candidate edge) of the lists, 64 per pass
+            int np = inB ? cntA * cntB : 0;
+            int pinc = np;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                int y = __shfl_up(pinc, o);
+                if (lane >= o) pinc += y;
+            }
+            int pexc = pinc - np;
+            int Tp = __builtin_amdgcn_readlane(pinc, 63);
+            for (int p0 = 0; p0 < Tp; p0 += 64) {
+                int qq = p0 + lane;
+                bool valid = qq < Tp;
+                // the pair's candidate: the last lane whose first pair is <= qq (lanes
+                // without pairs share their successor's start and are never the last)
+                int k = 0;
+#pragma unroll
+                for (int step = 32; step >= 1; step >>= 1) {
+                    int t = __shfl(pexc, k + step);
+                    if (t <= qq) k += step;
+                }
+                int r = qq - __shfl(pexc, k);
+                int nbS = __shfl(cntB, k), krk = __shfl(kr, k), offk = __shfl(off, k), nk = __shfl(nqb, k);
+                int Bk = __shfl(Bq, k), stk = __shfl(stq, k);
+                SegNum s;
+                d2 Q = mk(0.0, 0.0), Q1 = mk(0.0, 0.0);
+                int a = 0, b = 0;
+                bool may = false;
+                if (valid) {
+                    // r = ia * nbS + ib; the float quotient is exact here (r < 4096, nbS <= 64:
+                    // (r + 0.5) / nbS stays 1/128 away from an integer)
+                    int ia = (int)(((float)r + 0.5f) * __builtin_amdgcn_rcpf((float)nbS));
+                    int ib = r - ia * nbS;
+                    a = L.listA[krk * 64 + ia];
+                    b = L.listB[offk + ib];
+                    double2 q0 = L.V[offk + b], q1 = L.V[offk + ((b + 1 == nk) ? 0 : b + 1)];
+                    double2 pa = L.P[a], ra = L.R[a];
+                    Q = mk(q0.x, q0.y);
+                    Q1 = mk(q1.x, q1.y);
+                    s = seg_numerators(mk(pa.x, pa.y), mk(ra.x, ra.y), Q, mk(wrap(Q1.x - Q.x), wrap(Q1.y - Q.y)));
+                    may = seg_may_cross(s);
+                }
+                if (__ballot(may) == 0ull) continue;   // wavefront-uniform: no crossing here
+                uint64_t key = 0;
+                float2 tu;
+                bool rec = false;
+                if (may) {
+                    int cb = CELLS ? cell_xy(cellOf[stk + b], Bu) : edge_cell_xy(Q, Q1, Bu);
+                    if (cells_adjacent(L.C[a], cb, Bu)) rec = exact_record(s, sAl, a, Bk - s0, b, key, tu);
+                }
+                if (rec) {
+                    kmax = (key > kmax) ? key : kmax;
+                    if (CELLS) atomicAdd(&num[stA + a], 1);   // per-edge count of edge 1 (A's)
+                    emit_record(key, tu, s0, out);
+                }
+            }
         }
     }
-    // A's bucket counts; the largest key of the packing (the pairing's last-record case)
+    // the largest key of the packing (the pairing's last-record case)
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
         unsigned long long y = __shfl_xor((unsigned long long)kmax, o);
         kmax = (y > kmax) ? y : kmax;
     }
-    if (lane == 0) {
-        bcnt[A] = ap.nSi;
-        jcnt[A] = ap.nSj;
-        int mx = max(ap.nSi, ap.nSj);
-        if (mx > Kp) atomicMax(&scal[SC_BUCKET], mx);
-        if (kmax > 0 && kmax > packMaxKey[pk]) atomicMax(&packMaxKey[pk], (unsigned long long)kmax);
-    }
+    if (lane == 0 && kmax > 0 && kmax > packMaxKey[pk]) atomicMax(&packMaxKey[pk], (unsigned long long)kmax);
+}
+
+// One wavefront per polygon A (in cell order, XCD-contiguous): detect_polygon.
+// CELLS = 0: the user-grid cell of every edge is computed from the live positions (the
+// fused evaluation, where cells and crossings come from the same positions);
+// CELLS = 1: the cells of the last cell-list build are used (updateNeighbors after
+// updateNeighborCells, which may have been called at other positions).
+// The bucket counters bcnt/jcnt must be zero (k_bounds clears them).
+template <int CELLS>
+__global__ __launch_bounds__(BLK) void k_detect_poly(
+    int N, const int* __restrict__ polyStart, const int* __restrict__ polyPack,
+    const int* __restrict__ packPoly, const int* __restrict__ box, const double2* __restrict__ pos,
+    const int* __restrict__ cellOf, const double4* __restrict__ bounds, const int* __restrict__ pcStart,
+    const int* __restrict__ pcSorted, const double4* __restrict__ sBounds, const int* __restrict__ packB, int Kp,
+    int* __restrict__ bcnt, uint64_t* __restrict__ bKey, float2* __restrict__ bTU, int* __restrict__ jcnt,
+    uint64_t* __restrict__ jKey, unsigned long long* __restrict__ packMaxKey, int* __restrict__ num,
+    int* __restrict__ scal) {
+    __shared__ DetLds lds[BLK / 64];
+    int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int idx = xcd_block(blockIdx.x, gridDim.x) * (BLK / 64) + w;
+    if (idx >= N) return;   // whole wavefront
+    DetOut out{Kp, bcnt, bKey, bTU, jcnt, jKey, scal};
+    detect_polygon<CELLS>(__builtin_amdgcn_readfirstlane(pcSorted[idx]), lane, lds[w], polyStart, polyPack, packPoly,
+                          box, pos, cellOf, bounds, pcStart, pcSorted, sBounds, packB, out, packMaxKey, num);
 }
 
 // exact maximum of the per-edge crossing counts (updateNeighbors' overflow check)

@@ -268,8 +268,9 @@ int run_geometry(pcp_ctx* c, const double2* pos, double* edgeLengths, double* ar
 }
 
 // ------------------------------------------------------------------ cell list
-// updateNeighborCellsKernel kernels.cuh:661-707 (cell of the edge midpoint)
-__device__ __forceinline__ int edge_cell(d2 p, d2 q, int B) {
+// updateNeighborCellsKernel kernels.cuh:661-707 (cell of the edge midpoint), as the
+// coordinates ix | iy << 16
+__device__ __forceinline__ int edge_cell_xy(d2 p, d2 q, int B) {
     double x = q.x - p.x + 0.5;
     double y = q.y - p.y + 0.5;
     x = shift_into_unit(x);
@@ -287,7 +288,12 @@ __device__ __forceinline__ int edge_cell(d2 p, d2 q, int B) {
     int iy = (fy == fy && fy >= 0.0) ? (fy < 2147483647.0 ? (int)fy : B - 1) : 0;
     if (ix < 0) ix = 0; else if (ix >= B) ix = B - 1;
     if (iy < 0) iy = 0; else if (iy >= B) iy = B - 1;
-    return iy * B + ix;
+    return ix | (iy << 16);
+}
+// the same cell as the flat index iy * B + ix
+__device__ __forceinline__ int edge_cell(d2 p, d2 q, int B) {
+    int xy = edge_cell_xy(p, q, B);
+    return (xy >> 16) * B + (xy & 0xFFFF);
 }
 
 __global__ void k_cell_hash(int V, const double2* __restrict__ pos, const int* __restrict__ shapeId,
