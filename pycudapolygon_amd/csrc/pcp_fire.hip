@@ -137,11 +137,10 @@ __global__ __launch_bounds__(BLK) void k_fire_energy(int N, const int* __restric
                                                     const int* __restrict__ active, const double* __restrict__ epre,
                                                     double alphaStart, double fDec, double* dt, double* alpha,
                                                     int* nPos, int* flags) {
-    __shared__ double L[DR_CHUNKS * BLK];
-    __shared__ double part[BLK];
+    __shared__ double part[BLK + 1];
     int p = blockIdx.x;
     int a0 = packPoly[p], n = packPoly[p + 1] - a0;
-    double e = det_sum_block(epoly + a0, n, L, part);
+    double e = det_sum_block(epoly + a0, n, part);
     if (threadIdx.x == 0) {
         energy[p] = e;
         fire_control(p, active, energy, epre, alphaStart, fDec, dt, alpha, nPos, flags);
@@ -180,6 +179,66 @@ __global__ void k_fire_kick(int N, const int* __restrict__ polyStart, const int*
     red[N + s] = b;
     red[2LL * N + s] = c;
     red[3LL * N + s] = mf;
+}
+
+// k_fire_kick with one lane segment per polygon (n <= SEG): the vertices are read and
+// written coalesced; the per-polygon sums keep the sequential vertex order (x then y term
+// of vertex 0, of vertex 1, ...), replayed by each lane of the segment from the products
+// staged in LDS.
+template <int SEG>
+__global__ __launch_bounds__(BLK) void k_fire_kick_seg(int N, const int* __restrict__ polyStart,
+                                                      const int* __restrict__ polyPack,
+                                                      const int* __restrict__ active, const int* __restrict__ flags,
+                                                      const double* __restrict__ dt, double2* __restrict__ v,
+                                                      const double2* __restrict__ F, double* __restrict__ red) {
+    __shared__ double2 prod[3][BLK];   // (F.x v.x, F.y v.y), (v.x v.x, v.y v.y), (F.x F.x, F.y F.y)
+    int e = threadIdx.x & (SEG - 1);
+    int segBase = threadIdx.x - e;
+    int s = (blockIdx.x * BLK + threadIdx.x) / SEG;
+    int st = 0, n = 0, p = 0;
+    if (s < N) {
+        st = polyStart[s];
+        n = polyStart[s + 1] - st;
+        p = polyPack[s];
+    }
+    bool live = e < n;
+    double2 fi = live ? F[st + e] : make_double2(0.0, 0.0);
+    double mf = fmax(fabs(fi.x), fabs(fi.y));
+#pragma unroll
+    for (int o = SEG / 2; o >= 1; o >>= 1) mf = fmax(mf, __shfl_xor(mf, o, SEG));
+    bool kick = (s < N) && active[p] && flags[FL_NF * p + FL_ACCEPT];
+    double2 pa = make_double2(0.0, 0.0), pb = pa, pc = pa;
+    if (kick && live) {
+        double h = dt[p];
+        double2 vi = v[st + e];
+        double vx = vi.x + (0.5 * h) * fi.x;
+        double vy = vi.y + (0.5 * h) * fi.y;
+        v[st + e] = make_double2(vx, vy);
+        pa = make_double2(fi.x * vx, fi.y * vy);
+        pb = make_double2(vx * vx, vy * vy);
+        pc = make_double2(fi.x * fi.x, fi.y * fi.y);
+    }
+    prod[0][threadIdx.x] = pa;
+    prod[1][threadIdx.x] = pb;
+    prod[2][threadIdx.x] = pc;
+    __syncthreads();
+    double a = 0.0, b = 0.0, c = 0.0;
+    if (kick)
+        for (int m = 0; m < n; ++m) {
+            double2 qa = prod[0][segBase + m], qb = prod[1][segBase + m], qc = prod[2][segBase + m];
+            a += qa.x;
+            a += qa.y;
+            b += qb.x;
+            b += qb.y;
+            c += qc.x;
+            c += qc.y;
+        }
+    if (s < N && e == 0) {
+        red[s] = a;
+        red[N + s] = b;
+        red[2LL * N + s] = c;
+        red[3LL * N + s] = mf;
+    }
 }
 
 // bend + P>0 logic (model.cpp:471-485)
@@ -278,15 +337,15 @@ __global__ __launch_bounds__(BLK) void k_fire_stats(int N, const int* __restrict
                                                    int* __restrict__ steps, const double* __restrict__ energy,
                                                    double* __restrict__ fs, pcp_fire_status* __restrict__ status,
                                                    int* __restrict__ summary, const int* __restrict__ scal) {
-    __shared__ double L[DR_CHUNKS * BLK];
-    __shared__ double part[BLK];
+    __shared__ double L[BLK / 64];
+    __shared__ double part[BLK + 1];
     int p = blockIdx.x;
     int a0 = packPoly[p], n = packPoly[p + 1] - a0;
     double Pw = 0.0, vv = 0.0, ff = 0.0;
     if (full) {
-        Pw = det_sum_block(red + a0, n, L, part);
-        vv = det_sum_block(red + N + a0, n, L, part);
-        ff = det_sum_block(red + 2LL * N + a0, n, L, part);
+        Pw = det_sum_block(red + a0, n, part);
+        vv = det_sum_block(red + N + a0, n, part);
+        ff = det_sum_block(red + 2LL * N + a0, n, part);
     }
     double mf = max_block(red + 3LL * N + a0, n, L);
     if (threadIdx.x == 0) {
@@ -324,6 +383,29 @@ __global__ void k_zero2(long long n, double2* v) {
 }
 
 // the evaluation part of one step (positions already moved by k_fire_begin)
+// the kick: lane segments when every polygon fits one (n <= 64), else thread per polygon
+static int launch_fire_kick(pcp_ctx* c, double2* vel, const double2* force) {
+    int seg = c->seg;
+    if (c->maxN > 64) {
+        PCP_LAUNCH(c, k_fire_kick, dim3(nblk(c->N)), dim3(BLK), c->N, c->dPolyStart, c->dPolyPack, c->dActive,
+                   c->dFlags, c->dDt, vel, force, c->dRed);
+        return PCP_OK;
+    }
+    int grid = (c->N + BLK / seg - 1) / (BLK / seg);
+#define PCP_KICK_SEG(W)                                                                                     \
+    PCP_LAUNCH(c, k_fire_kick_seg<W>, dim3(grid), dim3(BLK), c->N, c->dPolyStart, c->dPolyPack, c->dActive, \
+               c->dFlags, c->dDt, vel, force, c->dRed)
+    switch (seg) {
+    case 4: PCP_KICK_SEG(4); break;
+    case 8: PCP_KICK_SEG(8); break;
+    case 16: PCP_KICK_SEG(16); break;
+    case 32: PCP_KICK_SEG(32); break;
+    default: PCP_KICK_SEG(64); break;
+    }
+#undef PCP_KICK_SEG
+    return PCP_OK;
+}
+
 static int fire_eval(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm) {
     double2* x = (double2*)st->pos;
     double2* v = (double2*)st->vel;
@@ -374,8 +456,7 @@ static int fire_step_impl(pcp_ctx* c, const pcp_state* st, const pcp_fire_params
                    (double2*)st->vel, (const double2*)st->force, (double2*)c->dXbak, (double2*)c->dVbak,
                    (double2*)c->dXpre);
         PCP_TRY(fire_eval(c, st, prm));
-        PCP_LAUNCH(c, k_fire_kick, dim3(nblk(c->N)), dim3(BLK), c->N, c->dPolyStart, c->dPolyPack, c->dActive,
-                   c->dFlags, c->dDt, (double2*)st->vel, (const double2*)st->force, c->dRed);
+        PCP_TRY(launch_fire_kick(c, (double2*)st->vel, (const double2*)st->force));
         PCP_LAUNCH(c, k_fire_stats, dim3(c->P), dim3(BLK), c->N, c->dPackPoly, c->dRed, 1, prm->nMin, prm->dtMax,
                    prm->alphaStart, prm->fAlpha, prm->fInc, prm->fDec, prm->fTol, c->dActive, c->dFlags, c->dDt,
                    c->dAlpha, c->dNPos, c->dSteps, st->energy, c->dFireScal, c->dStatus, c->dSummary, c->dScal);

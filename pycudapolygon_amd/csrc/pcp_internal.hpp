@@ -125,7 +125,8 @@ struct pcp_ctx {
 
 namespace pcp {
 int scan_exclusive(pcp_ctx* c, const int* in, int n, int* out);
-int launch_det_reduce(pcp_ctx* c, const double* partials, int nArrays, double* outSums, int nMax);
+int launch_det_reduce(pcp_ctx* c, const double* partials, int nArrays, double* outSums, int nMax,
+                      bool sticky = false);
 int run_cells(pcp_ctx* c, const double2* pos);
 int run_detect(pcp_ctx* c, const double2* pos, int cellsFromList);
 int detect_alloc(pcp_ctx* c);
@@ -137,7 +138,8 @@ int packing_counts(pcp_ctx* c, int p, std::vector<int>& cnt);
 int run_records(pcp_ctx* c, const double2* pairPos);
 int run_forces(pcp_ctx* c, int model, const double2* pos, double* edgeLengths, double* areas,
                const double* targetEdgeV, const double* targetAreas, double stiffness,
-               double compressibility, double2* force, double* energy, int geom, int project);
+               double compressibility, double2* force, double* energy, int geom, int project,
+               bool sticky = false);
 int run_project(pcp_ctx* c, const double2* pos, double2* force);
 int run_project_seg(pcp_ctx* c, const double2* pos, double2* force);
 int run_geometry(pcp_ctx* c, const double2* pos, double* edgeLengths, double* areas, double* comX,

@@ -397,42 +397,53 @@ __global__ void k_sort_keys(int n, uint64_t mask, const uint64_t* __restrict__ i
 // Per-packing deterministic sum of n values (fixed order, the same as orc_det_sum):
 // pairwise 256-wide trees over consecutive 256-value chunks (v[t] += v[t+s], s = 128..1),
 // then one such tree over the chunk sums (n <= 65536).  One block of BLK threads per
-// packing; the values of up to DR_CHUNKS chunks are loaded at once, every load issued
-// before the first tree level.  Result valid in every thread.
-constexpr int DR_CHUNKS = 16;
-__device__ double det_sum_block(const double* __restrict__ xa, int n, double* L, double* part) {
-    int t = threadIdx.x;
-    int chunks = (n + BLK - 1) / BLK;
-    for (int c0 = 0; c0 < chunks; c0 += DR_CHUNKS) {
-        int nc = min(DR_CHUNKS, chunks - c0);
-        double v[DR_CHUNKS];
+// packing.  A chunk's tree is evaluated by one wavefront in registers (chunk_tree); the
+// wavefronts take DR_BATCH chunks at a time, all their loads issued first.  Result valid
+// in every thread.
+constexpr int DR_BATCH = 4;
+// the tree of one chunk: lane l holds v[l], v[l+64], v[l+128], v[l+192]; level 128
+// (v[l] + v[l+128], v[l+64] + v[l+192]) and level 64 inside the lane, levels 32..1 by
+// shuffles (lane t adds lane t + s); the same additions in the same order.  Sum in lane 0.
+__device__ __forceinline__ double chunk_tree(double v0, double v1, double v2, double v3) {
+    double a = v0 + v2;
+    double b = v1 + v3;
+    double x = a + b;
 #pragma unroll
-        for (int c = 0; c < DR_CHUNKS; ++c) {
-            int q = (c0 + c) * BLK + t;
-            v[c] = (c < nc && q < n) ? xa[q] : 0.0;
-        }
+    for (int st = 32; st >= 1; st >>= 1) x += __shfl_down(x, st);
+    return x;
+}
+__device__ double det_sum_block(const double* __restrict__ xa, int n, double* part) {
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int chunks = (n + 255) >> 8;
+    for (int c0 = w * DR_BATCH; c0 < chunks; c0 += (BLK / 64) * DR_BATCH) {
+        double v[DR_BATCH][4];
 #pragma unroll
-        for (int c = 0; c < DR_CHUNKS; ++c) L[c * BLK + t] = v[c];
-        __syncthreads();
-        for (int st = BLK / 2; st >= 1; st >>= 1) {
-            if (t < st)
-                for (int c = 0; c < nc; ++c) L[c * BLK + t] += L[c * BLK + t + st];
-            __syncthreads();
-        }
-        if (t < nc) part[c0 + t] = L[t * BLK];
-        __syncthreads();
+        for (int u = 0; u < DR_BATCH; ++u)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                int q = (c0 + u) * 256 + lane + 64 * k;
+                v[u][k] = (q < n) ? xa[q] : 0.0;
+            }
+#pragma unroll
+        for (int u = 0; u < DR_BATCH; ++u)
+            if (c0 + u < chunks) {   // wavefront-uniform
+                double x = chunk_tree(v[u][0], v[u][1], v[u][2], v[u][3]);
+                if (lane == 0) part[c0 + u] = x;
+            }
     }
+    __syncthreads();
     double res;
     if (chunks <= 1) {
         res = (chunks == 1) ? part[0] : 0.0;
     } else {
-        L[t] = (t < chunks) ? part[t] : 0.0;
-        __syncthreads();
-        for (int st = BLK / 2; st >= 1; st >>= 1) {
-            if (t < st) L[t] += L[t + st];
-            __syncthreads();
+        if (w == 0) {
+            double x = chunk_tree(lane < chunks ? part[lane] : 0.0, lane + 64 < chunks ? part[lane + 64] : 0.0,
+                                  lane + 128 < chunks ? part[lane + 128] : 0.0,
+                                  lane + 192 < chunks ? part[lane + 192] : 0.0);
+            if (lane == 0) part[BLK] = x;
         }
-        res = L[0];
+        __syncthreads();
+        res = part[BLK];
     }
     __syncthreads();
     return res;
@@ -454,24 +465,28 @@ __device__ double max_block(const double* __restrict__ xa, int n, double* L) {
 
 // per-packing reductions over polygons: nArrays arrays of N values (stride N) are summed
 // with det_sum_block; the last nMax of them are max-reduced instead.  Block per packing.
+// With stickyScal, block 0 also records whether the evaluation overflowed a record bucket.
 __global__ __launch_bounds__(BLK) void k_det_reduce(const int* __restrict__ packPoly,
                                                    const double* __restrict__ x, int N, int nArrays,
-                                                   int nMax, double* __restrict__ out, int P) {
-    __shared__ double L[DR_CHUNKS * BLK];
-    __shared__ double part[BLK];
+                                                   int nMax, double* __restrict__ out, int P,
+                                                   const int* __restrict__ stickyScal, int Kp,
+                                                   int* __restrict__ sticky) {
+    __shared__ double L[BLK / 64];
+    __shared__ double part[BLK + 1];
     int p = blockIdx.x;
     int a0 = packPoly[p], n = packPoly[p + 1] - a0;
+    if (stickyScal && p == 0 && threadIdx.x == 0 && stickyScal[SC_BUCKET] > Kp) sticky[0] = 1;
     for (int a = 0; a < nArrays; ++a) {
         const double* xa = x + (long long)a * N + a0;
-        double res = (a >= nArrays - nMax) ? max_block(xa, n, L) : det_sum_block(xa, n, L, part);
+        double res = (a >= nArrays - nMax) ? max_block(xa, n, L) : det_sum_block(xa, n, part);
         if (threadIdx.x == 0) out[(long long)a * P + p] = res;
     }
 }
 
-int launch_det_reduce(pcp_ctx* c, const double* partials, int nArrays, double* outSums, int nMax) {
+int launch_det_reduce(pcp_ctx* c, const double* partials, int nArrays, double* outSums, int nMax, bool sticky) {
     if (c->P == 0) return PCP_OK;
     PCP_LAUNCH(c, k_det_reduce, dim3(c->P), dim3(BLK), c->dPackPoly, partials, c->N,
-                       nArrays, nMax, outSums, c->P);
+                       nArrays, nMax, outSums, c->P, sticky ? c->dScal : nullptr, c->Kp, c->dSticky);
     return PCP_OK;
 }
 
@@ -1512,10 +1527,12 @@ int pcp_force_eval(pcp_ctx* c, const pcp_state* st) {
         PCP_TRY(run_detect(c, pos, 0));
         PCP_TRY(run_records(c, (st->model & (PCP_AREA_ENERGY | PCP_AREA_SQUARED)) ? pos : nullptr));
     }
+    // the overflow check rides on the energy reduction when there is one
+    bool fused = overlap && st->energy;
     PCP_TRY(run_forces(c, st->model, pos, st->edgeLengths, st->areas, st->targetEdgeV, st->targetAreas,
                        st->stiffness, st->compressibility, (double2*)st->force, st->energy, 1,
-                       mt == PCP_NORMAL));
-    if (overlap) {
+                       mt == PCP_NORMAL, fused));
+    if (overlap && !fused) {
         hipLaunchKernelGGL(k_sticky, dim3(1), dim3(64), 0, c->stream, c->dScal, c->Kp, c->dSticky);
         PCP_LAUNCHED();
     }

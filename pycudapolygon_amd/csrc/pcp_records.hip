@@ -641,7 +641,7 @@ static int dispatch_gather(pcp_ctx* c, int overlap, int edgeOn, int areaOn, cons
 // them (fused evaluation); else read them.  project: project the force of every polygon.
 int run_forces(pcp_ctx* c, int model, const double2* pos, double* edgeLengths, double* areas,
                const double* targetEdgeV, const double* targetAreas, double stiffness,
-               double compressibility, double2* force, double* energy, int geom, int project) {
+               double compressibility, double2* force, double* energy, int geom, int project, bool sticky) {
     if (c->V == 0) return PCP_OK;
     int sq = (model & PCP_AREA_SQUARED) ? 1 : 0;
     int area = (model & (PCP_AREA_ENERGY | PCP_AREA_SQUARED)) ? 1 : 0;
@@ -698,7 +698,7 @@ int run_forces(pcp_ctx* c, int model, const double2* pos, double* edgeLengths, d
                                        targetAreas, stiffness, compressibility, force)));
     if (project && !proj) PCP_TRY(run_project(c, pos, force));
     if (!energy) return PCP_OK;   // the caller reduces the polygon energies (FIRE)
-    return launch_det_reduce(c, c->dEpoly, 1, energy, 0);
+    return launch_det_reduce(c, c->dEpoly, 1, energy, 0, sticky);
 }
 
 // counts of the si buckets of packing p (host)
