@@ -24,13 +24,14 @@
 
 namespace pcp {
 
-constexpr int REC_MAX_KP = 512;   // LDS of k_exterior_poly: Kp * 116 bytes per wavefront
+constexpr int REC_MAX_KP = 512;   // LDS of k_exterior_poly: Kp * 116 bytes per polygon (<= 64 KiB)
 
 // ---------------------------------------------------------------- pairing
 // The reference's binary search starts with end = I-1, so when every key of the packing
 // lies below (si<<48 | sj<<32) the last record of the packing is the only candidate;
 // kept.  EXACT: crossing parameters recomputed from pairPos (area energy).
-template <int EXACT>
+// SEGX lanes per polygon, 64 / SEGX polygons per wavefront (a polygon has a few records).
+template <int EXACT, int SEGX>
 __global__ __launch_bounds__(64) void k_pair_poly(int N, int Kp, const int* __restrict__ polyStart,
                                                  const int* __restrict__ polyPack, const int* __restrict__ packPoly,
                                                  const int* __restrict__ bcnt, const uint64_t* __restrict__ bKey,
@@ -39,8 +40,10 @@ __global__ __launch_bounds__(64) void k_pair_poly(int N, int Kp, const int* __re
                                                  const double2* __restrict__ pairPos, uint64_t* __restrict__ rKey,
                                                  float2* __restrict__ rTU, uint64_t* __restrict__ rOuter,
                                                  float2* __restrict__ rUT, const int* __restrict__ order) {
-    int lane = threadIdx.x;
-    int s = order[xcd_block(blockIdx.x, gridDim.x)];
+    int lane = threadIdx.x & (SEGX - 1);
+    int slot = xcd_block(blockIdx.x, gridDim.x) * (64 / SEGX) + threadIdx.x / SEGX;
+    if (slot >= N) return;   // no barriers or shuffles below
+    int s = order[slot];
     int cnt = min(bcnt[s], Kp);
     if (cnt == 0) return;
     int pk = polyPack[s];
@@ -48,7 +51,7 @@ __global__ __launch_bounds__(64) void k_pair_poly(int N, int Kp, const int* __re
     int st1 = polyStart[s], ni = polyStart[s + 1] - st1;
     uint64_t mxk = packMaxKey[pk];
     long long base = (long long)s * Kp;
-    for (int f = lane; f < cnt; f += 64) {
+    for (int f = lane; f < cnt; f += SEGX) {
         uint64_t key = bKey[base + f];
         float2 tu = bTU[base + f];
         uint64_t ord = bucket_order(key);
@@ -130,7 +133,8 @@ __device__ __forceinline__ ExtLds ext_lds(char* smem, int Kp) {
     L.il = L.t + 4 * Kp;
     return L;
 }
-size_t ext_lds_bytes(int Kp) { return (size_t)Kp * (64 + 16 + 8 + 8 + 16 + 4); }
+// one polygon's share of the LDS of k_exterior_poly, rounded to 16 bytes
+__host__ __device__ __forceinline__ size_t ext_lds_stride(int Kp) { return ((size_t)Kp * 116 + 15) & ~(size_t)15; }
 
 // the record of the packing whose key is `key`: its outer key and its row (false if absent)
 __device__ __forceinline__ bool find_record(uint64_t key, int pbase, int Kp, const int* __restrict__ bcnt,
@@ -167,6 +171,7 @@ __device__ __forceinline__ bool closed_cycle(uint64_t inter, uint64_t outer, int
 // Squared lens areas, first pass (wavefront / polygon s, lane / record): the h sums of
 // each record -- its exterior terms plus its chain of s's edges, measured from the same
 // origin as in k_exterior_poly -- twice the area the record covers.
+template <int SEGX>
 __global__ __launch_bounds__(64) void k_lens_energy(int N, int Kp, const int* __restrict__ polyStart,
                                                    const int* __restrict__ polyPack,
                                                    const int* __restrict__ packPoly, const double2* __restrict__ pos,
@@ -174,14 +179,16 @@ __global__ __launch_bounds__(64) void k_lens_energy(int N, int Kp, const int* __
                                                    const uint64_t* __restrict__ rOuter,
                                                    const float2* __restrict__ rTU, const float2* __restrict__ rUT,
                                                    double* __restrict__ Er, const int* __restrict__ order) {
-    int lane = threadIdx.x;
-    int s = order[xcd_block(blockIdx.x, gridDim.x)];
+    int lane = threadIdx.x & (SEGX - 1);
+    int slot = xcd_block(blockIdx.x, gridDim.x) * (64 / SEGX) + threadIdx.x / SEGX;
+    if (slot >= N) return;   // no barriers or shuffles below
+    int s = order[slot];
     int cnt = min(bcnt[s], Kp);
     int pk = polyPack[s];
     int pbase = packPoly[pk];
     int st1 = polyStart[s], n1 = polyStart[s + 1] - st1;
     long long base = (long long)s * Kp;
-    for (int f = lane; f < cnt; f += 64) {
+    for (int f = lane; f < cnt; f += SEGX) {
         uint64_t inter = rKey[base + f], outer = rOuter[base + f];
         double E = 0.0;
         if (inter != outer) {
@@ -223,8 +230,11 @@ __global__ __launch_bounds__(64) void k_lens_energy(int N, int Kp, const int* __
     }
 }
 
-// MODE 0: reference energy; 1: area energy; 2: squared lens areas
-template <int MODE>
+// MODE 0: reference energy; 1: area energy; 2: squared lens areas.
+// One wavefront holds 64 / SEGX polygons, SEGX lanes each: lanes take the polygon's
+// records, then its vertices, SEGX at a time (a polygon has a few records, so a
+// wavefront per polygon would leave most lanes idle).  Every sum keeps its order.
+template <int MODE, int SEGX>
 __global__ __launch_bounds__(64) void k_exterior_poly(
     int N, int Kp, const int* __restrict__ polyStart, const int* __restrict__ polyPack,
     const int* __restrict__ packPoly, const double2* __restrict__ pos, const int* __restrict__ bcnt,
@@ -234,18 +244,21 @@ __global__ __launch_bounds__(64) void k_exterior_poly(
     double2* __restrict__ LNo, double* __restrict__ accR, const double* __restrict__ Er,
     const int* __restrict__ order) {
     constexpr int AREA = (MODE >= 1), SQ = (MODE == 2);
+    constexpr int G = 64 / SEGX;
     extern __shared__ char smem[];
     __shared__ double leBuf[64];
-    ExtLds L = ext_lds(smem, Kp);
-    int lane = threadIdx.x;
-    int s = order[xcd_block(blockIdx.x, gridDim.x)];
-    int cnt = min(bcnt[s], Kp);
+    int lane = threadIdx.x & (SEGX - 1), g = threadIdx.x / SEGX, segBase = g * SEGX;
+    ExtLds L = ext_lds(smem + g * ext_lds_stride(Kp), Kp);
+    int slot = xcd_block(blockIdx.x, gridDim.x) * G + g;
+    bool has = slot < N;
+    int s = has ? order[slot] : 0;
+    int cnt = has ? min(bcnt[s], Kp) : 0;
     int pk = polyPack[s];
     int pbase = packPoly[pk];
-    int st1 = polyStart[s], n1 = polyStart[s + 1] - st1;
+    int st1 = polyStart[s], n1 = has ? polyStart[s + 1] - st1 : 0;
     long long base = (long long)s * Kp;
     const double sc = AREA ? 0.5 : 1.0;
-    for (int f = lane; f < cnt; f += 64) {
+    for (int f = lane; f < cnt; f += SEGX) {
         uint64_t inter = rKey[base + f], outer = rOuter[base + f];
         int sjg = pbase + key_sj(inter);
         int st2 = polyStart[sjg], n2 = polyStart[sjg + 1] - st2;
@@ -386,8 +399,11 @@ __global__ __launch_bounds__(64) void k_exterior_poly(
     double acc = 0.0;
     if (lane == 0)
         for (int f = 0; f < cnt; ++f) acc += L.e[f];
+    int nMax = n1;   // the wavefront's largest polygon sets the trip count
+#pragma unroll
+    for (int o = 32; o >= SEGX; o >>= 1) nMax = max(nMax, __shfl_xor(nMax, o));
     // per vertex: contributions acting on s (record order), interior chain terms
-    for (int m0 = 0; m0 < n1; m0 += 64) {
+    for (int m0 = 0; m0 < nMax; m0 += SEGX) {
         int mLoc = m0 + lane;
         double le = 0.0;
         if (mLoc < n1) {
@@ -434,15 +450,15 @@ __global__ __launch_bounds__(64) void k_exterior_poly(
             LMo[st1 + mLoc] = make_double2(LM.x, LM.y);
             LNo[st1 + mLoc] = make_double2(LN.x, LN.y);
         }
-        leBuf[lane] = le;
+        leBuf[threadIdx.x] = le;
         __syncthreads();
         if (lane == 0) {
-            int cl = min(64, n1 - m0);
-            for (int q = 0; q < cl; ++q) acc += leBuf[q];
+            int cl = min(SEGX, n1 - m0);
+            for (int q = 0; q < cl; ++q) acc += leBuf[segBase + q];
         }
         __syncthreads();
     }
-    if (lane == 0) accR[s] = acc;
+    if (has && lane == 0) accR[s] = acc;
 }
 
 // ---------------------------------------------------------------- per-vertex gather
@@ -591,17 +607,21 @@ int alloc_buckets(pcp_ctx* c, int Kp) {
     return PCP_OK;
 }
 
+template <int EXACT>
+static int launch_pair(pcp_ctx* c, const double2* pairPos) {
+    int slot = -1;
+    PCP_TRY(prof_begin(c, EXACT ? "k_pair_poly<1>" : "k_pair_poly<0>", &slot));
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_poly<EXACT, 16>), dim3((c->N + 3) / 4), dim3(64), 0, c->stream, c->N,
+                       c->Kp, c->dPolyStart, c->dPolyPack, c->dPackPoly, c->dBcnt, c->dBKey, c->dBTU, c->dPackMaxKey,
+                       pairPos, c->dRKey, c->dRTU, c->dROuter, c->dRUT, c->dPcSorted);
+    PCP_LAUNCHED();
+    return prof_end(c, slot);
+}
+
 int run_records(pcp_ctx* c, const double2* pairPos) {
     if (c->N == 0) return PCP_OK;
-    if (pairPos)
-        PCP_LAUNCH(c, k_pair_poly<1>, dim3(c->N), dim3(64), c->N, c->Kp, c->dPolyStart, c->dPolyPack,
-                   c->dPackPoly, c->dBcnt, c->dBKey, c->dBTU, c->dPackMaxKey, pairPos, c->dRKey, c->dRTU,
-                   c->dROuter, c->dRUT, c->dPcSorted);
-    else
-        PCP_LAUNCH(c, k_pair_poly<0>, dim3(c->N), dim3(64), c->N, c->Kp, c->dPolyStart, c->dPolyPack,
-                   c->dPackPoly, c->dBcnt, c->dBKey, c->dBTU, c->dPackMaxKey, pairPos, c->dRKey, c->dRTU,
-                   c->dROuter, c->dRUT, c->dPcSorted);
-    return PCP_OK;
+    if (pairPos) return launch_pair<1>(c, pairPos);
+    return launch_pair<0>(c, pairPos);
 }
 
 template <int SEG, int GEOM, int PROJ>
@@ -639,6 +659,37 @@ static int dispatch_gather(pcp_ctx* c, int overlap, int edgeOn, int areaOn, cons
 
 // Forces and per-packing energies.  geom: compute edge lengths / areas here and write
 // them (fused evaluation); else read them.  project: project the force of every polygon.
+// k_exterior_poly with as many polygons per wavefront as the LDS allows (64 KiB per
+// workgroup): 4 (16 lanes each) for buckets up to 141 records, then 2, then 1
+template <int MODE, int SEGX>
+static int launch_exterior_w(pcp_ctx* c, const double2* pos) {
+    constexpr int G = 64 / SEGX;
+    size_t lds = G * ext_lds_stride(c->Kp);
+    int grid = (c->N + G - 1) / G;
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_exterior_poly<MODE, SEGX>), dim3(grid), dim3(64), lds, c->stream, c->N,
+                       c->Kp, c->dPolyStart, c->dPolyPack, c->dPackPoly, pos, c->dBcnt, c->dJcnt, c->dJKey, c->dRKey,
+                       c->dROuter, c->dRTU, c->dRUT, c->dJC, c->dJT, c->dF1, c->dLM, c->dLN, c->dAccR, c->dEr,
+                       c->dPcSorted);
+    PCP_LAUNCHED();
+    return PCP_OK;
+}
+template <int MODE>
+static int launch_exterior_m(pcp_ctx* c, const double2* pos) {
+    int slot = -1;
+    PCP_TRY(prof_begin(c, MODE == 0 ? "k_exterior_poly<0>" : (MODE == 1 ? "k_exterior_poly<1>" : "k_exterior_poly<2>"),
+                       &slot));
+    size_t st = ext_lds_stride(c->Kp);
+    if (4 * st <= 65536) PCP_TRY((launch_exterior_w<MODE, 16>(c, pos)));
+    else if (2 * st <= 65536) PCP_TRY((launch_exterior_w<MODE, 32>(c, pos)));
+    else PCP_TRY((launch_exterior_w<MODE, 64>(c, pos)));
+    return prof_end(c, slot);
+}
+static int launch_exterior(pcp_ctx* c, int mode, const double2* pos) {
+    if (mode == 2) return launch_exterior_m<2>(c, pos);
+    if (mode == 1) return launch_exterior_m<1>(c, pos);
+    return launch_exterior_m<0>(c, pos);
+}
+
 int run_forces(pcp_ctx* c, int model, const double2* pos, double* edgeLengths, double* areas,
                const double* targetEdgeV, const double* targetAreas, double stiffness,
                double compressibility, double2* force, double* energy, int geom, int project, bool sticky) {
@@ -666,25 +717,11 @@ int run_forces(pcp_ctx* c, int model, const double2* pos, double* edgeLengths, d
         return PCP_OK;
     }
     if (overlap) {
-        size_t lds = ext_lds_bytes(c->Kp);
-        if (sq) {
-            PCP_LAUNCH(c, k_lens_energy, dim3(c->N), dim3(64), c->N, c->Kp, c->dPolyStart, c->dPolyPack,
-                       c->dPackPoly, pos, c->dBcnt, c->dRKey, c->dROuter, c->dRTU, c->dRUT, c->dEr, c->dPcSorted);
-            PCP_LAUNCH_LDS(c, k_exterior_poly<2>, dim3(c->N), dim3(64), lds, c->N, c->Kp,
-                           c->dPolyStart, c->dPolyPack, c->dPackPoly, pos, c->dBcnt, c->dJcnt, c->dJKey, c->dRKey,
-                           c->dROuter, c->dRTU, c->dRUT, c->dJC, c->dJT, c->dF1, c->dLM, c->dLN, c->dAccR, c->dEr,
-                           c->dPcSorted);
-        } else if (area) {
-            PCP_LAUNCH_LDS(c, k_exterior_poly<1>, dim3(c->N), dim3(64), lds, c->N, c->Kp,
-                           c->dPolyStart, c->dPolyPack, c->dPackPoly, pos, c->dBcnt, c->dJcnt, c->dJKey, c->dRKey,
-                           c->dROuter, c->dRTU, c->dRUT, c->dJC, c->dJT, c->dF1, c->dLM, c->dLN, c->dAccR, c->dEr,
-                           c->dPcSorted);
-        } else {
-            PCP_LAUNCH_LDS(c, k_exterior_poly<0>, dim3(c->N), dim3(64), lds, c->N, c->Kp,
-                           c->dPolyStart, c->dPolyPack, c->dPackPoly, pos, c->dBcnt, c->dJcnt, c->dJKey, c->dRKey,
-                           c->dROuter, c->dRTU, c->dRUT, c->dJC, c->dJT, c->dF1, c->dLM, c->dLN, c->dAccR, c->dEr,
-                           c->dPcSorted);
-        }
+        if (sq)
+            PCP_LAUNCH(c, k_lens_energy<16>, dim3((c->N + 3) / 4), dim3(64), c->N, c->Kp, c->dPolyStart,
+                       c->dPolyPack, c->dPackPoly, pos, c->dBcnt, c->dRKey, c->dROuter, c->dRTU, c->dRUT, c->dEr,
+                       c->dPcSorted);
+        PCP_TRY(launch_exterior(c, sq ? 2 : (area ? 1 : 0), pos));
     }
     bool proj = project && c->maxN <= 64;
     if (geom && proj)
