@@ -30,14 +30,15 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
 # FP64 operations of one segment test in k_detect_poly (2 minimum-image wraps of 5 ops,
-# cross product 3, |denom| test 1, two parameters 4+4, four range compares 4)
+# cross product 3, |denom| test 1, two parameters 4+4, four range compares 4), counted
+# over the edge pairs whose test ran (pcp_detect_stats: after the edge-circle cull)
 FLOPS_PER_EDGE_PAIR = 21
-# bytes the detection must move from HBM per launch: every vertex position once (16 B)
-# and its crossing counter (4 B), every polygon's bounding circle and sorted id (36 B),
-# and per crossing one slot (key 8 B, float2 t/u 8 B, inside flag 1 B)
-BYTES_PER_VERTEX = 20
-BYTES_PER_POLYGON = 36
-BYTES_PER_CROSSING = 17
+# bytes the detection must move from HBM per launch: every vertex position once (16 B),
+# every polygon's bounding circle and sorted id (36 B) and its two bucket counters (8 B),
+# and per crossing its two bucket entries (key 8 B and float2 t/u 8 B; key 8 B)
+BYTES_PER_VERTEX = 16
+BYTES_PER_POLYGON = 44
+BYTES_PER_CROSSING = 24
 DETECT_KERNEL = "k_detect_poly<0>"
 
 

@@ -225,9 +225,11 @@ int pcp_fire_get(pcp_ctx* ctx, pcp_fire_status* status);
 int pcp_profile_kernel(pcp_ctx* ctx, const char* name);
 int pcp_profile_read(pcp_ctx* ctx, double* totalMs, int* launches);
 int pcp_stream_copy(pcp_ctx* ctx, double* dst, const double* src, long long n);
-/* work of the last detection: tiles tested (each polygon with itself, each pair of
- * distinct polygons whose bounding circles meet once from each side) and the edge pairs
- * they hold (nA*nB per tile of distinct polygons, n(n-3)/2 per polygon with itself) */
+/* work of the last detection: polygon pairs tested (each polygon with itself, each pair
+ * of distinct polygons whose bounding circles meet once) and the edge pairs whose segment
+ * test ran (n(n-3)/2 per polygon with itself; per pair of distinct polygons the product
+ * of the counts of edges of each near the other's circle, or nA*nB where the cull does
+ * not apply).  The last detection's position buffer must still be valid. */
 int pcp_detect_stats(pcp_ctx* ctx, long long* polygonPairs, long long* edgePairs);
 
 #ifdef __cplusplus

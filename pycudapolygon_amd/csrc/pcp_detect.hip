@@ -135,6 +135,9 @@ __device__ __forceinline__ bool circles_meet(double4 a, double4 b) {
 // below 3 or a polygon is big.  Writes pcStart[p*(PC_CELLS+1) + 0..B*B], the sorted
 // polygon ids into pcSorted[packPoly[p]..] (their circles into sBounds[..]), and
 // packB[p] = B | big << 30.
+// The bounds of up to PC_REG * PP_BLK polygons are read once, all loads in flight
+// together, and kept in registers for the three passes; beyond that they are re-read.
+constexpr int PC_REG = 8;
 __global__ __launch_bounds__(PP_BLK) void k_poly_cells(const int* __restrict__ packPoly,
                                                       const double4* __restrict__ bounds,
                                                       int* __restrict__ pcStart, int* __restrict__ pcSorted,
@@ -147,9 +150,20 @@ __global__ __launch_bounds__(PP_BLK) void k_poly_cells(const int* __restrict__ p
     int p = blockIdx.x, t = threadIdx.x, lane = t & 63, w = t >> 6;
     int s0 = packPoly[p], s1 = packPoly[p + 1], Np = s1 - s0;
     long long gbase = (long long)p * (PC_CELLS + 1);
+    double4 bc[PC_REG];
+#pragma unroll
+    for (int k = 0; k < PC_REG; ++k) {
+        int s = s0 + t + k * PP_BLK;
+        bc[k] = (s < s1) ? bounds[s] : make_double4(0.0, 0.0, 0.0, 0.0);
+    }
     double r = 0.0;
     int big = 0;
-    for (int s = s0 + t; s < s1; s += PP_BLK) {
+#pragma unroll
+    for (int k = 0; k < PC_REG; ++k) {
+        r = fmax(r, bc[k].z);
+        big |= (bc[k].w != 0.0);
+    }
+    for (int s = s0 + t + PC_REG * PP_BLK; s < s1; s += PP_BLK) {
         double4 b = bounds[s];
         r = fmax(r, b.z);
         big |= (b.w != 0.0);
@@ -170,7 +184,14 @@ __global__ __launch_bounds__(PP_BLK) void k_poly_cells(const int* __restrict__ p
     int C = B * B;
     for (int q = t; q < C; q += PP_BLK) cCur[q] = 0;
     __syncthreads();
-    for (int s = s0 + t; s < s1; s += PP_BLK) {
+    int cellOf[PC_REG];
+#pragma unroll
+    for (int k = 0; k < PC_REG; ++k) {
+        int s = s0 + t + k * PP_BLK;
+        cellOf[k] = pc_coord(bc[k].y, B) * B + pc_coord(bc[k].x, B);
+        if (s < s1) atomicAdd(&cCur[cellOf[k]], 1);
+    }
+    for (int s = s0 + t + PC_REG * PP_BLK; s < s1; s += PP_BLK) {
         double4 b = bounds[s];
         atomicAdd(&cCur[pc_coord(b.y, B) * B + pc_coord(b.x, B)], 1);
     }
@@ -204,7 +225,16 @@ __global__ __launch_bounds__(PP_BLK) void k_poly_cells(const int* __restrict__ p
         pcStart[gbase + C] = Np;
         packB[p] = B | (big ? (1 << 30) : 0);
     }
-    for (int s = s0 + t; s < s1; s += PP_BLK) {
+#pragma unroll
+    for (int k = 0; k < PC_REG; ++k) {
+        int s = s0 + t + k * PP_BLK;
+        if (s < s1) {
+            int slot = atomicAdd(&cCur[cellOf[k]], 1);
+            pcSorted[s0 + slot] = s;
+            sBounds[s0 + slot] = bc[k];
+        }
+    }
+    for (int s = s0 + t + PC_REG * PP_BLK; s < s1; s += PP_BLK) {
         double4 b = bounds[s];
         int slot = atomicAdd(&cCur[pc_coord(b.y, B) * B + pc_coord(b.x, B)], 1);
         pcSorted[s0 + slot] = s;
@@ -699,12 +729,13 @@ __device__ void detect_polygon(
         }
     }
     // the largest key of the packing (the pairing's last-record case)
+    unsigned long long pmk = packMaxKey[pk];
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
         unsigned long long y = __shfl_xor((unsigned long long)kmax, o);
         kmax = (y > kmax) ? y : kmax;
     }
-    if (lane == 0 && kmax > 0 && kmax > packMaxKey[pk]) atomicMax(&packMaxKey[pk], (unsigned long long)kmax);
+    if (lane == 0 && kmax > 0 && kmax > pmk) atomicMax(&packMaxKey[pk], (unsigned long long)kmax);
 }
 
 // One wavefront per polygon A (in cell order, XCD-contiguous): detect_polygon.
@@ -769,6 +800,7 @@ int launch_bounds(pcp_ctx* c, const double2* pos) {
 // cellsFromList: use the cells of the last run_cells (API updateNeighbors)
 int run_detect(pcp_ctx* c, const double2* pos, int cellsFromList) {
     if (c->V == 0) return PCP_OK;
+    c->lastDetectPos = pos;
     PCP_TRY(launch_bounds(c, pos));
     PCP_LAUNCH(c, k_poly_cells, dim3(c->P), dim3(PP_BLK), c->dPackPoly, c->dBounds, c->dPcStart, c->dPcSorted,
                c->dPackB, c->dSBounds);
@@ -804,18 +836,36 @@ int detect_alloc(pcp_ctx* c) {
 }
 
 // Work of the last detection, recomputed on the host from the device's polygon cells
-// with the kernel's candidate rule: tiles tested (a polygon with itself, and each pair of
-// polygons once from each side) and the edge pairs they hold.
+// and positions with the kernel's rules: polygon pairs tested (each polygon with itself,
+// each pair of polygons once) and the edge pairs whose segment test ran (A with itself:
+// its non-adjacent pairs; A with B: the products of the edge lists left by the cull).
 int detect_stats(pcp_ctx* c, long long* polygonPairs, long long* edgePairs) {
     PCP_HIP(hipStreamSynchronize(c->stream));
+    if (!c->lastDetectPos) { *polygonPairs = 0; *edgePairs = 0; return PCP_OK; }
     std::vector<double4> bd(c->N);
     std::vector<int> sorted(c->N), packB(c->P), start((size_t)c->P * (PC_CELLS + 1));
+    std::vector<double2> X(c->V);
     PCP_HIP(hipMemcpy(bd.data(), c->dBounds, sizeof(double4) * c->N, hipMemcpyDeviceToHost));
     PCP_HIP(hipMemcpy(sorted.data(), c->dPcSorted, sizeof(int) * c->N, hipMemcpyDeviceToHost));
     PCP_HIP(hipMemcpy(packB.data(), c->dPackB, sizeof(int) * c->P, hipMemcpyDeviceToHost));
     PCP_HIP(hipMemcpy(start.data(), c->dPcStart, sizeof(int) * start.size(), hipMemcpyDeviceToHost));
+    PCP_HIP(hipMemcpy(X.data(), c->lastDetectPos, sizeof(double2) * c->V, hipMemcpyDeviceToHost));
     auto wrapH = [](double x) { double y = x + 1.5; y = y - std::floor(y); return y - 0.5; };
     auto coord = [](double x, int B) { double f = std::floor(x * B); int i = (f >= 0.0) ? (int)f : 0; return (i >= B) ? B - 1 : i; };
+    auto cullOk = [&](int q) { return bd[q].w == 0.0 && bd[q].z < CULL_RADIUS; };
+    // edges of polygon q near the circle of polygon o (edge_near_circle)
+    auto nearCount = [&](int q, int o) {
+        int st = c->hPolyStart[q], n = c->hPolyStart[q + 1] - st;
+        long long k = 0;
+        for (int e = 0; e < n; ++e) {
+            double2 p0 = X[st + e], p1 = X[st + ((e + 1 == n) ? 0 : e + 1)];
+            double Rx = wrapH(p1.x - p0.x), Ry = wrapH(p1.y - p0.y);
+            double dx = wrapH(bd[o].x - (p0.x + 0.5 * Rx)), dy = wrapH(bd[o].y - (p0.y + 0.5 * Ry));
+            double rr = (bd[o].z + 0.5 * (std::fabs(Rx) + std::fabs(Ry))) * (1.0 + 1e-6) + 1e-12;
+            k += (dx * dx + dy * dy <= rr * rr);
+        }
+        return k;
+    };
     long long pp = 0, ep = 0;
     for (int p = 0; p < c->P; ++p) {
         int B = packB[p] & 0xFFFF;
@@ -833,14 +883,16 @@ int detect_stats(pcp_ctx* c, long long* polygonPairs, long long* edgePairs) {
                 int cell = cy * B + cx;
                 for (int q = st[cell]; q < st[cell + 1]; ++q) {
                     int Bq = sorted[s0 + q];
-                    if (Bq == A) continue;
+                    if (Bq <= A) continue;
                     if (!big) {
                         double dx = wrapH(bd[Bq].x - bd[A].x), dy = wrapH(bd[Bq].y - bd[A].y);
                         double rr = (bd[A].z + bd[Bq].z) * (1.0 + 1e-6) + 1e-12;
                         if (!(dx * dx + dy * dy <= rr * rr)) continue;
                     }
                     ++pp;
-                    ep += nA * (long long)(c->hPolyStart[Bq + 1] - c->hPolyStart[Bq]);
+                    long long nB = c->hPolyStart[Bq + 1] - c->hPolyStart[Bq];
+                    if (nA <= 64 && nB <= 64 && cullOk(A) && cullOk(Bq)) ep += nearCount(A, Bq) * nearCount(Bq, A);
+                    else ep += nA * nB;
                 }
             }
         }

@@ -79,6 +79,7 @@ struct pcp_ctx {
     double4* dSBounds = nullptr;   // N: their circles in the same order
     int* dPcStart = nullptr;       // P x (PC_CELLS + 1): cell starts of each packing
     int* dPackB = nullptr;         // P: polygon grid size | big << 30
+    const double2* lastDetectPos = nullptr;   // positions of the last detection (statistics)
     int* dSticky = nullptr;        // [0] a fused evaluation overflowed a buffer
     int* dNum = nullptr;           // V: crossings per edge (updateNeighbors' count)
     unsigned long long* dPackMaxKey = nullptr;   // P: largest record key of each packing
