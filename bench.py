@@ -40,6 +40,9 @@ BYTES_PER_VERTEX = 16
 BYTES_PER_POLYGON = 44
 BYTES_PER_CROSSING = 24
 DETECT_KERNEL = "k_detect_poly<0>"
+# HBM bytes per k_detect_poly<0> launch on this workload, measured with rocprofv3 FETCH_SIZE
+# and WRITE_SIZE and scaled by a copy kernel's readings (profiles/k_detect_poly_hbm_traffic_N4096.md)
+DETECT_TRAFFIC_BYTES = 4047258
 
 
 def cpu_baseline(p, B, budget_s):
@@ -133,7 +136,7 @@ def main():
         achieved = flops / kavg_s / 1e12
         roof = dict(bound="mfma", compute="fp64 VALU (no MFMA in the geometry); peak = FP64 78.6 TFLOP/s",
                     achieved=achieved, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s", frac=achieved / FP64_PEAK_TFLOPS,
-                    traffic=None, kernel=DETECT_KERNEL, flops_per_launch=flops, edge_pair_tests=tests,
+                    traffic=DETECT_TRAFFIC_BYTES, kernel=DETECT_KERNEL, flops_per_launch=flops, edge_pair_tests=tests,
                     polygon_pairs=ppairs.value, algorithmic_bytes_per_launch=nbytes, avg_launch_us=kavg_s * 1e6,
                     hbm_gbs_algorithmic=nbytes / kavg_s / 1e9)
     else:
