@@ -384,6 +384,8 @@ struct DetLds {
     uint8_t listB[DET_BATCH];     // candidate edges near A's circle (local ids), per candidate
     uint8_t listA[DET_KMAX * 64]; // A's edges near each candidate's circle
     uint8_t klane[DET_KMAX];      // lane of the batch's k-th candidate
+    int4 kinfo[DET_KMAX];         // per candidate: first pair, near-edge count, first vertex in V, vertices
+    int2 kpoly[DET_KMAX];         // per candidate: polygon id, first vertex in pos
 };
 
 // Where the lanes take the vertices of polygon B from in edge_tile: global memory, or (A
@@ -680,20 +682,29 @@ __device__ void detect_polygon(
             }
             int pexc = pinc - np;
             int Tp = __builtin_amdgcn_readlane(pinc, 63);
+            if (inB) {
+                L.kinfo[kr] = make_int4(pexc, cntB, off, nqb);
+                L.kpoly[kr] = make_int2(Bq, stq);
+            }
+            wave_lds_sync();
+            // the batch's pair starts in registers: the owner of pair qq is the last candidate
+            // whose first pair is <= qq (a candidate without pairs shares its successor's start
+            // and so is never the last), found by counting, with no memory access per pass
+            int pst[DET_KMAX];
+#pragma unroll
+            for (int j = 0; j < DET_KMAX; ++j) pst[j] = (j < K) ? L.kinfo[j].x : 0x7FFFFFFF;
             for (int p0 = 0; p0 < Tp; p0 += 64) {
                 int qq = p0 + lane;
                 bool valid = qq < Tp;
-                // the pair's candidate: the last lane whose first pair is <= qq (lanes
-                // without pairs share their successor's start and are never the last)
-                int k = 0;
+                int krk = -1;
 #pragma unroll
-                for (int step = 32; step >= 1; step >>= 1) {
-                    int t = __shfl(pexc, k + step);
-                    if (t <= qq) k += step;
-                }
-                int r = qq - __shfl(pexc, k);
-                int nbS = __shfl(cntB, k), krk = __shfl(kr, k), offk = __shfl(off, k), nk = __shfl(nqb, k);
-                int Bk = __shfl(Bq, k), stk = __shfl(stq, k);
+                for (int j = 0; j < DET_KMAX; ++j) krk += (qq >= pst[j]) ? 1 : 0;
+                krk = valid ? krk : 0;
+                int4 ki = L.kinfo[krk];
+                int2 kp = L.kpoly[krk];
+                int r = qq - ki.x;
+                int nbS = ki.y, offk = ki.z, nk = ki.w;
+                int Bk = kp.x, stk = kp.y;
                 SegNum s;
                 d2 Q = mk(0.0, 0.0), Q1 = mk(0.0, 0.0);
                 int a = 0, b = 0;
