@@ -290,18 +290,20 @@ struct DetOut {
 // capacity is not written and raises the overflow maximum, after which the caller grows
 // the buckets and detects again.  The order inside a bucket is the order of arrival;
 // every consumer orders the records by key.
+// Both slots are taken before anything is stored: the stores would otherwise sit between
+// the two atomics and the wait for the second one (the vector-memory counter also counts
+// stores) would include their completion.
 __device__ __forceinline__ void emit_record(uint64_t key, float2 tu, int s0, const DetOut& o) {
     int si = s0 + key_si(key), sj = s0 + key_sj(key);
     int a = atomicAdd(&o.bcnt[si], 1);
+    int b = atomicAdd(&o.jcnt[sj], 1);
+    int over = max(a, b);   // needs both slots: one wait, before the stores
+    if (over >= o.Kp) atomicMax(&o.scal[SC_BUCKET], over + 1);
     if (a < o.Kp) {
         o.bKey[(long long)si * o.Kp + a] = key;
         o.bTU[(long long)si * o.Kp + a] = tu;
-    } else {
-        atomicMax(&o.scal[SC_BUCKET], a + 1);
     }
-    int b = atomicAdd(&o.jcnt[sj], 1);
     if (b < o.Kp) o.jKey[(long long)sj * o.Kp + b] = key;
-    else atomicMax(&o.scal[SC_BUCKET], b + 1);
 }
 
 // Segment test of updateNeighborsKernel (kernels.cuh:760-790) for edge 1 = (P1, P1 + R1)
