@@ -384,6 +384,7 @@ struct DetLds {
     double2 V[DET_BATCH];         // candidate vertices, polygon after polygon
     uint8_t own[DET_BATCH];       // lane of the candidate a vertex belongs to
     uint8_t listB[DET_BATCH];     // candidate edges near A's circle (local ids), per candidate
+    int cellB[DET_BATCH];         // user-grid cell (x | y << 16) of each candidate edge
     uint8_t listA[DET_KMAX * 64]; // A's edges near each candidate's circle
     uint8_t klane[DET_KMAX];      // lane of the batch's k-th candidate
     int4 kinfo[DET_KMAX];         // per candidate: first pair, near-edge count, first vertex in V, vertices
@@ -640,6 +641,7 @@ __device__ void detect_polygon(
                 bool valid = e < Vb;
                 int k = valid ? L.own[e] : 0;
                 int offk = __shfl(off, k), nk = __shfl(nqb, k), carry = __shfl(cntB, k);
+                int stk = CELLS ? __shfl(stq, k) : 0;
                 bool cu = __shfl((int)cullB, k) != 0;
                 bool near = false;
                 int v = e - offk;
@@ -647,6 +649,9 @@ __device__ void detect_polygon(
                     double2 q0 = L.V[e], q1 = L.V[(v + 1 == nk) ? offk : e + 1];
                     d2 R = mk(wrap(q1.x - q0.x), wrap(q1.y - q0.y));
                     near = !cu || edge_near_circle(mk(q0.x, q0.y), R, bA.x, bA.y, bA.z);
+                    // the edge's cell for the stencil filter, computed here for every edge at
+                    // once rather than per crossing candidate in the passes below
+                    L.cellB[e] = CELLS ? cell_xy(cellOf[stk + v], Bu) : edge_cell_xy(mk(q0.x, q0.y), mk(q1.x, q1.y), Bu);
                 }
                 unsigned long long nm = __ballot(near);
                 if (near) L.listB[offk + carry + __popcll(nm & bit_range(offk - e0, lane))] = (uint8_t)v;
@@ -706,7 +711,7 @@ __device__ void detect_polygon(
                 int2 kp = L.kpoly[krk];
                 int r = qq - ki.x;
                 int nbS = ki.y, offk = ki.z, nk = ki.w;
-                int Bk = kp.x, stk = kp.y;
+                int Bk = kp.x;
                 SegNum s;
                 d2 Q = mk(0.0, 0.0), Q1 = mk(0.0, 0.0);
                 int a = 0, b = 0;
@@ -729,10 +734,8 @@ __device__ void detect_polygon(
                 uint64_t key = 0;
                 float2 tu;
                 bool rec = false;
-                if (may) {
-                    int cb = CELLS ? cell_xy(cellOf[stk + b], Bu) : edge_cell_xy(Q, Q1, Bu);
-                    if (cells_adjacent(L.C[a], cb, Bu)) rec = exact_record(s, sAl, a, Bk - s0, b, key, tu);
-                }
+                if (may && cells_adjacent(L.C[a], L.cellB[offk + b], Bu))
+                    rec = exact_record(s, sAl, a, Bk - s0, b, key, tu);
                 if (rec) {
                     kmax = (key > kmax) ? key : kmax;
                     if (CELLS) atomicAdd(&num[stA + a], 1);   // per-edge count of edge 1 (A's)
