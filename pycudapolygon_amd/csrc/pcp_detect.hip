@@ -700,8 +700,17 @@ __device__ void detect_polygon(
             int pst[DET_KMAX];
 #pragma unroll
             for (int j = 0; j < DET_KMAX; ++j) pst[j] = (j < K) ? L.kinfo[j].x : 0x7FFFFFFF;
-            for (int p0 = 0; p0 < Tp; p0 += 64) {
-                int qq = p0 + lane;
+            // pair qq of the batch: its candidate, its two edges, the division-free test
+            struct PairT {
+                SegNum s;
+                int a, b, offk, Bk;
+                bool may;
+            };
+            auto pair_at = [&](int qq) {
+                PairT t;
+                t.may = false;
+                t.a = 0;
+                t.b = 0;
                 bool valid = qq < Tp;
                 int krk = -1;
 #pragma unroll
@@ -710,37 +719,43 @@ __device__ void detect_polygon(
                 int4 ki = L.kinfo[krk];
                 int2 kp = L.kpoly[krk];
                 int r = qq - ki.x;
-                int nbS = ki.y, offk = ki.z, nk = ki.w;
-                int Bk = kp.x;
-                SegNum s;
-                d2 Q = mk(0.0, 0.0), Q1 = mk(0.0, 0.0);
-                int a = 0, b = 0;
-                bool may = false;
+                int nbS = ki.y, nk = ki.w;
+                t.offk = ki.z;
+                t.Bk = kp.x;
                 if (valid) {
                     // r = ia * nbS + ib; the float quotient is exact here (r < 4096, nbS <= 64:
                     // (r + 0.5) / nbS stays 1/128 away from an integer)
                     int ia = (int)(((float)r + 0.5f) * __builtin_amdgcn_rcpf((float)nbS));
                     int ib = r - ia * nbS;
-                    a = L.listA[krk * 64 + ia];
-                    b = L.listB[offk + ib];
-                    double2 q0 = L.V[offk + b], q1 = L.V[offk + ((b + 1 == nk) ? 0 : b + 1)];
-                    double2 pa = L.P[a], ra = L.R[a];
-                    Q = mk(q0.x, q0.y);
-                    Q1 = mk(q1.x, q1.y);
-                    s = seg_numerators(mk(pa.x, pa.y), mk(ra.x, ra.y), Q, mk(wrap(Q1.x - Q.x), wrap(Q1.y - Q.y)));
-                    may = seg_may_cross(s);
+                    t.a = L.listA[krk * 64 + ia];
+                    t.b = L.listB[t.offk + ib];
+                    double2 q0 = L.V[t.offk + t.b], q1 = L.V[t.offk + ((t.b + 1 == nk) ? 0 : t.b + 1)];
+                    double2 pa = L.P[t.a], ra = L.R[t.a];
+                    t.s = seg_numerators(mk(pa.x, pa.y), mk(ra.x, ra.y), mk(q0.x, q0.y),
+                                         mk(wrap(q1.x - q0.x), wrap(q1.y - q0.y)));
+                    t.may = seg_may_cross(t.s);
                 }
-                if (__ballot(may) == 0ull) continue;   // wavefront-uniform: no crossing here
+                return t;
+            };
+            // the exact test, the stencil filter and the record of a pair that passed
+            auto exact_pass = [&](const PairT& t) {
                 uint64_t key = 0;
                 float2 tu;
                 bool rec = false;
-                if (may && cells_adjacent(L.C[a], L.cellB[offk + b], Bu))
-                    rec = exact_record(s, sAl, a, Bk - s0, b, key, tu);
+                if (t.may && cells_adjacent(L.C[t.a], L.cellB[t.offk + t.b], Bu))
+                    rec = exact_record(t.s, sAl, t.a, t.Bk - s0, t.b, key, tu);
                 if (rec) {
                     kmax = (key > kmax) ? key : kmax;
-                    if (CELLS) atomicAdd(&num[stA + a], 1);   // per-edge count of edge 1 (A's)
+                    if (CELLS) atomicAdd(&num[stA + t.a], 1);   // per-edge count of edge 1 (A's)
                     emit_record(key, tu, s0, out);
                 }
+            };
+            // two pairs per lane per pass: two independent chains in flight
+            for (int p0 = 0; p0 < Tp; p0 += 128) {
+                PairT t0 = pair_at(p0 + lane), t1 = pair_at(p0 + 64 + lane);
+                unsigned long long m0 = __ballot(t0.may), m1 = __ballot(t1.may);
+                if (m0) exact_pass(t0);   // wavefront-uniform branches: most passes skip both
+                if (m1) exact_pass(t1);
             }
         }
     }
