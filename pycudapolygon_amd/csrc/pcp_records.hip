@@ -26,6 +26,23 @@ namespace pcp {
 
 constexpr int REC_MAX_KP = 512;   // LDS of k_exterior_poly: Kp * 116 bytes per polygon (<= 64 KiB)
 
+// ---------------------------------------------------------------- bucket scans
+// Scans of a short bucket (a few to a few tens of keys) in groups of 8 with all the
+// group's loads issued before any is used, so that a scan costs one memory latency per
+// group instead of one per key.
+template <class Below>
+__device__ __forceinline__ int count_keys_below(const uint64_t* __restrict__ keys, int n, Below below) {
+    int c = 0;
+    for (int q0 = 0; q0 < n; q0 += 8) {
+        uint64_t v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = (q0 + k < n) ? keys[q0 + k] : 0ull;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) c += (q0 + k < n && below(v[k])) ? 1 : 0;
+    }
+    return c;
+}
+
 // ---------------------------------------------------------------- pairing
 // The reference's binary search starts with end = I-1, so when every key of the packing
 // lies below (si<<48 | sj<<32) the last record of the packing is the only candidate;
@@ -55,8 +72,7 @@ __global__ __launch_bounds__(64) void k_pair_poly(int N, int Kp, const int* __re
         uint64_t key = bKey[base + f];
         float2 tu = bTU[base + f];
         uint64_t ord = bucket_order(key);
-        int rank = 0;
-        for (int q = 0; q < cnt; ++q) rank += (bucket_order(bKey[base + q]) < ord);
+        int rank = count_keys_below(bKey + base, cnt, [&](uint64_t k2) { return bucket_order(k2) < ord; });
         int si_l = key_si(key), sj_l = key_sj(key), i = key_i(key);
         int sjg = pbase + sj_l;
         int st2 = polyStart[sjg], nj = polyStart[sjg + 1] - st2;
@@ -97,10 +113,17 @@ __global__ __launch_bounds__(64) void k_pair_poly(int N, int Kp, const int* __re
         } else {
             long long b2 = (long long)sjg * Kp;
             int c2 = min(bcnt[sjg], Kp);
-            for (int q = 0; q < c2; ++q) {
-                uint64_t k2 = bKey[b2 + q];
-                if (key_sj(k2) != si_l) continue;
-                consider(k2, bTU[b2 + q]);
+            for (int q0 = 0; q0 < c2; q0 += 8) {   // the group's loads first, then in order
+                uint64_t k8[8];
+                float2 t8[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    k8[k] = (q0 + k < c2) ? bKey[b2 + q0 + k] : 0ull;
+                    t8[k] = (q0 + k < c2) ? bTU[b2 + q0 + k] : make_float2(0.f, 0.f);
+                }
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (q0 + k < c2 && key_sj(k8[k]) == si_l) consider(k8[k], t8[k]);
             }
         }
         // a partner whose edge indices fall outside the two polygons (possible only in the
@@ -144,12 +167,18 @@ __device__ __forceinline__ bool find_record(uint64_t key, int pbase, int Kp, con
     int s2 = pbase + key_si(key);
     long long b2 = (long long)s2 * Kp;
     int c2 = min(bcnt[s2], Kp);
-    for (int q = 0; q < c2; ++q)
-        if (rKey[b2 + q] == key) {
-            *outer = rOuter[b2 + q];
-            if (row) *row = b2 + q;
-            return true;
-        }
+    for (int q0 = 0; q0 < c2; q0 += 8) {   // the group's loads first, then the first match
+        uint64_t k8[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) k8[k] = (q0 + k < c2) ? rKey[b2 + q0 + k] : ~key;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (k8[k] == key) {
+                *outer = rOuter[b2 + q0 + k];
+                if (row) *row = b2 + q0 + k;
+                return true;
+            }
+    }
     return false;
 }
 
@@ -384,8 +413,7 @@ __global__ __launch_bounds__(64) void k_exterior_poly(
         // sj side: at this record's rank among the records whose sj is sjg (key order)
         long long bj = (long long)sjg * Kp;
         int cj = min(jcnt[sjg], Kp);
-        int r2 = 0;
-        for (int q = 0; q < cj; ++q) r2 += (jKey[bj + q] < inter);
+        int r2 = count_keys_below(jKey + bj, cj, [&](uint64_t k2) { return k2 < inter; });
         if (r2 < Kp) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -517,13 +545,19 @@ __global__ __launch_bounds__(BLK) void k_gather_poly(
             if (overlap) {
                 double2 f1 = F1[m];
                 F = mk(f1.x, f1.y);
-                for (int r = 0; r < jn; ++r) {
+                for (int r0 = 0; r0 < jn; r0 += 4) {   // targets of 4 records loaded at once
+                    int4 t4[4];
 #pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                        if (jT[(bj + r) * 4 + q] == mLoc) {
-                            double2 cc = jC[(bj + r) * 4 + q];
-                            F.x += cc.x; F.y += cc.y;
-                        }
+                    for (int k = 0; k < 4; ++k)
+                        t4[k] = (r0 + k < jn) ? ((const int4*)jT)[bj + r0 + k] : make_int4(-1, -1, -1, -1);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        long long e = (bj + r0 + k) * 4;
+                        if (t4[k].x == mLoc) { double2 cc = jC[e + 0]; F.x += cc.x; F.y += cc.y; }
+                        if (t4[k].y == mLoc) { double2 cc = jC[e + 1]; F.x += cc.x; F.y += cc.y; }
+                        if (t4[k].z == mLoc) { double2 cc = jC[e + 2]; F.x += cc.x; F.y += cc.y; }
+                        if (t4[k].w == mLoc) { double2 cc = jC[e + 3]; F.x += cc.x; F.y += cc.y; }
+                    }
                 }
                 double2 LM = LMi[m], LN = LNi[m];
                 if (LM.x != 0.0 || LM.y != 0.0) { F.x += LM.x; F.y += LM.y; }
