@@ -852,6 +852,8 @@ int run_detect(pcp_ctx* c, const double2* pos, int cellsFromList) {
 int detect_alloc(pcp_ctx* c) {
     c->maxN = max_poly_vertices(c);
     c->seg = seg_width(c->maxN);
+    c->row16 = c->N > 0;
+    for (int s = 0; s < c->N; ++s) c->row16 = c->row16 && c->hPolyStart[s + 1] - c->hPolyStart[s] == 16;
     PCP_TRY(dalloc(&c->dBounds, std::max(c->N, 1)));
     PCP_TRY(dalloc(&c->dSBounds, std::max(c->N, 1)));
     PCP_TRY(dalloc(&c->dPcSorted, std::max(c->N, 1)));

@@ -74,6 +74,7 @@ struct pcp_ctx {
     // detection: polygon bounding circles and their per-packing cell sort
     int seg = 16;                  // lanes per polygon in the segment kernels
     int maxN = 1;                  // largest polygon (vertices)
+    bool row16 = false;            // every polygon has exactly 16 vertices (DPP row solves)
     double4* dBounds = nullptr;    // N: centre x, y, radius, big flag
     int* dPcSorted = nullptr;      // N: polygons sorted by cell, per packing
     double4* dSBounds = nullptr;   // N: their circles in the same order

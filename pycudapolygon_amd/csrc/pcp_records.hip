@@ -609,7 +609,7 @@ __global__ __launch_bounds__(BLK) void k_gather_poly(
             for (int q = 0; q < cl; ++q) e += __shfl(ee, q, SEG);
         }
         double2 fo = make_double2(F.x, F.y);
-        if (PROJ && n <= SEG && n >= 3) fo = seg_project_force<SEG>(sub, n, Pm, fo);
+        if (PROJ && n <= SEG && n >= 3) fo = seg_project_force<SEG, PROJ == 2>(sub, n, Pm, fo);
         if (live) force[st + mLoc] = fo;
     }
     if (areaOn) {
@@ -684,7 +684,11 @@ static int dispatch_gather(pcp_ctx* c, int overlap, int edgeOn, int areaOn, cons
     switch (c->seg) {
     case 4: PCP_GATHER(4);
     case 8: PCP_GATHER(8);
-    case 16: PCP_GATHER(16);
+    case 16:
+        if (PROJ && c->row16)   // PROJ 2: the projection's lane moves as DPP row moves
+            return launch_gather<16, GEOM, 2>(c, overlap, edgeOn, areaOn, pos, edgeLengths, areas, targetEdgeV,
+                                              targetAreas, stiffness, compressibility, force);
+        PCP_GATHER(16);
     case 32: PCP_GATHER(32);
     default: PCP_GATHER(64);
     }

@@ -251,6 +251,45 @@ def test_shake_parity(seed):
     assert np.abs(g["areas"] - p["targetAreas"]).max() < 1e-15
 
 
+@pytest.mark.parametrize("seed", range(2))
+def test_sixteen_vertex_solves_bit_identical_to_general_path(seed):
+    """Packings whose polygons all have 16 vertices run the projection and SHAKE with
+    DPP row moves; with one 15-vertex polygon in another packing of the same context they
+    take the general shuffle path.  The 16-vertex packing's results must be bit-identical
+    between the two, and match the oracle within the tolerances of the tests above."""
+    p = make_packing(64, 16, 0.8, seed=seed)
+    extra = make_packing(4, 15, 0.3, seed=seed + 10)
+    B, Bx = box_size_for(p["maxEdgeLength"]), box_size_for(extra["maxEdgeLength"])
+    rng = np.random.default_rng(seed)
+    F = rng.normal(size=p["positions"].shape)
+    Fx = rng.normal(size=extra["positions"].shape)
+    noisy = (p["positions"] + rng.normal(scale=2e-4, size=p["positions"].shape)) % 1.0
+    res = []
+    for packs, Bs, forces in (([p], [B], F), ([p, extra], [B, Bx], np.concatenate([F, Fx]))):
+        print(f"[row16] seed={seed} packings={len(packs)}", flush=True)
+        s = capi.System(packs, Bs)
+        s.force.upload(forces)
+        s.project()
+        Fp = s.get_forces(0).copy()
+        s.pos.upload(np.concatenate([noisy] + [q["positions"] for q in packs[1:]]))
+        it = s.shake(50)
+        res.append((Fp, s.get_positions(0).copy(), int(it[0])))
+        s2 = capi.System(packs, Bs)
+        prm = s2.fire_params(fTol=1e-10)
+        s2.fire_init(prm, 1e-3)
+        for _ in range(10):
+            s2.fire_step(prm)
+        res[-1] += (s2.get_positions(0).copy(),)
+    (Fa, xa, ia, fa), (Fb, xb, ib, fb) = res
+    assert np.array_equal(Fa, Fb) and np.array_equal(xa, xb) and ia == ib
+    assert np.array_equal(fa, fb)
+    Fo = O.project_force(p["positions"], p["startIndices"], F)
+    assert np.abs(Fa - Fo).max() <= 1e-12 * np.abs(F).max()
+    xo, ito = O.shake(noisy, p["startIndices"], p["targetEdgeV"], p["targetAreas"], 50)
+    d = (xa - xo + 0.5) % 1.0 - 0.5
+    assert np.abs(d).max() < 1e-13 and abs(ia - ito) <= 1
+
+
 def test_fire_steps_track_oracle():
     """First FIRE steps (reference convention) vs the oracle's FIRE: same step
     decisions and positions within 1e-12 (projection/SHAKE differ in rounding only)."""
