@@ -338,16 +338,19 @@ __global__ __launch_bounds__(BLK) void k_fire_stats(int N, const int* __restrict
                                                    double* __restrict__ fs, pcp_fire_status* __restrict__ status,
                                                    int* __restrict__ summary, const int* __restrict__ scal) {
     __shared__ double L[BLK / 64];
-    __shared__ double part[BLK + 1];
+    __shared__ double part[3 * (BLK + 1)];
     int p = blockIdx.x;
     int a0 = packPoly[p], n = packPoly[p + 1] - a0;
-    double Pw = 0.0, vv = 0.0, ff = 0.0;
-    if (full) {
-        Pw = det_sum_block(red + a0, n, part);
-        vv = det_sum_block(red + N + a0, n, part);
-        ff = det_sum_block(red + 2LL * N + a0, n, part);
+    double Pw = 0.0, vv = 0.0, ff = 0.0, mf;
+    if (full) {   // F.v, v.v, F.F and max |F| in one pass
+        double r[4];
+        det_reduce_block<3, true>(red + a0, N, n, part, L, r);
+        Pw = r[0]; vv = r[1]; ff = r[2]; mf = r[3];
+    } else {
+        double r[1];
+        det_reduce_block<0, true>(red + 3LL * N + a0, N, n, part, L, r);
+        mf = r[0];
     }
-    double mf = max_block(red + 3LL * N + a0, n, L);
     if (threadIdx.x == 0) {
         if (full)
             fire_control2(p, active, Pw, vv, ff, nMin, dtMax, alphaStart, fAlpha, fInc, fDec, dt, alpha, nPos,

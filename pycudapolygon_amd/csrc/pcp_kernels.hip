@@ -463,6 +463,68 @@ __device__ double max_block(const double* __restrict__ xa, int n, double* L) {
     return r;
 }
 
+// NS sums (det_sum_block's trees, bit-identical) of the arrays xa + a * stride, a < NS,
+// and with MX the maximum of array NS, all from one pass whose loads are issued together
+// (one memory latency instead of one per array).  part: NS * (BLK + 1) doubles, L: BLK / 64.
+// res[a] = sum of array a; res[NS] = the maximum.  Valid in every thread.
+template <int NS, bool MX>
+__device__ void det_reduce_block(const double* __restrict__ xa, long long stride, int n, double* part, double* L,
+                                 double* res) {
+    constexpr int NA = NS + (MX ? 1 : 0);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int chunks = (n + 255) >> 8;
+    double mx = 0.0;
+    for (int c0 = w * DR_BATCH; c0 < chunks; c0 += (BLK / 64) * DR_BATCH) {
+        double v[NA][DR_BATCH][4];
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int u = 0; u < DR_BATCH; ++u)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    int q = (c0 + u) * 256 + lane + 64 * k;
+                    v[a][u][k] = (q < n) ? xa[a * stride + q] : 0.0;
+                }
+#pragma unroll
+        for (int a = 0; a < NS; ++a)
+#pragma unroll
+            for (int u = 0; u < DR_BATCH; ++u)
+                if (c0 + u < chunks) {   // wavefront-uniform
+                    double x = chunk_tree(v[a][u][0], v[a][u][1], v[a][u][2], v[a][u][3]);
+                    if (lane == 0) part[a * (BLK + 1) + c0 + u] = x;
+                }
+        if constexpr (MX) {
+#pragma unroll
+            for (int u = 0; u < DR_BATCH; ++u)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) mx = fmax(mx, v[NS][u][k]);
+        }
+    }
+    if constexpr (MX) {
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+        if (lane == 0) L[w] = mx;
+    }
+    __syncthreads();
+    if (chunks > 1 && w < NS) {   // wavefront a: the top tree of array a
+        double* pa = part + w * (BLK + 1);
+        double x = chunk_tree(lane < chunks ? pa[lane] : 0.0, lane + 64 < chunks ? pa[lane + 64] : 0.0,
+                              lane + 128 < chunks ? pa[lane + 128] : 0.0, lane + 192 < chunks ? pa[lane + 192] : 0.0);
+        if (lane == 0) pa[BLK] = x;
+    }
+    static_assert(NS <= BLK / 64, "one wavefront per top tree");
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < NS; ++a)
+        res[a] = (chunks > 1) ? part[a * (BLK + 1) + BLK] : (chunks == 1 ? part[a * (BLK + 1)] : 0.0);
+    if constexpr (MX) {
+        double r = 0.0;
+        for (int q = 0; q < BLK / 64; ++q) r = fmax(r, L[q]);
+        res[NS] = r;
+    }
+    __syncthreads();
+}
+
 // per-packing reductions over polygons: nArrays arrays of N values (stride N) are summed
 // with det_sum_block; the last nMax of them are max-reduced instead.  Block per packing.
 // With stickyScal, block 0 also records whether the evaluation overflowed a record bucket.
