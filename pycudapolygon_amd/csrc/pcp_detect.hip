@@ -43,9 +43,11 @@ template <int SEG>
 __global__ __launch_bounds__(BLK) void k_bounds(int N, int P, const int* __restrict__ polyStart,
                                                const double2* __restrict__ pos, double4* __restrict__ bounds,
                                                int* __restrict__ num, int* __restrict__ bcnt, int* __restrict__ jcnt,
-                                               unsigned long long* __restrict__ packMaxKey, int* __restrict__ scal) {
+                                               unsigned long long* __restrict__ packMaxKey, long long pmkPitch,
+                                               int* __restrict__ scal) {
     if (blockIdx.x == 0 && threadIdx.x < SC_NSCAL) scal[threadIdx.x] = 0;
-    for (int q = blockIdx.x * BLK + threadIdx.x; q < P; q += gridDim.x * BLK) packMaxKey[q] = 0ull;
+    for (int q = blockIdx.x * BLK + threadIdx.x; q < PMK_SLOTS * P; q += gridDim.x * BLK)
+        packMaxKey[(q / P) * pmkPitch + q % P] = 0ull;
     int sub = threadIdx.x & (SEG - 1);
     int s = (blockIdx.x * BLK + threadIdx.x) / SEG;
     int st = 0, n = 0;
@@ -375,6 +377,11 @@ constexpr int TILE_UNROLL = 4;
 // vertices and polygons of one batch of candidates in a wavefront's LDS
 constexpr int DET_BATCH = 256;
 constexpr int DET_KMAX = 16;
+// pairs that pass the prefilter are collected (up to DET_MAY) and tested exactly 64 at a
+// time, so a polygon pays the exact test's latency (divisions, the record's atomics) once
+// or twice rather than once per pass that found any; pair ids of a batch are below
+// 64 * DET_BATCH = 2^14
+constexpr int DET_MAY = 128;
 
 // A wavefront's LDS: A's edges (chunk of 64), the vertices of a batch of candidate
 // polygons with the lane that loaded each, and the culled edge lists of the batch
@@ -389,6 +396,7 @@ struct DetLds {
     uint8_t klane[DET_KMAX];      // lane of the batch's k-th candidate
     int4 kinfo[DET_KMAX];         // per candidate: first pair, near-edge count, first vertex in V, vertices
     int2 kpoly[DET_KMAX];         // per candidate: polygon id, first vertex in pos
+    uint16_t may[DET_MAY];        // batch pairs that passed the prefilter, awaiting the exact test
 };
 
 // Where the lanes take the vertices of polygon B from in edge_tile: global memory, or (A
@@ -510,7 +518,7 @@ __device__ void detect_polygon(
     const int* __restrict__ packPoly, const int* __restrict__ box, const double2* __restrict__ pos,
     const int* __restrict__ cellOf, const double4* __restrict__ bounds, const int* __restrict__ pcStart,
     const int* __restrict__ pcSorted, const double4* __restrict__ sBounds, const int* __restrict__ packB,
-    const DetOut& out, unsigned long long* __restrict__ packMaxKey, int* __restrict__ num) {
+    const DetOut& out, unsigned long long* __restrict__ packMaxKey, long long pmkPitch, int* __restrict__ num) {
     int stA = polyStart[A], nA = polyStart[A + 1] - stA;
     int pk = polyPack[A];
     int Bu = box[pk];
@@ -750,23 +758,42 @@ __device__ void detect_polygon(
                     emit_record(key, tu, s0, out);
                 }
             };
+            // the collected pairs, 64 per exact pass (their tests recomputed from the ids)
+            int nmay = 0;
+            auto flush = [&]() {
+                wave_lds_sync();
+                for (int f0 = 0; f0 < nmay; f0 += 64) {
+                    PairT t = pair_at(f0 + lane < nmay ? (int)L.may[f0 + lane] : Tp);
+                    exact_pass(t);
+                }
+                wave_lds_sync();   // the list's reads are issued before it is refilled
+                nmay = 0;
+            };
             // two pairs per lane per pass: two independent chains in flight
             for (int p0 = 0; p0 < Tp; p0 += 128) {
                 PairT t0 = pair_at(p0 + lane), t1 = pair_at(p0 + 64 + lane);
                 unsigned long long m0 = __ballot(t0.may), m1 = __ballot(t1.may);
-                if (m0) exact_pass(t0);   // wavefront-uniform branches: most passes skip both
-                if (m1) exact_pass(t1);
+                if (m0 | m1) {   // wavefront-uniform: most passes find nothing
+                    int c0 = __popcll(m0), c1 = __popcll(m1);
+                    if (nmay + c0 + c1 > DET_MAY) flush();
+                    unsigned long long below = bits_below(lane);
+                    if (t0.may) L.may[nmay + __popcll(m0 & below)] = (uint16_t)(p0 + lane);
+                    if (t1.may) L.may[nmay + c0 + __popcll(m1 & below)] = (uint16_t)(p0 + 64 + lane);
+                    nmay += c0 + c1;
+                }
             }
+            if (nmay) flush();
         }
     }
     // the largest key of the packing (the pairing's last-record case)
-    unsigned long long pmk = packMaxKey[pk];
+    unsigned long long* pmkSlot = packMaxKey + (A & (PMK_SLOTS - 1)) * pmkPitch + pk;
+    unsigned long long pmk = *pmkSlot;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
         unsigned long long y = __shfl_xor((unsigned long long)kmax, o);
         kmax = (y > kmax) ? y : kmax;
     }
-    if (lane == 0 && kmax > 0 && kmax > pmk) atomicMax(&packMaxKey[pk], (unsigned long long)kmax);
+    if (lane == 0 && kmax > 0 && kmax > pmk) atomicMax(pmkSlot, (unsigned long long)kmax);
 }
 
 // One wavefront per polygon A (in cell order, XCD-contiguous): detect_polygon.
@@ -782,7 +809,8 @@ __global__ __launch_bounds__(BLK) void k_detect_poly(
     const int* __restrict__ cellOf, const double4* __restrict__ bounds, const int* __restrict__ pcStart,
     const int* __restrict__ pcSorted, const double4* __restrict__ sBounds, const int* __restrict__ packB, int Kp,
     int* __restrict__ bcnt, uint64_t* __restrict__ bKey, float2* __restrict__ bTU, int* __restrict__ jcnt,
-    uint64_t* __restrict__ jKey, unsigned long long* __restrict__ packMaxKey, int* __restrict__ num,
+    uint64_t* __restrict__ jKey, unsigned long long* __restrict__ packMaxKey, long long pmkPitch,
+    int* __restrict__ num,
     int* __restrict__ scal) {
     __shared__ DetLds lds[BLK / 64];
     int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -790,7 +818,8 @@ __global__ __launch_bounds__(BLK) void k_detect_poly(
     if (idx >= N) return;   // whole wavefront
     DetOut out{Kp, bcnt, bKey, bTU, jcnt, jKey, scal};
     detect_polygon<CELLS>(__builtin_amdgcn_readfirstlane(pcSorted[idx]), lane, lds[w], polyStart, polyPack, packPoly,
-                          box, pos, cellOf, bounds, pcStart, pcSorted, sBounds, packB, out, packMaxKey, num);
+                          box, pos, cellOf, bounds, pcStart, pcSorted, sBounds, packB, out, packMaxKey, pmkPitch,
+                          num);
 }
 
 // exact maximum of the per-edge crossing counts (updateNeighbors' overflow check)
@@ -819,11 +848,11 @@ int launch_bounds(pcp_ctx* c, const double2* pos) {
     int grid = (c->N + polysPerBlock - 1) / polysPerBlock;
     if (grid < 1) grid = 1;
     switch (seg) {
-    case 4: PCP_LAUNCH(c, k_bounds<4>, dim3(grid), dim3(BLK), c->N, c->P, c->dPolyStart, pos, c->dBounds, c->dNum, c->dBcnt, c->dJcnt, c->dPackMaxKey, c->dScal); break;
-    case 8: PCP_LAUNCH(c, k_bounds<8>, dim3(grid), dim3(BLK), c->N, c->P, c->dPolyStart, pos, c->dBounds, c->dNum, c->dBcnt, c->dJcnt, c->dPackMaxKey, c->dScal); break;
-    case 16: PCP_LAUNCH(c, k_bounds<16>, dim3(grid), dim3(BLK), c->N, c->P, c->dPolyStart, pos, c->dBounds, c->dNum, c->dBcnt, c->dJcnt, c->dPackMaxKey, c->dScal); break;
-    case 32: PCP_LAUNCH(c, k_bounds<32>, dim3(grid), dim3(BLK), c->N, c->P, c->dPolyStart, pos, c->dBounds, c->dNum, c->dBcnt, c->dJcnt, c->dPackMaxKey, c->dScal); break;
-    default: PCP_LAUNCH(c, k_bounds<64>, dim3(grid), dim3(BLK), c->N, c->P, c->dPolyStart, pos, c->dBounds, c->dNum, c->dBcnt, c->dJcnt, c->dPackMaxKey, c->dScal); break;
+    case 4: PCP_LAUNCH(c, k_bounds<4>, dim3(grid), dim3(BLK), c->N, c->P, c->dPolyStart, pos, c->dBounds, c->dNum, c->dBcnt, c->dJcnt, c->dPackMaxKey, c->pmkPitch, c->dScal); break;
+    case 8: PCP_LAUNCH(c, k_bounds<8>, dim3(grid), dim3(BLK), c->N, c->P, c->dPolyStart, pos, c->dBounds, c->dNum, c->dBcnt, c->dJcnt, c->dPackMaxKey, c->pmkPitch, c->dScal); break;
+    case 16: PCP_LAUNCH(c, k_bounds<16>, dim3(grid), dim3(BLK), c->N, c->P, c->dPolyStart, pos, c->dBounds, c->dNum, c->dBcnt, c->dJcnt, c->dPackMaxKey, c->pmkPitch, c->dScal); break;
+    case 32: PCP_LAUNCH(c, k_bounds<32>, dim3(grid), dim3(BLK), c->N, c->P, c->dPolyStart, pos, c->dBounds, c->dNum, c->dBcnt, c->dJcnt, c->dPackMaxKey, c->pmkPitch, c->dScal); break;
+    default: PCP_LAUNCH(c, k_bounds<64>, dim3(grid), dim3(BLK), c->N, c->P, c->dPolyStart, pos, c->dBounds, c->dNum, c->dBcnt, c->dJcnt, c->dPackMaxKey, c->pmkPitch, c->dScal); break;
     }
     return PCP_OK;
 }
@@ -840,12 +869,12 @@ int run_detect(pcp_ctx* c, const double2* pos, int cellsFromList) {
         PCP_LAUNCH(c, k_detect_poly<1>, dim3(grid), dim3(BLK), c->N, c->dPolyStart, c->dPolyPack, c->dPackPoly,
                    c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted, c->dSBounds, c->dPackB, c->Kp,
                    c->dBcnt,
-                   c->dBKey, c->dBTU, c->dJcnt, c->dJKey, c->dPackMaxKey, c->dNum, c->dScal);
+                   c->dBKey, c->dBTU, c->dJcnt, c->dJKey, c->dPackMaxKey, c->pmkPitch, c->dNum, c->dScal);
     else
         PCP_LAUNCH(c, k_detect_poly<0>, dim3(grid), dim3(BLK), c->N, c->dPolyStart, c->dPolyPack, c->dPackPoly,
                    c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted, c->dSBounds, c->dPackB, c->Kp,
                    c->dBcnt,
-                   c->dBKey, c->dBTU, c->dJcnt, c->dJKey, c->dPackMaxKey, c->dNum, c->dScal);
+                   c->dBKey, c->dBTU, c->dJcnt, c->dJKey, c->dPackMaxKey, c->pmkPitch, c->dNum, c->dScal);
     return PCP_OK;
 }
 

@@ -29,6 +29,14 @@ int fail_hip(hipError_t e, const char* what, const char* file, int line);
     } while (0)
 
 constexpr int BLK = 256;
+// The packing's largest record key is kept as PMK_SLOTS partial maxima, each polygon's
+// detection updating slot (polygon & (PMK_SLOTS - 1)): a single counter updated by one
+// atomic per polygon queues thousands of atomics on one address, and the memory channel
+// that holds it then stalls every other access for microseconds.  Slot rows are
+// 4 KiB + 256 B apart (and at least P entries long), so the slots fall on different
+// channels whether addresses interleave at 256 B or at 4 KiB.
+constexpr int PMK_SLOTS = 16;
+inline long long pmk_pitch(int P) { return ((long long)(P > 1 ? P : 1) + 511) / 512 * 512 + 32; }
 inline int nblk(long long n, int b = BLK) { return (int)((n + b - 1) / b); }
 
 // scalar slots in ctx->dScal
@@ -83,7 +91,10 @@ struct pcp_ctx {
     const double2* lastDetectPos = nullptr;   // positions of the last detection (statistics)
     int* dSticky = nullptr;        // [0] a fused evaluation overflowed a buffer
     int* dNum = nullptr;           // V: crossings per edge (updateNeighbors' count)
-    unsigned long long* dPackMaxKey = nullptr;   // P: largest record key of each packing
+    // largest record key of each packing, as the maximum of PMK_SLOTS partial maxima (slot k
+    // of packing p at k * pmkPitch + p)
+    unsigned long long* dPackMaxKey = nullptr;
+    long long pmkPitch = 0;
     int* dScal = nullptr;
 
     // records, per polygon s at rows [s*Kp, s*Kp + count)

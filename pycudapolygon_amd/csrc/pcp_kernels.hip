@@ -1198,7 +1198,8 @@ int pcp_set_topology(pcp_ctx* c, int P, const int* packPolyStart, const int* pol
     PCP_TRY(dalloc(&c->dCellOf, V)); PCP_TRY(dalloc(&c->dCellTmp, V)); PCP_TRY(dalloc(&c->dCellSorted, V));
     PCP_TRY(dalloc(&c->dSP, V)); PCP_TRY(dalloc(&c->dSR, V)); PCP_TRY(dalloc(&c->dSI, V));
     PCP_TRY(dalloc(&c->dNum, V));
-    PCP_TRY(dalloc(&c->dPackMaxKey, P)); PCP_TRY(dalloc(&c->dScal, SC_NSCAL));
+    c->pmkPitch = pmk_pitch(P);
+    PCP_TRY(dalloc(&c->dPackMaxKey, PMK_SLOTS * c->pmkPitch)); PCP_TRY(dalloc(&c->dScal, SC_NSCAL));
     PCP_HIP(hipMemsetAsync(c->dScal, 0, sizeof(int) * SC_NSCAL, c->stream));
     PCP_HIP(hipMemsetAsync(c->dNum, 0, sizeof(int) * (size_t)std::max(V, 1), c->stream));
     PCP_TRY(dalloc(&c->dBcnt, N)); PCP_TRY(dalloc(&c->dJcnt, N));

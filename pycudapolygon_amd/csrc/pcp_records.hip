@@ -53,20 +53,29 @@ __global__ __launch_bounds__(64) void k_pair_poly(int N, int Kp, const int* __re
                                                  const int* __restrict__ polyPack, const int* __restrict__ packPoly,
                                                  const int* __restrict__ bcnt, const uint64_t* __restrict__ bKey,
                                                  const float2* __restrict__ bTU,
-                                                 const unsigned long long* __restrict__ packMaxKey,
+                                                 const unsigned long long* __restrict__ packMaxKey, long long pmkPitch,
                                                  const double2* __restrict__ pairPos, uint64_t* __restrict__ rKey,
                                                  float2* __restrict__ rTU, uint64_t* __restrict__ rOuter,
                                                  float2* __restrict__ rUT, const int* __restrict__ order) {
     int lane = threadIdx.x & (SEGX - 1);
     int slot = xcd_block(blockIdx.x, gridDim.x) * (64 / SEGX) + threadIdx.x / SEGX;
-    if (slot >= N) return;   // no barriers or shuffles below
+    if (slot >= N) return;   // whole segments: the shuffles below stay inside a segment
     int s = order[slot];
     int cnt = min(bcnt[s], Kp);
     if (cnt == 0) return;
     int pk = polyPack[s];
     int pbase = packPoly[pk];
     int st1 = polyStart[s], ni = polyStart[s + 1] - st1;
-    uint64_t mxk = packMaxKey[pk];
+    uint64_t mxk = 0;   // the maximum of the packing's partial maxima, across the segment
+    for (int k = lane; k < PMK_SLOTS; k += SEGX) {
+        uint64_t v = packMaxKey[k * pmkPitch + pk];
+        mxk = (v > mxk) ? v : mxk;
+    }
+#pragma unroll
+    for (int o = SEGX / 2; o >= 1; o >>= 1) {
+        uint64_t y = __shfl_xor(mxk, o, SEGX);
+        mxk = (y > mxk) ? y : mxk;
+    }
     long long base = (long long)s * Kp;
     for (int f = lane; f < cnt; f += SEGX) {
         uint64_t key = bKey[base + f];
@@ -646,7 +655,7 @@ static int launch_pair(pcp_ctx* c, const double2* pairPos) {
     int slot = -1;
     PCP_TRY(prof_begin(c, EXACT ? "k_pair_poly<1>" : "k_pair_poly<0>", &slot));
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_poly<EXACT, 16>), dim3((c->N + 3) / 4), dim3(64), 0, c->stream, c->N,
-                       c->Kp, c->dPolyStart, c->dPolyPack, c->dPackPoly, c->dBcnt, c->dBKey, c->dBTU, c->dPackMaxKey,
+                       c->Kp, c->dPolyStart, c->dPolyPack, c->dPackPoly, c->dBcnt, c->dBKey, c->dBTU, c->dPackMaxKey, c->pmkPitch,
                        pairPos, c->dRKey, c->dRTU, c->dROuter, c->dRUT, c->dPcSorted);
     PCP_LAUNCHED();
     return prof_end(c, slot);
