@@ -232,6 +232,63 @@ int pcp_stream_copy(pcp_ctx* ctx, double* dst, const double* src, long long n);
  * not apply).  The last detection's position buffer must still be valid. */
 int pcp_detect_stats(pcp_ctx* ctx, long long* polygonPairs, long long* edgePairs);
 
+/* ---- learned warm start (no counterpart in the reference, which has no learned
+ *      component: an addition in front of the plain FIRE method) ----
+ * A message-passing network over the contact graph of the packings of a context.
+ * Nodes are vertices; vertex m is linked to prev(m), next(m) and to the start vertex of
+ * every edge crossing edge m (from the crossing records of the last evaluation).
+ *   h0 = relu(x W0 + b0)                                          x: 4 features / vertex
+ *   h_l = h_{l-1} + relu([h_l-1, sum over neighbours of h_l-1] W_l + b_l),  l = 1..3
+ *   y = h3 Wo + bo                                                2 outputs / vertex
+ * Parameters: float32 vector of PCP_MPNN_NPARAM entries (offsets below, matrices
+ * row-major [in][out]).  Products take bf16-rounded operands and accumulate in float32
+ * (the 128x64 layers on the matrix cores). */
+#define PCP_MPNN_H 64
+#define PCP_MPNN_DEG 16
+#define PCP_MPNN_OFF_W0 0          /* W0[4][64]   */
+#define PCP_MPNN_OFF_B0 256        /* b0[64]      */
+#define PCP_MPNN_EMBED_NP 320
+#define PCP_MPNN_OFF_W1 320        /* layer l: W_l[128][64] then b_l[64] at OFF_W1 + (l-1)*LAYER_NP */
+#define PCP_MPNN_LAYER_NP 8256
+#define PCP_MPNN_OFF_WO 25088      /* Wo[64][2]   */
+#define PCP_MPNN_OFF_BO 25216      /* bo[2]       */
+#define PCP_MPNN_OUT_NP 130
+#define PCP_MPNN_NPARAM 25218
+/* floats of the forward work buffer (activations kept for the backward pass) and of the
+ * backward work buffer, for V vertices */
+int pcp_mpnn_sizes(int V, long long* workFloats, long long* gradWorkFloats);
+/* contact graph of the context's last evaluation: adj device[V*PCP_MPNN_DEG] (global
+ * vertex ids: prev, next, crossing partners ascending, -1 empty); dropped host[1]
+ * (optional; synchronizes): partners beyond the width */
+int pcp_mpnn_graph(pcp_ctx* ctx, int* adj, int* dropped);
+/* input features of every vertex after an evaluation of st (force, per-polygon energy,
+ * crossings counted in adj): feat device[V*4] =
+ *   asinh(Fx/(ell^(e-1) s0)), asinh(Fy/(ell^(e-1) s1)), crossings/s2, asinh(E_poly/(n ell^e s3))
+ * ell device[P] = each packing's length unit, e = energyPow, s = inScale host[4] */
+int pcp_mpnn_features(pcp_ctx* ctx, const pcp_state* st, const int* adj, const double* ell, int energyPow,
+                      const float* inScale, float* feat);
+/* forward pass over V vertices: params/feat/adj/work/out device; out[V*2] */
+int pcp_mpnn_forward(pcp_ctx* ctx, int V, const float* params, const float* feat, const int* adj, float* work,
+                     float* out);
+/* mean squared error L = (1/V) sum |out - target|^2 and dOut = dL/dout (device[V*2]);
+ * loss host[1] optional (synchronizes) */
+int pcp_mpnn_loss(pcp_ctx* ctx, int V, const float* out, const float* target, float* dOut, float* gradWork,
+                  double* loss);
+/* backward pass of the last forward (same work buffer): grad device[PCP_MPNN_NPARAM] =
+ * dL/dparams for the output gradient dOut (overwritten, not accumulated) */
+int pcp_mpnn_backward(pcp_ctx* ctx, int V, const float* params, const float* feat, const int* adj,
+                      const float* work, const float* dOut, float* gradWork, float* grad);
+/* one Adam step (bias-corrected, step counts from 1) on params with moments m, v */
+int pcp_mpnn_adam(pcp_ctx* ctx, float* params, const float* grad, float* m, float* v, int step, float lr,
+                  float beta1, float beta2, float eps);
+/* training targets: target device[V*2] = minimum image (xTo - xFrom) / (ell * outScale) */
+int pcp_mpnn_targets(pcp_ctx* ctx, const double* xFrom, const double* xTo, const double* ell, double outScale,
+                     float* target);
+/* warm start: pos <- frac(pos + ell * outScale * out) for the packings with active != 0
+ * (active device[P] or NULL = all) */
+int pcp_mpnn_apply(pcp_ctx* ctx, double* pos, const float* out, const double* ell, double outScale,
+                   const int* active);
+
 #ifdef __cplusplus
 }
 #endif

@@ -83,6 +83,16 @@ SIGNATURES = {
     "pcp_profile_read": (_i, [_p, _p, _p]),
     "pcp_stream_copy": (_i, [_p, _p, _p, ctypes.c_longlong]),
     "pcp_detect_stats": (_i, [_p, _p, _p]),
+    "pcp_mpnn_sizes": (_i, [_i, _p, _p]),
+    "pcp_mpnn_graph": (_i, [_p, _p, _p]),
+    "pcp_mpnn_features": (_i, [_p, ctypes.POINTER(PcpState), _p, _p, _i, _p, _p]),
+    "pcp_mpnn_forward": (_i, [_p, _i, _p, _p, _p, _p, _p]),
+    "pcp_mpnn_loss": (_i, [_p, _i, _p, _p, _p, _p, _p]),
+    "pcp_mpnn_backward": (_i, [_p, _i, _p, _p, _p, _p, _p, _p, _p]),
+    "pcp_mpnn_adam": (_i, [_p, _p, _p, _p, _p, _i, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                           ctypes.c_float]),
+    "pcp_mpnn_targets": (_i, [_p, _p, _p, _p, _d, _p]),
+    "pcp_mpnn_apply": (_i, [_p, _p, _p, _p, _d, _p]),
 }
 
 _lib = None

@@ -42,7 +42,8 @@ inline int nblk(long long n, int b = BLK) { return (int)((n + b - 1) / b); }
 // scalar slots in ctx->dScal
 // SC_MAXCOUNT: largest per-edge crossing count (updateNeighbors); SC_BUCKET: largest
 // bucket count, set only when a bucket overflowed
-enum { SC_MAXCOUNT = 0, SC_BUCKET = 1, SC_NSCAL = 8 };
+// SC_MPNN_DROP: adjacency entries dropped by the last contact-graph build
+enum { SC_MAXCOUNT = 0, SC_BUCKET = 1, SC_MPNN_DROP = 7, SC_NSCAL = 8 };
 
 template <class T> int dalloc(T** p, size_t count) {
     *p = nullptr;

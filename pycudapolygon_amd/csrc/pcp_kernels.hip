@@ -1605,3 +1605,4 @@ int pcp_force_eval(pcp_ctx* c, const pcp_state* st) {
 }  // extern "C"
 
 #include "pcp_fire.hip"
+#include "pcp_mpnn.hip"

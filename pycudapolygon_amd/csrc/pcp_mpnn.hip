@@ -1,0 +1,741 @@
+// Learned warm start: a message-passing network over the contact graph of a packing.
+// No counterpart in the reference (camdennis/pyCudaPolygon has no learned component);
+// it sits in front of the plain method: its displacement is applied once, the polygons
+// are projected back onto their shapes (SHAKE), and FP64 FIRE finishes the minimization
+// and judges the residual force.
+//
+// Graph.  Nodes are vertices.  Vertex m (start of edge m) is linked to its polygon
+// neighbours prev(m) and next(m), and to the start vertex of every edge that crosses
+// edge m: each crossing record (si, sj, i, j) links vertex i of si with vertex j of sj,
+// found from the si bucket of si (key_i == m) and the sj bucket of sj (key_j == m).
+// Every crossing is one record, so the adjacency is symmetric.  It is stored with a
+// fixed width of PCP_MPNN_DEG slots: prev, next, then the crossing partners in
+// ascending global vertex id, -1 for empty slots (partners beyond the width are
+// dropped and counted).
+//
+// Network (H = 64, parameters float32, layout PCP_MPNN_OFF_* in pcp.h):
+//   h0  = relu(x W0 + b0)                                       x: 4 features
+//   h_l = h_{l-1} + relu([h_{l-1}, sum_{u in N(m)} h_{l-1}(u)] W_l + b_l),  l = 1, 2, 3
+//   y   = h3 Wo + bo                                            2 outputs per vertex
+// Every product takes bf16 operands (weights and activations rounded to nearest even
+// from float32) and accumulates in float32.  The 128x64 layers -- forward, input
+// gradient and weight gradient -- run on the matrix cores (v_mfma_f32_16x16x16_bf16,
+// one wavefront per 16 vertices); the 4x64 and 64x2 layers run on VALU in a fixed
+// order.  Gradients treat the bf16 rounding as the identity (straight-through).
+// Reductions over vertices are partial sums per chunk of MPNN_CH vertices added in
+// chunk order, so training is bit-reproducible run to run.
+//
+// This file is compiled as part of pcp_kernels.hip (one translation unit).
+
+namespace pcp {
+
+typedef short mpnn_s4 __attribute__((ext_vector_type(4)));
+typedef float mpnn_f4 __attribute__((ext_vector_type(4)));
+
+constexpr int MH = PCP_MPNN_H;           // hidden width
+constexpr int MD = PCP_MPNN_DEG;         // adjacency slots per vertex
+constexpr int MPNN_CH = 2048;            // vertices per partial-sum chunk (weight gradients)
+constexpr int WPAD = 4;                  // LDS row padding (bf16 elements) against bank conflicts
+
+__device__ __forceinline__ unsigned short bf16_rne(float f) {
+    unsigned int u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_f(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
+__device__ __forceinline__ float bfr(float f) { return bf16_f(bf16_rne(f)); }
+
+__device__ __forceinline__ mpnn_s4 pack_bf16(float a, float b, float c, float d) {
+    mpnn_s4 r;
+    r.x = (short)bf16_rne(a);
+    r.y = (short)bf16_rne(b);
+    r.z = (short)bf16_rne(c);
+    r.w = (short)bf16_rne(d);
+    return r;
+}
+
+// ---- contact graph -------------------------------------------------------------------
+__global__ void k_mpnn_graph(int V, int Kp, const int* __restrict__ polyStart, const int* __restrict__ shapeId,
+                             const int* __restrict__ polyPack, const int* __restrict__ packPoly,
+                             const int* __restrict__ bcnt, const uint64_t* __restrict__ bkey,
+                             const int* __restrict__ jcnt, const uint64_t* __restrict__ jkey,
+                             int* __restrict__ adj, int* __restrict__ dropped) {
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= V) return;
+    int s = shapeId[m];
+    int v0 = polyStart[s], n = polyStart[s + 1] - v0;
+    int a = m - v0;
+    int pbase = packPoly[polyPack[s]];
+    int part[MD - 2];
+    int np = 0, drop = 0;
+    auto insert = [&](int u) {
+        if (np == MD - 2) {   // keep the MD-2 smallest ids
+            if (u >= part[MD - 3]) { ++drop; return; }
+            --np;
+            ++drop;
+        }
+        int k = np;
+        while (k > 0 && part[k - 1] > u) {
+            part[k] = part[k - 1];
+            --k;
+        }
+        part[k] = u;
+        ++np;
+    };
+    int nb = min(bcnt[s], Kp);
+    for (int r = 0; r < nb; ++r) {
+        uint64_t k = bkey[(long long)s * Kp + r];
+        if (key_i(k) == a) insert(polyStart[pbase + key_sj(k)] + key_j(k));
+    }
+    int nj = min(jcnt[s], Kp);
+    for (int r = 0; r < nj; ++r) {
+        uint64_t k = jkey[(long long)s * Kp + r];
+        if (key_j(k) == a) insert(polyStart[pbase + key_si(k)] + key_i(k));
+    }
+    int* row = adj + (long long)m * MD;
+    row[0] = v0 + (a + n - 1) % n;
+    row[1] = v0 + (a + 1) % n;
+    for (int k = 0; k < MD - 2; ++k) row[2 + k] = (k < np) ? part[k] : -1;
+    if (drop) atomicAdd(dropped, drop);
+}
+
+// ---- input features --------------------------------------------------------------------
+// x = (asinh(Fx / (ell^(e-1) s0)), asinh(Fy / (ell^(e-1) s1)), crossings(m) / s2,
+//      asinh(E_poly / (n ell^e s3))),  ell = the packing's length unit, e = energyPow
+__global__ void k_mpnn_features(int V, const int* __restrict__ shapeId, const int* __restrict__ polyStart,
+                                const int* __restrict__ polyPack, const double2* __restrict__ F,
+                                const double* __restrict__ epoly, const int* __restrict__ adj,
+                                const double* __restrict__ ell, int energyPow, float s0, float s1, float s2,
+                                float s3, float4* __restrict__ feat) {
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= V) return;
+    int s = shapeId[m];
+    double L = ell[polyPack[s]];
+    double Lf = 1.0, Le;
+    for (int k = 1; k < energyPow; ++k) Lf *= L;
+    Le = Lf * L;
+    int n = polyStart[s + 1] - polyStart[s];
+    const int* row = adj + (long long)m * MD;
+    int nc = 0;
+    for (int k = 2; k < MD; ++k) nc += (row[k] >= 0);
+    double2 f = F[m];
+    float4 o;
+    o.x = (float)asinh(f.x / (Lf * (double)s0));
+    o.y = (float)asinh(f.y / (Lf * (double)s1));
+    o.z = (float)nc / s2;
+    o.w = (float)asinh(epoly[s] / ((double)n * Le * (double)s3));
+    feat[m] = o;
+}
+
+// ---- forward ---------------------------------------------------------------------------
+// h0 = relu(bf(x) bf(W0) + b0), thread per (vertex, channel), f = 0..3 in order
+__global__ void k_mpnn_embed(int V, const float* __restrict__ prm, const float4* __restrict__ feat,
+                             float* __restrict__ H0, float* __restrict__ Z0) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)V * MH) return;
+    int m = (int)(t / MH), c = (int)(t % MH);
+    const float* W0 = prm + PCP_MPNN_OFF_W0;
+    float4 x = feat[m];
+    float z = bfr(x.x) * bfr(W0[0 * MH + c]);
+    z = z + bfr(x.y) * bfr(W0[1 * MH + c]);
+    z = z + bfr(x.z) * bfr(W0[2 * MH + c]);
+    z = z + bfr(x.w) * bfr(W0[3 * MH + c]);
+    z = z + prm[PCP_MPNN_OFF_B0 + c];
+    Z0[t] = z;
+    H0[t] = fmaxf(z, 0.0f);
+}
+
+// One message-passing layer.  Wavefront w of the workgroup takes vertices
+// [16 (4 b + w), +16); lane (r = lane % 16, g = lane / 16) owns columns 16 s + 4 g .. +3
+// (s = 0..3) of vertex r for the A operand: its own h and the sum of its neighbours' h
+// (slots in order).  X = bf16([h, agg]) is stored for the weight gradient.
+__global__ __launch_bounds__(256) void k_mpnn_layer(int V, const float* __restrict__ W, const float* __restrict__ b,
+                                                    const float* __restrict__ Hin, const int* __restrict__ adj,
+                                                    float* __restrict__ Hout, float* __restrict__ Z,
+                                                    unsigned short* __restrict__ X) {
+    // W^T in LDS as bf16: sWt[n][k] (n = output column, k = input row), rows padded
+    __shared__ unsigned short sWt[MH * (2 * MH + WPAD)];
+    for (int e = threadIdx.x; e < 2 * MH * MH; e += blockDim.x) {
+        int k = e / MH, n = e % MH;
+        sWt[n * (2 * MH + WPAD) + k] = bf16_rne(W[e]);
+    }
+    __syncthreads();
+    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int r = lane & 15, g = lane >> 4;
+    int row0 = (blockIdx.x * 4 + wave) * 16;
+    if (row0 >= V) return;
+    int row = row0 + r;
+    bool ok = row < V;
+    float4 h[4], a[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        h[s] = ok ? *(const float4*)(Hin + (long long)row * MH + 16 * s + 4 * g) : make_float4(0.f, 0.f, 0.f, 0.f);
+        a[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    if (ok) {
+        const int4* ar = (const int4*)(adj + (long long)row * MD);
+        int nb[MD];
+#pragma unroll
+        for (int q = 0; q < MD / 4; ++q) {
+            int4 t = ar[q];
+            nb[4 * q] = t.x; nb[4 * q + 1] = t.y; nb[4 * q + 2] = t.z; nb[4 * q + 3] = t.w;
+        }
+#pragma unroll
+        for (int d = 0; d < MD; ++d) {
+            int u = nb[d];
+            if (u >= 0) {
+                const float* hu = Hin + (long long)u * MH + 4 * g;
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    float4 v = *(const float4*)(hu + 16 * s);
+                    a[s].x = a[s].x + v.x; a[s].y = a[s].y + v.y; a[s].z = a[s].z + v.z; a[s].w = a[s].w + v.w;
+                }
+            }
+        }
+    }
+    mpnn_s4 A[8];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        A[s] = pack_bf16(h[s].x, h[s].y, h[s].z, h[s].w);
+        A[4 + s] = pack_bf16(a[s].x, a[s].y, a[s].z, a[s].w);
+    }
+    if (ok) {
+        unsigned short* xr = X + (long long)row * (2 * MH) + 4 * g;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) *(mpnn_s4*)(xr + 16 * s) = A[s];
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        mpnn_f4 acc = {0.f, 0.f, 0.f, 0.f};
+        const unsigned short* wr = sWt + (16 * t + r) * (2 * MH + WPAD) + 4 * g;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            mpnn_s4 B = *(const mpnn_s4*)(wr + 16 * s);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[s], B, acc, 0, 0, 0);
+        }
+        int col = 16 * t + r;
+        float bias = b[col];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            int ro = row0 + 4 * g + i;
+            if (ro < V) {
+                float z = acc[i] + bias;
+                long long o = (long long)ro * MH + col;
+                Z[o] = z;
+                Hout[o] = Hin[o] + fmaxf(z, 0.0f);
+            }
+        }
+    }
+}
+
+// y = bf(h3) bf(Wo) + bo, thread per vertex, channels in order
+__global__ void k_mpnn_out(int V, const float* __restrict__ prm, const float* __restrict__ H3,
+                           float2* __restrict__ out) {
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= V) return;
+    const float* Wo = prm + PCP_MPNN_OFF_WO;
+    const float* h = H3 + (long long)m * MH;
+    float y0 = 0.0f, y1 = 0.0f;
+    for (int c = 0; c < MH; ++c) {
+        float hc = bfr(h[c]);
+        y0 = y0 + hc * bfr(Wo[2 * c]);
+        y1 = y1 + hc * bfr(Wo[2 * c + 1]);
+    }
+    out[m] = make_float2(y0 + prm[PCP_MPNN_OFF_BO], y1 + prm[PCP_MPNN_OFF_BO + 1]);
+}
+
+// ---- loss ------------------------------------------------------------------------------
+// L = (1/V) sum_m |y_m - t_m|^2 ; dY = 2 (y - t) / V ; per-workgroup partial sums (double)
+__global__ __launch_bounds__(256) void k_mpnn_loss(int V, const float2* __restrict__ y, const float2* __restrict__ tg,
+                                                   float2* __restrict__ dy, double* __restrict__ part) {
+    __shared__ double sh[256];
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0;
+    if (m < V) {
+        float2 a = y[m], t = tg[m];
+        float dx = a.x - t.x, dz = a.y - t.y;
+        e = (double)dx * (double)dx + (double)dz * (double)dz;
+        float sc = 2.0f / (float)V;
+        dy[m] = make_float2(sc * dx, sc * dz);
+    }
+    sh[threadIdx.x] = e;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// out[e] = sum over chunks q (in order) of part[q * width + e]
+__global__ void k_mpnn_sum_chunks(int nChunks, int width, const float* __restrict__ part, float* __restrict__ out) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= width) return;
+    float s = 0.0f;
+    for (int q = 0; q < nChunks; ++q) s = s + part[(long long)q * width + e];
+    out[e] = s;
+}
+__global__ void k_mpnn_sum_loss(int n, const double* __restrict__ part, double scale, double* __restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0.0;
+    for (int q = 0; q < n; ++q) s = s + part[q];
+    out[0] = s * scale;
+}
+
+// ---- backward --------------------------------------------------------------------------
+// dh3 = dY bf(Wo)^T (thread per (vertex, channel))
+__global__ void k_mpnn_out_bwd(int V, const float* __restrict__ prm, const float2* __restrict__ dy,
+                               float* __restrict__ dH) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)V * MH) return;
+    int m = (int)(t / MH), c = (int)(t % MH);
+    const float* Wo = prm + PCP_MPNN_OFF_WO;
+    float2 g = dy[m];
+    dH[t] = g.x * bfr(Wo[2 * c]) + g.y * bfr(Wo[2 * c + 1]);
+}
+
+// partial gradients of Wo (64x2) and bo over a chunk of vertices: entry e < 128 is
+// Wo[e/2][e%2], e = 128, 129 is bo[e-128]
+__global__ __launch_bounds__(192) void k_mpnn_out_grad(int V, const float* __restrict__ H3, const float2* __restrict__ dy,
+                                                       float* __restrict__ part) {
+    int e = threadIdx.x;
+    if (e >= 2 * MH + 2) return;
+    int m0 = blockIdx.x * MPNN_CH, m1 = min(V, m0 + MPNN_CH);
+    float s = 0.0f;
+    if (e < 2 * MH) {
+        int c = e >> 1, o = e & 1;
+        for (int m = m0; m < m1; ++m) {
+            float2 g = dy[m];
+            s = s + bfr(H3[(long long)m * MH + c]) * (o ? g.y : g.x);
+        }
+    } else {
+        int o = e - 2 * MH;
+        for (int m = m0; m < m1; ++m) {
+            float2 g = dy[m];
+            s = s + (o ? g.y : g.x);
+        }
+    }
+    part[(long long)blockIdx.x * PCP_MPNN_OUT_NP + e] = s;
+}
+
+// Layer backward, input side.  dZ = dH ⊙ [Z > 0]; dX = bf(dZ) bf(W)^T (MFMA, K = 64);
+// dHself = dH + dX[:, :64] (residual + own row), dA = dX[:, 64:] (to be gathered).
+// Lane (r, g) holds dZ[row][16 s + 4 g .. +3], s = 0..3, as the A operand; the B operand
+// B[k][j] = W[j][k] is read from W row-major in LDS.
+__global__ __launch_bounds__(256) void k_mpnn_layer_bwd(int V, const float* __restrict__ W,
+                                                        const float* __restrict__ dH, const float* __restrict__ Z,
+                                                        float* __restrict__ dHself, float* __restrict__ dA) {
+    __shared__ unsigned short sW[2 * MH * (MH + WPAD)];   // W[j][k], j < 128 inputs, k < 64 outputs
+    for (int e = threadIdx.x; e < 2 * MH * MH; e += blockDim.x) {
+        int j = e / MH, k = e % MH;
+        sW[j * (MH + WPAD) + k] = bf16_rne(W[e]);
+    }
+    __syncthreads();
+    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int r = lane & 15, g = lane >> 4;
+    int row0 = (blockIdx.x * 4 + wave) * 16;
+    if (row0 >= V) return;
+    int row = row0 + r;
+    bool ok = row < V;
+    mpnn_s4 A[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        float4 d = make_float4(0.f, 0.f, 0.f, 0.f), z = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (ok) {
+            d = *(const float4*)(dH + (long long)row * MH + 16 * s + 4 * g);
+            z = *(const float4*)(Z + (long long)row * MH + 16 * s + 4 * g);
+        }
+        A[s] = pack_bf16(z.x > 0.f ? d.x : 0.f, z.y > 0.f ? d.y : 0.f, z.z > 0.f ? d.z : 0.f,
+                         z.w > 0.f ? d.w : 0.f);
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        mpnn_f4 acc = {0.f, 0.f, 0.f, 0.f};
+        const unsigned short* wr = sW + (16 * t + r) * (MH + WPAD) + 4 * g;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            mpnn_s4 B = *(const mpnn_s4*)(wr + 16 * s);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[s], B, acc, 0, 0, 0);
+        }
+        int col = 16 * t + r;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            int ro = row0 + 4 * g + i;
+            if (ro >= V) continue;
+            if (t < 4) {
+                long long o = (long long)ro * MH + col;
+                dHself[o] = dH[o] + acc[i];
+            } else {
+                dA[(long long)ro * MH + (col - MH)] = acc[i];
+            }
+        }
+    }
+}
+
+// dH_prev[m] = dHself[m] + sum over the slots of m (in order) of dA[u]: the transpose of
+// the neighbour sum, as the adjacency is symmetric
+__global__ void k_mpnn_gather_bwd(int V, const int* __restrict__ adj, const float* __restrict__ dHself,
+                                  const float* __restrict__ dA, float* __restrict__ dHprev) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)V * MH) return;
+    int m = (int)(t / MH), c = (int)(t % MH);
+    const int* row = adj + (long long)m * MD;
+    float s = 0.0f;
+    for (int d = 0; d < MD; ++d) {
+        int u = row[d];
+        if (u >= 0) s = s + dA[(long long)u * MH + c];
+    }
+    dHprev[t] = dHself[t] + s;
+}
+
+// Weight gradient of a message layer over one chunk of MPNN_CH vertices:
+// dW[k][n] = sum_m X[m][k] bf(dZ[m][n]) (MFMA, M = 128 inputs, N = 64 outputs, K = vertices)
+// and db[n] = sum_m dZ[m][n] (float32, rows in order).  16 vertices at a time are staged
+// in LDS; wavefront w accumulates the input tiles 2w, 2w+1 against the 4 output tiles.
+// part row: 128*64 entries of dW (row-major) then 64 of db.
+__global__ __launch_bounds__(256) void k_mpnn_layer_grad(int V, const unsigned short* __restrict__ X,
+                                                         const float* __restrict__ dH, const float* __restrict__ Z,
+                                                         float* __restrict__ part) {
+    __shared__ unsigned short sX[16 * (2 * MH + WPAD)];
+    __shared__ unsigned short sD[16 * (MH + WPAD)];
+    __shared__ float sB[16 * MH];
+    int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    int r = lane & 15, g = lane >> 4;
+    int m0 = blockIdx.x * MPNN_CH, m1 = min(V, m0 + MPNN_CH);
+    mpnn_f4 acc[2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[a][t] = (mpnn_f4){0.f, 0.f, 0.f, 0.f};
+    // bias partial: thread tid owns column tid % 64 of rows with (row % 16) / 4 == tid / 64
+    float bsum = 0.0f;
+    for (int base = m0; base < m1; base += 16) {
+        // stage X rows (16 x 128 bf16): thread -> row tid / 16, 8 elements at 8 (tid % 16)
+        {
+            int rr = tid >> 4, c8 = (tid & 15) * 8;
+            int m = base + rr;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (m < m1) v = *(const uint4*)(X + (long long)m * (2 * MH) + c8);
+            unsigned short* d = sX + rr * (2 * MH + WPAD) + c8;
+            *(uint2*)d = make_uint2(v.x, v.y);
+            *(uint2*)(d + 4) = make_uint2(v.z, v.w);
+        }
+        // stage dZ rows (16 x 64): thread -> row tid / 16, 4 columns at 4 (tid % 16)
+        {
+            int rr = tid >> 4, c4 = (tid & 15) * 4;
+            int m = base + rr;
+            float4 d = make_float4(0.f, 0.f, 0.f, 0.f), z = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (m < m1) {
+                d = *(const float4*)(dH + (long long)m * MH + c4);
+                z = *(const float4*)(Z + (long long)m * MH + c4);
+            }
+            float4 dz = make_float4(z.x > 0.f ? d.x : 0.f, z.y > 0.f ? d.y : 0.f, z.z > 0.f ? d.z : 0.f,
+                                    z.w > 0.f ? d.w : 0.f);
+            *(mpnn_s4*)(sD + rr * (MH + WPAD) + c4) = pack_bf16(dz.x, dz.y, dz.z, dz.w);
+            *(float4*)(sB + rr * MH + c4) = dz;
+        }
+        __syncthreads();
+        {
+            // thread tid: column tid % 64, rows 4 (tid / 64) .. +3 of this block of 16
+            int c = tid & 63, q = tid >> 6;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) bsum = bsum + sB[(4 * q + i) * MH + c];
+        }
+        mpnn_s4 Bop[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            mpnn_s4 b;
+            b.x = (short)sD[(4 * g + 0) * (MH + WPAD) + 16 * t + r];
+            b.y = (short)sD[(4 * g + 1) * (MH + WPAD) + 16 * t + r];
+            b.z = (short)sD[(4 * g + 2) * (MH + WPAD) + 16 * t + r];
+            b.w = (short)sD[(4 * g + 3) * (MH + WPAD) + 16 * t + r];
+            Bop[t] = b;
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            int kcol = 16 * (2 * wave + a) + r;
+            mpnn_s4 Aop;
+            Aop.x = (short)sX[(4 * g + 0) * (2 * MH + WPAD) + kcol];
+            Aop.y = (short)sX[(4 * g + 1) * (2 * MH + WPAD) + kcol];
+            Aop.z = (short)sX[(4 * g + 2) * (2 * MH + WPAD) + kcol];
+            Aop.w = (short)sX[(4 * g + 3) * (2 * MH + WPAD) + kcol];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[a][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Aop, Bop[t], acc[a][t], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    float* out = part + (long long)blockIdx.x * PCP_MPNN_LAYER_NP;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                int k = 16 * (2 * wave + a) + 4 * g + i, n = 16 * t + r;
+                out[k * MH + n] = acc[a][t][i];
+            }
+    // bias: the four row groups of each column, added in order
+    sB[tid] = bsum;
+    __syncthreads();
+    if (tid < MH) out[2 * MH * MH + tid] = ((sB[tid] + sB[MH + tid]) + sB[2 * MH + tid]) + sB[3 * MH + tid];
+}
+
+// Embedding gradient over one chunk: dZ0 = dH0 ⊙ [Z0 > 0]; entry e < 256 is
+// W0[e/64][e%64] = sum_m bf(x_m[e/64]) dZ0[m][e%64], e = 256..319 is b0[e-256]
+__global__ __launch_bounds__(320) void k_mpnn_embed_grad(int V, const float4* __restrict__ feat,
+                                                         const float* __restrict__ dH0, const float* __restrict__ Z0,
+                                                         float* __restrict__ part) {
+    int e = threadIdx.x;
+    if (e >= PCP_MPNN_EMBED_NP) return;
+    int m0 = blockIdx.x * MPNN_CH, m1 = min(V, m0 + MPNN_CH);
+    int c = e & 63, f = e >> 6;   // f = 4: bias
+    float s = 0.0f;
+    for (int m = m0; m < m1; ++m) {
+        long long o = (long long)m * MH + c;
+        float dz = Z0[o] > 0.f ? dH0[o] : 0.f;
+        if (f < 4) {
+            float4 x = feat[m];
+            float xf = f == 0 ? x.x : (f == 1 ? x.y : (f == 2 ? x.z : x.w));
+            s = s + bfr(xf) * dz;
+        } else {
+            s = s + dz;
+        }
+    }
+    part[(long long)blockIdx.x * PCP_MPNN_EMBED_NP + e] = s;
+}
+
+// ---- optimizer, targets, application -----------------------------------------------------
+// Adam (Kingma & Ba 2015) with bias correction, float32
+__global__ void k_mpnn_adam(int n, float* __restrict__ p, const float* __restrict__ gr, float* __restrict__ mo,
+                            float* __restrict__ ve, float lr, float b1, float b2, float eps, float c1, float c2) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float g = gr[i];
+    float m = b1 * mo[i] + (1.0f - b1) * g;
+    float v = b2 * ve[i] + (1.0f - b2) * (g * g);
+    mo[i] = m;
+    ve[i] = v;
+    float mh = m / c1, vh = v / c2;
+    p[i] = p[i] - lr * mh / (sqrtf(vh) + eps);
+}
+
+// target_m = minimum image (xTo_m - xFrom_m) / (ell * outScale)
+__global__ void k_mpnn_targets(int V, const int* __restrict__ shapeId, const int* __restrict__ polyPack,
+                               const double2* __restrict__ xFrom, const double2* __restrict__ xTo,
+                               const double* __restrict__ ell, double outScale, float2* __restrict__ tg) {
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= V) return;
+    double L = ell[polyPack[shapeId[m]]] * outScale;
+    double2 a = xFrom[m], b = xTo[m];
+    double dx = b.x - a.x, dy = b.y - a.y;
+    dx = dx - rint(dx);
+    dy = dy - rint(dy);
+    tg[m] = make_float2((float)(dx / L), (float)(dy / L));
+}
+
+// x_m <- frac(x_m + ell outScale y_m) for vertices of packings with active != 0
+__global__ void k_mpnn_apply(int V, const int* __restrict__ shapeId, const int* __restrict__ polyPack,
+                             const int* __restrict__ active, const float2* __restrict__ y,
+                             const double* __restrict__ ell, double outScale, double2* __restrict__ x) {
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= V) return;
+    int p = polyPack[shapeId[m]];
+    if (active && !active[p]) return;
+    double L = ell[p] * outScale;
+    float2 d = y[m];
+    double2 xi = x[m];
+    double px = xi.x + L * (double)d.x, py = xi.y + L * (double)d.y;
+    px = px - floor(px);
+    py = py - floor(py);
+    if (px >= 1.0) px = 0.0;
+    if (py >= 1.0) py = 0.0;
+    x[m] = make_double2(px, py);
+}
+
+// work buffer slices (floats): H0..H3, Z0..Z3 (V x 64 each), X1..X3 (V x 128 bf16 each)
+struct MpnnWork {
+    float* H[4];
+    float* Z[4];
+    unsigned short* X[4];
+};
+static MpnnWork mpnn_slices(float* work, int V) {
+    MpnnWork w;
+    long long S = (long long)V * MH;
+    for (int l = 0; l < 4; ++l) {
+        w.H[l] = work + l * S;
+        w.Z[l] = work + (4 + l) * S;
+    }
+    w.X[0] = nullptr;
+    for (int l = 1; l < 4; ++l) w.X[l] = (unsigned short*)(work + (8 + (l - 1)) * S);
+    return w;
+}
+static int mpnn_chunks(int V) { return (V + MPNN_CH - 1) / MPNN_CH; }
+
+}  // namespace pcp
+
+extern "C" {
+
+int pcp_mpnn_sizes(int V, long long* workFloats, long long* gradWorkFloats) {
+    if (V < 0) {
+        pcp::set_error("pcp_mpnn_sizes: V < 0");
+        return PCP_ERR_ARG;
+    }
+    long long S = (long long)V * PCP_MPNN_H;
+    long long nc = pcp::mpnn_chunks(V);
+    if (workFloats) *workFloats = 11 * S;
+    // dH (two buffers), dHself, dA (V x 64 each), loss partials (V/256 + 2 doubles),
+    // weight-gradient partials (one row per chunk)
+    if (gradWorkFloats) *gradWorkFloats = 4 * S + 2LL * (V / 256 + 2) + nc * PCP_MPNN_LAYER_NP;
+    return PCP_OK;
+}
+
+int pcp_mpnn_graph(pcp_ctx* c, int* adj, int* dropped) {
+    using namespace pcp;
+    if (c->V == 0) {
+        if (dropped) *dropped = 0;
+        return PCP_OK;
+    }
+    if (!c->dBcnt || !c->dJcnt) {
+        set_error("pcp_mpnn_graph: no records (evaluate forces first)");
+        return PCP_ERR_ARG;
+    }
+    // the counter goes into the last scalar slot of the context
+    int* dDrop = c->dScal + SC_MPNN_DROP;
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, dDrop, 1LL, 0);
+    PCP_LAUNCHED();
+    PCP_LAUNCH(c, k_mpnn_graph, dim3(nblk(c->V)), dim3(BLK), c->V, c->Kp, c->dPolyStart, c->dShapeId, c->dPolyPack,
+               c->dPackPoly, c->dBcnt, c->dBKey, c->dJcnt, c->dJKey, adj, dDrop);
+    if (dropped) {
+        PCP_HIP(hipMemcpyAsync(dropped, dDrop, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        PCP_HIP(hipStreamSynchronize(c->stream));
+    }
+    return PCP_OK;
+}
+
+int pcp_mpnn_features(pcp_ctx* c, const pcp_state* st, const int* adj, const double* ell, int energyPow,
+                      const float* inScale, float* feat) {
+    using namespace pcp;
+    if (energyPow < 1 || energyPow > 8 || !inScale) {
+        set_error("pcp_mpnn_features: energyPow must be in 1..8 and inScale given");
+        return PCP_ERR_ARG;
+    }
+    if (c->V == 0) return PCP_OK;
+    PCP_LAUNCH(c, k_mpnn_features, dim3(nblk(c->V)), dim3(BLK), c->V, c->dShapeId, c->dPolyStart, c->dPolyPack,
+               (const double2*)st->force, c->dEpoly, adj, ell, energyPow, inScale[0], inScale[1], inScale[2],
+               inScale[3], (float4*)feat);
+    return PCP_OK;
+}
+
+int pcp_mpnn_forward(pcp_ctx* c, int V, const float* params, const float* feat, const int* adj, float* work,
+                     float* out) {
+    using namespace pcp;
+    if (V < 0) {
+        set_error("pcp_mpnn_forward: V < 0");
+        return PCP_ERR_ARG;
+    }
+    if (V == 0) return PCP_OK;
+    MpnnWork w = mpnn_slices(work, V);
+    long long S = (long long)V * MH;
+    PCP_LAUNCH(c, k_mpnn_embed, dim3(nblk(S)), dim3(BLK), V, params, (const float4*)feat, w.H[0], w.Z[0]);
+    int G = (V + 63) / 64;
+    for (int l = 1; l <= 3; ++l) {
+        const float* W = params + PCP_MPNN_OFF_W1 + (l - 1) * PCP_MPNN_LAYER_NP;
+        PCP_LAUNCH(c, k_mpnn_layer, dim3(G), dim3(256), V, W, W + 2 * MH * MH, w.H[l - 1], adj, w.H[l], w.Z[l],
+                   w.X[l]);
+    }
+    PCP_LAUNCH(c, k_mpnn_out, dim3(nblk(V)), dim3(BLK), V, params, w.H[3], (float2*)out);
+    return PCP_OK;
+}
+
+int pcp_mpnn_loss(pcp_ctx* c, int V, const float* out, const float* target, float* dOut, float* gradWork,
+                  double* loss) {
+    using namespace pcp;
+    if (V <= 0) {
+        set_error("pcp_mpnn_loss: V must be positive");
+        return PCP_ERR_ARG;
+    }
+    long long S = (long long)V * MH;
+    double* part = (double*)(gradWork + 4 * S);
+    int nb = nblk(V);
+    PCP_LAUNCH(c, k_mpnn_loss, dim3(nb), dim3(256), V, (const float2*)out, (const float2*)target, (float2*)dOut,
+               part);
+    double* dl = part + nb;
+    PCP_LAUNCH(c, k_mpnn_sum_loss, dim3(1), dim3(64), nb, part, 1.0 / (double)V, dl);
+    if (loss) {
+        PCP_HIP(hipMemcpyAsync(loss, dl, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        PCP_HIP(hipStreamSynchronize(c->stream));
+    }
+    return PCP_OK;
+}
+
+int pcp_mpnn_backward(pcp_ctx* c, int V, const float* params, const float* feat, const int* adj, const float* work,
+                      const float* dOut, float* gradWork, float* grad) {
+    using namespace pcp;
+    if (V <= 0) {
+        set_error("pcp_mpnn_backward: V must be positive");
+        return PCP_ERR_ARG;
+    }
+    MpnnWork w = mpnn_slices((float*)work, V);
+    long long S = (long long)V * MH;
+    float* dHa = gradWork;
+    float* dHb = gradWork + S;
+    float* dHs = gradWork + 2 * S;
+    float* dA = gradWork + 3 * S;
+    float* part = gradWork + 4 * S + 2LL * (V / 256 + 2);
+    int nc = mpnn_chunks(V);
+    int G = (V + 63) / 64;
+    // output layer
+    PCP_LAUNCH(c, k_mpnn_out_grad, dim3(nc), dim3(192), V, w.H[3], (const float2*)dOut, part);
+    PCP_LAUNCH(c, k_mpnn_sum_chunks, dim3(1), dim3(256), nc, PCP_MPNN_OUT_NP, part, grad + PCP_MPNN_OFF_WO);
+    PCP_LAUNCH(c, k_mpnn_out_bwd, dim3(nblk(S)), dim3(BLK), V, params, (const float2*)dOut, dHa);
+    float* dH = dHa;
+    float* dHn = dHb;
+    for (int l = 3; l >= 1; --l) {
+        const float* W = params + PCP_MPNN_OFF_W1 + (l - 1) * PCP_MPNN_LAYER_NP;
+        PCP_LAUNCH(c, k_mpnn_layer_grad, dim3(nc), dim3(256), V, w.X[l], dH, w.Z[l], part);
+        PCP_LAUNCH(c, k_mpnn_sum_chunks, dim3(nblk(PCP_MPNN_LAYER_NP)), dim3(BLK), nc, PCP_MPNN_LAYER_NP, part,
+                   grad + PCP_MPNN_OFF_W1 + (l - 1) * PCP_MPNN_LAYER_NP);
+        PCP_LAUNCH(c, k_mpnn_layer_bwd, dim3(G), dim3(256), V, W, dH, w.Z[l], dHs, dA);
+        PCP_LAUNCH(c, k_mpnn_gather_bwd, dim3(nblk(S)), dim3(BLK), V, adj, dHs, dA, dHn);
+        float* t = dH;
+        dH = dHn;
+        dHn = t;
+    }
+    PCP_LAUNCH(c, k_mpnn_embed_grad, dim3(nc), dim3(320), V, (const float4*)feat, dH, w.Z[0], part);
+    PCP_LAUNCH(c, k_mpnn_sum_chunks, dim3(nblk(PCP_MPNN_EMBED_NP)), dim3(BLK), nc, PCP_MPNN_EMBED_NP, part,
+               grad + PCP_MPNN_OFF_W0);
+    return PCP_OK;
+}
+
+int pcp_mpnn_adam(pcp_ctx* c, float* params, const float* grad, float* m, float* v, int step, float lr,
+                  float beta1, float beta2, float eps) {
+    using namespace pcp;
+    if (step < 1) {
+        set_error("pcp_mpnn_adam: step counts from 1");
+        return PCP_ERR_ARG;
+    }
+    float c1 = 1.0f - powf(beta1, (float)step), c2 = 1.0f - powf(beta2, (float)step);
+    PCP_LAUNCH(c, k_mpnn_adam, dim3(nblk(PCP_MPNN_NPARAM)), dim3(BLK), PCP_MPNN_NPARAM, params, grad, m, v, lr, beta1,
+               beta2, eps, c1, c2);
+    return PCP_OK;
+}
+
+int pcp_mpnn_targets(pcp_ctx* c, const double* xFrom, const double* xTo, const double* ell, double outScale,
+                     float* target) {
+    using namespace pcp;
+    if (c->V == 0) return PCP_OK;
+    PCP_LAUNCH(c, k_mpnn_targets, dim3(nblk(c->V)), dim3(BLK), c->V, c->dShapeId, c->dPolyPack,
+               (const double2*)xFrom, (const double2*)xTo, ell, outScale, (float2*)target);
+    return PCP_OK;
+}
+
+int pcp_mpnn_apply(pcp_ctx* c, double* pos, const float* out, const double* ell, double outScale,
+                   const int* active) {
+    using namespace pcp;
+    if (c->V == 0) return PCP_OK;
+    PCP_LAUNCH(c, k_mpnn_apply, dim3(nblk(c->V)), dim3(BLK), c->V, c->dShapeId, c->dPolyPack, active,
+               (const float2*)out, ell, outScale, (double2*)pos);
+    return PCP_OK;
+}
+
+}  // extern "C"

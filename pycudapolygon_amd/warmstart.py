@@ -1,0 +1,312 @@
+"""Learned warm start for FIRE minimization (host side of the PCP_MPNN_* entry points).
+
+No counterpart in the reference (camdennis/pyCudaPolygon has no learned component).
+A small message-passing network over the contact graph of a packing predicts a
+displacement of every vertex; the displacement is applied once, the polygons are put
+back on their shapes by SHAKE, and plain FP64 FIRE then minimizes to max|F| <= fTol
+and judges the residual force.  Network, features and kernels: include/pcp.h
+(PCP_MPNN_*) and csrc/pcp_mpnn.hip; everything here runs on the GPU through the C ABI.
+
+Training data are FIRE trajectories that the method itself produces: states captured
+at a few step counts along each trajectory, each labelled with the displacement (minimum
+image) from that state to the packing's minimized state.  Training is mean squared error
+with Adam, on the GPU, through the library's own forward and backward kernels.
+"""
+import ctypes
+import time
+
+import numpy as np
+
+from . import capi
+from .generate import make_packing, box_size_for
+
+NPARAM = 25218
+DEG = 16
+H = 64
+SQUARED = capi.NORMAL | 0x200
+
+
+def _p(a):
+    return a.ptr
+
+
+def _np_ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def init_params(seed):
+    """He-style initial parameters (float32), biases zero; layout of include/pcp.h."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(NPARAM, np.float32)
+    p[0:256] = rng.normal(0, np.sqrt(2.0 / 4), 256)
+    for l in range(3):
+        o = 320 + l * 8256
+        p[o:o + 8192] = rng.normal(0, 0.5 * np.sqrt(2.0 / 128), 8192)
+    p[25088:25216] = rng.normal(0, np.sqrt(1.0 / 64), 128)
+    return p
+
+
+def packing_length_units(packings):
+    """ell of each packing: its mean target edge length."""
+    return np.array([float(np.mean(p["targetEdgeV"])) for p in packings])
+
+
+class Network:
+    """Device-resident parameters plus forward/backward work buffers for up to Vmax vertices."""
+
+    def __init__(self, ctx, params, Vmax, train=False):
+        self.ctx = ctx
+        self.params = capi.DeviceArray.from_numpy(np.asarray(params, np.float32))
+        self.Vmax = 0
+        self.train = train
+        self._grow(Vmax)
+        if train:
+            self.grad = capi.DeviceArray((NPARAM,), np.float32)
+            self.m = capi.DeviceArray((NPARAM,), np.float32); self.m.zero()
+            self.v = capi.DeviceArray((NPARAM,), np.float32); self.v.zero()
+            self.step = 0
+
+    def _grow(self, V):
+        if V <= self.Vmax:
+            return
+        wf, gf = ctypes.c_longlong(0), ctypes.c_longlong(0)
+        capi.check(capi.lib().pcp_mpnn_sizes(int(V), ctypes.byref(wf), ctypes.byref(gf)))
+        self.work = capi.DeviceArray((wf.value,), np.float32)
+        self.out = capi.DeviceArray((2 * V,), np.float32)
+        if self.train:
+            self.gwork = capi.DeviceArray((gf.value,), np.float32)
+            self.dout = capi.DeviceArray((2 * V,), np.float32)
+        self.Vmax = V
+
+    def sync(self):
+        """wait for the context's stream (device buffers are then safe to read back)"""
+        self.ctx.call("pcp_synchronize")
+
+    def get_params(self):
+        self.sync()
+        return self.params.get()
+
+    def forward(self, V, feat_ptr, adj_ptr):
+        """queue one forward pass on the context's stream (asynchronous)"""
+        self._grow(V)
+        self.ctx.call("pcp_mpnn_forward", int(V), self.params.ptr, feat_ptr, adj_ptr, self.work.ptr, self.out.ptr)
+        return self.out
+
+    def train_step(self, V, feat_ptr, adj_ptr, target_ptr, lr, want_loss=False):
+        """forward, loss, backward and one Adam step; returns the loss (or None)."""
+        self.forward(V, feat_ptr, adj_ptr)
+        loss = ctypes.c_double(0.0)
+        self.ctx.call("pcp_mpnn_loss", int(V), self.out.ptr, target_ptr, self.dout.ptr, self.gwork.ptr,
+                      ctypes.byref(loss) if want_loss else None)
+        self.ctx.call("pcp_mpnn_backward", int(V), self.params.ptr, feat_ptr, adj_ptr, self.work.ptr,
+                      self.dout.ptr, self.gwork.ptr, self.grad.ptr)
+        self.step += 1
+        self.ctx.call("pcp_mpnn_adam", self.params.ptr, self.grad.ptr, self.m.ptr, self.v.ptr, self.step,
+                      ctypes.c_float(lr), ctypes.c_float(0.9), ctypes.c_float(0.999), ctypes.c_float(1e-8))
+        return loss.value if want_loss else None
+
+    def loss(self, V, feat_ptr, adj_ptr, target_ptr):
+        self._grow(V)
+        if not self.train:
+            raise RuntimeError("loss needs a training network")
+        self.forward(V, feat_ptr, adj_ptr)
+        loss = ctypes.c_double(0.0)
+        self.ctx.call("pcp_mpnn_loss", int(V), self.out.ptr, target_ptr, self.dout.ptr, self.gwork.ptr,
+                      ctypes.byref(loss))
+        return loss.value
+
+
+class GraphFeatures:
+    """Contact graph + input features of a System's current evaluation (device buffers)."""
+
+    def __init__(self, system, ell, inScale, energyPow=4):
+        self.s = system
+        self.adj = capi.DeviceArray((system.V * DEG,), np.int32)
+        self.feat = capi.DeviceArray((system.V * 4,), np.float32)
+        self.ell = capi.DeviceArray.from_numpy(np.asarray(ell, np.float64))
+        self.inScale = np.asarray(inScale, np.float32)
+        self.energyPow = int(energyPow)
+
+    def compute(self, want_dropped=False):
+        d = ctypes.c_int(0)
+        self.s.ctx.call("pcp_mpnn_graph", self.adj.ptr, ctypes.byref(d) if want_dropped else None)
+        self.s.ctx.call("pcp_mpnn_features", ctypes.byref(self.s._state), self.adj.ptr, self.ell.ptr,
+                        self.energyPow, _np_ptr(self.inScale), self.feat.ptr)
+        self.s.sync()
+        return d.value
+
+
+class WarmStart:
+    """A trained network applied to a System: features at the current positions (after an
+    evaluation), one forward pass, x += ell * outScale * y, then SHAKE."""
+
+    def __init__(self, weights):
+        self.params = np.asarray(weights["params"], np.float32)
+        self.inScale = np.asarray(weights["inScale"], np.float32)
+        self.outScale = float(weights["outScale"])
+        self.energyPow = int(weights.get("energyPow", 4))
+        self.shakeIter = int(weights.get("shakeIter", 1000))
+
+    def apply(self, system, ell, active=None):
+        """Apply once to every packing of `system` whose forces were just evaluated.
+        Returns the seconds spent (device work included)."""
+        t0 = time.perf_counter()
+        gf = GraphFeatures(system, ell, self.inScale, self.energyPow)
+        gf.compute()
+        net = Network(system.ctx, self.params, system.V)
+        out = net.forward(system.V, gf.feat.ptr, gf.adj.ptr)
+        system.ctx.call("pcp_mpnn_apply", system.pos.ptr, out.ptr, gf.ell.ptr, self.outScale,
+                        active.ptr if active is not None else None)
+        system.shake(self.shakeIter, 1e-15)
+        system.sync()
+        return time.perf_counter() - t0
+
+
+def save_weights(path, params, inScale, outScale, energyPow=4, **meta):
+    np.savez(path, params=np.asarray(params, np.float32), inScale=np.asarray(inScale, np.float32),
+             outScale=np.float64(outScale), energyPow=np.int64(energyPow),
+             meta=np.array(repr(meta)))
+
+
+def load_weights(path):
+    with np.load(path, allow_pickle=False) as z:
+        return dict(params=z["params"], inScale=z["inScale"], outScale=float(z["outScale"]),
+                    energyPow=int(z["energyPow"]))
+
+
+# ---- training data ---------------------------------------------------------------------
+def make_c4_packings(P, seed, N=1024, n=16, phi_range=(0.82, 0.88)):
+    """P packings of the learned-warm-start configuration: N polygons of n vertices,
+    phi ~ U[phi_range], centres at least (R_i + R_j)/2 apart."""
+    rng = np.random.default_rng(seed)
+    phis = rng.uniform(phi_range[0], phi_range[1], P)
+    seeds = rng.integers(0, 2 ** 31 - 1, P)
+    return [make_packing(N, n, float(ph), int(sd), min_center_frac=0.5) for ph, sd in zip(phis, seeds)]
+
+
+def fire_system(packings, fTol=1e-10):
+    B = [box_size_for(p["diameter"]) for p in packings]
+    s = capi.System(packings, B, model=SQUARED)
+    prm = s.fire_params(fTol=fTol)
+    return s, prm
+
+
+def fire_run(s, prm, maxSteps, dt0=1e-3, init=True):
+    """FIRE from the current positions until every packing converges or maxSteps; returns
+    (steps per packing (-1 = not converged), total steps, seconds)."""
+    if init:
+        s.fire_init(prm, dt0)
+    conv = np.full(s.P, -1)
+    t0 = time.perf_counter()
+    steps = 0
+    while steps < maxSteps:
+        st = s.fire_step(prm)
+        steps += 1
+        for k in range(s.P):
+            if conv[k] < 0 and st[k].converged:
+                conv[k] = st[k].steps
+        if (conv >= 0).all():
+            break
+    s.sync()
+    return conv, steps, time.perf_counter() - t0
+
+
+class Dataset:
+    """Captured states of FIRE trajectories, on the device: per sample (one System state of
+    P packings) features [V,4], adjacency [V,DEG] (ids local to the sample), positions [2V]
+    and, once labelled, targets [V,2]."""
+
+    def __init__(self):
+        self.samples = []   # list of dict(feat, adj, x, target, V)
+
+    def capture(self, s, gf):
+        gf.compute()
+        V = s.V
+        feat = capi.DeviceArray((V * 4,), np.float32)
+        adj = capi.DeviceArray((V * DEG,), np.int32)
+        x = capi.DeviceArray((2 * V,), np.float64)
+        capi.check(capi.lib().pcp_memcpy_d2d(feat.ptr, gf.feat.ptr, feat.nbytes))
+        capi.check(capi.lib().pcp_memcpy_d2d(adj.ptr, gf.adj.ptr, adj.nbytes))
+        capi.check(capi.lib().pcp_memcpy_d2d(x.ptr, s.pos.ptr, x.nbytes))
+        self.samples.append(dict(feat=feat, adj=adj, x=x, target=None, V=V, sys=s, ell=gf.ell))
+
+    def label(self, s, outScale):
+        """targets of every sample of system s = displacement to s's current positions"""
+        for d in self.samples:
+            if d["sys"] is not s or d["target"] is not None:
+                continue
+            d["target"] = capi.DeviceArray((2 * d["V"],), np.float32)
+            s.ctx.call("pcp_mpnn_targets", d["x"].ptr, s.pos.ptr, d["ell"].ptr, float(outScale), d["target"].ptr)
+        s.sync()
+
+
+def raw_feature_scales(s, ell, energyPow=4):
+    """RMS of the raw feature quantities of a System's current evaluation (host):
+    F/ell^(e-1) per component, crossings (1), E_poly/(n ell^e)."""
+    F = s.get_forces().reshape(-1, 2)
+    lv = np.repeat(np.asarray(ell), np.diff(s.packVert))
+    Fs = F / lv[:, None] ** (energyPow - 1)
+    rms = lambda a: float(np.sqrt(np.mean(a ** 2))) or 1.0
+    return np.array([rms(Fs[:, 0]), rms(Fs[:, 1]), 1.0, 1.0], np.float32)
+
+
+def generate(batches, P, seed, capture_steps=(0, 100, 300, 1000), maxSteps=12000, log=print, N=1024, n=16):
+    """Run FIRE on `batches` systems of P packings and capture states at capture_steps.
+    Returns (Dataset, list of (system, ell, conv, seconds)), with inScale from the first
+    system's initial forces and the energy-feature scale from its polygon energies."""
+    ds = Dataset()
+    runs = []
+    inScale = None
+    for b in range(batches):
+        pk = make_c4_packings(P, seed + 7919 * b, N=N, n=n)
+        ell = packing_length_units(pk)
+        s, prm = fire_system(pk)
+        s.fire_init(prm, 1e-3)
+        if inScale is None:
+            inScale = raw_feature_scales(s, ell)
+            # energy feature scale: mean polygon energy per vertex over ell^4 (positive part)
+            inScale[3] = 1.0
+        gf = GraphFeatures(s, ell, inScale)
+        conv = np.full(s.P, -1)
+        t0 = time.perf_counter()
+        steps = 0
+        caps = sorted(set(capture_steps))
+        if 0 in caps:
+            ds.capture(s, gf)
+        while steps < maxSteps:
+            st = s.fire_step(prm)
+            steps += 1
+            for k in range(s.P):
+                if conv[k] < 0 and st[k].converged:
+                    conv[k] = st[k].steps
+            if steps in caps:
+                ds.capture(s, gf)
+            if (conv >= 0).all():
+                break
+        el = time.perf_counter() - t0
+        runs.append((s, ell, conv, el))
+        log(f"[warmstart] batch {b}: {P} packings, FIRE {steps} steps in {el:.1f} s, converged "
+            f"{int((conv >= 0).sum())}/{P}, median steps {int(np.median(conv[conv >= 0])) if (conv >= 0).any() else -1}")
+    return ds, runs, inScale
+
+
+def train(ds, params, outScale, epochs, lr=1e-3, log=print, eval_every=0, ctx=None):
+    """Adam on the captured samples (one sample -- P packings -- per step, samples in a
+    seeded random order each epoch).  Returns (params, losses)."""
+    Vmax = max(d["V"] for d in ds.samples)
+    ctx = ctx or ds.samples[0]["sys"].ctx
+    net = Network(ctx, params, Vmax, train=True)
+    rng = np.random.default_rng(0)
+    losses = []
+    t0 = time.perf_counter()
+    for ep in range(epochs):
+        order = rng.permutation(len(ds.samples))
+        ep_loss = []
+        for k, i in enumerate(order):
+            d = ds.samples[i]
+            want = (k == len(order) - 1) or eval_every
+            L = net.train_step(d["V"], d["feat"].ptr, d["adj"].ptr, d["target"].ptr, lr, want_loss=True)
+            ep_loss.append(L)
+        losses.append(float(np.mean(ep_loss)))
+        log(f"[warmstart] epoch {ep}: mean loss {losses[-1]:.5f} ({time.perf_counter() - t0:.1f} s)")
+    return net.get_params(), losses
