@@ -61,7 +61,9 @@ const char* pcp_last_error(void);
 int pcp_device_count(int* count);
 
 /* ---- device memory plumbing (replaces the cudaMalloc/cudaMemcpy calls in
- *      src/model.cpp:60-101, 522-628 and the getters :680-889) ---- */
+ *      src/model.cpp:60-101, 522-628 and the getters :680-889).  Each copy / memset is
+ *      ordered after all device work queued before the call (any context) and has
+ *      completed when it returns (as the reference's synchronous cudaMemcpy calls). ---- */
 int pcp_malloc(void** ptr, size_t bytes);
 int pcp_free(void* ptr);
 int pcp_memcpy_h2d(void* dst_device, const void* src_host, size_t bytes);

@@ -1080,20 +1080,36 @@ int pcp_free(void* ptr) {
     if (ptr) PCP_HIP(hipFree(ptr));
     return PCP_OK;
 }
+// The plumbing copies and memsets run on the null stream, which is not ordered with the
+// contexts' non-blocking streams, and device-to-device copies and memsets may return
+// before they are done.  Each call therefore waits for all queued device work first (so it
+// sees what earlier kernels wrote and cannot overwrite a buffer they still read) and for
+// itself afterwards (so work queued later sees its result).
 int pcp_memcpy_h2d(void* dst, const void* src, size_t bytes) {
-    if (bytes) PCP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    if (!bytes) return PCP_OK;
+    PCP_HIP(hipDeviceSynchronize());
+    PCP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    PCP_HIP(hipDeviceSynchronize());
     return PCP_OK;
 }
 int pcp_memcpy_d2h(void* dst, const void* src, size_t bytes) {
-    if (bytes) PCP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    if (!bytes) return PCP_OK;
+    PCP_HIP(hipDeviceSynchronize());
+    PCP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return PCP_OK;
 }
 int pcp_memcpy_d2d(void* dst, const void* src, size_t bytes) {
-    if (bytes) PCP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    if (!bytes) return PCP_OK;
+    PCP_HIP(hipDeviceSynchronize());
+    PCP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    PCP_HIP(hipDeviceSynchronize());
     return PCP_OK;
 }
 int pcp_memset(void* ptr, int value, size_t bytes) {
-    if (bytes) PCP_HIP(hipMemset(ptr, value, bytes));
+    if (!bytes) return PCP_OK;
+    PCP_HIP(hipDeviceSynchronize());
+    PCP_HIP(hipMemset(ptr, value, bytes));
+    PCP_HIP(hipDeviceSynchronize());
     return PCP_OK;
 }
 

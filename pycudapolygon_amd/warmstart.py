@@ -13,6 +13,7 @@ image) from that state to the packing's minimized state.  Training is mean squar
 with Adam, on the GPU, through the library's own forward and backward kernels.
 """
 import ctypes
+import os
 import time
 
 import numpy as np
@@ -24,6 +25,10 @@ NPARAM = 25218
 DEG = 16
 H = 64
 SQUARED = capi.NORMAL | 0x200
+# weights trained by `python -m pycudapolygon_amd.train_warmstart` (its defaults)
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "warmstart_n1024_n16.npz")
+# FIRE time-step cap for the squared lens energy (see train_warmstart)
+DT_MAX = 100.0
 
 
 def _p(a):
@@ -140,6 +145,10 @@ class WarmStart:
     """A trained network applied to a System: features at the current positions (after an
     evaluation), one forward pass, x += ell * outScale * y, then SHAKE."""
 
+    @classmethod
+    def default(cls):
+        return cls(load_weights(DEFAULT_WEIGHTS))
+
     def __init__(self, weights):
         self.params = np.asarray(weights["params"], np.float32)
         self.inScale = np.asarray(weights["inScale"], np.float32)
@@ -184,10 +193,12 @@ def make_c4_packings(P, seed, N=1024, n=16, phi_range=(0.82, 0.88)):
     return [make_packing(N, n, float(ph), int(sd), min_center_frac=0.5) for ph, sd in zip(phis, seeds)]
 
 
-def fire_system(packings, fTol=1e-10):
+def fire_system(packings, fTol=1e-10, dtMax=0.1):
+    """System of the packings (squared lens areas, rigid polygons) and its FIRE parameters
+    (the reference driver's defaults except dtMax)."""
     B = [box_size_for(p["diameter"]) for p in packings]
     s = capi.System(packings, B, model=SQUARED)
-    prm = s.fire_params(fTol=fTol)
+    prm = s.fire_params(fTol=fTol, dtMax=dtMax)
     return s, prm
 
 
@@ -241,31 +252,34 @@ class Dataset:
 
 
 def raw_feature_scales(s, ell, energyPow=4):
-    """RMS of the raw feature quantities of a System's current evaluation (host):
-    F/ell^(e-1) per component, crossings (1), E_poly/(n ell^e)."""
+    """Input scales from a System's current evaluation: RMS of F/ell^(e-1) per component,
+    1 for the crossing count, RMS of E_poly/(n ell^e) over the polygons with energy."""
     F = s.get_forces().reshape(-1, 2)
     lv = np.repeat(np.asarray(ell), np.diff(s.packVert))
     Fs = F / lv[:, None] ** (energyPow - 1)
-    rms = lambda a: float(np.sqrt(np.mean(a ** 2))) or 1.0
-    return np.array([rms(Fs[:, 0]), rms(Fs[:, 1]), 1.0, 1.0], np.float32)
+    rms = lambda a: float(np.sqrt(np.mean(a.astype(np.float64) ** 2))) if len(a) and np.any(a) else 1.0
+    sc = np.array([rms(Fs[:, 0]), rms(Fs[:, 1]), 1.0, 1.0], np.float32)
+    gf = GraphFeatures(s, ell, sc, energyPow)
+    gf.compute()
+    e = np.sinh(gf.feat.get().reshape(-1, 4)[:, 3].astype(np.float64))
+    sc[3] = rms(e[e > 0])
+    return sc
 
 
-def generate(batches, P, seed, capture_steps=(0, 100, 300, 1000), maxSteps=12000, log=print, N=1024, n=16):
-    """Run FIRE on `batches` systems of P packings and capture states at capture_steps.
-    Returns (Dataset, list of (system, ell, conv, seconds)), with inScale from the first
-    system's initial forces and the energy-feature scale from its polygon energies."""
+def generate(batches, P, seed, capture_steps=(0, 100, 300, 1000), maxSteps=12000, log=print, N=1024, n=16,
+             inScale=None, dtMax=0.1):
+    """Run FIRE on `batches` systems of P packings (make_c4_packings) and capture states at
+    capture_steps.  Returns (Dataset, runs = [(system, ell, conv, seconds)], inScale); the
+    input scales come from the first system's initial evaluation unless given."""
     ds = Dataset()
     runs = []
-    inScale = None
     for b in range(batches):
         pk = make_c4_packings(P, seed + 7919 * b, N=N, n=n)
         ell = packing_length_units(pk)
-        s, prm = fire_system(pk)
+        s, prm = fire_system(pk, dtMax=dtMax)
         s.fire_init(prm, 1e-3)
         if inScale is None:
             inScale = raw_feature_scales(s, ell)
-            # energy feature scale: mean polygon energy per vertex over ell^4 (positive part)
-            inScale[3] = 1.0
         gf = GraphFeatures(s, ell, inScale)
         conv = np.full(s.P, -1)
         t0 = time.perf_counter()
@@ -285,28 +299,93 @@ def generate(batches, P, seed, capture_steps=(0, 100, 300, 1000), maxSteps=12000
                 break
         el = time.perf_counter() - t0
         runs.append((s, ell, conv, el))
+        ok = conv >= 0
         log(f"[warmstart] batch {b}: {P} packings, FIRE {steps} steps in {el:.1f} s, converged "
-            f"{int((conv >= 0).sum())}/{P}, median steps {int(np.median(conv[conv >= 0])) if (conv >= 0).any() else -1}")
+            f"{int(ok.sum())}/{P}, median steps {int(np.median(conv[ok])) if ok.any() else -1}")
     return ds, runs, inScale
 
 
-def train(ds, params, outScale, epochs, lr=1e-3, log=print, eval_every=0, ctx=None):
-    """Adam on the captured samples (one sample -- P packings -- per step, samples in a
-    seeded random order each epoch).  Returns (params, losses)."""
-    Vmax = max(d["V"] for d in ds.samples)
-    ctx = ctx or ds.samples[0]["sys"].ctx
+def label_all(ds, runs):
+    """Label every sample with the displacement to its system's final positions, scaled so
+    that the targets have unit RMS per component; returns that scale (outScale)."""
+    for s, _, _, _ in runs:
+        ds.label(s, 1.0)
+    acc, cnt = 0.0, 0
+    for d in ds.samples:
+        t = d["target"].get().astype(np.float64)
+        acc += float((t ** 2).sum())
+        cnt += t.size
+    outScale = float(np.sqrt(acc / max(cnt, 1))) or 1.0
+    for d in ds.samples:
+        d["target"] = None
+    for s, _, _, _ in runs:
+        ds.label(s, outScale)
+    return outScale
+
+
+def train(samples, params, epochs, lr=1e-3, log=print, val=(), lr_final=None):
+    """Adam on the captured samples (one sample -- one System state of P packings -- per
+    step, in a seeded random order each epoch; learning rate decayed geometrically to
+    lr_final).  Returns (params, train losses, validation losses)."""
+    Vmax = max(d["V"] for d in list(samples) + list(val))
+    ctx = samples[0]["sys"].ctx
     net = Network(ctx, params, Vmax, train=True)
     rng = np.random.default_rng(0)
-    losses = []
+    losses, vlosses = [], []
     t0 = time.perf_counter()
+    nsteps = max(1, epochs * len(samples))
+    k = 0
     for ep in range(epochs):
-        order = rng.permutation(len(ds.samples))
+        order = rng.permutation(len(samples))
         ep_loss = []
-        for k, i in enumerate(order):
-            d = ds.samples[i]
-            want = (k == len(order) - 1) or eval_every
-            L = net.train_step(d["V"], d["feat"].ptr, d["adj"].ptr, d["target"].ptr, lr, want_loss=True)
-            ep_loss.append(L)
+        for i in order:
+            d = samples[i]
+            a = lr if lr_final is None else lr * (lr_final / lr) ** (k / nsteps)
+            ep_loss.append(net.train_step(d["V"], d["feat"].ptr, d["adj"].ptr, d["target"].ptr, a, want_loss=True))
+            k += 1
         losses.append(float(np.mean(ep_loss)))
-        log(f"[warmstart] epoch {ep}: mean loss {losses[-1]:.5f} ({time.perf_counter() - t0:.1f} s)")
-    return net.get_params(), losses
+        if val:
+            vlosses.append(float(np.mean([net.loss(d["V"], d["feat"].ptr, d["adj"].ptr, d["target"].ptr)
+                                          for d in val])))
+        log(f"[warmstart] epoch {ep}: train loss {losses[-1]:.5f}"
+            + (f", validation {vlosses[-1]:.5f}" if val else "") + f" ({time.perf_counter() - t0:.1f} s)")
+    return net.get_params(), losses, vlosses
+
+
+def ab_compare(weights, packings, maxSteps=12000, applications=1, at_step=0, gamma=1.0, log=print, dtMax=0.1):
+    """Plain FIRE against FIRE with the learned displacement applied (one System each, same
+    packings).  The learned run does `at_step` plain FIRE steps first (0 = a warm start of
+    the initial packing), then applies the network `applications` times (each after a fresh
+    evaluation, displacement scaled by gamma), then FIRE to convergence from a fresh FIRE
+    state.  Steps per packing count both FIRE phases; seconds include the network."""
+    ell = packing_length_units(packings)
+    s0, prm = fire_system(packings, dtMax=dtMax)
+    conv0, steps0, t0 = fire_run(s0, prm, maxSteps)
+    E0 = s0.get_energy()
+    F0 = s0.max_abs_force()
+    s1, prm1 = fire_system(packings, dtMax=dtMax)
+    ws = WarmStart(weights)
+    ws.outScale *= gamma
+    pre, tpre = np.zeros(s1.P, int), 0.0
+    if at_step > 0:
+        c, n1, tpre = fire_run(s1, prm1, at_step)
+        pre[:] = at_step
+        pre[c >= 0] = c[c >= 0]
+    tw = 0.0
+    for _ in range(applications):
+        s1.force_eval()
+        tw += ws.apply(s1, ell)
+    conv1, steps1, t1 = fire_run(s1, prm1, maxSteps)
+    conv1 = np.where(conv1 >= 0, conv1 + pre, -1)
+    E1 = s1.get_energy()
+    F1 = s1.max_abs_force()
+    res = dict(packings=len(packings), at_step=at_step, gamma=gamma, applications=applications,
+               plain_steps=conv0.tolist(), warm_steps=conv1.tolist(),
+               plain_seconds=t0, warm_seconds=tpre + t1 + tw, warm_apply_seconds=tw,
+               plain_energy=E0.tolist(), warm_energy=E1.tolist(), plain_maxF=F0.tolist(), warm_maxF=F1.tolist())
+    ok0, ok1 = conv0 >= 0, conv1 >= 0
+    log(f"[warmstart] A/B on {len(packings)} packings (learned at step {at_step}, gamma {gamma}): plain mean steps "
+        f"{conv0[ok0].mean() if ok0.any() else -1:.0f} ({ok0.sum()} converged, {t0:.2f} s); learned mean steps "
+        f"{conv1[ok1].mean() if ok1.any() else -1:.0f} ({ok1.sum()} converged, {tpre + t1 + tw:.2f} s incl. "
+        f"{tw:.3f} s network)")
+    return res
