@@ -116,19 +116,35 @@ def fire_legs(args):
                                  ("n4096_learned", W.DT_MAX, ws, 5000)):
         out[tag] = fire_one([p2], dt, steps, warm, budget_s=args.fire_seconds)
         log(out[tag], tag)
+    from pycudapolygon_amd import batch
     p4 = W.make_c4_packings(32, 777777)
     for tag, warm in (("n1024_x32", None), ("n1024_x32_learned", ws)):
-        out[tag] = fire_one(p4, W.DT_MAX, 5000, warm, budget_s=args.fire_seconds)
-        log(out[tag], tag)
+        r = batch.minimize_batch(p4, W.SQUARED, dict(fTol=1e-10, dtMax=W.DT_MAX), 5000, warm=warm)
+        ok = r.steps >= 0
+        out[tag] = dict(packings=32, workload="32 packings of N=1024 x 16, phi ~ U[0.82,0.88], one context, "
+                        "re-packed when fewer than half are still running", warm_start=warm is not None,
+                        converged=int(ok.sum()), seconds=r.seconds, packings_per_s=float(ok.sum()) / r.seconds,
+                        mean_steps=float(r.steps[ok].mean()), max_steps=int(r.steps[ok].max()),
+                        contexts=r.contexts, max_final_force=float(r.max_force.max()), dtMax=W.DT_MAX)
+        print(f"[bench] FIRE {tag}: {int(ok.sum())}/32 converged, mean steps {out[tag]['mean_steps']:.1f}, "
+              f"{r.seconds:.3f} s, {out[tag]['packings_per_s']:.2f} packings/s", flush=True)
     if args.batch_packings > 0:
+        from pycudapolygon_amd import batch
         t0 = time.perf_counter()
         p3 = make_batch(args.batch_packings, args.seed + 3, min_center_frac=0.5)
         gen = time.perf_counter() - t0
-        out["batched_uneven"] = fire_one(p3, W.DT_MAX, 5000, None, budget_s=args.fire_seconds)
-        out["batched_uneven"]["workload"] = (f"{args.batch_packings} packings, N ~ U[64,1024], n ~ U[8,32], "
-                                             "phi ~ U[0.80,0.90], one context")
-        out["batched_uneven"]["generation_seconds"] = gen
-        log(out["batched_uneven"], "batched_uneven")
+        r = batch.minimize_batch(p3, W.SQUARED, dict(fTol=1e-10, dtMax=W.DT_MAX), 5000)
+        ok = r.steps >= 0
+        out["batched_uneven"] = dict(
+            workload=(f"{args.batch_packings} packings, N ~ U[64,1024], n ~ U[8,32], phi ~ U[0.80,0.90], "
+                      "one context, re-packed when fewer than half its packings are still running"),
+            packings=args.batch_packings, converged=int(ok.sum()), seconds=r.seconds,
+            packings_per_s=float(ok.sum()) / r.seconds, mean_steps=float(r.steps[ok].mean()),
+            max_steps=int(r.steps[ok].max()), step_cap=5000, contexts=r.contexts,
+            max_final_force_converged=float(r.max_force[ok].max()), generation_seconds=gen, dtMax=W.DT_MAX)
+        print(f"[bench] FIRE batched_uneven: {int(ok.sum())}/{args.batch_packings} converged, "
+              f"{r.seconds:.3f} s, {out['batched_uneven']['packings_per_s']:.2f} packings/s, "
+              f"contexts {r.contexts}", flush=True)
     out["note"] = ("dtMax 0.1 is the reference driver's default; the squared lens energy's stiffness scales as "
                    "ell^2 (ell = edge length), so its natural time step is ~1/ell and FIRE at dtMax 0.1 is "
                    "time-step bound; dtMax 100 is used for the other legs. learned = the committed warm-start "

@@ -1,0 +1,105 @@
+"""Batched FIRE minimization of many independent packings with compaction.
+
+One context holds P packings and every kernel covers all of them in one launch, but a
+packing that has converged still costs its share of each launch until the whole batch is
+done.  minimize_batch() therefore re-packs the batch when few packings remain: the
+unconverged ones continue in a new, smaller context built from their current positions,
+velocities and FIRE state (dt, alpha, nPos).  Every quantity of a packing is computed from
+that packing alone, so the continued trajectories are bit-identical to the uncompacted
+ones.  No counterpart in the reference (its Model holds one packing).
+"""
+import time
+
+import numpy as np
+
+from . import capi
+from .generate import box_size_for
+
+
+class BatchResult:
+    def __init__(self, P):
+        self.steps = np.full(P, -1)          # FIRE steps to convergence (-1: not converged)
+        self.energy = np.full(P, np.nan)
+        self.max_force = np.full(P, np.nan)
+        self.positions = [None] * P
+        self.seconds = 0.0
+        self.contexts = []                   # packings per context used, in order
+
+
+def _with_positions(p, x):
+    q = dict(p)
+    q["positions"] = x
+    return q
+
+
+def minimize_batch(packings, model, fire_kwargs, maxSteps, dt0=1e-3, compact_below=0.5, min_context=8,
+                   boxSizes=None, log=None, warm=None):
+    """FIRE on `packings` until each has max|F| <= fTol or maxSteps steps of its own.
+    compact_below: re-pack when fewer than this fraction of the context's packings are
+    still running (and the context holds more than min_context packings).
+    warm: a warmstart.WarmStart applied once to the first context before FIRE starts
+    (its time is included in the result's seconds)."""
+    P = len(packings)
+    res = BatchResult(P)
+    if boxSizes is None:
+        boxSizes = [box_size_for(p["diameter"]) for p in packings]
+    idx = np.arange(P)
+    cur = list(packings)
+    offset = np.zeros(P, int)           # steps done in earlier contexts
+    state = None                        # (vel per packing, dt, alpha, nPos) when continuing
+    t0 = time.perf_counter()
+    while len(idx):
+        s = capi.System(cur, [boxSizes[i] for i in idx], model=model)
+        prm = s.fire_params(**fire_kwargs)
+        if warm is not None and state is None:
+            s.force_eval()
+            warm.apply(s, np.array([float(np.mean(p["targetEdgeV"])) for p in cur]))
+        s.fire_init(prm, dt0)
+        if state is not None:
+            vel, dt, alpha, nPos = state
+            s.vel.upload(np.concatenate(vel))
+            s.ctx.call("pcp_fire_set", dt.ctypes.data_as(capi._p), alpha.ctypes.data_as(capi._p),
+                       nPos.ctypes.data_as(capi._p))
+        res.contexts.append(len(idx))
+        done = np.zeros(len(idx), bool)
+        local = 0
+        while True:
+            st = s.fire_step(prm)
+            local += 1
+            for k in range(len(idx)):
+                if not done[k] and st[k].converged:
+                    done[k] = True
+                    res.steps[idx[k]] = offset[idx[k]] + st[k].steps
+            n_run = int((~done).sum())
+            out_of_steps = offset[idx] + local >= maxSteps
+            if n_run == 0 or np.all(done | out_of_steps):
+                break
+            if len(idx) > min_context and n_run < compact_below * len(idx):
+                break
+        x = s.pos.get()
+        E = s.get_energy()
+        F = s.max_abs_force()
+        for k in range(len(idx)):
+            a, b = s.vertex_range(k)
+            g = idx[k]
+            res.positions[g] = x[2 * a:2 * b].copy()
+            res.energy[g] = E[k]
+            res.max_force[g] = F[k]
+        keep = np.where(~done & ~(offset[idx] + local >= maxSteps))[0]
+        if log:
+            log(f"[batch] context of {len(idx)} packings: {local} steps, {int(done.sum())} converged, "
+                f"{len(keep)} continue")
+        if len(keep) == 0:
+            break
+        v = s.vel.get()
+        st_k = [st[k] for k in keep]
+        state = ([v[2 * s.vertex_range(k)[0]:2 * s.vertex_range(k)[1]].copy() for k in keep],
+                 np.array([q.dt for q in st_k], np.float64), np.array([q.alpha for q in st_k], np.float64),
+                 np.array([q.nPos for q in st_k], np.int32))
+        for k in keep:
+            offset[idx[k]] += st[k].steps
+        cur = [_with_positions(packings[idx[k]], res.positions[idx[k]]) for k in keep]
+        idx = idx[keep]
+        s.close()
+    res.seconds = time.perf_counter() - t0
+    return res
