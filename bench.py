@@ -44,7 +44,7 @@ BYTES_PER_CROSSING = 24
 DETECT_KERNEL = "k_detect_poly<0>"
 # HBM bytes per k_detect_poly<0> launch on this workload, measured with rocprofv3 FETCH_SIZE
 # and WRITE_SIZE and scaled by a copy kernel's readings (profiles/k_detect_poly_hbm_traffic_N4096.md)
-DETECT_TRAFFIC_BYTES = 4047258
+DETECT_TRAFFIC_BYTES = 4228712
 
 
 def cpu_baseline(p, B, budget_s):
@@ -216,10 +216,13 @@ def main():
     s.ctx.call("pcp_profile_read", ctypes.byref(ms), ctypes.byref(nl))
     s.ctx.call("pcp_profile_kernel", None)
     evals_per_s = args.steps / el
+    # the profiled launch carries its own start/stop events (recorded by the dispatch), so this
+    # is the kernel's duration on the context's stream
     kavg_s = (ms.value / max(nl.value, 1)) * 1e-3
-    print(f"[bench] {args.steps} evals in {el:.3f} s; {args.roofline_kernel}: {nl.value} launches, "
-          f"avg {kavg_s * 1e6:.2f} us; records {nI}; polygon pairs {ppairs.value}; edge-pair tests {tests}",
-          flush=True)
+
+    print(f"[bench] {args.steps} evals in {el:.3f} s; {args.roofline_kernel}: {nl.value} launches, avg "
+          f"{kavg_s * 1e6:.2f} us (launch events); records {nI}; polygon pairs {ppairs.value}; edge-pair tests "
+          f"{tests}", flush=True)
 
     if args.roofline_kernel == DETECT_KERNEL and nl.value > 0:
         flops = FLOPS_PER_EDGE_PAIR * tests
@@ -229,7 +232,7 @@ def main():
                     achieved=achieved, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s", frac=achieved / FP64_PEAK_TFLOPS,
                     traffic=DETECT_TRAFFIC_BYTES, kernel=DETECT_KERNEL, flops_per_launch=flops, edge_pair_tests=tests,
                     polygon_pairs=ppairs.value, algorithmic_bytes_per_launch=nbytes, avg_launch_us=kavg_s * 1e6,
-                    hbm_gbs_algorithmic=nbytes / kavg_s / 1e9)
+                    hbm_gbs_algorithmic=nbytes / kavg_s / 1e9, hbm_gbs_measured_traffic=DETECT_TRAFFIC_BYTES / kavg_s / 1e9)
     else:
         roof = dict(kernel=args.roofline_kernel, avg_launch_us=kavg_s * 1e6, achieved=None, peak=None,
                     unit=None, frac=None, traffic=None, bound=None)

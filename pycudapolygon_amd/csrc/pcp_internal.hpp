@@ -1,6 +1,7 @@
 // Internal context of libpcp_hip.so (not part of the C ABI).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <stdint.h>
 #include <string>
 #include <vector>
@@ -164,15 +165,22 @@ int run_shake_seg(pcp_ctx* c, double2* pos, const double* targetEdgeV, const dou
                   double tol, int retryIter, const int* active, int* itOut);
 int prof_begin(pcp_ctx* c, const char* name, int* slot);
 int prof_end(pcp_ctx* c, int slot);
+int prof_slot(pcp_ctx* c, const char* name, int* slot);
 }  // namespace pcp
 
-// launch `kern` on the context stream, bracketed with events when it is the profiled kernel
+// launch `kern` on the context stream; when it is the profiled kernel, the launch carries a
+// start and a stop event that the dispatch itself records (hipExtLaunchKernelGGL), so their
+// elapsed time is the kernel's own duration, without the dispatch latency an event recorded
+// before the launch would add
 #define PCP_LAUNCH(c, kern, grid, block, ...) PCP_LAUNCH_LDS(c, kern, grid, block, 0, __VA_ARGS__)
-#define PCP_LAUNCH_LDS(c, kern, grid, block, lds, ...)                                 \
-    do {                                                                               \
-        int _slot = -1;                                                                \
-        PCP_TRY(::pcp::prof_begin((c), #kern, &_slot));                                \
-        hipLaunchKernelGGL(kern, grid, block, lds, (c)->stream, __VA_ARGS__);          \
-        PCP_LAUNCHED();                                                                \
-        PCP_TRY(::pcp::prof_end((c), _slot));                                          \
+#define PCP_LAUNCH_LDS(c, kern, grid, block, lds, ...)                                          \
+    do {                                                                                        \
+        int _slot = -1;                                                                         \
+        PCP_TRY(::pcp::prof_slot((c), #kern, &_slot));                                          \
+        if (_slot >= 0)                                                                         \
+            hipExtLaunchKernelGGL(kern, grid, block, lds, (c)->stream, (c)->profEvents[_slot].first, \
+                                  (c)->profEvents[_slot].second, 0, __VA_ARGS__);               \
+        else                                                                                    \
+            hipLaunchKernelGGL(kern, grid, block, lds, (c)->stream, __VA_ARGS__);               \
+        PCP_LAUNCHED();                                                                         \
     } while (0)

@@ -1047,6 +1047,19 @@ int prof_end(pcp_ctx* c, int slot) {
     PCP_HIP(hipEventRecord(c->profEvents[slot].second, c->stream));
     return PCP_OK;
 }
+// an event pair for the next launch of the profiled kernel, recorded by the launch itself
+int prof_slot(pcp_ctx* c, const char* name, int* slot) {
+    *slot = -1;
+    if (c->profName.empty() || c->profName != name) return PCP_OK;
+    if (c->profUsed == c->profEvents.size()) {
+        hipEvent_t a, b;
+        PCP_HIP(hipEventCreate(&a));
+        PCP_HIP(hipEventCreate(&b));
+        c->profEvents.push_back({a, b});
+    }
+    *slot = (int)c->profUsed++;
+    return PCP_OK;
+}
 
 // records whether the evaluation just enqueued overflowed a record bucket
 __global__ void k_sticky(const int* __restrict__ scal, int Kp, int* __restrict__ sticky) {
