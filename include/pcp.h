@@ -214,6 +214,16 @@ int pcp_fire_step(pcp_ctx* ctx, const pcp_state* st, const pcp_fire_params* prm,
 /* FIRE until every packing has max|F| <= fTol or maxSteps steps; status host[P] */
 int pcp_fire_minimize(pcp_ctx* ctx, const pcp_state* st, const pcp_fire_params* prm, int maxSteps,
                       pcp_fire_status* status);
+/* FIRE until every packing has max|F| <= fTol or maxSteps steps, in blocks of stepsPerBlock
+ * steps queued without host round trips (the reference's minimizeFIRE loop, model.cpp:488-504,
+ * reads back after every step).  A rolled-back packing takes the step's start x, F and energy
+ * from the step's backups instead of a fresh evaluation (model.cpp:454-464; the same values,
+ * evaluations being deterministic).  A block in which a record bucket overflowed is undone
+ * from a block-start snapshot and run again with larger buckets.  useGraph != 0: the launches
+ * of one block are captured once in a HIP graph and replayed (captured again when the block
+ * length, bucket capacity, parameters or state buffers change).  status host[P] (optional). */
+int pcp_fire_run(pcp_ctx* ctx, const pcp_state* st, const pcp_fire_params* prm, int maxSteps, int stepsPerBlock,
+                 int useGraph, pcp_fire_status* status);
 /* set the per-packing FIRE scalars (host[P] arrays, NULL = leave) and mark every packing
  * active with a zero step count; read them back */
 int pcp_fire_set(pcp_ctx* ctx, const double* dt, const double* alpha, const int* nPos);

@@ -33,12 +33,14 @@ def _with_positions(p, x):
 
 
 def minimize_batch(packings, model, fire_kwargs, maxSteps, dt0=1e-3, compact_below=0.5, min_context=8,
-                   boxSizes=None, log=None, warm=None):
+                   boxSizes=None, log=None, warm=None, block=8, graph=True):
     """FIRE on `packings` until each has max|F| <= fTol or maxSteps steps of its own.
     compact_below: re-pack when fewer than this fraction of the context's packings are
     still running (and the context holds more than min_context packings).
     warm: a warmstart.WarmStart applied once to the first context before FIRE starts
-    (its time is included in the result's seconds)."""
+    (its time is included in the result's seconds).  FIRE runs in blocks of `block` steps
+    (pcp_fire_run, replayed from a HIP graph when graph is set); the batch is checked for
+    convergence and compaction between blocks."""
     P = len(packings)
     res = BatchResult(P)
     if boxSizes is None:
@@ -63,16 +65,19 @@ def minimize_batch(packings, model, fire_kwargs, maxSteps, dt0=1e-3, compact_bel
         res.contexts.append(len(idx))
         done = np.zeros(len(idx), bool)
         local = 0
+        start = int(offset[idx].max())    # every running packing has run the same number of steps
         while True:
-            st = s.fire_step(prm)
-            local += 1
+            K = min(block, maxSteps - start - local)
+            if K <= 0:
+                break
+            st = s.fire_run(prm, K, K, graph and K == block)
+            local += K
             for k in range(len(idx)):
                 if not done[k] and st[k].converged:
                     done[k] = True
                     res.steps[idx[k]] = offset[idx[k]] + st[k].steps
             n_run = int((~done).sum())
-            out_of_steps = offset[idx] + local >= maxSteps
-            if n_run == 0 or np.all(done | out_of_steps):
+            if n_run == 0 or start + local >= maxSteps:
                 break
             if len(idx) > min_context and n_run < compact_below * len(idx):
                 break
@@ -85,7 +90,7 @@ def minimize_batch(packings, model, fire_kwargs, maxSteps, dt0=1e-3, compact_bel
             res.positions[g] = x[2 * a:2 * b].copy()
             res.energy[g] = E[k]
             res.max_force[g] = F[k]
-        keep = np.where(~done & ~(offset[idx] + local >= maxSteps))[0]
+        keep = np.where(~done & ~(start + local >= maxSteps))[0]
         if log:
             log(f"[batch] context of {len(idx)} packings: {local} steps, {int(done.sum())} converged, "
                 f"{len(keep)} continue")

@@ -77,6 +77,7 @@ SIGNATURES = {
     "pcp_fire_init": (_i, [_p, ctypes.POINTER(PcpState), ctypes.POINTER(FireParams), _d]),
     "pcp_fire_step": (_i, [_p, ctypes.POINTER(PcpState), ctypes.POINTER(FireParams), _p]),
     "pcp_fire_minimize": (_i, [_p, ctypes.POINTER(PcpState), ctypes.POINTER(FireParams), _i, _p]),
+    "pcp_fire_run": (_i, [_p, ctypes.POINTER(PcpState), ctypes.POINTER(FireParams), _i, _i, _i, _p]),
     "pcp_fire_set": (_i, [_p, _p, _p, _p]),
     "pcp_fire_get": (_i, [_p, _p]),
     "pcp_profile_kernel": (_i, [_p, ctypes.c_char_p]),
@@ -366,6 +367,13 @@ class System:
         st = (FireStatus * self.P)()
         check(lib().pcp_fire_minimize(self.ctx.h, ctypes.byref(self._state), ctypes.byref(prm),
                                       int(maxSteps), st))
+        return list(st)
+
+    def fire_run(self, prm, maxSteps, stepsPerBlock=16, graph=True):
+        """FIRE in blocks of stepsPerBlock steps with one status read per block (pcp_fire_run)"""
+        st = (FireStatus * self.P)()
+        check(lib().pcp_fire_run(self.ctx.h, ctypes.byref(self._state), ctypes.byref(prm), int(maxSteps),
+                                 int(stepsPerBlock), 1 if graph else 0, st))
         return list(st)
 
     def close(self):

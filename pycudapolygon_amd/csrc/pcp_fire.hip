@@ -17,7 +17,7 @@ namespace pcp {
 enum { FL_FAIL = 0, FL_ROLLBACK = 1, FL_ACCEPT = 2, FL_ZEROV = 3, FL_LASTSHAKE = 4, FL_ACTIVEBAK = 5,
        FL_NF = 8 };
 enum { FS_SCALE = 0, FS_ONEMA = 1, FS_MAXF = 2, FS_DTB = 3, FS_ALB = 4, FS_NPB = 5, FS_STEPB = 6,
-       FS_NF = 16 };
+       FS_MAXFB = 7, FS_NF = 16 };
 
 __global__ void k_fire_init(int P, double dtv, double alpha0, double* dt, double* alpha, int* nPos,
                             int* steps, int* active, int* flags, double* fs) {
@@ -42,7 +42,7 @@ __global__ void k_fire_begin(int V, int P, const int* __restrict__ shapeId, cons
                              double* __restrict__ epre, int* __restrict__ flags, double* __restrict__ fs,
                              int* __restrict__ shakeIt, int* __restrict__ summary, double2* __restrict__ x,
                              double2* __restrict__ v, const double2* __restrict__ F, double2* __restrict__ xbak,
-                             double2* __restrict__ vbak, double2* __restrict__ xpre) {
+                             double2* __restrict__ vbak, double2* __restrict__ xpre, double2* __restrict__ fbak) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < 4) summary[i] = 0;
     if (i < P) {
@@ -61,6 +61,7 @@ __global__ void k_fire_begin(int V, int P, const int* __restrict__ shapeId, cons
             fs[FS_NF * p + FS_ALB] = alpha[p];
             fs[FS_NF * p + FS_NPB] = (double)nPos[p];
             fs[FS_NF * p + FS_STEPB] = (double)steps[p];
+            fs[FS_NF * p + FS_MAXFB] = fs[FS_NF * p + FS_MAXF];
         }
     }
     if (i >= V) return;
@@ -70,6 +71,7 @@ __global__ void k_fire_begin(int V, int P, const int* __restrict__ shapeId, cons
     double2 xi = x[i], vi = v[i], fi = F[i];
     xbak[i] = xi;
     vbak[i] = vi;
+    if (fbak) fbak[i] = fi;
     double vhx = vi.x + (0.5 * h) * fi.x;
     double vhy = vi.y + (0.5 * h) * fi.y;
     double px = xi.x + h * vhx, py = xi.y + h * vhy;
@@ -273,7 +275,8 @@ __device__ void fire_control2(int p, const int* __restrict__ active, double Pw, 
 __global__ void k_fire_bend(int V, const int* __restrict__ shapeId, const int* __restrict__ polyPack,
                             const int* __restrict__ active, const int* __restrict__ flags,
                             const double* __restrict__ fs, double2* __restrict__ x, double2* __restrict__ v,
-                            const double2* __restrict__ F, const double2* __restrict__ xbak) {
+                            double2* __restrict__ F, const double2* __restrict__ xbak,
+                            const double2* __restrict__ fbak) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= V) return;
     int p = polyPack[shapeId[i]];
@@ -290,6 +293,35 @@ __global__ void k_fire_bend(int V, const int* __restrict__ shapeId, const int* _
     } else if (fl[FL_ROLLBACK]) {
         x[i] = xbak[i];
         v[i] = make_double2(0.0, 0.0);
+        if (fbak) F[i] = fbak[i];
+    }
+}
+
+// block-start snapshot of the FIRE state (save = 1) or its restoration (save = 0)
+__global__ void k_fire_snapshot(int V, int P, int save, double2* __restrict__ x, double2* __restrict__ v,
+                                double2* __restrict__ F, double* __restrict__ dt, double* __restrict__ alpha,
+                                double* __restrict__ energy, double* __restrict__ fs, int* __restrict__ nPos,
+                                int* __restrict__ steps, int* __restrict__ active, double2* __restrict__ sx,
+                                double2* __restrict__ sv, double2* __restrict__ sf, double* __restrict__ sd,
+                                int* __restrict__ si) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < V) {
+        if (save) {
+            sx[i] = x[i]; sv[i] = v[i]; sf[i] = F[i];
+        } else {
+            x[i] = sx[i]; v[i] = sv[i]; F[i] = sf[i];
+        }
+    }
+    if (i < P) {
+        double* d = sd + 4LL * i;
+        int* k = si + 3LL * i;
+        if (save) {
+            d[0] = dt[i]; d[1] = alpha[i]; d[2] = energy[i]; d[3] = fs[FS_NF * i + FS_MAXF];
+            k[0] = nPos[i]; k[1] = steps[i]; k[2] = active[i];
+        } else {
+            dt[i] = d[0]; alpha[i] = d[1]; energy[i] = d[2]; fs[FS_NF * i + FS_MAXF] = d[3];
+            nPos[i] = k[0]; steps[i] = k[1]; active[i] = k[2];
+        }
     }
 }
 
@@ -334,9 +366,10 @@ __global__ __launch_bounds__(BLK) void k_fire_stats(int N, const int* __restrict
                                                    double dtMax, double alphaStart, double fAlpha, double fInc,
                                                    double fDec, double fTol, int* __restrict__ active,
                                                    int* __restrict__ flags, double* dt, double* alpha, int* nPos,
-                                                   int* __restrict__ steps, const double* __restrict__ energy,
+                                                   int* __restrict__ steps, double* __restrict__ energy,
                                                    double* __restrict__ fs, pcp_fire_status* __restrict__ status,
-                                                   int* __restrict__ summary, const int* __restrict__ scal) {
+                                                   int* __restrict__ summary, const int* __restrict__ scal,
+                                                   const double* __restrict__ epre, int* __restrict__ blockOver) {
     __shared__ double L[BLK / 64];
     __shared__ double part[3 * (BLK + 1)];
     int p = blockIdx.x;
@@ -355,8 +388,17 @@ __global__ __launch_bounds__(BLK) void k_fire_stats(int N, const int* __restrict
         if (full)
             fire_control2(p, active, Pw, vv, ff, nMin, dtMax, alphaStart, fAlpha, fInc, fDec, dt, alpha, nPos,
                           flags, fs);
+        // device-side rollback (epre given): the step's start state comes back from its
+        // backups -- energy here, x, v and F in k_fire_bend -- as re-evaluating it would give
+        if (full && epre && active[p] && flags[FL_NF * p + FL_ROLLBACK]) {
+            energy[p] = epre[p];
+            mf = fs[FS_NF * p + FS_MAXFB];
+        }
         fire_finish_p(p, active, flags, mf, fTol, steps, dt, alpha, nPos, energy, fs, status, summary, full, full);
-        if (p == 0) summary[2] = scal[SC_BUCKET];
+        if (p == 0) {
+            summary[2] = scal[SC_BUCKET];
+            if (blockOver && scal[SC_BUCKET] > blockOver[0]) blockOver[0] = scal[SC_BUCKET];
+        }
     }
 }
 
@@ -448,24 +490,33 @@ static int fire_read_status(pcp_ctx* c) {
 }
 static const int* host_summary(pcp_ctx* c) { return (const int*)c->hStatusBlock.data(); }
 
+// Launches of one FIRE step (no host interaction).  block = 1: device-side rollback (the
+// step's start state comes back from its backups) and the block's largest bucket count is
+// kept in dBlockOver, so several steps can be queued without reading the status back.
+static int fire_step_launch(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm, int block) {
+    double2* fbak = block ? (double2*)c->dFbak : nullptr;
+    PCP_LAUNCH(c, k_fire_begin, dim3(nblk(std::max(std::max(c->V, c->P), 4))), dim3(BLK), c->V, c->P, c->dShapeId,
+               c->dPolyPack, c->dActive, c->dDt, c->dAlpha, c->dNPos, c->dSteps, st->energy, c->dEpre, c->dFlags,
+               c->dFireScal, c->dShakeIt, c->dSummary, (double2*)st->pos, (double2*)st->vel,
+               (const double2*)st->force, (double2*)c->dXbak, (double2*)c->dVbak, (double2*)c->dXpre, fbak);
+    PCP_TRY(fire_eval(c, st, prm));
+    PCP_TRY(launch_fire_kick(c, (double2*)st->vel, (const double2*)st->force));
+    PCP_LAUNCH(c, k_fire_stats, dim3(c->P), dim3(BLK), c->N, c->dPackPoly, c->dRed, 1, prm->nMin, prm->dtMax,
+               prm->alphaStart, prm->fAlpha, prm->fInc, prm->fDec, prm->fTol, c->dActive, c->dFlags, c->dDt, c->dAlpha,
+               c->dNPos, c->dSteps, st->energy, c->dFireScal, c->dStatus, c->dSummary, c->dScal,
+               block ? (const double*)c->dEpre : nullptr, block ? c->dBlockOver : nullptr);
+    PCP_LAUNCH(c, k_fire_bend, dim3(nblk(c->V)), dim3(BLK), c->V, c->dShapeId, c->dPolyPack, c->dActive, c->dFlags,
+               c->dFireScal, (double2*)st->pos, (double2*)st->vel, (double2*)st->force, (const double2*)c->dXbak,
+               (const double2*)fbak);
+    return PCP_OK;
+}
+
 // one step; *anyActive receives whether some packing is still unconverged
 static int fire_step_impl(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm, int* anyActive) {
     int mt0 = st->model & 0xFF;
     bool overlap = (mt0 == PCP_NORMAL || mt0 == PCP_HYBRID);
     for (int attempt = 0; attempt < 6; ++attempt) {
-        PCP_LAUNCH(c, k_fire_begin, dim3(nblk(std::max(std::max(c->V, c->P), 4))), dim3(BLK), c->V, c->P,
-                   c->dShapeId, c->dPolyPack, c->dActive, c->dDt, c->dAlpha, c->dNPos, c->dSteps, st->energy,
-                   c->dEpre, c->dFlags, c->dFireScal, c->dShakeIt, c->dSummary, (double2*)st->pos,
-                   (double2*)st->vel, (const double2*)st->force, (double2*)c->dXbak, (double2*)c->dVbak,
-                   (double2*)c->dXpre);
-        PCP_TRY(fire_eval(c, st, prm));
-        PCP_TRY(launch_fire_kick(c, (double2*)st->vel, (const double2*)st->force));
-        PCP_LAUNCH(c, k_fire_stats, dim3(c->P), dim3(BLK), c->N, c->dPackPoly, c->dRed, 1, prm->nMin, prm->dtMax,
-                   prm->alphaStart, prm->fAlpha, prm->fInc, prm->fDec, prm->fTol, c->dActive, c->dFlags, c->dDt,
-                   c->dAlpha, c->dNPos, c->dSteps, st->energy, c->dFireScal, c->dStatus, c->dSummary, c->dScal);
-        PCP_LAUNCH(c, k_fire_bend, dim3(nblk(c->V)), dim3(BLK), c->V, c->dShapeId, c->dPolyPack, c->dActive,
-                   c->dFlags, c->dFireScal, (double2*)st->pos, (double2*)st->vel, (const double2*)st->force,
-                   (const double2*)c->dXbak);
+        PCP_TRY(fire_step_launch(c, st, prm, 0));
         PCP_TRY(fire_read_status(c));
         const int* summary = host_summary(c);
         if (overlap && summary[2] > c->Kp) {
@@ -490,7 +541,7 @@ static int fire_step_impl(pcp_ctx* c, const pcp_state* st, const pcp_fire_params
             PCP_LAUNCH(c, k_fire_stats, dim3(c->P), dim3(BLK), c->N, c->dPackPoly, c->dRed, 0, prm->nMin,
                        prm->dtMax, prm->alphaStart, prm->fAlpha, prm->fInc, prm->fDec, prm->fTol, c->dActive,
                        c->dFlags, c->dDt, c->dAlpha, c->dNPos, c->dSteps, st->energy, c->dFireScal, c->dStatus,
-                       c->dSummary, c->dScal);
+                       c->dSummary, c->dScal, (const double*)nullptr, (int*)nullptr);
             PCP_TRY(fire_read_status(c));
             summary = host_summary(c);
         }
@@ -498,6 +549,82 @@ static int fire_step_impl(pcp_ctx* c, const pcp_state* st, const pcp_fire_params
         return PCP_OK;
     }
     set_error("pcp_fire_step: record buckets kept overflowing");
+    return PCP_ERR_CAPACITY;
+}
+
+static int fire_snapshot(pcp_ctx* c, const pcp_state* st, int save) {
+    PCP_LAUNCH(c, k_fire_snapshot, dim3(nblk(std::max(c->V, c->P))), dim3(BLK), c->V, c->P, save, (double2*)st->pos,
+               (double2*)st->vel, (double2*)st->force, c->dDt, c->dAlpha, st->energy, c->dFireScal, c->dNPos,
+               c->dSteps, c->dActive, (double2*)c->dSnapX, (double2*)c->dSnapV, (double2*)c->dSnapF, c->dSnapD,
+               c->dSnapI);
+    return PCP_OK;
+}
+
+// K steps queued back to back: replayed from a captured graph of the K-step launch sequence
+// (captured again when K, the bucket capacity, the parameters or the state buffers change)
+// or launched directly
+static int fire_block_launch(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm, int K, int useGraph) {
+    if (!useGraph || !c->profName.empty()) {
+        for (int k = 0; k < K; ++k) PCP_TRY(fire_step_launch(c, st, prm, 1));
+        return PCP_OK;
+    }
+    bool valid = c->fireGraph && c->graphK == K && c->graphKp == c->Kp &&
+                 memcmp(&c->graphPrm, prm, sizeof(*prm)) == 0 && memcmp(&c->graphSt, st, sizeof(*st)) == 0;
+    if (!valid) {
+        if (c->fireGraph) PCP_HIP(hipGraphExecDestroy(c->fireGraph));
+        c->fireGraph = nullptr;
+        PCP_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = PCP_OK;
+        for (int k = 0; k < K && rc == PCP_OK; ++k) rc = fire_step_launch(c, st, prm, 1);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc != PCP_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail_hip(e, "hipStreamEndCapture", __FILE__, __LINE__);
+        e = hipGraphInstantiate(&c->fireGraph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            c->fireGraph = nullptr;
+            return fail_hip(e, "hipGraphInstantiate", __FILE__, __LINE__);
+        }
+        c->graphK = K;
+        c->graphKp = c->Kp;
+        c->graphPrm = *prm;
+        c->graphSt = *st;
+    }
+    PCP_HIP(hipGraphLaunch(c->fireGraph, c->stream));
+    return PCP_OK;
+}
+
+// one block of K steps with one status read; a block in which a record bucket overflowed is
+// undone from the block-start snapshot and run again with larger buckets
+static int fire_block(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm, int K, int useGraph,
+                      int* anyActive) {
+    int mt0 = st->model & 0xFF;
+    bool overlap = (mt0 == PCP_NORMAL || mt0 == PCP_HYBRID);
+    for (int attempt = 0; attempt < 6; ++attempt) {
+        PCP_TRY(fire_snapshot(c, st, 1));
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->dBlockOver, 1LL, 0);
+        PCP_LAUNCHED();
+        PCP_TRY(fire_block_launch(c, st, prm, K, useGraph));
+        int over = 0;
+        PCP_HIP(hipMemcpyAsync(&over, c->dBlockOver, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        PCP_TRY(fire_read_status(c));   // synchronizes
+        const int* summary = host_summary(c);
+        if (overlap && over > c->Kp) {
+            PCP_TRY(fire_snapshot(c, st, 0));
+            PCP_TRY(grow_buckets(c, over));
+            PCP_TRY(pcp_force_eval(c, st));
+            PCP_HIP(hipMemsetAsync(c->dSticky, 0, sizeof(int), c->stream));
+            PCP_HIP(hipStreamSynchronize(c->stream));
+            continue;
+        }
+        if (anyActive) *anyActive = summary[1];
+        return PCP_OK;
+    }
+    set_error("pcp_fire_run: record buckets kept overflowing");
     return PCP_ERR_CAPACITY;
 }
 
@@ -531,6 +658,27 @@ int pcp_fire_minimize(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* pr
     for (int step = 0; step < maxSteps; ++step) {
         int any = 1;
         PCP_TRY(fire_step_impl(c, st, prm, &any));
+        if (!any) break;
+    }
+    if (status) {
+        PCP_TRY(fire_read_status(c));
+        memcpy(status, c->hStatusBlock.data() + 16, sizeof(pcp_fire_status) * c->P);
+    }
+    return PCP_OK;
+}
+
+int pcp_fire_run(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm, int maxSteps, int stepsPerBlock,
+                 int useGraph, pcp_fire_status* status) {
+    if (stepsPerBlock < 1 || stepsPerBlock > 256) {
+        set_error("pcp_fire_run: stepsPerBlock must be in 1..256");
+        return PCP_ERR_ARG;
+    }
+    int done = 0;
+    while (done < maxSteps) {
+        int K = std::min(stepsPerBlock, maxSteps - done);
+        int any = 1;
+        PCP_TRY(fire_block(c, st, prm, K, useGraph && K == stepsPerBlock, &any));
+        done += K;
         if (!any) break;
     }
     if (status) {

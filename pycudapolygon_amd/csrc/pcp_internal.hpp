@@ -133,6 +133,17 @@ struct pcp_ctx {
     pcp_fire_status* dStatus = nullptr;   // inside dStatusBlock
     int* dSummary = nullptr;       // inside dStatusBlock: [0] any rollback, [1] any unconverged,
                                    // [2] largest record bucket count of the step
+    // multi-step FIRE (pcp_fire_run): force backups for device-side rollback, a block-start
+    // snapshot (x, v, F; per packing dt, alpha, energy, max|F| and nPos, steps, active), the
+    // largest bucket count seen in the block, and the captured graph of one block of steps
+    double* dFbak = nullptr;
+    double *dSnapX = nullptr, *dSnapV = nullptr, *dSnapF = nullptr, *dSnapD = nullptr;
+    int* dSnapI = nullptr;
+    int* dBlockOver = nullptr;
+    hipGraphExec_t fireGraph = nullptr;
+    int graphK = 0, graphKp = 0;
+    pcp_fire_params graphPrm{};
+    pcp_state graphSt{};
     std::string profName;          // kernel bracketed with events (measurement)
     std::vector<std::pair<hipEvent_t, hipEvent_t>> profEvents;
     size_t profUsed = 0;

@@ -1138,6 +1138,10 @@ static void free_topology(pcp_ctx* c) {
     dfree(c->dEpoly); dfree(c->dWork); dfree(c->dRed);
     dfree(c->dRedOut); dfree(c->dScanPart); c->scanPartCap = 0;
     dfree(c->dXbak); dfree(c->dXpre); dfree(c->dVbak); dfree(c->dDt); dfree(c->dAlpha);
+    dfree(c->dFbak); dfree(c->dSnapX); dfree(c->dSnapV); dfree(c->dSnapF); dfree(c->dSnapD); dfree(c->dSnapI);
+    dfree(c->dBlockOver);
+    if (c->fireGraph) (void)hipGraphExecDestroy(c->fireGraph);
+    c->fireGraph = nullptr;
     dfree(c->dEpre); dfree(c->dNPos); dfree(c->dSteps); dfree(c->dActive); dfree(c->dFlags);
     dfree(c->dFireScal); dfree(c->dShakeIt); dfree(c->dCellInv); dfree(c->dCom); dfree(c->dStatusBlock);
     c->dStatus = nullptr; c->dSummary = nullptr;
@@ -1239,6 +1243,9 @@ int pcp_set_topology(pcp_ctx* c, int P, const int* packPolyStart, const int* pol
     PCP_TRY(dalloc(&c->dAccR, N)); PCP_TRY(dalloc(&c->dEpoly, N));
     PCP_TRY(dalloc(&c->dWork, 8LL * V)); PCP_TRY(dalloc(&c->dRed, 4LL * N)); PCP_TRY(dalloc(&c->dRedOut, 4LL * P));
     PCP_TRY(dalloc(&c->dXbak, 2LL * V)); PCP_TRY(dalloc(&c->dXpre, 2LL * V)); PCP_TRY(dalloc(&c->dVbak, 2LL * V));
+    PCP_TRY(dalloc(&c->dFbak, 2LL * V)); PCP_TRY(dalloc(&c->dSnapX, 2LL * V)); PCP_TRY(dalloc(&c->dSnapV, 2LL * V));
+    PCP_TRY(dalloc(&c->dSnapF, 2LL * V)); PCP_TRY(dalloc(&c->dSnapD, 4LL * P)); PCP_TRY(dalloc(&c->dSnapI, 3LL * P));
+    PCP_TRY(dalloc(&c->dBlockOver, 1));
     PCP_TRY(dalloc(&c->dDt, P)); PCP_TRY(dalloc(&c->dAlpha, P)); PCP_TRY(dalloc(&c->dEpre, P));
     PCP_TRY(dalloc(&c->dNPos, P)); PCP_TRY(dalloc(&c->dSteps, P)); PCP_TRY(dalloc(&c->dActive, P));
     PCP_TRY(dalloc(&c->dFlags, 8LL * P)); PCP_TRY(dalloc(&c->dFireScal, 16LL * P));
