@@ -237,6 +237,18 @@ int pcp_fire_get(pcp_ctx* ctx, pcp_fire_status* status);
 int pcp_profile_kernel(pcp_ctx* ctx, const char* name);
 int pcp_profile_read(pcp_ctx* ctx, double* totalMs, int* launches);
 int pcp_stream_copy(pcp_ctx* ctx, double* dst, const double* src, long long n);
+/* Schedules of the per-record kernels (no counterpart in the reference): lanes per polygon
+ * of the pairing (k_pair_poly), lens-area (k_lens_energy) and exterior-force (k_exterior_poly)
+ * kernels -- 16, 32 or 64, i.e. 4, 2 or 1 polygons per wavefront; exterior 0 = as many
+ * polygons per wavefront as the LDS allows.  Every schedule gives identical results (each
+ * lane keeps its sums in order); only the speed differs. */
+int pcp_set_schedule(pcp_ctx* ctx, int pairLanes, int lensLanes, int exteriorLanes);
+int pcp_get_schedule(pcp_ctx* ctx, int* pairLanes, int* lensLanes, int* exteriorLanes);
+/* autotuner: one evaluation of st fills the context's buffers, then every candidate width of
+ * each kernel is timed over `reps` back-to-back launches and the fastest is kept.
+ * timesUs host[9] (optional): microseconds per launch of pair 16/32/64, lens 16/32/64,
+ * exterior 16/32/64 (0 = not a candidate). */
+int pcp_autotune(pcp_ctx* ctx, const pcp_state* st, int reps, double* timesUs);
 /* work of the last detection: polygon pairs tested (each polygon with itself, each pair
  * of distinct polygons whose bounding circles meet once) and the edge pairs whose segment
  * test ran (n(n-3)/2 per polygon with itself; per pair of distinct polygons the product

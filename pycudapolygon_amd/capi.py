@@ -84,6 +84,9 @@ SIGNATURES = {
     "pcp_profile_read": (_i, [_p, _p, _p]),
     "pcp_stream_copy": (_i, [_p, _p, _p, ctypes.c_longlong]),
     "pcp_detect_stats": (_i, [_p, _p, _p]),
+    "pcp_set_schedule": (_i, [_p, _i, _i, _i]),
+    "pcp_get_schedule": (_i, [_p, _p, _p, _p]),
+    "pcp_autotune": (_i, [_p, ctypes.POINTER(PcpState), _i, _p]),
     "pcp_mpnn_sizes": (_i, [_i, _p, _p]),
     "pcp_mpnn_graph": (_i, [_p, _p, _p]),
     "pcp_mpnn_features": (_i, [_p, ctypes.POINTER(PcpState), _p, _p, _i, _p, _p]),
@@ -368,6 +371,21 @@ class System:
         check(lib().pcp_fire_minimize(self.ctx.h, ctypes.byref(self._state), ctypes.byref(prm),
                                       int(maxSteps), st))
         return list(st)
+
+    def set_schedule(self, pair=16, lens=16, exterior=0):
+        self.ctx.call("pcp_set_schedule", int(pair), int(lens), int(exterior))
+
+    def get_schedule(self):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self.ctx.call("pcp_get_schedule", ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return a.value, b.value, c.value
+
+    def autotune(self, reps=20):
+        """time every lanes-per-polygon candidate of the per-record kernels, keep the fastest;
+        returns (schedule, microseconds per launch of pair/lens/exterior x 16/32/64)"""
+        t = np.zeros(9)
+        check(lib().pcp_autotune(self.ctx.h, ctypes.byref(self._state), int(reps), _ptr(t)))
+        return self.get_schedule(), t.reshape(3, 3)
 
     def fire_run(self, prm, maxSteps, stepsPerBlock=16, graph=True):
         """FIRE in blocks of stepsPerBlock steps with one status read per block (pcp_fire_run)"""

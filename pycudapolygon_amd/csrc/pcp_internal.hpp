@@ -133,6 +133,10 @@ struct pcp_ctx {
     pcp_fire_status* dStatus = nullptr;   // inside dStatusBlock
     int* dSummary = nullptr;       // inside dStatusBlock: [0] any rollback, [1] any unconverged,
                                    // [2] largest record bucket count of the step
+    // lanes per polygon of the per-record kernels (pcp_set_schedule / pcp_autotune):
+    // k_pair_poly, k_lens_energy (16, 32 or 64) and k_exterior_poly (0 = the widest packing
+    // of polygons per wavefront the LDS allows)
+    int segPair = 16, segLens = 16, segExt = 0;
     // multi-step FIRE (pcp_fire_run): force backups for device-side rollback, a block-start
     // snapshot (x, v, F; per packing dt, alpha, energy, max|F| and nPos, steps, active), the
     // largest bucket count seen in the block, and the captured graph of one block of steps
@@ -177,6 +181,7 @@ int run_shake_seg(pcp_ctx* c, double2* pos, const double* targetEdgeV, const dou
 int prof_begin(pcp_ctx* c, const char* name, int* slot);
 int prof_end(pcp_ctx* c, int slot);
 int prof_slot(pcp_ctx* c, const char* name, int* slot);
+int autotune(pcp_ctx* c, const pcp_state* st, int reps, double* timesUs);
 }  // namespace pcp
 
 // launch `kern` on the context stream; when it is the profiled kernel, the launch carries a

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <mutex>
 
 using namespace pcp;
@@ -1591,6 +1592,32 @@ int pcp_profile_read(pcp_ctx* c, double* totalMs, int* launches) {
     *totalMs = tot;
     *launches = (int)c->profUsed;
     return PCP_OK;
+}
+int pcp_set_schedule(pcp_ctx* c, int pairLanes, int lensLanes, int exteriorLanes) {
+    auto ok = [](int w, bool zero) { return w == 16 || w == 32 || w == 64 || (zero && w == 0); };
+    if (!ok(pairLanes, false) || !ok(lensLanes, false) || !ok(exteriorLanes, true)) {
+        set_error("pcp_set_schedule: lanes per polygon must be 16, 32 or 64 (exterior also 0 = by LDS)");
+        return PCP_ERR_ARG;
+    }
+    PCP_HIP(hipStreamSynchronize(c->stream));
+    c->segPair = pairLanes;
+    c->segLens = lensLanes;
+    c->segExt = exteriorLanes;
+    c->graphK = 0;
+    return PCP_OK;
+}
+int pcp_get_schedule(pcp_ctx* c, int* pairLanes, int* lensLanes, int* exteriorLanes) {
+    if (pairLanes) *pairLanes = c->segPair;
+    if (lensLanes) *lensLanes = c->segLens;
+    if (exteriorLanes) *exteriorLanes = c->segExt;
+    return PCP_OK;
+}
+int pcp_autotune(pcp_ctx* c, const pcp_state* st, int reps, double* timesUs) {
+    if (reps < 1 || reps > 10000) {
+        set_error("pcp_autotune: reps must be in 1..10000");
+        return PCP_ERR_ARG;
+    }
+    return autotune(c, st, reps, timesUs);
 }
 int pcp_stream_copy(pcp_ctx* c, double* dst, const double* src, long long n) {
     int g = (int)std::min<long long>(nblk(n), 8192);

@@ -650,14 +650,22 @@ int alloc_buckets(pcp_ctx* c, int Kp) {
     return PCP_OK;
 }
 
+template <int EXACT, int SEGX>
+static int launch_pair_w(pcp_ctx* c, const double2* pairPos) {
+    constexpr int G = 64 / SEGX;
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_poly<EXACT, SEGX>), dim3((c->N + G - 1) / G), dim3(64), 0, c->stream,
+                       c->N, c->Kp, c->dPolyStart, c->dPolyPack, c->dPackPoly, c->dBcnt, c->dBKey, c->dBTU,
+                       c->dPackMaxKey, c->pmkPitch, pairPos, c->dRKey, c->dRTU, c->dROuter, c->dRUT, c->dPcSorted);
+    PCP_LAUNCHED();
+    return PCP_OK;
+}
 template <int EXACT>
 static int launch_pair(pcp_ctx* c, const double2* pairPos) {
     int slot = -1;
     PCP_TRY(prof_begin(c, EXACT ? "k_pair_poly<1>" : "k_pair_poly<0>", &slot));
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_poly<EXACT, 16>), dim3((c->N + 3) / 4), dim3(64), 0, c->stream, c->N,
-                       c->Kp, c->dPolyStart, c->dPolyPack, c->dPackPoly, c->dBcnt, c->dBKey, c->dBTU, c->dPackMaxKey, c->pmkPitch,
-                       pairPos, c->dRKey, c->dRTU, c->dROuter, c->dRUT, c->dPcSorted);
-    PCP_LAUNCHED();
+    if (c->segPair == 64) PCP_TRY((launch_pair_w<EXACT, 64>(c, pairPos)));
+    else if (c->segPair == 32) PCP_TRY((launch_pair_w<EXACT, 32>(c, pairPos)));
+    else PCP_TRY((launch_pair_w<EXACT, 16>(c, pairPos)));
     return prof_end(c, slot);
 }
 
@@ -726,8 +734,12 @@ static int launch_exterior_m(pcp_ctx* c, const double2* pos) {
     PCP_TRY(prof_begin(c, MODE == 0 ? "k_exterior_poly<0>" : (MODE == 1 ? "k_exterior_poly<1>" : "k_exterior_poly<2>"),
                        &slot));
     size_t st = ext_lds_stride(c->Kp);
-    if (4 * st <= 65536) PCP_TRY((launch_exterior_w<MODE, 16>(c, pos)));
-    else if (2 * st <= 65536) PCP_TRY((launch_exterior_w<MODE, 32>(c, pos)));
+    int w = c->segExt;
+    if (w == 16 && 4 * st > 65536) w = 0;   // a chosen width the LDS does not fit: the default rule
+    if (w == 32 && 2 * st > 65536) w = 0;
+    if (w == 0) w = (4 * st <= 65536) ? 16 : ((2 * st <= 65536) ? 32 : 64);
+    if (w == 16) PCP_TRY((launch_exterior_w<MODE, 16>(c, pos)));
+    else if (w == 32) PCP_TRY((launch_exterior_w<MODE, 32>(c, pos)));
     else PCP_TRY((launch_exterior_w<MODE, 64>(c, pos)));
     return prof_end(c, slot);
 }
@@ -735,6 +747,85 @@ static int launch_exterior(pcp_ctx* c, int mode, const double2* pos) {
     if (mode == 2) return launch_exterior_m<2>(c, pos);
     if (mode == 1) return launch_exterior_m<1>(c, pos);
     return launch_exterior_m<0>(c, pos);
+}
+
+static int launch_lens(pcp_ctx* c, const double2* pos) {
+#define PCP_LENS(W)                                                                                           \
+    PCP_LAUNCH(c, k_lens_energy<W>, dim3((c->N + 64 / W - 1) / (64 / W)), dim3(64), c->N, c->Kp, c->dPolyStart, \
+               c->dPolyPack, c->dPackPoly, pos, c->dBcnt, c->dRKey, c->dROuter, c->dRTU, c->dRUT, c->dEr,       \
+               c->dPcSorted)
+    if (c->segLens == 64) PCP_LENS(64);
+    else if (c->segLens == 32) PCP_LENS(32);
+    else PCP_LENS(16);
+#undef PCP_LENS
+    return PCP_OK;
+}
+
+// Autotuner: time every candidate lane-segment width of the per-record kernels on the
+// context's current buffers (filled by one evaluation of st), `reps` back-to-back launches
+// each between two events, and keep the fastest.  All candidates give identical results.
+static int time_launches(pcp_ctx* c, int reps, double* us, const std::function<int()>& launch) {
+    hipEvent_t a, b;
+    PCP_HIP(hipEventCreate(&a));
+    PCP_HIP(hipEventCreate(&b));
+    int rc = launch();   // warm-up
+    if (rc == PCP_OK) {
+        PCP_HIP(hipEventRecord(a, c->stream));
+        for (int r = 0; r < reps && rc == PCP_OK; ++r) rc = launch();
+        PCP_HIP(hipEventRecord(b, c->stream));
+        PCP_HIP(hipEventSynchronize(b));
+        float ms = 0.f;
+        PCP_HIP(hipEventElapsedTime(&ms, a, b));
+        *us = 1e3 * ms / reps;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return rc;
+}
+
+int autotune(pcp_ctx* c, const pcp_state* st, int reps, double* timesUs) {
+    const int W[3] = {16, 32, 64};
+    double t[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int mt = st->model & 0xFF;
+    bool overlap = (mt == PCP_NORMAL || mt == PCP_HYBRID);
+    if (overlap && c->N > 0) {
+        PCP_TRY(pcp_force_eval(c, st));
+        PCP_TRY(pcp_synchronize(c));
+        bool exact = (st->model & (PCP_AREA_ENERGY | PCP_AREA_SQUARED)) != 0;
+        bool sq = (st->model & PCP_AREA_SQUARED) != 0;
+        int mode = sq ? 2 : (exact ? 1 : 0);
+        const double2* x = (const double2*)st->pos;
+        int keepPair = c->segPair, keepLens = c->segLens, keepExt = c->segExt;
+        size_t lst = ext_lds_stride(c->Kp);
+        for (int k = 0; k < 3; ++k) {
+            c->segPair = W[k];
+            PCP_TRY(time_launches(c, reps, &t[k], [&]() { return exact ? launch_pair<1>(c, x) : launch_pair<0>(c, nullptr); }));
+            if (sq) {
+                c->segLens = W[k];
+                PCP_TRY(time_launches(c, reps, &t[3 + k], [&]() { return launch_lens(c, x); }));
+            }
+            if ((size_t)(64 / W[k]) * lst <= 65536) {
+                c->segExt = W[k];
+                PCP_TRY(time_launches(c, reps, &t[6 + k], [&]() { return launch_exterior(c, mode, x); }));
+            }
+        }
+        auto best = [&](int o, int keep) {
+            int b = -1;
+            for (int k = 0; k < 3; ++k)
+                if (t[o + k] > 0.0 && (b < 0 || t[o + k] < t[o + b])) b = k;
+            return b < 0 ? keep : W[b];
+        };
+        c->segPair = best(0, keepPair);
+        c->segLens = sq ? best(3, keepLens) : keepLens;
+        c->segExt = best(6, keepExt);
+        c->graphK = 0;   // a captured FIRE block holds the old schedule
+        // leave the buffers as a plain evaluation with the chosen schedule leaves them
+        PCP_TRY(pcp_force_eval(c, st));
+        PCP_TRY(pcp_synchronize(c));
+    }
+    if (timesUs)
+        for (int k = 0; k < 9; ++k) timesUs[k] = t[k];
+    return PCP_OK;
 }
 
 int run_forces(pcp_ctx* c, int model, const double2* pos, double* edgeLengths, double* areas,
@@ -764,10 +855,7 @@ int run_forces(pcp_ctx* c, int model, const double2* pos, double* edgeLengths, d
         return PCP_OK;
     }
     if (overlap) {
-        if (sq)
-            PCP_LAUNCH(c, k_lens_energy<16>, dim3((c->N + 3) / 4), dim3(64), c->N, c->Kp, c->dPolyStart,
-                       c->dPolyPack, c->dPackPoly, pos, c->dBcnt, c->dRKey, c->dROuter, c->dRTU, c->dRUT, c->dEr,
-                       c->dPcSorted);
+        if (sq) PCP_TRY(launch_lens(c, pos));
         PCP_TRY(launch_exterior(c, sq ? 2 : (area ? 1 : 0), pos));
     }
     bool proj = project && c->maxN <= 64;
