@@ -64,7 +64,7 @@ def cpu_baseline(p, B, budget_s):
                        f"oracle/pcp_oracle.c with OpenMP ({threads} threads)")
 
 
-def fire_one(packings, dtMax, maxSteps, warm=None, budget_s=None):
+def fire_one(packings, dtMax, maxSteps, warm=None, budget_s=None, graph=True):
     """FIRE (squared lens areas, rigid polygons) on the packings in one context until every
     packing has max|F| <= 1e-10, maxSteps, or budget_s seconds.  warm: a WarmStart applied
     once after the first evaluation (its time is counted).  Returns a result dict."""
@@ -83,7 +83,7 @@ def fire_one(packings, dtMax, maxSteps, warm=None, budget_s=None):
     # the time budget is checked every 512 steps
     while steps < maxSteps and (budget_s is None or time.perf_counter() - t0 < budget_s):
         n = min(512, maxSteps - steps)
-        st = s.fire_run(prm, n, 8, True)
+        st = s.fire_run(prm, n, 8, graph)
         steps += n
         for k in range(s.P):
             if conv[k] < 0 and st[k].converged:
@@ -97,7 +97,7 @@ def fire_one(packings, dtMax, maxSteps, warm=None, budget_s=None):
     E = s.get_energy()
     F = s.max_abs_force()
     return dict(packings=s.P, vertices=s.V, dtMax=dtMax, warm_start=warm is not None, batch_steps=steps,
-                fire="pcp_fire_run, blocks of 8 steps replayed from a HIP graph",
+                fire="pcp_fire_run, blocks of 8 steps" + (" replayed from a HIP graph" if graph else ""),
                 seconds=el, converged=int(ok.sum()), mean_steps=float(conv[ok].mean()) if ok.any() else None,
                 max_steps=int(conv.max()) if ok.all() else None, packings_per_s=float(ok.sum()) / el,
                 max_final_force=float(F.max()), median_final_energy=float(np.median(E)))
@@ -119,12 +119,13 @@ def fire_legs(args):
     p2 = make_packing(4096, 16, 0.85, seed=args.seed, min_center_frac=0.5)
     for tag, dt, warm, steps in (("n4096_reference_dtmax", 0.1, None, 20000), ("n4096", W.DT_MAX, None, 5000),
                                  ("n4096_learned", W.DT_MAX, ws, 5000)):
-        out[tag] = fire_one([p2], dt, steps, warm, budget_s=args.fire_seconds)
+        out[tag] = fire_one([p2], dt, steps, warm, budget_s=args.fire_seconds, graph=not args.no_graph)
         log(out[tag], tag)
     from pycudapolygon_amd import batch
     p4 = W.make_c4_packings(32, 777777)
     for tag, warm in (("n1024_x32", None), ("n1024_x32_learned", ws)):
-        r = batch.minimize_batch(p4, W.SQUARED, dict(fTol=1e-10, dtMax=W.DT_MAX), 5000, warm=warm)
+        r = batch.minimize_batch(p4, W.SQUARED, dict(fTol=1e-10, dtMax=W.DT_MAX), 5000, warm=warm,
+                                 graph=not args.no_graph)
         ok = r.steps >= 0
         out[tag] = dict(packings=32, workload="32 packings of N=1024 x 16, phi ~ U[0.82,0.88], one context, "
                         "re-packed when fewer than half are still running", warm_start=warm is not None,
@@ -138,7 +139,7 @@ def fire_legs(args):
         t0 = time.perf_counter()
         p3 = make_batch(args.batch_packings, args.seed + 3, min_center_frac=0.5)
         gen = time.perf_counter() - t0
-        r = batch.minimize_batch(p3, W.SQUARED, dict(fTol=1e-10, dtMax=W.DT_MAX), 5000)
+        r = batch.minimize_batch(p3, W.SQUARED, dict(fTol=1e-10, dtMax=W.DT_MAX), 5000, graph=not args.no_graph)
         ok = r.steps >= 0
         out["batched_uneven"] = dict(
             workload=(f"{args.batch_packings} packings, N ~ U[64,1024], n ~ U[8,32], phi ~ U[0.80,0.90], "
@@ -169,6 +170,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fire-seconds", type=float, default=20.0)
     ap.add_argument("--batch-packings", type=int, default=256)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="FIRE legs launch each block's kernels directly instead of replaying a HIP graph "
+                         "(for profilers that cannot trace graph launches)")
     ap.add_argument("--no-fire", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="N",
                     help="only run the plain copy kernel on N doubles (10 launches) for HBM counter "
