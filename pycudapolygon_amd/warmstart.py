@@ -31,10 +31,6 @@ DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weig
 DT_MAX = 100.0
 
 
-def _p(a):
-    return a.ptr
-
-
 def _np_ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
