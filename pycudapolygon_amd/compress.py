@@ -1,0 +1,91 @@
+"""Compression protocol: a packing-fraction ramp on several replicas in one context.
+
+At each step phi -> phi + dphi, every packing's targets are scaled to the new packing
+fraction (areas by phi'/phi, edge lengths by sqrt(phi'/phi), the reference's setPhi,
+pyCudaPolygon.py:223-234), every polygon is rescaled about its centroid to its new target
+area (resetAreas, kernels.cuh:1440-1455, through pcp_reset_areas), an optional learned warm
+start is applied, and FP64 FIRE (rigid polygons: SHAKE onto edge lengths and area every
+step, forces projected onto the constraint manifold) minimizes to max|F| <= fTol.  The box
+stays the periodic unit square, as in the reference.  No counterpart of the loop itself in
+the reference (its notebooks drive setPhi and minimizeFIRE by hand).
+"""
+import time
+
+import numpy as np
+
+from . import capi
+from .generate import box_size_for
+
+SQUARED = capi.NORMAL | 0x200
+
+
+def packing_fractions(s):
+    """sum of target areas of each packing of System s (the unit box has area 1)"""
+    ta = s.ta.get()
+    return np.array([ta[s.packPoly[p]:s.packPoly[p + 1]].sum() for p in range(s.P)])
+
+
+def set_phi(s, phi):
+    """Scale every packing's targets to packing fraction phi[p] and rescale each polygon about
+    its centroid to its new target area (setPhi + resetAreas)."""
+    s.sync()
+    phi = np.broadcast_to(np.asarray(phi, np.float64), (s.P,))
+    ta = s.ta.get()
+    tev = s.tev.get()
+    cur = packing_fractions(s)
+    for p in range(s.P):
+        r = phi[p] / cur[p]
+        a, b = int(s.packPoly[p]), int(s.packPoly[p + 1])
+        ta[a:b] *= r
+        va, vb = s.vertex_range(p)
+        tev[va:vb] *= np.sqrt(r)
+    s.ta.upload(ta)
+    s.tev.upload(tev)
+    s.geometry(constraints=False)
+    s.ctx.call("pcp_reset_areas", s.pos.ptr, s.areas.ptr, s.ta.ptr, s.comX.ptr, s.comY.ptr)
+    s.sync()
+
+
+def ramp(packings, phi_end, dphi, fire_kwargs, maxSteps, warm=None, block=8, graph=True, log=None,
+         boxSizes=None, budget_s=None):
+    """Minimize the packings at their starting packing fraction, then step the packing fraction
+    by dphi up to phi_end (or until budget_s seconds have passed), minimizing after each step.
+    Returns (list of per-step dicts: phi, FIRE steps per replica, seconds, converged, energies;
+    the System)."""
+    if boxSizes is None:
+        boxSizes = [box_size_for(p["diameter"] * np.sqrt(phi_end / float(np.sum(p["targetAreas"]))))
+                    for p in packings]
+    s = capi.System(packings, boxSizes, model=SQUARED)
+    prm = s.fire_params(**fire_kwargs)
+    ell = np.array([float(np.mean(p["targetEdgeV"])) for p in packings])
+    phi = packing_fractions(s)
+    out = []
+    k = 0
+    tstart = time.perf_counter()
+    while True:
+        t0 = time.perf_counter()
+        if k > 0:
+            prev = phi
+            phi = np.minimum(phi + dphi, phi_end)
+            set_phi(s, phi)
+            ell = ell * np.sqrt(phi / prev)
+        tw = 0.0
+        if warm is not None:
+            s.force_eval()
+            tw = warm.apply(s, ell)
+        s.fire_init(prm, 1e-3)
+        st = s.fire_run(prm, maxSteps, block, graph)
+        el = time.perf_counter() - t0
+        rec = dict(step=k, phi=phi.tolist(), fire_steps=[q.steps for q in st],
+                   converged=[bool(q.converged) for q in st], seconds=el, warm_seconds=tw,
+                   energy=[q.energy for q in st], max_force=[q.maxForce for q in st])
+        out.append(rec)
+        if log:
+            log(f"[compress] step {k}: phi {phi.mean():.4f}, FIRE steps {int(np.mean(rec['fire_steps']))} "
+                f"(max {max(rec['fire_steps'])}), {sum(rec['converged'])}/{s.P} converged, {el:.2f} s")
+        if np.all(phi >= phi_end - 1e-12):
+            break
+        if budget_s is not None and time.perf_counter() - tstart > budget_s:
+            break
+        k += 1
+    return out, s

@@ -22,21 +22,35 @@ def polygon_areas(verts):
 
 def separated_centers(rng, R, L, frac, tries=10000):
     """Centres uniform in the unit box, placed one by one and re-drawn while closer than
-    frac * (R_i + R_j) / L (minimum image) to a centre already placed."""
+    frac * (R_i + R_j) / L (minimum image) to a centre already placed.  Placed centres are
+    kept in a grid of cells at least as wide as the largest exclusion distance, so each
+    candidate is checked against the 3x3 cells around it only."""
     N = len(R)
     out = np.empty((N, 2))
+    rmax = frac * 2.0 * float(np.max(R)) / L
+    G = max(1, min(int(1.0 / rmax) if rmax > 0 else 1, 2048))
+    cells = {}
     for i in range(N):
         for _ in range(tries):
             c = rng.uniform(0.0, 1.0, 2)
-            if i == 0:
-                break
-            d = out[:i] - c
-            d -= np.round(d)
-            if np.all((d ** 2).sum(1) >= (frac * (R[:i] + R[i]) / L) ** 2):
-                break
+            cx, cy = int(c[0] * G) % G, int(c[1] * G) % G
+            near = []
+            for dx in (-1, 0, 1) if G > 2 else (0,):
+                for dy in (-1, 0, 1) if G > 2 else (0,):
+                    near.extend(cells.get(((cx + dx) % G, (cy + dy) % G), ()))
+            if G <= 2:
+                near = range(i)
+            if near:
+                idx = np.fromiter(near, int)
+                d = out[idx] - c
+                d -= np.round(d)
+                if not np.all((d ** 2).sum(1) >= (frac * (R[idx] + R[i]) / L) ** 2):
+                    continue
+            break
         else:
             raise RuntimeError("separated_centers: no room; lower frac or phi")
         out[i] = c
+        cells.setdefault((int(c[0] * G) % G, int(c[1] * G) % G), []).append(i)
     return out
 
 
