@@ -157,24 +157,32 @@ def fire_legs(args):
         t0 = time.perf_counter()
         p5 = [make_packing(16384, 24, 0.80, seed=args.seed + 500 + k, min_center_frac=0.5) for k in range(R)]
         gen = time.perf_counter() - t0
-        steps, s5 = compress.ramp(p5, 0.88, 1e-3, dict(fTol=1e-10, dtMax=W.DT_MAX), 5000,
-                                  budget_s=args.compress_seconds, graph=not args.no_graph)
-        conv = all(all(r["converged"]) for r in steps)
-        sec = sum(r["seconds"] for r in steps)
-        nst = len(steps) - 1
-        out["compression"] = dict(
-            workload=(f"{R} replicas of N=16384 x 24 in one context, phi 0.80 -> 0.88 in steps of 1e-3 "
-                      "(targets scaled, polygons rescaled about their centroids), FIRE to max|F| <= 1e-10 "
-                      "after every step"),
-            replicas=R, vertices=s5.V, phi_reached=float(np.mean(steps[-1]["phi"])), phi_steps=nst,
-            all_steps_converged=conv, seconds=sec, initial_minimization_seconds=steps[0]["seconds"],
-            seconds_per_phi_step=(sec - steps[0]["seconds"]) / max(nst, 1),
-            mean_fire_steps_per_phi_step=float(np.mean([np.mean(r["fire_steps"]) for r in steps[1:]]))
-            if nst else None,
-            replica_minimizations_per_s=R * (nst + 1) / sec, generation_seconds=gen,
-            final_energy=steps[-1]["energy"], budget_s=args.compress_seconds)
-        print(f"[bench] compression: {R} x N=16384 x 24, phi -> {out['compression']['phi_reached']:.3f} in "
-              f"{nst} steps, {sec:.1f} s, all converged {conv}", flush=True)
+        wc = W.WarmStart(W.load_weights(W.COMPRESS_WEIGHTS))
+        out["compression"] = {}
+        for tag, warm, dt0 in (("plain", None, 1e-3), ("plain_dt0_100", None, 100.0), ("learned", wc, 1e-3)):
+            steps, s5 = compress.ramp(p5, 0.88, 1e-3, dict(fTol=1e-10, dtMax=W.DT_MAX), 5000, warm=warm, dt0=dt0,
+                                      budget_s=args.compress_seconds, graph=not args.no_graph)
+            conv = all(all(r["converged"]) for r in steps)
+            sec = sum(r["seconds"] for r in steps)
+            nst = len(steps) - 1
+            out["compression"][tag] = dict(
+                workload=(f"{R} replicas of N=16384 x 24 in one context, phi 0.80 -> 0.88 in steps of 1e-3 "
+                          "(targets scaled, polygons rescaled about their centroids), FIRE to max|F| <= 1e-10 "
+                          "after every step"),
+                warm_start=("network trained on compression steps, applied after every rescale"
+                            if warm is not None else None), dt0=dt0,
+                replicas=R, vertices=s5.V, phi_reached=float(np.mean(steps[-1]["phi"])), phi_steps=nst,
+                all_steps_converged=conv, seconds=sec, initial_minimization_seconds=steps[0]["seconds"],
+                seconds_per_phi_step=(sec - steps[0]["seconds"]) / max(nst, 1),
+                mean_fire_steps_per_phi_step=float(np.mean([np.mean(r["fire_steps"]) for r in steps[1:]]))
+                if nst else None,
+                replica_minimizations_per_s=R * (nst + 1) / sec, generation_seconds=gen,
+                mean_final_energy=float(np.mean(steps[-1]["energy"])), budget_s=args.compress_seconds)
+            print(f"[bench] compression {tag}: {R} x N=16384 x 24, phi -> "
+                  f"{out['compression'][tag]['phi_reached']:.3f} in {nst} steps, {sec:.1f} s, "
+                  f"FIRE steps per phi step {out['compression'][tag]['mean_fire_steps_per_phi_step']:.1f}, "
+                  f"final E {out['compression'][tag]['mean_final_energy']:.2e}, all converged {conv}", flush=True)
+            del s5
     out["note"] = ("dtMax 0.1 is the reference driver's default; the squared lens energy's stiffness scales as "
                    "ell^2 (ell = edge length), so its natural time step is ~1/ell and FIRE at dtMax 0.1 is "
                    "time-step bound; dtMax 100 is used for the other legs. learned = the committed warm-start "

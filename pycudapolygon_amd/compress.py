@@ -47,7 +47,7 @@ def set_phi(s, phi):
 
 
 def ramp(packings, phi_end, dphi, fire_kwargs, maxSteps, warm=None, block=8, graph=True, log=None,
-         boxSizes=None, budget_s=None):
+         boxSizes=None, budget_s=None, dt0=1e-3, applications=1):
     """Minimize the packings at their starting packing fraction, then step the packing fraction
     by dphi up to phi_end (or until budget_s seconds have passed), minimizing after each step.
     Returns (list of per-step dicts: phi, FIRE steps per replica, seconds, converged, energies;
@@ -71,9 +71,10 @@ def ramp(packings, phi_end, dphi, fire_kwargs, maxSteps, warm=None, block=8, gra
             ell = ell * np.sqrt(phi / prev)
         tw = 0.0
         if warm is not None:
-            s.force_eval()
-            tw = warm.apply(s, ell)
-        s.fire_init(prm, 1e-3)
+            for _ in range(applications):
+                s.force_eval()
+                tw += warm.apply(s, ell)
+        s.fire_init(prm, dt0)
         st = s.fire_run(prm, maxSteps, block, graph)
         el = time.perf_counter() - t0
         rec = dict(step=k, phi=phi.tolist(), fire_steps=[q.steps for q in st],

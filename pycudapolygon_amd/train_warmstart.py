@@ -50,13 +50,23 @@ def main(argv=None):
     ap.add_argument("--eval-packings", type=int, default=32)
     ap.add_argument("--eval-seed", type=int, default=777777)
     ap.add_argument("--applications", type=int, default=1)
+    ap.add_argument("--mode", choices=["fresh", "compression"], default="fresh",
+                    help="fresh: FIRE trajectories from new packings (the defaults above); compression: "
+                         "states just after each packing-fraction step of a ramp, labelled with the state "
+                         "FIRE reaches from them")
+    ap.add_argument("--replicas", type=int, default=16)
+    ap.add_argument("--phi-steps", type=int, default=40)
     args = ap.parse_args(argv)
 
     log = lambda m: print(m, flush=True)
     caps = tuple(int(c) for c in args.captures.split(","))
     t0 = time.perf_counter()
-    ds, runs, inScale = W.generate(args.batches, args.packings, args.seed, capture_steps=caps,
-                                   maxSteps=args.max_steps, log=log, N=args.N, n=args.n, dtMax=args.dt_max)
+    if args.mode == "compression":
+        ds, runs, inScale = W.generate_compression(args.replicas, args.N, args.n, args.seed, steps=args.phi_steps,
+                                                   maxSteps=args.max_steps, dtMax=args.dt_max, log=log)
+    else:
+        ds, runs, inScale = W.generate(args.batches, args.packings, args.seed, capture_steps=caps,
+                                       maxSteps=args.max_steps, log=log, N=args.N, n=args.n, dtMax=args.dt_max)
     outScale = W.label_all(ds, runs)
     tgen = time.perf_counter() - t0
     conv = np.concatenate([r[2] for r in runs])
@@ -65,22 +75,28 @@ def main(argv=None):
                           median_fire_steps=float(np.median(conv[conv >= 0])), inScale=inScale.tolist(),
                           outScale=outScale)), flush=True)
 
-    last = runs[-1][0]
-    val = [d for d in ds.samples if d["sys"] is last] if args.batches > 1 else []
-    trn = [d for d in ds.samples if d["sys"] is not last] if args.batches > 1 else ds.samples
+    if args.mode == "compression":   # every 5th phi step held out
+        val = ds.samples[4::5]
+        trn = [d for i, d in enumerate(ds.samples) if i % 5 != 4]
+    else:
+        last = runs[-1][0]
+        val = [d for d in ds.samples if d["sys"] is last] if args.batches > 1 else []
+        trn = [d for d in ds.samples if d["sys"] is not last] if args.batches > 1 else ds.samples
     t1 = time.perf_counter()
     params, losses, vlosses = W.train(trn, W.init_params(0), args.epochs, lr=args.lr, log=log, val=val,
                                       lr_final=args.lr_final)
     ttrain = time.perf_counter() - t1
-    W.save_weights(args.out, params, inScale, outScale, 4, N=args.N, n=args.n, trajectories=int(len(conv)),
-                   held_out=args.packings if args.batches > 1 else 0, captures=caps, dtMax=args.dt_max,
-                   epochs=args.epochs, lr=args.lr, lr_final=args.lr_final, seed=args.seed)
+    W.save_weights(args.out, params, inScale, outScale, 4, mode=args.mode, N=args.N, n=args.n,
+                   trajectories=int(len(conv)), held_out=len(val), captures=caps, dtMax=args.dt_max,
+                   epochs=args.epochs, lr=args.lr, lr_final=args.lr_final, seed=args.seed,
+                   replicas=args.replicas if args.mode == "compression" else 0,
+                   phi_steps=args.phi_steps if args.mode == "compression" else 0)
     print(json.dumps(dict(phase="train", steps=args.epochs * len(trn), seconds=ttrain, train_loss=losses,
                           validation_loss=vlosses, zero_prediction_loss=2.0, weights=args.out)), flush=True)
 
     # free the training data before the comparison
     del ds, runs, trn, val
-    if args.eval_packings > 0:
+    if args.eval_packings > 0 and args.mode == "fresh":
         pk = W.make_c4_packings(args.eval_packings, args.eval_seed, N=args.N, n=args.n)
         res = W.ab_compare(W.load_weights(args.out), pk, maxSteps=args.max_steps,
                            applications=args.applications, log=log, dtMax=args.dt_max)

@@ -27,6 +27,11 @@ H = 64
 SQUARED = capi.NORMAL | 0x200
 # weights trained by `python -m pycudapolygon_amd.train_warmstart` (its defaults)
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "warmstart_n1024_n16.npz")
+# weights trained on packing-fraction steps of the compression protocol:
+# `python -m pycudapolygon_amd.train_warmstart --mode compression --N 4096 --n 24 --replicas 16
+#  --phi-steps 40 --epochs 10 --seed 2000`
+COMPRESS_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights",
+                                "warmstart_compress_n4096_n24.npz")
 # FIRE time-step cap for the squared lens energy (see train_warmstart)
 DT_MAX = 100.0
 
@@ -238,13 +243,22 @@ class Dataset:
         self.samples.append(dict(feat=feat, adj=adj, x=x, target=None, V=V, sys=s, ell=gf.ell))
 
     def label(self, s, outScale):
-        """targets of every sample of system s = displacement to s's current positions"""
+        """targets of every sample of system s = displacement to its own end state (xto, when
+        the sample has one) or else to s's current positions"""
         for d in self.samples:
             if d["sys"] is not s or d["target"] is not None:
                 continue
             d["target"] = capi.DeviceArray((2 * d["V"],), np.float32)
-            s.ctx.call("pcp_mpnn_targets", d["x"].ptr, s.pos.ptr, d["ell"].ptr, float(outScale), d["target"].ptr)
+            to = d["xto"].ptr if d.get("xto") is not None else s.pos.ptr
+            s.ctx.call("pcp_mpnn_targets", d["x"].ptr, to, d["ell"].ptr, float(outScale), d["target"].ptr)
         s.sync()
+
+    def set_end_state(self, s, first):
+        """the end state of samples[first:] of system s is s's current positions"""
+        for d in self.samples[first:]:
+            if d["sys"] is s:
+                d["xto"] = capi.DeviceArray((2 * d["V"],), np.float64)
+                capi.check(capi.lib().pcp_memcpy_d2d(d["xto"].ptr, s.pos.ptr, d["xto"].nbytes))
 
 
 def raw_feature_scales(s, ell, energyPow=4):
@@ -299,6 +313,47 @@ def generate(batches, P, seed, capture_steps=(0, 100, 300, 1000), maxSteps=12000
         log(f"[warmstart] batch {b}: {P} packings, FIRE {steps} steps in {el:.1f} s, converged "
             f"{int(ok.sum())}/{P}, median steps {int(np.median(conv[ok])) if ok.any() else -1}")
     return ds, runs, inScale
+
+
+def generate_compression(replicas, N, n, seed, phi0=0.80, steps=40, dphi=1e-3, maxSteps=3000, dtMax=DT_MAX,
+                         inScale=None, log=print):
+    """Training data of the compression protocol: `replicas` packings minimized at phi0, then
+    `steps` packing-fraction steps (targets scaled, polygons rescaled about their centroids);
+    each just-rescaled state is captured (after one evaluation) and labelled with the
+    minimized state that FIRE reaches from it.  Returns (Dataset, runs, inScale)."""
+    from . import compress
+    pk = [make_packing(N, n, phi0, seed + k, min_center_frac=0.5) for k in range(replicas)]
+    phi_end = phi0 + steps * dphi
+    B = [box_size_for(p["diameter"] * np.sqrt(phi_end / phi0)) for p in pk]
+    s = capi.System(pk, B, model=SQUARED)
+    prm = s.fire_params(fTol=1e-10, dtMax=dtMax)
+    ell = packing_length_units(pk)
+    s.fire_init(prm, 1e-3)
+    s.fire_run(prm, maxSteps, 8, True)
+    ds = Dataset()
+    phi = compress.packing_fractions(s)
+    t0 = time.perf_counter()
+    fire_steps = []
+    for k in range(steps):
+        prev = phi
+        phi = phi + dphi
+        compress.set_phi(s, phi)
+        ell = ell * np.sqrt(phi / prev)
+        s.force_eval()
+        if inScale is None:
+            inScale = raw_feature_scales(s, ell)
+        gf = GraphFeatures(s, ell, inScale)
+        first = len(ds.samples)
+        ds.capture(s, gf)
+        s.fire_init(prm, 1e-3)
+        st = s.fire_run(prm, maxSteps, 8, True)
+        fire_steps.append([q.steps for q in st])
+        ds.set_end_state(s, first)
+    el = time.perf_counter() - t0
+    conv = np.array(fire_steps[-1])
+    log(f"[warmstart] compression data: {replicas} x N={N} x {n}, {steps} steps of {dphi}, "
+        f"mean FIRE steps per step {np.mean(fire_steps):.0f}, {el:.1f} s")
+    return ds, [(s, ell, conv, el)], inScale
 
 
 def label_all(ds, runs):
