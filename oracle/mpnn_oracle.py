@@ -28,7 +28,9 @@ OFF_W0, OFF_B0, EMBED_NP = 0, 256, 320
 OFF_W1, LAYER_NP = 320, 8256
 OFF_WO, OFF_BO, OUT_NP = 25088, 25216, 130
 NPARAM = 25218
-CH = 2048   # vertices per partial-sum chunk of the GPU's weight gradients
+CH = 2048   # vertices per partial-sum chunk of the GPU's matrix-core weight gradients
+CHV = 256   # vertices per partial-sum chunk of its VALU weight gradients
+GRP = 64    # chunks per group when the chunk partials are summed
 
 
 def bf(a):
@@ -198,10 +200,12 @@ def _seqsum(a):
     return np.cumsum(a, axis=0, dtype=np.float32)[-1]
 
 
-def _chunked(fn, V):
-    """GPU order of vertex sums: sequential inside chunks of CH vertices, then over chunks."""
-    parts = [fn(m0, min(V, m0 + CH)) for m0 in range(0, V, CH)]
-    return _seqsum(np.stack(parts))
+def _chunked(fn, V, ch=CH):
+    """GPU order of vertex sums: sequential inside chunks of `ch` vertices, then over groups of
+    GRP consecutive chunks, then over the groups."""
+    parts = np.stack([fn(m0, min(V, m0 + ch)) for m0 in range(0, V, ch)])
+    groups = np.stack([_seqsum(parts[g:g + GRP]) for g in range(0, len(parts), GRP)])
+    return _seqsum(groups)
 
 
 def _bias_grad_gpu_order(dZ):
@@ -255,7 +259,7 @@ def backward(params, feat, adj, cache, dY, exact=False):
     h3 = bf(Hs[3])
     gwo = _chunked(lambda a, b: np.concatenate([
         np.stack([_seqsum(h3[a:b, c] * dY[a:b, o]) for c in range(H) for o in range(2)]),
-        _seqsum(dY[a:b])]), V)
+        _seqsum(dY[a:b])]), V, CHV)
     g[OFF_WO:OFF_WO + OUT_NP] = gwo
     wo = bf(Wo)
     dH = (dY[:, 0:1] * wo[:, 0] + dY[:, 1:2] * wo[:, 1]).astype(np.float32)
@@ -277,7 +281,7 @@ def backward(params, feat, adj, cache, dY, exact=False):
     x = bf(feat)
     gw0 = _chunked(lambda a, b: np.concatenate([
         np.stack([_seqsum(x[a:b, f] * dZ0[a:b, c]) for f in range(4) for c in range(H)]),
-        _seqsum(dZ0[a:b])]), V)
+        _seqsum(dZ0[a:b])]), V, CHV)
     g[OFF_W0:OFF_W0 + EMBED_NP] = gw0
     return g
 

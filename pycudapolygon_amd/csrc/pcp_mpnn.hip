@@ -22,8 +22,10 @@
 // gradient and weight gradient -- run on the matrix cores (v_mfma_f32_16x16x16_bf16,
 // one wavefront per 16 vertices); the 4x64 and 64x2 layers run on VALU in a fixed
 // order.  Gradients treat the bf16 rounding as the identity (straight-through).
-// Reductions over vertices are partial sums per chunk of MPNN_CH vertices added in
-// chunk order, so training is bit-reproducible run to run.
+// Reductions over vertices are partial sums per chunk of vertices (MPNN_CH for the
+// matrix-core weight gradients, MPNN_CHV for the VALU ones), then sums of groups of
+// MPNN_GRP consecutive chunks, then the sum of the groups, each in order, so training is
+// bit-reproducible run to run.
 //
 // This file is compiled as part of pcp_kernels.hip (one translation unit).
 
@@ -34,7 +36,9 @@ typedef float mpnn_f4 __attribute__((ext_vector_type(4)));
 
 constexpr int MH = PCP_MPNN_H;           // hidden width
 constexpr int MD = PCP_MPNN_DEG;         // adjacency slots per vertex
-constexpr int MPNN_CH = 2048;            // vertices per partial-sum chunk (weight gradients)
+constexpr int MPNN_CH = 2048;            // vertices per partial-sum chunk (MFMA weight gradients)
+constexpr int MPNN_CHV = 256;            // vertices per partial-sum chunk (VALU weight gradients)
+constexpr int MPNN_GRP = 64;             // chunks per group in the sum of the chunk partials
 constexpr int WPAD = 4;                  // LDS row padding (bf16 elements) against bank conflicts
 
 __device__ __forceinline__ unsigned short bf16_rne(float f) {
@@ -228,18 +232,30 @@ __global__ __launch_bounds__(256) void k_mpnn_layer(int V, const float* __restri
     }
 }
 
-// y = bf(h3) bf(Wo) + bo, thread per vertex, channels in order
-__global__ void k_mpnn_out(int V, const float* __restrict__ prm, const float* __restrict__ H3,
-                           float2* __restrict__ out) {
-    int m = blockIdx.x * blockDim.x + threadIdx.x;
+// y = bf(h3) bf(Wo) + bo, thread per vertex, channels in order; the workgroup's 128 rows of
+// h3 are staged in LDS with coalesced loads (rows padded to 65 floats against bank conflicts)
+__global__ __launch_bounds__(128) void k_mpnn_out(int V, const float* __restrict__ prm, const float* __restrict__ H3,
+                                                  float2* __restrict__ out) {
+    __shared__ float sh[128 * (MH + 1)];
+    __shared__ float wo[2 * MH];
+    int base = blockIdx.x * 128;
+    for (int k = threadIdx.x; k < 128 * (MH / 4); k += 128) {
+        int r = k / (MH / 4), q = k % (MH / 4);
+        int m = base + r;
+        float4 v = m < V ? *(const float4*)(H3 + (long long)m * MH + 4 * q) : make_float4(0.f, 0.f, 0.f, 0.f);
+        float* d = sh + r * (MH + 1) + 4 * q;
+        d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+    }
+    wo[threadIdx.x] = bfr(prm[PCP_MPNN_OFF_WO + threadIdx.x]);
+    __syncthreads();
+    int m = base + threadIdx.x;
     if (m >= V) return;
-    const float* Wo = prm + PCP_MPNN_OFF_WO;
-    const float* h = H3 + (long long)m * MH;
+    const float* h = sh + threadIdx.x * (MH + 1);
     float y0 = 0.0f, y1 = 0.0f;
     for (int c = 0; c < MH; ++c) {
         float hc = bfr(h[c]);
-        y0 = y0 + hc * bfr(Wo[2 * c]);
-        y1 = y1 + hc * bfr(Wo[2 * c + 1]);
+        y0 = y0 + hc * wo[2 * c];
+        y1 = y1 + hc * wo[2 * c + 1];
     }
     out[m] = make_float2(y0 + prm[PCP_MPNN_OFF_BO], y1 + prm[PCP_MPNN_OFF_BO + 1]);
 }
@@ -267,19 +283,39 @@ __global__ __launch_bounds__(256) void k_mpnn_loss(int V, const float2* __restri
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-// out[e] = sum over chunks q (in order) of part[q * width + e]
-__global__ void k_mpnn_sum_chunks(int nChunks, int width, const float* __restrict__ part, float* __restrict__ out) {
+// grp[gi * width + e] = sum over the chunks q of group gi (MPNN_GRP consecutive chunks, in
+// order) of part[q * width + e]; thread per (entry, group)
+__global__ void k_mpnn_sum_groups(int nChunks, int width, const float* __restrict__ part, float* __restrict__ grp) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    int gi = blockIdx.y;
+    if (e >= width) return;
+    int q0 = gi * MPNN_GRP, q1 = min(nChunks, q0 + MPNN_GRP);
+    float s = 0.0f;
+    for (int q = q0; q < q1; ++q) s = s + part[(long long)q * width + e];
+    grp[(long long)gi * width + e] = s;
+}
+// out[e] = sum over groups (in order) of grp[gi * width + e]
+__global__ void k_mpnn_sum_chunks(int nGroups, int width, const float* __restrict__ grp, float* __restrict__ out) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= width) return;
     float s = 0.0f;
-    for (int q = 0; q < nChunks; ++q) s = s + part[(long long)q * width + e];
+    for (int q = 0; q < nGroups; ++q) s = s + grp[(long long)q * width + e];
     out[e] = s;
 }
-__global__ void k_mpnn_sum_loss(int n, const double* __restrict__ part, double scale, double* __restrict__ out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// loss = scale * sum of the partials: lane t adds partials t, t + 256, ... in order, then a
+// fixed pairwise tree over the 256 lane sums
+__global__ __launch_bounds__(256) void k_mpnn_sum_loss(int n, const double* __restrict__ part, double scale,
+                                                       double* __restrict__ out) {
+    __shared__ double sh[256];
     double s = 0.0;
-    for (int q = 0; q < n; ++q) s = s + part[q];
-    out[0] = s * scale;
+    for (int q = threadIdx.x; q < n; q += 256) s = s + part[q];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0] * scale;
 }
 
 // ---- backward --------------------------------------------------------------------------
@@ -294,28 +330,30 @@ __global__ void k_mpnn_out_bwd(int V, const float* __restrict__ prm, const float
     dH[t] = g.x * bfr(Wo[2 * c]) + g.y * bfr(Wo[2 * c + 1]);
 }
 
-// partial gradients of Wo (64x2) and bo over a chunk of vertices: entry e < 128 is
-// Wo[e/2][e%2], e = 128, 129 is bo[e-128]
-__global__ __launch_bounds__(192) void k_mpnn_out_grad(int V, const float* __restrict__ H3, const float2* __restrict__ dy,
-                                                       float* __restrict__ part) {
-    int e = threadIdx.x;
-    if (e >= 2 * MH + 2) return;
-    int m0 = blockIdx.x * MPNN_CH, m1 = min(V, m0 + MPNN_CH);
-    float s = 0.0f;
-    if (e < 2 * MH) {
-        int c = e >> 1, o = e & 1;
-        for (int m = m0; m < m1; ++m) {
-            float2 g = dy[m];
-            s = s + bfr(H3[(long long)m * MH + c]) * (o ? g.y : g.x);
-        }
-    } else {
-        int o = e - 2 * MH;
-        for (int m = m0; m < m1; ++m) {
-            float2 g = dy[m];
-            s = s + (o ? g.y : g.x);
-        }
+// partial gradients of Wo (64x2) and bo over a chunk of MPNN_CHV vertices: entry e < 128 is
+// Wo[e/2][e%2], e = 128, 129 is bo[e-128].  One wavefront per chunk, lane c owns channel c
+// (and lane 0 the biases); each entry is summed over the chunk's vertices in order.
+__global__ __launch_bounds__(64) void k_mpnn_out_grad(int V, const float* __restrict__ H3, const float2* __restrict__ dy,
+                                                      float* __restrict__ part) {
+    int c = threadIdx.x;
+    int m0 = blockIdx.x * MPNN_CHV, m1 = min(V, m0 + MPNN_CHV);
+    float s0 = 0.0f, s1 = 0.0f, b0 = 0.0f, b1 = 0.0f;
+#pragma unroll 4
+    for (int m = m0; m < m1; ++m) {
+        float2 g = dy[m];
+        float h = bfr(H3[(long long)m * MH + c]);
+        s0 = s0 + h * g.x;
+        s1 = s1 + h * g.y;
+        b0 = b0 + g.x;
+        b1 = b1 + g.y;
     }
-    part[(long long)blockIdx.x * PCP_MPNN_OUT_NP + e] = s;
+    float* out = part + (long long)blockIdx.x * PCP_MPNN_OUT_NP;
+    out[2 * c] = s0;
+    out[2 * c + 1] = s1;
+    if (c == 0) {
+        out[2 * MH] = b0;
+        out[2 * MH + 1] = b1;
+    }
 }
 
 // Layer backward, input side.  dZ = dH ⊙ [Z > 0]; dX = bf(dZ) bf(W)^T (MFMA, K = 64);
@@ -379,12 +417,19 @@ __global__ void k_mpnn_gather_bwd(int V, const int* __restrict__ adj, const floa
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)V * MH) return;
     int m = (int)(t / MH), c = (int)(t % MH);
-    const int* row = adj + (long long)m * MD;
-    float s = 0.0f;
-    for (int d = 0; d < MD; ++d) {
-        int u = row[d];
-        if (u >= 0) s = s + dA[(long long)u * MH + c];
+    const int4* row = (const int4*)(adj + (long long)m * MD);
+    int nb[MD];
+#pragma unroll
+    for (int q = 0; q < MD / 4; ++q) {
+        int4 r4 = row[q];
+        nb[4 * q] = r4.x; nb[4 * q + 1] = r4.y; nb[4 * q + 2] = r4.z; nb[4 * q + 3] = r4.w;
     }
+    float v[MD];
+#pragma unroll
+    for (int d = 0; d < MD; ++d) v[d] = dA[(long long)(nb[d] >= 0 ? nb[d] : m) * MH + c];   // all loads in flight
+    float s = 0.0f;
+#pragma unroll
+    for (int d = 0; d < MD; ++d) s = s + (nb[d] >= 0 ? v[d] : 0.0f);   // empty slots add +0.0
     dHprev[t] = dHself[t] + s;
 }
 
@@ -480,28 +525,32 @@ __global__ __launch_bounds__(256) void k_mpnn_layer_grad(int V, const unsigned s
     if (tid < MH) out[2 * MH * MH + tid] = ((sB[tid] + sB[MH + tid]) + sB[2 * MH + tid]) + sB[3 * MH + tid];
 }
 
-// Embedding gradient over one chunk: dZ0 = dH0 ⊙ [Z0 > 0]; entry e < 256 is
-// W0[e/64][e%64] = sum_m bf(x_m[e/64]) dZ0[m][e%64], e = 256..319 is b0[e-256]
-__global__ __launch_bounds__(320) void k_mpnn_embed_grad(int V, const float4* __restrict__ feat,
-                                                         const float* __restrict__ dH0, const float* __restrict__ Z0,
-                                                         float* __restrict__ part) {
-    int e = threadIdx.x;
-    if (e >= PCP_MPNN_EMBED_NP) return;
-    int m0 = blockIdx.x * MPNN_CH, m1 = min(V, m0 + MPNN_CH);
-    int c = e & 63, f = e >> 6;   // f = 4: bias
-    float s = 0.0f;
+// Embedding gradient over one chunk of MPNN_CHV vertices: dZ0 = dH0 ⊙ [Z0 > 0]; entry e < 256 is
+// W0[e/64][e%64] = sum_m bf(x_m[e/64]) dZ0[m][e%64], e = 256..319 is b0[e-256].  One wavefront
+// per chunk, lane c owns the five entries of channel c; vertices in order.
+__global__ __launch_bounds__(64) void k_mpnn_embed_grad(int V, const float4* __restrict__ feat,
+                                                        const float* __restrict__ dH0, const float* __restrict__ Z0,
+                                                        float* __restrict__ part) {
+    int c = threadIdx.x;
+    int m0 = blockIdx.x * MPNN_CHV, m1 = min(V, m0 + MPNN_CHV);
+    float w0 = 0.0f, w1 = 0.0f, w2 = 0.0f, w3 = 0.0f, bs = 0.0f;
+#pragma unroll 4
     for (int m = m0; m < m1; ++m) {
         long long o = (long long)m * MH + c;
         float dz = Z0[o] > 0.f ? dH0[o] : 0.f;
-        if (f < 4) {
-            float4 x = feat[m];
-            float xf = f == 0 ? x.x : (f == 1 ? x.y : (f == 2 ? x.z : x.w));
-            s = s + bfr(xf) * dz;
-        } else {
-            s = s + dz;
-        }
+        float4 x = feat[m];
+        w0 = w0 + bfr(x.x) * dz;
+        w1 = w1 + bfr(x.y) * dz;
+        w2 = w2 + bfr(x.z) * dz;
+        w3 = w3 + bfr(x.w) * dz;
+        bs = bs + dz;
     }
-    part[(long long)blockIdx.x * PCP_MPNN_EMBED_NP + e] = s;
+    float* out = part + (long long)blockIdx.x * PCP_MPNN_EMBED_NP;
+    out[0 * MH + c] = w0;
+    out[1 * MH + c] = w1;
+    out[2 * MH + c] = w2;
+    out[3 * MH + c] = w3;
+    out[4 * MH + c] = bs;
 }
 
 // ---- optimizer, targets, application -----------------------------------------------------
@@ -570,6 +619,22 @@ static MpnnWork mpnn_slices(float* work, int V) {
     return w;
 }
 static int mpnn_chunks(int V) { return (V + MPNN_CH - 1) / MPNN_CH; }
+static int mpnn_chunks_v(int V) { return (V + MPNN_CHV - 1) / MPNN_CHV; }
+static int mpnn_groups(int nChunks) { return (nChunks + MPNN_GRP - 1) / MPNN_GRP; }
+static long long mpnn_part_floats(int V) {
+    return std::max((long long)mpnn_chunks(V) * PCP_MPNN_LAYER_NP, (long long)mpnn_chunks_v(V) * PCP_MPNN_EMBED_NP);
+}
+static long long mpnn_group_floats(int V) {
+    return std::max((long long)mpnn_groups(mpnn_chunks(V)) * PCP_MPNN_LAYER_NP,
+                    (long long)mpnn_groups(mpnn_chunks_v(V)) * PCP_MPNN_EMBED_NP);
+}
+// out[0..width) = sum of the chunk partials part[nChunks][width] (groups, then groups in order)
+static int mpnn_sum_parts(pcp_ctx* c, int nChunks, int width, const float* part, float* grp, float* out) {
+    int ng = mpnn_groups(nChunks);
+    PCP_LAUNCH(c, k_mpnn_sum_groups, dim3(nblk(width), ng), dim3(BLK), nChunks, width, part, grp);
+    PCP_LAUNCH(c, k_mpnn_sum_chunks, dim3(nblk(width)), dim3(BLK), ng, width, grp, out);
+    return PCP_OK;
+}
 
 }  // namespace pcp
 
@@ -581,11 +646,11 @@ int pcp_mpnn_sizes(int V, long long* workFloats, long long* gradWorkFloats) {
         return PCP_ERR_ARG;
     }
     long long S = (long long)V * PCP_MPNN_H;
-    long long nc = pcp::mpnn_chunks(V);
     if (workFloats) *workFloats = 11 * S;
     // dH (two buffers), dHself, dA (V x 64 each), loss partials (V/256 + 2 doubles),
-    // weight-gradient partials (one row per chunk)
-    if (gradWorkFloats) *gradWorkFloats = 4 * S + 2LL * (V / 256 + 2) + nc * PCP_MPNN_LAYER_NP;
+    // weight-gradient partials (one row per chunk), their group sums
+    if (gradWorkFloats)
+        *gradWorkFloats = 4 * S + 2LL * (V / 256 + 2) + pcp::mpnn_part_floats(V) + pcp::mpnn_group_floats(V);
     return PCP_OK;
 }
 
@@ -643,7 +708,7 @@ int pcp_mpnn_forward(pcp_ctx* c, int V, const float* params, const float* feat, 
         PCP_LAUNCH(c, k_mpnn_layer, dim3(G), dim3(256), V, W, W + 2 * MH * MH, w.H[l - 1], adj, w.H[l], w.Z[l],
                    w.X[l]);
     }
-    PCP_LAUNCH(c, k_mpnn_out, dim3(nblk(V)), dim3(BLK), V, params, w.H[3], (float2*)out);
+    PCP_LAUNCH(c, k_mpnn_out, dim3((V + 127) / 128), dim3(128), V, params, w.H[3], (float2*)out);
     return PCP_OK;
 }
 
@@ -660,7 +725,7 @@ int pcp_mpnn_loss(pcp_ctx* c, int V, const float* out, const float* target, floa
     PCP_LAUNCH(c, k_mpnn_loss, dim3(nb), dim3(256), V, (const float2*)out, (const float2*)target, (float2*)dOut,
                part);
     double* dl = part + nb;
-    PCP_LAUNCH(c, k_mpnn_sum_loss, dim3(1), dim3(64), nb, part, 1.0 / (double)V, dl);
+    PCP_LAUNCH(c, k_mpnn_sum_loss, dim3(1), dim3(256), nb, part, 1.0 / (double)V, dl);
     if (loss) {
         PCP_HIP(hipMemcpyAsync(loss, dl, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         PCP_HIP(hipStreamSynchronize(c->stream));
@@ -682,28 +747,27 @@ int pcp_mpnn_backward(pcp_ctx* c, int V, const float* params, const float* feat,
     float* dHs = gradWork + 2 * S;
     float* dA = gradWork + 3 * S;
     float* part = gradWork + 4 * S + 2LL * (V / 256 + 2);
-    int nc = mpnn_chunks(V);
+    float* grp = part + mpnn_part_floats(V);
+    int nc = mpnn_chunks(V), ncv = mpnn_chunks_v(V);
     int G = (V + 63) / 64;
     // output layer
-    PCP_LAUNCH(c, k_mpnn_out_grad, dim3(nc), dim3(192), V, w.H[3], (const float2*)dOut, part);
-    PCP_LAUNCH(c, k_mpnn_sum_chunks, dim3(1), dim3(256), nc, PCP_MPNN_OUT_NP, part, grad + PCP_MPNN_OFF_WO);
+    PCP_LAUNCH(c, k_mpnn_out_grad, dim3(ncv), dim3(64), V, w.H[3], (const float2*)dOut, part);
+    PCP_TRY(mpnn_sum_parts(c, ncv, PCP_MPNN_OUT_NP, part, grp, grad + PCP_MPNN_OFF_WO));
     PCP_LAUNCH(c, k_mpnn_out_bwd, dim3(nblk(S)), dim3(BLK), V, params, (const float2*)dOut, dHa);
     float* dH = dHa;
     float* dHn = dHb;
     for (int l = 3; l >= 1; --l) {
         const float* W = params + PCP_MPNN_OFF_W1 + (l - 1) * PCP_MPNN_LAYER_NP;
         PCP_LAUNCH(c, k_mpnn_layer_grad, dim3(nc), dim3(256), V, w.X[l], dH, w.Z[l], part);
-        PCP_LAUNCH(c, k_mpnn_sum_chunks, dim3(nblk(PCP_MPNN_LAYER_NP)), dim3(BLK), nc, PCP_MPNN_LAYER_NP, part,
-                   grad + PCP_MPNN_OFF_W1 + (l - 1) * PCP_MPNN_LAYER_NP);
+        PCP_TRY(mpnn_sum_parts(c, nc, PCP_MPNN_LAYER_NP, part, grp, grad + PCP_MPNN_OFF_W1 + (l - 1) * PCP_MPNN_LAYER_NP));
         PCP_LAUNCH(c, k_mpnn_layer_bwd, dim3(G), dim3(256), V, W, dH, w.Z[l], dHs, dA);
         PCP_LAUNCH(c, k_mpnn_gather_bwd, dim3(nblk(S)), dim3(BLK), V, adj, dHs, dA, dHn);
         float* t = dH;
         dH = dHn;
         dHn = t;
     }
-    PCP_LAUNCH(c, k_mpnn_embed_grad, dim3(nc), dim3(320), V, (const float4*)feat, dH, w.Z[0], part);
-    PCP_LAUNCH(c, k_mpnn_sum_chunks, dim3(nblk(PCP_MPNN_EMBED_NP)), dim3(BLK), nc, PCP_MPNN_EMBED_NP, part,
-               grad + PCP_MPNN_OFF_W0);
+    PCP_LAUNCH(c, k_mpnn_embed_grad, dim3(ncv), dim3(64), V, (const float4*)feat, dH, w.Z[0], part);
+    PCP_TRY(mpnn_sum_parts(c, ncv, PCP_MPNN_EMBED_NP, part, grp, grad + PCP_MPNN_OFF_W0));
     return PCP_OK;
 }
 

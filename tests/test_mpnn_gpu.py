@@ -141,9 +141,10 @@ def test_forward_layers_match_oracle(seed):
     assert np.linalg.norm(y - yo) <= 1e-2 * np.linalg.norm(yo)
 
 
-@pytest.mark.parametrize("seed", [0, 1])
+@pytest.mark.parametrize("seed", [0, 1, 2])
 def test_backward_matches_oracle(seed):
-    s, pk = evaluated_system(P=2, seed=20 + seed)
+    # seed 2: 32768 vertices -- 128 chunks of the VALU gradients in two groups
+    s, pk = evaluated_system(P=2, seed=20 + seed) if seed < 2 else evaluated_system(P=2, N=1024, n=16, seed=22)
     adjd, adj, _ = gpu_graph(s)
     V = s.V
     rng = np.random.default_rng(100 + seed)
@@ -252,7 +253,7 @@ def test_trained_warm_start_gain_over_plain_fire():
     of other seeds) against plain FIRE on 32 held-out packings of N=1024 x 16: both reach
     max|F| <= 1e-10 (FP64 FIRE judges the residual force) and the warm start needs fewer
     FIRE steps on average.  Deterministic end to end, so the numbers are exact for this
-    seed: plain 297.8 mean steps, warm start 266 (one application)."""
+    seed: plain 297.8 mean steps, warm start 270.0 (one application)."""
     pk = W.make_c4_packings(32, 777777)
     res = W.ab_compare(W.load_weights(W.DEFAULT_WEIGHTS), pk, maxSteps=3000, dtMax=W.DT_MAX,
                        log=lambda m: print(m, flush=True))
