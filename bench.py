@@ -12,8 +12,9 @@ Alongside: a live roofline of one kernel (HIP events on the context's stream aro
 every launch of that kernel inside the timed region), the CPU oracle timed on the
 host cores on a bounded sample of the same workload, and bounded FIRE legs reporting
 packings minimized/s to max|F| <= 1e-10: the N=4096 packing (reference dtMax, scaled
-dtMax, learned warm start), 32 packings of N=1024 x 16 in one context (plain vs learned),
-and 256 packings of uneven N and n in one context.
+dtMax, learned warm start), 32 packings of N=4096 x 16 in one context (plain vs learned,
+against the one-packing contexts), 32 packings of N=1024 x 16 in one context (plain vs
+learned), 256 packings of uneven N and n in one context, and the compression protocol.
 """
 import argparse
 import ctypes
@@ -122,6 +123,49 @@ def fire_legs(args):
         out[tag] = fire_one([p2], dt, steps, warm, budget_s=args.fire_seconds, graph=not args.no_graph)
         log(out[tag], tag)
     from pycudapolygon_amd import batch
+    if args.n4096_batch > 0:
+        # P packings of the N=4096 workload in one context (the first is the packing of the
+        # single-packing legs above), plain and with the learned warm start applied to the
+        # whole context once, re-packed when fewer than half are still running
+        P = args.n4096_batch
+        t0 = time.perf_counter()
+        pb = [p2] + [make_packing(4096, 16, 0.85, seed=args.seed + 1000 + k, min_center_frac=0.5)
+                     for k in range(P - 1)]
+        gen = time.perf_counter() - t0
+        for tag, warm, single in (("n4096_batch", None, "n4096"), ("n4096_batch_learned", ws, "n4096_learned")):
+            r = batch.minimize_batch(pb, W.SQUARED, dict(fTol=1e-10, dtMax=W.DT_MAX), 5000, warm=warm,
+                                     graph=not args.no_graph)
+            ok = r.steps >= 0
+            e0 = out[single]["median_final_energy"]
+            out[tag] = dict(packings=P, workload=(f"{P} packings of N=4096 x 16, phi=0.85 (seeds {args.seed} and "
+                                                 f"{args.seed + 1000}..{args.seed + 998 + P}), one context, "
+                                                 "re-packed when fewer than half are still running"),
+                            warm_start=warm is not None, converged=int(ok.sum()), seconds=r.seconds,
+                            packings_per_s=float(ok.sum()) / r.seconds,
+                            mean_steps=float(r.steps[ok].mean()) if ok.any() else None,
+                            max_steps=int(r.steps[ok].max()) if ok.any() else None, contexts=r.contexts,
+                            max_final_force=float(r.max_force.max()), dtMax=W.DT_MAX, generation_seconds=gen,
+                            packing0_steps=int(r.steps[0]), packing0_steps_single_context=out[single]["batch_steps"],
+                            packing0_final_energy=float(r.energy[0]),
+                            packing0_final_energy_rel_diff_single_context=abs(float(r.energy[0]) - e0) / abs(e0))
+            print(f"[bench] FIRE {tag}: {int(ok.sum())}/{P} converged, mean steps {out[tag]['mean_steps']}, "
+                  f"{r.seconds:.3f} s, {out[tag]['packings_per_s']:.2f} packings/s; packing 0: "
+                  f"{r.steps[0]} steps, E rel. diff vs single context "
+                  f"{out[tag]['packing0_final_energy_rel_diff_single_context']:.1e}", flush=True)
+        plain1 = out["n4096"]["packings_per_s"]
+        out["n4096_summary"] = dict(
+            plain_single_context=plain1, learned_single_context=out["n4096_learned"]["packings_per_s"],
+            plain_batched=out["n4096_batch"]["packings_per_s"],
+            learned_batched=out["n4096_batch_learned"]["packings_per_s"],
+            learned_batched_over_plain_single=out["n4096_batch_learned"]["packings_per_s"] / plain1,
+            learned_batched_over_plain_batched=(out["n4096_batch_learned"]["packings_per_s"]
+                                                / out["n4096_batch"]["packings_per_s"]),
+            batching_alone=out["n4096_batch"]["packings_per_s"] / plain1,
+            note="plain GPU FIRE = one packing per context; the learned-only gain is learned_batched_over_plain_batched")
+        print(f"[bench] N=4096 packings/s: plain single {plain1:.1f}, plain batched "
+              f"{out['n4096_summary']['plain_batched']:.1f}, learned batched "
+              f"{out['n4096_summary']['learned_batched']:.1f} "
+              f"({out['n4096_summary']['learned_batched_over_plain_single']:.2f}x plain single)", flush=True)
     p4 = W.make_c4_packings(32, 777777)
     for tag, warm in (("n1024_x32", None), ("n1024_x32_learned", ws)):
         r = batch.minimize_batch(p4, W.SQUARED, dict(fTol=1e-10, dtMax=W.DT_MAX), 5000, warm=warm,
@@ -201,6 +245,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fire-seconds", type=float, default=20.0)
+    ap.add_argument("--n4096-batch", type=int, default=32,
+                    help="packings of the N=4096 workload minimized together in one context (0: skip)")
     ap.add_argument("--batch-packings", type=int, default=256)
     ap.add_argument("--compress-replicas", type=int, default=16)
     ap.add_argument("--compress-seconds", type=float, default=90.0)

@@ -42,7 +42,9 @@ _p = ctypes.c_void_p
 class _System(ctypes.Structure):
     _fields_ = [("V", _i), ("N", _i), ("boxSize", _i), ("model", _i), ("maxNeighbors", _i),
                 ("start", _p), ("targetEdgeV", _p), ("targetAreas", _p),
-                ("stiffness", _d), ("compressibility", _d)]
+                ("stiffness", _d), ("compressibility", _d), ("solver", _i)]
+
+SOLVERS = {"dense": 0, "lanes": 1}
 
 
 class FireState(ctypes.Structure):
@@ -65,6 +67,10 @@ def _declare(L):
     L.orc_project_force.argtypes = [_i, _i, _p, _p, _p]
     L.orc_shake.restype = _i
     L.orc_shake.argtypes = [_i, _i, _p, _p, _p, _p, _i, _d]
+    L.orc_project_force_lanes.restype = _i
+    L.orc_project_force_lanes.argtypes = [_i, _i, _p, _p, _p]
+    L.orc_shake_lanes.restype = _i
+    L.orc_shake_lanes.argtypes = [_i, _i, _p, _p, _p, _p, _i, _d]
     L.orc_xpbd.restype = _i
     L.orc_xpbd.argtypes = [_i, _i, _p, _p, _p, _p, _i, _d]
     L.orc_det_sum.restype = _d
@@ -201,6 +207,23 @@ def shake(pos, start, targetEdgeV, targetAreas, maxIter, tol=1e-15):
     return pos, it
 
 
+def project_force_lanes(pos, start, force):
+    pos = _f64(pos); start = _i32(start); f = _f64(force).copy()
+    rc = lib().orc_project_force_lanes(int(start[-1]), len(start) - 1, _ptr(start), _ptr(pos), _ptr(f))
+    if rc != 0:
+        raise ValueError("lane-order projection takes polygons of 3..64 vertices")
+    return f
+
+
+def shake_lanes(pos, start, targetEdgeV, targetAreas, maxIter, tol=1e-15):
+    pos = _f64(pos).copy(); start = _i32(start)
+    it = lib().orc_shake_lanes(int(start[-1]), len(start) - 1, _ptr(start), _ptr(pos), _ptr(_f64(targetEdgeV)),
+                               _ptr(_f64(targetAreas)), int(maxIter), float(tol))
+    if it < 0:
+        raise ValueError("lane-order SHAKE takes polygons of 3..64 vertices")
+    return pos, it
+
+
 def xpbd(pos, start, targetEdgeLengths, targetAreas, nIter, tol=0.0):
     pos = _f64(pos).copy(); start = _i32(start)
     it = lib().orc_xpbd(int(start[-1]), len(start) - 1, _ptr(start), _ptr(pos),
@@ -226,14 +249,16 @@ class System:
     """Holds the arrays behind an orc_system so they outlive the ctypes struct."""
 
     def __init__(self, start, boxSize, model="normal", targetEdgeV=None, targetAreas=None,
-                 stiffness=0.0, compressibility=0.0, maxNeighbors=100):
+                 stiffness=0.0, compressibility=0.0, maxNeighbors=100, solver="dense"):
+        """solver: "dense" (Householder projection, dense Gauss-Newton SHAKE) or "lanes"
+        (the same solves in the HIP kernels' lane order, bit-identical to the library)."""
         self.start = _i32(start)
         N = len(self.start) - 1; V = int(self.start[-1])
         self.tev = _f64(targetEdgeV if targetEdgeV is not None else np.zeros(V))
         self.ta = _f64(targetAreas if targetAreas is not None else np.zeros(N))
         self.s = _System(V, N, int(boxSize), MODEL_IDS[model] if isinstance(model, str) else int(model),
                          int(maxNeighbors), _ptr(self.start), _ptr(self.tev), _ptr(self.ta),
-                         float(stiffness), float(compressibility))
+                         float(stiffness), float(compressibility), SOLVERS[solver])
 
     @property
     def V(self):

@@ -994,6 +994,216 @@ int orc_shake(int V, int N, const int* start, double* pos, const double* targetE
     return worst;
 }
 
+/* ---------- the same projection and SHAKE in the HIP kernels' lane order ----------
+ * projectForceCUDA model.cu:165-206 and the SHAKE contract above, solved the way the
+ * library's segment kernels solve them (one lane e per vertex/edge of the polygon):
+ * the normal equations [[G, b], [b^T, c]] [y; z] = [r; s] of the n edge-length
+ * constraints and the area constraint, G cyclic tridiagonal.  The leading (n-1)x(n-1)
+ * block is solved by parallel cyclic reduction for three right-hand sides at once, then
+ * the cyclic corner and the area row by their 1x1 Schur complements.  Sums over the
+ * polygon are butterfly all-reductions over the next power of two >= n lanes (zero
+ * padding).  Every operation is written in the kernels' order, so FIRE with these
+ * solves follows the GPU's trajectory bit for bit; the mathematics is pinned by the
+ * dense solvers above (orc_project_force: Householder QR; orc_shake: Gaussian
+ * elimination), which these must equal to rounding.  Polygons of 3..64 vertices. */
+#define LANES_MAX 64
+static double lanes_sum(const double* v, int n) {
+    double a[LANES_MAX], b[LANES_MAX];
+    int P = 1;
+    while (P < n) P <<= 1;
+    for (int e = 0; e < P; ++e) a[e] = (e < n) ? v[e] : 0.0;
+    for (int o = P / 2; o >= 1; o >>= 1) {
+        for (int e = 0; e < P; ++e) b[e] = a[e] + a[e ^ o];
+        for (int e = 0; e < P; ++e) a[e] = b[e];
+    }
+    return a[0];
+}
+
+/* y[e] (edge multipliers) and *zOut (area multiplier) for edges e < n.  t = unit edge
+ * tangents, a = area gradient, r = edge right-hand side, c = |a|^2, s = area
+ * right-hand side; the area direction is dropped when its Schur complement is
+ * <= zThresh * c. */
+static void lanes_bordered_solve(int n, const double* tx, const double* ty, const double* ax,
+                                 const double* ay, const double* r, double c, double s,
+                                 double zThresh, double* y, double* zOut) {
+    double off[LANES_MAX] = {0}, diag[LANES_MAX] = {0}, bv[LANES_MAX] = {0};
+    double A[LANES_MAX] = {0}, C[LANES_MAX] = {0}, Bd[LANES_MAX] = {0}, d1[LANES_MAX] = {0},
+           d2[LANES_MAX] = {0}, d3[LANES_MAX] = {0};
+    double inv[LANES_MAX] = {0}, nA[LANES_MAX] = {0}, nC[LANES_MAX] = {0}, nB[LANES_MAX] = {0},
+           n1[LANES_MAX] = {0}, n2[LANES_MAX] = {0}, n3[LANES_MAX] = {0};
+    double x1[LANES_MAX] = {0}, x2[LANES_MAX] = {0}, x3[LANES_MAX] = {0}, u[LANES_MAX] = {0},
+           tmp[LANES_MAX] = {0};
+    int m = n - 1;
+    for (int e = 0; e < n; ++e) {
+        int en = (e + 1 >= n) ? 0 : e + 1;
+        off[e] = -(tx[e] * tx[en] + ty[e] * ty[en]);      /* G_{e,e+1 mod n} */
+        double dsq = tx[e] * tx[e] + ty[e] * ty[e];
+        diag[e] = (dsq > 0.0) ? 2.0 * dsq : 1.0;
+        bv[e] = tx[e] * (ax[en] - ax[e]) + ty[e] * (ay[en] - ay[e]);
+    }
+    double offM = off[m], offM1 = off[m - 1];
+    double rLast = r[m], bLast = bv[m], diagLast = diag[m];
+    for (int e = 0; e < n; ++e) {
+        int inb = e < m;
+        double offPrev = off[(e == 0) ? 0 : e - 1];
+        A[e] = (inb && e > 0) ? offPrev : 0.0;
+        C[e] = (inb && e < m - 1) ? off[e] : 0.0;
+        Bd[e] = inb ? diag[e] : 1.0;
+        d1[e] = inb ? r[e] : 0.0;
+        d2[e] = inb ? bv[e] : 0.0;
+        d3[e] = 0.0;
+        if (inb) {
+            if (e == 0) d3[e] += offM;
+            if (e == m - 1) d3[e] += offM1;
+        }
+    }
+    double w0 = offM + ((m - 1 == 0) ? offM1 : 0.0);
+    double wl = (m - 1 == 0) ? 0.0 : offM1;
+    for (int st = 1; st < m; st <<= 1) {                  /* one PCR step, all lanes at once */
+        for (int e = 0; e < n; ++e) inv[e] = 1.0 / Bd[e];
+        for (int e = 0; e < m; ++e) {
+            int lo = e - st, hi = e + st;
+            int hasLo = lo >= 0, hasHi = hi < m;
+            double al = hasLo ? -A[e] * inv[lo] : 0.0;
+            double ga = hasHi ? -C[e] * inv[hi] : 0.0;
+            nA[e] = hasLo ? al * A[lo] : 0.0;
+            nC[e] = hasHi ? ga * C[hi] : 0.0;
+            nB[e] = Bd[e]; n1[e] = d1[e]; n2[e] = d2[e]; n3[e] = d3[e];
+            if (hasLo) { nB[e] += al * C[lo]; n1[e] += al * d1[lo]; n2[e] += al * d2[lo]; n3[e] += al * d3[lo]; }
+            if (hasHi) { nB[e] += ga * A[hi]; n1[e] += ga * d1[hi]; n2[e] += ga * d2[hi]; n3[e] += ga * d3[hi]; }
+        }
+        for (int e = 0; e < m; ++e) {
+            A[e] = nA[e]; C[e] = nC[e]; Bd[e] = nB[e]; d1[e] = n1[e]; d2[e] = n2[e]; d3[e] = n3[e];
+        }
+    }
+    for (int e = 0; e < n; ++e) {
+        int inb = e < m;
+        x1[e] = inb ? d1[e] / Bd[e] : 0.0;
+        x2[e] = inb ? d2[e] / Bd[e] : 0.0;
+        x3[e] = inb ? d3[e] / Bd[e] : 0.0;
+    }
+    double wX1 = w0 * x1[0] + wl * x1[m - 1];
+    double wX2 = w0 * x2[0] + wl * x2[m - 1];
+    double wX3 = w0 * x3[0] + wl * x3[m - 1];
+    double schur = diagLast - wX3;
+    double y1l = (rLast - wX1) / schur;
+    double ul = (bLast - wX2) / schur;
+    for (int e = 0; e < n; ++e) {
+        int inb = e < m;
+        y[e] = inb ? x1[e] - x3[e] * y1l : ((e == m) ? y1l : 0.0);
+        u[e] = inb ? x2[e] - x3[e] * ul : ((e == m) ? ul : 0.0);
+    }
+    for (int e = 0; e < n; ++e) tmp[e] = bv[e] * u[e];
+    double bu = lanes_sum(tmp, n);
+    for (int e = 0; e < n; ++e) tmp[e] = bv[e] * y[e];
+    double by = lanes_sum(tmp, n);
+    double sigma = c - bu;
+    double z = (sigma > zThresh * c) ? (s - by) / sigma : 0.0;
+    *zOut = z;
+    for (int e = 0; e < n; ++e) y[e] = y[e] - z * u[e];
+}
+
+/* projection of each polygon's force (getConstrainedForcePython pyCudaPolygon.py:441-487) */
+int orc_project_force_lanes(int V, int N, const int* start, const double* pos, double* force) {
+    (void)V;
+    for (int s = 0; s < N; ++s)
+        if (start[s + 1] - start[s] < 3 || start[s + 1] - start[s] > LANES_MAX) return -1;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int s = 0; s < N; ++s) {
+        int st = start[s], n = start[s + 1] - st;
+        double tx[LANES_MAX], ty[LANES_MAX], ax[LANES_MAX], ay[LANES_MAX], r[LANES_MAX], y[LANES_MAX];
+        double sq[LANES_MAX], af[LANES_MAX], z;
+        for (int e = 0; e < n; ++e) {
+            int ef = (e + 1 >= n) ? 0 : e + 1, eg = (e == 0) ? n - 1 : e - 1;
+            const double* pe = pos + 2 * (st + e);
+            const double* pf = pos + 2 * (st + ef);
+            const double* pg = pos + 2 * (st + eg);
+            double dx = orc_wrap(pf[0] - pe[0]), dy = orc_wrap(pf[1] - pe[1]);
+            double len = sqrt(dx * dx + dy * dy);
+            tx[e] = 0.0; ty[e] = 0.0;
+            if (!(len < 1e-14)) { tx[e] = dx / len; ty[e] = dy / len; }
+            ax[e] = orc_wrap(pf[1] - pg[1]);
+            ay[e] = orc_wrap(pg[0] - pf[0]);
+            double fx = force[2 * (st + e)], fy = force[2 * (st + e) + 1];
+            double ffx = force[2 * (st + ef)], ffy = force[2 * (st + ef) + 1];
+            r[e] = tx[e] * (ffx - fx) + ty[e] * (ffy - fy);
+            sq[e] = ax[e] * ax[e] + ay[e] * ay[e];
+            af[e] = ax[e] * fx + ay[e] * fy;
+        }
+        double c = lanes_sum(sq, n), sd = lanes_sum(af, n);
+        lanes_bordered_solve(n, tx, ty, ax, ay, r, c, sd, 1e-28, y, &z);
+        double fo[2 * LANES_MAX];
+        for (int e = 0; e < n; ++e) {
+            int eg = (e == 0) ? n - 1 : e - 1;
+            double px = -y[e] * tx[e] + y[eg] * tx[eg] + z * ax[e];
+            double py = -y[e] * ty[e] + y[eg] * ty[eg] + z * ay[e];
+            fo[2 * e] = force[2 * (st + e)] - px;
+            fo[2 * e + 1] = force[2 * (st + e) + 1] - py;
+        }
+        for (int q = 0; q < 2 * n; ++q) force[2 * st + q] = fo[q];
+    }
+    return 0;
+}
+
+/* Gauss-Newton iterations of one polygon (positions p[2n]) until max|C| < tol or maxIter */
+static int lanes_shake_poly(int n, double* p, const double* L, double A0, int maxIter, double tol) {
+    double tx[LANES_MAX], ty[LANES_MAX], ax[LANES_MAX], ay[LANES_MAX], ce[LANES_MAX], y[LANES_MAX];
+    double sq[LANES_MAX], part[LANES_MAX], z;
+    int it = 0;
+    for (; it < maxIter; ++it) {
+        double cmax = 0.0;
+        for (int e = 0; e < n; ++e) {
+            int ef = (e + 1 >= n) ? 0 : e + 1, eg = (e == 0) ? n - 1 : e - 1;
+            double pfx = p[2 * ef], pfy = p[2 * ef + 1], pgx = p[2 * eg], pgy = p[2 * eg + 1];
+            double dx = orc_wrap(pfx - p[2 * e]), dy = orc_wrap(pfy - p[2 * e + 1]);
+            double len = sqrt(dx * dx + dy * dy);
+            ce[e] = len - L[e];
+            tx[e] = 0.0; ty[e] = 0.0;
+            if (len > 0.0) { tx[e] = dx / len; ty[e] = dy / len; }
+            ax[e] = orc_wrap(pfy - pgy) / 2;
+            ay[e] = orc_wrap(pgx - pfx) / 2;
+            double x1 = orc_wrap(p[2 * e] - p[0]);
+            double dy2 = orc_wrap(pfy - p[2 * e + 1]);
+            double x2 = orc_wrap(pfx - p[0]);
+            part[e] = dy2 * (x2 + x1) / 2.0;
+            sq[e] = ax[e] * ax[e] + ay[e] * ay[e];
+            cmax = fmax(cmax, fabs(ce[e]));
+        }
+        double c = lanes_sum(sq, n);
+        double cA = lanes_sum(part, n) - A0;
+        cmax = fmax(cmax, fabs(cA));
+        if (cmax < tol) break;
+        lanes_bordered_solve(n, tx, ty, ax, ay, ce, c, cA, 1e-28, y, &z);
+        double q[2 * LANES_MAX];
+        for (int e = 0; e < n; ++e) {
+            int eg = (e == 0) ? n - 1 : e - 1;
+            double dx = -y[e] * tx[e] + y[eg] * tx[eg] + z * ax[e];
+            double dy = -y[e] * ty[e] + y[eg] * ty[eg] + z * ay[e];
+            double qx = p[2 * e] - dx, qy = p[2 * e + 1] - dy;
+            q[2 * e] = qx - floor(qx);
+            q[2 * e + 1] = qy - floor(qy);
+        }
+        for (int k = 0; k < 2 * n; ++k) p[k] = q[k];
+    }
+    return it;
+}
+
+int orc_shake_lanes(int V, int N, const int* start, double* pos, const double* targetEdgeV,
+                    const double* targetAreas, int maxIter, double tol) {
+    (void)V;
+    if (N == 0 || maxIter == 0) return 0;
+    for (int s = 0; s < N; ++s)
+        if (start[s + 1] - start[s] < 3 || start[s + 1] - start[s] > LANES_MAX) return -1;
+    int worst = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(max : worst)
+    for (int s = 0; s < N; ++s) {
+        int st = start[s], n = start[s + 1] - st;
+        int it = lanes_shake_poly(n, pos + 2 * st, targetEdgeV + st, targetAreas[s], maxIter, tol);
+        if (it > worst) worst = it;
+    }
+    return worst;
+}
+
 /* ---------- XPBD: xpbdProjectCUDA model.cu:209-278, kernels.cuh:532-623 ----------
  * Colour 0 then colour 1 edge projections (disjoint within a colour for even n;
  * here applied in ascending edge order, which is the reference's result for even n),
@@ -1190,8 +1400,16 @@ static double forces(const orc_system* s, const double* pos, orc_ws* w, double* 
     double E = orc_force_energy(s->model, s->V, s->N, s->start, pos, w->I, w->inter, w->outer,
                                 w->tu, w->ut, w->edgeLengths, w->areas, s->targetEdgeV, s->targetAreas,
                                 s->stiffness, s->compressibility, force);
-    if ((s->model & 0xFF) == ORC_NORMAL) orc_project_force(s->V, s->N, s->start, pos, force);
+    if ((s->model & 0xFF) == ORC_NORMAL) {
+        if (s->solver == ORC_SOLVER_LANES) orc_project_force_lanes(s->V, s->N, s->start, pos, force);
+        else orc_project_force(s->V, s->N, s->start, pos, force);
+    }
     return E;
+}
+static int fire_shake(const orc_system* s, double* pos, int maxIter) {
+    if (s->solver == ORC_SOLVER_LANES)
+        return orc_shake_lanes(s->V, s->N, s->start, pos, s->targetEdgeV, s->targetAreas, maxIter, 1e-15);
+    return orc_shake(s->V, s->N, s->start, pos, s->targetEdgeV, s->targetAreas, maxIter, 1e-15);
 }
 
 double orc_force_eval(const orc_system* s, const double* pos, double* force, int* numIntersections) {
@@ -1265,10 +1483,10 @@ void orc_fire_step(const orc_system* s, double* pos, double* vel, double* force,
         if (!areaMode) detect(s, pos, &w);   /* reference order: detect, SHAKE, records */
         if (rigid && shakeIter > 0) {
             memcpy(tmp, pos, sizeof(double) * V2);
-            int it = orc_shake(s->V, s->N, s->start, pos, s->targetEdgeV, s->targetAreas, shakeIter, 1e-15);
+            int it = fire_shake(s, pos, shakeIter);
             st->lastShakeIters = it;
             if (it >= shakeIter) {
-                it = orc_shake(s->V, s->N, s->start, pos, s->targetEdgeV, s->targetAreas, shakeIter * 10, 1e-15);
+                it = fire_shake(s, pos, shakeIter * 10);
                 st->lastShakeIters = it;
                 if (it >= shakeIter * 10) shakeFailed = 1;
             }

@@ -58,6 +58,11 @@ double orc_force_energy(int model, int V, int N, const int* start, const double*
 void orc_project_force(int V, int N, const int* start, const double* pos, double* force);
 int orc_shake(int V, int N, const int* start, double* pos, const double* targetEdgeV,
               const double* targetAreas, int maxIter, double tol);
+/* the same two solves in the HIP kernels' lane order (polygons of 3..64 vertices;
+ * -1 otherwise): bit-identical to the library, equal to the two above to rounding */
+int orc_project_force_lanes(int V, int N, const int* start, const double* pos, double* force);
+int orc_shake_lanes(int V, int N, const int* start, double* pos, const double* targetEdgeV,
+                    const double* targetAreas, int maxIter, double tol);
 int orc_xpbd(int V, int N, const int* start, double* pos, const double* targetEdgeP,
              const double* targetAreas, int nIter, double tol);
 double orc_det_sum(const double* x, int n);
@@ -75,7 +80,9 @@ typedef struct {
     const double* targetEdgeV;   /* [V] per-edge target length (edge m -> next(m)) */
     const double* targetAreas;   /* [N] */
     double stiffness, compressibility;
+    int solver;                  /* projection/SHAKE: ORC_SOLVER_DENSE or ORC_SOLVER_LANES */
 } orc_system;
+enum { ORC_SOLVER_DENSE = 0, ORC_SOLVER_LANES = 1 };
 double orc_force_eval(const orc_system* s, const double* pos, double* force, int* numIntersections);
 
 /* FIRE (host contract model.cpp:405-504) */

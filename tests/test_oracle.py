@@ -305,6 +305,68 @@ def test_shake_satisfies_constraints():
     assert it2 == 0 and np.array_equal(out2, out)
 
 
+@pytest.mark.parametrize("n", [6, 7, 8, 12, 16, 23, 32, 64])
+def test_lane_order_solves_equal_dense_solves(n):
+    """The lane-order projection and SHAKE (parallel cyclic reduction, butterfly sums:
+    the HIP kernels' operation order) against the independent dense solvers (Householder
+    QR; Gauss-Newton with Gaussian elimination): the same projected force to 1e-12
+    relative, the same SHAKE positions to 1e-13, constraints met, same iteration counts,
+    and the lane-order projection is idempotent and keeps rigid translations.  A wrong
+    index, sign or dropped Schur term in the reduction fails the dense comparison.
+    (Near-regular 4- and 5-gons are left out: they are close to the cyclic polygon, where
+    the area is stationary under the polygon's one or two flexes, so the area row is
+    nearly dependent on the edge rows and two exact methods differ by up to ~1e-9.)"""
+    p = make_packing(32, n, 0.8, seed=n)
+    start, pos = p["startIndices"], p["positions"]
+    rng = np.random.default_rng(n)
+    f = rng.normal(size=len(pos))
+    fd = O.project_force(pos, start, f)
+    fl = O.project_force_lanes(pos, start, f)
+    assert np.abs(fl - fd).max() <= 1e-12 * np.abs(f).max()
+    np.testing.assert_allclose(O.project_force_lanes(pos, start, fl), fl, rtol=0, atol=1e-13 * np.abs(f).max())
+    tr = np.tile([1.0, 0.0], len(pos) // 2)
+    np.testing.assert_allclose(O.project_force_lanes(pos, start, tr), tr, rtol=0, atol=1e-12)
+    noisy = (pos + rng.normal(scale=2e-4, size=pos.shape)) % 1.0
+    xd, itd = O.shake(noisy, start, p["targetEdgeV"], p["targetAreas"], 50)
+    xl, itl = O.shake_lanes(noisy, start, p["targetEdgeV"], p["targetAreas"], 50)
+    d = (xl - xd + 0.5) % 1.0 - 0.5
+    assert np.abs(d).max() < 1e-13 and 0 < itl < 50 and abs(itl - itd) <= 1
+    g = O.geometry(xl, start)
+    np.testing.assert_allclose(g["edgeLengths"], p["targetEdgeV"], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(g["areas"], p["targetAreas"], rtol=0, atol=1e-15)
+    x2, it2 = O.shake_lanes(xl, start, p["targetEdgeV"], p["targetAreas"], 50)
+    assert it2 == 0 and np.array_equal(x2, xl)
+
+
+def test_lane_order_solves_reject_polygons_beyond_64():
+    p = make_packing(4, 65, 0.5, seed=0)
+    with pytest.raises(ValueError):
+        O.project_force_lanes(p["positions"], p["startIndices"], np.zeros(len(p["positions"])))
+    with pytest.raises(ValueError):
+        O.shake_lanes(p["positions"], p["startIndices"], p["targetEdgeV"], p["targetAreas"], 5)
+
+
+def test_fire_with_lane_order_solves_tracks_dense_fire():
+    """The FIRE oracle with either solver pair: same step decisions and positions within
+    1e-12 over the first 30 steps (squared lens areas, rigid polygons)."""
+    p = make_packing(64, 12, 0.80, seed=0, min_center_frac=0.5)
+    B = box_size_for(p["maxEdgeLength"])
+    runs = []
+    for solver in ("dense", "lanes"):
+        sysO = O.System(p["startIndices"], B, 0x200, p["targetEdgeV"], p["targetAreas"], solver=solver)
+        pos = p["positions"].copy()
+        st, vel, f = sysO.fire_init(pos, 1e-3, 0.1)
+        hist = []
+        for _ in range(30):
+            sysO.fire_step(pos, vel, f, st, dtMax=100.0)
+            hist.append((st.rolledBack, st.shakeFailed, st.nPos, st.dt, pos.copy(), st.energy))
+        runs.append(hist)
+    for a, b in zip(*runs):
+        assert a[:4] == b[:4]
+        assert np.abs((a[4] - b[4] + 0.5) % 1.0 - 0.5).max() < 1e-12
+        assert abs(a[5] - b[5]) <= 1e-9 * abs(b[5])
+
+
 @pytest.mark.parametrize("seed", range(4))
 def test_squared_lens_energy_closed_form(seed):
     """Squared lens areas on one lens between convex polygons: E = A^2/2 with A the
