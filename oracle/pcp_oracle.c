@@ -594,8 +594,9 @@ double orc_force_energy(int model, int V, int N, const int* start, const double*
     int* next = (int*)malloc(sizeof(int) * (V > 0 ? V : 1));
     int* prev = (int*)malloc(sizeof(int) * (V > 0 ? V : 1));
     orc_topology(N, start, V, shapeId, next, prev);
-    int sqMode = (model & ORC_AREA_SQUARED) != 0;
-    int areaMode = (model & (ORC_AREA_ENERGY | ORC_AREA_SQUARED)) != 0;
+    int pairMode = (model & ORC_PAIR_SQUARED) != 0;
+    int sqMode = (model & (ORC_AREA_SQUARED | ORC_PAIR_SQUARED)) != 0;
+    int areaMode = (model & ORC_AREA_MODES) != 0;
     model &= 0xFF;
     int overlap = (model == ORC_NORMAL || model == ORC_HYBRID);
     int edgeOn = (model == ORC_SOFT_BODY || model == ORC_EDGE_ONLY || model == ORC_HYBRID);
@@ -654,6 +655,13 @@ double orc_force_energy(int model, int V, int N, const int* start, const double*
      * at its smallest key; a record off a cycle is its own lens.  Contributions scale by A. */
     double* lensA = (double*)malloc(sizeof(double) * Ialloc);
     int* lensN = (int*)malloc(sizeof(int) * Ialloc);
+    /* with ORC_PAIR_SQUARED: the scale of a record's contributions is the overlap area of
+     * its polygon pair, pairA = sum of the pair's lens areas (one representative record
+     * per lens: the smallest key of its cycle, or a record off any cycle), added over the
+     * lower polygon's records with sj = the higher one, then the higher polygon's records
+     * with sj = the lower one, each in record order */
+    double* scaleR = (double*)malloc(sizeof(double) * Ialloc);
+    unsigned char* isRep = (unsigned char*)calloc(Ialloc, 1);
     if (sqMode && I > 0) {
         double* Er = (double*)malloc(sizeof(double) * I);
         for (int r = 0; r < I; ++r) {
@@ -673,7 +681,7 @@ double orc_force_energy(int model, int V, int N, const int* start, const double*
         for (int r = 0; r < I; ++r) {
             lensA[r] = 0.0; lensN[r] = 1;
             if (self[r]) continue;
-            if (loopMode[r] != 1) { lensA[r] = 0.5 * Er[r]; continue; }
+            if (loopMode[r] != 1) { lensA[r] = 0.5 * Er[r]; isRep[r] = 1; continue; }
             int mem[16], cnt = 0, q = r;
             do { mem[cnt++] = q; q = partner[q]; } while (q != r && cnt < 16);
             int k0 = 0;
@@ -682,9 +690,24 @@ double orc_force_energy(int model, int V, int N, const int* start, const double*
             for (int k = 0; k < cnt; ++k) sum += Er[mem[(k0 + k) % cnt]];
             lensA[r] = 0.5 * sum;
             lensN[r] = cnt;
+            isRep[r] = (mem[k0] == r);
+        }
+        for (int r = 0; r < I; ++r) {
+            scaleR[r] = lensA[r];
+            if (!pairMode || self[r]) continue;
+            int si = (int)((intersections[r] >> 32) & 0xFFFF), sj = (int)((intersections[r] >> 48) & 0xFFFF);
+            int a = si < sj ? si : sj, b = si < sj ? sj : si;
+            double pa = 0.0;
+            for (int pass = 0; pass < (a == b ? 1 : 2); ++pass) {
+                int own = pass ? b : a, other = pass ? a : b;
+                if (shapeStart[own] > shapeEnd[own]) continue;
+                for (int q = shapeStart[own]; q <= shapeEnd[own]; ++q)
+                    if (isRep[q] && (int)((intersections[q] >> 48) & 0xFFFF) == other) pa += lensA[q];
+            }
+            scaleR[r] = pa;
         }
         for (int r = 0; r < I; ++r)
-            for (int q = 0; q < 8; ++q) { cc[8 * r + q].x *= lensA[r]; cc[8 * r + q].y *= lensA[r]; }
+            for (int q = 0; q < 8; ++q) { cc[8 * r + q].x *= scaleR[r]; cc[8 * r + q].y *= scaleR[r]; }
         free(Er);
     }
     /* records owned through sj, in record order */
@@ -731,7 +754,7 @@ double orc_force_energy(int model, int V, int N, const int* start, const double*
                     d2 g1, g2;
                     g12(pm, pn, sp, &g1, &g2);
                     if (areaMode) { g1.x *= 0.5; g1.y *= 0.5; g2.x *= 0.5; g2.y *= 0.5; }
-                    if (sqMode) { g1.x *= lensA[k]; g1.y *= lensA[k]; g2.x *= lensA[k]; g2.y *= lensA[k]; }
+                    if (sqMode) { g1.x *= scaleR[k]; g1.y *= scaleR[k]; g2.x *= scaleR[k]; g2.y *= scaleR[k]; }
                     lm.x -= g1.x; lm.y -= g1.y;
                     ln.x -= g2.x; ln.y -= g2.y;
                 }
@@ -799,7 +822,7 @@ double orc_force_energy(int model, int V, int N, const int* start, const double*
         if (sqMode) {   /* each lens's A^2/2 split evenly over its records */
             if (overlap && shapeStart[s] <= shapeEnd[s])
                 for (int r = shapeStart[s]; r <= shapeEnd[s]; ++r)
-                    acc += (lensA[r] * lensA[r]) / (2.0 * lensN[r]);
+                    acc += (scaleR[r] * lensA[r]) / (2.0 * lensN[r]);
             e = acc;
         } else {
             if (overlap) {
@@ -819,7 +842,7 @@ double orc_force_energy(int model, int V, int N, const int* start, const double*
     double E = orc_det_sum(ep, N);
     free(ep); free(eedge); free(LM); free(LN); free(eint); free(sjCount); free(sjList); free(fill);
     free(shapeStart); free(shapeEnd); free(tgt); free(cc); free(eext); free(self); free(spr);
-    free(loopMode); free(partner); free(lensA); free(lensN);
+    free(loopMode); free(partner); free(lensA); free(lensN); free(scaleR); free(isRep);
     free(shapeId); free(next); free(prev);
     return E;
 }
@@ -1388,7 +1411,7 @@ static void detect(const orc_system* s, const double* pos, orc_ws* w) {
     }
 }
 static void records(const orc_system* s, orc_ws* w, const double* pos) {
-    const double* ex = (s->model & (ORC_AREA_ENERGY | ORC_AREA_SQUARED)) ? pos : NULL;
+    const double* ex = (s->model & ORC_AREA_MODES) ? pos : NULL;
     w->I = orc_outersections_ex(s->V, s->N, s->start, w->M, w->neighbors, w->inside, w->tuSlots,
                                 w->numNeighbors, w->inter, w->tu, w->outer, w->ut, w->shapeCounts, ex);
 }
@@ -1478,7 +1501,7 @@ void orc_fire_step(const orc_system* s, double* pos, double* vel, double* force,
     ws_alloc(&w, s->V, s->N, s->boxSize, s->maxNeighbors > 0 ? s->maxNeighbors : 100);
     int shakeFailed = 0;
     st->rolledBack = 0;
-    int areaMode = (s->model & (ORC_AREA_ENERGY | ORC_AREA_SQUARED)) != 0;
+    int areaMode = (s->model & ORC_AREA_MODES) != 0;
     if (needsInter) {
         if (!areaMode) detect(s, pos, &w);   /* reference order: detect, SHAKE, records */
         if (rigid && shakeIter > 0) {

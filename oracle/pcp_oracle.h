@@ -29,6 +29,11 @@ enum { ORC_NORMAL = 0, ORC_EDGE_ONLY = 1, ORC_AREA_ONLY = 2, ORC_SOFT_BODY = 3,
 /* squared lens areas (implies ORC_AREA_ENERGY): energy sum over overlap lenses of A^2/2,
  * force -sum A grad A (smooth where a lens closes, unlike the area itself) */
 #define ORC_AREA_SQUARED 0x200
+/* squared overlap area per polygon pair (implies ORC_AREA_ENERGY): energy sum over
+ * polygon pairs of (sum of the pair's lens areas)^2/2, force -A_pair grad A_pair;
+ * continuous where two lenses of one pair merge or split, unlike ORC_AREA_SQUARED */
+#define ORC_PAIR_SQUARED 0x400
+#define ORC_AREA_MODES (ORC_AREA_ENERGY | ORC_AREA_SQUARED | ORC_PAIR_SQUARED)
 
 double orc_wrap(double x);
 void orc_topology(int N, const int* start, int V, int* shapeId, int* next, int* prev);
