@@ -183,18 +183,24 @@ def fire_legs(args):
         t0 = time.perf_counter()
         p3 = make_batch(args.batch_packings, args.seed + 3, min_center_frac=0.5)
         gen = time.perf_counter() - t0
-        r = batch.minimize_batch(p3, W.SQUARED, dict(fTol=1e-10, dtMax=W.DT_MAX), 5000, graph=not args.no_graph)
-        ok = r.steps >= 0
-        out["batched_uneven"] = dict(
-            workload=(f"{args.batch_packings} packings, N ~ U[64,1024], n ~ U[8,32], phi ~ U[0.80,0.90], "
-                      "one context, re-packed when fewer than half its packings are still running"),
-            packings=args.batch_packings, converged=int(ok.sum()), seconds=r.seconds,
-            packings_per_s=float(ok.sum()) / r.seconds, mean_steps=float(r.steps[ok].mean()),
-            max_steps=int(r.steps[ok].max()), step_cap=5000, contexts=r.contexts,
-            max_final_force_converged=float(r.max_force[ok].max()), generation_seconds=gen, dtMax=W.DT_MAX)
-        print(f"[bench] FIRE batched_uneven: {int(ok.sum())}/{args.batch_packings} converged, "
-              f"{r.seconds:.3f} s, {out['batched_uneven']['packings_per_s']:.2f} packings/s, "
-              f"contexts {r.contexts}", flush=True)
+        # squared lens areas, then squared pair areas (continuous where two lenses of one
+        # polygon pair merge or split)
+        for tag, model, energy in (("batched_uneven", W.SQUARED, "squared lens areas"),
+                                   ("batched_uneven_pair", W.PAIR_SQUARED, "squared pair areas")):
+            r = batch.minimize_batch(p3, model, dict(fTol=1e-10, dtMax=W.DT_MAX), 5000, graph=not args.no_graph)
+            ok = r.steps >= 0
+            out[tag] = dict(
+                workload=(f"{args.batch_packings} packings, N ~ U[64,1024], n ~ U[8,32], phi ~ U[0.80,0.90], "
+                          "one context, re-packed when fewer than half its packings are still running"),
+                energy=energy, packings=args.batch_packings, converged=int(ok.sum()), seconds=r.seconds,
+                packings_per_s=float(ok.sum()) / r.seconds, mean_steps=float(r.steps[ok].mean()),
+                max_steps=int(r.steps[ok].max()), step_cap=5000, contexts=r.contexts,
+                not_converged=[int(k) for k in np.where(~ok)[0]],
+                max_final_force_converged=float(r.max_force[ok].max()), generation_seconds=gen, dtMax=W.DT_MAX)
+            print(f"[bench] FIRE {tag}: {int(ok.sum())}/{args.batch_packings} converged, "
+                  f"{r.seconds:.3f} s, {out[tag]['packings_per_s']:.2f} packings/s, "
+                  f"mean steps {out[tag]['mean_steps']:.1f}, contexts {r.contexts}, not converged "
+                  f"{out[tag]['not_converged']}", flush=True)
     if args.compress_replicas > 0:
         from pycudapolygon_amd import compress
         R = args.compress_replicas

@@ -54,6 +54,12 @@ enum { PCP_NORMAL = 0, PCP_EDGE_ONLY = 1, PCP_AREA_ONLY = 2, PCP_SOFT_BODY = 3,
  * no force tolerance is reachable; the squared energy is smooth there and FIRE converges
  * to max|F| <= 1e-10. */
 #define PCP_AREA_SQUARED 0x200
+/* Flag (no counterpart in the reference): squared overlap area per polygon pair.  As
+ * PCP_AREA_SQUARED, but the energy is sum over polygon pairs of A_pair^2/2 with A_pair the
+ * sum of the pair's lens areas, and the force -sum A_pair grad A_pair.  Where a vertex
+ * crosses an edge, two lenses of one pair merge or split; sum A^2/2 jumps there by
+ * A1*A2 and FIRE can stall against the jump, while A_pair is continuous. */
+#define PCP_PAIR_SQUARED 0x400
 
 typedef struct pcp_ctx pcp_ctx;
 

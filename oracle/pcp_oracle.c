@@ -657,9 +657,9 @@ double orc_force_energy(int model, int V, int N, const int* start, const double*
     int* lensN = (int*)malloc(sizeof(int) * Ialloc);
     /* with ORC_PAIR_SQUARED: the scale of a record's contributions is the overlap area of
      * its polygon pair, pairA = sum of the pair's lens areas (one representative record
-     * per lens: the smallest key of its cycle, or a record off any cycle), added over the
-     * lower polygon's records with sj = the higher one, then the higher polygon's records
-     * with sj = the lower one, each in record order */
+     * per lens: the smallest key of its cycle, or a record off any cycle; every other
+     * record adds 0), added over the lower polygon's records with sj = the higher one,
+     * then the higher polygon's records with sj = the lower one, each in record order */
     double* scaleR = (double*)malloc(sizeof(double) * Ialloc);
     unsigned char* isRep = (unsigned char*)calloc(Ialloc, 1);
     if (sqMode && I > 0) {
@@ -702,7 +702,7 @@ double orc_force_energy(int model, int V, int N, const int* start, const double*
                 int own = pass ? b : a, other = pass ? a : b;
                 if (shapeStart[own] > shapeEnd[own]) continue;
                 for (int q = shapeStart[own]; q <= shapeEnd[own]; ++q)
-                    if (isRep[q] && (int)((intersections[q] >> 48) & 0xFFFF) == other) pa += lensA[q];
+                    if ((int)((intersections[q] >> 48) & 0xFFFF) == other) pa += isRep[q] ? lensA[q] : 0.0;
             }
             scaleR[r] = pa;
         }

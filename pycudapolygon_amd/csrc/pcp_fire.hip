@@ -455,7 +455,7 @@ static int fire_eval(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm
     double2* x = (double2*)st->pos;
     double2* v = (double2*)st->vel;
     int mt = st->model & 0xFF;
-    bool area = (st->model & (PCP_AREA_ENERGY | PCP_AREA_SQUARED)) != 0;
+    bool area = (st->model & PCP_AREA_MODES) != 0;
     bool overlap = (mt == PCP_NORMAL || mt == PCP_HYBRID);
     bool rigid = (mt == PCP_NORMAL);
     if (overlap) {

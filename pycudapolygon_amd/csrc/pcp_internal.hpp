@@ -8,6 +8,10 @@
 #include "pcp_device.hpp"
 #include "../../include/pcp.h"
 
+// every energy flag that recomputes exact crossings and measures closed cycles from the
+// pair origin
+#define PCP_AREA_MODES (PCP_AREA_ENERGY | PCP_AREA_SQUARED | PCP_PAIR_SQUARED)
+
 namespace pcp {
 
 void set_error(const std::string& msg);
@@ -111,6 +115,7 @@ struct pcp_ctx {
     double2 *dF1 = nullptr, *dLM = nullptr, *dLN = nullptr;   // V: per-vertex partial sums
     double* dAccR = nullptr;       // N: record + interior energy sums
     double* dEr = nullptr;         // per record: its h sums (squared lens energy)
+    double* dLaRep = nullptr;      // per record: its lens area if it represents the lens, else 0
 
     // per-vertex / per-polygon scratch
     double* dEpoly = nullptr;

@@ -1135,7 +1135,7 @@ static void free_topology(pcp_ctx* c) {
     dfree(c->dNum); dfree(c->dPackMaxKey); dfree(c->dScal);
     dfree(c->dBcnt); dfree(c->dJcnt); dfree(c->dBKey); dfree(c->dBTU); dfree(c->dJKey); dfree(c->dRKey);
     dfree(c->dROuter); dfree(c->dRTU); dfree(c->dRUT); dfree(c->dJC); dfree(c->dJT);
-    dfree(c->dF1); dfree(c->dLM); dfree(c->dLN); dfree(c->dAccR); dfree(c->dEr);
+    dfree(c->dF1); dfree(c->dLM); dfree(c->dLN); dfree(c->dAccR); dfree(c->dEr); dfree(c->dLaRep);
     dfree(c->dEpoly); dfree(c->dWork); dfree(c->dRed);
     dfree(c->dRedOut); dfree(c->dScanPart); c->scanPartCap = 0;
     dfree(c->dXbak); dfree(c->dXpre); dfree(c->dVbak); dfree(c->dDt); dfree(c->dAlpha);
@@ -1651,7 +1651,7 @@ int pcp_force_eval(pcp_ctx* c, const pcp_state* st) {
     int overlap = (mt == PCP_NORMAL || mt == PCP_HYBRID);
     if (overlap) {
         PCP_TRY(run_detect(c, pos, 0));
-        PCP_TRY(run_records(c, (st->model & (PCP_AREA_ENERGY | PCP_AREA_SQUARED)) ? pos : nullptr));
+        PCP_TRY(run_records(c, (st->model & PCP_AREA_MODES) ? pos : nullptr));
     }
     // the overflow check rides on the energy reduction when there is one
     bool fused = overlap && st->energy;

@@ -206,6 +206,85 @@ __device__ __forceinline__ bool closed_cycle(uint64_t inter, uint64_t outer, int
     return ok && cur == inter;
 }
 
+// Area of the lens of a record: half the h sums (Er) of the records of its closed cycle,
+// added in cycle order from the record with the smallest key; off a cycle, the record
+// alone.  *rep: the record represents its lens (smallest key of the cycle, or off any).
+__device__ __forceinline__ double lens_area(uint64_t inter, uint64_t outer, long long row, bool closed, int pbase,
+                                            int Kp, const int* __restrict__ bcnt, const uint64_t* __restrict__ rKey,
+                                            const uint64_t* __restrict__ rOuter, const double* __restrict__ Er,
+                                            int* lensN, bool* rep) {
+    if (!closed) {
+        *lensN = 1;
+        *rep = true;
+        return 0.5 * Er[row];
+    }
+    uint64_t mk2[16];
+    long long mr[16];
+    int cntL = 0;
+    uint64_t cur = inter, nxt = 0;
+    do {
+        mk2[cntL] = cur;
+        mr[cntL] = row;
+        ++cntL;
+        uint64_t o2 = (cntL == 1) ? outer : nxt;
+        find_record(o2, pbase, Kp, bcnt, rKey, rOuter, &nxt, &row);
+        cur = o2;
+    } while (cur != inter && cntL < 16);
+    int k0 = 0;
+    for (int k = 1; k < cntL; ++k) if (mk2[k] < mk2[k0]) k0 = k;
+    double sum = 0.0;
+    for (int k = 0; k < cntL; ++k) sum += Er[mr[(k0 + k) % cntL]];
+    *lensN = cntL;
+    *rep = (k0 == 0);
+    return 0.5 * sum;
+}
+
+// Overlap area of the polygon pair (s, sj) (packing-global ids): laRep summed over the
+// lower polygon's records whose sj is the higher one, then over the higher polygon's
+// records whose sj is the lower one, each in record order.
+__device__ __forceinline__ double pair_area(int s, int sj, int pbase, int Kp, const int* __restrict__ bcnt,
+                                            const uint64_t* __restrict__ rKey, const double* __restrict__ laRep) {
+    int a = min(s, sj), b = max(s, sj);
+    double pa = 0.0;
+    for (int pass = 0; pass < (a == b ? 1 : 2); ++pass) {
+        int own = pass ? b : a, other = (pass ? a : b) - pbase;
+        long long bo = (long long)own * Kp;
+        int co = min(bcnt[own], Kp);
+        for (int q = 0; q < co; ++q)
+            if (key_sj(rKey[bo + q]) == other) pa += laRep[bo + q];
+    }
+    return pa;
+}
+
+// Squared pair areas, second pass (wavefront / polygon s, lane / record): each record's
+// lens area if the record represents its lens, else 0.
+template <int SEGX>
+__global__ __launch_bounds__(64) void k_lens_rep(int N, int Kp, const int* __restrict__ polyPack,
+                                                const int* __restrict__ packPoly, const int* __restrict__ bcnt,
+                                                const uint64_t* __restrict__ rKey,
+                                                const uint64_t* __restrict__ rOuter, const double* __restrict__ Er,
+                                                double* __restrict__ laRep, const int* __restrict__ order) {
+    int lane = threadIdx.x & (SEGX - 1);
+    int slot = xcd_block(blockIdx.x, gridDim.x) * (64 / SEGX) + threadIdx.x / SEGX;
+    if (slot >= N) return;   // no barriers or shuffles below
+    int s = order[slot];
+    int cnt = min(bcnt[s], Kp);
+    int pbase = packPoly[polyPack[s]];
+    long long base = (long long)s * Kp;
+    for (int f = lane; f < cnt; f += SEGX) {
+        uint64_t inter = rKey[base + f], outer = rOuter[base + f];
+        double la = 0.0;
+        if (inter != outer) {
+            bool closed = closed_cycle(inter, outer, pbase, Kp, bcnt, rKey, rOuter);
+            int lensN;
+            bool rep;
+            double a = lens_area(inter, outer, base + f, closed, pbase, Kp, bcnt, rKey, rOuter, Er, &lensN, &rep);
+            if (rep) la = a;
+        }
+        laRep[base + f] = la;
+    }
+}
+
 // Squared lens areas, first pass (wavefront / polygon s, lane / record): the h sums of
 // each record -- its exterior terms plus its chain of s's edges, measured from the same
 // origin as in k_exterior_poly -- twice the area the record covers.
@@ -268,7 +347,7 @@ __global__ __launch_bounds__(64) void k_lens_energy(int N, int Kp, const int* __
     }
 }
 
-// MODE 0: reference energy; 1: area energy; 2: squared lens areas.
+// MODE 0: reference energy; 1: area energy; 2: squared lens areas; 3: squared pair areas.
 // One wavefront holds 64 / SEGX polygons, SEGX lanes each: lanes take the polygon's
 // records, then its vertices, SEGX at a time (a polygon has a few records, so a
 // wavefront per polygon would leave most lanes idle).  Every sum keeps its order.
@@ -280,8 +359,8 @@ __global__ __launch_bounds__(64) void k_exterior_poly(
     const uint64_t* __restrict__ rOuter, const float2* __restrict__ rTU, const float2* __restrict__ rUT,
     double2* __restrict__ jC, int* __restrict__ jT, double2* __restrict__ F1, double2* __restrict__ LMo,
     double2* __restrict__ LNo, double* __restrict__ accR, const double* __restrict__ Er,
-    const int* __restrict__ order) {
-    constexpr int AREA = (MODE >= 1), SQ = (MODE == 2);
+    const double* __restrict__ laRep, const int* __restrict__ order) {
+    constexpr int AREA = (MODE >= 1), SQ = (MODE >= 2), PAIR = (MODE == 3);
     constexpr int G = 64 / SEGX;
     extern __shared__ char smem[];
     __shared__ double leBuf[64];
@@ -304,7 +383,7 @@ __global__ __launch_bounds__(64) void k_exterior_poly(
         int tg[8];
         double e = 0.0;
         d2 sp = mk(0.0, 0.0);
-        double lensA = 0.0;
+        double lensA = 0.0, scale = 0.0;   // scale: lens area, or with PAIR the pair's overlap area
         int lensN = 1;
         bool valid = (inter != outer);
         if (valid) {
@@ -333,31 +412,9 @@ __global__ __launch_bounds__(64) void k_exterior_poly(
                 if (closed) sp = ldpos(pos, polyStart[min(s, sjg)]);
             }
             if (SQ) {
-                // lens area: half the h sums of the records of the cycle, added in cycle
-                // order from the record with the smallest key; off a cycle, the record alone
-                if (closed) {
-                    uint64_t mk2[16];
-                    long long mr[16];
-                    int cntL = 0;
-                    uint64_t cur = inter, nxt = 0;
-                    long long row = base + f;
-                    do {
-                        mk2[cntL] = cur;
-                        mr[cntL] = row;
-                        ++cntL;
-                        uint64_t o2 = (cntL == 1) ? outer : nxt;
-                        find_record(o2, pbase, Kp, bcnt, rKey, rOuter, &nxt, &row);
-                        cur = o2;
-                    } while (cur != inter && cntL < 16);
-                    int k0 = 0;
-                    for (int k = 1; k < cntL; ++k) if (mk2[k] < mk2[k0]) k0 = k;
-                    double sum = 0.0;
-                    for (int k = 0; k < cntL; ++k) sum += Er[mr[(k0 + k) % cntL]];
-                    lensA = 0.5 * sum;
-                    lensN = cntL;
-                } else {
-                    lensA = 0.5 * Er[base + f];
-                }
+                bool rep;
+                lensA = lens_area(inter, outer, base + f, closed, pbase, Kp, bcnt, rKey, rOuter, Er, &lensN, &rep);
+                scale = PAIR ? pair_area(s, sjg, pbase, Kp, bcnt, rKey, laRep) : lensA;
             }
             double dfa[16], dfb[16];
             if (iL == lL) {
@@ -405,7 +462,7 @@ __global__ __launch_bounds__(64) void k_exterior_poly(
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             double cx = -inc[q].x * sc, cy = -inc[q].y * sc;
-            if (SQ) { cx *= lensA; cy *= lensA; }
+            if (SQ) { cx *= scale; cy *= scale; }
             inc[q] = mk(cx, cy);
         }
 #pragma unroll
@@ -414,10 +471,11 @@ __global__ __launch_bounds__(64) void k_exterior_poly(
             L.t[4 * f + q] = tg[q];
         }
         // energy of the record: its h terms (the polygon sums them with its interior
-        // terms and halves), or with SQ its share of the lens energy A^2/2, doubled so
-        // that the same halving gives it exactly
-        L.e[f] = SQ ? 2.0 * ((lensA * lensA) / (2.0 * lensN)) : e;
-        L.la[f] = lensA;
+        // terms and halves), or with SQ its share scale*A/(2 lensN) of the lens's part of
+        // the energy (A^2/2, or with PAIR A_pair*A/2), doubled so that the same halving
+        // gives it exactly
+        L.e[f] = SQ ? 2.0 * ((scale * lensA) / (2.0 * lensN)) : e;
+        L.la[f] = scale;
         L.sp[f] = make_double2(sp.x, sp.y);
         // sj side: at this record's rank among the records whose sj is sjg (key order)
         long long bj = (long long)sjg * Kp;
@@ -640,12 +698,13 @@ int grow_buckets(pcp_ctx* c, int need) {
 
 int alloc_buckets(pcp_ctx* c, int Kp) {
     dfree(c->dBKey); dfree(c->dBTU); dfree(c->dJKey); dfree(c->dRKey); dfree(c->dROuter);
-    dfree(c->dRTU); dfree(c->dRUT); dfree(c->dJC); dfree(c->dJT); dfree(c->dEr);
+    dfree(c->dRTU); dfree(c->dRUT); dfree(c->dJC); dfree(c->dJT); dfree(c->dEr); dfree(c->dLaRep);
     long long n = (long long)std::max(c->N, 1) * Kp;
     PCP_TRY(dalloc(&c->dBKey, n)); PCP_TRY(dalloc(&c->dBTU, n)); PCP_TRY(dalloc(&c->dJKey, n));
     PCP_TRY(dalloc(&c->dRKey, n)); PCP_TRY(dalloc(&c->dROuter, n)); PCP_TRY(dalloc(&c->dRTU, n));
     PCP_TRY(dalloc(&c->dRUT, n)); PCP_TRY(dalloc(&c->dJC, 4 * n)); PCP_TRY(dalloc(&c->dJT, 4 * n));
     PCP_TRY(dalloc(&c->dEr, n));
+    PCP_TRY(dalloc(&c->dLaRep, n));
     c->Kp = Kp;
     return PCP_OK;
 }
@@ -724,15 +783,16 @@ static int launch_exterior_w(pcp_ctx* c, const double2* pos) {
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_exterior_poly<MODE, SEGX>), dim3(grid), dim3(64), lds, c->stream, c->N,
                        c->Kp, c->dPolyStart, c->dPolyPack, c->dPackPoly, pos, c->dBcnt, c->dJcnt, c->dJKey, c->dRKey,
                        c->dROuter, c->dRTU, c->dRUT, c->dJC, c->dJT, c->dF1, c->dLM, c->dLN, c->dAccR, c->dEr,
-                       c->dPcSorted);
+                       c->dLaRep, c->dPcSorted);
     PCP_LAUNCHED();
     return PCP_OK;
 }
 template <int MODE>
 static int launch_exterior_m(pcp_ctx* c, const double2* pos) {
     int slot = -1;
-    PCP_TRY(prof_begin(c, MODE == 0 ? "k_exterior_poly<0>" : (MODE == 1 ? "k_exterior_poly<1>" : "k_exterior_poly<2>"),
-                       &slot));
+    static const char* names[4] = {"k_exterior_poly<0>", "k_exterior_poly<1>", "k_exterior_poly<2>",
+                                   "k_exterior_poly<3>"};
+    PCP_TRY(prof_begin(c, names[MODE], &slot));
     size_t st = ext_lds_stride(c->Kp);
     int w = c->segExt;
     if (w == 16 && 4 * st > 65536) w = 0;   // a chosen width the LDS does not fit: the default rule
@@ -744,6 +804,7 @@ static int launch_exterior_m(pcp_ctx* c, const double2* pos) {
     return prof_end(c, slot);
 }
 static int launch_exterior(pcp_ctx* c, int mode, const double2* pos) {
+    if (mode == 3) return launch_exterior_m<3>(c, pos);
     if (mode == 2) return launch_exterior_m<2>(c, pos);
     if (mode == 1) return launch_exterior_m<1>(c, pos);
     return launch_exterior_m<0>(c, pos);
@@ -758,6 +819,18 @@ static int launch_lens(pcp_ctx* c, const double2* pos) {
     else if (c->segLens == 32) PCP_LENS(32);
     else PCP_LENS(16);
 #undef PCP_LENS
+    return PCP_OK;
+}
+
+// representative lens areas of the squared pair energy (the lens kernel's schedule)
+static int launch_lens_rep(pcp_ctx* c) {
+#define PCP_LENS_REP(W)                                                                                     \
+    PCP_LAUNCH(c, k_lens_rep<W>, dim3((c->N + 64 / W - 1) / (64 / W)), dim3(64), c->N, c->Kp, c->dPolyPack, \
+               c->dPackPoly, c->dBcnt, c->dRKey, c->dROuter, c->dEr, c->dLaRep, c->dPcSorted)
+    if (c->segLens == 64) PCP_LENS_REP(64);
+    else if (c->segLens == 32) PCP_LENS_REP(32);
+    else PCP_LENS_REP(16);
+#undef PCP_LENS_REP
     return PCP_OK;
 }
 
@@ -791,9 +864,9 @@ int autotune(pcp_ctx* c, const pcp_state* st, int reps, double* timesUs) {
     if (overlap && c->N > 0) {
         PCP_TRY(pcp_force_eval(c, st));
         PCP_TRY(pcp_synchronize(c));
-        bool exact = (st->model & (PCP_AREA_ENERGY | PCP_AREA_SQUARED)) != 0;
-        bool sq = (st->model & PCP_AREA_SQUARED) != 0;
-        int mode = sq ? 2 : (exact ? 1 : 0);
+        bool exact = (st->model & PCP_AREA_MODES) != 0;
+        bool sq = (st->model & (PCP_AREA_SQUARED | PCP_PAIR_SQUARED)) != 0;
+        int mode = (st->model & PCP_PAIR_SQUARED) ? 3 : (sq ? 2 : (exact ? 1 : 0));
         const double2* x = (const double2*)st->pos;
         int keepPair = c->segPair, keepLens = c->segLens, keepExt = c->segExt;
         size_t lst = ext_lds_stride(c->Kp);
@@ -832,8 +905,9 @@ int run_forces(pcp_ctx* c, int model, const double2* pos, double* edgeLengths, d
                const double* targetEdgeV, const double* targetAreas, double stiffness,
                double compressibility, double2* force, double* energy, int geom, int project, bool sticky) {
     if (c->V == 0) return PCP_OK;
-    int sq = (model & PCP_AREA_SQUARED) ? 1 : 0;
-    int area = (model & (PCP_AREA_ENERGY | PCP_AREA_SQUARED)) ? 1 : 0;
+    int pair = (model & PCP_PAIR_SQUARED) ? 1 : 0;
+    int sq = (model & (PCP_AREA_SQUARED | PCP_PAIR_SQUARED)) ? 1 : 0;
+    int area = (model & PCP_AREA_MODES) ? 1 : 0;
     model &= 0xFF;
     int overlap = (model == PCP_NORMAL || model == PCP_HYBRID);
     int edgeOn = (model == PCP_SOFT_BODY || model == PCP_EDGE_ONLY || model == PCP_HYBRID);
@@ -856,7 +930,8 @@ int run_forces(pcp_ctx* c, int model, const double2* pos, double* edgeLengths, d
     }
     if (overlap) {
         if (sq) PCP_TRY(launch_lens(c, pos));
-        PCP_TRY(launch_exterior(c, sq ? 2 : (area ? 1 : 0), pos));
+        if (pair) PCP_TRY(launch_lens_rep(c));
+        PCP_TRY(launch_exterior(c, pair ? 3 : (sq ? 2 : (area ? 1 : 0)), pos));
     }
     bool proj = project && c->maxN <= 64;
     if (geom && proj)

@@ -25,6 +25,7 @@ NPARAM = 25218
 DEG = 16
 H = 64
 SQUARED = capi.NORMAL | 0x200
+PAIR_SQUARED = capi.NORMAL | 0x400
 # weights trained by `python -m pycudapolygon_amd.train_warmstart` (its defaults)
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "warmstart_n1024_n16.npz")
 # weights trained on packing-fraction steps of the compression protocol:

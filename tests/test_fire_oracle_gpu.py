@@ -17,6 +17,7 @@ from pycudapolygon_amd.generate import make_packing, make_batch, box_size_for
 pytestmark = pytest.mark.gpu
 
 SQUARED = 0x200
+PAIR = 0x400
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -120,3 +121,46 @@ def test_fire_final_energy_matches_oracle_n4096_plain_and_warm_start():
         assert gs.steps == steps
         assert abs(gs.energy - sto.energy) <= 1e-12 * abs(sto.energy)
         assert np.array_equal(xg, xo)
+
+
+def test_pair_areas_bit_identical_with_two_lenses_per_pair():
+    """A U-shaped polygon under a bar: two lenses of one polygon pair, then one after they
+    merge.  Squared pair areas (and squared lens areas) on the GPU equal the oracle bit
+    for bit at every bar height."""
+    from tests.test_oracle import _u_and_bar
+    from tests import _geom as G
+    for yc in np.linspace(3.1, 1.2, 12):
+        U, bar, _ = _u_and_bar(yc)
+        start, pos = G.to_model([U, bar])
+        pk = dict(startIndices=start, positions=pos, targetEdgeV=np.zeros(12), targetAreas=np.zeros(2))
+        for flag in (SQUARED, PAIR):
+            print(f"[pair] yc={yc:.3f} flag={flag:#x}", flush=True)
+            s = capi.System([pk], [2])
+            s.geometry()
+            s.cells()
+            s.neighbors()
+            s.records(exact_pairing=True)
+            s.forces(capi.NORMAL | flag)
+            F, E = s.get_forces(0), s.get_energy()[0]
+            Eo, Fo, nIo = O.System(start, 2, flag | 5, np.zeros(12), np.zeros(2)).force_eval(pos)
+            assert np.array_equal(F, Fo) and E == Eo, (yc, flag, np.abs(F - Fo).max(), E, Eo)
+
+
+def test_pair_areas_fire_converges_where_lens_areas_stall():
+    """Packing 41 of the 256-packing uneven batch (seed 3; N=65 x 16): with squared lens
+    areas FIRE stalls (two lenses of one pair merge where a vertex meets an edge, and the
+    energy jumps there); with squared pair areas it reaches max|F| <= 1e-10, on the GPU
+    in the oracle's number of steps and at the oracle's final energy."""
+    p = make_batch(256, 3, min_center_frac=0.5)[41]
+    B = box_size_for(p["diameter"])
+    s = capi.System([p], [B], model=capi.NORMAL | PAIR)
+    prm = s.fire_params(fTol=1e-10, dtMax=100.0)
+    s.fire_init(prm, 1e-3)
+    gs = s.fire_run(prm, 5000, 8, True)[0]
+    sysO = O.System(p["startIndices"], B, PAIR, p["targetEdgeV"], p["targetAreas"], solver="lanes")
+    xo, fo, sto, steps = sysO.fire_minimize(p["positions"], fTol=1e-10, dt=1e-3, maxSteps=5000, dtMax=100.0)
+    print(f"[pair] packing 41: GPU {gs.steps} steps E {gs.energy!r}; oracle {steps} steps E {sto.energy!r}",
+          flush=True)
+    assert gs.converged and gs.maxForce <= 1e-10 and np.abs(fo).max() <= 1e-10
+    assert gs.steps == steps
+    assert abs(gs.energy - sto.energy) <= 1e-12 * abs(sto.energy)

@@ -14,6 +14,7 @@ pytestmark = pytest.mark.gpu
 
 AREA = 0x100
 SQUARED = 0x200
+PAIR = 0x400
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -113,10 +114,10 @@ def test_force_energy_parity(seed, model):
 
 
 @pytest.mark.parametrize("seed", range(6))
-@pytest.mark.parametrize("flag", [AREA, SQUARED])
+@pytest.mark.parametrize("flag", [AREA, SQUARED, PAIR])
 def test_area_energy_parity(seed, flag):
-    """Area energy and squared lens areas: GPU vs oracle, bit for bit (same exact
-    crossings, cycle origins and accumulation order)."""
+    """Area energy, squared lens areas and squared pair areas: GPU vs oracle, bit for bit
+    (same exact crossings, cycle origins, lens representatives and accumulation order)."""
     p, B = c1(seed)
     s, mx, nI = run_gpu(p, B, exact=True)
     s.forces(capi.NORMAL | flag)

@@ -398,6 +398,78 @@ def test_squared_lens_energy_closed_form(seed):
     np.testing.assert_allclose(f, fe, rtol=0, atol=1e-6 * np.abs(fe).max())
 
 
+def _u_and_bar(yc, th=0.05):
+    """A U-shaped 8-gon (arms x in [0,1] and [2,3], y in [1,3], base y in [0,1], unit
+    s = 0.02) and a slightly rotated bar centred at height yc (in units of s) across both
+    arms: the bar overlaps U in two separate regions while its lower edge is above the
+    notch bottom (y = 1), in one region after it passes below."""
+    s, off = 0.02, np.array([0.4, 0.4])
+    U = np.array([[0, 0], [3, 0], [3, 3], [2, 3], [2, 1], [1, 1], [1, 3], [0, 3]], float) * s + off
+    hw, hh = 2.2 * s, 0.5 * s
+    R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
+    bar = off + np.array([1.5 * s, yc * s]) + np.array([[-hw, -hh], [hw, -hh], [hw, hh], [-hw, hh]]) @ R.T
+    arms = [np.array([[0, 1], [1, 1], [1, 3], [0, 3]], float) * s + off,
+            np.array([[2, 1], [3, 1], [3, 3], [2, 3]], float) * s + off]
+    return U, bar, arms
+
+
+def test_pair_squared_energy_is_the_squared_pair_overlap_through_a_merge():
+    """Squared pair areas: E = (A1 + A2)^2/2 with A the overlap of an independent clipping
+    geometry, for two separate lenses of one polygon pair and after they merge into one
+    as the bar slides down (the energy is continuous through the merge).  Squared lens
+    areas give (A1^2 + A2^2)/2 before the merge and A^2/2 after: they jump by A1*A2."""
+    for yc in np.linspace(3.1, 1.2, 39):
+        U, bar, arms = _u_and_bar(yc)
+        start, pos = G.to_model([U, bar])
+        A = G.overlap_area(U, bar)
+        E = {}
+        for flag in (0x200, 0x400):
+            sysO = O.System(start, 2, flag | 5, np.zeros(12), np.zeros(2))   # hybrid: no projection
+            E[flag] = sysO.force_eval(pos)[0]
+        np.testing.assert_allclose(E[0x400], A * A / 2, rtol=1e-10)
+        if yc - 0.5 - 0.11 > 1.0:   # the bar's lower edge (rotation included) above the notch
+            A1, A2 = (G.overlap_area(a, bar) for a in arms)
+            np.testing.assert_allclose(A1 + A2, A, rtol=1e-12)
+            np.testing.assert_allclose(E[0x200], (A1 * A1 + A2 * A2) / 2, rtol=1e-10)
+            assert E[0x200] < 0.51 * E[0x400]
+        elif yc - 0.5 + 0.11 < 1.0:   # the whole lower edge below the notch: merged
+            np.testing.assert_allclose(E[0x200], A * A / 2, rtol=1e-10)
+
+
+def test_pair_squared_force_is_minus_gradient_with_two_lenses():
+    """Two lenses of one pair (the U and the bar at yc = 2.0, 8 crossings): F = -grad of
+    (overlap area)^2/2 by central differences of the clipping geometry."""
+    U, bar, _ = _u_and_bar(2.0)
+    start, pos = G.to_model([U, bar])
+    sysO = O.System(start, 2, 0x400 | 5, np.zeros(12), np.zeros(2))
+    E, f, nI = sysO.force_eval(pos)
+    assert nI == 8
+    h = 1e-8
+    fe = np.zeros_like(pos)
+    for q in range(len(pos)):
+        xp, xm = pos.copy(), pos.copy()
+        xp[q] += h
+        xm[q] -= h
+        Ap = G.overlap_area(xp[:16].reshape(-1, 2), xp[16:].reshape(-1, 2))
+        Am = G.overlap_area(xm[:16].reshape(-1, 2), xm[16:].reshape(-1, 2))
+        fe[q] = -(Ap * Ap / 2 - Am * Am / 2) / (2 * h)
+    np.testing.assert_allclose(f, fe, rtol=0, atol=1e-6 * np.abs(fe).max())
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_pair_squared_equals_lens_squared_for_single_lens_pairs(seed):
+    """With one lens per polygon pair the two squared energies are the same numbers,
+    bit for bit (the pair sum of one lens is that lens's area)."""
+    rng = np.random.default_rng(seed)
+    p1 = G.regular(int(rng.integers(5, 12)), 0.06, (0.4, 0.4), rng.uniform(0, 6))
+    p2 = G.regular(int(rng.integers(5, 12)), 0.05, (0.47, 0.42), rng.uniform(0, 6))
+    start, pos = G.to_model([p1, p2])
+    V = len(pos) // 2
+    a = O.System(start, 2, 0x200 | 5, np.zeros(V), np.zeros(2)).force_eval(pos)
+    b = O.System(start, 2, 0x400 | 5, np.zeros(V), np.zeros(2)).force_eval(pos)
+    assert a[0] == b[0] and np.array_equal(a[1], b[1]) and a[2] == b[2] == 2
+
+
 def test_squared_lens_fire_converges_where_area_energy_stalls():
     """C1 (N=64, phi=0.80, no polygon starting inside another): FIRE on the area energy
     stalls at a kink (flush contacts, rollbacks shrink dt); on squared lens areas it
