@@ -65,10 +65,11 @@ def cpu_baseline(p, B, budget_s):
                        f"oracle/pcp_oracle.c with OpenMP ({threads} threads)")
 
 
-def fire_one(packings, dtMax, maxSteps, warm=None, budget_s=None, graph=True):
-    """FIRE (squared lens areas, rigid polygons) on the packings in one context until every
-    packing has max|F| <= 1e-10, maxSteps, or budget_s seconds.  warm: a WarmStart applied
-    once after the first evaluation (its time is counted).  Returns a result dict."""
+def fire_one(packings, dtMax, maxSteps, warm=None, budget_s=None, graph=True, dt0=1e-3):
+    """FIRE (squared lens areas, rigid polygons, initial time step dt0) on the packings in one
+    context until every packing has max|F| <= 1e-10, maxSteps, or budget_s seconds.  warm: a
+    WarmStart applied once after the first evaluation (its time is counted).  Returns a
+    result dict."""
     from pycudapolygon_amd import warmstart as W
     ell = W.packing_length_units(packings)
     s, prm = W.fire_system(packings, dtMax=dtMax)
@@ -77,7 +78,7 @@ def fire_one(packings, dtMax, maxSteps, warm=None, budget_s=None, graph=True):
     t0 = time.perf_counter()
     if warm is not None:
         warm.apply(s, ell)
-    s.fire_init(prm, 1e-3)
+    s.fire_init(prm, dt0)
     conv = np.full(s.P, -1)
     steps = 0
     # blocks of 8 steps replayed from a HIP graph, one status read per block (pcp_fire_run);
@@ -97,7 +98,7 @@ def fire_one(packings, dtMax, maxSteps, warm=None, budget_s=None, graph=True):
     ok = conv >= 0
     E = s.get_energy()
     F = s.max_abs_force()
-    return dict(packings=s.P, vertices=s.V, dtMax=dtMax, warm_start=warm is not None, batch_steps=steps,
+    return dict(packings=s.P, vertices=s.V, dtMax=dtMax, dt0=dt0, warm_start=warm is not None, batch_steps=steps,
                 fire="pcp_fire_run, blocks of 8 steps" + (" replayed from a HIP graph" if graph else ""),
                 seconds=el, converged=int(ok.sum()), mean_steps=float(conv[ok].mean()) if ok.any() else None,
                 max_steps=int(conv.max()) if ok.all() else None, packings_per_s=float(ok.sum()) / el,
@@ -117,26 +118,37 @@ def fire_legs(args):
                                f"{r['batch_steps']} steps, {r['seconds']:.3f} s, "
                                f"{r['packings_per_s']:.2f} packings/s", flush=True)
     out = {}
+    legs = set(args.fire_legs.split(","))
     p2 = make_packing(4096, 16, 0.85, seed=args.seed, min_center_frac=0.5)
-    for tag, dt, warm, steps in (("n4096_reference_dtmax", 0.1, None, 20000), ("n4096", W.DT_MAX, None, 5000),
-                                 ("n4096_learned", W.DT_MAX, ws, 5000)):
-        out[tag] = fire_one([p2], dt, steps, warm, budget_s=args.fire_seconds, graph=not args.no_graph)
+    for tag, dt, warm, steps, dt0 in (("n4096_reference_dtmax", 0.1, None, 20000, 1e-3),
+                                      ("n4096", W.DT_MAX, None, 5000, 1e-3),
+                                      ("n4096_learned", W.DT_MAX, ws, 5000, 1e-3),
+                                      ("n4096_dt0_10", W.DT_MAX, None, 5000, 10.0)):
+        if "n4096" not in legs:
+            break
+        out[tag] = fire_one([p2], dt, steps, warm, budget_s=args.fire_seconds, graph=not args.no_graph, dt0=dt0)
         log(out[tag], tag)
     from pycudapolygon_amd import batch
-    if args.n4096_batch > 0:
+    if args.n4096_batch > 0 and "n4096_batch" in legs:
         # P packings of the N=4096 workload in one context (the first is the packing of the
         # single-packing legs above), plain and with the learned warm start applied to the
-        # whole context once, re-packed when fewer than half are still running
+        # whole context once, re-packed when fewer than half are still running; FIRE from the
+        # reference driver's initial time step 1e-3 and from 10
         P = args.n4096_batch
         t0 = time.perf_counter()
         pb = [p2] + [make_packing(4096, 16, 0.85, seed=args.seed + 1000 + k, min_center_frac=0.5)
                      for k in range(P - 1)]
         gen = time.perf_counter() - t0
-        for tag, warm, single in (("n4096_batch", None, "n4096"), ("n4096_batch_learned", ws, "n4096_learned")):
+        for tag, warm, single, dt0 in (("n4096_batch", None, "n4096", 1e-3),
+                                       ("n4096_batch_learned", ws, "n4096_learned", 1e-3),
+                                       ("n4096_batch_dt0_10", None, "n4096_dt0_10", 10.0),
+                                       ("n4096_batch_learned_dt0_10", ws, None, 10.0)):
             r = batch.minimize_batch(pb, W.SQUARED, dict(fTol=1e-10, dtMax=W.DT_MAX), 5000, warm=warm,
-                                     graph=not args.no_graph)
+                                     graph=not args.no_graph, dt0=dt0)
             ok = r.steps >= 0
-            e0 = out[single]["median_final_energy"]
+            if single not in out:   # the one-packing legs were not run
+                single = None
+            e0 = out[single]["median_final_energy"] if single else None
             out[tag] = dict(packings=P, workload=(f"{P} packings of N=4096 x 16, phi=0.85 (seeds {args.seed} and "
                                                  f"{args.seed + 1000}..{args.seed + 998 + P}), one context, "
                                                  "re-packed when fewer than half are still running"),
@@ -144,30 +156,41 @@ def fire_legs(args):
                             packings_per_s=float(ok.sum()) / r.seconds,
                             mean_steps=float(r.steps[ok].mean()) if ok.any() else None,
                             max_steps=int(r.steps[ok].max()) if ok.any() else None, contexts=r.contexts,
-                            max_final_force=float(r.max_force.max()), dtMax=W.DT_MAX, generation_seconds=gen,
-                            packing0_steps=int(r.steps[0]), packing0_steps_single_context=out[single]["batch_steps"],
+                            max_final_force=float(r.max_force.max()), dtMax=W.DT_MAX, dt0=dt0,
+                            generation_seconds=gen, packing0_steps=int(r.steps[0]),
+                            packing0_steps_single_context=out[single]["batch_steps"] if single else None,
                             packing0_final_energy=float(r.energy[0]),
-                            packing0_final_energy_rel_diff_single_context=abs(float(r.energy[0]) - e0) / abs(e0))
+                            packing0_final_energy_rel_diff_single_context=(abs(float(r.energy[0]) - e0) / abs(e0)
+                                                                           if single else None))
             print(f"[bench] FIRE {tag}: {int(ok.sum())}/{P} converged, mean steps {out[tag]['mean_steps']}, "
                   f"{r.seconds:.3f} s, {out[tag]['packings_per_s']:.2f} packings/s; packing 0: "
                   f"{r.steps[0]} steps, E rel. diff vs single context "
-                  f"{out[tag]['packing0_final_energy_rel_diff_single_context']:.1e}", flush=True)
-        plain1 = out["n4096"]["packings_per_s"]
+                  f"{out[tag]['packing0_final_energy_rel_diff_single_context']}", flush=True)
+        pps = lambda tag: out[tag]["packings_per_s"]
+        plain1 = pps("n4096") if "n4096" in out else float("nan")
         out["n4096_summary"] = dict(
-            plain_single_context=plain1, learned_single_context=out["n4096_learned"]["packings_per_s"],
-            plain_batched=out["n4096_batch"]["packings_per_s"],
-            learned_batched=out["n4096_batch_learned"]["packings_per_s"],
-            learned_batched_over_plain_single=out["n4096_batch_learned"]["packings_per_s"] / plain1,
-            learned_batched_over_plain_batched=(out["n4096_batch_learned"]["packings_per_s"]
-                                                / out["n4096_batch"]["packings_per_s"]),
-            batching_alone=out["n4096_batch"]["packings_per_s"] / plain1,
-            note="plain GPU FIRE = one packing per context; the learned-only gain is learned_batched_over_plain_batched")
-        print(f"[bench] N=4096 packings/s: plain single {plain1:.1f}, plain batched "
-              f"{out['n4096_summary']['plain_batched']:.1f}, learned batched "
-              f"{out['n4096_summary']['learned_batched']:.1f} "
-              f"({out['n4096_summary']['learned_batched_over_plain_single']:.2f}x plain single)", flush=True)
+            plain_single_context=plain1, learned_single_context=pps("n4096_learned") if "n4096" in out else None,
+            plain_single_context_dt0_10=pps("n4096_dt0_10") if "n4096" in out else float("nan"),
+            plain_batched=pps("n4096_batch"), learned_batched=pps("n4096_batch_learned"),
+            plain_batched_dt0_10=pps("n4096_batch_dt0_10"), learned_batched_dt0_10=pps("n4096_batch_learned_dt0_10"),
+            learned_batched_over_plain_single=pps("n4096_batch_learned") / plain1,
+            learned_batched_dt0_10_over_plain_single=pps("n4096_batch_learned_dt0_10") / plain1,
+            learned_over_plain_batched=pps("n4096_batch_learned") / pps("n4096_batch"),
+            learned_over_plain_batched_dt0_10=pps("n4096_batch_learned_dt0_10") / pps("n4096_batch_dt0_10"),
+            batching_alone=pps("n4096_batch") / plain1,
+            note=("plain GPU FIRE = one packing per context from the reference driver's dt0 = 1e-3; the gain "
+                  "of the learned warm start alone is learned_over_plain_batched at the same dt0: it shortens "
+                  "FIRE's dt ramp from 1e-3 and gives nothing when FIRE starts at dt0 = 10"))
+        sm = out["n4096_summary"]
+        print(f"[bench] N=4096 packings/s: plain single {plain1:.1f} (dt0 10: {sm['plain_single_context_dt0_10']:.1f}); "
+              f"batched plain/learned {sm['plain_batched']:.1f}/{sm['learned_batched']:.1f} at dt0 1e-3, "
+              f"{sm['plain_batched_dt0_10']:.1f}/{sm['learned_batched_dt0_10']:.1f} at dt0 10; learned batched = "
+              f"{sm['learned_batched_over_plain_single']:.2f}x / {sm['learned_batched_dt0_10_over_plain_single']:.2f}x "
+              f"plain single", flush=True)
     p4 = W.make_c4_packings(32, 777777)
     for tag, warm in (("n1024_x32", None), ("n1024_x32_learned", ws)):
+        if "n1024" not in legs:
+            break
         r = batch.minimize_batch(p4, W.SQUARED, dict(fTol=1e-10, dtMax=W.DT_MAX), 5000, warm=warm,
                                  graph=not args.no_graph)
         ok = r.steps >= 0
@@ -178,21 +201,23 @@ def fire_legs(args):
                         contexts=r.contexts, max_final_force=float(r.max_force.max()), dtMax=W.DT_MAX)
         print(f"[bench] FIRE {tag}: {int(ok.sum())}/32 converged, mean steps {out[tag]['mean_steps']:.1f}, "
               f"{r.seconds:.3f} s, {out[tag]['packings_per_s']:.2f} packings/s", flush=True)
-    if args.batch_packings > 0:
+    if args.batch_packings > 0 and "uneven" in legs:
         from pycudapolygon_amd import batch
         t0 = time.perf_counter()
         p3 = make_batch(args.batch_packings, args.seed + 3, min_center_frac=0.5)
         gen = time.perf_counter() - t0
         # squared lens areas, then squared pair areas (continuous where two lenses of one
         # polygon pair merge or split)
-        for tag, model, energy in (("batched_uneven", W.SQUARED, "squared lens areas"),
-                                   ("batched_uneven_pair", W.PAIR_SQUARED, "squared pair areas")):
-            r = batch.minimize_batch(p3, model, dict(fTol=1e-10, dtMax=W.DT_MAX), 5000, graph=not args.no_graph)
+        for tag, model, energy, dt0 in (("batched_uneven", W.SQUARED, "squared lens areas", 1e-3),
+                                        ("batched_uneven_pair", W.PAIR_SQUARED, "squared pair areas", 1e-3),
+                                        ("batched_uneven_pair_dt0_10", W.PAIR_SQUARED, "squared pair areas", 10.0)):
+            r = batch.minimize_batch(p3, model, dict(fTol=1e-10, dtMax=W.DT_MAX), 5000, graph=not args.no_graph,
+                                     dt0=dt0)
             ok = r.steps >= 0
             out[tag] = dict(
                 workload=(f"{args.batch_packings} packings, N ~ U[64,1024], n ~ U[8,32], phi ~ U[0.80,0.90], "
                           "one context, re-packed when fewer than half its packings are still running"),
-                energy=energy, packings=args.batch_packings, converged=int(ok.sum()), seconds=r.seconds,
+                energy=energy, dt0=dt0, packings=args.batch_packings, converged=int(ok.sum()), seconds=r.seconds,
                 packings_per_s=float(ok.sum()) / r.seconds, mean_steps=float(r.steps[ok].mean()),
                 max_steps=int(r.steps[ok].max()), step_cap=5000, contexts=r.contexts,
                 not_converged=[int(k) for k in np.where(~ok)[0]],
@@ -201,7 +226,7 @@ def fire_legs(args):
                   f"{r.seconds:.3f} s, {out[tag]['packings_per_s']:.2f} packings/s, "
                   f"mean steps {out[tag]['mean_steps']:.1f}, contexts {r.contexts}, not converged "
                   f"{out[tag]['not_converged']}", flush=True)
-    if args.compress_replicas > 0:
+    if args.compress_replicas > 0 and "compression" in legs:
         from pycudapolygon_amd import compress
         R = args.compress_replicas
         t0 = time.perf_counter()
@@ -260,6 +285,8 @@ def main():
                     help="FIRE legs launch each block's kernels directly instead of replaying a HIP graph "
                          "(for profilers that cannot trace graph launches)")
     ap.add_argument("--no-fire", action="store_true")
+    ap.add_argument("--fire-legs", default="n4096,n4096_batch,n1024,uneven,compression",
+                    help="comma-separated FIRE legs to run (for profiling one of them)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="N",
                     help="only run the plain copy kernel on N doubles (10 launches) for HBM counter "
                          "calibration, then exit")
