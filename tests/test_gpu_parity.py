@@ -352,3 +352,76 @@ def test_model_binding_smoke():
     Eo, Fo = O.force_energy("normal", p["positions"], p["startIndices"], rec, g)
     assert m.getEnergy() == Eo
     np.testing.assert_array_equal(np.array(m.getForces()), Fo)
+
+
+def _two_noisy_polygons(n, noise, seed):
+    """Two noisy n-gons of radius 0.1 almost on top of each other (centres 0.002 apart, the
+    second turned by half a vertex step): their boundaries cross at many edges."""
+    from tests import _geom as G
+    rng = np.random.default_rng(seed)
+    c = np.array([0.5, 0.5])
+
+    def noisy(cen, th):
+        P = G.regular(n, 0.1, cen, th)
+        return cen + (P - cen) * (1 + rng.uniform(-noise, noise, n))[:, None]
+
+    polys = [noisy(c, 0.0), noisy(c + 0.002, np.pi / n)]
+    start, pos = G.to_model(polys)
+    g = O.geometry(pos, start)
+    return dict(startIndices=start, positions=pos, targetEdgeV=g["edgeLengths"].copy(),
+                targetAreas=g["areas"].copy())
+
+
+def test_bucket_overflow_regrows_and_polygons_beyond_64_vertices():
+    """Two noisy 384-gons crossing 168 times (84 records per polygon, above the initial
+    64-entry record buckets): the first evaluation reports PCP_ERR_CAPACITY at the next
+    synchronisation with the buckets enlarged; the evaluation after it gives the oracle's
+    records bit for bit and its energy exactly, and forces within 1e-12 (polygons beyond 64
+    vertices are projected by the thread-serial solver).  SHAKE of those polygons matches
+    the dense oracle to 1e-13.  The reference warns and resizes (model.cpp:264-305)."""
+    pk = _two_noisy_polygons(384, 0.05, 5)
+    start, pos = pk["startIndices"], pk["positions"]
+    nb, rec = oracle_records(pk, 2)
+    assert rec["numIntersections"] == 168 and list(rec["shapeCounts"]) == [84, 84]
+    s = capi.System([pk], [2])
+    s.force_eval()
+    with pytest.raises(capi.PcpError, match="overflow"):
+        s.sync()
+    s.force_eval()
+    s.sync()
+    r = s.get_records(0)
+    for key in ("intersections", "outersections", "tu", "ut", "shapeCounts"):
+        np.testing.assert_array_equal(r[key], rec[key], err_msg=key)
+    F, E = s.get_forces(0), s.get_energy()[0]
+    Eo, Fo, nIo = O.System(start, 2, "normal", pk["targetEdgeV"], pk["targetAreas"]).force_eval(pos)
+    assert nIo == 168 and E == Eo
+    assert np.abs(F - Fo).max() <= 1e-12 * np.abs(Fo).max()
+    rng = np.random.default_rng(6)
+    noisy = (pos + rng.normal(scale=2e-5, size=pos.shape)) % 1.0
+    s.pos.upload(noisy)
+    it = s.shake(50)
+    xo, ito = O.shake(noisy, start, pk["targetEdgeV"], pk["targetAreas"], 50)
+    assert np.abs((s.get_positions(0) - xo + 0.5) % 1.0 - 0.5).max() < 1e-13 and abs(int(it[0]) - ito) <= 1
+
+
+def test_packing_without_overlaps():
+    """A dilute packing (phi = 0.3, no two centres closer than the sum of the radii): no
+    crossings, zero records, zero energy and force on the GPU as in the oracle, and FIRE
+    stops after its first step with max|F| = 0."""
+    p = make_packing(64, 12, 0.3, seed=1, min_center_frac=1.0)
+    B = box_size_for(p["diameter"])
+    nb, rec = oracle_records(p, B)
+    assert rec["numIntersections"] == 0
+    for model in (capi.NORMAL, capi.NORMAL | SQUARED, capi.NORMAL | PAIR):
+        s = capi.System([p], [B], model=model)
+        s.force_eval()
+        s.sync()
+        assert s.get_records(0)["numIntersections"] == 0
+        F, E = s.get_forces(0), s.get_energy()[0]
+        Eo, Fo, nIo = O.System(p["startIndices"], B, model, p["targetEdgeV"], p["targetAreas"]).force_eval(
+            p["positions"])
+        assert nIo == 0 and E == Eo == 0.0 and np.array_equal(F, Fo) and not F.any()
+        prm = s.fire_params(fTol=1e-10, dtMax=100.0)
+        s.fire_init(prm, 1e-3)
+        st = s.fire_minimize(prm, 10)[0]
+        assert st.converged and st.steps == 1 and st.maxForce == 0.0
