@@ -240,9 +240,10 @@ def fire_legs(args):
         gen = time.perf_counter() - t0
         wc = W.WarmStart(W.load_weights(W.COMPRESS_WEIGHTS))
         out["compression"] = {}
-        for tag, warm, dt0 in (("plain", None, 1e-3), ("plain_dt0_100", None, 100.0), ("learned", wc, 1e-3)):
+        for tag, warm, dt0, shape in (("plain", None, 1e-3, False), ("plain_dt0_100", None, 100.0, False),
+                                      ("plain_dt0_100_shape_match", None, 100.0, True), ("learned", wc, 1e-3, False)):
             steps, s5 = compress.ramp(p5, 0.88, 1e-3, dict(fTol=1e-10, dtMax=W.DT_MAX), 5000, warm=warm, dt0=dt0,
-                                      budget_s=args.compress_seconds, graph=not args.no_graph)
+                                      budget_s=args.compress_seconds, graph=not args.no_graph, shape_match=shape)
             conv = all(all(r["converged"]) for r in steps)
             sec = sum(r["seconds"] for r in steps)
             nst = len(steps) - 1
@@ -252,6 +253,8 @@ def fire_legs(args):
                           "after every step"),
                 warm_start=("network trained on compression steps, applied after every rescale"
                             if warm is not None else None), dt0=dt0,
+                after_rescale=("shape matching (pcp_shape_project: rest shape at the new target area, 2x2 "
+                               "polar decomposition)" if shape else "pcp_reset_areas"),
                 replicas=R, vertices=s5.V, phi_reached=float(np.mean(steps[-1]["phi"])), phi_steps=nst,
                 all_steps_converged=conv, seconds=sec, initial_minimization_seconds=steps[0]["seconds"],
                 seconds_per_phi_step=(sec - steps[0]["seconds"]) / max(nst, 1),

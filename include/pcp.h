@@ -153,6 +153,17 @@ int pcp_update_positions(pcp_ctx* ctx, double* pos, const double* force, double 
 /* resetAreasCUDA model.cu:639-645 */
 int pcp_reset_areas(pcp_ctx* ctx, double* pos, const double* areas, const double* targetAreas,
                     const double* comX, const double* comY);
+/* Shape matching, the "batched 2x2 SVD shape projection" of the compression protocol (no
+ * counterpart in the reference, whose projection acts on forces only): every polygon is
+ * replaced by its rest shape refShape (device[2V]: each vertex's offset from the rest
+ * shape's vertex mean), scaled to its target area (targetAreas device[N], or NULL for no
+ * scaling), turned by the rotation of the polar decomposition of the 2x2 matrix
+ * M = sum_i q_i refShape_i^T (q_i: current minimum-image offsets from their vertex mean;
+ * the SVD polar factor is [[a, -b], [b, a]] / sqrt(a^2 + b^2), a = M00 + M11,
+ * b = M10 - M01) and placed at the current vertex mean: the rigid motion of the rest shape
+ * closest to the current vertices in least squares.  Polygons of 3..64 vertices
+ * (PCP_ERR_ARG otherwise). */
+int pcp_shape_project(pcp_ctx* ctx, double* pos, const double* refShape, const double* targetAreas);
 /* saveTentativePositionsCUDA + getMaxEffectiveForceCUDA model.cu:280-306: out host[P] */
 int pcp_max_effective_force(pcp_ctx* ctx, const double* pos, const double* tentPos,
                             const double* force, double scale, double* out);

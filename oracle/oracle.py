@@ -71,6 +71,8 @@ def _declare(L):
     L.orc_project_force_lanes.argtypes = [_i, _i, _p, _p, _p]
     L.orc_shake_lanes.restype = _i
     L.orc_shake_lanes.argtypes = [_i, _i, _p, _p, _p, _p, _i, _d]
+    L.orc_shape_project.restype = _i
+    L.orc_shape_project.argtypes = [_i, _i, _p, _p, _p, _p]
     L.orc_xpbd.restype = _i
     L.orc_xpbd.argtypes = [_i, _i, _p, _p, _p, _p, _i, _d]
     L.orc_det_sum.restype = _d
@@ -222,6 +224,15 @@ def shake_lanes(pos, start, targetEdgeV, targetAreas, maxIter, tol=1e-15):
     if it < 0:
         raise ValueError("lane-order SHAKE takes polygons of 3..64 vertices")
     return pos, it
+
+
+def shape_project(pos, start, ref, targetAreas=None):
+    pos = _f64(pos).copy(); start = _i32(start)
+    ta = _f64(targetAreas) if targetAreas is not None else None
+    rc = lib().orc_shape_project(int(start[-1]), len(start) - 1, _ptr(start), _ptr(pos), _ptr(_f64(ref)), _ptr(ta))
+    if rc != 0:
+        raise ValueError("shape matching takes polygons of 3..64 vertices")
+    return pos
 
 
 def xpbd(pos, start, targetEdgeLengths, targetAreas, nIter, tol=0.0):

@@ -1227,6 +1227,61 @@ int orc_shake_lanes(int V, int N, const int* start, double* pos, const double* t
     return worst;
 }
 
+/* ---------- shape matching (no counterpart in the reference) ----------
+ * Every polygon becomes its rest shape ref (offsets from the rest shape's vertex mean),
+ * scaled to its target area (when targetAreas is given), turned by the rotation that
+ * best aligns ref with the current offsets q (minimum image from vertex 0, minus their
+ * mean) and placed at the current vertex mean.  The rotation maximizes
+ * trace(R^T M), M = sum q r^T: in 2-D the polar factor of the SVD of M, R = [[a, -b],
+ * [b, a]] / sqrt(a^2 + b^2), a = M00 + M11, b = M10 - M01.  Sums in lane order
+ * (lanes_sum), so the library's kernel reproduces it bit for bit; the least-squares
+ * property is pinned by a brute-force search over the angle in the tests. */
+int orc_shape_project(int V, int N, const int* start, double* pos, const double* ref,
+                      const double* targetAreas) {
+    (void)V;
+    for (int s = 0; s < N; ++s)
+        if (start[s + 1] - start[s] < 3 || start[s + 1] - start[s] > LANES_MAX) return -1;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int s = 0; s < N; ++s) {
+        int st = start[s], n = start[s + 1] - st;
+        double dx[LANES_MAX], dy[LANES_MAX], t[LANES_MAX];
+        double p0x = pos[2 * st], p0y = pos[2 * st + 1];
+        for (int e = 0; e < n; ++e) {
+            dx[e] = orc_wrap(pos[2 * (st + e)] - p0x);
+            dy[e] = orc_wrap(pos[2 * (st + e) + 1] - p0y);
+        }
+        double mx = lanes_sum(dx, n) / n, my = lanes_sum(dy, n) / n;
+        double qx[LANES_MAX], qy[LANES_MAX];
+        for (int e = 0; e < n; ++e) { qx[e] = dx[e] - mx; qy[e] = dy[e] - my; }
+        const double* r = ref + 2 * st;
+        for (int e = 0; e < n; ++e) t[e] = qx[e] * r[2 * e];
+        double m00 = lanes_sum(t, n);
+        for (int e = 0; e < n; ++e) t[e] = qx[e] * r[2 * e + 1];
+        double m01 = lanes_sum(t, n);
+        for (int e = 0; e < n; ++e) t[e] = qy[e] * r[2 * e];
+        double m10 = lanes_sum(t, n);
+        for (int e = 0; e < n; ++e) t[e] = qy[e] * r[2 * e + 1];
+        double m11 = lanes_sum(t, n);
+        for (int e = 0; e < n; ++e) {
+            int en = (e + 1 >= n) ? 0 : e + 1;
+            t[e] = r[2 * e] * r[2 * en + 1] - r[2 * en] * r[2 * e + 1];
+        }
+        double twiceA = lanes_sum(t, n);
+        double a = m00 + m11, b = m10 - m01;
+        double h = sqrt(a * a + b * b);
+        double ca = (h > 0.0) ? a / h : 1.0, sb = (h > 0.0) ? b / h : 0.0;
+        double sc = 1.0;
+        if (targetAreas && twiceA > 0.0) sc = sqrt(targetAreas[s] / (twiceA / 2.0));
+        for (int e = 0; e < n; ++e) {
+            double ux = ca * r[2 * e] - sb * r[2 * e + 1], uy = sb * r[2 * e] + ca * r[2 * e + 1];
+            double px = p0x + (mx + sc * ux), py = p0y + (my + sc * uy);
+            pos[2 * (st + e)] = px - floor(px);
+            pos[2 * (st + e) + 1] = py - floor(py);
+        }
+    }
+    return 0;
+}
+
 /* ---------- XPBD: xpbdProjectCUDA model.cu:209-278, kernels.cuh:532-623 ----------
  * Colour 0 then colour 1 edge projections (disjoint within a colour for even n;
  * here applied in ascending edge order, which is the reference's result for even n),

@@ -68,6 +68,10 @@ int orc_shake(int V, int N, const int* start, double* pos, const double* targetE
 int orc_project_force_lanes(int V, int N, const int* start, const double* pos, double* force);
 int orc_shake_lanes(int V, int N, const int* start, double* pos, const double* targetEdgeV,
                     const double* targetAreas, int maxIter, double tol);
+/* shape matching: rest shape ref (offsets from its vertex mean), scaled to targetAreas
+ * (or not, when NULL), at the least-squares rotation and translation (3..64 vertices) */
+int orc_shape_project(int V, int N, const int* start, double* pos, const double* ref,
+                      const double* targetAreas);
 int orc_xpbd(int V, int N, const int* start, double* pos, const double* targetEdgeP,
              const double* targetAreas, int nIter, double tol);
 double orc_det_sum(const double* x, int n);

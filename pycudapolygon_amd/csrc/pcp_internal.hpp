@@ -177,6 +177,7 @@ int run_forces(pcp_ctx* c, int model, const double2* pos, double* edgeLengths, d
                bool sticky = false);
 int run_project(pcp_ctx* c, const double2* pos, double2* force);
 int run_project_seg(pcp_ctx* c, const double2* pos, double2* force);
+int run_shape_project_seg(pcp_ctx* c, double2* pos, const double2* ref, const double* targetAreas);
 int run_geometry(pcp_ctx* c, const double2* pos, double* edgeLengths, double* areas, double* comX,
                  double* comY, double* constraints, double* maxEdge);
 int run_shake(pcp_ctx* c, double2* pos, const double* targetEdgeV, const double* targetAreas,

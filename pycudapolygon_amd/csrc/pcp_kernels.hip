@@ -1419,6 +1419,14 @@ int pcp_reset_areas(pcp_ctx* c, double* pos, const double* areas, const double* 
     PCP_LAUNCHED();
     return PCP_OK;
 }
+int pcp_shape_project(pcp_ctx* c, double* pos, const double* refShape, const double* targetAreas) {
+    if (c->N == 0) return PCP_OK;
+    if (c->maxN > 64) {
+        set_error("pcp_shape_project: polygons of at most 64 vertices");
+        return PCP_ERR_ARG;
+    }
+    return run_shape_project_seg(c, (double2*)pos, (const double2*)refShape, targetAreas);
+}
 int pcp_max_effective_force(pcp_ctx* c, const double* pos, const double* tentPos, const double* force,
                             double scale, double* out) {
     if (c->N == 0) return PCP_OK;

@@ -373,6 +373,59 @@ __global__ __launch_bounds__(BLK) void k_shake_seg(int N, int P, const int* __re
     block_pack_max<BLK / SEG>(threadIdx.x / SEG, e == 0, pk, it1, it2, P, itOut);
 }
 
+// Shape matching of each polygon (lane e owns vertex e): its rest shape ref (offsets from
+// the rest shape's vertex mean), scaled to the target area when targetAreas is given,
+// turned by the rotation of the polar decomposition of M = sum_e q_e ref_e^T (q_e = the
+// current minimum-image offsets from their vertex mean; in 2-D the rotation of the SVD
+// polar factor is [[a, -b], [b, a]] / sqrt(a^2 + b^2), a = M00 + M11, b = M10 - M01) and
+// placed at the current vertex mean: the rigid motion of the rest shape nearest to the
+// current vertices in least squares.  Sums are segment butterflies (zero padding).
+template <int SEG>
+__global__ __launch_bounds__(BLK) void k_shape_project_seg(int N, const int* __restrict__ polyStart,
+                                                          double2* __restrict__ pos,
+                                                          const double2* __restrict__ ref,
+                                                          const double* __restrict__ targetAreas) {
+    int e = threadIdx.x & (SEG - 1);
+    int s = (blockIdx.x * BLK + threadIdx.x) / SEG;
+    if (s >= N) return;   // whole segments leave together
+    int st = polyStart[s], n = polyStart[s + 1] - st;
+    if (n < 3 || n > SEG) return;
+    bool live = e < n;
+    int en = (e + 1 >= n) ? 0 : e + 1;
+    double2 p = live ? pos[st + e] : make_double2(0.0, 0.0);
+    double2 r = live ? ref[st + e] : make_double2(0.0, 0.0);
+    double p0x = seg_get<SEG>(p.x, 0), p0y = seg_get<SEG>(p.y, 0);
+    double rnx = seg_get<SEG>(r.x, en), rny = seg_get<SEG>(r.y, en);
+    double dx = live ? wrap(p.x - p0x) : 0.0, dy = live ? wrap(p.y - p0y) : 0.0;
+    double mx = seg_sum<SEG>(dx) / n, my = seg_sum<SEG>(dy) / n;
+    double qx = live ? dx - mx : 0.0, qy = live ? dy - my : 0.0;
+    double m00 = seg_sum<SEG>(qx * r.x), m01 = seg_sum<SEG>(qx * r.y);
+    double m10 = seg_sum<SEG>(qy * r.x), m11 = seg_sum<SEG>(qy * r.y);
+    double twiceA = seg_sum<SEG>(live ? r.x * rny - rnx * r.y : 0.0);   // shoelace of the rest shape
+    double a = m00 + m11, b = m10 - m01;
+    double h = sqrt(a * a + b * b);
+    double ca = (h > 0.0) ? a / h : 1.0, sb = (h > 0.0) ? b / h : 0.0;
+    double sc = 1.0;
+    if (targetAreas && twiceA > 0.0) sc = sqrt(targetAreas[s] / (twiceA / 2.0));
+    if (!live) return;
+    double ux = ca * r.x - sb * r.y, uy = sb * r.x + ca * r.y;
+    double px = p0x + (mx + sc * ux), py = p0y + (my + sc * uy);
+    pos[st + e] = make_double2(px - floor(px), py - floor(py));
+}
+
+int run_shape_project_seg(pcp_ctx* c, double2* pos, const double2* ref, const double* targetAreas) {
+    int seg = c->seg;
+    int grid = (c->N + BLK / seg - 1) / (BLK / seg);
+    switch (seg) {
+    case 4: PCP_LAUNCH(c, k_shape_project_seg<4>, dim3(grid), dim3(BLK), c->N, c->dPolyStart, pos, ref, targetAreas); break;
+    case 8: PCP_LAUNCH(c, k_shape_project_seg<8>, dim3(grid), dim3(BLK), c->N, c->dPolyStart, pos, ref, targetAreas); break;
+    case 16: PCP_LAUNCH(c, k_shape_project_seg<16>, dim3(grid), dim3(BLK), c->N, c->dPolyStart, pos, ref, targetAreas); break;
+    case 32: PCP_LAUNCH(c, k_shape_project_seg<32>, dim3(grid), dim3(BLK), c->N, c->dPolyStart, pos, ref, targetAreas); break;
+    default: PCP_LAUNCH(c, k_shape_project_seg<64>, dim3(grid), dim3(BLK), c->N, c->dPolyStart, pos, ref, targetAreas); break;
+    }
+    return PCP_OK;
+}
+
 int run_project_seg(pcp_ctx* c, const double2* pos, double2* force) {
     int seg = c->seg;
     int grid = (c->N + BLK / seg - 1) / (BLK / seg);

@@ -73,3 +73,55 @@ def test_short_compression_ramp_minimizes_every_step():
     np.testing.assert_allclose(compress.packing_fractions(s), 0.804, rtol=1e-12)
     s.geometry(constraints=False)
     np.testing.assert_allclose(s.areas.get(), s.ta.get(), rtol=1e-9)   # SHAKE keeps the areas
+
+
+def test_shape_projection_bit_identical_to_oracle():
+    """pcp_shape_project (2x2 polar decomposition per polygon) equals the oracle's shape
+    matching bit for bit: 16-gons (16-lane segments) and uneven packings (n 8..32), with
+    and without scaling to target areas, on positions moved away from the rest shapes."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(7)
+    cases = [[make_packing(64, 16, 0.8, seed=3)], make_batch(4, 61, N_range=(40, 200), n_range=(8, 32))]
+    for packs in cases:
+        ref = compress.rest_shapes(packs)
+        s = capi.System(packs, [box_size_for(p["diameter"]) for p in packs])
+        x = (np.concatenate([p["positions"] for p in packs]) + rng.normal(scale=1e-3, size=len(ref))) % 1.0
+        ta = np.concatenate([p["targetAreas"] for p in packs]) * 1.01
+        dref, dta = capi.DeviceArray.from_numpy(ref), capi.DeviceArray.from_numpy(ta)
+        for targets in (None, dta):
+            print(f"[shape] packings={len(packs)} scaled={targets is not None}", flush=True)
+            s.pos.upload(x)
+            s.ctx.call("pcp_shape_project", s.pos.ptr, dref.ptr, targets.ptr if targets is not None else None)
+            s.sync()
+            y = s.pos.get()
+            for k, p in enumerate(packs):
+                a, b = s.vertex_range(k)
+                pa, pb = int(s.packPoly[k]), int(s.packPoly[k + 1])
+                yo = O.shape_project(x[2 * a:2 * b], p["startIndices"], ref[2 * a:2 * b],
+                                     ta[pa:pb] if targets is not None else None)
+                assert np.array_equal(y[2 * a:2 * b], yo)
+
+
+def test_compression_ramp_with_shape_matching():
+    """Ramp with shape matching after every step: right after a step every polygon is its
+    initial shape (edge lengths scaled by sqrt(phi'/phi0), area = target), and every step
+    ends minimized."""
+    pk = [make_packing(256, 12, 0.80, seed=71 + k, min_center_frac=0.5) for k in range(2)]
+    ref = compress.rest_shapes(pk)
+    s = capi.System(pk, [box_size_for(p["diameter"] * 1.05) for p in pk], model=capi.NORMAL | 0x200)
+    dref = capi.DeviceArray.from_numpy(ref)
+    phi0 = compress.packing_fractions(s)
+    compress.set_phi(s, phi0 + 0.01, dref)
+    x = s.pos.get()
+    e0 = np.concatenate([np.linalg.norm(np.roll(d, -1, axis=0) - d, axis=1)
+                         for d in unwrapped(np.concatenate([p["positions"] for p in pk]), s.polyStart)])
+    e1 = np.concatenate([np.linalg.norm(np.roll(d, -1, axis=0) - d, axis=1) for d in unwrapped(x, s.polyStart)])
+    f = np.repeat(np.sqrt((phi0 + 0.01) / phi0), [int(s.polyStart[s.packPoly[k + 1]] - s.polyStart[s.packPoly[k]])
+                                                   for k in range(s.P)])
+    np.testing.assert_allclose(e1, e0 * f, rtol=1e-12)
+    areas = np.array([shoelace(d) for d in unwrapped(x, s.polyStart)])
+    np.testing.assert_allclose(areas, s.ta.get(), rtol=1e-12)
+    steps, s2 = compress.ramp(pk, 0.82, 1e-2, dict(fTol=1e-10, dtMax=100.0), 5000, dt0=100.0, shape_match=True)
+    assert len(steps) == 3
+    for r in steps:
+        assert all(r["converged"]) and max(r["max_force"]) <= 1e-10

@@ -597,3 +597,70 @@ def test_area_energy_fire_separates_two_polygons():
     E0 = O.System(start, 2, AREA, tev, ta).force_eval(pos)[0]
     x, f, st, steps = O.System(start, 2, AREA, tev, ta).fire_minimize(pos, fTol=1e-10, maxSteps=20000)
     assert np.abs(f).max() <= 1e-10 and st.energy == 0.0 and E0 > 0 and steps < 20000
+
+
+def _rest_and_moved(seed, n, c0, theta, scale, noise=0.0):
+    """A noisy n-gon rest shape r (offsets from its vertex mean) and the polygon
+    c0 + scale * R(theta) r (+ Gaussian vertex noise), wrapped into the unit box."""
+    rng = np.random.default_rng(seed)
+    P = G.regular(n, 0.05, (0.0, 0.0), rng.uniform(0, 6))
+    P = P * (1 + rng.uniform(-0.05, 0.05, n))[:, None]
+    r = P - P.mean(axis=0)
+    R = np.array([[np.cos(theta), -np.sin(theta)], [np.sin(theta), np.cos(theta)]])
+    x = np.asarray(c0) + scale * r @ R.T + rng.normal(scale=noise, size=r.shape)
+    return r, x % 1.0
+
+
+def test_shape_projection_keeps_a_moved_rest_shape():
+    """A rest shape turned, scaled and moved (across the periodic boundary) projects onto
+    itself: with the target area of the scaled shape the result equals the input to
+    1e-15, its area is the target and a second projection changes nothing."""
+    for seed, (c0, th, sc) in enumerate([((0.5, 0.5), 0.3, 1.0), ((0.99, 0.01), -2.0, 1.3), ((0.02, 0.6), 3.0, 0.7)]):
+        r, x = _rest_and_moved(seed, 12, c0, th, sc)
+        start = np.array([0, 12], np.int32)
+        ta = np.array([sc * sc * G.shoelace(r)])
+        y = O.shape_project(x.reshape(-1), start, r.reshape(-1), ta)
+        assert np.abs((y - x.reshape(-1) + 0.5) % 1.0 - 0.5).max() < 1e-15
+        d = (y.reshape(-1, 2) - y[:2] + 0.5) % 1.0 - 0.5
+        np.testing.assert_allclose(G.shoelace(d), ta[0], rtol=1e-14)
+        assert np.abs(O.shape_project(y, start, r.reshape(-1), ta) - y).max() <= 2.3e-16   # 2 ulp at 0.5
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_shape_projection_is_the_least_squares_rigid_motion(seed):
+    """Noisy vertices: the result is the rest shape (unscaled) at the vertex mean of the
+    input and at the rotation angle that minimizes sum |q_i - R(theta) r_i|^2, found here
+    by a brute-force scan plus golden-section refinement of theta (independent of the
+    closed form used by the oracle)."""
+    r, x = _rest_and_moved(10 + seed, 16, (0.3, 0.7), 1.0 + seed, 1.0, noise=3e-3)
+    start = np.array([0, 16], np.int32)
+    y = O.shape_project(x.reshape(-1), start, r.reshape(-1)).reshape(-1, 2)
+    q = (x - x[0] + 0.5) % 1.0 - 0.5
+    q = q - q.mean(axis=0)
+
+    def cost(t):
+        R = np.array([[np.cos(t), -np.sin(t)], [np.sin(t), np.cos(t)]])
+        return np.sum((q - r @ R.T) ** 2)
+    ts = np.linspace(-np.pi, np.pi, 20001)
+    t0 = ts[np.argmin([cost(t) for t in ts])]
+    lo, hi = t0 - 1e-3, t0 + 1e-3
+    g = (np.sqrt(5) - 1) / 2
+    for _ in range(80):
+        a, b = hi - g * (hi - lo), lo + g * (hi - lo)
+        if cost(a) < cost(b):
+            hi = b
+        else:
+            lo = a
+    tbest = 0.5 * (lo + hi)
+    yd = (y - x[0] + 0.5) % 1.0 - 0.5
+    np.testing.assert_allclose(yd.mean(axis=0), ((x - x[0] + 0.5) % 1.0 - 0.5).mean(axis=0), atol=1e-15)
+    yo = yd - yd.mean(axis=0)
+    tgot = np.arctan2(np.sum(r[:, 0] * yo[:, 1] - r[:, 1] * yo[:, 0]), np.sum(r[:, 0] * yo[:, 0] + r[:, 1] * yo[:, 1]))
+    assert abs((tgot - tbest + np.pi) % (2 * np.pi) - np.pi) < 1e-8
+    np.testing.assert_allclose(np.linalg.norm(yo, axis=1), np.linalg.norm(r, axis=1), rtol=1e-13)
+
+
+def test_shape_projection_rejects_polygons_beyond_64():
+    p = make_packing(4, 65, 0.5, seed=0)
+    with pytest.raises(ValueError):
+        O.shape_project(p["positions"], p["startIndices"], np.zeros(len(p["positions"])))
