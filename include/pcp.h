@@ -161,8 +161,8 @@ int pcp_reset_areas(pcp_ctx* ctx, double* pos, const double* areas, const double
  * M = sum_i q_i refShape_i^T (q_i: current minimum-image offsets from their vertex mean;
  * the SVD polar factor is [[a, -b], [b, a]] / sqrt(a^2 + b^2), a = M00 + M11,
  * b = M10 - M01) and placed at the current vertex mean: the rigid motion of the rest shape
- * closest to the current vertices in least squares.  Polygons of 3..64 vertices
- * (PCP_ERR_ARG otherwise). */
+ * closest to the current vertices in least squares.  Polygons of at most 64 vertices
+ * (PCP_ERR_ARG otherwise); polygons of fewer than 3 are left as they are. */
 int pcp_shape_project(pcp_ctx* ctx, double* pos, const double* refShape, const double* targetAreas);
 /* saveTentativePositionsCUDA + getMaxEffectiveForceCUDA model.cu:280-306: out host[P] */
 int pcp_max_effective_force(pcp_ctx* ctx, const double* pos, const double* tentPos,

@@ -213,7 +213,7 @@ def project_force_lanes(pos, start, force):
     pos = _f64(pos); start = _i32(start); f = _f64(force).copy()
     rc = lib().orc_project_force_lanes(int(start[-1]), len(start) - 1, _ptr(start), _ptr(pos), _ptr(f))
     if rc != 0:
-        raise ValueError("lane-order projection takes polygons of 3..64 vertices")
+        raise ValueError("lane-order projection takes polygons of at most 64 vertices")
     return f
 
 
@@ -222,7 +222,7 @@ def shake_lanes(pos, start, targetEdgeV, targetAreas, maxIter, tol=1e-15):
     it = lib().orc_shake_lanes(int(start[-1]), len(start) - 1, _ptr(start), _ptr(pos), _ptr(_f64(targetEdgeV)),
                                _ptr(_f64(targetAreas)), int(maxIter), float(tol))
     if it < 0:
-        raise ValueError("lane-order SHAKE takes polygons of 3..64 vertices")
+        raise ValueError("lane-order SHAKE takes polygons of at most 64 vertices")
     return pos, it
 
 
@@ -231,7 +231,7 @@ def shape_project(pos, start, ref, targetAreas=None):
     ta = _f64(targetAreas) if targetAreas is not None else None
     rc = lib().orc_shape_project(int(start[-1]), len(start) - 1, _ptr(start), _ptr(pos), _ptr(_f64(ref)), _ptr(ta))
     if rc != 0:
-        raise ValueError("shape matching takes polygons of 3..64 vertices")
+        raise ValueError("shape matching takes polygons of at most 64 vertices")
     return pos
 
 

@@ -1028,7 +1028,8 @@ int orc_shake(int V, int N, const int* start, double* pos, const double* targetE
  * padding).  Every operation is written in the kernels' order, so FIRE with these
  * solves follows the GPU's trajectory bit for bit; the mathematics is pinned by the
  * dense solvers above (orc_project_force: Householder QR; orc_shake: Gaussian
- * elimination), which these must equal to rounding.  Polygons of 3..64 vertices. */
+ * elimination), which these must equal to rounding.  Polygons of at most 64 vertices
+ * (-1 otherwise); polygons of fewer than 3 are left as they are, as in the kernels. */
 #define LANES_MAX 64
 static double lanes_sum(const double* v, int n) {
     double a[LANES_MAX], b[LANES_MAX];
@@ -1130,10 +1131,11 @@ static void lanes_bordered_solve(int n, const double* tx, const double* ty, cons
 int orc_project_force_lanes(int V, int N, const int* start, const double* pos, double* force) {
     (void)V;
     for (int s = 0; s < N; ++s)
-        if (start[s + 1] - start[s] < 3 || start[s + 1] - start[s] > LANES_MAX) return -1;
+        if (start[s + 1] - start[s] > LANES_MAX) return -1;
 #pragma omp parallel for schedule(dynamic, 16)
     for (int s = 0; s < N; ++s) {
         int st = start[s], n = start[s + 1] - st;
+        if (n < 3) continue;   /* as the kernel: a degenerate polygon is left as it is */
         double tx[LANES_MAX], ty[LANES_MAX], ax[LANES_MAX], ay[LANES_MAX], r[LANES_MAX], y[LANES_MAX];
         double sq[LANES_MAX], af[LANES_MAX], z;
         for (int e = 0; e < n; ++e) {
@@ -1216,11 +1218,12 @@ int orc_shake_lanes(int V, int N, const int* start, double* pos, const double* t
     (void)V;
     if (N == 0 || maxIter == 0) return 0;
     for (int s = 0; s < N; ++s)
-        if (start[s + 1] - start[s] < 3 || start[s + 1] - start[s] > LANES_MAX) return -1;
+        if (start[s + 1] - start[s] > LANES_MAX) return -1;
     int worst = 0;
 #pragma omp parallel for schedule(dynamic, 16) reduction(max : worst)
     for (int s = 0; s < N; ++s) {
         int st = start[s], n = start[s + 1] - st;
+        if (n < 3) continue;   /* as the kernel: a degenerate polygon is left as it is */
         int it = lanes_shake_poly(n, pos + 2 * st, targetEdgeV + st, targetAreas[s], maxIter, tol);
         if (it > worst) worst = it;
     }
@@ -1240,10 +1243,11 @@ int orc_shape_project(int V, int N, const int* start, double* pos, const double*
                       const double* targetAreas) {
     (void)V;
     for (int s = 0; s < N; ++s)
-        if (start[s + 1] - start[s] < 3 || start[s + 1] - start[s] > LANES_MAX) return -1;
+        if (start[s + 1] - start[s] > LANES_MAX) return -1;
 #pragma omp parallel for schedule(dynamic, 16)
     for (int s = 0; s < N; ++s) {
         int st = start[s], n = start[s + 1] - st;
+        if (n < 3) continue;   /* as the kernel: a degenerate polygon is left as it is */
         double dx[LANES_MAX], dy[LANES_MAX], t[LANES_MAX];
         double p0x = pos[2 * st], p0y = pos[2 * st + 1];
         for (int e = 0; e < n; ++e) {

@@ -63,13 +63,15 @@ double orc_force_energy(int model, int V, int N, const int* start, const double*
 void orc_project_force(int V, int N, const int* start, const double* pos, double* force);
 int orc_shake(int V, int N, const int* start, double* pos, const double* targetEdgeV,
               const double* targetAreas, int maxIter, double tol);
-/* the same two solves in the HIP kernels' lane order (polygons of 3..64 vertices;
- * -1 otherwise): bit-identical to the library, equal to the two above to rounding */
+/* the same two solves in the HIP kernels' lane order (polygons of at most 64 vertices,
+ * -1 otherwise; fewer than 3 are left as they are): bit-identical to the library, equal
+ * to the two above to rounding */
 int orc_project_force_lanes(int V, int N, const int* start, const double* pos, double* force);
 int orc_shake_lanes(int V, int N, const int* start, double* pos, const double* targetEdgeV,
                     const double* targetAreas, int maxIter, double tol);
 /* shape matching: rest shape ref (offsets from its vertex mean), scaled to targetAreas
- * (or not, when NULL), at the least-squares rotation and translation (3..64 vertices) */
+ * (or not, when NULL), at the least-squares rotation and translation (at most 64
+ * vertices, -1 otherwise; fewer than 3 are left as they are) */
 int orc_shape_project(int V, int N, const int* start, double* pos, const double* ref,
                       const double* targetAreas);
 int orc_xpbd(int V, int N, const int* start, double* pos, const double* targetEdgeP,
