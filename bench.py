@@ -216,7 +216,7 @@ def fire_legs(args):
                                              ("batched_uneven_pair_dt0_10", W.PAIR_SQUARED, "squared pair areas",
                                               10.0, 25000)):
             r = batch.minimize_batch(p3, model, dict(fTol=1e-10, dtMax=W.DT_MAX), cap, graph=not args.no_graph,
-                                     dt0=dt0)
+                                     dt0=dt0, autotune_reps=5)
             ok = r.steps >= 0
             out[tag] = dict(
                 workload=(f"{args.batch_packings} packings, N ~ U[64,1024], n ~ U[8,32], phi ~ U[0.80,0.90], "
@@ -226,11 +226,14 @@ def fire_legs(args):
                 max_steps=int(r.steps[ok].max()), step_cap=cap, contexts=r.contexts,
                 converged_within_5000_steps=int(((r.steps >= 0) & (r.steps <= 5000)).sum()),
                 not_converged=[int(k) for k in np.where(~ok)[0]],
+                schedule=dict(lanes_per_polygon_pair_lens_exterior=list(r.schedule),
+                              autotuner_us_per_launch=np.round(r.schedule_us, 1).tolist()),
                 max_final_force_converged=float(r.max_force[ok].max()), generation_seconds=gen, dtMax=W.DT_MAX)
             print(f"[bench] FIRE {tag}: {int(ok.sum())}/{args.batch_packings} converged, "
                   f"{r.seconds:.3f} s, {out[tag]['packings_per_s']:.2f} packings/s, "
                   f"mean steps {out[tag]['mean_steps']:.1f}, max {out[tag]['max_steps']}, cap {cap}, "
-                  f"{out[tag]['converged_within_5000_steps']} within 5000 steps, contexts {r.contexts}, "
+                  f"{out[tag]['converged_within_5000_steps']} within 5000 steps, schedule {r.schedule}, "
+                  f"contexts {r.contexts}, "
                   f"not converged {out[tag]['not_converged']}", flush=True)
     if args.compress_replicas > 0 and "compression" in legs:
         from pycudapolygon_amd import compress

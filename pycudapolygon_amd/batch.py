@@ -24,6 +24,8 @@ class BatchResult:
         self.positions = [None] * P
         self.seconds = 0.0
         self.contexts = []                   # packings per context used, in order
+        self.schedule = None                 # lanes per polygon (pair, lens, exterior) when autotuned
+        self.schedule_us = None              # the autotuner's microseconds per candidate launch
 
 
 def _with_positions(p, x):
@@ -33,14 +35,17 @@ def _with_positions(p, x):
 
 
 def minimize_batch(packings, model, fire_kwargs, maxSteps, dt0=1e-3, compact_below=0.5, min_context=8,
-                   boxSizes=None, log=None, warm=None, block=8, graph=True):
+                   boxSizes=None, log=None, warm=None, block=8, graph=True, autotune_reps=0):
     """FIRE on `packings` until each has max|F| <= fTol or maxSteps steps of its own.
     compact_below: re-pack when fewer than this fraction of the context's packings are
     still running (and the context holds more than min_context packings).
     warm: a warmstart.WarmStart applied once to the first context before FIRE starts
     (its time is included in the result's seconds).  FIRE runs in blocks of `block` steps
     (pcp_fire_run, replayed from a HIP graph when graph is set); the batch is checked for
-    convergence and compaction between blocks."""
+    convergence and compaction between blocks.  autotune_reps > 0: the lanes-per-polygon
+    schedule of the per-record kernels is picked by pcp_autotune on the first context
+    (timed over that many launches per candidate; its time is included) and kept for the
+    compacted ones.  Every schedule gives the same results bit for bit."""
     P = len(packings)
     res = BatchResult(P)
     if boxSizes is None:
@@ -53,6 +58,10 @@ def minimize_batch(packings, model, fire_kwargs, maxSteps, dt0=1e-3, compact_bel
     while len(idx):
         s = capi.System(cur, [boxSizes[i] for i in idx], model=model)
         prm = s.fire_params(**fire_kwargs)
+        if autotune_reps > 0 and res.schedule is None:
+            res.schedule, res.schedule_us = s.autotune(autotune_reps)
+        elif res.schedule is not None:
+            s.set_schedule(*res.schedule)
         if warm is not None and state is None:
             s.force_eval()
             warm.apply(s, np.array([float(np.mean(p["targetEdgeV"])) for p in cur]))

@@ -32,3 +32,20 @@ def test_compaction_keeps_trajectories_bit_identical():
     np.testing.assert_array_equal(a.energy, b.energy)
     for x, y in zip(a.positions, b.positions):
         np.testing.assert_array_equal(x, y)
+
+
+def test_autotuned_batch_gives_the_same_trajectories():
+    """minimize_batch with the schedule picked by pcp_autotune on its first context (kept for
+    the compacted contexts): the same steps, energies and positions as the default schedule,
+    with squared pair areas."""
+    pk = make_batch(12, 23, N_range=(32, 160), n_range=(8, 20), min_center_frac=0.5)
+    kw = dict(fTol=1e-10, dtMax=100.0)
+    pair = capi.NORMAL | 0x400
+    a = batch.minimize_batch(pk, pair, kw, 1500, compact_below=0.75, min_context=1)
+    b = batch.minimize_batch(pk, pair, kw, 1500, compact_below=0.75, min_context=1, autotune_reps=3)
+    print(f"[batch] autotuned schedule {b.schedule}", flush=True)
+    assert b.schedule is not None and a.schedule is None and len(b.contexts) > 1
+    np.testing.assert_array_equal(a.steps, b.steps)
+    np.testing.assert_array_equal(a.energy, b.energy)
+    for x, y in zip(a.positions, b.positions):
+        np.testing.assert_array_equal(x, y)
