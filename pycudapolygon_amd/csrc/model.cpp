@@ -85,7 +85,7 @@ pcp_state Model::state() const {
     s.targetEdgeV = targetEdgeV; s.targetAreas = targetAreas;
     s.edgeLengths = edgeLengths; s.areas = areas; s.energy = energy;
     s.stiffness = stiffness; s.compressibility = compressibility;
-    s.model = (int)simControl.modelType;
+    s.model = (int)simControl.modelType | energyFlags;
     return s;
 }
 
@@ -157,6 +157,21 @@ void Model::setStartIndices(const vector<int>& s) {
 }
 
 void Model::setModelEnum(simControlStruct::modelEnum m) { simControl.modelType = m; }
+void Model::setOverlapEnergy(const std::string& name) {
+    if (name == "reference") energyFlags = 0;
+    else if (name == "area") energyFlags = PCP_AREA_ENERGY;
+    else if (name == "squaredLens") energyFlags = PCP_AREA_SQUARED;
+    else if (name == "squaredPair") energyFlags = PCP_PAIR_SQUARED;
+    else throw std::invalid_argument("setOverlapEnergy: expected reference, area, squaredLens or squaredPair, got " + name);
+}
+std::string Model::getOverlapEnergy() const {
+    switch (energyFlags) {
+        case PCP_AREA_ENERGY: return "area";
+        case PCP_AREA_SQUARED: return "squaredLens";
+        case PCP_PAIR_SQUARED: return "squaredPair";
+        default: return "reference";
+    }
+}
 
 // setMaxEdgeLength model.cpp:513-516: boxSize = floor(1/L); the device copy of L is
 // overwritten by updatePolygonGeometry, as in the reference
@@ -402,7 +417,7 @@ void Model::updateValidAndCounts() {
 void Model::updateCompactedIntersections() {
     ensureCells();
     int I = 0;
-    check(pcp_update_records(ctx, nullptr, &I));
+    check(pcp_update_records(ctx, energyFlags ? positions : nullptr, &I));   // exact pairing for the area energies
     numIntersections = I;
     viewKeys.assign(I, 0);
     viewTU.assign(2 * (size_t)I, 0.0);
@@ -414,7 +429,7 @@ void Model::updateCompactedIntersections() {
 void Model::updateOutersections() {
     ensureCells();
     int I = 0;
-    check(pcp_update_records(ctx, nullptr, &I));
+    check(pcp_update_records(ctx, energyFlags ? positions : nullptr, &I));   // exact pairing for the area energies
     numIntersections = I;
     compactedView = false;
     keysView = false;
@@ -437,7 +452,7 @@ void Model::updateForceEnergy() {
     bool overlap = simControl.modelType == simControlStruct::modelEnum::normal ||
                    simControl.modelType == simControlStruct::modelEnum::hybrid;
     if (overlap) ensureCells();
-    check(pcp_update_force_energy(ctx, (int)simControl.modelType, positions, edgeLengths, areas, targetEdgeV,
+    check(pcp_update_force_energy(ctx, (int)simControl.modelType | energyFlags, positions, edgeLengths, areas, targetEdgeV,
                                   targetAreas, stiffness, compressibility, force, energy));
 }
 // updatePositions model.cpp:397-399
@@ -486,9 +501,21 @@ std::tuple<double, double, int> Model::minimizeFIRE(double thr, double dtInit, i
                    simControl.modelType == simControlStruct::modelEnum::hybrid;
     if (overlap) ensureCells(); else ensureTopology();
     resetVelocities();
-    updatePolygonGeometry();
-    updateForceEnergy();
-    if (simControl.modelType == simControlStruct::modelEnum::normal) projectForce();
+    if (energyFlags && overlap) {
+        // the area energies need crossings with exact parameters: one full evaluation
+        pcp_state st0 = state();
+        int rc = PCP_OK;
+        for (int attempt = 0; attempt < 4; ++attempt) {   // the buckets grow on overflow: evaluate again
+            rc = pcp_force_eval(ctx, &st0);
+            if (rc == PCP_OK) rc = pcp_synchronize(ctx);
+            if (rc != PCP_ERR_CAPACITY) break;
+        }
+        check(rc);
+    } else {
+        updatePolygonGeometry();
+        updateForceEnergy();
+        if (simControl.modelType == simControlStruct::modelEnum::normal) projectForce();
+    }
     int nPos = 0;
     check(pcp_fire_set(ctx, &dtInit, &alphaStart, &nPos));
     pcp_fire_params prm{dtMax, alphaStart, fAlpha, fInc, fDec, nMin, shakeIter, thr};

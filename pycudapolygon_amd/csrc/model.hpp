@@ -34,6 +34,12 @@ public:
     void setForces(const std::vector<double>& forces_);
     void setStartIndices(const std::vector<int>& startIndices_);
     void setModelEnum(simControlStruct::modelEnum modelType_);
+    // overlap energy of the overlap models (no counterpart in the reference): "reference"
+    // (its convention, the default), "area" (overlap area, force = -gradient), "squaredLens"
+    // (sum over lenses of A^2/2) or "squaredPair" (sum over polygon pairs of A_pair^2/2);
+    // the last two are what FIRE can minimize to |F|max <= 1e-10
+    void setOverlapEnergy(const std::string& name);
+    std::string getOverlapEnergy() const;
     void setMaxEdgeLength(double maxEdgeLength);
     void setTargetEdgeLengths(const std::vector<double>& targetEdgeLengths_);
     void setTargetAreas(const std::vector<double>& targetAreas_);
@@ -122,6 +128,7 @@ private:
     template <class T> std::vector<T> d2h(const T* d, size_t n) const;
 
     simControlStruct simControl{simControlStruct::modelEnum::normal};
+    int energyFlags = 0;   // PCP_AREA_ENERGY / PCP_AREA_SQUARED / PCP_PAIR_SQUARED or 0
     mutable pcp_ctx* ctx = nullptr;
     mutable bool topoDirty = true, boxDirty = true;
     int size = 0, numPolygons = 0;

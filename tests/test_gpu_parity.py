@@ -425,3 +425,40 @@ def test_packing_without_overlaps():
         s.fire_init(prm, 1e-3)
         st = s.fire_minimize(prm, 10)[0]
         assert st.converged and st.steps == 1 and st.maxForce == 0.0
+
+
+def test_model_overlap_energy_selection():
+    """Model.setOverlapEnergy (added to the reference's Model API; no bound name removed):
+    with "squaredPair", minimizeFIRE through the reference-style calls reaches
+    max|F| <= 1e-10 in the same number of steps and at the same energy, bit for bit, as the
+    C-ABI FIRE of the same packing and cell grid; unknown names raise ValueError and
+    "reference" restores the reference's energy."""
+    from pycudapolygon_amd.pyCudaPolygonLink import libpyCudaPolygon as lpcp
+    p = make_packing(64, 12, 0.80, seed=0, min_center_frac=0.5)
+    B = box_size_for(p["diameter"])
+    m = lpcp.Model(0)
+    V = int(p["startIndices"][-1])
+    m.setNumVertices(V)
+    m.setStartIndices(p["startIndices"].tolist())
+    m.setPositions(p["positions"].tolist())
+    m.setMaxEdgeLength(1.0 / B)
+    m.setTargetEdgeLengthsPerEdge(p["targetEdgeV"].tolist())
+    m.setTargetAreas(p["targetAreas"].tolist())
+    m.setModelEnum(lpcp.modelEnum.normal)
+    m.initializeNeighborCells()
+    assert m.getOverlapEnergy() == "reference"
+    with pytest.raises(ValueError):
+        m.setOverlapEnergy("lens")
+    m.setOverlapEnergy("squaredPair")
+    assert m.getOverlapEnergy() == "squaredPair"
+    E, dt, steps = m.minimizeFIRE(1e-10, 1e-3, 5000, 100.0, 0.1, 0.99, 1.1, 0.5, 5, 5)
+    assert np.abs(np.array(m.getForces())).max() <= 1e-10 and steps < 5000
+    s = capi.System([p], [B], model=capi.NORMAL | PAIR)
+    prm = s.fire_params(fTol=1e-10, dtMax=100.0)
+    s.fire_init(prm, 1e-3)
+    st = s.fire_minimize(prm, 5000)[0]
+    print(f"[model] squaredPair: Model {steps} steps E {E!r}; C ABI {st.steps} steps E {st.energy!r}", flush=True)
+    assert st.converged and steps == st.steps and E == st.energy
+    np.testing.assert_array_equal(np.array(m.getPositions()), s.get_positions(0))
+    m.setOverlapEnergy("reference")
+    assert m.getOverlapEnergy() == "reference"
