@@ -39,12 +39,33 @@ constexpr int PP_BLK = 1024;
 // u_{k+1} = u_k + wrap(p_{k+1} - p_k) (the edge vectors the segment test uses);
 // centre = p_0 + mean u_k, radius = max |u_k - mean u|.  bounds[s] = (cx, cy, r, big),
 // big = 1 when the chain does not close (the polygon wraps the torus) or r is large.
-template <int SEG>
+//
+// With POST, FIRE's step after SHAKE (model.cpp:431-437) is done here first, where the
+// detection follows SHAKE (area energies): a packing whose SHAKE failed (first call at
+// shakeIter iterations and the retry at 10 * shakeIter) takes back its step-start
+// positions and zero velocities, the others re-derive their velocities from the SHAKE
+// displacement, v += wrap(x - xpre) / dt; the packing's first vertex records the outcome
+// in flags.  The bounds are those of the resulting positions (read from xbak after a
+// failure, so nothing written here is read back).
+struct PostShake {
+    int shakeIter, fnf, fFail, fLast;   // flags of packing p at fnf * p + fFail / fLast
+    const int* polyPack;
+    const int* packVert;
+    const int* active;
+    const double* dt;
+    const int* shakeIt;                 // [P] first call, [P + p] retry iterations
+    double2* x;
+    double2* v;
+    const double2* xbak;
+    const double2* xpre;
+    int* flags;
+};
+template <int SEG, bool POST = false>
 __global__ __launch_bounds__(BLK) void k_bounds(int N, int P, const int* __restrict__ polyStart,
-                                               const double2* __restrict__ pos, double4* __restrict__ bounds,
+                                               const double2* pos, double4* __restrict__ bounds,
                                                int* __restrict__ num, int* __restrict__ bcnt, int* __restrict__ jcnt,
                                                unsigned long long* __restrict__ packMaxKey, long long pmkPitch,
-                                               int* __restrict__ scal) {
+                                               int* __restrict__ scal, PostShake ps) {
     if (blockIdx.x == 0 && threadIdx.x < SC_NSCAL) scal[threadIdx.x] = 0;
     for (int q = blockIdx.x * BLK + threadIdx.x; q < PMK_SLOTS * P; q += gridDim.x * BLK)
         packMaxKey[(q / P) * pmkPitch + q % P] = 0ull;
@@ -55,6 +76,32 @@ __global__ __launch_bounds__(BLK) void k_bounds(int N, int P, const int* __restr
         st = polyStart[s];
         n = polyStart[s + 1] - st;
         if (sub == 0) { bcnt[s] = 0; jcnt[s] = 0; }
+    }
+    if constexpr (POST) {
+        if (s < N) {
+            int p = ps.polyPack[s];
+            if (ps.active[p]) {
+                int it1 = ps.shakeIt[p], it2 = ps.shakeIt[P + p];
+                bool retried = it1 >= ps.shakeIter;
+                bool fail = retried && it2 >= ps.shakeIter * 10;
+                double h = ps.dt[p];
+                for (int k = sub; k < n; k += SEG) {
+                    int i = st + k;
+                    if (fail) {
+                        ps.x[i] = ps.xbak[i];
+                        ps.v[i] = make_double2(0.0, 0.0);
+                    } else {
+                        double2 xi = pos[i], xp = ps.xpre[i], vi = ps.v[i];
+                        ps.v[i] = make_double2(vi.x + wrap(xi.x - xp.x) / h, vi.y + wrap(xi.y - xp.y) / h);
+                    }
+                }
+                if (fail) pos = ps.xbak;
+                if (sub == 0 && st == ps.packVert[p]) {
+                    ps.flags[ps.fnf * p + ps.fFail] = fail ? 1 : 0;
+                    ps.flags[ps.fnf * p + ps.fLast] = retried ? it2 : it1;
+                }
+            }
+        }
     }
     d2 p0 = (n > 0) ? ldpos(pos, st) : mk(0.0, 0.0);
     // pass 1: chain and its sum (chunks of SEG vertices, running carry)
@@ -842,26 +889,37 @@ int seg_width(int maxN) {
     return seg;
 }
 
-int launch_bounds(pcp_ctx* c, const double2* pos) {
+template <bool POST>
+static int launch_bounds_t(pcp_ctx* c, const double2* pos, const PostShake& ps) {
     int seg = c->seg;
     int polysPerBlock = BLK / seg;
     int grid = (c->N + polysPerBlock - 1) / polysPerBlock;
     if (grid < 1) grid = 1;
+#define PCP_BOUNDS(W)                                                                                          \
+    PCP_LAUNCH(c, (k_bounds<W, POST>), dim3(grid), dim3(BLK), c->N, c->P, c->dPolyStart, pos, c->dBounds,     \
+               c->dNum, c->dBcnt, c->dJcnt, c->dPackMaxKey, c->pmkPitch, c->dScal, ps)
     switch (seg) {
-    case 4: PCP_LAUNCH(c, k_bounds<4>, dim3(grid), dim3(BLK), c->N, c->P, c->dPolyStart, pos, c->dBounds, c->dNum, c->dBcnt, c->dJcnt, c->dPackMaxKey, c->pmkPitch, c->dScal); break;
-    case 8: PCP_LAUNCH(c, k_bounds<8>, dim3(grid), dim3(BLK), c->N, c->P, c->dPolyStart, pos, c->dBounds, c->dNum, c->dBcnt, c->dJcnt, c->dPackMaxKey, c->pmkPitch, c->dScal); break;
-    case 16: PCP_LAUNCH(c, k_bounds<16>, dim3(grid), dim3(BLK), c->N, c->P, c->dPolyStart, pos, c->dBounds, c->dNum, c->dBcnt, c->dJcnt, c->dPackMaxKey, c->pmkPitch, c->dScal); break;
-    case 32: PCP_LAUNCH(c, k_bounds<32>, dim3(grid), dim3(BLK), c->N, c->P, c->dPolyStart, pos, c->dBounds, c->dNum, c->dBcnt, c->dJcnt, c->dPackMaxKey, c->pmkPitch, c->dScal); break;
-    default: PCP_LAUNCH(c, k_bounds<64>, dim3(grid), dim3(BLK), c->N, c->P, c->dPolyStart, pos, c->dBounds, c->dNum, c->dBcnt, c->dJcnt, c->dPackMaxKey, c->pmkPitch, c->dScal); break;
+    case 4: PCP_BOUNDS(4); break;
+    case 8: PCP_BOUNDS(8); break;
+    case 16: PCP_BOUNDS(16); break;
+    case 32: PCP_BOUNDS(32); break;
+    default: PCP_BOUNDS(64); break;
     }
+#undef PCP_BOUNDS
     return PCP_OK;
 }
+int launch_bounds(pcp_ctx* c, const double2* pos, const PostShake* ps) {
+    if (ps) return launch_bounds_t<true>(c, pos, *ps);
+    PostShake none{};
+    return launch_bounds_t<false>(c, pos, none);
+}
 
-// cellsFromList: use the cells of the last run_cells (API updateNeighbors)
-int run_detect(pcp_ctx* c, const double2* pos, int cellsFromList) {
+// cellsFromList: use the cells of the last run_cells (API updateNeighbors).  ps: FIRE's
+// post-SHAKE step, done by the bounds kernel (see k_bounds)
+int run_detect(pcp_ctx* c, const double2* pos, int cellsFromList, const PostShake* ps) {
     if (c->V == 0) return PCP_OK;
     c->lastDetectPos = pos;
-    PCP_TRY(launch_bounds(c, pos));
+    PCP_TRY(launch_bounds(c, pos, ps));
     PCP_LAUNCH(c, k_poly_cells, dim3(c->P), dim3(PP_BLK), c->dPackPoly, c->dBounds, c->dPcStart, c->dPcSorted,
                c->dPackB, c->dSBounds);
     int grid = (c->N + BLK / 64 - 1) / (BLK / 64);

@@ -461,16 +461,25 @@ static int fire_eval(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm
     if (overlap) {
         if (!area)   // reference order (model.cpp:416-440): detect, SHAKE, records
             PCP_TRY(run_detect(c, x, 0));
-        if (rigid && prm->shakeIter > 0) {
+        bool shake = rigid && prm->shakeIter > 0;
+        if (shake)
             PCP_TRY(run_shake(c, x, st->targetEdgeV, st->targetAreas, prm->shakeIter, 1e-15,
                               10 * prm->shakeIter, c->dActive, c->dShakeIt));
-            PCP_LAUNCH(c, k_fire_post_shake, dim3(nblk(c->V)), dim3(BLK), c->V, c->P,
-                               prm->shakeIter, c->dShapeId, c->dPolyPack, c->dPackVert, c->dActive, c->dDt,
-                               c->dShakeIt, x, v, (const double2*)c->dXbak, (const double2*)c->dXpre,
-                               c->dFlags);
+        if (shake && area) {
+            // area energies: crossings of the final positions; the restore-or-rederive step
+            // after SHAKE is done by the detection's first kernel (k_bounds<POST>)
+            PostShake ps{prm->shakeIter, FL_NF, FL_FAIL, FL_LASTSHAKE, c->dPolyPack, c->dPackVert, c->dActive,
+                         c->dDt, c->dShakeIt, x, v, (const double2*)c->dXbak, (const double2*)c->dXpre, c->dFlags};
+            PCP_TRY(run_detect(c, x, 0, &ps));
+        } else {
+            if (shake)
+                PCP_LAUNCH(c, k_fire_post_shake, dim3(nblk(c->V)), dim3(BLK), c->V, c->P,
+                           prm->shakeIter, c->dShapeId, c->dPolyPack, c->dPackVert, c->dActive, c->dDt,
+                           c->dShakeIt, x, v, (const double2*)c->dXbak, (const double2*)c->dXpre,
+                           c->dFlags);
+            if (area)   // area energy: crossings of the final positions
+                PCP_TRY(run_detect(c, x, 0));
         }
-        if (area)   // area energy: crossings of the final positions
-            PCP_TRY(run_detect(c, x, 0));
         PCP_TRY(run_records(c, area ? x : nullptr));
     }
     PCP_TRY(run_forces(c, st->model, x, st->edgeLengths, st->areas, st->targetEdgeV, st->targetAreas,

@@ -163,7 +163,8 @@ int scan_exclusive(pcp_ctx* c, const int* in, int n, int* out);
 int launch_det_reduce(pcp_ctx* c, const double* partials, int nArrays, double* outSums, int nMax,
                       bool sticky = false);
 int run_cells(pcp_ctx* c, const double2* pos);
-int run_detect(pcp_ctx* c, const double2* pos, int cellsFromList);
+struct PostShake;   // FIRE's post-SHAKE step, done by the detection's bounds kernel (pcp_detect.hip)
+int run_detect(pcp_ctx* c, const double2* pos, int cellsFromList, const PostShake* ps = nullptr);
 int detect_alloc(pcp_ctx* c);
 int detect_stats(pcp_ctx* c, long long* polygonPairs, long long* edgePairs);
 int seg_width(int maxN);
