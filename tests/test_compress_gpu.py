@@ -125,3 +125,12 @@ def test_compression_ramp_with_shape_matching():
     assert len(steps) == 3
     for r in steps:
         assert all(r["converged"]) and max(r["max_force"]) <= 1e-10
+
+
+def test_shape_projection_rejects_polygons_beyond_64():
+    """pcp_shape_project takes polygons of at most 64 vertices and says so (PCP_ERR_ARG)."""
+    p = make_packing(4, 65, 0.5, seed=0)
+    s = capi.System([p], [2])
+    ref = capi.DeviceArray.from_numpy(compress.rest_shapes([p]))
+    with pytest.raises(capi.PcpError, match="64"):
+        s.ctx.call("pcp_shape_project", s.pos.ptr, ref.ptr, s.ta.ptr)
