@@ -65,14 +65,14 @@ def cpu_baseline(p, B, budget_s):
                        f"oracle/pcp_oracle.c with OpenMP ({threads} threads)")
 
 
-def fire_one(packings, dtMax, maxSteps, warm=None, budget_s=None, graph=True, dt0=1e-3):
+def fire_one(packings, dtMax, maxSteps, warm=None, budget_s=None, graph=True, dt0=1e-3, system=None):
     """FIRE (squared lens areas, rigid polygons, initial time step dt0) on the packings in one
     context until every packing has max|F| <= 1e-10, maxSteps, or budget_s seconds.  warm: a
-    WarmStart applied once after the first evaluation (its time is counted).  Returns a
-    result dict."""
+    WarmStart applied once after the first evaluation (its time is counted).  system: a
+    prepared (System, FireParams) to run instead.  Returns a result dict."""
     from pycudapolygon_amd import warmstart as W
     ell = W.packing_length_units(packings)
-    s, prm = W.fire_system(packings, dtMax=dtMax)
+    s, prm = system if system is not None else W.fire_system(packings, dtMax=dtMax)
     s.force_eval()
     s.sync()
     t0 = time.perf_counter()
@@ -112,7 +112,7 @@ def fire_legs(args):
     both).  Packings have no centre closer than (R_i + R_j)/2 to another (no polygon starts
     inside another, which no edge-crossing energy can see)."""
     from pycudapolygon_amd import warmstart as W
-    from pycudapolygon_amd.generate import make_packing, make_batch
+    from pycudapolygon_amd.generate import make_packing, make_batch, box_size_for
     ws = W.WarmStart.default()
     log = lambda r, tag: print(f"[bench] FIRE {tag}: {r['converged']}/{r['packings']} converged, "
                                f"{r['batch_steps']} steps, {r['seconds']:.3f} s, "
@@ -128,6 +128,20 @@ def fire_legs(args):
             break
         out[tag] = fire_one([p2], dt, steps, warm, budget_s=args.fire_seconds, graph=not args.no_graph, dt0=dt0)
         log(out[tag], tag)
+    if "xpbd" in legs:
+        # the N=4096 configuration with area and perimeter constraints of stiffness 1e3 integrated
+        # by XPBD (hybrid model: squared pair areas + the two springs; the springs by XPBD at the
+        # step's dt, 20 iterations, tol 1e-15; the overlap force explicitly)
+        from pycudapolygon_amd import capi
+        s = capi.System([p2], [box_size_for(p2["diameter"])], model=capi.HYBRID | W.PAIR_SQUARED, stiffness=0.0,
+                        compressibility=1e3, perimStiffness=1e3)
+        prm = s.fire_params(fTol=1e-10, dtMax=W.DT_MAX, shakeIter=0, xpbdIter=20, xpbdTol=1e-15)
+        r = fire_one([p2], W.DT_MAX, 20000, budget_s=args.fire_seconds, graph=not args.no_graph, system=(s, prm))
+        r["model"] = ("hybrid: squared pair overlap areas + area and perimeter springs of stiffness 1e3 "
+                      "integrated by XPBD (20 iterations, tol 1e-15); deformable polygons")
+        out["n4096_xpbd"] = r
+        log(r, "n4096_xpbd")
+        s.close()
     from pycudapolygon_amd import batch
     if args.n4096_batch > 0 and "n4096_batch" in legs:
         # P packings of the N=4096 workload in one context (the first is the packing of the
@@ -297,7 +311,7 @@ def main():
                     help="FIRE legs launch each block's kernels directly instead of replaying a HIP graph "
                          "(for profilers that cannot trace graph launches)")
     ap.add_argument("--no-fire", action="store_true")
-    ap.add_argument("--fire-legs", default="n4096,n4096_batch,n1024,uneven,compression",
+    ap.add_argument("--fire-legs", default="n4096,xpbd,n4096_batch,n1024,uneven,compression",
                     help="comma-separated FIRE legs to run (for profiling one of them)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="N",
                     help="only run the plain copy kernel on N doubles (10 launches) for HBM counter "

@@ -140,10 +140,36 @@ int pcp_project_force(pcp_ctx* ctx, const double* pos, double* force);
 int pcp_shake(pcp_ctx* ctx, double* pos, const double* targetEdgeV, const double* targetAreas,
               int maxIter, double tol, int* iters);
 
-/* ---- XPBD: xpbdProjectCUDA model.cu:209-278.  targetEdgeP device[N] per polygon.
- *      Returns the iteration count in iters host[1]. ---- */
+/* ---- XPBD: xpbdProjectCUDA model.cu:209-278 (kernels.cuh:532-623): per iteration the edge
+ *      projections of colour 0 then colour 1, then the area correction, with the reference's
+ *      arithmetic; tol > 0 stops a polygon once its edge deviations (after the iteration) and
+ *      area deviation (before the correction) are below tol.  targetEdgeP device[N] per
+ *      polygon.  iters host[P] (optional; synchronizes): per packing the largest iteration
+ *      index at which a polygon stopped (nIter when tol <= 0 or one did not), which is the
+ *      reference's returned count.  Deviations from the reference: one lane segment per
+ *      polygon, so edges of one colour never race (the reference races for odd n; here the
+ *      last edge of an odd polygon follows the other even edges) and the area correction
+ *      reads consistent positions; convergence is decided per polygon; polygons of at most
+ *      64 vertices (PCP_ERR_ARG otherwise), fewer than 3 are left as they are. ---- */
 int pcp_xpbd(pcp_ctx* ctx, double* pos, const double* targetEdgeP, const double* targetAreas,
              int nIter, double tol, int* iters);
+/* XPBD with compliance (an extension of xpbdProjectCUDA; Macklin, Mueller & Chentanez,
+ * MIG 2016): every constraint j keeps a multiplier lambda_j (zero at the call's start) and
+ * each projection applies dlambda = -(C_j + at_j lambda_j) / (sum |grad C_j|^2 + at_j),
+ * x += grad C_j dlambda, with at_j = 1/(k_j dt^2).  Constraints per polygon: edge lengths
+ * (targetEdgeV device[V]), area (targetAreas device[N]) and perimeter (target: the sum of the
+ * polygon's edge targets); each type with its stiffness k (0 = off, INFINITY = hard: the
+ * reference's projection).  The area and perimeter constraints of a polygon are solved
+ * together (their 2x2 block) when both are on.  Same per-polygon stopping rule (residual
+ * |C + at lambda|), iteration counts and limits as pcp_xpbd. */
+typedef struct {
+    double edgeStiffness, areaStiffness, perimStiffness;
+    double dt;        /* time step of the compliance 1/(k dt^2); > 0 */
+    int nIter;
+    double tol;       /* <= 0: run all nIter iterations */
+} pcp_xpbd_params;
+int pcp_xpbd_ex(pcp_ctx* ctx, double* pos, const double* targetEdgeV, const double* targetAreas,
+                const pcp_xpbd_params* prm, int* iters);
 
 /* ---- reductions / small updates ---- */
 /* getMaxUnbalancedForceCUDA model.cu:612-614: max |component| per packing, out device[P] */
@@ -208,6 +234,10 @@ typedef struct {
     double* energy;             /* device [P]  */
     double stiffness, compressibility;
     int model;
+    /* perimeter spring of the softBody and hybrid models (no counterpart in the reference):
+     * E += perimStiffness (P - P0)^2 / 2 per polygon, P0 = the sum of its edge targets;
+     * 0 = none */
+    double perimStiffness;
 } pcp_state;
 int pcp_force_eval(pcp_ctx* ctx, const pcp_state* st);
 
@@ -218,6 +248,16 @@ typedef struct {
     double dtMax, alphaStart, fAlpha, fInc, fDec;
     int nMin, shakeIter;
     double fTol;        /* convergence: max |force component| <= fTol */
+    /* xpbdIter > 0 (no counterpart in the reference, whose XPBD launcher is never called):
+     * the springs of a model that has them (edge: stiffness, area: compressibility,
+     * perimeter: perimStiffness) are integrated implicitly by XPBD (pcp_xpbd_ex with the
+     * packing's dt, xpbdIter iterations, tolerance xpbdTol) and the rest of the force (the
+     * overlap force) explicitly, XPBD's own time step inside FIRE:
+     *   v' = v + dt Fext;  x~ = frac(x + dt v');  x = XPBD(x~), wrapped;  v = v' + wrap(x - x~)/dt
+     * then the energy test and mixing on the total energy and force (no second half-kick).
+     * status.lastShakeIters reports the step's XPBD iteration count.  0: velocity Verlet. */
+    int xpbdIter;
+    double xpbdTol;
 } pcp_fire_params;
 typedef struct {
     double dt, alpha, energy, maxForce;

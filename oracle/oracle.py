@@ -42,7 +42,7 @@ _p = ctypes.c_void_p
 class _System(ctypes.Structure):
     _fields_ = [("V", _i), ("N", _i), ("boxSize", _i), ("model", _i), ("maxNeighbors", _i),
                 ("start", _p), ("targetEdgeV", _p), ("targetAreas", _p),
-                ("stiffness", _d), ("compressibility", _d), ("solver", _i)]
+                ("stiffness", _d), ("compressibility", _d), ("solver", _i), ("perimStiffness", _d)]
 
 SOLVERS = {"dense": 0, "lanes": 1}
 
@@ -75,6 +75,18 @@ def _declare(L):
     L.orc_shape_project.argtypes = [_i, _i, _p, _p, _p, _p]
     L.orc_xpbd.restype = _i
     L.orc_xpbd.argtypes = [_i, _i, _p, _p, _p, _p, _i, _d]
+    L.orc_xpbd_lanes.restype = _i
+    L.orc_xpbd_lanes.argtypes = [_i, _i, _p, _p, _p, _p, _p, _d, _d, _d, _d, _i, _d]
+    L.orc_force_energy_ex.restype = _d
+    L.orc_force_energy_ex.argtypes = [_i, _i, _i, _p, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _d, _d, _d,
+                                      _p, _p]
+    L.orc_force_eval_ex.restype = _d
+    L.orc_force_eval_ex.argtypes = [ctypes.POINTER(_System), _p, _p, _p, _p]
+    L.orc_fire_step_ex.argtypes = [ctypes.POINTER(_System), _p, _p, _p, ctypes.POINTER(FireState),
+                                   _d, _d, _d, _d, _d, _i, _i, _i, _d]
+    L.orc_fire_minimize_ex.restype = _i
+    L.orc_fire_minimize_ex.argtypes = [ctypes.POINTER(_System), _p, _p, _p, ctypes.POINTER(FireState),
+                                       _d, _i, _d, _d, _d, _d, _d, _i, _i, _i, _d]
     L.orc_det_sum.restype = _d
     L.orc_det_sum.argtypes = [_p, _i]
     L.orc_overlap_count.restype = ctypes.c_longlong
@@ -173,7 +185,8 @@ def per_vertex_targets(start, targetEdgeLengths):
 
 
 def force_energy(model, pos, start, rec, geom, targetEdgeV=None, targetAreas=None,
-                 stiffness=0.0, compressibility=0.0):
+                 stiffness=0.0, compressibility=0.0, perimStiffness=0.0, want_fext=False):
+    """orc_force_energy_ex: (E, force) or, with want_fext, (E, force, fext)."""
     pos = _f64(pos); start = _i32(start)
     N = len(start) - 1; V = int(start[-1])
     mid = MODEL_IDS[model] if isinstance(model, str) else int(model)
@@ -185,11 +198,12 @@ def force_energy(model, pos, start, rec, geom, targetEdgeV=None, targetAreas=Non
     outer = _as_u64(rec["outersections"]) if rec is not None else np.zeros(1, np.uint64)
     tu = _f64(rec["tu"]) if rec is not None else np.zeros(2)
     ut = _f64(rec["ut"]) if rec is not None else np.zeros(2)
-    E = lib().orc_force_energy(mid, V, N, _ptr(start), _ptr(pos), I, _ptr(inter), _ptr(outer),
-                               _ptr(tu), _ptr(ut), _ptr(_f64(geom["edgeLengths"])),
-                               _ptr(_f64(geom["areas"])), _ptr(tev), _ptr(ta), float(stiffness),
-                               float(compressibility), _ptr(force))
-    return E, force
+    fext = np.zeros(2 * V) if want_fext else None
+    E = lib().orc_force_energy_ex(mid, V, N, _ptr(start), _ptr(pos), I, _ptr(inter), _ptr(outer),
+                                  _ptr(tu), _ptr(ut), _ptr(_f64(geom["edgeLengths"])),
+                                  _ptr(_f64(geom["areas"])), _ptr(tev), _ptr(ta), float(stiffness),
+                                  float(compressibility), float(perimStiffness), _ptr(force), _ptr(fext))
+    return (E, force, fext) if want_fext else (E, force)
 
 
 def _as_u64(a):
@@ -242,6 +256,21 @@ def xpbd(pos, start, targetEdgeLengths, targetAreas, nIter, tol=0.0):
     return pos, it
 
 
+def xpbd_lanes(pos, start, kE, kA, kP, dt, nIter, tol=0.0, targetEdgeV=None, targetEdgeP=None,
+               targetAreas=None):
+    """Compliant XPBD in the kernel's lane order: (positions, iteration count)."""
+    pos = _f64(pos).copy(); start = _i32(start)
+    N = len(start) - 1
+    tev = _f64(targetEdgeV) if targetEdgeV is not None else None
+    tep = _f64(targetEdgeP) if targetEdgeP is not None else None
+    ta = _f64(targetAreas) if targetAreas is not None else np.zeros(N)
+    it = lib().orc_xpbd_lanes(int(start[-1]), N, _ptr(start), _ptr(pos), _ptr(tev), _ptr(tep), _ptr(ta),
+                              float(kE), float(kA), float(kP), float(dt), int(nIter), float(tol))
+    if it < 0:
+        raise ValueError("XPBD takes polygons of at most 64 vertices")
+    return pos, it
+
+
 def det_sum(x):
     x = _f64(x)
     return lib().orc_det_sum(_ptr(x), len(x))
@@ -260,7 +289,7 @@ class System:
     """Holds the arrays behind an orc_system so they outlive the ctypes struct."""
 
     def __init__(self, start, boxSize, model="normal", targetEdgeV=None, targetAreas=None,
-                 stiffness=0.0, compressibility=0.0, maxNeighbors=100, solver="dense"):
+                 stiffness=0.0, compressibility=0.0, maxNeighbors=100, solver="dense", perimStiffness=0.0):
         """solver: "dense" (Householder projection, dense Gauss-Newton SHAKE) or "lanes"
         (the same solves in the HIP kernels' lane order, bit-identical to the library)."""
         self.start = _i32(start)
@@ -269,7 +298,7 @@ class System:
         self.ta = _f64(targetAreas if targetAreas is not None else np.zeros(N))
         self.s = _System(V, N, int(boxSize), MODEL_IDS[model] if isinstance(model, str) else int(model),
                          int(maxNeighbors), _ptr(self.start), _ptr(self.tev), _ptr(self.ta),
-                         float(stiffness), float(compressibility), SOLVERS[solver])
+                         float(stiffness), float(compressibility), SOLVERS[solver], float(perimStiffness))
 
     @property
     def V(self):
@@ -280,6 +309,12 @@ class System:
         E = lib().orc_force_eval(ctypes.byref(self.s), _ptr(pos), _ptr(f), ctypes.byref(nI))
         return E, f, nI.value
 
+    def force_eval_ext(self, pos):
+        """(E, total force, force before the springs, records)"""
+        pos = _f64(pos); f = np.zeros(2 * self.V); fe = np.zeros(2 * self.V); nI = ctypes.c_int(0)
+        E = lib().orc_force_eval_ex(ctypes.byref(self.s), _ptr(pos), _ptr(f), _ptr(fe), ctypes.byref(nI))
+        return E, f, fe, nI.value
+
     def fire_init(self, pos, dt, alpha):
         st = FireState(dt, alpha, 0, 0.0, 0, 0, 0)
         vel = np.zeros(2 * self.V); f = np.zeros(2 * self.V)
@@ -287,15 +322,16 @@ class System:
         return st, vel, f
 
     def fire_step(self, pos, vel, force, st, dtMax=0.1, alphaStart=0.1, fAlpha=0.99, fInc=1.1,
-                  fDec=0.5, nMin=5, shakeIter=5):
-        lib().orc_fire_step(ctypes.byref(self.s), _ptr(pos), _ptr(vel), _ptr(force), ctypes.byref(st),
-                            dtMax, alphaStart, fAlpha, fInc, fDec, nMin, shakeIter)
+                  fDec=0.5, nMin=5, shakeIter=5, xpbdIter=0, xpbdTol=0.0):
+        lib().orc_fire_step_ex(ctypes.byref(self.s), _ptr(pos), _ptr(vel), _ptr(force), ctypes.byref(st),
+                               dtMax, alphaStart, fAlpha, fInc, fDec, nMin, shakeIter, int(xpbdIter),
+                               float(xpbdTol))
 
     def fire_minimize(self, pos, fTol=1e-10, dt=1e-3, maxSteps=100000, dtMax=0.1, alphaStart=0.1,
-                      fAlpha=0.99, fInc=1.1, fDec=0.5, nMin=5, shakeIter=5):
+                      fAlpha=0.99, fInc=1.1, fDec=0.5, nMin=5, shakeIter=5, xpbdIter=0, xpbdTol=0.0):
         pos = _f64(pos).copy(); vel = np.zeros(2 * self.V); f = np.zeros(2 * self.V)
         st = FireState(dt, alphaStart, 0, 0.0, 0, 0, 0)
-        steps = lib().orc_fire_minimize(ctypes.byref(self.s), _ptr(pos), _ptr(vel), _ptr(f),
-                                        ctypes.byref(st), fTol, maxSteps, dtMax, alphaStart, fAlpha,
-                                        fInc, fDec, nMin, shakeIter)
+        steps = lib().orc_fire_minimize_ex(ctypes.byref(self.s), _ptr(pos), _ptr(vel), _ptr(f),
+                                           ctypes.byref(st), fTol, maxSteps, dtMax, alphaStart, fAlpha,
+                                           fInc, fDec, nMin, shakeIter, int(xpbdIter), float(xpbdTol))
         return pos, f, st, steps

@@ -590,6 +590,31 @@ double orc_force_energy(int model, int V, int N, const int* start, const double*
                         const double* tu, const double* ut, const double* edgeLengths,
                         const double* areas, const double* targetEdgeV, const double* targetAreas,
                         double stiffness, double compressibility, double* force) {
+    return orc_force_energy_ex(model, V, N, start, pos, I, intersections, outersections, tu, ut,
+                               edgeLengths, areas, targetEdgeV, targetAreas, stiffness, compressibility,
+                               0.0, force, NULL);
+}
+
+/* perimeter of polygon s and its target (sum of the edge targets), both summed in
+ * vertex order (no counterpart in the reference: the perimeter spring below) */
+static void perimeter_and_target(int s, const int* start, const double* edgeLengths,
+                                 const double* targetEdgeV, double* P, double* P0) {
+    double a = 0.0, b = 0.0;
+    for (int m = start[s]; m < start[s + 1]; ++m) { a += edgeLengths[m]; b += targetEdgeV[m]; }
+    *P = a; *P0 = b;
+}
+
+/* As orc_force_energy, plus (no counterpart in the reference) a perimeter spring
+ * E = kP (P - P0)^2 / 2 per polygon of the softBody and hybrid models (P0 = sum of the
+ * polygon's edge targets; force -kP (P - P0) grad P, grad_m P = t_{m-1} - t_m with t the
+ * unit edge vectors), added after the area spring; and fext (optional, [2V]): each
+ * vertex's force before any spring term (the overlap force; zero without overlap). */
+double orc_force_energy_ex(int model, int V, int N, const int* start, const double* pos,
+                           int I, const uint64_t* intersections, const uint64_t* outersections,
+                           const double* tu, const double* ut, const double* edgeLengths,
+                           const double* areas, const double* targetEdgeV, const double* targetAreas,
+                           double stiffness, double compressibility, double perimStiffness,
+                           double* force, double* fext) {
     int* shapeId = (int*)malloc(sizeof(int) * (V > 0 ? V : 1));
     int* next = (int*)malloc(sizeof(int) * (V > 0 ? V : 1));
     int* prev = (int*)malloc(sizeof(int) * (V > 0 ? V : 1));
@@ -601,6 +626,7 @@ double orc_force_energy(int model, int V, int N, const int* start, const double*
     int overlap = (model == ORC_NORMAL || model == ORC_HYBRID);
     int edgeOn = (model == ORC_SOFT_BODY || model == ORC_EDGE_ONLY || model == ORC_HYBRID);
     int areaOn = (model == ORC_SOFT_BODY || model == ORC_AREA_ONLY || model == ORC_HYBRID);
+    int perimOn = (model == ORC_SOFT_BODY || model == ORC_HYBRID) && perimStiffness != 0.0;
     if (!overlap) I = 0;
     int Ialloc = I > 0 ? I : 1;
     int* tgt = (int*)malloc(sizeof(int) * 8 * Ialloc);
@@ -785,6 +811,7 @@ double orc_force_energy(int model, int V, int N, const int* start, const double*
             int pm = prev[m];
             if (LN[pm].x != 0.0 || LN[pm].y != 0.0) { F.x += LN[pm].x; F.y += LN[pm].y; }
         }
+        if (fext) { fext[2 * m] = F.x; fext[2 * m + 1] = F.y; }
         if (edgeOn) {
             int prv = prev[m], nxt = next[m];
             double l0 = targetEdgeV[m], l0prv = targetEdgeV[prv];
@@ -812,6 +839,19 @@ double orc_force_energy(int model, int V, int N, const int* start, const double*
                 F.x += fx; F.y += fy;
             }
         }
+        if (perimOn) {
+            int prv = prev[m], nxt = next[m];
+            double P, P0;
+            perimeter_and_target(s, start, edgeLengths, targetEdgeV, &P, &P0);
+            double l = edgeLengths[m], lp = edgeLengths[prv];
+            double dx = orc_wrap(pos[2 * nxt] - pos[2 * m]), dy = orc_wrap(pos[2 * nxt + 1] - pos[2 * m + 1]);
+            double px = orc_wrap(pos[2 * m] - pos[2 * prv]), py = orc_wrap(pos[2 * m + 1] - pos[2 * prv + 1]);
+            double tx = (l > 1e-14) ? dx / l : 0.0, ty = (l > 1e-14) ? dy / l : 0.0;
+            double qx = (lp > 1e-14) ? px / lp : 0.0, qy = (lp > 1e-14) ? py / lp : 0.0;
+            double coeff = perimStiffness * (P - P0);
+            F.x += -coeff * (qx - tx);
+            F.y += -coeff * (qy - ty);
+        }
         force[2 * m] = F.x; force[2 * m + 1] = F.y;
     }
     /* energies per polygon, then tree over polygons */
@@ -836,6 +876,11 @@ double orc_force_energy(int model, int V, int N, const int* start, const double*
         if (areaOn) {
             double A = areas[s], A0 = targetAreas[s];
             if (!(A0 < 1e-14)) e += compressibility * (A - A0) * (A - A0) / 2.0;
+        }
+        if (perimOn) {
+            double P, P0;
+            perimeter_and_target(s, start, edgeLengths, targetEdgeV, &P, &P0);
+            e += perimStiffness * (P - P0) * (P - P0) / 2.0;
         }
         ep[s] = e;
     }
@@ -1355,6 +1400,172 @@ int orc_xpbd(int V, int N, const int* start, double* pos, const double* targetEd
     return iter;
 }
 
+/* ---------- XPBD with compliance, in the HIP kernel's lane order ----------
+ * Extends xpbdProjectCUDA model.cu:209-278 / kernels.cuh:532-623 (no compliance, edges +
+ * area) to compliant constraints (Macklin, Mueller & Chentanez, "XPBD: position-based
+ * simulation of compliant constrained dynamics", MIG 2016): every constraint j keeps a
+ * multiplier lambda_j, zero at the start of the call, and a projection of j applies
+ *     dlambda = (-C_j - at_j lambda_j) / (sum_k w_k |grad_k C_j|^2 + at_j),
+ *     lambda_j += dlambda,  x_k += w_k grad_k C_j dlambda        (unit inverse masses)
+ * with at_j = 1 / (k_j dt^2) for stiffness k_j (k = INFINITY: at = 0, the reference's
+ * hard projection; k = 0: the constraint is off).  Constraints per polygon: its n edge
+ * lengths (target per vertex, or one per polygon), its area (shoelace relative to the
+ * start vertex, kernels.cuh:567-577) and its perimeter (target: the sum of the edge
+ * targets in vertex order).  One iteration: edges of colour 0 (even index; for odd n the
+ * last edge after the others, as it shares vertex 0 with edge 0), colour 1, then the area,
+ * then the perimeter.  With at = 0 each edge and area step is the reference's arithmetic
+ * (c = 0.5 (len - L)/len, alpha = (A - A0)/max(|grad A|^2, 1e-24)).  Sums over a polygon
+ * are butterfly sums (lanes_sum).  Convergence (tol > 0) is per polygon: after an
+ * iteration the largest |C + at lambda| of the edges (at the positions after the
+ * iteration), of the area and of the perimeter (each before its own correction) is
+ * compared with tol, as the reference's deviation kernels (kernels.cuh:603-623) do with
+ * at = 0; the returned count is the largest index at which a polygon stopped (nIter if
+ * one did not).  Positions are not wrapped into the box (the reference does not either).
+ * Polygons of at most 64 vertices (-1 otherwise); fewer than 3 are left as they are. */
+int orc_xpbd_lanes(int V, int N, const int* start, double* pos, const double* targetEdgeV,
+                   const double* targetEdgeP, const double* targetAreas, double kE, double kA,
+                   double kP, double dt, int nIter, double tol) {
+    (void)V;
+    for (int s = 0; s < N; ++s)
+        if (start[s + 1] - start[s] > LANES_MAX) return -1;
+    int edgeOn = kE != 0.0, areaOn = kA != 0.0, perimOn = kP != 0.0;
+    double atE = edgeOn ? 1.0 / (kE * (dt * dt)) : 0.0;
+    double atA = areaOn ? 1.0 / (kA * (dt * dt)) : 0.0;
+    double atP = perimOn ? 1.0 / (kP * (dt * dt)) : 0.0;
+    int worst = 0;
+    for (int s = 0; s < N; ++s) {
+        int st = start[s], n = start[s + 1] - st;
+        if (n < 3) continue;
+        double x[LANES_MAX], y[LANES_MAX], L[LANES_MAX], lam[LANES_MAX];
+        double t1[LANES_MAX], t2[LANES_MAX];
+        double P0 = 0.0;
+        for (int e = 0; e < n; ++e) {
+            x[e] = pos[2 * (st + e)]; y[e] = pos[2 * (st + e) + 1];
+            L[e] = targetEdgeV ? targetEdgeV[st + e] : (targetEdgeP ? targetEdgeP[s] : 0.0);
+            lam[e] = 0.0;
+        }
+        for (int e = 0; e < n; ++e) P0 += L[e];
+        double A0 = areaOn ? targetAreas[s] : 0.0;
+        double lamA = 0.0, lamP = 0.0;
+        int it = 0;
+        for (; it < nIter; ++it) {
+            if (edgeOn) {
+                int nPass = (n & 1) ? 3 : 2;
+                for (int pass = 0; pass < nPass; ++pass) {
+                    double dx[LANES_MAX], dy[LANES_MAX];
+                    int act[LANES_MAX];
+                    for (int e = 0; e < n; ++e) {
+                        int en = (e + 1 == n) ? 0 : e + 1;
+                        int last = (n & 1) && e == n - 1;
+                        act[e] = (pass == 0) ? (!(e & 1) && !last) : (pass == 1 && nPass == 3) ? last : (e & 1);
+                        dx[e] = 0.0; dy[e] = 0.0;
+                        if (!act[e]) continue;
+                        double ex = orc_wrap(x[en] - x[e]), ey = orc_wrap(y[en] - y[e]);
+                        double len = sqrt(ex * ex + ey * ey);
+                        if (len < 1e-14) { act[e] = 0; continue; }
+                        double C = len - L[e];
+                        double dl = (-C - atE * lam[e]) / (2.0 + atE);
+                        lam[e] += dl;
+                        double c = -dl / len;
+                        dx[e] = c * ex; dy[e] = c * ey;
+                    }
+                    double nx[LANES_MAX], ny[LANES_MAX];
+                    for (int e = 0; e < n; ++e) {
+                        int ep = (e == 0) ? n - 1 : e - 1;
+                        nx[e] = x[e]; ny[e] = y[e];
+                        if (act[e]) { nx[e] = x[e] + dx[e]; ny[e] = y[e] + dy[e]; }
+                        if (act[ep]) { nx[e] = x[e] - dx[ep]; ny[e] = y[e] - dy[ep]; }
+                    }
+                    for (int e = 0; e < n; ++e) { x[e] = nx[e]; y[e] = ny[e]; }
+                }
+            }
+            double devA = 0.0, devP = 0.0, devE = 0.0;
+            double gAx[LANES_MAX], gAy[LANES_MAX], gPx[LANES_MAX], gPy[LANES_MAX];
+            double rA = 0.0, rP = 0.0, aa = 0.0, pp = 0.0, ap = 0.0;
+            if (areaOn) {
+                double xs = x[0], ys = y[0];
+                double term[LANES_MAX], g2v[LANES_MAX];
+                for (int e = 0; e < n; ++e) {
+                    int en = (e + 1 == n) ? 0 : e + 1, ep = (e == 0) ? n - 1 : e - 1;
+                    double xk = orc_wrap(x[e] - xs), yk = orc_wrap(y[e] - ys);
+                    double xnk = orc_wrap(x[en] - xs), ynk = orc_wrap(y[en] - ys);
+                    double xpk = orc_wrap(x[ep] - xs), ypk = orc_wrap(y[ep] - ys);
+                    term[e] = 0.5 * (xk * ynk - xnk * yk);
+                    gAx[e] = 0.5 * (ynk - ypk);
+                    gAy[e] = 0.5 * (xpk - xnk);
+                    g2v[e] = gAx[e] * gAx[e] + gAy[e] * gAy[e];
+                }
+                double area = lanes_sum(term, n);
+                aa = lanes_sum(g2v, n);
+                rA = (area - A0) + atA * lamA;
+                devA = fabs(rA);
+            }
+            if (perimOn) {
+                double len[LANES_MAX], gp2v[LANES_MAX];
+                for (int e = 0; e < n; ++e) {
+                    int en = (e + 1 == n) ? 0 : e + 1;
+                    double ex = orc_wrap(x[en] - x[e]), ey = orc_wrap(y[en] - y[e]);
+                    len[e] = sqrt(ex * ex + ey * ey);
+                    t1[e] = (len[e] >= 1e-14) ? ex / len[e] : 0.0;
+                    t2[e] = (len[e] >= 1e-14) ? ey / len[e] : 0.0;
+                }
+                for (int e = 0; e < n; ++e) {
+                    int ep = (e == 0) ? n - 1 : e - 1;
+                    gPx[e] = t1[ep] - t1[e];
+                    gPy[e] = t2[ep] - t2[e];
+                    gp2v[e] = gPx[e] * gPx[e] + gPy[e] * gPy[e];
+                }
+                double P = lanes_sum(len, n);
+                pp = lanes_sum(gp2v, n);
+                rP = (P - P0) + atP * lamP;
+                devP = fabs(rP);
+            }
+            if (areaOn && perimOn) {   /* the 2x2 block of the two polygon-wide constraints */
+                double apv[LANES_MAX];
+                for (int e = 0; e < n; ++e) apv[e] = gAx[e] * gPx[e] + gAy[e] * gPy[e];
+                ap = lanes_sum(apv, n);
+                double m11 = aa + atA, m22 = pp + atP;
+                double det = m11 * m22 - ap * ap;
+                if (det > 1e-12 * (m11 * m22)) {
+                    double dA = (ap * rP - m22 * rA) / det;
+                    double dP = (ap * rA - m11 * rP) / det;
+                    lamA += dA;
+                    lamP += dP;
+                    for (int e = 0; e < n; ++e) {
+                        x[e] += gAx[e] * dA + gPx[e] * dP;
+                        y[e] += gAy[e] * dA + gPy[e] * dP;
+                    }
+                } else {   /* (nearly) parallel gradients: area step only */
+                    double alpha = rA / (fmax(aa, 1e-24) + atA);
+                    lamA -= alpha;
+                    for (int e = 0; e < n; ++e) { x[e] -= alpha * gAx[e]; y[e] -= alpha * gAy[e]; }
+                }
+            } else if (areaOn) {
+                double alpha = rA / (fmax(aa, 1e-24) + atA);
+                lamA -= alpha;
+                for (int e = 0; e < n; ++e) { x[e] -= alpha * gAx[e]; y[e] -= alpha * gAy[e]; }
+            } else if (perimOn) {
+                double alpha = rP / (fmax(pp, 1e-24) + atP);
+                lamP -= alpha;
+                for (int e = 0; e < n; ++e) { x[e] -= alpha * gPx[e]; y[e] -= alpha * gPy[e]; }
+            }
+            if (tol > 0) {
+                if (edgeOn)
+                    for (int e = 0; e < n; ++e) {
+                        int en = (e + 1 == n) ? 0 : e + 1;
+                        double ex = orc_wrap(x[en] - x[e]), ey = orc_wrap(y[en] - y[e]);
+                        double d = fabs(sqrt(ex * ex + ey * ey) - L[e] + atE * lam[e]);
+                        if (d > devE) devE = d;
+                    }
+                if ((!edgeOn || devE < tol) && (!areaOn || devA < tol) && (!perimOn || devP < tol)) break;
+            }
+        }
+        for (int e = 0; e < n; ++e) { pos[2 * (st + e)] = x[e]; pos[2 * (st + e) + 1] = y[e]; }
+        if (it > worst) worst = it;
+    }
+    return worst;
+}
+
 /* ---------- Monte-Carlo overlap counter: updateOverlapAreaKernel kernels.cuh:879-979 ---------- */
 static double wrapPeriodic(double x) {   /* kernels.cuh:108-113 */
     x += 1.5;
@@ -1478,15 +1689,18 @@ static void geometry(const orc_system* s, const double* pos, orc_ws* w) {
     orc_geometry(s->V, s->N, pos, s->start, w->edgeLengths, w->areas, w->comX, w->comY,
                  w->constraints, NULL, NULL);
 }
-static double forces(const orc_system* s, const double* pos, orc_ws* w, double* force) {
-    double E = orc_force_energy(s->model, s->V, s->N, s->start, pos, w->I, w->inter, w->outer,
-                                w->tu, w->ut, w->edgeLengths, w->areas, s->targetEdgeV, s->targetAreas,
-                                s->stiffness, s->compressibility, force);
+static double forces_ex(const orc_system* s, const double* pos, orc_ws* w, double* force, double* fext) {
+    double E = orc_force_energy_ex(s->model, s->V, s->N, s->start, pos, w->I, w->inter, w->outer,
+                                   w->tu, w->ut, w->edgeLengths, w->areas, s->targetEdgeV, s->targetAreas,
+                                   s->stiffness, s->compressibility, s->perimStiffness, force, fext);
     if ((s->model & 0xFF) == ORC_NORMAL) {
         if (s->solver == ORC_SOLVER_LANES) orc_project_force_lanes(s->V, s->N, s->start, pos, force);
         else orc_project_force(s->V, s->N, s->start, pos, force);
     }
     return E;
+}
+static double forces(const orc_system* s, const double* pos, orc_ws* w, double* force) {
+    return forces_ex(s, pos, w, force, NULL);
 }
 static int fire_shake(const orc_system* s, double* pos, int maxIter) {
     if (s->solver == ORC_SOLVER_LANES)
@@ -1495,12 +1709,17 @@ static int fire_shake(const orc_system* s, double* pos, int maxIter) {
 }
 
 double orc_force_eval(const orc_system* s, const double* pos, double* force, int* numIntersections) {
+    return orc_force_eval_ex(s, pos, force, NULL, numIntersections);
+}
+/* as orc_force_eval, with fext [2V] (optional): the force before the spring terms */
+double orc_force_eval_ex(const orc_system* s, const double* pos, double* force, double* fext,
+                         int* numIntersections) {
     orc_ws w;
     ws_alloc(&w, s->V, s->N, s->boxSize, s->maxNeighbors > 0 ? s->maxNeighbors : 100);
     geometry(s, pos, &w);
     int overlap = ((s->model & 0xFF) == ORC_NORMAL || (s->model & 0xFF) == ORC_HYBRID);
     if (overlap) { detect(s, pos, &w); records(s, &w, pos); }
-    double E = forces(s, pos, &w, force);
+    double E = forces_ex(s, pos, &w, force, fext);
     if (numIntersections) *numIntersections = w.I;
     ws_free(&w);
     return E;
@@ -1542,26 +1761,73 @@ void orc_fire_init(const orc_system* s, double* pos, double* vel, double* force,
 void orc_fire_step(const orc_system* s, double* pos, double* vel, double* force, orc_fire_state* st,
                    double dtMax, double alphaStart, double fAlpha, double fInc, double fDec,
                    int nMin, int shakeIter) {
+    orc_fire_step_ex(s, pos, vel, force, st, dtMax, alphaStart, fAlpha, fInc, fDec, nMin, shakeIter, 0, 0.0);
+}
+
+/* orc_fire_step; with xpbdIter > 0 (models with springs; no counterpart in the reference,
+ * whose XPBD launcher is never called, K10) the springs of the model (edge: stiffness,
+ * area: compressibility, perimeter: perimStiffness) are integrated implicitly by XPBD and
+ * only the rest of the force (fext, the overlap force) explicitly -- XPBD's own time
+ * step (Macklin et al. 2016, Algorithm 1) inside FIRE:
+ *   v' = v + dt fext;  x~ = frac(x + dt v');  x' = XPBD(x~; at = 1/(k dt^2)), wrapped;
+ *   v = v' + wrap(x' - x~)/dt;  F = total force at x' (springs included)
+ * then the energy test and the mixing of model.cpp:444-485 on the total energy and force,
+ * with no second half-kick.  A fixed point (v = 0) has fext = sum k C grad C, i.e. a zero
+ * total force. */
+void orc_fire_step_ex(const orc_system* s, double* pos, double* vel, double* force, orc_fire_state* st,
+                      double dtMax, double alphaStart, double fAlpha, double fInc, double fDec,
+                      int nMin, int shakeIter, int xpbdIter, double xpbdTol) {
     int V2 = 2 * s->V;
-    int needsInter = ((s->model & 0xFF) == ORC_NORMAL || (s->model & 0xFF) == ORC_HYBRID);
-    int rigid = ((s->model & 0xFF) == ORC_NORMAL);
+    int mt = s->model & 0xFF;
+    int needsInter = (mt == ORC_NORMAL || mt == ORC_HYBRID);
+    int rigid = (mt == ORC_NORMAL);
+    int xpbd = xpbdIter > 0 && !rigid && mt != ORC_ABNORMAL;
     double dt = st->dt;
     double* bak = (double*)malloc(sizeof(double) * V2);
     double* tmp = (double*)malloc(sizeof(double) * V2);
     memcpy(bak, pos, sizeof(double) * V2);
     double ePre = st->energy;
-    for (int q = 0; q < V2; ++q) {
-        double vh = vel[q] + (0.5 * dt) * force[q];
-        double p = pos[q] + dt * vh;
-        pos[q] = p - floor(p);
-        vel[q] = vh;
+    if (xpbd) {
+        double* fext = (double*)malloc(sizeof(double) * V2);
+        double* ftmp = (double*)malloc(sizeof(double) * V2);
+        orc_force_eval_ex(s, pos, ftmp, fext, NULL);   /* the step's start state, as stored */
+        for (int q = 0; q < V2; ++q) {
+            double vh = vel[q] + dt * fext[q];
+            double p = pos[q] + dt * vh;
+            pos[q] = p - floor(p);
+            vel[q] = vh;
+        }
+        memcpy(tmp, pos, sizeof(double) * V2);
+        int edgeOn = (mt == ORC_SOFT_BODY || mt == ORC_EDGE_ONLY || mt == ORC_HYBRID);
+        int areaOn = (mt == ORC_SOFT_BODY || mt == ORC_AREA_ONLY || mt == ORC_HYBRID);
+        int perimOn = (mt == ORC_SOFT_BODY || mt == ORC_HYBRID);
+        int it = orc_xpbd_lanes(s->V, s->N, s->start, pos, s->targetEdgeV, NULL, s->targetAreas,
+                                edgeOn ? s->stiffness : 0.0, areaOn ? s->compressibility : 0.0,
+                                perimOn ? s->perimStiffness : 0.0, dt, xpbdIter, xpbdTol);
+        st->lastShakeIters = it;
+        for (int q = 0; q < V2; ++q) {
+            double p = pos[q] - floor(pos[q]);
+            pos[q] = p;
+            vel[q] = vel[q] + orc_wrap(p - tmp[q]) / dt;
+        }
+        free(fext); free(ftmp);
+    } else {
+        for (int q = 0; q < V2; ++q) {
+            double vh = vel[q] + (0.5 * dt) * force[q];
+            double p = pos[q] + dt * vh;
+            pos[q] = p - floor(p);
+            vel[q] = vh;
+        }
     }
     orc_ws w;
     ws_alloc(&w, s->V, s->N, s->boxSize, s->maxNeighbors > 0 ? s->maxNeighbors : 100);
     int shakeFailed = 0;
     st->rolledBack = 0;
     int areaMode = (s->model & ORC_AREA_MODES) != 0;
-    if (needsInter) {
+    if (needsInter && xpbd) {
+        detect(s, pos, &w);
+        records(s, &w, pos);
+    } else if (needsInter) {
         if (!areaMode) detect(s, pos, &w);   /* reference order: detect, SHAKE, records */
         if (rigid && shakeIter > 0) {
             memcpy(tmp, pos, sizeof(double) * V2);
@@ -1597,7 +1863,8 @@ void orc_fire_step(const orc_system* s, double* pos, double* vel, double* force,
         st->dt = dt * fDec; st->alpha = alphaStart; st->nPos = 0;
         st->rolledBack = 1;
     } else {
-        for (int q = 0; q < V2; ++q) vel[q] = vel[q] + (0.5 * dt) * force[q];
+        if (!xpbd)
+            for (int q = 0; q < V2; ++q) vel[q] = vel[q] + (0.5 * dt) * force[q];
         double Pw = fire_dot(s, force, vel);
         double vnorm = sqrt(fire_dot(s, vel, vel));
         double fnorm = sqrt(fire_dot(s, force, force));
@@ -1626,11 +1893,19 @@ int orc_fire_minimize(const orc_system* s, double* pos, double* vel, double* for
                       orc_fire_state* st, double fTol, int maxSteps, double dtMax,
                       double alphaStart, double fAlpha, double fInc, double fDec, int nMin,
                       int shakeIter) {
+    return orc_fire_minimize_ex(s, pos, vel, force, st, fTol, maxSteps, dtMax, alphaStart, fAlpha, fInc,
+                                fDec, nMin, shakeIter, 0, 0.0);
+}
+int orc_fire_minimize_ex(const orc_system* s, double* pos, double* vel, double* force,
+                         orc_fire_state* st, double fTol, int maxSteps, double dtMax,
+                         double alphaStart, double fAlpha, double fInc, double fDec, int nMin,
+                         int shakeIter, int xpbdIter, double xpbdTol) {
     st->alpha = alphaStart;
     st->nPos = 0;
     orc_fire_init(s, pos, vel, force, st);
     for (int step = 0; step < maxSteps; ++step) {
-        orc_fire_step(s, pos, vel, force, st, dtMax, alphaStart, fAlpha, fInc, fDec, nMin, shakeIter);
+        orc_fire_step_ex(s, pos, vel, force, st, dtMax, alphaStart, fAlpha, fInc, fDec, nMin, shakeIter,
+                         xpbdIter, xpbdTol);
         if (orc_max_abs(force, 2 * s->V) <= fTol) return step + 1;
     }
     return maxSteps;

@@ -55,6 +55,12 @@ int orc_outersections_ex(int V, int N, const int* start, int maxNeighbors,
                          const int* numNeighbors, uint64_t* intersections, double* tu,
                          uint64_t* outersections, double* ut, int* shapeCounts,
                          const double* pairExactPos);
+double orc_force_energy_ex(int model, int V, int N, const int* start, const double* pos,
+                           int I, const uint64_t* intersections, const uint64_t* outersections,
+                           const double* tu, const double* ut, const double* edgeLengths,
+                           const double* areas, const double* targetEdgeV, const double* targetAreas,
+                           double stiffness, double compressibility, double perimStiffness,
+                           double* force, double* fext);
 double orc_force_energy(int model, int V, int N, const int* start, const double* pos,
                         int I, const uint64_t* intersections, const uint64_t* outersections,
                         const double* tu, const double* ut, const double* edgeLengths,
@@ -76,6 +82,10 @@ int orc_shape_project(int V, int N, const int* start, double* pos, const double*
                       const double* targetAreas);
 int orc_xpbd(int V, int N, const int* start, double* pos, const double* targetEdgeP,
              const double* targetAreas, int nIter, double tol);
+/* compliant XPBD in the kernel's lane order (see pcp_oracle.c) */
+int orc_xpbd_lanes(int V, int N, const int* start, double* pos, const double* targetEdgeV,
+                   const double* targetEdgeP, const double* targetAreas, double kE, double kA,
+                   double kP, double dt, int nIter, double tol);
 double orc_det_sum(const double* x, int n);
 double orc_max_abs(const double* x, int n);
 long long orc_overlap_count(int V, int N, const int* start, const double* pos, int boxSize,
@@ -92,9 +102,12 @@ typedef struct {
     const double* targetAreas;   /* [N] */
     double stiffness, compressibility;
     int solver;                  /* projection/SHAKE: ORC_SOLVER_DENSE or ORC_SOLVER_LANES */
+    double perimStiffness;       /* perimeter spring of softBody / hybrid (0 = none) */
 } orc_system;
 enum { ORC_SOLVER_DENSE = 0, ORC_SOLVER_LANES = 1 };
 double orc_force_eval(const orc_system* s, const double* pos, double* force, int* numIntersections);
+double orc_force_eval_ex(const orc_system* s, const double* pos, double* force, double* fext,
+                         int* numIntersections);
 
 /* FIRE (host contract model.cpp:405-504) */
 typedef struct {
@@ -112,6 +125,14 @@ int orc_fire_minimize(const orc_system* s, double* pos, double* vel, double* for
                       orc_fire_state* st, double fTol, int maxSteps, double dtMax,
                       double alphaStart, double fAlpha, double fInc, double fDec, int nMin,
                       int shakeIter);
+/* with XPBD integration of the springs (xpbdIter > 0): see pcp_oracle.c */
+void orc_fire_step_ex(const orc_system* s, double* pos, double* vel, double* force, orc_fire_state* st,
+                      double dtMax, double alphaStart, double fAlpha, double fInc, double fDec,
+                      int nMin, int shakeIter, int xpbdIter, double xpbdTol);
+int orc_fire_minimize_ex(const orc_system* s, double* pos, double* vel, double* force,
+                         orc_fire_state* st, double fTol, int maxSteps, double dtMax,
+                         double alphaStart, double fAlpha, double fInc, double fDec, int nMin,
+                         int shakeIter, int xpbdIter, double xpbdTol);
 
 #ifdef __cplusplus
 }

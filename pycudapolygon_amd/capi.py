@@ -23,12 +23,17 @@ _sz = ctypes.c_size_t
 class PcpState(ctypes.Structure):
     _fields_ = [("pos", _p), ("vel", _p), ("force", _p), ("targetEdgeV", _p), ("targetAreas", _p),
                 ("edgeLengths", _p), ("areas", _p), ("energy", _p), ("stiffness", _d),
-                ("compressibility", _d), ("model", _i)]
+                ("compressibility", _d), ("model", _i), ("perimStiffness", _d)]
 
 
 class FireParams(ctypes.Structure):
     _fields_ = [("dtMax", _d), ("alphaStart", _d), ("fAlpha", _d), ("fInc", _d), ("fDec", _d),
-                ("nMin", _i), ("shakeIter", _i), ("fTol", _d)]
+                ("nMin", _i), ("shakeIter", _i), ("fTol", _d), ("xpbdIter", _i), ("xpbdTol", _d)]
+
+
+class XpbdParams(ctypes.Structure):
+    _fields_ = [("edgeStiffness", _d), ("areaStiffness", _d), ("perimStiffness", _d), ("dt", _d),
+                ("nIter", _i), ("tol", _d)]
 
 
 class FireStatus(ctypes.Structure):
@@ -62,6 +67,7 @@ SIGNATURES = {
     "pcp_project_force": (_i, [_p, _p, _p]),
     "pcp_shake": (_i, [_p, _p, _p, _p, _i, _d, _p]),
     "pcp_xpbd": (_i, [_p, _p, _p, _p, _i, _d, _p]),
+    "pcp_xpbd_ex": (_i, [_p, _p, _p, _p, ctypes.POINTER(XpbdParams), _p]),
     "pcp_max_abs": (_i, [_p, _p, _p]),
     "pcp_update_positions": (_i, [_p, _p, _p, _d]),
     "pcp_reset_areas": (_i, [_p, _p, _p, _p, _p, _p]),
@@ -207,7 +213,7 @@ class System:
     """
 
     def __init__(self, packings, boxSizes, model=NORMAL, stiffness=0.0, compressibility=0.0,
-                 maxNeighbors=100):
+                 maxNeighbors=100, perimStiffness=0.0):
         self.ctx = Context()
         self.P = len(packings)
         starts, packPoly, pos, tev, ta = [0], [0], [], [], []
@@ -247,7 +253,7 @@ class System:
         self.compressibility = compressibility
         self._state = PcpState(self.pos.ptr, self.vel.ptr, self.force.ptr, self.tev.ptr, self.ta.ptr,
                                self.edgeLengths.ptr, self.areas.ptr, self.energy.ptr, stiffness,
-                               compressibility, model)
+                               compressibility, model, perimStiffness)
 
     # --- reference operations (single calls through the C ABI)
     def geometry(self, constraints=True):
@@ -280,6 +286,23 @@ class System:
     def shake(self, maxIter, tol=1e-15):
         it = np.zeros(self.P, np.int32)
         self.ctx.call("pcp_shake", self.pos.ptr, self.tev.ptr, self.ta.ptr, int(maxIter), float(tol), _ptr(it))
+        return it
+
+    def xpbd(self, nIter, tol=0.0, targetEdgeP=None):
+        """pcp_xpbd (the reference's hard edge + area projection); targetEdgeP: one edge target
+        per polygon (default: the first edge target of each polygon)"""
+        if targetEdgeP is None:
+            targetEdgeP = self.tev.get()[self.polyStart[:-1]]
+        tp = DeviceArray.from_numpy(np.asarray(targetEdgeP, np.float64))
+        it = np.zeros(self.P, np.int32)
+        self.ctx.call("pcp_xpbd", self.pos.ptr, tp.ptr, self.ta.ptr, int(nIter), float(tol), _ptr(it))
+        return it
+
+    def xpbd_ex(self, kE, kA, kP, dt, nIter, tol=0.0):
+        """pcp_xpbd_ex: compliant XPBD with the context's per-edge and area targets"""
+        prm = XpbdParams(float(kE), float(kA), float(kP), float(dt), int(nIter), float(tol))
+        it = np.zeros(self.P, np.int32)
+        self.ctx.call("pcp_xpbd_ex", self.pos.ptr, self.tev.ptr, self.ta.ptr, ctypes.byref(prm), _ptr(it))
         return it
 
     def max_abs_force(self):
@@ -356,8 +379,9 @@ class System:
 
     # --- FIRE
     def fire_params(self, fTol=1e-10, dtMax=0.1, alphaStart=0.1, fAlpha=0.99, fInc=1.1, fDec=0.5, nMin=5,
-                    shakeIter=5):
-        return FireParams(dtMax, alphaStart, fAlpha, fInc, fDec, nMin, shakeIter, fTol)
+                    shakeIter=5, xpbdIter=0, xpbdTol=0.0):
+        return FireParams(dtMax, alphaStart, fAlpha, fInc, fDec, nMin, shakeIter, fTol, int(xpbdIter),
+                          float(xpbdTol))
 
     def fire_init(self, prm, dt0=1e-3):
         check(lib().pcp_fire_init(self.ctx.h, ctypes.byref(self._state), ctypes.byref(prm), float(dt0)))

@@ -86,6 +86,7 @@ pcp_state Model::state() const {
     s.edgeLengths = edgeLengths; s.areas = areas; s.energy = energy;
     s.stiffness = stiffness; s.compressibility = compressibility;
     s.model = (int)simControl.modelType | energyFlags;
+    s.perimStiffness = 0.0;
     return s;
 }
 

@@ -34,7 +34,9 @@ __global__ void k_fire_init(int P, double dtv, double alpha0, double* dt, double
 
 // Step start.  Per packing (thread p < P): backups for rollback / retry, flag reset,
 // E_pre (model.cpp:411-412).  Per vertex (thread i < V): velocity-Verlet half kick and
-// drift, with backups of x and v.
+// drift, with backups of x and v.  XPBD: a full kick with the force before the springs
+// (Fe; the springs are integrated by the XPBD projection that follows), backed up in febak.
+template <bool XPBD>
 __global__ void k_fire_begin(int V, int P, const int* __restrict__ shapeId, const int* __restrict__ polyPack,
                              const int* __restrict__ active, const double* __restrict__ dt,
                              const double* __restrict__ alpha, const int* __restrict__ nPos,
@@ -42,7 +44,8 @@ __global__ void k_fire_begin(int V, int P, const int* __restrict__ shapeId, cons
                              double* __restrict__ epre, int* __restrict__ flags, double* __restrict__ fs,
                              int* __restrict__ shakeIt, int* __restrict__ summary, double2* __restrict__ x,
                              double2* __restrict__ v, const double2* __restrict__ F, double2* __restrict__ xbak,
-                             double2* __restrict__ vbak, double2* __restrict__ xpre, double2* __restrict__ fbak) {
+                             double2* __restrict__ vbak, double2* __restrict__ xpre, double2* __restrict__ fbak,
+                             const double2* __restrict__ Fe, double2* __restrict__ febak) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < 4) summary[i] = 0;
     if (i < P) {
@@ -72,8 +75,16 @@ __global__ void k_fire_begin(int V, int P, const int* __restrict__ shapeId, cons
     xbak[i] = xi;
     vbak[i] = vi;
     if (fbak) fbak[i] = fi;
-    double vhx = vi.x + (0.5 * h) * fi.x;
-    double vhy = vi.y + (0.5 * h) * fi.y;
+    double vhx, vhy;
+    if (XPBD) {
+        double2 fe = Fe[i];
+        if (febak) febak[i] = fe;
+        vhx = vi.x + h * fe.x;
+        vhy = vi.y + h * fe.y;
+    } else {
+        vhx = vi.x + (0.5 * h) * fi.x;
+        vhy = vi.y + (0.5 * h) * fi.y;
+    }
     double px = xi.x + h * vhx, py = xi.y + h * vhy;
     px = px - floor(px);
     py = py - floor(py);
@@ -138,18 +149,21 @@ __global__ __launch_bounds__(BLK) void k_fire_energy(int N, const int* __restric
                                                     const double* __restrict__ epoly, double* __restrict__ energy,
                                                     const int* __restrict__ active, const double* __restrict__ epre,
                                                     double alphaStart, double fDec, double* dt, double* alpha,
-                                                    int* nPos, int* flags) {
+                                                    int* nPos, int* flags, const int* __restrict__ xpbdIt) {
     __shared__ double part[BLK + 1];
     int p = blockIdx.x;
     int a0 = packPoly[p], n = packPoly[p + 1] - a0;
     double e = det_sum_block(epoly + a0, n, part);
     if (threadIdx.x == 0) {
         energy[p] = e;
+        if (xpbdIt && active[p]) flags[FL_NF * p + FL_LASTSHAKE] = xpbdIt[p];   // XPBD iterations
         fire_control(p, active, energy, epre, alphaStart, fDec, dt, alpha, nPos, flags);
     }
 }
 
 // second half-kick and per-polygon partial sums of F.v, v.v, F.F (vertex order); max |F|
+// NOKICK (FIRE with XPBD): no second half-kick, the sums only
+template <bool NOKICK>
 __global__ void k_fire_kick(int N, const int* __restrict__ polyStart, const int* __restrict__ polyPack,
                             const int* __restrict__ active, const int* __restrict__ flags,
                             const double* __restrict__ dt, double2* __restrict__ v,
@@ -166,9 +180,12 @@ __global__ void k_fire_kick(int N, const int* __restrict__ polyStart, const int*
         double h = dt[p];
         for (int m = polyStart[s]; m < polyStart[s + 1]; ++m) {
             double2 vi = v[m], fi = F[m];
-            double vx = vi.x + (0.5 * h) * fi.x;
-            double vy = vi.y + (0.5 * h) * fi.y;
-            v[m] = make_double2(vx, vy);
+            double vx = vi.x, vy = vi.y;
+            if (!NOKICK) {
+                vx = vi.x + (0.5 * h) * fi.x;
+                vy = vi.y + (0.5 * h) * fi.y;
+                v[m] = make_double2(vx, vy);
+            }
             a += fi.x * vx;
             a += fi.y * vy;
             b += vx * vx;
@@ -187,7 +204,7 @@ __global__ void k_fire_kick(int N, const int* __restrict__ polyStart, const int*
 // written coalesced; the per-polygon sums keep the sequential vertex order (x then y term
 // of vertex 0, of vertex 1, ...), replayed by each lane of the segment from the products
 // staged in LDS.
-template <int SEG>
+template <int SEG, bool NOKICK>
 __global__ __launch_bounds__(BLK) void k_fire_kick_seg(int N, const int* __restrict__ polyStart,
                                                       const int* __restrict__ polyPack,
                                                       const int* __restrict__ active, const int* __restrict__ flags,
@@ -213,9 +230,12 @@ __global__ __launch_bounds__(BLK) void k_fire_kick_seg(int N, const int* __restr
     if (kick && live) {
         double h = dt[p];
         double2 vi = v[st + e];
-        double vx = vi.x + (0.5 * h) * fi.x;
-        double vy = vi.y + (0.5 * h) * fi.y;
-        v[st + e] = make_double2(vx, vy);
+        double vx = vi.x, vy = vi.y;
+        if (!NOKICK) {
+            vx = vi.x + (0.5 * h) * fi.x;
+            vy = vi.y + (0.5 * h) * fi.y;
+            v[st + e] = make_double2(vx, vy);
+        }
         pa = make_double2(fi.x * vx, fi.y * vy);
         pb = make_double2(vx * vx, vy * vy);
         pc = make_double2(fi.x * fi.x, fi.y * fi.y);
@@ -276,7 +296,8 @@ __global__ void k_fire_bend(int V, const int* __restrict__ shapeId, const int* _
                             const int* __restrict__ active, const int* __restrict__ flags,
                             const double* __restrict__ fs, double2* __restrict__ x, double2* __restrict__ v,
                             double2* __restrict__ F, const double2* __restrict__ xbak,
-                            const double2* __restrict__ fbak) {
+                            const double2* __restrict__ fbak, double2* __restrict__ Fe,
+                            const double2* __restrict__ febak) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= V) return;
     int p = polyPack[shapeId[i]];
@@ -294,6 +315,7 @@ __global__ void k_fire_bend(int V, const int* __restrict__ shapeId, const int* _
         x[i] = xbak[i];
         v[i] = make_double2(0.0, 0.0);
         if (fbak) F[i] = fbak[i];
+        if (febak) Fe[i] = febak[i];
     }
 }
 
@@ -429,17 +451,18 @@ __global__ void k_zero2(long long n, double2* v) {
 
 // the evaluation part of one step (positions already moved by k_fire_begin)
 // the kick: lane segments when every polygon fits one (n <= 64), else thread per polygon
+template <bool NOKICK>
 static int launch_fire_kick(pcp_ctx* c, double2* vel, const double2* force) {
     int seg = c->seg;
     if (c->maxN > 64) {
-        PCP_LAUNCH(c, k_fire_kick, dim3(nblk(c->N)), dim3(BLK), c->N, c->dPolyStart, c->dPolyPack, c->dActive,
-                   c->dFlags, c->dDt, vel, force, c->dRed);
+        PCP_LAUNCH(c, k_fire_kick<NOKICK>, dim3(nblk(c->N)), dim3(BLK), c->N, c->dPolyStart, c->dPolyPack,
+                   c->dActive, c->dFlags, c->dDt, vel, force, c->dRed);
         return PCP_OK;
     }
     int grid = (c->N + BLK / seg - 1) / (BLK / seg);
 #define PCP_KICK_SEG(W)                                                                                     \
-    PCP_LAUNCH(c, k_fire_kick_seg<W>, dim3(grid), dim3(BLK), c->N, c->dPolyStart, c->dPolyPack, c->dActive, \
-               c->dFlags, c->dDt, vel, force, c->dRed)
+    PCP_LAUNCH(c, (k_fire_kick_seg<W, NOKICK>), dim3(grid), dim3(BLK), c->N, c->dPolyStart, c->dPolyPack, \
+               c->dActive, c->dFlags, c->dDt, vel, force, c->dRed)
     switch (seg) {
     case 4: PCP_KICK_SEG(4); break;
     case 8: PCP_KICK_SEG(8); break;
@@ -458,7 +481,21 @@ static int fire_eval(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm
     bool area = (st->model & PCP_AREA_MODES) != 0;
     bool overlap = (mt == PCP_NORMAL || mt == PCP_HYBRID);
     bool rigid = (mt == PCP_NORMAL);
-    if (overlap) {
+    bool xpbd = prm->xpbdIter > 0;
+    if (xpbd) {
+        // the springs of the model by XPBD at the packing's dt, then velocities from the
+        // displacement; detection and records at the final positions
+        bool edgeOn = (mt == PCP_SOFT_BODY || mt == PCP_EDGE_ONLY || mt == PCP_HYBRID);
+        bool areaOn = (mt == PCP_SOFT_BODY || mt == PCP_AREA_ONLY || mt == PCP_HYBRID);
+        bool perimOn = (mt == PCP_SOFT_BODY || mt == PCP_HYBRID);
+        PCP_TRY(run_xpbd_seg(c, x, st->targetEdgeV, nullptr, st->targetAreas, edgeOn ? st->stiffness : 0.0,
+                             areaOn ? st->compressibility : 0.0, perimOn ? st->perimStiffness : 0.0, 0.0, c->dDt,
+                             prm->xpbdIter, prm->xpbdTol, c->dActive, v, c->dShakeIt));
+        if (overlap) {
+            PCP_TRY(run_detect(c, x, 0));
+            PCP_TRY(run_records(c, area ? x : nullptr));
+        }
+    } else if (overlap) {
         if (!area)   // reference order (model.cpp:416-440): detect, SHAKE, records
             PCP_TRY(run_detect(c, x, 0));
         bool shake = rigid && prm->shakeIter > 0;
@@ -483,9 +520,11 @@ static int fire_eval(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm
         PCP_TRY(run_records(c, area ? x : nullptr));
     }
     PCP_TRY(run_forces(c, st->model, x, st->edgeLengths, st->areas, st->targetEdgeV, st->targetAreas,
-                       st->stiffness, st->compressibility, (double2*)st->force, nullptr, 1, rigid));
+                       st->stiffness, st->compressibility, (double2*)st->force, nullptr, 1, rigid, false,
+                       st->perimStiffness));
     PCP_LAUNCH(c, k_fire_energy, dim3(c->P), dim3(BLK), c->N, c->dPackPoly, c->dEpoly, st->energy, c->dActive,
-               c->dEpre, prm->alphaStart, prm->fDec, c->dDt, c->dAlpha, c->dNPos, c->dFlags);
+               c->dEpre, prm->alphaStart, prm->fDec, c->dDt, c->dAlpha, c->dNPos, c->dFlags,
+               xpbd ? (const int*)c->dShakeIt : (const int*)nullptr);
     return PCP_OK;
 }
 
@@ -504,19 +543,30 @@ static const int* host_summary(pcp_ctx* c) { return (const int*)c->hStatusBlock.
 // kept in dBlockOver, so several steps can be queued without reading the status back.
 static int fire_step_launch(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm, int block) {
     double2* fbak = block ? (double2*)c->dFbak : nullptr;
-    PCP_LAUNCH(c, k_fire_begin, dim3(nblk(std::max(std::max(c->V, c->P), 4))), dim3(BLK), c->V, c->P, c->dShapeId,
-               c->dPolyPack, c->dActive, c->dDt, c->dAlpha, c->dNPos, c->dSteps, st->energy, c->dEpre, c->dFlags,
-               c->dFireScal, c->dShakeIt, c->dSummary, (double2*)st->pos, (double2*)st->vel,
-               (const double2*)st->force, (double2*)c->dXbak, (double2*)c->dVbak, (double2*)c->dXpre, fbak);
+    bool xpbd = prm->xpbdIter > 0;
+    double2* febak = (block && xpbd) ? (double2*)c->dFextBak : nullptr;
+    if (xpbd)
+        PCP_LAUNCH(c, k_fire_begin<true>, dim3(nblk(std::max(std::max(c->V, c->P), 4))), dim3(BLK), c->V, c->P,
+                   c->dShapeId, c->dPolyPack, c->dActive, c->dDt, c->dAlpha, c->dNPos, c->dSteps, st->energy,
+                   c->dEpre, c->dFlags, c->dFireScal, c->dShakeIt, c->dSummary, (double2*)st->pos,
+                   (double2*)st->vel, (const double2*)st->force, (double2*)c->dXbak, (double2*)c->dVbak,
+                   (double2*)c->dXpre, fbak, (const double2*)c->dFext, febak);
+    else
+        PCP_LAUNCH(c, k_fire_begin<false>, dim3(nblk(std::max(std::max(c->V, c->P), 4))), dim3(BLK), c->V, c->P,
+                   c->dShapeId, c->dPolyPack, c->dActive, c->dDt, c->dAlpha, c->dNPos, c->dSteps, st->energy,
+                   c->dEpre, c->dFlags, c->dFireScal, c->dShakeIt, c->dSummary, (double2*)st->pos,
+                   (double2*)st->vel, (const double2*)st->force, (double2*)c->dXbak, (double2*)c->dVbak,
+                   (double2*)c->dXpre, fbak, (const double2*)nullptr, (double2*)nullptr);
     PCP_TRY(fire_eval(c, st, prm));
-    PCP_TRY(launch_fire_kick(c, (double2*)st->vel, (const double2*)st->force));
+    if (xpbd) PCP_TRY(launch_fire_kick<true>(c, (double2*)st->vel, (const double2*)st->force));
+    else PCP_TRY(launch_fire_kick<false>(c, (double2*)st->vel, (const double2*)st->force));
     PCP_LAUNCH(c, k_fire_stats, dim3(c->P), dim3(BLK), c->N, c->dPackPoly, c->dRed, 1, prm->nMin, prm->dtMax,
                prm->alphaStart, prm->fAlpha, prm->fInc, prm->fDec, prm->fTol, c->dActive, c->dFlags, c->dDt, c->dAlpha,
                c->dNPos, c->dSteps, st->energy, c->dFireScal, c->dStatus, c->dSummary, c->dScal,
                block ? (const double*)c->dEpre : nullptr, block ? c->dBlockOver : nullptr);
     PCP_LAUNCH(c, k_fire_bend, dim3(nblk(c->V)), dim3(BLK), c->V, c->dShapeId, c->dPolyPack, c->dActive, c->dFlags,
                c->dFireScal, (double2*)st->pos, (double2*)st->vel, (double2*)st->force, (const double2*)c->dXbak,
-               (const double2*)fbak);
+               (const double2*)fbak, (double2*)c->dFext, (const double2*)febak);
     return PCP_OK;
 }
 
@@ -641,7 +691,22 @@ static int fire_block(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* pr
 
 extern "C" {
 
+static int check_fire_params(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm) {
+    int mt = st->model & 0xFF;
+    if (prm->xpbdIter > 0 && (mt == PCP_NORMAL || mt == PCP_ABNORMAL)) {
+        set_error("FIRE with XPBD (xpbdIter > 0) integrates the springs of a model that has them "
+                  "(edgeOnly, areaOnly, softBody, hybrid)");
+        return PCP_ERR_ARG;
+    }
+    if (prm->xpbdIter > 0 && c->maxN > 64) {
+        set_error("FIRE with XPBD: polygons of at most 64 vertices");
+        return PCP_ERR_ARG;
+    }
+    return PCP_OK;
+}
+
 int pcp_fire_init(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm, double dt0) {
+    PCP_TRY(check_fire_params(c, st, prm));
     if (!c->haveBoxes && ((st->model & 0xFF) == PCP_NORMAL || (st->model & 0xFF) == PCP_HYBRID)) {
         set_error("pcp_fire_init: set the box sizes first");
         return PCP_ERR_ARG;
@@ -657,6 +722,7 @@ int pcp_fire_init(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm, d
 }
 
 int pcp_fire_step(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm, pcp_fire_status* status) {
+    PCP_TRY(check_fire_params(c, st, prm));
     PCP_TRY(fire_step_impl(c, st, prm, nullptr));
     if (status) memcpy(status, c->hStatusBlock.data() + 16, sizeof(pcp_fire_status) * c->P);
     return PCP_OK;
@@ -664,6 +730,7 @@ int pcp_fire_step(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm, p
 
 int pcp_fire_minimize(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm, int maxSteps,
                       pcp_fire_status* status) {
+    PCP_TRY(check_fire_params(c, st, prm));
     for (int step = 0; step < maxSteps; ++step) {
         int any = 1;
         PCP_TRY(fire_step_impl(c, st, prm, &any));
@@ -678,6 +745,7 @@ int pcp_fire_minimize(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* pr
 
 int pcp_fire_run(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm, int maxSteps, int stepsPerBlock,
                  int useGraph, pcp_fire_status* status) {
+    PCP_TRY(check_fire_params(c, st, prm));
     if (stepsPerBlock < 1 || stepsPerBlock > 256) {
         set_error("pcp_fire_run: stepsPerBlock must be in 1..256");
         return PCP_ERR_ARG;

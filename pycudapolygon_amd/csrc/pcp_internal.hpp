@@ -146,6 +146,9 @@ struct pcp_ctx {
     // snapshot (x, v, F; per packing dt, alpha, energy, max|F| and nPos, steps, active), the
     // largest bucket count seen in the block, and the captured graph of one block of steps
     double* dFbak = nullptr;
+    // FIRE with XPBD springs: the force before the springs (written by every evaluation of a
+    // model with springs) and its backup for device-side rollback
+    double *dFext = nullptr, *dFextBak = nullptr;   // 2V
     double *dSnapX = nullptr, *dSnapV = nullptr, *dSnapF = nullptr, *dSnapD = nullptr;
     int* dSnapI = nullptr;
     int* dBlockOver = nullptr;
@@ -175,7 +178,10 @@ int run_records(pcp_ctx* c, const double2* pairPos);
 int run_forces(pcp_ctx* c, int model, const double2* pos, double* edgeLengths, double* areas,
                const double* targetEdgeV, const double* targetAreas, double stiffness,
                double compressibility, double2* force, double* energy, int geom, int project,
-               bool sticky = false);
+               bool sticky = false, double perimK = 0.0);
+int run_xpbd_seg(pcp_ctx* c, double2* pos, const double* targetEdgeV, const double* targetEdgeP,
+                 const double* targetAreas, double kE, double kA, double kP, double dt, const double* dtPack,
+                 int nIter, double tol, const int* active, double2* vel, int* itOut);
 int run_project(pcp_ctx* c, const double2* pos, double2* force);
 int run_project_seg(pcp_ctx* c, const double2* pos, double2* force);
 int run_shape_project_seg(pcp_ctx* c, double2* pos, const double2* ref, const double* targetAreas);

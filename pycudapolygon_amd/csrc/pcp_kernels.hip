@@ -793,68 +793,6 @@ int run_shake(pcp_ctx* c, double2* pos, const double* targetEdgeV, const double*
 #include "pcp_records.hip"
 namespace pcp {
 
-// ------------------------------------------------------------------ XPBD
-// xpbdProjectCUDA model.cu:209-278 with kernels.cuh:532-623: colour-0 then colour-1
-// edge projections, then the area correction; thread per polygon (edges of one
-// colour are applied in ascending order, which equals the reference's concurrent
-// update for even n and removes its race for odd n; the area correction reads the
-// positions before the correction, which the reference's concurrent kernel races on).
-__global__ void k_xpbd_iter(int N, const int* __restrict__ polyStart, double2* __restrict__ pos,
-                            const double* __restrict__ targetEdgeP, const double* __restrict__ targetAreas,
-                            double* __restrict__ devOut, double* __restrict__ work, int V) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= N) return;
-    int st = polyStart[s], n = polyStart[s + 1] - st;
-    double L0 = targetEdgeP[s];
-    for (int color = 0; color < 2; ++color)
-        for (int e = color; e < n; e += 2) {
-            int k = st + e, nk = st + ((e + 1 == n) ? 0 : e + 1);
-            double2 a = pos[k], b = pos[nk];
-            double ex = wrap(b.x - a.x), ey = wrap(b.y - a.y);
-            double len = sqrt(ex * ex + ey * ey);
-            if (len < 1e-14) continue;
-            double cc = 0.5 * (len - L0) / len;
-            pos[k] = make_double2(a.x + cc * ex, a.y + cc * ey);
-            double2 b2 = pos[nk];
-            pos[nk] = make_double2(b2.x - cc * ex, b2.y - cc * ey);
-        }
-    double2 ps = pos[st];
-    double xs = ps.x, ys = ps.y;
-    double area = 0.0, g2 = 0.0;
-    double* old = work + 2LL * st;
-    for (int e = 0; e < n; ++e) {
-        double2 q = pos[st + e];
-        old[2 * e] = q.x; old[2 * e + 1] = q.y;
-    }
-    for (int e = 0; e < n; ++e) {
-        int nk = (e + 1 == n) ? 0 : e + 1, pk = (e == 0) ? n - 1 : e - 1;
-        double xk = wrap(old[2 * e] - xs), yk = wrap(old[2 * e + 1] - ys);
-        double xnk = wrap(old[2 * nk] - xs), ynk = wrap(old[2 * nk + 1] - ys);
-        double xpk = wrap(old[2 * pk] - xs), ypk = wrap(old[2 * pk + 1] - ys);
-        area += 0.5 * (xk * ynk - xnk * yk);
-        double gx = 0.5 * (ynk - ypk), gy = 0.5 * (xpk - xnk);
-        g2 += gx * gx + gy * gy;
-    }
-    double alpha = (area - targetAreas[s]) / fmax(g2, 1e-24);
-    for (int e = 0; e < n; ++e) {
-        int nk = (e + 1 == n) ? 0 : e + 1, pk = (e == 0) ? n - 1 : e - 1;
-        double xnk = wrap(old[2 * nk] - xs), ynk = wrap(old[2 * nk + 1] - ys);
-        double xpk = wrap(old[2 * pk] - xs), ypk = wrap(old[2 * pk + 1] - ys);
-        double gx = 0.5 * (ynk - ypk), gy = 0.5 * (xpk - xnk);
-        pos[st + e] = make_double2(old[2 * e] - alpha * gx, old[2 * e + 1] - alpha * gy);
-    }
-    double maxE = 0.0;
-    for (int e = 0; e < n; ++e) {
-        int k = st + e, nk = st + ((e + 1 == n) ? 0 : e + 1);
-        double2 a = pos[k], b = pos[nk];
-        double ex = wrap(b.x - a.x), ey = wrap(b.y - a.y);
-        double d = fabs(sqrt(ex * ex + ey * ey) - L0);
-        if (d > maxE) maxE = d;
-    }
-    devOut[s] = maxE;
-    devOut[N + s] = fabs(area - targetAreas[s]);
-}
-
 // ------------------------------------------------------------------ misc kernels
 __global__ void k_max_abs_poly(int N, const int* __restrict__ polyStart, const double2* __restrict__ x,
                                double* __restrict__ out) {
@@ -1139,7 +1077,7 @@ static void free_topology(pcp_ctx* c) {
     dfree(c->dEpoly); dfree(c->dWork); dfree(c->dRed);
     dfree(c->dRedOut); dfree(c->dScanPart); c->scanPartCap = 0;
     dfree(c->dXbak); dfree(c->dXpre); dfree(c->dVbak); dfree(c->dDt); dfree(c->dAlpha);
-    dfree(c->dFbak); dfree(c->dSnapX); dfree(c->dSnapV); dfree(c->dSnapF); dfree(c->dSnapD); dfree(c->dSnapI);
+    dfree(c->dFbak); dfree(c->dFext); dfree(c->dFextBak); dfree(c->dSnapX); dfree(c->dSnapV); dfree(c->dSnapF); dfree(c->dSnapD); dfree(c->dSnapI);
     dfree(c->dBlockOver);
     if (c->fireGraph) (void)hipGraphExecDestroy(c->fireGraph);
     c->fireGraph = nullptr;
@@ -1244,7 +1182,7 @@ int pcp_set_topology(pcp_ctx* c, int P, const int* packPolyStart, const int* pol
     PCP_TRY(dalloc(&c->dAccR, N)); PCP_TRY(dalloc(&c->dEpoly, N));
     PCP_TRY(dalloc(&c->dWork, 8LL * V)); PCP_TRY(dalloc(&c->dRed, 4LL * N)); PCP_TRY(dalloc(&c->dRedOut, 4LL * P));
     PCP_TRY(dalloc(&c->dXbak, 2LL * V)); PCP_TRY(dalloc(&c->dXpre, 2LL * V)); PCP_TRY(dalloc(&c->dVbak, 2LL * V));
-    PCP_TRY(dalloc(&c->dFbak, 2LL * V)); PCP_TRY(dalloc(&c->dSnapX, 2LL * V)); PCP_TRY(dalloc(&c->dSnapV, 2LL * V));
+    PCP_TRY(dalloc(&c->dFbak, 2LL * V)); PCP_TRY(dalloc(&c->dFext, 2LL * V)); PCP_TRY(dalloc(&c->dFextBak, 2LL * V)); PCP_TRY(dalloc(&c->dSnapX, 2LL * V)); PCP_TRY(dalloc(&c->dSnapV, 2LL * V));
     PCP_TRY(dalloc(&c->dSnapF, 2LL * V)); PCP_TRY(dalloc(&c->dSnapD, 4LL * P)); PCP_TRY(dalloc(&c->dSnapI, 3LL * P));
     PCP_TRY(dalloc(&c->dBlockOver, 1));
     PCP_TRY(dalloc(&c->dDt, P)); PCP_TRY(dalloc(&c->dAlpha, P)); PCP_TRY(dalloc(&c->dEpre, P));
@@ -1376,26 +1314,36 @@ int pcp_shake(pcp_ctx* c, double* pos, const double* targetEdgeV, const double* 
     }
     return PCP_OK;
 }
+static int xpbd_common(pcp_ctx* c, double* pos, const double* targetEdgeV, const double* targetEdgeP,
+                       const double* targetAreas, double kE, double kA, double kP, double dt, int nIter, double tol,
+                       int* iters) {
+    if (c->maxN > 64) return fail_arg("pcp_xpbd: polygons of at most 64 vertices");
+    if (nIter < 0) return fail_arg("pcp_xpbd: nIter < 0");
+    if ((kE != 0.0 || kA != 0.0 || kP != 0.0) && !(dt > 0.0)) return fail_arg("pcp_xpbd: dt must be > 0");
+    if ((kE != 0.0 || kP != 0.0) && !targetEdgeV && !targetEdgeP) return fail_arg("pcp_xpbd: no edge targets");
+    if (kA != 0.0 && !targetAreas) return fail_arg("pcp_xpbd: no area targets");
+    hipLaunchKernelGGL(k_fill_int, dim3(nblk(2LL * c->P)), dim3(BLK), 0, c->stream, c->dShakeIt, 2LL * c->P, 0);
+    PCP_LAUNCHED();
+    if (c->N > 0 && nIter > 0)
+        PCP_TRY(run_xpbd_seg(c, (double2*)pos, targetEdgeV, targetEdgeP, targetAreas, kE, kA, kP, dt, nullptr,
+                             nIter, tol, nullptr, nullptr, c->dShakeIt));
+    if (iters) {
+        PCP_HIP(hipMemcpyAsync(iters, c->dShakeIt, sizeof(int) * c->P, hipMemcpyDeviceToHost, c->stream));
+        PCP_HIP(hipStreamSynchronize(c->stream));
+        if (tol <= 0.0)
+            for (int p = 0; p < c->P; ++p) iters[p] = nIter;   // the reference's loop runs to nIter
+    }
+    return PCP_OK;
+}
 int pcp_xpbd(pcp_ctx* c, double* pos, const double* targetEdgeP, const double* targetAreas, int nIter,
              double tol, int* iters) {
-    int it = 0;
-    std::vector<double> dev(2 * (size_t)c->N);
-    for (; it < nIter; ++it) {
-        hipLaunchKernelGGL(k_xpbd_iter, dim3(nblk(c->N)), dim3(BLK), 0, c->stream, c->N, c->dPolyStart,
-                           (double2*)pos, targetEdgeP, targetAreas, c->dRed, c->dWork, c->V);
-        PCP_LAUNCHED();
-        if (tol > 0) {
-            PCP_HIP(hipMemcpyAsync(dev.data(), c->dRed, sizeof(double) * 2 * c->N, hipMemcpyDeviceToHost,
-                                   c->stream));
-            PCP_HIP(hipStreamSynchronize(c->stream));
-            double mE = 0.0, mA = 0.0;
-            for (int s = 0; s < c->N; ++s) { mE = std::max(mE, dev[s]); mA = std::max(mA, dev[c->N + s]); }
-            if (mE < tol && mA < tol) break;
-        }
-    }
-    if (iters) *iters = it;
-    PCP_HIP(hipStreamSynchronize(c->stream));
-    return PCP_OK;
+    return xpbd_common(c, pos, nullptr, targetEdgeP, targetAreas, INFINITY, INFINITY, 0.0, 1.0, nIter, tol, iters);
+}
+int pcp_xpbd_ex(pcp_ctx* c, double* pos, const double* targetEdgeV, const double* targetAreas,
+                const pcp_xpbd_params* prm, int* iters) {
+    if (!prm) return fail_arg("pcp_xpbd_ex: no parameters");
+    return xpbd_common(c, pos, targetEdgeV, nullptr, targetAreas, prm->edgeStiffness, prm->areaStiffness,
+                       prm->perimStiffness, prm->dt, prm->nIter, prm->tol, iters);
 }
 int pcp_max_abs(pcp_ctx* c, const double* x, double* out) {
     if (c->N == 0) return PCP_OK;
@@ -1665,7 +1613,7 @@ int pcp_force_eval(pcp_ctx* c, const pcp_state* st) {
     bool fused = overlap && st->energy;
     PCP_TRY(run_forces(c, st->model, pos, st->edgeLengths, st->areas, st->targetEdgeV, st->targetAreas,
                        st->stiffness, st->compressibility, (double2*)st->force, st->energy, 1,
-                       mt == PCP_NORMAL, fused));
+                       mt == PCP_NORMAL, fused, st->perimStiffness));
     if (overlap && !fused) {
         hipLaunchKernelGGL(k_sticky, dim3(1), dim3(64), 0, c->stream, c->dScal, c->Kp, c->dSticky);
         PCP_LAUNCHED();

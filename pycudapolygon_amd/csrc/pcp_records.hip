@@ -585,7 +585,7 @@ __global__ __launch_bounds__(BLK) void k_gather_poly(
     const double2* __restrict__ LNi, const double* __restrict__ accR, double* __restrict__ edgeLengths,
     double* __restrict__ areas, const double* __restrict__ targetEdgeV, const double* __restrict__ targetAreas,
     double stiffness, double compressibility, double2* __restrict__ force, double* __restrict__ epoly,
-    const int* __restrict__ order) {
+    const int* __restrict__ order, double perimK, double2* __restrict__ fext) {
     int sub = threadIdx.x & (SEG - 1);
     int si = (xcd_block(blockIdx.x, gridDim.x) * BLK + threadIdx.x) / SEG;
     if (si >= N) return;   // whole segments leave together
@@ -611,6 +611,31 @@ __global__ __launch_bounds__(BLK) void k_gather_poly(
         if (sub == 0) areas[s] = A;
     } else if (areaOn) {
         A = areas[s];
+    }
+    // perimeter spring (no counterpart in the reference): perimeter and its target (the sum
+    // of the edge targets), both summed in vertex order
+    bool perimOn = perimK != 0.0;
+    double Pp = 0.0, P0 = 0.0;
+    if (perimOn) {
+        for (int k0 = 0; k0 < n; k0 += SEG) {
+            int k = k0 + sub;
+            double lk = 0.0, tk = 0.0;
+            if (k < n) {
+                if (GEOM) {
+                    d2 pk = ldpos(pos, st + k), pn = ldpos(pos, st + ((k + 1 == n) ? 0 : k + 1));
+                    double dx = wrap(pn.x - pk.x), dy = wrap(pn.y - pk.y);
+                    lk = sqrt(dx * dx + dy * dy);
+                } else {
+                    lk = edgeLengths[st + k];
+                }
+                tk = targetEdgeV[st + k];
+            }
+            int cl = min(SEG, n - k0);
+            for (int q = 0; q < cl; ++q) {
+                Pp += __shfl(lk, q, SEG);
+                P0 += __shfl(tk, q, SEG);
+            }
+        }
     }
     double e = 0.5 * ((overlap) ? accR[s] : 0.0);
     int jn = overlap ? min(jcnt[s], Kp) : 0;
@@ -646,8 +671,9 @@ __global__ __launch_bounds__(BLK) void k_gather_poly(
                 if (LM.x != 0.0 || LM.y != 0.0) { F.x += LM.x; F.y += LM.y; }
                 if (LN.x != 0.0 || LN.y != 0.0) { F.x += LN.x; F.y += LN.y; }
             }
+            if (fext) fext[m] = make_double2(F.x, F.y);
             double l = 0.0, prvl = 0.0;
-            if (GEOM || edgeOn) {
+            if (GEOM || edgeOn || perimOn) {
                 if (GEOM) {
                     d2 Pn = ldpos(pos, nx), Pp = ldpos(pos, pm);
                     double dx = wrap(Pn.x - Pm.x), dy = wrap(Pn.y - Pm.y);
@@ -686,6 +712,16 @@ __global__ __launch_bounds__(BLK) void k_gather_poly(
                     F.x += fx; F.y += fy;
                 }
             }
+            if (perimOn) {   // -kP (P - P0) (t_{m-1} - t_m), t the unit edge vectors
+                d2 Pn = ldpos(pos, nx), Pq = ldpos(pos, pm);
+                double dx = wrap(Pn.x - Pm.x), dy = wrap(Pn.y - Pm.y);
+                double px = wrap(Pm.x - Pq.x), py = wrap(Pm.y - Pq.y);
+                double tx = (l > 1e-14) ? dx / l : 0.0, ty = (l > 1e-14) ? dy / l : 0.0;
+                double qx = (prvl > 1e-14) ? px / prvl : 0.0, qy = (prvl > 1e-14) ? py / prvl : 0.0;
+                double coeff = perimK * (Pp - P0);
+                F.x += -coeff * (qx - tx);
+                F.y += -coeff * (qy - ty);
+            }
         }
         if (edgeOn) {   // edge energies in vertex order
             int cl = min(SEG, n - m0);
@@ -699,6 +735,7 @@ __global__ __launch_bounds__(BLK) void k_gather_poly(
         double A0 = targetAreas[s];
         if (!(A0 < 1e-14)) e += compressibility * (A - A0) * (A - A0) / 2.0;
     }
+    if (perimOn) e += perimK * (Pp - P0) * (Pp - P0) / 2.0;
     if (sub == 0) epoly[s] = e;
 }
 
@@ -753,14 +790,14 @@ int run_records(pcp_ctx* c, const double2* pairPos) {
 template <int SEG, int GEOM, int PROJ>
 static int launch_gather(pcp_ctx* c, int overlap, int edgeOn, int areaOn, const double2* pos, double* edgeLengths,
                          double* areas, const double* targetEdgeV, const double* targetAreas, double stiffness,
-                         double compressibility, double2* force) {
+                         double compressibility, double2* force, double perimK, double2* fext) {
     int grid = (c->N + BLK / SEG - 1) / (BLK / SEG);
     int slot = -1;
     PCP_TRY(prof_begin(c, "k_gather_poly", &slot));
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gather_poly<SEG, GEOM, PROJ>), dim3(grid), dim3(BLK), 0, c->stream, c->N,
                        c->Kp, overlap, edgeOn, areaOn, c->dPolyStart, pos, c->dJcnt, c->dJC, c->dJT, c->dF1, c->dLM,
                        c->dLN, c->dAccR, edgeLengths, areas, targetEdgeV, targetAreas, stiffness, compressibility,
-                       force, c->dEpoly, c->dPcSorted);
+                       force, c->dEpoly, c->dPcSorted, perimK, fext);
     PCP_LAUNCHED();
     PCP_TRY(prof_end(c, slot));
     return PCP_OK;
@@ -769,17 +806,18 @@ static int launch_gather(pcp_ctx* c, int overlap, int edgeOn, int areaOn, const 
 template <int GEOM, int PROJ>
 static int dispatch_gather(pcp_ctx* c, int overlap, int edgeOn, int areaOn, const double2* pos,
                            double* edgeLengths, double* areas, const double* targetEdgeV,
-                           const double* targetAreas, double stiffness, double compressibility, double2* force) {
+                           const double* targetAreas, double stiffness, double compressibility, double2* force,
+                           double perimK, double2* fext) {
 #define PCP_GATHER(W) \
     return launch_gather<W, GEOM, PROJ>(c, overlap, edgeOn, areaOn, pos, edgeLengths, areas, targetEdgeV, \
-                                        targetAreas, stiffness, compressibility, force)
+                                        targetAreas, stiffness, compressibility, force, perimK, fext)
     switch (c->seg) {
     case 4: PCP_GATHER(4);
     case 8: PCP_GATHER(8);
     case 16:
         if (PROJ && c->row16)   // PROJ 2: the projection's lane moves as DPP row moves
             return launch_gather<16, GEOM, 2>(c, overlap, edgeOn, areaOn, pos, edgeLengths, areas, targetEdgeV,
-                                              targetAreas, stiffness, compressibility, force);
+                                              targetAreas, stiffness, compressibility, force, perimK, fext);
         PCP_GATHER(16);
     case 32: PCP_GATHER(32);
     default: PCP_GATHER(64);
@@ -919,7 +957,8 @@ int autotune(pcp_ctx* c, const pcp_state* st, int reps, double* timesUs) {
 
 int run_forces(pcp_ctx* c, int model, const double2* pos, double* edgeLengths, double* areas,
                const double* targetEdgeV, const double* targetAreas, double stiffness,
-               double compressibility, double2* force, double* energy, int geom, int project, bool sticky) {
+               double compressibility, double2* force, double* energy, int geom, int project, bool sticky,
+               double perimK) {
     if (c->V == 0) return PCP_OK;
     int pair = (model & PCP_PAIR_SQUARED) ? 1 : 0;
     int sq = (model & (PCP_AREA_SQUARED | PCP_PAIR_SQUARED)) ? 1 : 0;
@@ -928,6 +967,10 @@ int run_forces(pcp_ctx* c, int model, const double2* pos, double* edgeLengths, d
     int overlap = (model == PCP_NORMAL || model == PCP_HYBRID);
     int edgeOn = (model == PCP_SOFT_BODY || model == PCP_EDGE_ONLY || model == PCP_HYBRID);
     int areaOn = (model == PCP_SOFT_BODY || model == PCP_AREA_ONLY || model == PCP_HYBRID);
+    // perimeter spring of the softBody and hybrid models; the force before the springs is
+    // kept for FIRE with XPBD (every model with springs)
+    if (!(model == PCP_SOFT_BODY || model == PCP_HYBRID)) perimK = 0.0;
+    double2* fext = (model == PCP_NORMAL) ? nullptr : (double2*)c->dFext;
     if (model == PCP_ABNORMAL) {   // model.cpp:390-391: forces and energy zeroed only
         hipLaunchKernelGGL(k_fill_double, dim3(nblk(2LL * c->V)), dim3(BLK), 0, c->stream, (double*)force,
                            2LL * c->V, 0.0);
@@ -952,13 +995,13 @@ int run_forces(pcp_ctx* c, int model, const double2* pos, double* edgeLengths, d
     bool proj = project && c->maxN <= 64;
     if (geom && proj)
         PCP_TRY((dispatch_gather<1, 1>(c, overlap, edgeOn, areaOn, pos, edgeLengths, areas, targetEdgeV,
-                                       targetAreas, stiffness, compressibility, force)));
+                                       targetAreas, stiffness, compressibility, force, perimK, fext)));
     else if (geom)
         PCP_TRY((dispatch_gather<1, 0>(c, overlap, edgeOn, areaOn, pos, edgeLengths, areas, targetEdgeV,
-                                       targetAreas, stiffness, compressibility, force)));
+                                       targetAreas, stiffness, compressibility, force, perimK, fext)));
     else
         PCP_TRY((dispatch_gather<0, 0>(c, overlap, edgeOn, areaOn, pos, edgeLengths, areas, targetEdgeV,
-                                       targetAreas, stiffness, compressibility, force)));
+                                       targetAreas, stiffness, compressibility, force, perimK, fext)));
     if (project && !proj) PCP_TRY(run_project(c, pos, force));
     if (!energy) return PCP_OK;   // the caller reduces the polygon energies (FIRE)
     return launch_det_reduce(c, c->dEpoly, 1, energy, 0, sticky);

@@ -373,6 +373,208 @@ __global__ __launch_bounds__(BLK) void k_shake_seg(int N, int P, const int* __re
     block_pack_max<BLK / SEG>(threadIdx.x / SEG, e == 0, pk, it1, it2, P, itOut);
 }
 
+// XPBD with compliance (xpbdProjectCUDA model.cu:209-278 / kernels.cuh:532-623, extended as
+// in Macklin, Mueller & Chentanez, MIG 2016).  One lane segment per polygon, lane e owns
+// vertex e and edge e (p_e -> p_{e+1}); positions, edge multipliers and the polygon's area
+// and perimeter multipliers stay in registers for all iterations.  Per iteration: the edge
+// constraints by colour (even edges -- for odd n the last one after the others, as it shares
+// vertex 0 with edge 0 -- then odd edges; the edges of one pass share no vertex), then the
+// area and the perimeter constraint together (their 2x2 block; the two gradients are nearly
+// parallel, so one after the other converges slowly), or alone.  Every projection applies
+//   dlambda = -(C + at lambda) / (sum |grad C|^2 + at),  x += grad C dlambda,
+// at = 1/(k dt^2) (k = INFINITY: at = 0 and the edge and area steps are the reference's
+// arithmetic, c = 0.5 (len - L)/len and alpha = (A - A0)/max(|grad A|^2, 1e-24)).  The
+// perimeter target is the sum of the edge targets in vertex order.  tol > 0: the polygon
+// stops after an iteration whose largest |C + at lambda| (edges after the iteration, area
+// and perimeter before their step) is below tol; itOut receives the largest stopping
+// index per packing.  vel != nullptr (FIRE): dt is the packing's, inactive packings are
+// skipped, the result is wrapped into the box and v += wrap(x - x_start)/dt; otherwise
+// positions are left unwrapped, as the reference leaves them.
+template <int SEG>
+__global__ __launch_bounds__(BLK) void k_xpbd_seg(int N, int P, const int* __restrict__ polyStart,
+                                                 const int* __restrict__ polyPack, double2* __restrict__ pos,
+                                                 const double* __restrict__ targetEdgeV,
+                                                 const double* __restrict__ targetEdgeP,
+                                                 const double* __restrict__ targetAreas, double kE, double kA,
+                                                 double kP, double dtS, const double* __restrict__ dtPack,
+                                                 int nIter, double tol, const int* __restrict__ active,
+                                                 double2* __restrict__ vel, int* __restrict__ itOut) {
+    int e = threadIdx.x & (SEG - 1);
+    int s = (blockIdx.x * BLK + threadIdx.x) / SEG;
+    int pk = -1, itDone = 0;
+    if (s < N) {   // no early return: the whole block meets in block_pack_max
+        int pkS = polyPack[s];
+        int st = polyStart[s], n = polyStart[s + 1] - st;
+        if ((!active || active[pkS]) && n >= 3 && n <= SEG) {
+            pk = pkS;
+            bool live = e < n;
+            int en = (e + 1 >= n) ? 0 : e + 1, ep = (e == 0) ? n - 1 : e - 1;
+            double2 p0 = live ? pos[st + e] : make_double2(0.0, 0.0);
+            double x = p0.x, y = p0.y;
+            double L = 0.0;
+            if (live) L = targetEdgeV ? targetEdgeV[st + e] : (targetEdgeP ? targetEdgeP[s] : 0.0);
+            double P0 = 0.0;
+            for (int q = 0; q < n; ++q) P0 += seg_get<SEG>(L, q);
+            bool edgeOn = kE != 0.0, areaOn = kA != 0.0, perimOn = kP != 0.0;
+            double h = dtPack ? dtPack[pkS] : dtS;
+            double atE = edgeOn ? 1.0 / (kE * (h * h)) : 0.0;
+            double atA = areaOn ? 1.0 / (kA * (h * h)) : 0.0;
+            double atP = perimOn ? 1.0 / (kP * (h * h)) : 0.0;
+            double A0 = areaOn ? targetAreas[s] : 0.0;
+            double lam = 0.0, lamA = 0.0, lamP = 0.0;
+            bool odd = (n & 1) != 0;
+            int it = 0;
+            for (; it < nIter; ++it) {
+                if (edgeOn) {
+                    int nPass = odd ? 3 : 2;
+                    for (int pass = 0; pass < nPass; ++pass) {
+                        bool last = odd && e == n - 1;
+                        bool act = live && ((pass == 0) ? (!(e & 1) && !last)
+                                                        : (pass == 1 && nPass == 3) ? last : (e & 1) != 0);
+                        double xn = seg_get<SEG>(x, en), yn = seg_get<SEG>(y, en);
+                        double dx = 0.0, dy = 0.0;
+                        if (act) {
+                            double ex = wrap(xn - x), ey = wrap(yn - y);
+                            double len = sqrt(ex * ex + ey * ey);
+                            if (len < 1e-14) {
+                                act = false;
+                            } else {
+                                double C = len - L;
+                                double dl = (-C - atE * lam) / (2.0 + atE);
+                                lam += dl;
+                                double c = -dl / len;
+                                dx = c * ex;
+                                dy = c * ey;
+                            }
+                        }
+                        double dxp = seg_get<SEG>(dx, ep), dyp = seg_get<SEG>(dy, ep);
+                        int actp = __shfl((int)act, ep, SEG);
+                        double nx = x, ny = y;
+                        if (act) { nx = x + dx; ny = y + dy; }
+                        if (actp) { nx = x - dxp; ny = y - dyp; }
+                        x = nx;
+                        y = ny;
+                    }
+                }
+                double devA = 0.0, devP = 0.0;
+                double gAx = 0.0, gAy = 0.0, gPx = 0.0, gPy = 0.0, rA = 0.0, rP = 0.0, aa = 0.0, pp = 0.0;
+                if (areaOn) {
+                    double xs = seg_get<SEG>(x, 0), ys = seg_get<SEG>(y, 0);
+                    double xnk = seg_get<SEG>(x, en), ynk = seg_get<SEG>(y, en);
+                    double xpk = seg_get<SEG>(x, ep), ypk = seg_get<SEG>(y, ep);
+                    double term = 0.0, g2 = 0.0;
+                    if (live) {
+                        double xk = wrap(x - xs), yk = wrap(y - ys);
+                        double xn2 = wrap(xnk - xs), yn2 = wrap(ynk - ys);
+                        double xp2 = wrap(xpk - xs), yp2 = wrap(ypk - ys);
+                        term = 0.5 * (xk * yn2 - xn2 * yk);
+                        gAx = 0.5 * (yn2 - yp2);
+                        gAy = 0.5 * (xp2 - xn2);
+                        g2 = gAx * gAx + gAy * gAy;
+                    }
+                    double area = seg_sum<SEG>(term);
+                    aa = seg_sum<SEG>(g2);
+                    rA = (area - A0) + atA * lamA;
+                    devA = fabs(rA);
+                }
+                if (perimOn) {
+                    double xn = seg_get<SEG>(x, en), yn = seg_get<SEG>(y, en);
+                    double len = 0.0, t1 = 0.0, t2 = 0.0;
+                    if (live) {
+                        double ex = wrap(xn - x), ey = wrap(yn - y);
+                        len = sqrt(ex * ex + ey * ey);
+                        t1 = (len >= 1e-14) ? ex / len : 0.0;
+                        t2 = (len >= 1e-14) ? ey / len : 0.0;
+                    }
+                    double t1p = seg_get<SEG>(t1, ep), t2p = seg_get<SEG>(t2, ep);
+                    double gp2 = 0.0;
+                    if (live) {
+                        gPx = t1p - t1;
+                        gPy = t2p - t2;
+                        gp2 = gPx * gPx + gPy * gPy;
+                    }
+                    double Pc = seg_sum<SEG>(len);
+                    pp = seg_sum<SEG>(gp2);
+                    rP = (Pc - P0) + atP * lamP;
+                    devP = fabs(rP);
+                }
+                if (areaOn && perimOn) {
+                    double ap = seg_sum<SEG>(live ? gAx * gPx + gAy * gPy : 0.0);
+                    double m11 = aa + atA, m22 = pp + atP;
+                    double det = m11 * m22 - ap * ap;
+                    if (det > 1e-12 * (m11 * m22)) {
+                        double dA = (ap * rP - m22 * rA) / det;
+                        double dP = (ap * rA - m11 * rP) / det;
+                        lamA += dA;
+                        lamP += dP;
+                        x += gAx * dA + gPx * dP;
+                        y += gAy * dA + gPy * dP;
+                    } else {
+                        double alpha = rA / (fmax(aa, 1e-24) + atA);
+                        lamA -= alpha;
+                        x -= alpha * gAx;
+                        y -= alpha * gAy;
+                    }
+                } else if (areaOn) {
+                    double alpha = rA / (fmax(aa, 1e-24) + atA);
+                    lamA -= alpha;
+                    x -= alpha * gAx;
+                    y -= alpha * gAy;
+                } else if (perimOn) {
+                    double alpha = rP / (fmax(pp, 1e-24) + atP);
+                    lamP -= alpha;
+                    x -= alpha * gPx;
+                    y -= alpha * gPy;
+                }
+                if (tol > 0.0) {
+                    double devE = 0.0;
+                    if (edgeOn) {
+                        double xn = seg_get<SEG>(x, en), yn = seg_get<SEG>(y, en);
+                        double d = 0.0;
+                        if (live) {
+                            double ex = wrap(xn - x), ey = wrap(yn - y);
+                            d = fabs(sqrt(ex * ex + ey * ey) - L + atE * lam);
+                        }
+                        devE = seg_max<SEG>(d);
+                    }
+                    if ((!edgeOn || devE < tol) && (!areaOn || devA < tol) && (!perimOn || devP < tol)) break;
+                }
+            }
+            itDone = it;
+            if (live) {
+                if (vel) {
+                    double px = x - floor(x), py = y - floor(y);
+                    double2 v = vel[st + e];
+                    vel[st + e] = make_double2(v.x + wrap(px - p0.x) / h, v.y + wrap(py - p0.y) / h);
+                    pos[st + e] = make_double2(px, py);
+                } else {
+                    pos[st + e] = make_double2(x, y);
+                }
+            }
+        }
+    }
+    block_pack_max<BLK / SEG>(threadIdx.x / SEG, e == 0, pk, itDone, -1, P, itOut);
+}
+
+int run_xpbd_seg(pcp_ctx* c, double2* pos, const double* targetEdgeV, const double* targetEdgeP,
+                 const double* targetAreas, double kE, double kA, double kP, double dt, const double* dtPack,
+                 int nIter, double tol, const int* active, double2* vel, int* itOut) {
+    int seg = c->seg;
+    int grid = (c->N + BLK / seg - 1) / (BLK / seg);
+#define PCP_XPBD_SEG(W)                                                                                      \
+    PCP_LAUNCH(c, k_xpbd_seg<W>, dim3(grid), dim3(BLK), c->N, c->P, c->dPolyStart, c->dPolyPack, pos,        \
+               targetEdgeV, targetEdgeP, targetAreas, kE, kA, kP, dt, dtPack, nIter, tol, active, vel, itOut)
+    switch (seg) {
+    case 4: PCP_XPBD_SEG(4); break;
+    case 8: PCP_XPBD_SEG(8); break;
+    case 16: PCP_XPBD_SEG(16); break;
+    case 32: PCP_XPBD_SEG(32); break;
+    default: PCP_XPBD_SEG(64); break;
+    }
+#undef PCP_XPBD_SEG
+    return PCP_OK;
+}
+
 // Shape matching of each polygon (lane e owns vertex e): its rest shape ref (offsets from
 // the rest shape's vertex mean), scaled to the target area when targetAreas is given,
 // turned by the rotation of the polar decomposition of M = sum_e q_e ref_e^T (q_e = the
