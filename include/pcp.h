@@ -193,8 +193,13 @@ int pcp_shape_project(pcp_ctx* ctx, double* pos, const double* refShape, const d
 /* saveTentativePositionsCUDA + getMaxEffectiveForceCUDA model.cu:280-306: out host[P] */
 int pcp_max_effective_force(pcp_ctx* ctx, const double* pos, const double* tentPos,
                             const double* force, double scale, double* out);
-/* updateOverlapAreaCUDA model.cu:466-507 (grid estimator): counter device[D*D] or NULL,
- * sum host[P] = sum of k(k-1)/2 over the D*D points of each packing */
+/* updateOverlapAreaCUDA model.cu:466-507 (grid estimator, kernels.cuh:879-979): for every point
+ * ((ix + 0.5)/D, (iy + 0.5)/D) of a D x D grid (D = pointDensity) in each packing, k = the number
+ * of polygons containing it among those with an edge midpoint in the 3x3 block of cells of the
+ * last pcp_update_cells around the point's cell (the first 513 such polygons with >= 3 vertices,
+ * in the reference's visiting order); counter device[P*D*D] (packing-major, then iy*D + ix;
+ * 64-bit offsets) or NULL receives k(k-1)/2, sum host[P] its sum per packing.  At most 65536
+ * polygons per packing. */
 int pcp_overlap_count(pcp_ctx* ctx, const double* pos, int pointDensity, int* counter, long long* sum);
 
 /* ---- read-back of the context's internal arrays for one packing (the getters

@@ -305,6 +305,14 @@ class System:
         self.ctx.call("pcp_xpbd_ex", self.pos.ptr, self.tev.ptr, self.ta.ptr, ctypes.byref(prm), _ptr(it))
         return it
 
+    def overlap_count(self, D, counter=True):
+        """pcp_overlap_count on the cells of the last cells() call: (sum per packing,
+        counter [P, D*D] or None)"""
+        cnt = DeviceArray((self.P * D * D,), np.int32) if counter else None
+        sums = np.zeros(self.P, np.int64)
+        self.ctx.call("pcp_overlap_count", self.pos.ptr, int(D), cnt.ptr if cnt is not None else None, _ptr(sums))
+        return sums, (cnt.get().reshape(self.P, D * D) if cnt is not None else None)
+
     def max_abs_force(self):
         self.ctx.call("pcp_max_abs", self.force.ptr, self.scratchP.ptr)
         self.sync()

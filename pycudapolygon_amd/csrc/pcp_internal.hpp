@@ -149,6 +149,9 @@ struct pcp_ctx {
     // FIRE with XPBD springs: the force before the springs (written by every evaluation of a
     // model with springs) and its backup for device-side rollback
     double *dFext = nullptr, *dFextBak = nullptr;   // 2V
+    // grid overlap estimator: per-cell partial sums, then per-packing sums (grown on demand)
+    long long* dOvScratch = nullptr;
+    long long ovScratchCap = 0;
     double *dSnapX = nullptr, *dSnapV = nullptr, *dSnapF = nullptr, *dSnapD = nullptr;
     int* dSnapI = nullptr;
     int* dBlockOver = nullptr;

@@ -850,63 +850,6 @@ __device__ __forceinline__ double wrapPeriodic(double x) {
     while (x > 1.0 && guard < 64) { x -= 1.0; ++guard; }
     return x - 0.5;
 }
-// updateOverlapAreaKernel kernels.cuh:879-979 (grid points of one packing per launch)
-__global__ void k_overlap_count(int D, int B, int Vp, const int* __restrict__ shapeIdLocal0, int vbase,
-                                const int* __restrict__ shapeId, const int* __restrict__ polyStart,
-                                const int* __restrict__ cellStartP, const int* __restrict__ cellSorted,
-                                const double2* __restrict__ pos, int* __restrict__ counter,
-                                long long* __restrict__ sum) {
-    (void)shapeIdLocal0; (void)Vp;
-    const double eps = 1e-12, pi = 3.141592653589793238462643383279;
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    int total = D * D;
-    if (idx >= total) return;
-    int ix = idx % D, iy = idx / D;
-    double px = (ix + 0.5) / D, py = (iy + 0.5) / D;
-    int num = 0;
-    int cellX = (int)(px * B), cellY = (int)(py * B);
-    int tracked[512];
-    int nt = 0;
-    bool full = false;
-    for (int dx = -1; dx <= 1 && !full; dx++) {
-        int nx = cellX + dx;
-        if (nx < 0) nx += B; else if (nx >= B) nx -= B;
-        for (int dy = -1; dy <= 1 && !full; dy++) {
-            int ny = cellY + dy;
-            if (ny < 0) ny += B; else if (ny >= B) ny -= B;
-            int boxId = ny * B + nx;
-            int q0 = cellStartP[boxId], q1 = cellStartP[boxId + 1];
-            for (int q = q0; q < q1 && !full; q++) {
-                int poly = shapeId[cellSorted[q]];
-                bool seen = false;
-                for (int t = 0; t < nt; t++) if (tracked[t] == poly) { seen = true; break; }
-                if (seen) continue;
-                int st = polyStart[poly], en = polyStart[poly + 1];
-                if (en - st < 3) continue;
-                double angleSum = 0.0;
-                double vxPrev = wrapPeriodic(pos[st].x - px);
-                double vyPrev = wrapPeriodic(pos[st].y - py);
-                double vx0 = vxPrev + 0.0, vy0 = vyPrev + 0.0;
-                for (int e = st + 1; e < en; e++) {
-                    double vx = vxPrev + wrapPeriodic(pos[e].x - pos[e - 1].x);
-                    double vy = vyPrev + wrapPeriodic(pos[e].y - pos[e - 1].y);
-                    double cross = vxPrev * vy - vyPrev * vx;
-                    double dot = vxPrev * vx + vyPrev * vy;
-                    angleSum += atan2(cross, dot);
-                    vxPrev = vx; vyPrev = vy;
-                }
-                angleSum += atan2(vxPrev * vy0 - vyPrev * vx0, vxPrev * vx0 + vyPrev * vy0);
-                if (fabs(angleSum - 2.0 * pi) < eps) num++;
-                if (nt < 512) tracked[nt++] = poly; else full = true;
-            }
-        }
-    }
-    (void)vbase;
-    int c = num * (num - 1) / 2;
-    if (counter) counter[idx] = c;
-    if (c) atomicAdd((unsigned long long*)sum, (unsigned long long)c);
-}
-
 // ordered neighbour lists for the getters (reference slot order: 3x3 stencil in
 // (dx, dy) order with de-duplication, ascending vertex id inside a cell)
 __global__ void k_neighbor_lists(int v0, int v1, int M, const int* __restrict__ shapeId,
@@ -1066,6 +1009,8 @@ int pcp_memset(void* ptr, int value, size_t bytes) {
 }
 
 static void free_topology(pcp_ctx* c) {
+    dfree(c->dOvScratch);
+    c->ovScratchCap = 0;
     dfree(c->dPackPoly); dfree(c->dPackVert); dfree(c->dPolyStart); dfree(c->dShapeId);
     dfree(c->dNext); dfree(c->dPrev); dfree(c->dPolyPack); dfree(c->dBox); dfree(c->dCellOff);
     dfree(c->dCellOf); dfree(c->dCellCount); dfree(c->dCellStart); dfree(c->dCellTmp);
@@ -1386,26 +1331,6 @@ int pcp_max_effective_force(pcp_ctx* c, const double* pos, const double* tentPos
     PCP_HIP(hipStreamSynchronize(c->stream));
     return PCP_OK;
 }
-int pcp_overlap_count(pcp_ctx* c, const double* pos, int D, int* counter, long long* sum) {
-    if (!c->haveBoxes) return fail_arg("pcp_overlap_count: set the box sizes first");
-    long long* dsum = nullptr;
-    PCP_TRY(dalloc(&dsum, c->P));
-    PCP_HIP(hipMemsetAsync(dsum, 0, sizeof(long long) * c->P, c->stream));
-    for (int p = 0; p < c->P; ++p) {
-        int total = D * D;
-        if (total == 0) continue;
-        hipLaunchKernelGGL(k_overlap_count, dim3(nblk(total)), dim3(BLK), 0, c->stream, D, c->hBox[p],
-                           c->hPackVert[p + 1] - c->hPackVert[p], (const int*)nullptr, c->hPackVert[p],
-                           c->dShapeId, c->dPolyStart, c->dCellStart + c->hCellOff[p], c->dCellSorted,
-                           (const double2*)pos, counter ? counter + (long long)p * total : nullptr, dsum + p);
-        PCP_LAUNCHED();
-    }
-    PCP_HIP(hipMemcpyAsync(sum, dsum, sizeof(long long) * c->P, hipMemcpyDeviceToHost, c->stream));
-    PCP_HIP(hipStreamSynchronize(c->stream));
-    dfree(dsum);
-    return PCP_OK;
-}
-
 // ---- getters
 int pcp_get_cells(pcp_ctx* c, int p, int* cellLocation, int* neighborIndices, int* countPerBox) {
     if (p < 0 || p >= c->P) return fail_arg("packing out of range");
@@ -1623,5 +1548,6 @@ int pcp_force_eval(pcp_ctx* c, const pcp_state* st) {
 
 }  // extern "C"
 
+#include "pcp_overlap.hip"
 #include "pcp_fire.hip"
 #include "pcp_mpnn.hip"
