@@ -201,6 +201,18 @@ int pcp_max_effective_force(pcp_ctx* ctx, const double* pos, const double* tentP
  * 64-bit offsets) or NULL receives k(k-1)/2, sum host[P] its sum per packing.  At most 65536
  * polygons per packing. */
 int pcp_overlap_count(pcp_ctx* ctx, const double* pos, int pointDensity, int* counter, long long* sum);
+/* No counterpart in the reference: pcp_overlap_count for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results; the estimator's per-cell partial sums go to a
+ * fixed device array of the library (one per GPU, held until the process ends) instead of a
+ * buffer the context allocates, so the call allocates nothing.  Limits: the context's cells
+ * (the sum of boxSize^2 over its packings) at most PCP_OVERLAP_NOALLOC_MAX_CELLS and its
+ * packings at most PCP_OVERLAP_NOALLOC_MAX_PACKINGS, else PCP_ERR_ARG.  Every call on one GPU
+ * shares that array: calls that overlap in time on one GPU (other threads, other contexts'
+ * streams) race and must be serialized by the caller. */
+#define PCP_OVERLAP_NOALLOC_MAX_CELLS (1 << 19)
+#define PCP_OVERLAP_NOALLOC_MAX_PACKINGS 4096
+int pcp_overlap_count_noalloc(pcp_ctx* ctx, const double* pos, int pointDensity, int* counter, long long* sum);
 
 /* ---- read-back of the context's internal arrays for one packing (the getters
  *      getNeighborCells/getNeighborIndices/getBoxCounts/getNumNeighbors/

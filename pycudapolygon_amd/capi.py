@@ -74,6 +74,7 @@ SIGNATURES = {
     "pcp_shape_project": (_i, [_p, _p, _p, _p]),
     "pcp_max_effective_force": (_i, [_p, _p, _p, _p, _d, _p]),
     "pcp_overlap_count": (_i, [_p, _p, _i, _p, _p]),
+    "pcp_overlap_count_noalloc": (_i, [_p, _p, _i, _p, _p]),
     "pcp_get_cells": (_i, [_p, _i, _p, _p, _p]),
     "pcp_get_num_intersections": (_i, [_p, _i, _p]),
     "pcp_get_records": (_i, [_p, _i, _p, _p, _p, _p, _p]),

@@ -65,6 +65,7 @@ template <class T> void dfree(T*& p) {
 
 struct pcp_ctx {
     hipStream_t stream = nullptr;
+    int device = 0;      // the device current at pcp_create (the stream's)
     int P = 0, N = 0, V = 0;
     int Kp = 64;         // records per polygon bucket (grown on demand)
     int M = 100;         // reference maxNeighbors for the neighbour-list getters
@@ -198,6 +199,7 @@ int prof_begin(pcp_ctx* c, const char* name, int* slot);
 int prof_end(pcp_ctx* c, int slot);
 int prof_slot(pcp_ctx* c, const char* name, int* slot);
 int autotune(pcp_ctx* c, const pcp_state* st, int reps, double* timesUs);
+int scratch_setup(int device);
 }  // namespace pcp
 
 // launch `kern` on the context stream; when it is the profiled kernel, the launch carries a

@@ -1035,10 +1035,18 @@ static void free_topology(pcp_ctx* c) {
 
 int pcp_create(pcp_ctx** out) {
     pcp_ctx* c = new pcp_ctx();
-    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    hipError_t e = hipGetDevice(&c->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
         return pcp::fail_hip(e, "hipStreamCreate", __FILE__, __LINE__);
+    }
+    // the device address of the no-allocation entry point's scratch array, once per device
+    int rc = pcp::scratch_setup(c->device);
+    if (rc != PCP_OK) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
     }
     *out = c;
     return PCP_OK;

@@ -216,7 +216,56 @@ static int overlap_check(pcp_ctx* c, int D, long long* sum) {
     return PCP_OK;
 }
 
+// ---- the variant that allocates nothing (pcp_overlap_count_noalloc)
+// Its partial sums live in a fixed device array of this library.  In host code `scratch`
+// only names the symbol: its device address is looked up once per device with that device
+// current (scratch_setup, called when a context is created) and kept per device; every use
+// on the host goes through that address.
+constexpr long long OV_SCRATCH = (long long)PCP_OVERLAP_NOALLOC_MAX_CELLS + PCP_OVERLAP_NOALLOC_MAX_PACKINGS;
+static __device__ __align__(256) long long scratch[OV_SCRATCH];
+constexpr int OV_MAX_DEVICES = 64;
+static std::mutex g_scratchMutex;
+static long long* g_scratchPtr[OV_MAX_DEVICES] = {};
+
+int scratch_setup(int device) {
+    if (device < 0 || device >= OV_MAX_DEVICES) {
+        set_error("scratch_setup: device index out of range");
+        return PCP_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (g_scratchPtr[device]) return PCP_OK;
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return fail_hip(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+    g_scratchPtr[device] = (long long*)ptr;
+    return PCP_OK;
+}
+
+static int scratch_of(int device, long long** out) {
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (device < 0 || device >= OV_MAX_DEVICES || !g_scratchPtr[device]) {
+        set_error("pcp_overlap_count_noalloc: the device's scratch array was not set up");
+        return PCP_ERR_ARG;
+    }
+    *out = g_scratchPtr[device];
+    return PCP_OK;
+}
+
 }  // namespace pcp
+
+extern "C" int pcp_overlap_count_noalloc(pcp_ctx* c, const double* pos, int D, int* counter, long long* sum) {
+    using namespace pcp;
+    PCP_TRY(overlap_check(c, D, sum));
+    if (c->Ctot > PCP_OVERLAP_NOALLOC_MAX_CELLS || c->P > PCP_OVERLAP_NOALLOC_MAX_PACKINGS) {
+        set_error("pcp_overlap_count_noalloc: the context has " + std::to_string(c->Ctot) + " cells in " +
+                  std::to_string(c->P) + " packings; at most " + std::to_string(PCP_OVERLAP_NOALLOC_MAX_CELLS) +
+                  " cells and " + std::to_string(PCP_OVERLAP_NOALLOC_MAX_PACKINGS) + " packings");
+        return PCP_ERR_ARG;
+    }
+    long long* ptr = nullptr;
+    PCP_TRY(scratch_of(c->device, &ptr));
+    return overlap_run(c, pos, D, counter, sum, ptr);
+}
 
 extern "C" int pcp_overlap_count(pcp_ctx* c, const double* pos, int D, int* counter, long long* sum) {
     using namespace pcp;
