@@ -87,6 +87,9 @@ def _declare(L):
     L.orc_fire_minimize_ex.restype = _i
     L.orc_fire_minimize_ex.argtypes = [ctypes.POINTER(_System), _p, _p, _p, ctypes.POINTER(FireState),
                                        _d, _i, _d, _d, _d, _d, _d, _i, _i, _i, _d]
+    L.orc_update_positions.argtypes = [_i, _p, _p, _d]
+    L.orc_max_effective_force.restype = _d
+    L.orc_max_effective_force.argtypes = [_i, _p, _p, _p, _d]
     L.orc_det_sum.restype = _d
     L.orc_det_sum.argtypes = [_p, _i]
     L.orc_overlap_count.restype = ctypes.c_longlong
@@ -269,6 +272,19 @@ def xpbd_lanes(pos, start, kE, kA, kP, dt, nIter, tol=0.0, targetEdgeV=None, tar
     if it < 0:
         raise ValueError("XPBD takes polygons of at most 64 vertices")
     return pos, it
+
+
+def update_positions(pos, force, dt):
+    """updatePositionsKernel: frac(x + F dt)"""
+    pos = _f64(pos).copy()
+    lib().orc_update_positions(len(pos) // 2, _ptr(pos), _ptr(_f64(force)), float(dt))
+    return pos
+
+
+def max_effective_force(pos, tent, force, scale):
+    pos = _f64(pos)
+    return lib().orc_max_effective_force(len(pos) // 2, _ptr(pos), _ptr(_f64(tent)), _ptr(_f64(force)),
+                                         float(scale))
 
 
 def det_sum(x):

@@ -1566,6 +1566,28 @@ int orc_xpbd_lanes(int V, int N, const int* start, double* pos, const double* ta
     return worst;
 }
 
+/* ---------- GD pieces ----------
+ * updatePositionsKernel kernels.cuh:1433-1438: every coordinate x <- frac(x + F dt). */
+void orc_update_positions(int V, double* pos, const double* force, double dt) {
+    for (int q = 0; q < 2 * V; ++q) {
+        double p = pos[q] + force[q] * dt;
+        pos[q] = p - floor(p);
+    }
+}
+/* effectiveForceMagKernel kernels.cuh:625-633 with the max of getMaxEffectiveForceCUDA
+ * model.cu:286-306: max over vertices of |F_k + scale (x_k - tent_k)| (no wrap; scale = 1/dt for
+ * GD and 2/dt^2 for FIRE, Model::getMaxEffectiveForce model.cpp:332-335). */
+double orc_max_effective_force(int V, const double* pos, const double* tent, const double* force,
+                               double scale) {
+    double m = 0.0;
+    for (int k = 0; k < V; ++k) {
+        double fx = force[2 * k] + scale * (pos[2 * k] - tent[2 * k]);
+        double fy = force[2 * k + 1] + scale * (pos[2 * k + 1] - tent[2 * k + 1]);
+        m = fmax(m, sqrt(fx * fx + fy * fy));
+    }
+    return m;
+}
+
 /* ---------- Monte-Carlo overlap counter: updateOverlapAreaKernel kernels.cuh:879-979 ---------- */
 static double wrapPeriodic(double x) {   /* kernels.cuh:108-113 */
     x += 1.5;

@@ -86,6 +86,9 @@ int orc_xpbd(int V, int N, const int* start, double* pos, const double* targetEd
 int orc_xpbd_lanes(int V, int N, const int* start, double* pos, const double* targetEdgeV,
                    const double* targetEdgeP, const double* targetAreas, double kE, double kA,
                    double kP, double dt, int nIter, double tol);
+void orc_update_positions(int V, double* pos, const double* force, double dt);
+double orc_max_effective_force(int V, const double* pos, const double* tent, const double* force,
+                               double scale);
 double orc_det_sum(const double* x, int n);
 double orc_max_abs(const double* x, int n);
 long long orc_overlap_count(int V, int N, const int* start, const double* pos, int boxSize,

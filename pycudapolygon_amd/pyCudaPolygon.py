@@ -481,7 +481,9 @@ class model(lpcp.Model):
         self.setMaxEdgeLength(float(vals["maxEdgeLength"]))
         self.setStiffness(float(vals["stiffness"]))
         self.setCompressibility(float(vals["compressibility"]))
-        lpcp.Model.initializeNeighborCells(self)
+        # the reference calls its own initializeNeighborCells here, which resets the cell
+        # side to the 0.5 default after allocating (pyCudaPolygon.py:810, :54-57)
+        self.initializeNeighborCells()
         arrays = os.path.join(dirName, "arrays.npz")
         if os.path.exists(arrays):
             a = np.load(arrays, allow_pickle=False)
