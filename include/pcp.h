@@ -318,10 +318,18 @@ int pcp_stream_copy(pcp_ctx* ctx, double* dst, const double* src, long long n);
  * lane keeps its sums in order); only the speed differs. */
 int pcp_set_schedule(pcp_ctx* ctx, int pairLanes, int lensLanes, int exteriorLanes);
 int pcp_get_schedule(pcp_ctx* ctx, int* pairLanes, int* lensLanes, int* exteriorLanes);
+/* Detection schedule (no counterpart in the reference): 0 = one wavefront per polygon A lists,
+ * for each candidate polygon, the edges of each near the other's bounding circle and tests the
+ * products of the lists, candidates batched so that lanes stay busy; 1 = one wavefront per
+ * polygon A tests whole edge tiles, one candidate polygon per segment of lanes (polygons of at
+ * most 64 vertices); -1 = by the context's shape (tiles for 4 or more packings of polygons of
+ * at most 16 vertices).  Both give identical crossings; only the speed differs. */
+int pcp_set_detect_schedule(pcp_ctx* ctx, int mode);
+int pcp_get_detect_schedule(pcp_ctx* ctx, int* mode);
 /* autotuner: one evaluation of st fills the context's buffers, then every candidate width of
- * each kernel is timed over `reps` back-to-back launches and the fastest is kept.
- * timesUs host[9] (optional): microseconds per launch of pair 16/32/64, lens 16/32/64,
- * exterior 16/32/64 (0 = not a candidate). */
+ * each kernel, and each detection schedule, is timed over `reps` back-to-back launches and the
+ * fastest is kept.  timesUs host[11] (optional): microseconds per launch of pair 16/32/64,
+ * lens 16/32/64, exterior 16/32/64, detection 0/1 (0 = not a candidate). */
 int pcp_autotune(pcp_ctx* ctx, const pcp_state* st, int reps, double* timesUs);
 /* work of the last detection: polygon pairs tested (each polygon with itself, each pair
  * of distinct polygons whose bounding circles meet once) and the edge pairs whose segment

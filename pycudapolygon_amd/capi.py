@@ -95,6 +95,8 @@ SIGNATURES = {
     "pcp_set_schedule": (_i, [_p, _i, _i, _i]),
     "pcp_get_schedule": (_i, [_p, _p, _p, _p]),
     "pcp_autotune": (_i, [_p, ctypes.POINTER(PcpState), _i, _p]),
+    "pcp_set_detect_schedule": (_i, [_p, _i]),
+    "pcp_get_detect_schedule": (_i, [_p, _p]),
     "pcp_mpnn_sizes": (_i, [_i, _p, _p]),
     "pcp_mpnn_graph": (_i, [_p, _p, _p]),
     "pcp_mpnn_features": (_i, [_p, ctypes.POINTER(PcpState), _p, _p, _i, _p, _p]),
@@ -414,12 +416,23 @@ class System:
         self.ctx.call("pcp_get_schedule", ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
         return a.value, b.value, c.value
 
+    def set_detect_schedule(self, mode):
+        """0 = culled edge pairs, 1 = whole edge tiles, -1 = by the context's shape"""
+        self.ctx.call("pcp_set_detect_schedule", int(mode))
+
+    def get_detect_schedule(self):
+        m = ctypes.c_int()
+        self.ctx.call("pcp_get_detect_schedule", ctypes.byref(m))
+        return m.value
+
     def autotune(self, reps=20):
-        """time every lanes-per-polygon candidate of the per-record kernels, keep the fastest;
-        returns (schedule, microseconds per launch of pair/lens/exterior x 16/32/64)"""
-        t = np.zeros(9)
+        """time every lanes-per-polygon candidate of the per-record kernels and both detection
+        schedules, keep the fastest; returns (schedule, microseconds per launch of
+        pair/lens/exterior x 16/32/64); the detection times are kept in self.detect_us"""
+        t = np.zeros(11)
         check(lib().pcp_autotune(self.ctx.h, ctypes.byref(self._state), int(reps), _ptr(t)))
-        return self.get_schedule(), t.reshape(3, 3)
+        self.detect_us = t[9:11].copy()
+        return self.get_schedule(), t[:9].reshape(3, 3)
 
     def fire_run(self, prm, maxSteps, stepsPerBlock=16, graph=True):
         """FIRE in blocks of stepsPerBlock steps with one status read per block (pcp_fire_run)"""

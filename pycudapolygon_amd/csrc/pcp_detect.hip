@@ -453,7 +453,8 @@ enum { SRC_GLOBAL = 0, SRC_STAGE = 2 };
 // the reference's edge 1: updateNeighborsKernel tests edge id1 against larger ids)
 enum { TILE_SELF = 0, TILE_A_FIRST = 1 };
 
-__device__ __forceinline__ void stage_a(DetLds& L, int lane, int stA, int nA, int a0, int Bu,
+template <class LdsT>
+__device__ __forceinline__ void stage_a(LdsT& L, int lane, int stA, int nA, int a0, int Bu,
                                         const double2* __restrict__ pos, const int* __restrict__ cellOf, bool cells) {
     int na = min(64, nA - a0);
     wave_lds_sync();
@@ -843,6 +844,206 @@ __device__ void detect_polygon(
     if (lane == 0 && kmax > 0 && kmax > pmk) atomicMax(pmkSlot, (unsigned long long)kmax);
 }
 
+// ---------------------------------------------------------------- tiled polygon pairs
+// k_detect_tile, used when every polygon of the context has at most 64 vertices (the
+// segment width SEG is the smallest power of two >= the largest polygon, at least 8): one
+// wavefront per polygon A, A's edges staged in LDS.  The polygon pairs A owns -- A with
+// itself, and A with every polygon of larger id in the 3x3 block of polygon cells whose
+// circle meets A's -- are laid out one per segment of SEG lanes, 64/SEG pairs per pass;
+// lane b of a segment holds edge b of the pair's other polygon B in registers.  Every edge
+// a of A is then read from LDS by all lanes at once (a broadcast) and tested against every
+// lane's edge: all nA x nB edge pairs (the nA(nA-3)/2 non-adjacent ones for A itself) with
+// the division-free prefilter of the reference's segment test; a pass where some lane
+// passes it takes the exact test, the user-grid stencil filter and the record, as in
+// edge_tile.  There is no edge-circle cull: it removes only edge pairs that cannot cross
+// (the records are the same), and the regular tile costs fewer instructions than listing
+// and enumerating the culled pairs.
+// edge pairs that passed the prefilter, awaiting the exact test: (a, owner lane) per entry
+constexpr int TILE_MAY = 512;
+struct TileLds {
+    double2 P[64], R[64];   // A's edges: start and wrapped edge vector
+    int C[64];              // their user-grid cells (x | y << 16)
+    int uB[65], uSt[65], uN[65];   // the pairs of a pass group: other polygon, first vertex, vertices
+    uint16_t may[TILE_MAY];        // a | lane << 6
+};
+
+template <int CELLS, int SEG>
+__device__ void tile_pairs(TileLds& L, int lane, int nU, int A, int stA, int nA, int sAl, int s0, int Bu,
+                           const double2* __restrict__ pos, const int* __restrict__ cellOf,
+                           const DetOut& out, int* __restrict__ num, uint64_t& kmax) {
+    constexpr int S = 64 / SEG;
+    const int j = lane / SEG, b = lane & (SEG - 1);
+    for (int u0 = 0; u0 < nU; u0 += S) {
+        int u = u0 + j;
+        bool has = u < nU;
+        int Bq = has ? L.uB[u] : -1, stB = has ? L.uSt[u] : 0, nB = has ? L.uN[u] : 0;
+        bool self = Bq == A;
+        bool act = has && b < nB;
+        int bn = (b + 1 == nB) ? 0 : b + 1;
+        d2 Q = mk(0.0, 0.0), Sv = mk(0.0, 0.0);
+        int cb = 0;   // the user-grid cell of B's edge (for the stencil filter)
+        if (act) {
+            d2 q0 = ldpos(pos, stB + b), q1 = ldpos(pos, stB + bn);
+            Q = q0;
+            Sv = mk(wrap(q1.x - q0.x), wrap(q1.y - q0.y));
+            cb = CELLS ? cell_xy(cellOf[stB + b], Bu) : edge_cell_xy(q0, q1, Bu);
+        }
+        int nmay = 0;
+        // the exact test, the stencil filter and the record of the collected pairs, 64 at a
+        // time (one memory round trip for all their records): B's edge comes from the lane
+        // that holds it
+        auto flush = [&]() {
+            wave_lds_sync();
+            for (int f0 = 0; f0 < nmay; f0 += 64) {
+                bool valid = f0 + lane < nmay;
+                int e = valid ? (int)L.may[f0 + lane] : 0;
+                int a = e & 63, ow = e >> 6;
+                double qx = __shfl(Q.x, ow), qy = __shfl(Q.y, ow), sx = __shfl(Sv.x, ow), sy = __shfl(Sv.y, ow);
+                int cbo = __shfl(cb, ow);
+                int bo = ow & (SEG - 1), uo = u0 + ow / SEG;
+                uint64_t key = 0;
+                float2 tu;
+                bool rec = false;
+                if (valid && cells_adjacent(L.C[a], cbo, Bu)) {
+                    double2 pa = L.P[a], ra = L.R[a];
+                    SegNum sn = seg_numerators(mk(pa.x, pa.y), mk(ra.x, ra.y), mk(qx, qy), mk(sx, sy));
+                    rec = exact_record(sn, sAl, a, L.uB[uo] - s0, bo, key, tu);
+                }
+                if (rec) {
+                    kmax = (key > kmax) ? key : kmax;
+                    if (CELLS) atomicAdd(&num[stA + a], 1);   // per-edge count of edge 1 (A's)
+                    emit_record(key, tu, s0, out);
+                }
+            }
+            wave_lds_sync();   // the list's reads are issued before it is refilled
+            nmay = 0;
+        };
+        for (int a0 = 0; a0 < nA; a0 += TILE_UNROLL) {
+            bool may[TILE_UNROLL];
+            bool anyMay = false;
+#pragma unroll
+            for (int k = 0; k < TILE_UNROLL; ++k) {
+                int a = a0 + k;
+                bool ok = act && a < nA;
+                if (self) ok = ok && b > a && b != a + 1 && !(a == 0 && b == nA - 1);
+                int al = (a < nA) ? a : 0;
+                double2 pa = L.P[al], ra = L.R[al];
+                SegNum sn = seg_numerators(mk(pa.x, pa.y), mk(ra.x, ra.y), Q, Sv);
+                may[k] = ok && seg_may_cross(sn);
+                anyMay = anyMay || may[k];
+            }
+            if (__ballot(anyMay) == 0ull) continue;   // wavefront-uniform: most blocks find nothing
+            if (nmay > TILE_MAY - TILE_UNROLL * 64) flush();
+            unsigned long long below = bits_below(lane);
+#pragma unroll
+            for (int k = 0; k < TILE_UNROLL; ++k) {
+                unsigned long long m = __ballot(may[k]);
+                if (may[k]) L.may[nmay + __popcll(m & below)] = (uint16_t)((a0 + k) | (lane << 6));
+                nmay += __popcll(m);
+            }
+        }
+        if (nmay) flush();
+    }
+}
+
+template <int CELLS, int SEG>
+__global__ __launch_bounds__(BLK) void k_detect_tile(
+    int N, const int* __restrict__ polyStart, const int* __restrict__ polyPack,
+    const int* __restrict__ packPoly, const int* __restrict__ box, const double2* __restrict__ pos,
+    const int* __restrict__ cellOf, const double4* __restrict__ bounds, const int* __restrict__ pcStart,
+    const int* __restrict__ pcSorted, const double4* __restrict__ sBounds, const int* __restrict__ packB, int Kp,
+    int* __restrict__ bcnt, uint64_t* __restrict__ bKey, float2* __restrict__ bTU, int* __restrict__ jcnt,
+    uint64_t* __restrict__ jKey, unsigned long long* __restrict__ packMaxKey, long long pmkPitch,
+    int* __restrict__ num, int* __restrict__ scal) {
+    __shared__ TileLds lds[BLK / 64];
+    int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int idx = xcd_block(blockIdx.x, gridDim.x) * (BLK / 64) + w;
+    if (idx >= N) return;   // whole wavefront
+    TileLds& L = lds[w];
+    DetOut out{Kp, bcnt, bKey, bTU, jcnt, jKey, scal};
+    int A = __builtin_amdgcn_readfirstlane(pcSorted[idx]);
+    int stA = polyStart[A], nA = polyStart[A + 1] - stA;
+    int pk = polyPack[A];
+    int Bu = box[pk];
+    int s0 = packPoly[pk];
+    int pb = packB[pk];
+    int Bp = pb & 0xFFFF;
+    bool big = (pb >> 30) & 1;
+    int sAl = A - s0;
+    double4 bA = bounds[A];
+    uint64_t kmax = 0;
+    // candidate cells: lanes 0..8 hold the ranges of the 3x3 block (one cell when Bp == 1)
+    long long gbase = (long long)pk * (PC_CELLS + 1);
+    int ncell = (Bp == 1) ? 1 : 9;
+    int lo = 0, cnt = 0;
+    if (lane < ncell) {
+        int ax = pc_coord(bA.x, Bp), ay = pc_coord(bA.y, Bp);
+        int cx = ax, cy = ay;
+        if (Bp > 1) {
+            cx = ax + lane / 3 - 1; cx += (cx < 0) ? Bp : 0; cx -= (cx >= Bp) ? Bp : 0;
+            cy = ay + lane % 3 - 1; cy += (cy < 0) ? Bp : 0; cy -= (cy >= Bp) ? Bp : 0;
+        }
+        int cell = cy * Bp + cx;
+        lo = pcStart[gbase + cell];
+        cnt = pcStart[gbase + cell + 1] - lo;
+    }
+    stage_a(L, lane, stA, nA, 0, Bu, pos, cellOf, CELLS != 0);
+    int incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+        int y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    int T = __shfl(incl, ncell - 1);
+    bool selfPair = nA >= 4;   // a polygon of fewer vertices has no non-adjacent edge pair
+    for (int c0 = 0; c0 < T || (c0 == 0 && selfPair); c0 += 64) {
+        int ci = c0 + lane;
+        int q = 0;
+        bool found = false;
+        for (int k = 0; k < ncell; ++k) {   // every lane takes part in the shuffles
+            int hiK = __shfl(incl, k), cK = __shfl(cnt, k), loK = __shfl(lo, k);
+            if (!found && ci < hiK) { q = loK + ci - (hiK - cK); found = true; }
+        }
+        bool keep = false;
+        int cand = -1;
+        if (ci < T) {
+            cand = pcSorted[s0 + q];
+            keep = cand > A && (big || circles_meet(bA, sBounds[s0 + q]));
+        }
+        int stq = 0, nq = 0;
+        if (keep) {
+            stq = polyStart[cand];
+            nq = polyStart[cand + 1] - stq;
+        }
+        unsigned long long km = __ballot(keep);
+        int first = (c0 == 0 && selfPair) ? 1 : 0;
+        int nU = first + __popcll(km);
+        wave_lds_sync();   // the previous group's unit reads are issued
+        if (first && lane == 0) {
+            L.uB[0] = A;
+            L.uSt[0] = stA;
+            L.uN[0] = nA;
+        }
+        if (keep) {
+            int u = first + __popcll(km & bits_below(lane));
+            L.uB[u] = cand;
+            L.uSt[u] = stq;
+            L.uN[u] = nq;
+        }
+        wave_lds_sync();
+        tile_pairs<CELLS, SEG>(L, lane, nU, A, stA, nA, sAl, s0, Bu, pos, cellOf, out, num, kmax);
+    }
+    // the largest key of the packing (the pairing's last-record case)
+    unsigned long long* pmkSlot = packMaxKey + (A & (PMK_SLOTS - 1)) * pmkPitch + pk;
+    unsigned long long pmk = *pmkSlot;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        unsigned long long y = __shfl_xor((unsigned long long)kmax, o);
+        kmax = (y > kmax) ? y : kmax;
+    }
+    if (lane == 0 && kmax > 0 && kmax > pmk) atomicMax(pmkSlot, (unsigned long long)kmax);
+}
+
 // One wavefront per polygon A (in cell order, XCD-contiguous): detect_polygon.
 // CELLS = 0: the user-grid cell of every edge is computed from the live positions (the
 // fused evaluation, where cells and crossings come from the same positions);
@@ -914,6 +1115,51 @@ int launch_bounds(pcp_ctx* c, const double2* pos, const PostShake* ps) {
     return launch_bounds_t<false>(c, pos, none);
 }
 
+// k_detect_tile<CELLS, SEG> on the context's stream; its launches are profiled under the
+// name "k_detect_tile" (pcp_profile_kernel), with events attached to the dispatch
+template <int CELLS, int SEG>
+static int launch_tile_t(pcp_ctx* c, const double2* pos, int grid) {
+    int slot = -1;
+    PCP_TRY(prof_slot(c, "k_detect_tile", &slot));
+    auto kern = k_detect_tile<CELLS, SEG>;
+    if (slot >= 0)
+        hipExtLaunchKernelGGL(kern, dim3(grid), dim3(BLK), 0, c->stream, c->profEvents[slot].first,
+                              c->profEvents[slot].second, 0, c->N, c->dPolyStart, c->dPolyPack, c->dPackPoly,
+                              c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted, c->dSBounds, c->dPackB,
+                              c->Kp, c->dBcnt, c->dBKey, c->dBTU, c->dJcnt, c->dJKey, c->dPackMaxKey, c->pmkPitch,
+                              c->dNum, c->dScal);
+    else
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(BLK), 0, c->stream, c->N, c->dPolyStart, c->dPolyPack,
+                           c->dPackPoly, c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted,
+                           c->dSBounds, c->dPackB, c->Kp, c->dBcnt, c->dBKey, c->dBTU, c->dJcnt, c->dJKey,
+                           c->dPackMaxKey, c->pmkPitch, c->dNum, c->dScal);
+    PCP_LAUNCHED();
+    return PCP_OK;
+}
+template <int CELLS>
+static int launch_tile_c(pcp_ctx* c, const double2* pos, int grid) {
+    switch (c->seg) {
+    case 4:
+    case 8: return launch_tile_t<CELLS, 8>(c, pos, grid);
+    case 16: return launch_tile_t<CELLS, 16>(c, pos, grid);
+    case 32: return launch_tile_t<CELLS, 32>(c, pos, grid);
+    default: return launch_tile_t<CELLS, 64>(c, pos, grid);
+    }
+}
+static int launch_detect_tile(pcp_ctx* c, const double2* pos, bool cellsFromList, int grid) {
+    return cellsFromList ? launch_tile_c<1>(c, pos, grid) : launch_tile_c<0>(c, pos, grid);
+}
+
+// the detection schedule in effect: the one set (pcp_set_detect_schedule / pcp_autotune), else
+// tiles for contexts of several packings of small polygons (there the kernel is bound by its
+// instruction issue, and the tile needs fewer instructions than the culled enumeration) and the
+// culled enumeration otherwise (one packing is bound by each wavefront's latency, which the
+// cull shortens; with 32-lane segments small polygons leave tile lanes idle)
+int detect_schedule(const pcp_ctx* c) {
+    if (c->detSched >= 0) return (c->maxN <= 64) ? c->detSched : 0;
+    return (c->maxN <= 64 && c->P >= 4 && c->seg <= 16) ? 1 : 0;
+}
+
 // cellsFromList: use the cells of the last run_cells (API updateNeighbors).  ps: FIRE's
 // post-SHAKE step, done by the bounds kernel (see k_bounds)
 int run_detect(pcp_ctx* c, const double2* pos, int cellsFromList, const PostShake* ps) {
@@ -923,6 +1169,7 @@ int run_detect(pcp_ctx* c, const double2* pos, int cellsFromList, const PostShak
     PCP_LAUNCH(c, k_poly_cells, dim3(c->P), dim3(PP_BLK), c->dPackPoly, c->dBounds, c->dPcStart, c->dPcSorted,
                c->dPackB, c->dSBounds);
     int grid = (c->N + BLK / 64 - 1) / (BLK / 64);
+    if (c->maxN <= 64 && detect_schedule(c) == 1) return launch_detect_tile(c, pos, cellsFromList != 0, grid);
     if (cellsFromList)
         PCP_LAUNCH(c, k_detect_poly<1>, dim3(grid), dim3(BLK), c->N, c->dPolyStart, c->dPolyPack, c->dPackPoly,
                    c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted, c->dSBounds, c->dPackB, c->Kp,

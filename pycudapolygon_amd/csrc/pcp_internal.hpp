@@ -143,6 +143,9 @@ struct pcp_ctx {
     // k_pair_poly, k_lens_energy (16, 32 or 64) and k_exterior_poly (0 = the widest packing
     // of polygons per wavefront the LDS allows)
     int segPair = 16, segLens = 16, segExt = 0;
+    // detection schedule: 0 = culled edge pairs (k_detect_poly), 1 = whole edge tiles of polygon
+    // pairs (k_detect_tile, polygons of at most 64 vertices), -1 = by the context's shape
+    int detSched = -1;
     // multi-step FIRE (pcp_fire_run): force backups for device-side rollback, a block-start
     // snapshot (x, v, F; per packing dt, alpha, energy, max|F| and nPos, steps, active), the
     // largest bucket count seen in the block, and the captured graph of one block of steps
@@ -172,6 +175,7 @@ int launch_det_reduce(pcp_ctx* c, const double* partials, int nArrays, double* o
 int run_cells(pcp_ctx* c, const double2* pos);
 struct PostShake;   // FIRE's post-SHAKE step, done by the detection's bounds kernel (pcp_detect.hip)
 int run_detect(pcp_ctx* c, const double2* pos, int cellsFromList, const PostShake* ps = nullptr);
+int detect_schedule(const pcp_ctx* c);
 int detect_alloc(pcp_ctx* c);
 int detect_stats(pcp_ctx* c, long long* polygonPairs, long long* edgePairs);
 int seg_width(int maxN);

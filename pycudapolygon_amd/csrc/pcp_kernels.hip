@@ -1482,6 +1482,16 @@ int pcp_profile_read(pcp_ctx* c, double* totalMs, int* launches) {
     *launches = (int)c->profUsed;
     return PCP_OK;
 }
+int pcp_set_detect_schedule(pcp_ctx* c, int mode) {
+    if (mode < -1 || mode > 1) return fail_arg("pcp_set_detect_schedule: mode must be -1, 0 or 1");
+    c->detSched = mode;
+    c->graphK = 0;   // a captured FIRE block holds the old kernels
+    return PCP_OK;
+}
+int pcp_get_detect_schedule(pcp_ctx* c, int* mode) {
+    *mode = detect_schedule(c);
+    return PCP_OK;
+}
 int pcp_set_schedule(pcp_ctx* c, int pairLanes, int lensLanes, int exteriorLanes) {
     auto ok = [](int w, bool zero) { return w == 16 || w == 32 || w == 64 || (zero && w == 0); };
     if (!ok(pairLanes, false) || !ok(lensLanes, false) || !ok(exteriorLanes, true)) {

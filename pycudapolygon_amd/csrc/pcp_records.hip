@@ -912,7 +912,7 @@ static int time_launches(pcp_ctx* c, int reps, double* us, const std::function<i
 
 int autotune(pcp_ctx* c, const pcp_state* st, int reps, double* timesUs) {
     const int W[3] = {16, 32, 64};
-    double t[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double t[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     int mt = st->model & 0xFF;
     bool overlap = (mt == PCP_NORMAL || mt == PCP_HYBRID);
     if (overlap && c->N > 0) {
@@ -945,13 +945,22 @@ int autotune(pcp_ctx* c, const pcp_state* st, int reps, double* timesUs) {
         c->segPair = best(0, keepPair);
         c->segLens = sq ? best(3, keepLens) : keepLens;
         c->segExt = best(6, keepExt);
+        // detection: the culled enumeration against whole edge tiles (bounds, cells and
+        // detection timed together; each run starts from cleared buckets)
+        int keepDet = c->detSched;
+        const double2* xd = (const double2*)st->pos;
+        for (int m = 0; m < (c->maxN <= 64 ? 2 : 1); ++m) {
+            c->detSched = m;
+            PCP_TRY(time_launches(c, reps, &t[9 + m], [&]() { return run_detect(c, xd, 0); }));
+        }
+        c->detSched = (t[10] > 0.0 && t[10] < t[9]) ? 1 : (t[9] > 0.0 ? 0 : keepDet);
         c->graphK = 0;   // a captured FIRE block holds the old schedule
         // leave the buffers as a plain evaluation with the chosen schedule leaves them
         PCP_TRY(pcp_force_eval(c, st));
         PCP_TRY(pcp_synchronize(c));
     }
     if (timesUs)
-        for (int k = 0; k < 9; ++k) timesUs[k] = t[k];
+        for (int k = 0; k < 11; ++k) timesUs[k] = t[k];
     return PCP_OK;
 }
 

@@ -1,6 +1,8 @@
 """Schedules of the per-record kernels (lanes per polygon of k_pair_poly, k_lens_energy and
-k_exterior_poly) and the autotuner that picks them: every schedule must give the same forces,
-energies and records bit for bit, and the autotuner keeps one of the candidates it timed."""
+k_exterior_poly), the two detection schedules (culled edge pairs, k_detect_poly; whole edge
+tiles, k_detect_tile) and the autotuner that picks them: every schedule must give the same
+forces, energies and records bit for bit, and the autotuner keeps one of the candidates it
+timed."""
 import itertools
 
 import numpy as np
@@ -40,6 +42,27 @@ def test_every_schedule_gives_identical_results(model):
             np.testing.assert_array_equal(a, b, err_msg=f"schedule {(pair, lens, ext)}")
 
 
+@pytest.mark.parametrize("shape", ["c1", "n16x4", "uneven"])
+def test_detection_schedules_give_identical_results(shape):
+    from pycudapolygon_amd.generate import make_packing
+    if shape == "c1":
+        pk = [make_packing(64, 12, 0.80, seed=3)]
+    elif shape == "n16x4":
+        pk = [make_packing(1024, 16, 0.85, seed=k) for k in range(4)]
+    else:
+        pk = make_batch(6, 44, N_range=(64, 512), n_range=(5, 32))
+    for model in (capi.NORMAL, SQUARED):
+        s = capi.System(pk, [box_size_for(p["maxEdgeLength"]) for p in pk], model=model)
+        out = []
+        for mode in (0, 1):
+            s.set_detect_schedule(mode)
+            assert s.get_detect_schedule() == mode
+            out.append(evaluate(s) + tuple(s.get_records(k)["intersections"] for k in range(len(pk))))
+        for a, b in zip(*out):
+            np.testing.assert_array_equal(a, b, err_msg=f"{shape} model {model}")
+        s.close()
+
+
 def test_autotune_keeps_a_timed_candidate():
     pk = make_batch(8, 43, N_range=(256, 1024), n_range=(8, 32), min_center_frac=0.5)
     s = capi.System(pk, [box_size_for(p["diameter"]) for p in pk], model=SQUARED)
@@ -51,5 +74,7 @@ def test_autotune_keeps_a_timed_candidate():
     assert t[0, [16, 32, 64].index(pair)] == t[0].min()
     assert t[1, [16, 32, 64].index(lens)] == t[1].min()
     assert t[2, [16, 32, 64].index(ext)] == t[2][t[2] > 0].min()
+    d = s.detect_us
+    assert (d > 0).all() and s.get_detect_schedule() == int(np.argmin(d))
     for a, b in zip(ref, evaluate(s)):
         np.testing.assert_array_equal(a, b)
