@@ -43,26 +43,156 @@ BYTES_PER_VERTEX = 16
 BYTES_PER_POLYGON = 44
 BYTES_PER_CROSSING = 24
 DETECT_KERNEL = "k_detect_poly<0>"
-# HBM bytes per k_detect_poly<0> launch on this workload, measured with rocprofv3 FETCH_SIZE
-# and WRITE_SIZE and scaled by a copy kernel's readings (profiles/k_detect_poly_hbm_traffic_N4096.md)
-DETECT_TRAFFIC_BYTES = 4228712
+# HBM bytes per k_detect_poly<0> launch on this workload when the in-run measurement below
+# cannot run: rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a copy kernel's readings, from
+# profiles/k_detect_poly_hbm_traffic_N4096.md (the command is given there)
+DETECT_TRAFFIC_CITED = 4228712
+DETECT_TRAFFIC_SOURCE = "profiles/k_detect_poly_hbm_traffic_N4096.md"
+COPY_DOUBLES = 1 << 25
 
 
-def cpu_baseline(p, B, budget_s):
+def _pmc_pass(counter, child_args, kernel_prefix, timeout_s=90):
+    """One rocprofv3 --pmc pass over a child run of this script; returns the counter's mean per
+    dispatch (KiB) over the dispatches whose kernel name starts with kernel_prefix."""
+    import csv
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        raise RuntimeError("rocprofv3 not found")
+    d = tempfile.mkdtemp(prefix="pcp_pmc_", dir="/tmp")
+    cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--",
+           sys.executable, "-u", os.path.abspath(__file__)] + child_args
+    env = dict(os.environ, TMPDIR="/tmp")
+    p = subprocess.Popen(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, env=env, cwd="/tmp",
+                         start_new_session=True)
+    try:
+        rc = p.wait(timeout=timeout_s)
+    except subprocess.TimeoutExpired:
+        os.killpg(p.pid, signal.SIGKILL)
+        p.wait()
+        raise RuntimeError(f"rocprofv3 --pmc {counter} timed out")
+    vals = []
+    for root, _, files in os.walk(d):
+        for f in files:
+            if f.endswith("counter_collection.csv"):
+                with open(os.path.join(root, f)) as fh:
+                    for r in csv.DictReader(fh):
+                        name = r["Kernel_Name"].replace("void ", "").replace("pcp::", "")
+                        if name.startswith(kernel_prefix) and r["Counter_Name"] == counter:
+                            vals.append(float(r["Counter_Value"]))
+    shutil.rmtree(d, ignore_errors=True)
+    if not vals:
+        raise RuntimeError(f"rocprofv3 --pmc {counter} (rc {rc}) recorded no {kernel_prefix} dispatch")
+    return sum(vals) / len(vals), len(vals)
+
+
+def measure_traffic(kernel, seed):
+    """HBM bytes per launch of `kernel` on the N=4096 workload: FETCH_SIZE and WRITE_SIZE of the
+    kernel (20 force evaluations in a child run) and of the plain copy kernel k_copy on 2^25
+    doubles, each scaled by the copy's true bytes over its reading."""
+    run = ["--steps", "20", "--warmup", "2", "--no-fire", "--no-cpu-baseline", "--traffic", "cited",
+           "--seed", str(seed)]
+    cal = ["--calibrate-copy", str(COPY_DOUBLES)]
+    true_kib = 8 * COPY_DOUBLES / 1024
+    out = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        kib, n = _pmc_pass(counter, run, kernel)
+        ckib, _ = _pmc_pass(counter, cal, "k_copy")
+        factor = true_kib / ckib
+        out[counter] = dict(kib_per_dispatch=kib, dispatches=n, copy_kib=ckib, factor=factor,
+                            bytes=kib * 1024 * factor)
+    total = out["FETCH_SIZE"]["bytes"] + out["WRITE_SIZE"]["bytes"]
+    return total, out
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_worker(seconds, seed):
+    """One timing run of the oracle's force evaluation on the N=4096 workload, in a process of its
+    own (PCP_ORACLE_LIB picks the build, OMP_NUM_THREADS the threads); prints one JSON line with
+    the rate and a digest of the first evaluation's energy and forces."""
+    import hashlib
     from oracle import oracle as O
+    from pycudapolygon_amd.generate import make_packing, box_size_for
+    p = make_packing(4096, 16, 0.85, seed=seed)
+    B = box_size_for(p["diameter"])
     sysO = O.System(p["startIndices"], B, "normal", p["targetEdgeV"], p["targetAreas"])
-    sysO.force_eval(p["positions"])   # warm-up (page-in, thread pool)
+    E, F, _ = sysO.force_eval(p["positions"])   # also the warm-up (page-in, thread pool)
+    digest = hashlib.sha256(np.float64(E).tobytes() + F.tobytes()).hexdigest()[:16]
     n, t0 = 0, time.perf_counter()
     while True:
         sysO.force_eval(p["positions"])
         n += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or n >= 1000:
+        if el >= seconds or n >= 1000:
             break
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=n / el, unit="force evals/s", cores=threads, kind="port",
-                sample=f"{n} force evaluations of the same N=4096 workload ({el:.1f} s), "
-                       f"oracle/pcp_oracle.c with OpenMP ({threads} threads)")
+    print(json.dumps(dict(evals=n, seconds=el, rate=n / el, digest=digest, lib=O._LIB_PATH)), flush=True)
+
+
+def _run_worker(lib, threads, seconds, seed):
+    import subprocess
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    if lib:
+        env["PCP_ORACLE_LIB"] = lib
+    else:
+        env.pop("PCP_ORACLE_LIB", None)
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-worker", str(seconds), "--seed", str(seed)],
+                       env=env, capture_output=True, text=True, timeout=seconds + 300)
+    if r.returncode != 0:
+        raise RuntimeError(f"CPU worker failed ({r.returncode}): {r.stderr[-500:]}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline(seed, budget_s):
+    """The oracle's force evaluation timed on this host's cores: the -O3 -march=native timing
+    build (oracle/Makefile `timing`, compiled here for this CPU) at all threads for budget_s and
+    on one thread, checked bit for bit against the parity build (-O2 x86-64-v2), which is timed
+    briefly too. The parity build is the reported value if the timing build does not compile or
+    differs."""
+    import subprocess
+    threads = int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0))
+    out = os.path.join("/tmp", f"pcp_oracle_native_{os.getpid()}")
+    native = os.path.join(out, "liboracle.so")
+    try:
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "timing", f"TOUT={out}"], check=True,
+                       capture_output=True, timeout=300)
+    except Exception as e:   # no compiler on this host: time the parity build only
+        print(f"[bench] CPU timing build failed ({e}); timing the parity build", flush=True)
+        native = None
+    par = _run_worker(None, threads, min(5.0, budget_s), seed)
+    res = dict(unit="force evals/s", kind="port", cpu_model=_cpu_model(), logical_cpus=os.cpu_count(),
+               parity_build=dict(flags="-O2 -fno-tree-vectorize -march=x86-64-v2 -ffp-contract=off", threads=threads,
+                                 value=par["rate"], evals=par["evals"], seconds=par["seconds"]))
+    nat = _run_worker(native, threads, budget_s, seed) if native else None
+    if nat is not None and nat["digest"] == par["digest"]:
+        one = _run_worker(native, 1, budget_s / 2, seed)
+        res.update(value=nat["rate"], cores=threads, build="-O3 -march=native -fno-tree-slp-vectorize "
+                   "-ffp-contract=off (oracle/Makefile timing), results equal to the parity build bit for bit",
+                   single_thread=dict(value=one["rate"], evals=one["evals"], seconds=one["seconds"]),
+                   sample=f"{nat['evals']} force evaluations of the N=4096 workload ({nat['seconds']:.1f} s) on "
+                          f"{threads} OpenMP threads; {one['evals']} on one thread ({one['seconds']:.1f} s)")
+    else:
+        why = "did not compile" if nat is None else "differed from the parity build"
+        one = _run_worker(None, 1, budget_s / 2, seed)
+        res.update(value=par["rate"], cores=threads, build=f"parity build (the timing build {why})",
+                   single_thread=dict(value=one["rate"], evals=one["evals"], seconds=one["seconds"]),
+                   sample=f"{par['evals']} force evaluations of the N=4096 workload ({par['seconds']:.1f} s) on "
+                          f"{threads} OpenMP threads; {one['evals']} on one thread")
+    if native:
+        subprocess.run(["rm", "-rf", out], check=False)
+    return res
 
 
 def fire_one(packings, dtMax, maxSteps, warm=None, budget_s=None, graph=True, dt0=1e-3, system=None):
@@ -313,10 +443,17 @@ def main():
     ap.add_argument("--no-fire", action="store_true")
     ap.add_argument("--fire-legs", default="n4096,xpbd,n4096_batch,n1024,uneven,compression",
                     help="comma-separated FIRE legs to run (for profiling one of them)")
+    ap.add_argument("--traffic", choices=["measure", "cited"], default="measure",
+                    help="HBM traffic of the roofline kernel: measured by child rocprofv3 --pmc passes, or the "
+                         "cited profile figure")
+    ap.add_argument("--cpu-worker", type=float, default=None, metavar="SECONDS", help=argparse.SUPPRESS)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="N",
                     help="only run the plain copy kernel on N doubles (10 launches) for HBM counter "
                          "calibration, then exit")
     args = ap.parse_args()
+    if args.cpu_worker is not None:
+        cpu_worker(args.cpu_worker, args.seed)
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; the multi-GPU design is described in DESIGN.md "
               "(--gpus must be 1)", file=sys.stderr)
@@ -375,12 +512,30 @@ def main():
     if args.roofline_kernel == DETECT_KERNEL and nl.value > 0:
         flops = FLOPS_PER_EDGE_PAIR * tests
         nbytes = BYTES_PER_VERTEX * s.V + BYTES_PER_POLYGON * s.N + BYTES_PER_CROSSING * nI
-        achieved = flops / kavg_s / 1e12
-        roof = dict(bound="mfma", compute="fp64 VALU (no MFMA in the geometry); peak = FP64 78.6 TFLOP/s",
-                    achieved=achieved, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s", frac=achieved / FP64_PEAK_TFLOPS,
-                    traffic=DETECT_TRAFFIC_BYTES, kernel=DETECT_KERNEL, flops_per_launch=flops, edge_pair_tests=tests,
-                    polygon_pairs=ppairs.value, algorithmic_bytes_per_launch=nbytes, avg_launch_us=kavg_s * 1e6,
-                    hbm_gbs_algorithmic=nbytes / kavg_s / 1e9, hbm_gbs_measured_traffic=DETECT_TRAFFIC_BYTES / kavg_s / 1e9)
+        gbs = nbytes / kavg_s / 1e9
+        traffic, tsrc, tdetail = DETECT_TRAFFIC_CITED, f"cited: {DETECT_TRAFFIC_SOURCE}", None
+        if args.traffic == "measure":
+            try:
+                traffic, tdetail = measure_traffic(DETECT_KERNEL, args.seed)
+                traffic = int(round(traffic))
+                tsrc = ("measured in this run: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over 20 evaluations, "
+                        "each scaled by k_copy's true bytes / reading")
+            except Exception as e:   # profiler missing or refused: keep the cited figure
+                print(f"[bench] traffic measurement failed ({e}); using the cited figure", flush=True)
+        roof = dict(bound="hbm", achieved=gbs, peak=HBM_PEAK_GBS, unit="GB/s", frac=gbs / HBM_PEAK_GBS,
+                    traffic=traffic, traffic_source=tsrc, traffic_counters=tdetail,
+                    traffic_over_algorithmic=traffic / nbytes, kernel=DETECT_KERNEL,
+                    algorithmic_bytes_per_launch=nbytes, avg_launch_us=kavg_s * 1e6,
+                    fp64=dict(achieved_tflops=flops / kavg_s / 1e12, peak_tflops=FP64_PEAK_TFLOPS,
+                              frac=flops / kavg_s / 1e12 / FP64_PEAK_TFLOPS, flops_per_launch=flops,
+                              note="FP64 VALU (no MFMA in the geometry): 21 FP64 operations per edge-pair test"),
+                    edge_pair_tests=tests, polygon_pairs=ppairs.value,
+                    limiter="neither bound: the launch is one round of resident wavefronts (4096 waves, 16 per "
+                            "CU) and lasts as long as its slowest wavefront's chain of dependent memory "
+                            "round trips (profiles/k_detect_poly_counters_N4096.md)")
+        print(f"[bench] roofline {DETECT_KERNEL}: {gbs:.1f} GB/s algorithmic ({100 * gbs / HBM_PEAK_GBS:.2f} % of "
+              f"HBM), traffic {traffic} B/launch = {traffic / nbytes:.2f}x algorithmic ({tsrc.split(':')[0]})",
+              flush=True)
     else:
         roof = dict(kernel=args.roofline_kernel, avg_launch_us=kavg_s * 1e6, achieved=None, peak=None,
                     unit=None, frac=None, traffic=None, bound=None)
@@ -391,8 +546,9 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(p, B, args.cpu_seconds)
-        print(f"[bench] CPU oracle: {cpu['value']:.2f} evals/s on {cpu['cores']} threads", flush=True)
+        cpu = cpu_baseline(args.seed, args.cpu_seconds)
+        print(f"[bench] CPU oracle ({cpu['cpu_model']}): {cpu['value']:.2f} evals/s on {cpu['cores']} threads, "
+              f"{cpu['single_thread']['value']:.2f} on one; {cpu['build']}", flush=True)
 
     out = {
         "metric": METRIC, "value": evals_per_s, "unit": "force evals/s", "n_gpus": 1, "steps": args.steps,

@@ -11,7 +11,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+# PCP_ORACLE_LIB names another build of the same source (the CPU baseline's timing build)
+_LIB_PATH = os.environ.get("PCP_ORACLE_LIB") or os.path.join(_HERE, "build", "liboracle.so")
 
 NORMAL, EDGE_ONLY, AREA_ONLY, SOFT_BODY, ABNORMAL, HYBRID = range(6)
 MODEL_IDS = {"normal": NORMAL, "edgeOnly": EDGE_ONLY, "areaOnly": AREA_ONLY,
