@@ -388,7 +388,8 @@ def fire_legs(args):
         wc = W.WarmStart(W.load_weights(W.COMPRESS_WEIGHTS))
         out["compression"] = {}
         for tag, warm, dt0, shape in (("plain", None, 1e-3, False), ("plain_dt0_100", None, 100.0, False),
-                                      ("plain_dt0_100_shape_match", None, 100.0, True), ("learned", wc, 1e-3, False)):
+                                      ("plain_dt0_100_shape_match", None, 100.0, True), ("learned", wc, 1e-3, False),
+                                      ("learned_shape_match", wc, 1e-3, True)):
             steps, s5 = compress.ramp(p5, 0.88, 1e-3, dict(fTol=1e-10, dtMax=W.DT_MAX), 5000, warm=warm, dt0=dt0,
                                       budget_s=args.compress_seconds, graph=not args.no_graph, shape_match=shape)
             conv = all(all(r["converged"]) for r in steps)

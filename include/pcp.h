@@ -337,6 +337,15 @@ int pcp_autotune(pcp_ctx* ctx, const pcp_state* st, int reps, double* timesUs);
  * of the counts of edges of each near the other's circle, or nA*nB where the cull does
  * not apply).  The last detection's position buffer must still be valid. */
 int pcp_detect_stats(pcp_ctx* ctx, long long* polygonPairs, long long* edgePairs);
+/* records of the last pairing (pcp_force_eval, pcp_update_outersections) that belong to a
+ * pair of distinct polygons with an odd number of crossings.  The boundary of every overlap
+ * crosses the other polygon's boundary an even number of times; an odd count arises where a
+ * vertex lies within rounding of the other polygon's edge and the segment test
+ * (updateNeighborsKernel kernels.cuh:760-790, kept bit for bit) counts the crossing of one of
+ * the vertex's two edges and not of the other.  Such a pair's records are then paired with
+ * wrong exits and its overlap energy jumps.  A diagnostic with no counterpart in the
+ * reference (which has the same window): 0 in a healthy state. */
+int pcp_odd_crossing_records(pcp_ctx* ctx, int* count);
 
 /* ---- learned warm start (no counterpart in the reference, which has no learned
  *      component: an addition in front of the plain FIRE method) ----

@@ -183,6 +183,20 @@ def outersections(start, nbr):
                 shapeCounts=sc[:N], numIntersections=I)
 
 
+def odd_crossing_records(intersections):
+    """Records (keys [sj|si|i|j]) that belong to a pair of distinct polygons with an odd number
+    of crossings: the check that pcp_odd_crossing_records makes on the device.  A closed
+    overlap boundary crosses the other polygon's boundary an even number of times."""
+    k = np.asarray(intersections, np.uint64)
+    si = ((k >> np.uint64(32)) & np.uint64(0xFFFF)).astype(np.int64)
+    sj = ((k >> np.uint64(48)) & np.uint64(0xFFFF)).astype(np.int64)
+    lo, hi = np.minimum(si, sj), np.maximum(si, sj)
+    pair = lo * 65536 + hi
+    u, inv, cnt = np.unique(pair, return_inverse=True, return_counts=True)
+    odd = (cnt[inv] % 2 == 1) & (si != sj)
+    return int(odd.sum()), sorted({(int(a), int(b)) for a, b, o in zip(lo, hi, odd) if o})
+
+
 def per_vertex_targets(start, targetEdgeLengths):
     start = np.asarray(start)
     return np.repeat(np.asarray(targetEdgeLengths, np.float64), np.diff(start))

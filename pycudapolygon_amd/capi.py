@@ -92,6 +92,7 @@ SIGNATURES = {
     "pcp_profile_read": (_i, [_p, _p, _p]),
     "pcp_stream_copy": (_i, [_p, _p, _p, ctypes.c_longlong]),
     "pcp_detect_stats": (_i, [_p, _p, _p]),
+    "pcp_odd_crossing_records": (_i, [_p, _p]),
     "pcp_set_schedule": (_i, [_p, _i, _i, _i]),
     "pcp_get_schedule": (_i, [_p, _p, _p, _p]),
     "pcp_autotune": (_i, [_p, ctypes.POINTER(PcpState), _i, _p]),
@@ -419,6 +420,12 @@ class System:
     def set_detect_schedule(self, mode):
         """0 = culled edge pairs, 1 = whole edge tiles, -1 = by the context's shape"""
         self.ctx.call("pcp_set_detect_schedule", int(mode))
+
+    def odd_crossing_records(self):
+        """records of the last pairing in polygon pairs with an odd crossing count (0 = healthy)"""
+        n = ctypes.c_int()
+        self.ctx.call("pcp_odd_crossing_records", ctypes.byref(n))
+        return n.value
 
     def get_detect_schedule(self):
         m = ctypes.c_int()

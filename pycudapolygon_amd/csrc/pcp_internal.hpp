@@ -47,8 +47,11 @@ inline int nblk(long long n, int b = BLK) { return (int)((n + b - 1) / b); }
 // scalar slots in ctx->dScal
 // SC_MAXCOUNT: largest per-edge crossing count (updateNeighbors); SC_BUCKET: largest
 // bucket count, set only when a bucket overflowed
+// SC_ODD: records of the last pairing whose polygon pair has an odd number of crossings
+// (a closed overlap boundary crosses the other polygon's boundary an even number of times;
+// an odd count means a crossing was lost or gained in rounding, see k_pair_poly)
 // SC_MPNN_DROP: adjacency entries dropped by the last contact-graph build
-enum { SC_MAXCOUNT = 0, SC_BUCKET = 1, SC_MPNN_DROP = 7, SC_NSCAL = 8 };
+enum { SC_MAXCOUNT = 0, SC_BUCKET = 1, SC_ODD = 2, SC_MPNN_DROP = 7, SC_NSCAL = 8 };
 
 template <class T> int dalloc(T** p, size_t count) {
     *p = nullptr;

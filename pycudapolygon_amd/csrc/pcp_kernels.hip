@@ -1527,6 +1527,14 @@ int pcp_stream_copy(pcp_ctx* c, double* dst, const double* src, long long n) {
 int pcp_detect_stats(pcp_ctx* c, long long* polygonPairs, long long* edgePairs) {
     return detect_stats(c, polygonPairs, edgePairs);
 }
+int pcp_odd_crossing_records(pcp_ctx* c, int* count) {
+    if (!count) return fail_arg("pcp_odd_crossing_records: count is null");
+    *count = 0;
+    if (!c->dScal) return PCP_OK;
+    PCP_HIP(hipStreamSynchronize(c->stream));
+    PCP_HIP(hipMemcpy(count, c->dScal + SC_ODD, sizeof(int), hipMemcpyDeviceToHost));
+    return PCP_OK;
+}
 
 int pcp_sort_keys(const uint64_t* keys, int n, int endBit, uint64_t* sorted, uint32_t* perm) {
     if (n <= 0) return PCP_OK;

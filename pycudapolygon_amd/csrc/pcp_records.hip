@@ -56,7 +56,8 @@ __global__ __launch_bounds__(64) void k_pair_poly(int N, int Kp, const int* __re
                                                  const unsigned long long* __restrict__ packMaxKey, long long pmkPitch,
                                                  const double2* __restrict__ pairPos, uint64_t* __restrict__ rKey,
                                                  float2* __restrict__ rTU, uint64_t* __restrict__ rOuter,
-                                                 float2* __restrict__ rUT, const int* __restrict__ order) {
+                                                 float2* __restrict__ rUT, const int* __restrict__ order,
+                                                 int* __restrict__ scal) {
     int lane = threadIdx.x & (SEGX - 1);
     int slot = xcd_block(blockIdx.x, gridDim.x) * (64 / SEGX) + threadIdx.x / SEGX;
     if (slot >= N) return;   // whole segments: the shuffles below stay inside a segment
@@ -81,8 +82,21 @@ __global__ __launch_bounds__(64) void k_pair_poly(int N, int Kp, const int* __re
         uint64_t key = bKey[base + f];
         float2 tu = bTU[base + f];
         uint64_t ord = bucket_order(key);
-        int rank = count_keys_below(bKey + base, cnt, [&](uint64_t k2) { return bucket_order(k2) < ord; });
         int si_l = key_si(key), sj_l = key_sj(key), i = key_i(key);
+        // rank in the bucket, and the crossings of this polygon pair held in this bucket
+        int rank = 0, nSame = 0;
+        for (int q0 = 0; q0 < cnt; q0 += 8) {
+            uint64_t v[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] = (q0 + k < cnt) ? bKey[base + q0 + k] : 0ull;
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (q0 + k < cnt) {
+                    rank += (bucket_order(v[k]) < ord) ? 1 : 0;
+                    nSame += (key_sj(v[k]) == sj_l) ? 1 : 0;
+                }
+        }
+        int nOther = 0;   // the crossings of the pair held in the other polygon's bucket
         int sjg = pbase + sj_l;
         int st2 = polyStart[sjg], nj = polyStart[sjg + 1] - st2;
         // exact parameter along s's edge ei of its crossing with sj's edge ej (area energy)
@@ -119,6 +133,8 @@ __global__ __launch_bounds__(64) void k_pair_poly(int N, int Kp, const int* __re
             int c3 = min(bcnt[s3], Kp);
             for (int q = 0; q < c3; ++q)
                 if (bKey[b3 + q] == mxk) { consider(mxk, bTU[b3 + q]); break; }
+            // (no key of the packing reaches si<<48 | sj<<32, so sj's bucket holds no record of
+            // this pair: nOther stays 0)
         } else {
             long long b2 = (long long)sjg * Kp;
             int c2 = min(bcnt[sjg], Kp);
@@ -132,9 +148,14 @@ __global__ __launch_bounds__(64) void k_pair_poly(int N, int Kp, const int* __re
                 }
 #pragma unroll
                 for (int k = 0; k < 8; ++k)
-                    if (q0 + k < c2 && key_sj(k8[k]) == si_l) consider(k8[k], t8[k]);
+                    if (q0 + k < c2 && key_sj(k8[k]) == si_l) {
+                        consider(k8[k], t8[k]);
+                        ++nOther;
+                    }
             }
         }
+        // a polygon pair (not a polygon with itself) with an odd number of crossings
+        if (sjg != s && ((nSame + nOther) & 1)) atomicAdd(&scal[SC_ODD], 1);
         // a partner whose edge indices fall outside the two polygons (possible only in the
         // last-record case above) is treated as no partner: the reference would read
         // vertices of other polygons there
@@ -767,7 +788,8 @@ static int launch_pair_w(pcp_ctx* c, const double2* pairPos) {
     constexpr int G = 64 / SEGX;
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_poly<EXACT, SEGX>), dim3((c->N + G - 1) / G), dim3(64), 0, c->stream,
                        c->N, c->Kp, c->dPolyStart, c->dPolyPack, c->dPackPoly, c->dBcnt, c->dBKey, c->dBTU,
-                       c->dPackMaxKey, c->pmkPitch, pairPos, c->dRKey, c->dRTU, c->dROuter, c->dRUT, c->dPcSorted);
+                       c->dPackMaxKey, c->pmkPitch, pairPos, c->dRKey, c->dRTU, c->dROuter, c->dRUT, c->dPcSorted,
+                       c->dScal);
     PCP_LAUNCHED();
     return PCP_OK;
 }

@@ -134,3 +134,53 @@ def test_shape_projection_rejects_polygons_beyond_64():
     ref = capi.DeviceArray.from_numpy(compress.rest_shapes([p]))
     with pytest.raises(capi.PcpError, match="64"):
         s.ctx.call("pcp_shape_project", s.pos.ptr, ref.ptr, s.ta.ptr)
+
+
+def test_c5_step_learned_warm_start_shake_and_shape_projection():
+    """The compression configuration at reduced size (2 replicas of N=256 x 24, the committed
+    compression network): minimized at phi0, then one phi step of all three pieces -- every
+    polygon put to its rest shape at the new target area (2x2 polar decomposition), the learned
+    warm start (with its SHAKE), FIRE with SHAKE to max|F| <= 1e-10.  From the warm-started
+    positions, FIRE reaches the oracle's FIRE (lane-order solves) in the same steps, at the same
+    energy and positions, replica by replica; a two-step ramp of the same pieces converges."""
+    from oracle import oracle as O
+    from pycudapolygon_amd import warmstart as W
+    pk = [make_packing(256, 24, 0.80, seed=91 + k, min_center_frac=0.5) for k in range(2)]
+    B = [box_size_for(p["diameter"] * np.sqrt(0.81 / 0.80)) for p in pk]
+    s = capi.System(pk, B, model=compress.SQUARED)
+    prm = s.fire_params(fTol=1e-10, dtMax=W.DT_MAX)
+    s.fire_init(prm, 1e-3)
+    assert all(q.converged for q in s.fire_run(prm, 5000, 8, True))
+    ref = capi.DeviceArray.from_numpy(compress.rest_shapes(pk))
+    phi0 = compress.packing_fractions(s)
+    compress.set_phi(s, phi0 + 0.005, ref)
+    xs = s.pos.get().copy()
+    areas = np.array([shoelace(d) for d in unwrapped(xs, s.polyStart)])
+    np.testing.assert_allclose(areas, s.ta.get(), rtol=1e-12)
+    ell = np.array([float(np.mean(p["targetEdgeV"])) for p in pk]) * np.sqrt((phi0 + 0.005) / phi0)
+    s.force_eval()
+    W.WarmStart(W.load_weights(W.COMPRESS_WEIGHTS)).apply(s, ell)
+    x0 = s.pos.get().copy()
+    assert not np.array_equal(x0, xs)   # the network moved the vertices
+    s.fire_init(prm, 1e-3)
+    st = s.fire_run(prm, 5000, 8, True)
+    xg = s.pos.get()
+    ta, tev = s.ta.get(), s.tev.get()
+    for k, p in enumerate(pk):
+        a, b = s.vertex_range(k)
+        pa, pb = int(s.packPoly[k]), int(s.packPoly[k + 1])
+        print(f"[c5] replica {k}: GPU {st[k].steps} steps E {st[k].energy!r}", flush=True)
+        sysO = O.System(p["startIndices"], B[k], O.NORMAL | 0x200, tev[a:b], ta[pa:pb],
+                        solver="lanes")
+        xo, fo, sto, steps = sysO.fire_minimize(x0[2 * a:2 * b], fTol=1e-10, dt=1e-3, maxSteps=5000,
+                                                dtMax=W.DT_MAX)
+        assert st[k].converged and st[k].maxForce <= 1e-10 and np.abs(fo).max() <= 1e-10
+        assert st[k].steps == steps and st[k].energy == sto.energy
+        assert np.array_equal(xg[2 * a:2 * b], xo)
+    s.close()
+    steps, s2 = compress.ramp(pk, 0.81, 5e-3, dict(fTol=1e-10, dtMax=W.DT_MAX), 5000,
+                              warm=W.WarmStart(W.load_weights(W.COMPRESS_WEIGHTS)), shape_match=True)
+    assert len(steps) == 3
+    for r in steps:
+        assert all(r["converged"]) and max(r["max_force"]) <= 1e-10
+    s2.close()

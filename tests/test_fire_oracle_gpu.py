@@ -164,3 +164,36 @@ def test_pair_areas_fire_converges_where_lens_areas_stall():
     assert gs.converged and gs.maxForce <= 1e-10 and np.abs(fo).max() <= 1e-10
     assert gs.steps == steps
     assert abs(gs.energy - sto.energy) <= 1e-12 * abs(sto.energy)
+
+
+def test_odd_crossing_records_flag_the_window():
+    """Packing 41 with squared lens areas: the GPU's FIRE stalls where the oracle's does (same
+    positions after 5000 steps) with every polygon pair crossing an even number of times;
+    moved by 1e-16 along the force, pair (22, 58) crosses 3 times and pcp_odd_crossing_records
+    reports its 3 records, as the oracle's check does, at the oracle's energy (the oracle's side
+    is pinned in tests/test_odd_crossings.py)."""
+    p = make_batch(256, 3, min_center_frac=0.5)[41]
+    B = box_size_for(p["diameter"])
+    s = capi.System([p], [B], model=capi.NORMAL | SQUARED)
+    prm = s.fire_params(fTol=1e-10, dtMax=100.0)
+    s.fire_init(prm, 1e-3)
+    gs = s.fire_run(prm, 5000, 8, True)[0]
+    sysO = O.System(p["startIndices"], B, SQUARED, p["targetEdgeV"], p["targetAreas"], solver="lanes")
+    xo, fo, sto, steps = sysO.fire_minimize(p["positions"], fTol=1e-10, dt=1e-3, maxSteps=5000, dtMax=100.0)
+    print(f"[odd] packing 41: GPU {gs.steps} steps E {gs.energy!r}; oracle {steps} steps E {sto.energy!r}",
+          flush=True)
+    assert not gs.converged and steps == 5000
+    x = s.get_positions(0)
+    assert np.array_equal(x, xo)
+    s.force_eval()
+    assert s.odd_crossing_records() == 0
+    F = s.get_forces(0)
+    y = x + 1e-16 * F / np.abs(F).max()
+    y -= np.floor(y)
+    s.pos.upload(y)
+    s.force_eval()
+    Eo, _, _ = sysO.force_eval(y)
+    rec = O.outersections(p["startIndices"], O.neighbors(y, p["startIndices"], B))
+    assert O.odd_crossing_records(rec["intersections"]) == (3, [(22, 58)])
+    assert s.odd_crossing_records() == 3 and s.get_energy()[0] == Eo
+    s.close()
