@@ -60,6 +60,17 @@ enum { PCP_NORMAL = 0, PCP_EDGE_ONLY = 1, PCP_AREA_ONLY = 2, PCP_SOFT_BODY = 3,
  * crosses an edge, two lenses of one pair merge or split; sum A^2/2 jumps there by
  * A1*A2 and FIRE can stall against the jump, while A_pair is continuous. */
 #define PCP_PAIR_SQUARED 0x400
+/* Flag (no counterpart in the reference): side-consistent crossings.  The reference decides
+ * each edge pair by its parameters t, u in (0, 1) (updateNeighborsKernel kernels.cuh:760-790);
+ * where a vertex lies within rounding of another polygon's edge, that test can count the
+ * crossing of one of the vertex's two edges and not of the other, and the pair then has an odd
+ * number of crossings (pcp_odd_crossing_records), wrong exits and an energy jump that FIRE can
+ * stall against.  With this flag an edge pair crosses when each edge's endpoints lie on
+ * different sides of the other's line, every side being a function of one edge and one point
+ * (so the two edges at a vertex agree about it); |denom| >= 1e-12 as in the reference, and
+ * t, u are still the reference's.  Away from such near-ties the crossings are the reference's
+ * bit for bit.  Applies to pcp_force_eval, FIRE and pcp_autotune. */
+#define PCP_CONSISTENT 0x800
 
 typedef struct pcp_ctx pcp_ctx;
 

@@ -15,6 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.environ.get("PCP_ORACLE_LIB") or os.path.join(_HERE, "build", "liboracle.so")
 
 NORMAL, EDGE_ONLY, AREA_ONLY, SOFT_BODY, ABNORMAL, HYBRID = range(6)
+CONSISTENT = 0x800   # ORC_CONSISTENT: side-consistent segment decision (OR into the model)
 MODEL_IDS = {"normal": NORMAL, "edgeOnly": EDGE_ONLY, "areaOnly": AREA_ONLY,
              "softBody": SOFT_BODY, "abnormal": ABNORMAL, "hybrid": HYBRID}
 
@@ -61,6 +62,7 @@ def _declare(L):
     L.orc_cells.argtypes = [_i, _i, _p, _p, _i, _p, _p, _p]
     L.orc_neighbors.restype = _i
     L.orc_neighbors.argtypes = [_i, _i, _p, _p, _i, _p, _p, _p, _i, _p, _p, _p, _p]
+    L.orc_neighbors_ex.argtypes = [_i, _i, _p, _p, _i, _p, _p, _p, _i, _p, _p, _p, _p, _i]
     L.orc_outersections.restype = _i
     L.orc_outersections.argtypes = [_i, _i, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p]
     L.orc_force_energy.restype = _d
@@ -152,8 +154,9 @@ def cells(pos, start, boxSize):
     return cl, ni, cpb
 
 
-def neighbors(pos, start, boxSize, maxNeighbors=100, cell=None):
-    """updateNeighbors incl. the overflow resize-and-retry of model.cpp:264-305."""
+def neighbors(pos, start, boxSize, maxNeighbors=100, cell=None, consistent=False):
+    """updateNeighbors incl. the overflow resize-and-retry of model.cpp:264-305; consistent:
+    the side-consistent decision (ORC_CONSISTENT) instead of the parameter test."""
     pos = _f64(pos); start = _i32(start)
     N = len(start) - 1; V = int(start[-1])
     cl, ni, cpb = cell if cell is not None else cells(pos, start, boxSize)
@@ -161,8 +164,8 @@ def neighbors(pos, start, boxSize, maxNeighbors=100, cell=None):
     while True:
         nb = np.full(V * M, -1, np.int32); ins = np.zeros(V * M, np.uint8)
         tu = np.zeros(2 * V * M); num = np.zeros(V, np.int32)
-        mx = lib().orc_neighbors(V, N, _ptr(pos), _ptr(start), int(boxSize), _ptr(cl), _ptr(ni), _ptr(cpb),
-                                 M, _ptr(nb), _ptr(ins), _ptr(tu), _ptr(num))
+        mx = lib().orc_neighbors_ex(V, N, _ptr(pos), _ptr(start), int(boxSize), _ptr(cl), _ptr(ni), _ptr(cpb),
+                                    M, _ptr(nb), _ptr(ins), _ptr(tu), _ptr(num), int(bool(consistent)))
         if mx <= M:
             break
         M = max(2 * M + 1, mx)

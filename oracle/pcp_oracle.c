@@ -179,10 +179,27 @@ void orc_cells(int V, int N, const double* pos, const int* start, int boxSize,
 
 /* updateNeighborsKernel kernels.cuh:709-812.  tu holds double2 per slot [2*V*M].
  * Returns max(numNeighbors) (maxReduceKernel :65-92). */
+/* Side of point X relative to the line of the edge from vertex e0 to vertex e1: the sign of
+ * wrap(e1 - e0) x wrap(X - e0), as 1 (negative) or 0 (zero or positive).  A function of the
+ * edge's two vertices and X only, so every test that involves the same edge and point agrees. */
+static int orc_side(const double* pos, int e0, int e1, int X) {
+    double ex = orc_wrap(pos[2 * e1] - pos[2 * e0]), ey = orc_wrap(pos[2 * e1 + 1] - pos[2 * e0 + 1]);
+    double wx = orc_wrap(pos[2 * X] - pos[2 * e0]), wy = orc_wrap(pos[2 * X + 1] - pos[2 * e0 + 1]);
+    return (ex * wy - ey * wx) < 0.0;
+}
+
 int orc_neighbors(int V, int N, const double* pos, const int* start, int boxSize,
                   const int* cellLocation, const int* neighborIndices, const int* countPerBox,
                   int maxNeighbors, int* neighbors, unsigned char* inside, double* tu,
                   int* numNeighbors) {
+    return orc_neighbors_ex(V, N, pos, start, boxSize, cellLocation, neighborIndices, countPerBox,
+                            maxNeighbors, neighbors, inside, tu, numNeighbors, 0);
+}
+
+int orc_neighbors_ex(int V, int N, const double* pos, const int* start, int boxSize,
+                     const int* cellLocation, const int* neighborIndices, const int* countPerBox,
+                     int maxNeighbors, int* neighbors, unsigned char* inside, double* tu,
+                     int* numNeighbors, int consistent) {
     const double eps = 1e-12;
     int* shapeId = (int*)malloc(sizeof(int) * (V > 0 ? V : 1));
     for (int s = 0; s < N; ++s) for (int i = start[s]; i < start[s + 1]; ++i) shapeId[i] = s;
@@ -230,7 +247,11 @@ int orc_neighbors(int V, int N, const double* pos, const int* start, int boxSize
                     if (fabs(denom) < eps) continue;
                     double tt = (gx * sy - gy * sx) / denom;
                     double uu = (gx * ry - gy * rx) / denom;
-                    if (tt > 0.0 && tt < 1.0 && uu > 0.0 && uu < 1.0) {
+                    int hit = consistent
+                        ? (orc_side(pos, nid, nid2, id1) != orc_side(pos, nid, nid2, id2) &&
+                           orc_side(pos, id1, id2, nid) != orc_side(pos, id1, id2, nid2))
+                        : (tt > 0.0 && tt < 1.0 && uu > 0.0 && uu < 1.0);
+                    if (hit) {
                         if (count < maxNeighbors) {
                             long slot = (long)id1 * maxNeighbors + count;
                             neighbors[slot] = nid;
@@ -1691,15 +1712,16 @@ static void ws_alloc(orc_ws* w, int V, int N, int B, int M) {
 
 static void detect(const orc_system* s, const double* pos, orc_ws* w) {
     orc_cells(s->V, s->N, pos, s->start, s->boxSize, w->cellLocation, w->neighborIndices, w->countPerBox);
-    int mx = orc_neighbors(s->V, s->N, pos, s->start, s->boxSize, w->cellLocation, w->neighborIndices,
-                           w->countPerBox, w->M, w->neighbors, w->inside, w->tuSlots, w->numNeighbors);
+    int cons = (s->model & ORC_CONSISTENT) != 0;
+    int mx = orc_neighbors_ex(s->V, s->N, pos, s->start, s->boxSize, w->cellLocation, w->neighborIndices,
+                              w->countPerBox, w->M, w->neighbors, w->inside, w->tuSlots, w->numNeighbors, cons);
     if (mx > w->M) {   /* model.cpp:264-305: grow and retry once */
         int newM = (2 * w->M + 1 > mx) ? 2 * w->M + 1 : mx;
         ws_free(w);
         ws_alloc(w, s->V, s->N, s->boxSize, newM);
         orc_cells(s->V, s->N, pos, s->start, s->boxSize, w->cellLocation, w->neighborIndices, w->countPerBox);
-        orc_neighbors(s->V, s->N, pos, s->start, s->boxSize, w->cellLocation, w->neighborIndices,
-                      w->countPerBox, w->M, w->neighbors, w->inside, w->tuSlots, w->numNeighbors);
+        orc_neighbors_ex(s->V, s->N, pos, s->start, s->boxSize, w->cellLocation, w->neighborIndices,
+                         w->countPerBox, w->M, w->neighbors, w->inside, w->tuSlots, w->numNeighbors, cons);
     }
 }
 static void records(const orc_system* s, orc_ws* w, const double* pos) {

@@ -34,6 +34,11 @@ enum { ORC_NORMAL = 0, ORC_EDGE_ONLY = 1, ORC_AREA_ONLY = 2, ORC_SOFT_BODY = 3,
  * continuous where two lenses of one pair merge or split, unlike ORC_AREA_SQUARED */
 #define ORC_PAIR_SQUARED 0x400
 #define ORC_AREA_MODES (ORC_AREA_ENERGY | ORC_AREA_SQUARED | ORC_PAIR_SQUARED)
+/* OR-ed into the model: decide each segment test by the sides of the four endpoints (each side
+ * a function of one edge and one point, so the two edges at a vertex agree about that vertex:
+ * a polygon pair always crosses an even number of times) instead of the reference's parameter
+ * test; the crossing parameters are still the reference's */
+#define ORC_CONSISTENT 0x800
 
 double orc_wrap(double x);
 void orc_topology(int N, const int* start, int V, int* shapeId, int* next, int* prev);
@@ -46,6 +51,11 @@ int orc_neighbors(int V, int N, const double* pos, const int* start, int boxSize
                   const int* cellLocation, const int* neighborIndices, const int* countPerBox,
                   int maxNeighbors, int* neighbors, unsigned char* inside, double* tu,
                   int* numNeighbors);
+/* orc_neighbors with the side-consistent decision when consistent != 0 (ORC_CONSISTENT) */
+int orc_neighbors_ex(int V, int N, const double* pos, const int* start, int boxSize,
+                  const int* cellLocation, const int* neighborIndices, const int* countPerBox,
+                  int maxNeighbors, int* neighbors, unsigned char* inside, double* tu,
+                  int* numNeighbors, int consistent);
 int orc_outersections(int V, int N, const int* start, int maxNeighbors,
                       const int* neighbors, const unsigned char* inside, const double* tuSlots,
                       const int* numNeighbors, uint64_t* intersections, double* tu,

@@ -404,6 +404,39 @@ __device__ __forceinline__ bool exact_record(const SegNum& s, int s1l, int n1, i
     return true;
 }
 
+// The side-consistent decision (PCP_CONSISTENT; no counterpart in the reference): the side of
+// point X relative to the line of an edge (start E, wrapped edge vector ER) is the sign of
+// ER x wrap(X - E), a function of that edge's two vertices and X only.  Edge 1 (P1, R1, next
+// vertex P1n) and edge 2 (P2, R2, P2n) cross when each edge's endpoints lie on different sides
+// of the other's line (a zero cross product counts with the positive side) and
+// |denom| >= 1e-12 as in the reference.  The two edges at a vertex then agree about that
+// vertex, and two polygons cross an even number of times, where the parameter test can count
+// one of the two crossings near a vertex and not the other.  On configurations without such
+// near-ties it decides exactly as the parameter test (t, u are still the reference's).
+__device__ __forceinline__ bool side_neg(d2 E, d2 ER, d2 X) {
+    double wx = wrap(X.x - E.x), wy = wrap(X.y - E.y);
+    return (ER.x * wy - ER.y * wx) < 0.0;
+}
+__device__ __forceinline__ bool seg_cons_cross(const SegNum& s, d2 P1, d2 R1, d2 P1n, d2 P2, d2 R2, d2 P2n) {
+    return !(fabs(s.denom) < 1e-12) && side_neg(P2, R2, P1) != side_neg(P2, R2, P1n) &&
+           side_neg(P1, R1, P2) != side_neg(P1, R1, P2n);
+}
+// the record of a pair the side-consistent decision took (no parameter range test)
+__device__ __forceinline__ void cons_record(const SegNum& s, int s1l, int n1, int s2l, int l2, uint64_t& key,
+                                           float2& tu) {
+    double tt = s.nt / s.denom;
+    double uu = s.nu / s.denom;
+    if (s.denom > 0) {
+        key = ((uint64_t)(uint16_t)s1l << 48) | ((uint64_t)(uint16_t)s2l << 32) | ((uint64_t)(uint16_t)l2 << 16) |
+              (uint64_t)(uint16_t)n1;
+        tu = make_float2((float)tt, (float)uu);
+    } else {
+        key = ((uint64_t)(uint16_t)s2l << 48) | ((uint64_t)(uint16_t)s1l << 32) | ((uint64_t)(uint16_t)n1 << 16) |
+              (uint64_t)(uint16_t)l2;
+        tu = make_float2((float)uu, (float)tt);
+    }
+}
+
 // An edge (start P, vector R) that crosses an edge of a polygon whose vertices lie in the
 // circle (c, r) meets that circle (the crossing point lies on both edges, and the circle
 // contains the polygon's edges).  Conservative test with the edge's own bounding circle:
@@ -474,7 +507,19 @@ __device__ __forceinline__ void stage_a(LdsT& L, int lane, int stA, int nA, int 
 // non-adjacent edges of A once.  The hot loop evaluates the division-free necessary
 // condition; iterations where some lane passes it take the exact test, the user-grid cell
 // filter and the record.  Used for A itself and for polygons of more than 64 vertices.
-template <int CELLS, int ROLE, int SRC>
+// The start of the edge after edge a0 + al of polygon A (its next vertex), from A's stage
+// when the chunk [a0, a0 + na) holds it, else from global memory
+__device__ __forceinline__ d2 edge_next_start(const DetLds& L, const double2* __restrict__ pos, int stA, int nA,
+                                              int a0, int na, int al) {
+    int a = a0 + al, an = (a + 1 == nA) ? 0 : a + 1;
+    if (an >= a0 && an < a0 + na) {
+        double2 p = L.P[an - a0];
+        return mk(p.x, p.y);
+    }
+    return ldpos(pos, stA + an);
+}
+
+template <int CELLS, int ROLE, int SRC, bool CONS>
 __device__ void edge_tile(DetLds& L, int lane, int stA, int nA, int stB, int nB, int sAl, int sBl, int Bu, int s0,
                           const double2* __restrict__ pos, const int* __restrict__ cellOf, int& staged,
                           const DetOut& out, int* __restrict__ num, uint64_t& kmax) {
@@ -491,16 +536,21 @@ __device__ void edge_tile(DetLds& L, int lane, int stA, int nA, int stB, int nB,
             bool act = g < G;
             int b = b0 + bl;
             int bn = (b + 1 == nB) ? 0 : b + 1;
-            d2 Q = mk(0.0, 0.0), S = mk(0.0, 0.0);
+            d2 Q = mk(0.0, 0.0), S = mk(0.0, 0.0), Qn = mk(0.0, 0.0);
             if (act) {
                 if (SRC == SRC_STAGE) {   // A itself, all of it in the stage (nA <= 64)
                     double2 q = L.P[b], r = L.R[b];
                     Q = mk(q.x, q.y);
                     S = mk(r.x, r.y);
+                    if (CONS) {
+                        double2 qn = L.P[bn];
+                        Qn = mk(qn.x, qn.y);
+                    }
                 } else {
                     d2 q0 = ldpos(pos, stB + b), q1 = ldpos(pos, stB + bn);
                     Q = q0;
                     S = mk(wrap(q1.x - q0.x), wrap(q1.y - q0.y));
+                    Qn = q1;
                 }
             }
             int iters = (na + TILE_UNROLL * G - 1) / (TILE_UNROLL * G);
@@ -515,7 +565,12 @@ __device__ void edge_tile(DetLds& L, int lane, int stA, int nA, int stB, int nB,
                     if (ROLE == TILE_SELF) ok = ok && b > a && b != a + 1 && !(a == 0 && b == nA - 1);
                     double2 pa = L.P[ok ? al : 0], ra = L.R[ok ? al : 0];
                     SegNum s = seg_numerators(mk(pa.x, pa.y), mk(ra.x, ra.y), Q, S);
-                    may[u] = ok && seg_may_cross(s);
+                    if (CONS) {
+                        d2 pn = ok ? edge_next_start(L, pos, stA, nA, a0, na, al) : mk(0.0, 0.0);
+                        may[u] = ok && seg_cons_cross(s, mk(pa.x, pa.y), mk(ra.x, ra.y), pn, Q, S, Qn);
+                    } else {
+                        may[u] = ok && seg_may_cross(s);
+                    }
                     anyMay = anyMay || may[u];
                 }
                 if (__ballot(anyMay) == 0ull) continue;   // wavefront-uniform: no crossing here
@@ -535,7 +590,12 @@ __device__ void edge_tile(DetLds& L, int lane, int stA, int nA, int stB, int nB,
                     if (may[u] && cells_adjacent(L.C[al], cb, Bu)) {
                         double2 pa = L.P[al], ra = L.R[al];
                         SegNum s = seg_numerators(mk(pa.x, pa.y), mk(ra.x, ra.y), Q, S);
-                        rec = exact_record(s, sAl, a, sBl, b, key, tu);
+                        if (CONS) {
+                            cons_record(s, sAl, a, sBl, b, key, tu);
+                            rec = true;
+                        } else {
+                            rec = exact_record(s, sAl, a, sBl, b, key, tu);
+                        }
                     }
                     if (rec) {
                         kmax = (key > kmax) ? key : kmax;
@@ -560,7 +620,7 @@ __device__ void edge_tile(DetLds& L, int lane, int stA, int nA, int stB, int nB,
 //      edge pairs that can cross, enumerated over the batch and tested 64 per pass, every
 //      lane busy whatever the polygons' sizes.
 //   4. Other candidates: one edge_tile each.
-template <int CELLS>
+template <int CELLS, bool CONS>
 __device__ void detect_polygon(
     int A, int lane, DetLds& L, const int* __restrict__ polyStart, const int* __restrict__ polyPack,
     const int* __restrict__ packPoly, const int* __restrict__ box, const double2* __restrict__ pos,
@@ -599,10 +659,10 @@ __device__ void detect_polygon(
         stage_a(L, lane, stA, nA, 0, Bu, pos, cellOf, CELLS != 0);
         staged = 0;
         if (nA >= 4)
-            edge_tile<CELLS, TILE_SELF, SRC_STAGE>(L, lane, stA, nA, stA, nA, sAl, sAl, Bu, s0, pos, cellOf, staged,
+            edge_tile<CELLS, TILE_SELF, SRC_STAGE, CONS>(L, lane, stA, nA, stA, nA, sAl, sAl, Bu, s0, pos, cellOf, staged,
                                                    out, num, kmax);
     } else {
-        edge_tile<CELLS, TILE_SELF, SRC_GLOBAL>(L, lane, stA, nA, stA, nA, sAl, sAl, Bu, s0, pos, cellOf, staged, out,
+        edge_tile<CELLS, TILE_SELF, SRC_GLOBAL, CONS>(L, lane, stA, nA, stA, nA, sAl, sAl, Bu, s0, pos, cellOf, staged, out,
                                                 num, kmax);
     }
     int incl = cnt;
@@ -640,7 +700,7 @@ __device__ void detect_polygon(
             tiles &= tiles - 1;
             int Bsel = __builtin_amdgcn_readlane(Bq, j);
             int stB = __builtin_amdgcn_readlane(stq, j), nB = __builtin_amdgcn_readlane(nq, j);
-            edge_tile<CELLS, TILE_A_FIRST, SRC_GLOBAL>(L, lane, stA, nA, stB, nB, sAl, Bsel - s0, Bu, s0, pos, cellOf,
+            edge_tile<CELLS, TILE_A_FIRST, SRC_GLOBAL, CONS>(L, lane, stA, nA, stB, nB, sAl, Bsel - s0, Bu, s0, pos, cellOf,
                                                        staged, out, num, kmax);
         }
         // packed candidates, in lane order, cut into batches
@@ -787,9 +847,15 @@ __device__ void detect_polygon(
                     t.b = L.listB[t.offk + ib];
                     double2 q0 = L.V[t.offk + t.b], q1 = L.V[t.offk + ((t.b + 1 == nk) ? 0 : t.b + 1)];
                     double2 pa = L.P[t.a], ra = L.R[t.a];
-                    t.s = seg_numerators(mk(pa.x, pa.y), mk(ra.x, ra.y), mk(q0.x, q0.y),
-                                         mk(wrap(q1.x - q0.x), wrap(q1.y - q0.y)));
-                    t.may = seg_may_cross(t.s);
+                    d2 Sb = mk(wrap(q1.x - q0.x), wrap(q1.y - q0.y));
+                    t.s = seg_numerators(mk(pa.x, pa.y), mk(ra.x, ra.y), mk(q0.x, q0.y), Sb);
+                    if (CONS) {   // A is staged whole here (nA <= 64)
+                        double2 pn = L.P[(t.a + 1 == nA) ? 0 : t.a + 1];
+                        t.may = seg_cons_cross(t.s, mk(pa.x, pa.y), mk(ra.x, ra.y), mk(pn.x, pn.y), mk(q0.x, q0.y),
+                                               Sb, mk(q1.x, q1.y));
+                    } else {
+                        t.may = seg_may_cross(t.s);
+                    }
                 }
                 return t;
             };
@@ -798,8 +864,14 @@ __device__ void detect_polygon(
                 uint64_t key = 0;
                 float2 tu;
                 bool rec = false;
-                if (t.may && cells_adjacent(L.C[t.a], L.cellB[t.offk + t.b], Bu))
-                    rec = exact_record(t.s, sAl, t.a, t.Bk - s0, t.b, key, tu);
+                if (t.may && cells_adjacent(L.C[t.a], L.cellB[t.offk + t.b], Bu)) {
+                    if (CONS) {
+                        cons_record(t.s, sAl, t.a, t.Bk - s0, t.b, key, tu);
+                        rec = true;
+                    } else {
+                        rec = exact_record(t.s, sAl, t.a, t.Bk - s0, t.b, key, tu);
+                    }
+                }
                 if (rec) {
                     kmax = (key > kmax) ? key : kmax;
                     if (CELLS) atomicAdd(&num[stA + t.a], 1);   // per-edge count of edge 1 (A's)
@@ -867,7 +939,7 @@ struct TileLds {
     uint16_t may[TILE_MAY];        // a | lane << 6
 };
 
-template <int CELLS, int SEG>
+template <int CELLS, int SEG, bool CONS>
 __device__ void tile_pairs(TileLds& L, int lane, int nU, int A, int stA, int nA, int sAl, int s0, int Bu,
                            const double2* __restrict__ pos, const int* __restrict__ cellOf,
                            const DetOut& out, int* __restrict__ num, uint64_t& kmax) {
@@ -880,11 +952,12 @@ __device__ void tile_pairs(TileLds& L, int lane, int nU, int A, int stA, int nA,
         bool self = Bq == A;
         bool act = has && b < nB;
         int bn = (b + 1 == nB) ? 0 : b + 1;
-        d2 Q = mk(0.0, 0.0), Sv = mk(0.0, 0.0);
+        d2 Q = mk(0.0, 0.0), Sv = mk(0.0, 0.0), Qn = mk(0.0, 0.0);
         int cb = 0;   // the user-grid cell of B's edge (for the stencil filter)
         if (act) {
             d2 q0 = ldpos(pos, stB + b), q1 = ldpos(pos, stB + bn);
             Q = q0;
+            Qn = q1;
             Sv = mk(wrap(q1.x - q0.x), wrap(q1.y - q0.y));
             cb = CELLS ? cell_xy(cellOf[stB + b], Bu) : edge_cell_xy(q0, q1, Bu);
         }
@@ -907,7 +980,12 @@ __device__ void tile_pairs(TileLds& L, int lane, int nU, int A, int stA, int nA,
                 if (valid && cells_adjacent(L.C[a], cbo, Bu)) {
                     double2 pa = L.P[a], ra = L.R[a];
                     SegNum sn = seg_numerators(mk(pa.x, pa.y), mk(ra.x, ra.y), mk(qx, qy), mk(sx, sy));
-                    rec = exact_record(sn, sAl, a, L.uB[uo] - s0, bo, key, tu);
+                    if (CONS) {   // decided when the pair was collected
+                        cons_record(sn, sAl, a, L.uB[uo] - s0, bo, key, tu);
+                        rec = true;
+                    } else {
+                        rec = exact_record(sn, sAl, a, L.uB[uo] - s0, bo, key, tu);
+                    }
                 }
                 if (rec) {
                     kmax = (key > kmax) ? key : kmax;
@@ -929,7 +1007,12 @@ __device__ void tile_pairs(TileLds& L, int lane, int nU, int A, int stA, int nA,
                 int al = (a < nA) ? a : 0;
                 double2 pa = L.P[al], ra = L.R[al];
                 SegNum sn = seg_numerators(mk(pa.x, pa.y), mk(ra.x, ra.y), Q, Sv);
-                may[k] = ok && seg_may_cross(sn);
+                if (CONS) {   // A is staged whole (every polygon of the context has <= 64 vertices)
+                    double2 pn = L.P[(al + 1 == nA) ? 0 : al + 1];
+                    may[k] = ok && seg_cons_cross(sn, mk(pa.x, pa.y), mk(ra.x, ra.y), mk(pn.x, pn.y), Q, Sv, Qn);
+                } else {
+                    may[k] = ok && seg_may_cross(sn);
+                }
                 anyMay = anyMay || may[k];
             }
             if (__ballot(anyMay) == 0ull) continue;   // wavefront-uniform: most blocks find nothing
@@ -946,7 +1029,7 @@ __device__ void tile_pairs(TileLds& L, int lane, int nU, int A, int stA, int nA,
     }
 }
 
-template <int CELLS, int SEG>
+template <int CELLS, int SEG, bool CONS = false>
 __global__ __launch_bounds__(BLK) void k_detect_tile(
     int N, const int* __restrict__ polyStart, const int* __restrict__ polyPack,
     const int* __restrict__ packPoly, const int* __restrict__ box, const double2* __restrict__ pos,
@@ -1031,7 +1114,7 @@ __global__ __launch_bounds__(BLK) void k_detect_tile(
             L.uN[u] = nq;
         }
         wave_lds_sync();
-        tile_pairs<CELLS, SEG>(L, lane, nU, A, stA, nA, sAl, s0, Bu, pos, cellOf, out, num, kmax);
+        tile_pairs<CELLS, SEG, CONS>(L, lane, nU, A, stA, nA, sAl, s0, Bu, pos, cellOf, out, num, kmax);
     }
     // the largest key of the packing (the pairing's last-record case)
     unsigned long long* pmkSlot = packMaxKey + (A & (PMK_SLOTS - 1)) * pmkPitch + pk;
@@ -1050,7 +1133,7 @@ __global__ __launch_bounds__(BLK) void k_detect_tile(
 // CELLS = 1: the cells of the last cell-list build are used (updateNeighbors after
 // updateNeighborCells, which may have been called at other positions).
 // The bucket counters bcnt/jcnt must be zero (k_bounds clears them).
-template <int CELLS>
+template <int CELLS, bool CONS = false>
 __global__ __launch_bounds__(BLK) void k_detect_poly(
     int N, const int* __restrict__ polyStart, const int* __restrict__ polyPack,
     const int* __restrict__ packPoly, const int* __restrict__ box, const double2* __restrict__ pos,
@@ -1065,7 +1148,7 @@ __global__ __launch_bounds__(BLK) void k_detect_poly(
     int idx = xcd_block(blockIdx.x, gridDim.x) * (BLK / 64) + w;
     if (idx >= N) return;   // whole wavefront
     DetOut out{Kp, bcnt, bKey, bTU, jcnt, jKey, scal};
-    detect_polygon<CELLS>(__builtin_amdgcn_readfirstlane(pcSorted[idx]), lane, lds[w], polyStart, polyPack, packPoly,
+    detect_polygon<CELLS, CONS>(__builtin_amdgcn_readfirstlane(pcSorted[idx]), lane, lds[w], polyStart, polyPack, packPoly,
                           box, pos, cellOf, bounds, pcStart, pcSorted, sBounds, packB, out, packMaxKey, pmkPitch,
                           num);
 }
@@ -1117,11 +1200,11 @@ int launch_bounds(pcp_ctx* c, const double2* pos, const PostShake* ps) {
 
 // k_detect_tile<CELLS, SEG> on the context's stream; its launches are profiled under the
 // name "k_detect_tile" (pcp_profile_kernel), with events attached to the dispatch
-template <int CELLS, int SEG>
+template <int CELLS, int SEG, bool CONS>
 static int launch_tile_t(pcp_ctx* c, const double2* pos, int grid) {
     int slot = -1;
     PCP_TRY(prof_slot(c, "k_detect_tile", &slot));
-    auto kern = k_detect_tile<CELLS, SEG>;
+    auto kern = k_detect_tile<CELLS, SEG, CONS>;
     if (slot >= 0)
         hipExtLaunchKernelGGL(kern, dim3(grid), dim3(BLK), 0, c->stream, c->profEvents[slot].first,
                               c->profEvents[slot].second, 0, c->N, c->dPolyStart, c->dPolyPack, c->dPackPoly,
@@ -1136,18 +1219,19 @@ static int launch_tile_t(pcp_ctx* c, const double2* pos, int grid) {
     PCP_LAUNCHED();
     return PCP_OK;
 }
-template <int CELLS>
+template <int CELLS, bool CONS>
 static int launch_tile_c(pcp_ctx* c, const double2* pos, int grid) {
     switch (c->seg) {
     case 4:
-    case 8: return launch_tile_t<CELLS, 8>(c, pos, grid);
-    case 16: return launch_tile_t<CELLS, 16>(c, pos, grid);
-    case 32: return launch_tile_t<CELLS, 32>(c, pos, grid);
-    default: return launch_tile_t<CELLS, 64>(c, pos, grid);
+    case 8: return launch_tile_t<CELLS, 8, CONS>(c, pos, grid);
+    case 16: return launch_tile_t<CELLS, 16, CONS>(c, pos, grid);
+    case 32: return launch_tile_t<CELLS, 32, CONS>(c, pos, grid);
+    default: return launch_tile_t<CELLS, 64, CONS>(c, pos, grid);
     }
 }
-static int launch_detect_tile(pcp_ctx* c, const double2* pos, bool cellsFromList, int grid) {
-    return cellsFromList ? launch_tile_c<1>(c, pos, grid) : launch_tile_c<0>(c, pos, grid);
+static int launch_detect_tile(pcp_ctx* c, const double2* pos, bool cellsFromList, bool cons, int grid) {
+    if (cons) return cellsFromList ? launch_tile_c<1, true>(c, pos, grid) : launch_tile_c<0, true>(c, pos, grid);
+    return cellsFromList ? launch_tile_c<1, false>(c, pos, grid) : launch_tile_c<0, false>(c, pos, grid);
 }
 
 // the detection schedule in effect: the one set (pcp_set_detect_schedule / pcp_autotune), else
@@ -1161,15 +1245,29 @@ int detect_schedule(const pcp_ctx* c) {
 }
 
 // cellsFromList: use the cells of the last run_cells (API updateNeighbors).  ps: FIRE's
-// post-SHAKE step, done by the bounds kernel (see k_bounds)
-int run_detect(pcp_ctx* c, const double2* pos, int cellsFromList, const PostShake* ps) {
+// post-SHAKE step, done by the bounds kernel (see k_bounds).  cons: the side-consistent
+// decision (PCP_CONSISTENT) instead of the reference's parameter test.
+int run_detect(pcp_ctx* c, const double2* pos, int cellsFromList, const PostShake* ps, bool cons) {
     if (c->V == 0) return PCP_OK;
     c->lastDetectPos = pos;
     PCP_TRY(launch_bounds(c, pos, ps));
     PCP_LAUNCH(c, k_poly_cells, dim3(c->P), dim3(PP_BLK), c->dPackPoly, c->dBounds, c->dPcStart, c->dPcSorted,
                c->dPackB, c->dSBounds);
     int grid = (c->N + BLK / 64 - 1) / (BLK / 64);
-    if (c->maxN <= 64 && detect_schedule(c) == 1) return launch_detect_tile(c, pos, cellsFromList != 0, grid);
+    if (c->maxN <= 64 && detect_schedule(c) == 1) return launch_detect_tile(c, pos, cellsFromList != 0, cons, grid);
+    if (cons) {
+        if (cellsFromList)
+            PCP_LAUNCH(c, (k_detect_poly<1, true>), dim3(grid), dim3(BLK), c->N, c->dPolyStart, c->dPolyPack,
+                       c->dPackPoly, c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted, c->dSBounds,
+                       c->dPackB, c->Kp, c->dBcnt, c->dBKey, c->dBTU, c->dJcnt, c->dJKey, c->dPackMaxKey,
+                       c->pmkPitch, c->dNum, c->dScal);
+        else
+            PCP_LAUNCH(c, (k_detect_poly<0, true>), dim3(grid), dim3(BLK), c->N, c->dPolyStart, c->dPolyPack,
+                       c->dPackPoly, c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted, c->dSBounds,
+                       c->dPackB, c->Kp, c->dBcnt, c->dBKey, c->dBTU, c->dJcnt, c->dJKey, c->dPackMaxKey,
+                       c->pmkPitch, c->dNum, c->dScal);
+        return PCP_OK;
+    }
     if (cellsFromList)
         PCP_LAUNCH(c, k_detect_poly<1>, dim3(grid), dim3(BLK), c->N, c->dPolyStart, c->dPolyPack, c->dPackPoly,
                    c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted, c->dSBounds, c->dPackB, c->Kp,

@@ -482,6 +482,7 @@ static int fire_eval(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm
     bool overlap = (mt == PCP_NORMAL || mt == PCP_HYBRID);
     bool rigid = (mt == PCP_NORMAL);
     bool xpbd = prm->xpbdIter > 0;
+    bool cons = (st->model & PCP_CONSISTENT) != 0;
     if (xpbd) {
         // the springs of the model by XPBD at the packing's dt, then velocities from the
         // displacement; detection and records at the final positions
@@ -492,12 +493,12 @@ static int fire_eval(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm
                              areaOn ? st->compressibility : 0.0, perimOn ? st->perimStiffness : 0.0, 0.0, c->dDt,
                              prm->xpbdIter, prm->xpbdTol, c->dActive, v, c->dShakeIt));
         if (overlap) {
-            PCP_TRY(run_detect(c, x, 0));
+            PCP_TRY(run_detect(c, x, 0, nullptr, cons));
             PCP_TRY(run_records(c, area ? x : nullptr));
         }
     } else if (overlap) {
         if (!area)   // reference order (model.cpp:416-440): detect, SHAKE, records
-            PCP_TRY(run_detect(c, x, 0));
+            PCP_TRY(run_detect(c, x, 0, nullptr, cons));
         bool shake = rigid && prm->shakeIter > 0;
         if (shake)
             PCP_TRY(run_shake(c, x, st->targetEdgeV, st->targetAreas, prm->shakeIter, 1e-15,
@@ -507,7 +508,7 @@ static int fire_eval(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm
             // after SHAKE is done by the detection's first kernel (k_bounds<POST>)
             PostShake ps{prm->shakeIter, FL_NF, FL_FAIL, FL_LASTSHAKE, c->dPolyPack, c->dPackVert, c->dActive,
                          c->dDt, c->dShakeIt, x, v, (const double2*)c->dXbak, (const double2*)c->dXpre, c->dFlags};
-            PCP_TRY(run_detect(c, x, 0, &ps));
+            PCP_TRY(run_detect(c, x, 0, &ps, cons));
         } else {
             if (shake)
                 PCP_LAUNCH(c, k_fire_post_shake, dim3(nblk(c->V)), dim3(BLK), c->V, c->P,
@@ -515,7 +516,7 @@ static int fire_eval(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm
                            c->dShakeIt, x, v, (const double2*)c->dXbak, (const double2*)c->dXpre,
                            c->dFlags);
             if (area)   // area energy: crossings of the final positions
-                PCP_TRY(run_detect(c, x, 0));
+                PCP_TRY(run_detect(c, x, 0, nullptr, cons));
         }
         PCP_TRY(run_records(c, area ? x : nullptr));
     }

@@ -177,7 +177,8 @@ int launch_det_reduce(pcp_ctx* c, const double* partials, int nArrays, double* o
                       bool sticky = false);
 int run_cells(pcp_ctx* c, const double2* pos);
 struct PostShake;   // FIRE's post-SHAKE step, done by the detection's bounds kernel (pcp_detect.hip)
-int run_detect(pcp_ctx* c, const double2* pos, int cellsFromList, const PostShake* ps = nullptr);
+int run_detect(pcp_ctx* c, const double2* pos, int cellsFromList, const PostShake* ps = nullptr,
+               bool cons = false);
 int detect_schedule(const pcp_ctx* c);
 int detect_alloc(pcp_ctx* c);
 int detect_stats(pcp_ctx* c, long long* polygonPairs, long long* edgePairs);

@@ -1557,7 +1557,7 @@ int pcp_force_eval(pcp_ctx* c, const pcp_state* st) {
     int mt = st->model & 0xFF;
     int overlap = (mt == PCP_NORMAL || mt == PCP_HYBRID);
     if (overlap) {
-        PCP_TRY(run_detect(c, pos, 0));
+        PCP_TRY(run_detect(c, pos, 0, nullptr, (st->model & PCP_CONSISTENT) != 0));
         PCP_TRY(run_records(c, (st->model & PCP_AREA_MODES) ? pos : nullptr));
     }
     // the overflow check rides on the energy reduction when there is one

@@ -19,7 +19,7 @@
 //   y   = h3 Wo + bo                                            2 outputs per vertex
 // Every product takes bf16 operands (weights and activations rounded to nearest even
 // from float32) and accumulates in float32.  The 128x64 layers -- forward, input
-// gradient and weight gradient -- run on the matrix cores (v_mfma_f32_16x16x16_bf16,
+// gradient and weight gradient -- run on the matrix cores (v_mfma_f32_16x16x32_bf16,
 // one wavefront per 16 vertices); the 4x64 and 64x2 layers run on VALU in a fixed
 // order.  Gradients treat the bf16 rounding as the identity (straight-through).
 // Reductions over vertices are partial sums per chunk of vertices (MPNN_CH for the
@@ -32,7 +32,23 @@
 namespace pcp {
 
 typedef short mpnn_s4 __attribute__((ext_vector_type(4)));
+typedef short mpnn_s8 __attribute__((ext_vector_type(8)));
+typedef __bf16 mpnn_b8 __attribute__((ext_vector_type(8)));
 typedef float mpnn_f4 __attribute__((ext_vector_type(4)));
+
+// One v_mfma_f32_16x16x32_bf16 step: lane (r, g = lane / 16) gives 8 bf16 of row r of A and 8
+// of column r of B, element e of both at the same position k of the 32-long inner dimension
+// (the hardware pairs element e of lane group g of A with element e of lane group g of B).
+// Here the 8 elements are two groups of 4 (lo, hi) in the layout of the 16-wide steps: lane g
+// holds k = 4g..4g+3 of two 16-wide halves, so one 32-wide step covers the same products as
+// two 16-wide ones, in another addition order.
+__device__ __forceinline__ mpnn_s8 cat_s4(mpnn_s4 lo, mpnn_s4 hi) {
+    return (mpnn_s8){lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+}
+__device__ __forceinline__ mpnn_f4 mfma32(mpnn_s8 a, mpnn_s8 b, mpnn_f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mpnn_b8, a), __builtin_bit_cast(mpnn_b8, b), c,
+                                                   0, 0, 0);
+}
 
 constexpr int MH = PCP_MPNN_H;           // hidden width
 constexpr int MD = PCP_MPNN_DEG;         // adjacency slots per vertex
@@ -213,9 +229,9 @@ __global__ __launch_bounds__(256) void k_mpnn_layer(int V, const float* __restri
         mpnn_f4 acc = {0.f, 0.f, 0.f, 0.f};
         const unsigned short* wr = sWt + (16 * t + r) * (2 * MH + WPAD) + 4 * g;
 #pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            mpnn_s4 B = *(const mpnn_s4*)(wr + 16 * s);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[s], B, acc, 0, 0, 0);
+        for (int s = 0; s < 8; s += 2) {
+            mpnn_s8 B = cat_s4(*(const mpnn_s4*)(wr + 16 * s), *(const mpnn_s4*)(wr + 16 * s + 16));
+            acc = mfma32(cat_s4(A[s], A[s + 1]), B, acc);
         }
         int col = 16 * t + r;
         float bias = b[col];
@@ -391,9 +407,9 @@ __global__ __launch_bounds__(256) void k_mpnn_layer_bwd(int V, const float* __re
         mpnn_f4 acc = {0.f, 0.f, 0.f, 0.f};
         const unsigned short* wr = sW + (16 * t + r) * (MH + WPAD) + 4 * g;
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            mpnn_s4 B = *(const mpnn_s4*)(wr + 16 * s);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[s], B, acc, 0, 0, 0);
+        for (int s = 0; s < 4; s += 2) {
+            mpnn_s8 B = cat_s4(*(const mpnn_s4*)(wr + 16 * s), *(const mpnn_s4*)(wr + 16 * s + 16));
+            acc = mfma32(cat_s4(A[s], A[s + 1]), B, acc);
         }
         int col = 16 * t + r;
 #pragma unroll
@@ -435,15 +451,16 @@ __global__ void k_mpnn_gather_bwd(int V, const int* __restrict__ adj, const floa
 
 // Weight gradient of a message layer over one chunk of MPNN_CH vertices:
 // dW[k][n] = sum_m X[m][k] bf(dZ[m][n]) (MFMA, M = 128 inputs, N = 64 outputs, K = vertices)
-// and db[n] = sum_m dZ[m][n] (float32, rows in order).  16 vertices at a time are staged
-// in LDS; wavefront w accumulates the input tiles 2w, 2w+1 against the 4 output tiles.
+// and db[n] = sum_m dZ[m][n] (float32, rows in order).  32 vertices at a time are staged
+// in LDS (one 32-deep MFMA step: lane group g takes vertices 4g..4g+3 and 16+4g..16+4g+3 of
+// the block); wavefront w accumulates the input tiles 2w, 2w+1 against the 4 output tiles.
 // part row: 128*64 entries of dW (row-major) then 64 of db.
 __global__ __launch_bounds__(256) void k_mpnn_layer_grad(int V, const unsigned short* __restrict__ X,
                                                          const float* __restrict__ dH, const float* __restrict__ Z,
                                                          float* __restrict__ part) {
-    __shared__ unsigned short sX[16 * (2 * MH + WPAD)];
-    __shared__ unsigned short sD[16 * (MH + WPAD)];
-    __shared__ float sB[16 * MH];
+    __shared__ unsigned short sX[32 * (2 * MH + WPAD)];
+    __shared__ unsigned short sD[32 * (MH + WPAD)];
+    __shared__ float sB[32 * MH];
     int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     int r = lane & 15, g = lane >> 4;
     int m0 = blockIdx.x * MPNN_CH, m1 = min(V, m0 + MPNN_CH);
@@ -452,60 +469,73 @@ __global__ __launch_bounds__(256) void k_mpnn_layer_grad(int V, const unsigned s
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int t = 0; t < 4; ++t) acc[a][t] = (mpnn_f4){0.f, 0.f, 0.f, 0.f};
-    // bias partial: thread tid owns column tid % 64 of rows with (row % 16) / 4 == tid / 64
+    // bias partial: thread tid owns column tid % 64 of the rows with (row % 16) / 4 == tid / 64,
+    // added in row order
     float bsum = 0.0f;
-    for (int base = m0; base < m1; base += 16) {
-        // stage X rows (16 x 128 bf16): thread -> row tid / 16, 8 elements at 8 (tid % 16)
+    for (int base = m0; base < m1; base += 32) {
+        // stage X rows (32 x 128 bf16): thread -> row tid / 8, 16 elements at 16 (tid % 8)
         {
-            int rr = tid >> 4, c8 = (tid & 15) * 8;
+            int rr = tid >> 3, c16 = (tid & 7) * 16;
             int m = base + rr;
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (m < m1) v = *(const uint4*)(X + (long long)m * (2 * MH) + c8);
-            unsigned short* d = sX + rr * (2 * MH + WPAD) + c8;
-            *(uint2*)d = make_uint2(v.x, v.y);
-            *(uint2*)(d + 4) = make_uint2(v.z, v.w);
-        }
-        // stage dZ rows (16 x 64): thread -> row tid / 16, 4 columns at 4 (tid % 16)
-        {
-            int rr = tid >> 4, c4 = (tid & 15) * 4;
-            int m = base + rr;
-            float4 d = make_float4(0.f, 0.f, 0.f, 0.f), z = make_float4(0.f, 0.f, 0.f, 0.f);
+            uint4 v0 = make_uint4(0u, 0u, 0u, 0u), v1 = make_uint4(0u, 0u, 0u, 0u);
             if (m < m1) {
-                d = *(const float4*)(dH + (long long)m * MH + c4);
-                z = *(const float4*)(Z + (long long)m * MH + c4);
+                v0 = *(const uint4*)(X + (long long)m * (2 * MH) + c16);
+                v1 = *(const uint4*)(X + (long long)m * (2 * MH) + c16 + 8);
             }
-            float4 dz = make_float4(z.x > 0.f ? d.x : 0.f, z.y > 0.f ? d.y : 0.f, z.z > 0.f ? d.z : 0.f,
-                                    z.w > 0.f ? d.w : 0.f);
-            *(mpnn_s4*)(sD + rr * (MH + WPAD) + c4) = pack_bf16(dz.x, dz.y, dz.z, dz.w);
-            *(float4*)(sB + rr * MH + c4) = dz;
+            unsigned short* d = sX + rr * (2 * MH + WPAD) + c16;
+            *(uint2*)d = make_uint2(v0.x, v0.y);
+            *(uint2*)(d + 4) = make_uint2(v0.z, v0.w);
+            *(uint2*)(d + 8) = make_uint2(v1.x, v1.y);
+            *(uint2*)(d + 12) = make_uint2(v1.z, v1.w);
+        }
+        // stage dZ rows (32 x 64): thread -> row tid / 8, 8 columns at 8 (tid % 8)
+        {
+            int rr = tid >> 3, c8 = (tid & 7) * 8;
+            int m = base + rr;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                float4 d = make_float4(0.f, 0.f, 0.f, 0.f), z = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (m < m1) {
+                    d = *(const float4*)(dH + (long long)m * MH + c8 + 4 * h);
+                    z = *(const float4*)(Z + (long long)m * MH + c8 + 4 * h);
+                }
+                float4 dz = make_float4(z.x > 0.f ? d.x : 0.f, z.y > 0.f ? d.y : 0.f, z.z > 0.f ? d.z : 0.f,
+                                        z.w > 0.f ? d.w : 0.f);
+                *(mpnn_s4*)(sD + rr * (MH + WPAD) + c8 + 4 * h) = pack_bf16(dz.x, dz.y, dz.z, dz.w);
+                *(float4*)(sB + rr * MH + c8 + 4 * h) = dz;
+            }
         }
         __syncthreads();
         {
-            // thread tid: column tid % 64, rows 4 (tid / 64) .. +3 of this block of 16
+            // thread tid: column tid % 64, rows 4 (tid / 64) .. +3 of each half of this block of 32
             int c = tid & 63, q = tid >> 6;
 #pragma unroll
             for (int i = 0; i < 4; ++i) bsum = bsum + sB[(4 * q + i) * MH + c];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) bsum = bsum + sB[(16 + 4 * q + i) * MH + c];
         }
-        mpnn_s4 Bop[4];
+        mpnn_s8 Bop[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            mpnn_s4 b;
-            b.x = (short)sD[(4 * g + 0) * (MH + WPAD) + 16 * t + r];
-            b.y = (short)sD[(4 * g + 1) * (MH + WPAD) + 16 * t + r];
-            b.z = (short)sD[(4 * g + 2) * (MH + WPAD) + 16 * t + r];
-            b.w = (short)sD[(4 * g + 3) * (MH + WPAD) + 16 * t + r];
+            mpnn_s8 b;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                b[e] = (short)sD[(4 * g + e) * (MH + WPAD) + 16 * t + r];
+                b[4 + e] = (short)sD[(16 + 4 * g + e) * (MH + WPAD) + 16 * t + r];
+            }
             Bop[t] = b;
         }
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
             int kcol = 16 * (2 * wave + a) + r;
-            mpnn_s4 Aop;
-            Aop.x = (short)sX[(4 * g + 0) * (2 * MH + WPAD) + kcol];
-            Aop.y = (short)sX[(4 * g + 1) * (2 * MH + WPAD) + kcol];
-            Aop.z = (short)sX[(4 * g + 2) * (2 * MH + WPAD) + kcol];
-            Aop.w = (short)sX[(4 * g + 3) * (2 * MH + WPAD) + kcol];
+            mpnn_s8 Aop;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) acc[a][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Aop, Bop[t], acc[a][t], 0, 0, 0);
+            for (int e = 0; e < 4; ++e) {
+                Aop[e] = (short)sX[(4 * g + e) * (2 * MH + WPAD) + kcol];
+                Aop[4 + e] = (short)sX[(16 + 4 * g + e) * (2 * MH + WPAD) + kcol];
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[a][t] = mfma32(Aop, Bop[t], acc[a][t]);
         }
         __syncthreads();
     }

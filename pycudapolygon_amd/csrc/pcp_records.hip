@@ -973,7 +973,9 @@ int autotune(pcp_ctx* c, const pcp_state* st, int reps, double* timesUs) {
         const double2* xd = (const double2*)st->pos;
         for (int m = 0; m < (c->maxN <= 64 ? 2 : 1); ++m) {
             c->detSched = m;
-            PCP_TRY(time_launches(c, reps, &t[9 + m], [&]() { return run_detect(c, xd, 0); }));
+            PCP_TRY(time_launches(c, reps, &t[9 + m], [&]() {
+                return run_detect(c, xd, 0, nullptr, (st->model & PCP_CONSISTENT) != 0);
+            }));
         }
         c->detSched = (t[10] > 0.0 && t[10] < t[9]) ? 1 : (t[9] > 0.0 ? 0 : keepDet);
         c->graphK = 0;   // a captured FIRE block holds the old schedule

@@ -35,7 +35,7 @@ else:
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--batches", type=int, default=16)
+    ap.add_argument("--batches", type=int, default=157)
     ap.add_argument("--packings", type=int, default=64)
     ap.add_argument("--N", type=int, default=1024)
     ap.add_argument("--n", type=int, default=16)
@@ -50,6 +50,8 @@ def main(argv=None):
     ap.add_argument("--eval-packings", type=int, default=32)
     ap.add_argument("--eval-seed", type=int, default=777777)
     ap.add_argument("--applications", type=int, default=1)
+    ap.add_argument("--eval-dt0", default="0.001",
+                    help="comma-separated initial FIRE time steps of the A/B comparison (both arms)")
     ap.add_argument("--mode", choices=["fresh", "compression"], default="fresh",
                     help="fresh: FIRE trajectories from new packings (the defaults above); compression: "
                          "states just after each packing-fraction step of a ramp, labelled with the state "
@@ -98,10 +100,11 @@ def main(argv=None):
     del ds, runs, trn, val
     if args.eval_packings > 0 and args.mode == "fresh":
         pk = W.make_c4_packings(args.eval_packings, args.eval_seed, N=args.N, n=args.n)
-        res = W.ab_compare(W.load_weights(args.out), pk, maxSteps=args.max_steps,
-                           applications=args.applications, log=log, dtMax=args.dt_max)
-        res["phase"] = "ab"
-        print(json.dumps(res), flush=True)
+        for dt0 in (float(v) for v in args.eval_dt0.split(",")):
+            res = W.ab_compare(W.load_weights(args.out), pk, maxSteps=args.max_steps,
+                               applications=args.applications, log=log, dtMax=args.dt_max, dt0=dt0)
+            res["phase"] = "ab"
+            print(json.dumps(res), flush=True)
 
 
 if __name__ == "__main__":

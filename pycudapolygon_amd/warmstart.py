@@ -404,15 +404,17 @@ def train(samples, params, epochs, lr=1e-3, log=print, val=(), lr_final=None):
     return net.get_params(), losses, vlosses
 
 
-def ab_compare(weights, packings, maxSteps=12000, applications=1, at_step=0, gamma=1.0, log=print, dtMax=0.1):
+def ab_compare(weights, packings, maxSteps=12000, applications=1, at_step=0, gamma=1.0, log=print, dtMax=0.1,
+               dt0=1e-3):
     """Plain FIRE against FIRE with the learned displacement applied (one System each, same
     packings).  The learned run does `at_step` plain FIRE steps first (0 = a warm start of
     the initial packing), then applies the network `applications` times (each after a fresh
     evaluation, displacement scaled by gamma), then FIRE to convergence from a fresh FIRE
-    state.  Steps per packing count both FIRE phases; seconds include the network."""
+    state.  Both start FIRE at time step dt0.  Steps per packing count both FIRE phases;
+    seconds include the network."""
     ell = packing_length_units(packings)
     s0, prm = fire_system(packings, dtMax=dtMax)
-    conv0, steps0, t0 = fire_run(s0, prm, maxSteps)
+    conv0, steps0, t0 = fire_run(s0, prm, maxSteps, dt0=dt0)
     E0 = s0.get_energy()
     F0 = s0.max_abs_force()
     s1, prm1 = fire_system(packings, dtMax=dtMax)
@@ -420,23 +422,24 @@ def ab_compare(weights, packings, maxSteps=12000, applications=1, at_step=0, gam
     ws.outScale *= gamma
     pre, tpre = np.zeros(s1.P, int), 0.0
     if at_step > 0:
-        c, n1, tpre = fire_run(s1, prm1, at_step)
+        c, n1, tpre = fire_run(s1, prm1, at_step, dt0=dt0)
         pre[:] = at_step
         pre[c >= 0] = c[c >= 0]
     tw = 0.0
     for _ in range(applications):
         s1.force_eval()
         tw += ws.apply(s1, ell)
-    conv1, steps1, t1 = fire_run(s1, prm1, maxSteps)
+    conv1, steps1, t1 = fire_run(s1, prm1, maxSteps, dt0=dt0)
     conv1 = np.where(conv1 >= 0, conv1 + pre, -1)
     E1 = s1.get_energy()
     F1 = s1.max_abs_force()
-    res = dict(packings=len(packings), at_step=at_step, gamma=gamma, applications=applications,
+    res = dict(packings=len(packings), at_step=at_step, gamma=gamma, applications=applications, dt0=dt0,
                plain_steps=conv0.tolist(), warm_steps=conv1.tolist(),
                plain_seconds=t0, warm_seconds=tpre + t1 + tw, warm_apply_seconds=tw,
                plain_energy=E0.tolist(), warm_energy=E1.tolist(), plain_maxF=F0.tolist(), warm_maxF=F1.tolist())
     ok0, ok1 = conv0 >= 0, conv1 >= 0
-    log(f"[warmstart] A/B on {len(packings)} packings (learned at step {at_step}, gamma {gamma}): plain mean steps "
+    log(f"[warmstart] A/B on {len(packings)} packings (dt0 {dt0:g}, learned at step {at_step}, gamma {gamma}): "
+        f"plain mean steps "
         f"{conv0[ok0].mean() if ok0.any() else -1:.0f} ({ok0.sum()} converged, {t0:.2f} s); learned mean steps "
         f"{conv1[ok1].mean() if ok1.any() else -1:.0f} ({ok1.sum()} converged, {tpre + t1 + tw:.2f} s incl. "
         f"{tw:.3f} s network)")

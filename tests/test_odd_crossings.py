@@ -61,3 +61,67 @@ def test_odd_check_on_hand_built_keys():
         return (sj << 48) | (si << 32) | (i << 16) | j
     keys = [key(1, 2, 0, 1), key(1, 2, 3, 1), key(2, 1, 5, 2), key(0, 3, 1, 1), key(3, 0, 2, 2), key(4, 4, 0, 5)]
     assert O.odd_crossing_records(keys) == (3, [(1, 2)])
+
+
+# ---- the side-consistent decision (ORC_CONSISTENT / PCP_CONSISTENT) -----------------------
+
+def _spike_configuration(rng):
+    """A square B (random turn) and a kite A whose tip v lies within a few ulps of B's first
+    edge, v's two edges leaving on the same side of that edge's line (both cross it or neither,
+    geometrically) or across it"""
+    th = rng.uniform(0, 2 * np.pi)
+    R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
+    B = (np.array([[-1, -1], [1, -1], [1, 1], [-1, 1]]) * 0.1) @ R.T + 0.5
+    b0, b1 = B[0], B[1]
+    v = b0 + rng.uniform(0.2, 0.8) * (b1 - b0)
+    v = v + rng.integers(-3, 4, 2) * np.spacing(v)
+    e = (b1 - b0) / np.linalg.norm(b1 - b0)
+    n = np.array([-e[1], e[0]])
+    side = rng.choice([-1.0, 1.0])
+    vm, vp, w = v - 0.02 * e + side * 0.03 * n, v + 0.02 * e + side * 0.03 * n, v + side * 0.06 * n
+    A = np.array([v, vp, w, vm]) if side < 0 else np.array([v, vm, w, vp])
+    return np.concatenate([A.reshape(-1), B.reshape(-1)]), np.array([0, 4, 8], np.int32)
+
+
+def test_side_consistent_crossings_are_even_near_a_vertex_on_an_edge():
+    """Two closed polygons cross an even number of times.  With a vertex within a few ulps of
+    the other polygon's edge the parameter test (the reference's) counts an odd number in a
+    sizeable share of configurations; the side-consistent decision never does (a side computed
+    from anything but the edge's own vertices and the point, e.g. from P + R, fails this)."""
+    rng = np.random.default_rng(0)
+    odd = {False: 0, True: 0}
+    for _ in range(1000):
+        pos, start = _spike_configuration(rng)
+        for cons in (False, True):
+            odd[cons] += int(O.neighbors(pos, start, 1, consistent=cons)["numNeighbors"].sum()) % 2
+    assert odd[False] > 50 and odd[True] == 0
+
+
+def test_side_consistent_closes_the_window_at_the_stall(stalled):
+    """At the packing-41 stall, 1e-16 along the force: with the side-consistent decision both
+    crossings of the tiny lens are gone together (292 records, every pair even), and the energy
+    changes only by the lens merge of squared lens areas (+2.2e-12), not by +3.7e-9."""
+    p, B, sysO, x, f, st, steps = stalled
+    sysC = O.System(p["startIndices"], B, O.NORMAL | SQUARED | O.CONSISTENT, p["targetEdgeV"], p["targetAreas"],
+                    solver="lanes")
+    E0, F0, n0 = sysC.force_eval(x)
+    assert n0 == 294
+    for h in (1e-17, 1e-16):
+        y = x + h * F0 / np.abs(F0).max()
+        y -= np.floor(y)
+        E1, _, n1 = sysC.force_eval(y)
+        rec = O.outersections(p["startIndices"], O.neighbors(y, p["startIndices"], B, consistent=True))
+        assert n1 == 292 and rec["numIntersections"] == 292
+        assert O.odd_crossing_records(rec["intersections"]) == (0, [])
+        assert 1.5e-12 < E1 - E0 < 3.0e-12
+
+
+@pytest.mark.parametrize("N,n,phi,seed", [(64, 12, 0.80, 0), (1024, 16, 0.88, 5)])
+def test_side_consistent_equals_the_reference_on_generic_packings(N, n, phi, seed):
+    from pycudapolygon_amd.generate import make_packing
+    p = make_packing(N, n, phi, seed=seed, min_center_frac=0.5)
+    B = box_size_for(p["diameter"])
+    a = O.neighbors(p["positions"], p["startIndices"], B)
+    b = O.neighbors(p["positions"], p["startIndices"], B, consistent=True)
+    for k in ("neighbors", "inside", "tuSlots", "numNeighbors"):
+        assert np.array_equal(a[k], b[k]), k
