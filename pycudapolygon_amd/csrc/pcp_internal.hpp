@@ -120,6 +120,8 @@ struct pcp_ctx {
     double* dAccR = nullptr;       // N: record + interior energy sums
     double* dEr = nullptr;         // per record: its h sums (squared lens energy)
     double* dLaRep = nullptr;      // per record: its lens area if it represents the lens, else 0
+    char* dExtStage = nullptr;     // k_exterior_poly's record staging when a bucket outgrows LDS
+    size_t extStageBytes = 0;
 
     // per-vertex / per-polygon scratch
     double* dEpoly = nullptr;

@@ -1021,6 +1021,7 @@ static void free_topology(pcp_ctx* c) {
     dfree(c->dF1); dfree(c->dLM); dfree(c->dLN); dfree(c->dAccR); dfree(c->dEr); dfree(c->dLaRep);
     dfree(c->dEpoly); dfree(c->dWork); dfree(c->dRed);
     dfree(c->dRedOut); dfree(c->dScanPart); c->scanPartCap = 0;
+    dfree(c->dExtStage); c->extStageBytes = 0;
     dfree(c->dXbak); dfree(c->dXpre); dfree(c->dVbak); dfree(c->dDt); dfree(c->dAlpha);
     dfree(c->dFbak); dfree(c->dFext); dfree(c->dFextBak); dfree(c->dSnapX); dfree(c->dSnapV); dfree(c->dSnapF); dfree(c->dSnapD); dfree(c->dSnapI);
     dfree(c->dBlockOver);

@@ -24,7 +24,9 @@
 
 namespace pcp {
 
-constexpr int REC_MAX_KP = 512;   // LDS of k_exterior_poly: Kp * 116 bytes per polygon (<= 64 KiB)
+// k_exterior_poly keeps a polygon's records in LDS (Kp * 116 bytes, at most 64 KiB per
+// workgroup); beyond that (Kp > 564) in a global staging buffer of the context
+constexpr size_t EXT_LDS_MAX = 65536;
 
 // ---------------------------------------------------------------- bucket scans
 // Scans of a short bucket (a few to a few tens of keys) in groups of 8 with all the
@@ -385,7 +387,10 @@ __global__ __launch_bounds__(64) void k_lens_energy(int N, int Kp, const int* __
 // One wavefront holds 64 / SEGX polygons, SEGX lanes each: lanes take the polygon's
 // records, then its vertices, SEGX at a time (a polygon has a few records, so a
 // wavefront per polygon would leave most lanes idle).  Every sum keeps its order.
-template <int MODE, int SEGX>
+// GLOB: the per-polygon record staging lives in global memory (gstage, ext_lds_stride(Kp)
+// bytes per polygon slot of the launch) instead of LDS, for buckets too large for LDS; the
+// workgroup barriers order it as they order LDS (one wavefront per workgroup, one CU).
+template <int MODE, int SEGX, bool GLOB = false>
 __global__ __launch_bounds__(64) void k_exterior_poly(
     int N, int Kp, const int* __restrict__ polyStart, const int* __restrict__ polyPack,
     const int* __restrict__ packPoly, const double2* __restrict__ pos, const int* __restrict__ bcnt,
@@ -393,13 +398,14 @@ __global__ __launch_bounds__(64) void k_exterior_poly(
     const uint64_t* __restrict__ rOuter, const float2* __restrict__ rTU, const float2* __restrict__ rUT,
     double2* __restrict__ jC, int* __restrict__ jT, double2* __restrict__ F1, double2* __restrict__ LMo,
     double2* __restrict__ LNo, double* __restrict__ accR, const double* __restrict__ Er,
-    const double* __restrict__ laRep, const int* __restrict__ order) {
+    const double* __restrict__ laRep, const int* __restrict__ order, char* __restrict__ gstage) {
     constexpr int AREA = (MODE >= 1), SQ = (MODE >= 2), PAIR = (MODE == 3);
     constexpr int G = 64 / SEGX;
     extern __shared__ char smem[];
     __shared__ double leBuf[64];
     int lane = threadIdx.x & (SEGX - 1), g = threadIdx.x / SEGX, segBase = g * SEGX;
-    ExtLds L = ext_lds(smem + g * ext_lds_stride(Kp), Kp);
+    ExtLds L = GLOB ? ext_lds(gstage + ((size_t)blockIdx.x * G + g) * ext_lds_stride(Kp), Kp)
+                    : ext_lds(smem + g * ext_lds_stride(Kp), Kp);
     int slot = xcd_block(blockIdx.x, gridDim.x) * G + g;
     bool has = slot < N;
     int s = has ? order[slot] : 0;
@@ -762,11 +768,12 @@ __global__ __launch_bounds__(BLK) void k_gather_poly(
 
 // ---------------------------------------------------------------- host side
 int grow_buckets(pcp_ctx* c, int need) {
-    if (need > REC_MAX_KP) {
-        set_error("more than 512 crossings on one polygon");
+    // record keys hold 16-bit indices; a bucket never needs more than 2^20 slots
+    if (need > (1 << 20)) {
+        set_error("more than 2^20 crossings on one polygon");
         return PCP_ERR_CAPACITY;
     }
-    int Kp = std::min(std::max(need + need / 4, 2 * c->Kp), REC_MAX_KP);
+    int Kp = std::min(std::max(need + need / 4, 2 * c->Kp), 1 << 20);
     return alloc_buckets(c, Kp);
 }
 
@@ -851,15 +858,24 @@ static int dispatch_gather(pcp_ctx* c, int overlap, int edgeOn, int areaOn, cons
 // them (fused evaluation); else read them.  project: project the force of every polygon.
 // k_exterior_poly with as many polygons per wavefront as the LDS allows (64 KiB per
 // workgroup): 4 (16 lanes each) for buckets up to 141 records, then 2, then 1
-template <int MODE, int SEGX>
+template <int MODE, int SEGX, bool GLOB = false>
 static int launch_exterior_w(pcp_ctx* c, const double2* pos) {
     constexpr int G = 64 / SEGX;
-    size_t lds = G * ext_lds_stride(c->Kp);
+    size_t lds = GLOB ? 0 : G * ext_lds_stride(c->Kp);
     int grid = (c->N + G - 1) / G;
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_exterior_poly<MODE, SEGX>), dim3(grid), dim3(64), lds, c->stream, c->N,
-                       c->Kp, c->dPolyStart, c->dPolyPack, c->dPackPoly, pos, c->dBcnt, c->dJcnt, c->dJKey, c->dRKey,
-                       c->dROuter, c->dRTU, c->dRUT, c->dJC, c->dJT, c->dF1, c->dLM, c->dLN, c->dAccR, c->dEr,
-                       c->dLaRep, c->dPcSorted);
+    if (GLOB) {   // the staging buffer: one polygon slot per wavefront segment of the launch
+        size_t need = (size_t)grid * G * ext_lds_stride(c->Kp);
+        if (need > c->extStageBytes) {
+            dfree(c->dExtStage);
+            c->extStageBytes = 0;
+            PCP_TRY(dalloc(&c->dExtStage, need));
+            c->extStageBytes = need;
+        }
+    }
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_exterior_poly<MODE, SEGX, GLOB>), dim3(grid), dim3(64), lds, c->stream,
+                       c->N, c->Kp, c->dPolyStart, c->dPolyPack, c->dPackPoly, pos, c->dBcnt, c->dJcnt, c->dJKey,
+                       c->dRKey, c->dROuter, c->dRTU, c->dRUT, c->dJC, c->dJT, c->dF1, c->dLM, c->dLN, c->dAccR,
+                       c->dEr, c->dLaRep, c->dPcSorted, GLOB ? c->dExtStage : (char*)nullptr);
     PCP_LAUNCHED();
     return PCP_OK;
 }
@@ -871,10 +887,11 @@ static int launch_exterior_m(pcp_ctx* c, const double2* pos) {
     PCP_TRY(prof_begin(c, names[MODE], &slot));
     size_t st = ext_lds_stride(c->Kp);
     int w = c->segExt;
-    if (w == 16 && 4 * st > 65536) w = 0;   // a chosen width the LDS does not fit: the default rule
-    if (w == 32 && 2 * st > 65536) w = 0;
-    if (w == 0) w = (4 * st <= 65536) ? 16 : ((2 * st <= 65536) ? 32 : 64);
-    if (w == 16) PCP_TRY((launch_exterior_w<MODE, 16>(c, pos)));
+    if (w == 16 && 4 * st > EXT_LDS_MAX) w = 0;   // a chosen width the LDS does not fit: the default rule
+    if (w == 32 && 2 * st > EXT_LDS_MAX) w = 0;
+    if (w == 0) w = (4 * st <= EXT_LDS_MAX) ? 16 : ((2 * st <= EXT_LDS_MAX) ? 32 : 64);
+    if (st > EXT_LDS_MAX) PCP_TRY((launch_exterior_w<MODE, 64, true>(c, pos)));
+    else if (w == 16) PCP_TRY((launch_exterior_w<MODE, 16>(c, pos)));
     else if (w == 32) PCP_TRY((launch_exterior_w<MODE, 32>(c, pos)));
     else PCP_TRY((launch_exterior_w<MODE, 64>(c, pos)));
     return prof_end(c, slot);

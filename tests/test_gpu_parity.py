@@ -404,6 +404,39 @@ def test_bucket_overflow_regrows_and_polygons_beyond_64_vertices():
     assert np.abs((s.get_positions(0) - xo + 0.5) % 1.0 - 0.5).max() < 1e-13 and abs(int(it[0]) - ito) <= 1
 
 
+@pytest.mark.parametrize("model", [0, 0x400])
+def test_buckets_beyond_the_lds_stage(model):
+    """Two noisy 3000-gons crossing 1444 times (722 records per polygon, where the earlier
+    design stopped at 512): the buckets grow past what k_exterior_poly can stage in LDS
+    (Kp * 116 bytes > 64 KiB), so the exterior forces stage the records in global memory;
+    the records equal the oracle's bit for bit, the energy exactly, the forces to 1e-12
+    (reference energy and squared pair areas)."""
+    pk = _two_noisy_polygons(3000, 0.05, 5)
+    start, pos = pk["startIndices"], pk["positions"]
+    nb, rec = oracle_records(pk, 2)
+    assert rec["numIntersections"] == 1444 and list(rec["shapeCounts"]) == [722, 722]
+    print(f"[buckets] 2 x 3000-gons, model {model:#x}", flush=True)
+    s = capi.System([pk], [2], model=capi.NORMAL | model)
+    s.force_eval()
+    with pytest.raises(capi.PcpError, match="overflow"):
+        s.sync()
+    s.force_eval()
+    s.sync()
+    r = s.get_records(0)
+    for key in ("intersections", "shapeCounts"):
+        np.testing.assert_array_equal(r[key], rec[key], err_msg=key)
+    if model == 0:
+        for key in ("outersections", "tu", "ut"):
+            np.testing.assert_array_equal(r[key], rec[key], err_msg=key)
+    F, E = s.get_forces(0), s.get_energy()[0]
+    # the lane-order oracle (its projection of a 3000-gon is a linear-time solve; the dense
+    # one would take minutes)
+    Eo, Fo, nIo = O.System(start, 2, model, pk["targetEdgeV"], pk["targetAreas"], solver="lanes").force_eval(pos)
+    assert nIo == 1444 and E == Eo
+    assert np.abs(F - Fo).max() <= 1e-12 * np.abs(Fo).max()
+    s.close()
+
+
 def test_packing_without_overlaps():
     """A dilute packing (phi = 0.3, no two centres closer than the sum of the radii): no
     crossings, zero records, zero energy and force on the GPU as in the oracle, and FIRE
