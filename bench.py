@@ -81,6 +81,8 @@ def _pmc_pass(counter, child_args, kernel_prefix, timeout_s=90):
                 with open(os.path.join(root, f)) as fh:
                     for r in csv.DictReader(fh):
                         name = r["Kernel_Name"].replace("void ", "").replace("pcp::", "")
+                        # a defaulted last template argument is spelled out in the symbol
+                        name = name.replace(", false>", ">")
                         if name.startswith(kernel_prefix) and r["Counter_Name"] == counter:
                             vals.append(float(r["Counter_Value"]))
     shutil.rmtree(d, ignore_errors=True)
@@ -258,6 +260,31 @@ def fire_legs(args):
             break
         out[tag] = fire_one([p2], dt, steps, warm, budget_s=args.fire_seconds, graph=not args.no_graph, dt0=dt0)
         log(out[tag], tag)
+    if "n4096" in legs:
+        # FIRE on the reference's own energy (model normal, no area flag), uniform centres (the
+        # configuration's generator), the reference driver's dtMax 0.1 and dt0 1e-3: the energy is
+        # not the potential of its force, so FIRE does not reach max|F| <= 1e-10 (it stalls on
+        # rolled-back steps); what is measured is its rate, FIRE steps per second over 2000 steps
+        from pycudapolygon_amd import capi
+        pu = make_packing(4096, 16, 0.85, seed=args.seed)
+        s = capi.System([pu], [box_size_for(pu["diameter"])], model=capi.NORMAL)
+        prm = s.fire_params(fTol=1e-10, dtMax=0.1)
+        s.fire_init(prm, 1e-3)
+        s.fire_run(prm, 16, 8, not args.no_graph)   # warm-up (graph capture)
+        s.sync()
+        t0 = time.perf_counter()
+        st = s.fire_run(prm, 2000, 8, not args.no_graph)[0]
+        s.sync()
+        el = time.perf_counter() - t0
+        E0 = float(s.get_energy()[0])
+        out["n4096_reference_energy"] = dict(
+            workload="N=4096 x 16, phi=0.85, uniform centres, reference energy (model normal), dtMax 0.1, dt0 1e-3",
+            fire_steps=2000, seconds=el, fire_steps_per_s=2000 / el, ms_per_fire_step=1e3 * el / 2000,
+            converged=bool(st.converged), final_max_force=float(st.maxForce), final_energy=E0, final_dt=float(st.dt),
+            fire="pcp_fire_run, blocks of 8 steps" + ("" if args.no_graph else " replayed from a HIP graph"))
+        print(f"[bench] FIRE n4096_reference_energy: {2000 / el:.0f} FIRE steps/s ({1e3 * el / 2000:.3f} ms/step), "
+              f"max|F| {st.maxForce:.2e} after 2016 steps, converged {bool(st.converged)}", flush=True)
+        s.close()
     if "xpbd" in legs:
         # the N=4096 configuration with area and perimeter constraints of stiffness 1e3 integrated
         # by XPBD (hybrid model: squared pair areas + the two springs; the springs by XPBD at the
