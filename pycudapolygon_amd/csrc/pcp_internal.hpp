@@ -70,7 +70,7 @@ struct pcp_ctx {
     hipStream_t stream = nullptr;
     int device = 0;      // the device current at pcp_create (the stream's)
     int P = 0, N = 0, V = 0;
-    int Kp = 64;         // records per polygon bucket (grown on demand)
+    int Kp = 16;         // records per polygon bucket (grown on demand)
     int M = 100;         // reference maxNeighbors for the neighbour-list getters
     long long Ctot = 0;  // total cells over all packings
     bool haveBoxes = false;

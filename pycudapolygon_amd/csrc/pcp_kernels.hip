@@ -1131,7 +1131,10 @@ int pcp_set_topology(pcp_ctx* c, int P, const int* packPolyStart, const int* pol
     PCP_TRY(dalloc(&c->dBcnt, N)); PCP_TRY(dalloc(&c->dJcnt, N));
     PCP_HIP(hipMemsetAsync(c->dBcnt, 0, sizeof(int) * (size_t)std::max(N, 1), c->stream));
     PCP_HIP(hipMemsetAsync(c->dJcnt, 0, sizeof(int) * (size_t)std::max(N, 1), c->stream));
-    PCP_TRY(alloc_buckets(c, 64));
+    // a polygon of a packing rarely has more than ~10 records; buckets start at 16 and grow on
+    // overflow (the exterior kernel stages Kp records per polygon in LDS, so a small Kp keeps
+    // more of its workgroups resident)
+    PCP_TRY(alloc_buckets(c, 16));
     PCP_TRY(dalloc(&c->dF1, V)); PCP_TRY(dalloc(&c->dLM, V)); PCP_TRY(dalloc(&c->dLN, V));
     PCP_TRY(dalloc(&c->dAccR, N)); PCP_TRY(dalloc(&c->dEpoly, N));
     PCP_TRY(dalloc(&c->dWork, 8LL * V)); PCP_TRY(dalloc(&c->dRed, 4LL * N)); PCP_TRY(dalloc(&c->dRedOut, 4LL * P));

@@ -374,7 +374,7 @@ def _two_noisy_polygons(n, noise, seed):
 
 def test_bucket_overflow_regrows_and_polygons_beyond_64_vertices():
     """Two noisy 384-gons crossing 168 times (84 records per polygon, above the initial
-    64-entry record buckets): the first evaluation reports PCP_ERR_CAPACITY at the next
+    16-entry record buckets): the first evaluation reports PCP_ERR_CAPACITY at the next
     synchronisation with the buckets enlarged; the evaluation after it gives the oracle's
     records bit for bit and its energy exactly, and forces within 1e-12 (polygons beyond 64
     vertices are projected by the thread-serial solver).  SHAKE of those polygons matches
@@ -404,19 +404,20 @@ def test_bucket_overflow_regrows_and_polygons_beyond_64_vertices():
     assert np.abs((s.get_positions(0) - xo + 0.5) % 1.0 - 0.5).max() < 1e-13 and abs(int(it[0]) - ito) <= 1
 
 
-@pytest.mark.parametrize("model", [0, 0x400])
-def test_buckets_beyond_the_lds_stage(model):
+@pytest.mark.parametrize("flag", [0, 0x400])
+def test_buckets_beyond_the_lds_stage(flag):
     """Two noisy 3000-gons crossing 1444 times (722 records per polygon, where the earlier
     design stopped at 512): the buckets grow past what k_exterior_poly can stage in LDS
-    (Kp * 116 bytes > 64 KiB), so the exterior forces stage the records in global memory;
-    the records equal the oracle's bit for bit, the energy exactly, the forces to 1e-12
-    (reference energy and squared pair areas)."""
+    (Kp * 116 bytes > 64 KiB), so the exterior forces stage the records in global memory.
+    The hybrid model (overlap forces with zero springs, no projection) shows the exterior
+    forces themselves: records, energy and forces equal the oracle's bit for bit, with the
+    reference energy and with squared pair areas."""
     pk = _two_noisy_polygons(3000, 0.05, 5)
     start, pos = pk["startIndices"], pk["positions"]
     nb, rec = oracle_records(pk, 2)
     assert rec["numIntersections"] == 1444 and list(rec["shapeCounts"]) == [722, 722]
-    print(f"[buckets] 2 x 3000-gons, model {model:#x}", flush=True)
-    s = capi.System([pk], [2], model=capi.NORMAL | model)
+    print(f"[buckets] 2 x 3000-gons, flag {flag:#x}", flush=True)
+    s = capi.System([pk], [2], model=capi.HYBRID | flag)
     s.force_eval()
     with pytest.raises(capi.PcpError, match="overflow"):
         s.sync()
@@ -425,15 +426,12 @@ def test_buckets_beyond_the_lds_stage(model):
     r = s.get_records(0)
     for key in ("intersections", "shapeCounts"):
         np.testing.assert_array_equal(r[key], rec[key], err_msg=key)
-    if model == 0:
+    if flag == 0:
         for key in ("outersections", "tu", "ut"):
             np.testing.assert_array_equal(r[key], rec[key], err_msg=key)
     F, E = s.get_forces(0), s.get_energy()[0]
-    # the lane-order oracle (its projection of a 3000-gon is a linear-time solve; the dense
-    # one would take minutes)
-    Eo, Fo, nIo = O.System(start, 2, model, pk["targetEdgeV"], pk["targetAreas"], solver="lanes").force_eval(pos)
-    assert nIo == 1444 and E == Eo
-    assert np.abs(F - Fo).max() <= 1e-12 * np.abs(Fo).max()
+    Eo, Fo, nIo = O.System(start, 2, O.HYBRID | flag, pk["targetEdgeV"], pk["targetAreas"]).force_eval(pos)
+    assert nIo == 1444 and E == Eo and np.array_equal(F, Fo)
     s.close()
 
 
