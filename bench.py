@@ -323,7 +323,8 @@ def fire_legs(args):
             out[tag] = dict(packings=P, workload=(f"{P} packings of N=4096 x 16, phi=0.85 (seeds {args.seed} and "
                                                  f"{args.seed + 1000}..{args.seed + 998 + P}), one context, "
                                                  "re-packed when fewer than half are still running"),
-                            warm_start=warm is not None, converged=int(ok.sum()), seconds=r.seconds,
+                            warm_start=warm is not None, warm_guard_taken_back=r.warm_taken_back,
+                            converged=int(ok.sum()), seconds=r.seconds,
                             packings_per_s=float(ok.sum()) / r.seconds,
                             mean_steps=float(r.steps[ok].mean()) if ok.any() else None,
                             max_steps=int(r.steps[ok].max()) if ok.any() else None, contexts=r.contexts,
@@ -333,7 +334,8 @@ def fire_legs(args):
                             packing0_final_energy=float(r.energy[0]),
                             packing0_final_energy_rel_diff_single_context=(abs(float(r.energy[0]) - e0) / abs(e0)
                                                                            if single else None))
-            print(f"[bench] FIRE {tag}: {int(ok.sum())}/{P} converged, mean steps {out[tag]['mean_steps']}, "
+            print(f"[bench] FIRE {tag}: {int(ok.sum())}/{P} converged (guard set back {r.warm_taken_back}), "
+                  f"mean steps {out[tag]['mean_steps']}, "
                   f"{r.seconds:.3f} s, {out[tag]['packings_per_s']:.2f} packings/s; packing 0: "
                   f"{r.steps[0]} steps, E rel. diff vs single context "
                   f"{out[tag]['packing0_final_energy_rel_diff_single_context']}", flush=True)

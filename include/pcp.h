@@ -414,6 +414,15 @@ int pcp_mpnn_targets(pcp_ctx* ctx, const double* xFrom, const double* xTo, const
  * (active device[P] or NULL = all) */
 int pcp_mpnn_apply(pcp_ctx* ctx, double* pos, const float* out, const double* ell, double outScale,
                    const int* active);
+/* guard of the warm start: an edge-crossing energy cannot see a polygon that lies inside
+ * another.  Where the bounding circle of one polygon (of pos) lies inside another's,
+ * |c_A - c_B| + min(r_A, r_B) < kappa * max(r_A, r_B), both polygons take back their
+ * positions xPrev (device[2V], e.g. pos before pcp_mpnn_apply); a packing that still holds
+ * such a pair afterwards takes back all of its positions.  flags device[P] (out): 0 kept,
+ * 1 polygons taken back, 2 the packing taken back.  nTaken host[1] (optional; synchronizes):
+ * the number of packings with flags != 0.  kappa in [0, 2], else PCP_ERR_ARG.  Resets the
+ * context's records (a move makes them stale). */
+int pcp_mpnn_guard(pcp_ctx* ctx, double* pos, const double* xPrev, double kappa, int* flags, int* nTaken);
 
 #ifdef __cplusplus
 }

@@ -26,6 +26,7 @@ class BatchResult:
         self.contexts = []                   # packings per context used, in order
         self.schedule = None                 # lanes per polygon (pair, lens, exterior) when autotuned
         self.schedule_us = None              # the autotuner's microseconds per candidate launch
+        self.warm_taken_back = None          # packings the warm start's guard set back (pcp_mpnn_guard)
 
 
 def _with_positions(p, x):
@@ -65,6 +66,7 @@ def minimize_batch(packings, model, fire_kwargs, maxSteps, dt0=1e-3, compact_bel
         if warm is not None and state is None:
             s.force_eval()
             warm.apply(s, np.array([float(np.mean(p["targetEdgeV"])) for p in cur]))
+            res.warm_taken_back = warm.taken_back
         s.fire_init(prm, dt0)
         if state is not None:
             vel, dt, alpha, nPos = state

@@ -108,6 +108,7 @@ SIGNATURES = {
                            ctypes.c_float]),
     "pcp_mpnn_targets": (_i, [_p, _p, _p, _p, _d, _p]),
     "pcp_mpnn_apply": (_i, [_p, _p, _p, _p, _d, _p]),
+    "pcp_mpnn_guard": (_i, [_p, _p, _p, _d, _p, _p]),
 }
 
 _lib = None

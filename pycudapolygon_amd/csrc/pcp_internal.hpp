@@ -139,6 +139,7 @@ struct pcp_ctx {
     double* dFireScal = nullptr;   // per packing: bend scale, 1-alpha, maxF, backups
     int* dShakeIt = nullptr;       // per packing: [0] call-1 max, [1] call-2 max
     double* dCom = nullptr;        // 2N scratch (COM of internal geometry passes)
+    int* dGuard = nullptr;         // N: polygons taken back by pcp_mpnn_guard (allocated there)
     char* dStatusBlock = nullptr;  // summary (16 bytes) then status[P], read back in one copy
     std::vector<char> hStatusBlock;
     pcp_fire_status* dStatus = nullptr;   // inside dStatusBlock

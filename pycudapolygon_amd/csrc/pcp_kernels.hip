@@ -1019,7 +1019,7 @@ static void free_topology(pcp_ctx* c) {
     dfree(c->dBcnt); dfree(c->dJcnt); dfree(c->dBKey); dfree(c->dBTU); dfree(c->dJKey); dfree(c->dRKey);
     dfree(c->dROuter); dfree(c->dRTU); dfree(c->dRUT); dfree(c->dJC); dfree(c->dJT);
     dfree(c->dF1); dfree(c->dLM); dfree(c->dLN); dfree(c->dAccR); dfree(c->dEr); dfree(c->dLaRep);
-    dfree(c->dEpoly); dfree(c->dWork); dfree(c->dRed);
+    dfree(c->dEpoly); dfree(c->dWork); dfree(c->dRed); dfree(c->dGuard);
     dfree(c->dRedOut); dfree(c->dScanPart); c->scanPartCap = 0;
     dfree(c->dExtStage); c->extStageBytes = 0;
     dfree(c->dXbak); dfree(c->dXpre); dfree(c->dVbak); dfree(c->dDt); dfree(c->dAlpha);

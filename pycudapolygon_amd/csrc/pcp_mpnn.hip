@@ -631,6 +631,74 @@ __global__ void k_mpnn_apply(int V, const int* __restrict__ shapeId, const int* 
     x[m] = make_double2(px, py);
 }
 
+// Guard of the warm start.  An edge-crossing energy is blind to a polygon that lies inside
+// another (no edge crosses), and while one vertex still pokes out the whole polygon's area
+// counts, so FIRE pulls a polygon that is mostly inside another all the way in and cannot
+// bring it out.  Where the prediction puts one polygon's bounding circle inside another's,
+// the two polygons take back their positions; where that leaves such a pair in the packing
+// (with a neighbour that moved), the whole packing takes back its positions.
+// Thread per polygon A: the polygons of larger id in the 3x3 block of polygon cells around
+// A (k_poly_cells of the current positions; side >= the largest circle diameter, so
+// circles that meet lie in neighbouring cells); a pair is contained when
+// |c_A - c_B| + min(r_A, r_B) < kappa * max(r_A, r_B) (minimum image).  PER_POLY: both
+// polygons of every contained pair get polyFlag = 1 and their packing flag = 1; else the
+// packing gets flag = 2.  Polygons that wind around the torus are not judged.
+template <bool PER_POLY>
+__global__ void k_contained(int N, const int* __restrict__ polyPack, const int* __restrict__ packPoly,
+                            const double4* __restrict__ bounds, const int* __restrict__ pcStart,
+                            const int* __restrict__ pcSorted, const double4* __restrict__ sBounds,
+                            const int* __restrict__ packB, double kappa, int* __restrict__ flag,
+                            int* __restrict__ polyFlag) {
+    int A = blockIdx.x * blockDim.x + threadIdx.x;
+    if (A >= N) return;
+    int pk = polyPack[A];
+    int s0 = packPoly[pk];
+    int Bp = packB[pk] & 0xFFFF;
+    double4 bA = bounds[A];
+    if (bA.w != 0.0) return;
+    long long gbase = (long long)pk * (PC_CELLS + 1);
+    int ax = pc_coord(bA.x, Bp), ay = pc_coord(bA.y, Bp);
+    int ncell = (Bp == 1) ? 1 : 9;
+    for (int k = 0; k < ncell; ++k) {
+        int cx = ax, cy = ay;
+        if (Bp > 1) {
+            cx = ax + k / 3 - 1; cx += (cx < 0) ? Bp : 0; cx -= (cx >= Bp) ? Bp : 0;
+            cy = ay + k % 3 - 1; cy += (cy < 0) ? Bp : 0; cy -= (cy >= Bp) ? Bp : 0;
+        }
+        int cell = cy * Bp + cx;
+        int q1 = pcStart[gbase + cell + 1];
+        for (int q = pcStart[gbase + cell]; q < q1; ++q) {
+            int Bq = pcSorted[s0 + q];
+            if (Bq <= A) continue;
+            double4 b = sBounds[s0 + q];
+            if (b.w != 0.0) continue;
+            double lim = kappa * fmax(bA.z, b.z) - fmin(bA.z, b.z);
+            if (!(lim > 0.0)) continue;
+            double dx = wrap(b.x - bA.x), dy = wrap(b.y - bA.y);
+            if (dx * dx + dy * dy < lim * lim) {
+                if (!PER_POLY) {
+                    flag[pk] = 2;
+                    return;
+                }
+                polyFlag[A] = 1;
+                polyFlag[Bq] = 1;
+                flag[pk] = 1;
+            }
+        }
+    }
+}
+
+// x = xPrev for the vertices of flagged polygons (PER_POLY) or packings (flag == 2)
+template <bool PER_POLY>
+__global__ void k_take_back(int V, const int* __restrict__ shapeId, const int* __restrict__ polyPack,
+                            const int* __restrict__ flag, const int* __restrict__ polyFlag,
+                            const double2* __restrict__ xPrev, double2* __restrict__ x) {
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= V) return;
+    int s = shapeId[m];
+    if (PER_POLY ? polyFlag[s] != 0 : flag[polyPack[s]] == 2) x[m] = xPrev[m];
+}
+
 // work buffer slices (floats): H0..H3, Z0..Z3 (V x 64 each), X1..X3 (V x 128 bf16 each)
 struct MpnnWork {
     float* H[4];
@@ -829,6 +897,47 @@ int pcp_mpnn_apply(pcp_ctx* c, double* pos, const float* out, const double* ell,
     if (c->V == 0) return PCP_OK;
     PCP_LAUNCH(c, k_mpnn_apply, dim3(nblk(c->V)), dim3(BLK), c->V, c->dShapeId, c->dPolyPack, active,
                (const float2*)out, ell, outScale, (double2*)pos);
+    return PCP_OK;
+}
+
+int pcp_mpnn_guard(pcp_ctx* c, double* pos, const double* xPrev, double kappa, int* flags, int* nTaken) {
+    using namespace pcp;
+    if (!(kappa >= 0.0 && kappa <= 2.0)) return fail_arg("pcp_mpnn_guard: kappa must lie in [0, 2]");
+    if (!flags) return fail_arg("pcp_mpnn_guard: flags is NULL");
+    if (c->V > 0) {
+        if (!c->dGuard) PCP_TRY(dalloc(&c->dGuard, std::max(c->N, 1)));
+        PCP_LAUNCH(c, k_fill_int, dim3(nblk(c->P)), dim3(BLK), flags, (long long)c->P, 0);
+        PCP_LAUNCH(c, k_fill_int, dim3(nblk(c->N)), dim3(BLK), c->dGuard, (long long)c->N, 0);
+        // round 1 the contained pairs' polygons, round 2 the packings still holding one; each
+        // on the bounding circles and polygon cells of the current positions (the first two
+        // kernels of a detection; the records they reset are stale after a move anyway)
+        for (int round = 0; round < 2; ++round) {
+            PCP_TRY(launch_bounds(c, (const double2*)pos, nullptr));
+            PCP_LAUNCH(c, k_poly_cells, dim3(c->P), dim3(PP_BLK), c->dPackPoly, c->dBounds, c->dPcStart,
+                       c->dPcSorted, c->dPackB, c->dSBounds);
+            if (round == 0) {
+                PCP_LAUNCH(c, k_contained<true>, dim3(nblk(c->N)), dim3(BLK), c->N, c->dPolyPack, c->dPackPoly,
+                           c->dBounds, c->dPcStart, c->dPcSorted, c->dSBounds, c->dPackB, kappa, flags, c->dGuard);
+                PCP_LAUNCH(c, k_take_back<true>, dim3(nblk(c->V)), dim3(BLK), c->V, c->dShapeId, c->dPolyPack,
+                           flags, c->dGuard, (const double2*)xPrev, (double2*)pos);
+            } else {
+                PCP_LAUNCH(c, k_contained<false>, dim3(nblk(c->N)), dim3(BLK), c->N, c->dPolyPack, c->dPackPoly,
+                           c->dBounds, c->dPcStart, c->dPcSorted, c->dSBounds, c->dPackB, kappa, flags, c->dGuard);
+                PCP_LAUNCH(c, k_take_back<false>, dim3(nblk(c->V)), dim3(BLK), c->V, c->dShapeId, c->dPolyPack,
+                           flags, c->dGuard, (const double2*)xPrev, (double2*)pos);
+            }
+        }
+    }
+    if (nTaken) {
+        std::vector<int> h(std::max(c->P, 1), 0);
+        if (c->V > 0) {
+            PCP_HIP(hipMemcpyAsync(h.data(), flags, sizeof(int) * c->P, hipMemcpyDeviceToHost, c->stream));
+            PCP_HIP(hipStreamSynchronize(c->stream));
+        }
+        int n = 0;
+        for (int p = 0; p < c->P; ++p) n += h[p] ? 1 : 0;
+        *nTaken = n;
+    }
     return PCP_OK;
 }
 

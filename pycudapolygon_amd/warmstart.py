@@ -35,6 +35,11 @@ COMPRESS_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "wei
                                 "warmstart_compress_n4096_n24.npz")
 # FIRE time-step cap for the squared lens energy (see train_warmstart)
 DT_MAX = 100.0
+# pcp_mpnn_guard: a packing keeps its input positions when the warm start puts one polygon's
+# bounding circle inside another's, |c_A - c_B| + r_small < GUARD_KAPPA * r_large (the
+# predicted displacements routinely bring centres to 0.2-0.4 of the radius sum, which FIRE
+# resolves; a polygon whose circle lies inside its neighbour's is pulled all the way in)
+GUARD_KAPPA = 1.0
 
 
 def _np_ptr(a):
@@ -160,15 +165,24 @@ class WarmStart:
 
     def apply(self, system, ell, active=None):
         """Apply once to every packing of `system` whose forces were just evaluated.
-        Returns the seconds spent (device work included)."""
+        A packing in which the prediction puts one polygon's bounding circle inside
+        another's keeps its input positions (pcp_mpnn_guard, GUARD_KAPPA: no edge-crossing
+        energy sees a polygon inside another); their number is left in self.taken_back.  Returns the seconds spent (device work
+        included)."""
         t0 = time.perf_counter()
         gf = GraphFeatures(system, ell, self.inScale, self.energyPow)
         gf.compute()
         net = Network(system.ctx, self.params, system.V)
         out = net.forward(system.V, gf.feat.ptr, gf.adj.ptr)
+        prev = capi.DeviceArray((2 * system.V,), np.float64)
+        capi.check(capi.lib().pcp_memcpy_d2d(prev.ptr, system.pos.ptr, 16 * system.V))
         system.ctx.call("pcp_mpnn_apply", system.pos.ptr, out.ptr, gf.ell.ptr, self.outScale,
                         active.ptr if active is not None else None)
         system.shake(self.shakeIter, 1e-15)
+        flags = capi.DeviceArray((max(system.P, 1),), np.int32)
+        n = ctypes.c_int(0)
+        system.ctx.call("pcp_mpnn_guard", system.pos.ptr, prev.ptr, GUARD_KAPPA, flags.ptr, ctypes.byref(n))
+        self.taken_back = n.value
         system.sync()
         return time.perf_counter() - t0
 

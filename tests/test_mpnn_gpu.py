@@ -263,3 +263,96 @@ def test_trained_warm_start_gain_over_plain_fire():
     print(f"[mpnn] mean FIRE steps: plain {plain.mean():.1f}, warm start {warm.mean():.1f} "
           f"({1 - warm.mean() / plain.mean():+.1%} fewer)", flush=True)
     assert warm.mean() < 0.95 * plain.mean()
+
+
+def _circles(x, start):
+    """bounding circles as k_bounds builds them: chain of wrapped edge vectors from vertex 0,
+    centre = p0 + mean of the chain, radius = largest distance of a chain point to it"""
+    w = lambda d: (d + 0.5) % 1.0 - 0.5
+    X = np.asarray(x).reshape(-1, 2)
+    C, R = [], []
+    for s in range(len(start) - 1):
+        v = X[start[s]:start[s + 1]]
+        u = np.vstack([np.zeros(2), np.cumsum(w(np.diff(v, axis=0)), axis=0)])
+        m = u.mean(axis=0)
+        C.append(v[0] + m)
+        R.append(np.sqrt(((u - m) ** 2).sum(axis=1)).max())
+    return np.array(C), np.array(R)
+
+
+def _contained_pairs(x, start, kappa):
+    """polygon pairs (a < b) whose smaller bounding circle lies inside the larger one:
+    |c_a - c_b| + r_small < kappa r_large, minimum image (brute force over all pairs)"""
+    C, R = _circles(x, start)
+    d = C[:, None, :] - C[None, :, :]
+    d = (d + 0.5) % 1.0 - 0.5
+    D = np.sqrt((d ** 2).sum(axis=2))
+    lim = kappa * np.maximum(R[:, None], R[None, :]) - np.minimum(R[:, None], R[None, :])
+    a, b = np.nonzero(np.triu(D < lim, 1))
+    return list(zip(a.tolist(), b.tolist())), (D + np.minimum(R[:, None], R[None, :])) / np.maximum(R[:, None], R[None, :])
+
+
+def _guard_cpu(x, xprev, start, kappa):
+    """the guard's rule on the CPU: the polygons of contained pairs take back xprev; if the
+    packing still holds a contained pair, all of it does.  Returns (flag, positions)."""
+    x = x.copy().reshape(-1, 2)
+    pairs, _ = _contained_pairs(x, start, kappa)
+    if not pairs:
+        return 0, x.reshape(-1)
+    for s in {s for pr in pairs for s in pr}:
+        x[start[s]:start[s + 1]] = xprev.reshape(-1, 2)[start[s]:start[s + 1]]
+    if _contained_pairs(x, start, kappa)[0]:
+        return 2, xprev.copy()
+    return 1, x.reshape(-1)
+
+
+def test_warm_start_guard_takes_back_contained_polygons():
+    """pcp_mpnn_guard against its rule restated on the CPU (brute force over polygon pairs):
+    where one polygon's bounding circle lies inside another's (|c_A - c_B| + r_small <
+    kappa r_large) both take back the earlier positions; a packing that still holds such a
+    pair then takes back all of them"""
+    pk = [make_packing(64, 12, 0.80, seed=s, min_center_frac=0.5) for s in range(5)]
+    x_in = [p["positions"].copy() for p in pk]
+    moved = [q.copy() for q in x_in]
+
+    def carry(p, a, target):
+        st = pk[p]["startIndices"]
+        C, _ = _circles(moved[p], st)
+        v = moved[p].reshape(-1, 2)
+        v[st[a]:st[a + 1]] = (v[st[a]:st[a + 1]] + (target - C[a])) % 1.0
+
+    # packings 1-3: the smallest polygon carried towards the largest, its centre at 0.0, 0.6
+    # and 1.6 x (r_large - r_small) from the large one's
+    for p, f in ((1, 0.0), (2, 0.6), (3, 1.6)):
+        C, R = _circles(moved[p], pk[p]["startIndices"])
+        a, b = int(np.argmin(R)), int(np.argmax(R))
+        d = (C[a] - C[b] + 0.5) % 1.0 - 0.5
+        carry(p, a, C[b] + f * (R[b] - R[a]) * d / np.linalg.norm(d))
+    # packing 4: the smallest polygon onto the largest one's centre, and another large polygon
+    # onto the smallest one's old centre (taking back the first pair makes the second)
+    C, R = _circles(moved[4], pk[4]["startIndices"])
+    order = np.argsort(R)
+    a, b, c = int(order[0]), int(order[-1]), int(order[-2])
+    carry(4, c, C[a].copy())
+    carry(4, a, C[b].copy())
+    expect = [_guard_cpu(moved[p], x_in[p], pk[p]["startIndices"], W.GUARD_KAPPA) for p in range(5)]
+    assert [e[0] for e in expect] == [0, 1, 1, 0, 2]
+    for p in range(5):   # every pair clear of the limit, so that rounding cannot decide
+        q = _contained_pairs(moved[p], pk[p]["startIndices"], W.GUARD_KAPPA)[1]
+        assert np.abs(q[np.triu_indices(len(q), 1)] - W.GUARD_KAPPA).min() > 0.05
+    sysq = [dict(p, positions=m) for p, m in zip(pk, moved)]
+    s = capi.System(sysq, [box_size_for(p["diameter"]) for p in pk], model=W.SQUARED)
+    prev = capi.DeviceArray.from_numpy(np.concatenate(x_in))
+    flags = capi.DeviceArray((5,), np.int32)
+    n = ctypes.c_int(-1)
+    print("[mpnn] guard, 5 packings of N=64 x 12", flush=True)
+    s.ctx.call("pcp_mpnn_guard", s.pos.ptr, prev.ptr, W.GUARD_KAPPA, flags.ptr, ctypes.byref(n))
+    np.testing.assert_array_equal(flags.get(), [e[0] for e in expect])
+    assert n.value == 3
+    for p in range(5):
+        np.testing.assert_array_equal(s.get_positions(p), expect[p][1])
+    # a second call on the result keeps every packing
+    s.ctx.call("pcp_mpnn_guard", s.pos.ptr, prev.ptr, W.GUARD_KAPPA, flags.ptr, ctypes.byref(n))
+    assert n.value == 0 and not flags.get().any()
+    with pytest.raises(RuntimeError):
+        s.ctx.call("pcp_mpnn_guard", s.pos.ptr, prev.ptr, 2.5, flags.ptr, None)
