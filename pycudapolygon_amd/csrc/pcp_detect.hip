@@ -182,15 +182,15 @@ __device__ __forceinline__ bool circles_meet(double4 a, double4 b) {
 // centre in a B x B grid with side >= (1 + 1e-5) x the largest circle diameter, so that
 // circles that meet have centres in neighbouring cells.  B = 1 (one cell) when B would be
 // below 3 or a polygon is big.  Writes pcStart[p*(PC_CELLS+1) + 0..B*B], the sorted
-// polygon ids into pcSorted[packPoly[p]..] (their circles into sBounds[..]), and
-// packB[p] = B | big << 30.
+// polygon ids into pcSorted[packPoly[p]..] and packB[p] = B | big << 30 (the detection
+// reads a candidate's circle from bounds[id], in flight together with its vertex range).
 // The bounds of up to PC_REG * PP_BLK polygons are read once, all loads in flight
 // together, and kept in registers for the three passes; beyond that they are re-read.
 constexpr int PC_REG = 8;
 __global__ __launch_bounds__(PP_BLK) void k_poly_cells(const int* __restrict__ packPoly,
                                                       const double4* __restrict__ bounds,
                                                       int* __restrict__ pcStart, int* __restrict__ pcSorted,
-                                                      int* __restrict__ packB, double4* __restrict__ sBounds) {
+                                                      int* __restrict__ packB) {
     __shared__ int cStart[PC_CELLS + 1];
     __shared__ int cCur[PC_CELLS];
     __shared__ double redR[PP_BLK / 64];
@@ -274,20 +274,29 @@ __global__ __launch_bounds__(PP_BLK) void k_poly_cells(const int* __restrict__ p
         pcStart[gbase + C] = Np;
         packB[p] = B | (big ? (1 << 30) : 0);
     }
+    // the ids in cell order: one workgroup's scattered stores are limited to about one
+    // cache line per clock, so up to PC_CELLS + 1 polygons the ids are placed in LDS (over
+    // cStart, which has been read out above) and written out in order
+    bool inLds = Np <= PC_CELLS + 1;
+    if (inLds) __syncthreads();
 #pragma unroll
     for (int k = 0; k < PC_REG; ++k) {
         int s = s0 + t + k * PP_BLK;
         if (s < s1) {
             int slot = atomicAdd(&cCur[cellOf[k]], 1);
-            pcSorted[s0 + slot] = s;
-            sBounds[s0 + slot] = bc[k];
+            if (inLds) cStart[slot] = s;
+            else pcSorted[s0 + slot] = s;
         }
     }
     for (int s = s0 + t + PC_REG * PP_BLK; s < s1; s += PP_BLK) {
         double4 b = bounds[s];
         int slot = atomicAdd(&cCur[pc_coord(b.y, B) * B + pc_coord(b.x, B)], 1);
-        pcSorted[s0 + slot] = s;
-        sBounds[s0 + slot] = b;
+        if (inLds) cStart[slot] = s;
+        else pcSorted[s0 + slot] = s;
+    }
+    if (inLds) {
+        __syncthreads();
+        for (int i = t; i < Np; i += PP_BLK) pcSorted[s0 + i] = cStart[i];
     }
 }
 
@@ -625,7 +634,7 @@ __device__ void detect_polygon(
     int A, int lane, DetLds& L, const int* __restrict__ polyStart, const int* __restrict__ polyPack,
     const int* __restrict__ packPoly, const int* __restrict__ box, const double2* __restrict__ pos,
     const int* __restrict__ cellOf, const double4* __restrict__ bounds, const int* __restrict__ pcStart,
-    const int* __restrict__ pcSorted, const double4* __restrict__ sBounds, const int* __restrict__ packB,
+    const int* __restrict__ pcSorted, const int* __restrict__ packB,
     const DetOut& out, unsigned long long* __restrict__ packMaxKey, long long pmkPitch, int* __restrict__ num) {
     int stA = polyStart[A], nA = polyStart[A + 1] - stA;
     int pk = polyPack[A];
@@ -636,6 +645,10 @@ __device__ void detect_polygon(
     bool big = (pb >> 30) & 1;
     int sAl = A - s0;
     double4 bA = bounds[A];
+    // the packing's largest key so far (read here, with the other loads of the polygon: it only
+    // lets the final atomicMax be skipped, and a stale value can only be lower)
+    unsigned long long* pmkSlot = packMaxKey + (A & (PMK_SLOTS - 1)) * pmkPitch + pk;
+    unsigned long long pmk = *pmkSlot;
     bool cullA = cullable(bA);
     uint64_t kmax = 0;
     // candidate cells: lanes 0..8 hold the ranges of the 3x3 block (one cell when Bp == 1);
@@ -682,14 +695,15 @@ __device__ void detect_polygon(
         }
         int Bq = -1, stq = 0, nq = 0;
         double4 bc = make_double4(0.0, 0.0, 0.0, 1.0);
-        if (ci < T) {
+        if (ci < T) {   // the candidate's circle and vertex range, loaded together
             int cand = pcSorted[s0 + q];
-            double4 c4 = sBounds[s0 + q];
+            double4 c4 = bounds[cand];
+            int st4 = polyStart[cand], en4 = polyStart[cand + 1];
             if (cand > A && (big || circles_meet(bA, c4))) {
                 Bq = cand;
                 bc = c4;
-                stq = polyStart[cand];
-                nq = polyStart[cand + 1] - stq;
+                stq = st4;
+                nq = en4 - st4;
             }
         }
         bool packed = Bq >= 0 && nA <= 64 && nq <= 64;
@@ -906,8 +920,6 @@ __device__ void detect_polygon(
         }
     }
     // the largest key of the packing (the pairing's last-record case)
-    unsigned long long* pmkSlot = packMaxKey + (A & (PMK_SLOTS - 1)) * pmkPitch + pk;
-    unsigned long long pmk = *pmkSlot;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
         unsigned long long y = __shfl_xor((unsigned long long)kmax, o);
@@ -1034,7 +1046,7 @@ __global__ __launch_bounds__(BLK) void k_detect_tile(
     int N, const int* __restrict__ polyStart, const int* __restrict__ polyPack,
     const int* __restrict__ packPoly, const int* __restrict__ box, const double2* __restrict__ pos,
     const int* __restrict__ cellOf, const double4* __restrict__ bounds, const int* __restrict__ pcStart,
-    const int* __restrict__ pcSorted, const double4* __restrict__ sBounds, const int* __restrict__ packB, int Kp,
+    const int* __restrict__ pcSorted, const int* __restrict__ packB, int Kp,
     int* __restrict__ bcnt, uint64_t* __restrict__ bKey, float2* __restrict__ bTU, int* __restrict__ jcnt,
     uint64_t* __restrict__ jKey, unsigned long long* __restrict__ packMaxKey, long long pmkPitch,
     int* __restrict__ num, int* __restrict__ scal) {
@@ -1054,6 +1066,10 @@ __global__ __launch_bounds__(BLK) void k_detect_tile(
     bool big = (pb >> 30) & 1;
     int sAl = A - s0;
     double4 bA = bounds[A];
+    // the packing's largest key so far (read here, with the other loads of the polygon: it only
+    // lets the final atomicMax be skipped, and a stale value can only be lower)
+    unsigned long long* pmkSlot = packMaxKey + (A & (PMK_SLOTS - 1)) * pmkPitch + pk;
+    unsigned long long pmk = *pmkSlot;
     uint64_t kmax = 0;
     // candidate cells: lanes 0..8 hold the ranges of the 3x3 block (one cell when Bp == 1)
     long long gbase = (long long)pk * (PC_CELLS + 1);
@@ -1089,14 +1105,16 @@ __global__ __launch_bounds__(BLK) void k_detect_tile(
         }
         bool keep = false;
         int cand = -1;
-        if (ci < T) {
-            cand = pcSorted[s0 + q];
-            keep = cand > A && (big || circles_meet(bA, sBounds[s0 + q]));
-        }
         int stq = 0, nq = 0;
-        if (keep) {
-            stq = polyStart[cand];
-            nq = polyStart[cand + 1] - stq;
+        if (ci < T) {   // the candidate's circle and vertex range, loaded together
+            cand = pcSorted[s0 + q];
+            double4 c4 = bounds[cand];
+            int st4 = polyStart[cand], en4 = polyStart[cand + 1];
+            keep = cand > A && (big || circles_meet(bA, c4));
+            if (keep) {
+                stq = st4;
+                nq = en4 - st4;
+            }
         }
         unsigned long long km = __ballot(keep);
         int first = (c0 == 0 && selfPair) ? 1 : 0;
@@ -1117,8 +1135,6 @@ __global__ __launch_bounds__(BLK) void k_detect_tile(
         tile_pairs<CELLS, SEG, CONS>(L, lane, nU, A, stA, nA, sAl, s0, Bu, pos, cellOf, out, num, kmax);
     }
     // the largest key of the packing (the pairing's last-record case)
-    unsigned long long* pmkSlot = packMaxKey + (A & (PMK_SLOTS - 1)) * pmkPitch + pk;
-    unsigned long long pmk = *pmkSlot;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
         unsigned long long y = __shfl_xor((unsigned long long)kmax, o);
@@ -1138,7 +1154,7 @@ __global__ __launch_bounds__(BLK) void k_detect_poly(
     int N, const int* __restrict__ polyStart, const int* __restrict__ polyPack,
     const int* __restrict__ packPoly, const int* __restrict__ box, const double2* __restrict__ pos,
     const int* __restrict__ cellOf, const double4* __restrict__ bounds, const int* __restrict__ pcStart,
-    const int* __restrict__ pcSorted, const double4* __restrict__ sBounds, const int* __restrict__ packB, int Kp,
+    const int* __restrict__ pcSorted, const int* __restrict__ packB, int Kp,
     int* __restrict__ bcnt, uint64_t* __restrict__ bKey, float2* __restrict__ bTU, int* __restrict__ jcnt,
     uint64_t* __restrict__ jKey, unsigned long long* __restrict__ packMaxKey, long long pmkPitch,
     int* __restrict__ num,
@@ -1149,7 +1165,7 @@ __global__ __launch_bounds__(BLK) void k_detect_poly(
     if (idx >= N) return;   // whole wavefront
     DetOut out{Kp, bcnt, bKey, bTU, jcnt, jKey, scal};
     detect_polygon<CELLS, CONS>(__builtin_amdgcn_readfirstlane(pcSorted[idx]), lane, lds[w], polyStart, polyPack, packPoly,
-                          box, pos, cellOf, bounds, pcStart, pcSorted, sBounds, packB, out, packMaxKey, pmkPitch,
+                          box, pos, cellOf, bounds, pcStart, pcSorted, packB, out, packMaxKey, pmkPitch,
                           num);
 }
 
@@ -1208,13 +1224,13 @@ static int launch_tile_t(pcp_ctx* c, const double2* pos, int grid) {
     if (slot >= 0)
         hipExtLaunchKernelGGL(kern, dim3(grid), dim3(BLK), 0, c->stream, c->profEvents[slot].first,
                               c->profEvents[slot].second, 0, c->N, c->dPolyStart, c->dPolyPack, c->dPackPoly,
-                              c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted, c->dSBounds, c->dPackB,
+                              c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted, c->dPackB,
                               c->Kp, c->dBcnt, c->dBKey, c->dBTU, c->dJcnt, c->dJKey, c->dPackMaxKey, c->pmkPitch,
                               c->dNum, c->dScal);
     else
         hipLaunchKernelGGL(kern, dim3(grid), dim3(BLK), 0, c->stream, c->N, c->dPolyStart, c->dPolyPack,
                            c->dPackPoly, c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted,
-                           c->dSBounds, c->dPackB, c->Kp, c->dBcnt, c->dBKey, c->dBTU, c->dJcnt, c->dJKey,
+                           c->dPackB, c->Kp, c->dBcnt, c->dBKey, c->dBTU, c->dJcnt, c->dJKey,
                            c->dPackMaxKey, c->pmkPitch, c->dNum, c->dScal);
     PCP_LAUNCHED();
     return PCP_OK;
@@ -1252,30 +1268,30 @@ int run_detect(pcp_ctx* c, const double2* pos, int cellsFromList, const PostShak
     c->lastDetectPos = pos;
     PCP_TRY(launch_bounds(c, pos, ps));
     PCP_LAUNCH(c, k_poly_cells, dim3(c->P), dim3(PP_BLK), c->dPackPoly, c->dBounds, c->dPcStart, c->dPcSorted,
-               c->dPackB, c->dSBounds);
+               c->dPackB);
     int grid = (c->N + BLK / 64 - 1) / (BLK / 64);
     if (c->maxN <= 64 && detect_schedule(c) == 1) return launch_detect_tile(c, pos, cellsFromList != 0, cons, grid);
     if (cons) {
         if (cellsFromList)
             PCP_LAUNCH(c, (k_detect_poly<1, true>), dim3(grid), dim3(BLK), c->N, c->dPolyStart, c->dPolyPack,
-                       c->dPackPoly, c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted, c->dSBounds,
+                       c->dPackPoly, c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted,
                        c->dPackB, c->Kp, c->dBcnt, c->dBKey, c->dBTU, c->dJcnt, c->dJKey, c->dPackMaxKey,
                        c->pmkPitch, c->dNum, c->dScal);
         else
             PCP_LAUNCH(c, (k_detect_poly<0, true>), dim3(grid), dim3(BLK), c->N, c->dPolyStart, c->dPolyPack,
-                       c->dPackPoly, c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted, c->dSBounds,
+                       c->dPackPoly, c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted,
                        c->dPackB, c->Kp, c->dBcnt, c->dBKey, c->dBTU, c->dJcnt, c->dJKey, c->dPackMaxKey,
                        c->pmkPitch, c->dNum, c->dScal);
         return PCP_OK;
     }
     if (cellsFromList)
         PCP_LAUNCH(c, k_detect_poly<1>, dim3(grid), dim3(BLK), c->N, c->dPolyStart, c->dPolyPack, c->dPackPoly,
-                   c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted, c->dSBounds, c->dPackB, c->Kp,
+                   c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted, c->dPackB, c->Kp,
                    c->dBcnt,
                    c->dBKey, c->dBTU, c->dJcnt, c->dJKey, c->dPackMaxKey, c->pmkPitch, c->dNum, c->dScal);
     else
         PCP_LAUNCH(c, k_detect_poly<0>, dim3(grid), dim3(BLK), c->N, c->dPolyStart, c->dPolyPack, c->dPackPoly,
-                   c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted, c->dSBounds, c->dPackB, c->Kp,
+                   c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted, c->dPackB, c->Kp,
                    c->dBcnt,
                    c->dBKey, c->dBTU, c->dJcnt, c->dJKey, c->dPackMaxKey, c->pmkPitch, c->dNum, c->dScal);
     return PCP_OK;
@@ -1287,7 +1303,6 @@ int detect_alloc(pcp_ctx* c) {
     c->row16 = c->N > 0;
     for (int s = 0; s < c->N; ++s) c->row16 = c->row16 && c->hPolyStart[s + 1] - c->hPolyStart[s] == 16;
     PCP_TRY(dalloc(&c->dBounds, std::max(c->N, 1)));
-    PCP_TRY(dalloc(&c->dSBounds, std::max(c->N, 1)));
     PCP_TRY(dalloc(&c->dPcSorted, std::max(c->N, 1)));
     {   // identity until the first detection sorts the polygons by cell (kernels that visit
         // polygons in cell order also run for models without overlap)

@@ -95,7 +95,6 @@ struct pcp_ctx {
     bool row16 = false;            // every polygon has exactly 16 vertices (DPP row solves)
     double4* dBounds = nullptr;    // N: centre x, y, radius, big flag
     int* dPcSorted = nullptr;      // N: polygons sorted by cell, per packing
-    double4* dSBounds = nullptr;   // N: their circles in the same order
     int* dPcStart = nullptr;       // P x (PC_CELLS + 1): cell starts of each packing
     int* dPackB = nullptr;         // P: polygon grid size | big << 30
     const double2* lastDetectPos = nullptr;   // positions of the last detection (statistics)

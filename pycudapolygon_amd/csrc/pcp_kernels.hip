@@ -1030,7 +1030,7 @@ static void free_topology(pcp_ctx* c) {
     dfree(c->dEpre); dfree(c->dNPos); dfree(c->dSteps); dfree(c->dActive); dfree(c->dFlags);
     dfree(c->dFireScal); dfree(c->dShakeIt); dfree(c->dCellInv); dfree(c->dCom); dfree(c->dStatusBlock);
     c->dStatus = nullptr; c->dSummary = nullptr;
-    dfree(c->dBounds); dfree(c->dSBounds); dfree(c->dPcSorted); dfree(c->dPcStart); dfree(c->dPackB); dfree(c->dSticky);
+    dfree(c->dBounds); dfree(c->dPcSorted); dfree(c->dPcStart); dfree(c->dPackB); dfree(c->dSticky);
     c->haveBoxes = false;
 }
 

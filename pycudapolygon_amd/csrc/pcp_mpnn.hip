@@ -646,8 +646,7 @@ __global__ void k_mpnn_apply(int V, const int* __restrict__ shapeId, const int* 
 template <bool PER_POLY>
 __global__ void k_contained(int N, const int* __restrict__ polyPack, const int* __restrict__ packPoly,
                             const double4* __restrict__ bounds, const int* __restrict__ pcStart,
-                            const int* __restrict__ pcSorted, const double4* __restrict__ sBounds,
-                            const int* __restrict__ packB, double kappa, int* __restrict__ flag,
+                            const int* __restrict__ pcSorted, const int* __restrict__ packB, double kappa, int* __restrict__ flag,
                             int* __restrict__ polyFlag) {
     int A = blockIdx.x * blockDim.x + threadIdx.x;
     if (A >= N) return;
@@ -670,7 +669,7 @@ __global__ void k_contained(int N, const int* __restrict__ polyPack, const int* 
         for (int q = pcStart[gbase + cell]; q < q1; ++q) {
             int Bq = pcSorted[s0 + q];
             if (Bq <= A) continue;
-            double4 b = sBounds[s0 + q];
+            double4 b = bounds[Bq];
             if (b.w != 0.0) continue;
             double lim = kappa * fmax(bA.z, b.z) - fmin(bA.z, b.z);
             if (!(lim > 0.0)) continue;
@@ -914,15 +913,15 @@ int pcp_mpnn_guard(pcp_ctx* c, double* pos, const double* xPrev, double kappa, i
         for (int round = 0; round < 2; ++round) {
             PCP_TRY(launch_bounds(c, (const double2*)pos, nullptr));
             PCP_LAUNCH(c, k_poly_cells, dim3(c->P), dim3(PP_BLK), c->dPackPoly, c->dBounds, c->dPcStart,
-                       c->dPcSorted, c->dPackB, c->dSBounds);
+                       c->dPcSorted, c->dPackB);
             if (round == 0) {
                 PCP_LAUNCH(c, k_contained<true>, dim3(nblk(c->N)), dim3(BLK), c->N, c->dPolyPack, c->dPackPoly,
-                           c->dBounds, c->dPcStart, c->dPcSorted, c->dSBounds, c->dPackB, kappa, flags, c->dGuard);
+                           c->dBounds, c->dPcStart, c->dPcSorted, c->dPackB, kappa, flags, c->dGuard);
                 PCP_LAUNCH(c, k_take_back<true>, dim3(nblk(c->V)), dim3(BLK), c->V, c->dShapeId, c->dPolyPack,
                            flags, c->dGuard, (const double2*)xPrev, (double2*)pos);
             } else {
                 PCP_LAUNCH(c, k_contained<false>, dim3(nblk(c->N)), dim3(BLK), c->N, c->dPolyPack, c->dPackPoly,
-                           c->dBounds, c->dPcStart, c->dPcSorted, c->dSBounds, c->dPackB, kappa, flags, c->dGuard);
+                           c->dBounds, c->dPcStart, c->dPcSorted, c->dPackB, kappa, flags, c->dGuard);
                 PCP_LAUNCH(c, k_take_back<false>, dim3(nblk(c->V)), dim3(BLK), c->V, c->dShapeId, c->dPolyPack,
                            flags, c->dGuard, (const double2*)xPrev, (double2*)pos);
             }
