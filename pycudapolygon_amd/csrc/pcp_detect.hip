@@ -936,27 +936,49 @@ __device__ void detect_polygon(
 // circle meets A's -- are laid out one per segment of SEG lanes, 64/SEG pairs per pass;
 // lane b of a segment holds edge b of the pair's other polygon B in registers.  Every edge
 // a of A is then read from LDS by all lanes at once (a broadcast) and tested against every
-// lane's edge: all nA x nB edge pairs (the nA(nA-3)/2 non-adjacent ones for A itself) with
-// the division-free prefilter of the reference's segment test; a pass where some lane
-// passes it takes the exact test, the user-grid stencil filter and the record, as in
-// edge_tile.  There is no edge-circle cull: it removes only edge pairs that cannot cross
-// (the records are the same), and the regular tile costs fewer instructions than listing
-// and enumerating the culled pairs.
-// edge pairs that passed the prefilter, awaiting the exact test: (a, owner lane) per entry
+// lane's edge: all nA x nB edge pairs (the nA(nA-3)/2 non-adjacent ones for A itself) are
+// screened by the FP32 bounding boxes of the two edges about A's circle centre (four
+// comparisons, where the division-free FP64 prefilter of the segment test takes about 28
+// FP64 operations); the pairs whose boxes meet are collected and take the segment test
+// (prefilter, then the exact one), the user-grid stencil filter and the record 64 at a
+// time.  There is no edge-circle cull: it removes only edge pairs that cannot cross (the
+// records are the same), and the regular tile costs fewer instructions than listing and
+// enumerating the culled pairs.
+// edge pairs whose boxes met, awaiting the segment test: (a, owner lane) per entry
 constexpr int TILE_MAY = 512;
 struct TileLds {
     double2 P[64], R[64];   // A's edges: start and wrapped edge vector
     int C[64];              // their user-grid cells (x | y << 16)
+    float4 box[64];         // A's edges: bounding boxes relative to A's circle centre (x0, x1, y0, y1)
     int uB[65], uSt[65], uN[65];   // the pairs of a pass group: other polygon, first vertex, vertices
+    int uBox[65];                  // whether the pair is judged by bounding boxes (both polygons cullable)
     uint16_t may[TILE_MAY];        // a | lane << 6
 };
 
+// Bounding box of an edge (start at rel, vector R; rel relative to an origin within a few
+// polygon radii) widened by TILE_BOX_EPS and rounded to float: an edge pair whose boxes do
+// not meet has no crossing that the segment test can find (its t, u lie in (0, 1) only for
+// segments that meet to within rounding, ~1e-16, far inside the widening).  Valid where the
+// minimum images of both polygons' vertices about the origin follow their chains, i.e. for
+// cullable polygons (radius < CULL_RADIUS, chain closed), as the edge-circle cull.
+constexpr double TILE_BOX_EPS = 1e-6;
+__device__ __forceinline__ float4 edge_box(double rx, double ry, d2 R) {
+    double x0 = fmin(rx, rx + R.x), x1 = fmax(rx, rx + R.x);
+    double y0 = fmin(ry, ry + R.y), y1 = fmax(ry, ry + R.y);
+    return make_float4((float)(x0 - TILE_BOX_EPS), (float)(x1 + TILE_BOX_EPS), (float)(y0 - TILE_BOX_EPS),
+                       (float)(y1 + TILE_BOX_EPS));
+}
+__device__ __forceinline__ bool boxes_meet(float4 a, float4 b) {
+    return a.x <= b.y && b.x <= a.y && a.z <= b.w && b.z <= a.w;
+}
+
 template <int CELLS, int SEG, bool CONS>
-__device__ void tile_pairs(TileLds& L, int lane, int nU, int A, int stA, int nA, int sAl, int s0, int Bu,
-                           const double2* __restrict__ pos, const int* __restrict__ cellOf,
+__device__ void tile_pairs(TileLds& L, int lane, int nU, int A, int stA, int nA, int sAl, int s0, int Bu, double ox,
+                           double oy, const double2* __restrict__ pos, const int* __restrict__ cellOf,
                            const DetOut& out, int* __restrict__ num, uint64_t& kmax) {
     constexpr int S = 64 / SEG;
     const int j = lane / SEG, b = lane & (SEG - 1);
+    const float4 anyBox = make_float4(-INFINITY, INFINITY, -INFINITY, INFINITY);
     for (int u0 = 0; u0 < nU; u0 += S) {
         int u = u0 + j;
         bool has = u < nU;
@@ -966,15 +988,17 @@ __device__ void tile_pairs(TileLds& L, int lane, int nU, int A, int stA, int nA,
         int bn = (b + 1 == nB) ? 0 : b + 1;
         d2 Q = mk(0.0, 0.0), Sv = mk(0.0, 0.0), Qn = mk(0.0, 0.0);
         int cb = 0;   // the user-grid cell of B's edge (for the stencil filter)
+        float4 bb = anyBox;   // B's edge box (every box meets it when the pair is not judged by boxes)
         if (act) {
             d2 q0 = ldpos(pos, stB + b), q1 = ldpos(pos, stB + bn);
             Q = q0;
             Qn = q1;
             Sv = mk(wrap(q1.x - q0.x), wrap(q1.y - q0.y));
             cb = CELLS ? cell_xy(cellOf[stB + b], Bu) : edge_cell_xy(q0, q1, Bu);
+            if (L.uBox[u]) bb = edge_box(wrap(q0.x - ox), wrap(q0.y - oy), Sv);
         }
         int nmay = 0;
-        // the exact test, the stencil filter and the record of the collected pairs, 64 at a
+        // the segment test, the stencil filter and the record of the collected pairs, 64 at a
         // time (one memory round trip for all their records): B's edge comes from the lane
         // that holds it
         auto flush = [&]() {
@@ -984,6 +1008,7 @@ __device__ void tile_pairs(TileLds& L, int lane, int nU, int A, int stA, int nA,
                 int e = valid ? (int)L.may[f0 + lane] : 0;
                 int a = e & 63, ow = e >> 6;
                 double qx = __shfl(Q.x, ow), qy = __shfl(Q.y, ow), sx = __shfl(Sv.x, ow), sy = __shfl(Sv.y, ow);
+                double qnx = CONS ? __shfl(Qn.x, ow) : 0.0, qny = CONS ? __shfl(Qn.y, ow) : 0.0;
                 int cbo = __shfl(cb, ow);
                 int bo = ow & (SEG - 1), uo = u0 + ow / SEG;
                 uint64_t key = 0;
@@ -992,10 +1017,14 @@ __device__ void tile_pairs(TileLds& L, int lane, int nU, int A, int stA, int nA,
                 if (valid && cells_adjacent(L.C[a], cbo, Bu)) {
                     double2 pa = L.P[a], ra = L.R[a];
                     SegNum sn = seg_numerators(mk(pa.x, pa.y), mk(ra.x, ra.y), mk(qx, qy), mk(sx, sy));
-                    if (CONS) {   // decided when the pair was collected
-                        cons_record(sn, sAl, a, L.uB[uo] - s0, bo, key, tu);
-                        rec = true;
-                    } else {
+                    if (CONS) {   // A is staged whole (every polygon of the context has <= 64 vertices)
+                        double2 pn = L.P[(a + 1 == nA) ? 0 : a + 1];
+                        if (seg_cons_cross(sn, mk(pa.x, pa.y), mk(ra.x, ra.y), mk(pn.x, pn.y), mk(qx, qy), mk(sx, sy),
+                                           mk(qnx, qny))) {
+                            cons_record(sn, sAl, a, L.uB[uo] - s0, bo, key, tu);
+                            rec = true;
+                        }
+                    } else if (seg_may_cross(sn)) {
                         rec = exact_record(sn, sAl, a, L.uB[uo] - s0, bo, key, tu);
                     }
                 }
@@ -1008,6 +1037,7 @@ __device__ void tile_pairs(TileLds& L, int lane, int nU, int A, int stA, int nA,
             wave_lds_sync();   // the list's reads are issued before it is refilled
             nmay = 0;
         };
+        // every edge a of A against every lane's edge: the pairs whose boxes meet are collected
         for (int a0 = 0; a0 < nA; a0 += TILE_UNROLL) {
             bool may[TILE_UNROLL];
             bool anyMay = false;
@@ -1016,15 +1046,8 @@ __device__ void tile_pairs(TileLds& L, int lane, int nU, int A, int stA, int nA,
                 int a = a0 + k;
                 bool ok = act && a < nA;
                 if (self) ok = ok && b > a && b != a + 1 && !(a == 0 && b == nA - 1);
-                int al = (a < nA) ? a : 0;
-                double2 pa = L.P[al], ra = L.R[al];
-                SegNum sn = seg_numerators(mk(pa.x, pa.y), mk(ra.x, ra.y), Q, Sv);
-                if (CONS) {   // A is staged whole (every polygon of the context has <= 64 vertices)
-                    double2 pn = L.P[(al + 1 == nA) ? 0 : al + 1];
-                    may[k] = ok && seg_cons_cross(sn, mk(pa.x, pa.y), mk(ra.x, ra.y), mk(pn.x, pn.y), Q, Sv, Qn);
-                } else {
-                    may[k] = ok && seg_may_cross(sn);
-                }
+                float4 ab = L.box[(a < nA) ? a : 0];
+                may[k] = ok && boxes_meet(ab, bb);
                 anyMay = anyMay || may[k];
             }
             if (__ballot(anyMay) == 0ull) continue;   // wavefront-uniform: most blocks find nothing
@@ -1087,6 +1110,14 @@ __global__ __launch_bounds__(BLK) void k_detect_tile(
         cnt = pcStart[gbase + cell + 1] - lo;
     }
     stage_a(L, lane, stA, nA, 0, Bu, pos, cellOf, CELLS != 0);
+    // A's edge boxes about A's circle centre (every box of a polygon that is not cullable
+    // meets every other: its pairs take the segment test)
+    bool cullA = cullable(bA);
+    if (lane < nA) {
+        double2 p = L.P[lane], r = L.R[lane];
+        L.box[lane] = cullA ? edge_box(wrap(p.x - bA.x), wrap(p.y - bA.y), mk(r.x, r.y))
+                            : make_float4(-INFINITY, INFINITY, -INFINITY, INFINITY);
+    }
     int incl = cnt;
 #pragma unroll
     for (int o = 1; o < 16; o <<= 1) {
@@ -1103,7 +1134,7 @@ __global__ __launch_bounds__(BLK) void k_detect_tile(
             int hiK = __shfl(incl, k), cK = __shfl(cnt, k), loK = __shfl(lo, k);
             if (!found && ci < hiK) { q = loK + ci - (hiK - cK); found = true; }
         }
-        bool keep = false;
+        bool keep = false, boxed = false;
         int cand = -1;
         int stq = 0, nq = 0;
         if (ci < T) {   // the candidate's circle and vertex range, loaded together
@@ -1111,6 +1142,7 @@ __global__ __launch_bounds__(BLK) void k_detect_tile(
             double4 c4 = bounds[cand];
             int st4 = polyStart[cand], en4 = polyStart[cand + 1];
             keep = cand > A && (big || circles_meet(bA, c4));
+            boxed = cullA && cullable(c4);
             if (keep) {
                 stq = st4;
                 nq = en4 - st4;
@@ -1124,15 +1156,17 @@ __global__ __launch_bounds__(BLK) void k_detect_tile(
             L.uB[0] = A;
             L.uSt[0] = stA;
             L.uN[0] = nA;
+            L.uBox[0] = cullA ? 1 : 0;
         }
         if (keep) {
             int u = first + __popcll(km & bits_below(lane));
             L.uB[u] = cand;
             L.uSt[u] = stq;
             L.uN[u] = nq;
+            L.uBox[u] = boxed ? 1 : 0;
         }
         wave_lds_sync();
-        tile_pairs<CELLS, SEG, CONS>(L, lane, nU, A, stA, nA, sAl, s0, Bu, pos, cellOf, out, num, kmax);
+        tile_pairs<CELLS, SEG, CONS>(L, lane, nU, A, stA, nA, sAl, s0, Bu, bA.x, bA.y, pos, cellOf, out, num, kmax);
     }
     // the largest key of the packing (the pairing's last-record case)
 #pragma unroll
