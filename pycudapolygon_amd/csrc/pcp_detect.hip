@@ -945,11 +945,11 @@ __device__ void detect_polygon(
 // records are the same), and the regular tile costs fewer instructions than listing and
 // enumerating the culled pairs.
 // edge pairs whose boxes met, awaiting the segment test: (a, owner lane) per entry
+// (4 wavefronts x (LDS stage + list) stay within 20 KiB: eight workgroups per CU)
 constexpr int TILE_MAY = 512;
 struct TileLds {
     double2 P[64], R[64];   // A's edges: start and wrapped edge vector
     int C[64];              // their user-grid cells (x | y << 16)
-    float4 box[64];         // A's edges: bounding boxes relative to A's circle centre (x0, x1, y0, y1)
     int uB[65], uSt[65], uN[65];   // the pairs of a pass group: other polygon, first vertex, vertices
     int uBox[65];                  // whether the pair is judged by bounding boxes (both polygons cullable)
     uint16_t may[TILE_MAY];        // a | lane << 6
@@ -971,10 +971,17 @@ __device__ __forceinline__ float4 edge_box(double rx, double ry, d2 R) {
 __device__ __forceinline__ bool boxes_meet(float4 a, float4 b) {
     return a.x <= b.y && b.x <= a.y && a.z <= b.w && b.z <= a.w;
 }
+// the box held by lane l (wavefront-uniform l): four scalar reads, no LDS round trip
+__device__ __forceinline__ float4 lane_box(float4 v, int l) {
+    return make_float4(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.x), l)),
+                       __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.y), l)),
+                       __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.z), l)),
+                       __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.w), l)));
+}
 
 template <int CELLS, int SEG, bool CONS>
 __device__ void tile_pairs(TileLds& L, int lane, int nU, int A, int stA, int nA, int sAl, int s0, int Bu, double ox,
-                           double oy, const double2* __restrict__ pos, const int* __restrict__ cellOf,
+                           double oy, float4 aBox, const double2* __restrict__ pos, const int* __restrict__ cellOf,
                            const DetOut& out, int* __restrict__ num, uint64_t& kmax) {
     constexpr int S = 64 / SEG;
     const int j = lane / SEG, b = lane & (SEG - 1);
@@ -1046,7 +1053,7 @@ __device__ void tile_pairs(TileLds& L, int lane, int nU, int A, int stA, int nA,
                 int a = a0 + k;
                 bool ok = act && a < nA;
                 if (self) ok = ok && b > a && b != a + 1 && !(a == 0 && b == nA - 1);
-                float4 ab = L.box[(a < nA) ? a : 0];
+                float4 ab = lane_box(aBox, (a < nA) ? a : 0);   // A's edge a, from lane a
                 may[k] = ok && boxes_meet(ab, bb);
                 anyMay = anyMay || may[k];
             }
@@ -1110,13 +1117,13 @@ __global__ __launch_bounds__(BLK) void k_detect_tile(
         cnt = pcStart[gbase + cell + 1] - lo;
     }
     stage_a(L, lane, stA, nA, 0, Bu, pos, cellOf, CELLS != 0);
-    // A's edge boxes about A's circle centre (every box of a polygon that is not cullable
-    // meets every other: its pairs take the segment test)
+    // A's edge boxes about A's circle centre, edge a in lane a (every box of a polygon that
+    // is not cullable meets every other: its pairs take the segment test)
     bool cullA = cullable(bA);
-    if (lane < nA) {
+    float4 aBox = make_float4(-INFINITY, INFINITY, -INFINITY, INFINITY);
+    if (lane < nA && cullA) {
         double2 p = L.P[lane], r = L.R[lane];
-        L.box[lane] = cullA ? edge_box(wrap(p.x - bA.x), wrap(p.y - bA.y), mk(r.x, r.y))
-                            : make_float4(-INFINITY, INFINITY, -INFINITY, INFINITY);
+        aBox = edge_box(wrap(p.x - bA.x), wrap(p.y - bA.y), mk(r.x, r.y));
     }
     int incl = cnt;
 #pragma unroll
@@ -1166,7 +1173,8 @@ __global__ __launch_bounds__(BLK) void k_detect_tile(
             L.uBox[u] = boxed ? 1 : 0;
         }
         wave_lds_sync();
-        tile_pairs<CELLS, SEG, CONS>(L, lane, nU, A, stA, nA, sAl, s0, Bu, bA.x, bA.y, pos, cellOf, out, num, kmax);
+        tile_pairs<CELLS, SEG, CONS>(L, lane, nU, A, stA, nA, sAl, s0, Bu, bA.x, bA.y, aBox, pos, cellOf, out, num,
+                                     kmax);
     }
     // the largest key of the packing (the pairing's last-record case)
 #pragma unroll

@@ -216,16 +216,17 @@ __device__ __forceinline__ bool find_record(uint64_t key, int pbase, int Kp, con
 
 // The exit links from record (inter, outer) (row ownRow): whether they close back to it
 // within 8 hops (a closed lens boundary).  Each link's record is searched for once; with
-// KEEP the cycle's members (inter first, then in link order) and their rows go to
-// keys/rows, *cnt of them (at most CYCLE_MAX).
+// KEEP the rows of the cycle's members (inter first, then in link order) go to rows,
+// *cnt of them (at most CYCLE_MAX), and *k0 is the place of the smallest key among them
+// (only the rows are kept, not the keys: registers set the kernels' occupancy).
 constexpr int CYCLE_MAX = 10;
 template <bool KEEP>
 __device__ __forceinline__ bool cycle_walk(uint64_t inter, uint64_t outer, long long ownRow, int pbase, int Kp,
                                            const int* __restrict__ bcnt, const uint64_t* __restrict__ rKey,
-                                           const uint64_t* __restrict__ rOuter, uint64_t* keys, long long* rows,
-                                           int* cnt) {
+                                           const uint64_t* __restrict__ rOuter, long long* rows, int* cnt, int* k0) {
     int n = 1;
-    if (KEEP) { keys[0] = inter; rows[0] = ownRow; }
+    uint64_t kmin = inter;
+    if (KEEP) { rows[0] = ownRow; *k0 = 0; }
     uint64_t cur = outer;
     for (int hop = 0;; ++hop) {
         if (cur == inter) {
@@ -235,12 +236,15 @@ __device__ __forceinline__ bool cycle_walk(uint64_t inter, uint64_t outer, long 
         uint64_t nxt = 0;
         long long row = 0;
         if (!find_record(cur, pbase, Kp, bcnt, rKey, rOuter, &nxt, &row)) break;
-        if (KEEP) { keys[n] = cur; rows[n] = row; }
+        if (KEEP) {
+            rows[n] = row;
+            if (cur < kmin) { kmin = cur; *k0 = n; }
+        }
         ++n;
         if (hop >= 8 || nxt == cur) break;
         cur = nxt;
     }
-    if (KEEP) *cnt = 1;
+    if (KEEP) { *cnt = 1; *k0 = 0; }
     return false;
 }
 __device__ __forceinline__ bool closed_cycle(uint64_t inter, uint64_t outer, int pbase, int Kp,
@@ -250,18 +254,16 @@ __device__ __forceinline__ bool closed_cycle(uint64_t inter, uint64_t outer, int
 }
 
 // Area of the lens of a record: half the h sums (Er) of the records of its closed cycle
-// (keys/rows/cnt from cycle_walk<true>), added in cycle order from the record with the
+// (rows/cnt/k0 from cycle_walk<true>), added in cycle order from the record with the
 // smallest key; off a cycle, the record alone.  *rep: the record represents its lens
 // (smallest key of the cycle, or off any).
-__device__ __forceinline__ double lens_area(bool closed, const uint64_t* keys, const long long* rows, int cnt,
+__device__ __forceinline__ double lens_area(bool closed, const long long* rows, int cnt, int k0,
                                             const double* __restrict__ Er, int* lensN, bool* rep) {
     if (!closed) {
         *lensN = 1;
         *rep = true;
         return 0.5 * Er[rows[0]];
     }
-    int k0 = 0;
-    for (int k = 1; k < cnt; ++k) if (keys[k] < keys[k0]) k0 = k;
     double sum = 0.0;
     for (int k = 0; k < cnt; ++k) {
         int q = k0 + k;
@@ -308,13 +310,12 @@ __global__ __launch_bounds__(64) void k_lens_rep(int N, int Kp, const int* __res
         uint64_t inter = rKey[base + f], outer = rOuter[base + f];
         double la = 0.0;
         if (inter != outer) {
-            uint64_t ck[CYCLE_MAX];
             long long cr[CYCLE_MAX];
-            int cn = 1;
-            bool closed = cycle_walk<true>(inter, outer, base + f, pbase, Kp, bcnt, rKey, rOuter, ck, cr, &cn);
+            int cn = 1, k0 = 0;
+            bool closed = cycle_walk<true>(inter, outer, base + f, pbase, Kp, bcnt, rKey, rOuter, cr, &cn, &k0);
             int lensN;
             bool rep;
-            double a = lens_area(closed, ck, cr, cn, Er, &lensN, &rep);
+            double a = lens_area(closed, cr, cn, k0, Er, &lensN, &rep);
             if (rep) la = a;
         }
         laRep[base + f] = la;
@@ -445,18 +446,17 @@ __global__ __launch_bounds__(64) void k_exterior_poly(
             d2 fkl = wrap2(mk(pkk.x + t2 * r2.x, pkk.y + t2 * r2.y));
             sp = fij;
             bool closed = false;
-            uint64_t ck[SQ ? CYCLE_MAX : 1];
             long long cr[SQ ? CYCLE_MAX : 1];
-            int cn = 1;
+            int cn = 1, k0 = 0;
             if (AREA) {
                 // a record on a closed cycle of exit links (a closed lens boundary) measures
                 // from one origin per polygon pair: the first vertex of the lower polygon
-                closed = cycle_walk<SQ>(inter, outer, base + f, pbase, Kp, bcnt, rKey, rOuter, ck, cr, &cn);
+                closed = cycle_walk<SQ>(inter, outer, base + f, pbase, Kp, bcnt, rKey, rOuter, cr, &cn, &k0);
                 if (closed) sp = ldpos(pos, polyStart[min(s, sjg)]);
             }
             if (SQ) {
                 bool rep;
-                lensA = lens_area(closed, ck, cr, cn, Er, &lensN, &rep);
+                lensA = lens_area(closed, cr, cn, k0, Er, &lensN, &rep);
                 scale = PAIR ? pair_area(s, sjg, pbase, Kp, bcnt, rKey, laRep) : lensA;
             }
             double dfa[16], dfb[16];
