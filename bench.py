@@ -32,9 +32,10 @@ METRIC = "packings minimized/sec to |F|max<1e-10 (learned vs plain); force evals
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
-# FP64 operations of one segment test in k_detect_poly (2 minimum-image wraps of 5 ops,
-# cross product 3, |denom| test 1, two parameters 4+4, four range compares 4), counted
-# over the edge pairs whose test ran (pcp_detect_stats: after the edge-circle cull)
+# FP64 operations of one segment test (2 minimum-image wraps of 5 ops, cross product 3,
+# |denom| test 1, two parameters 4+4, four range compares 4), counted over the edge pairs
+# whose test ran (pcp_detect_stats: after the edge-circle cull of k_detect_poly, after the
+# FP32 edge-box screen of k_detect_tile)
 FLOPS_PER_EDGE_PAIR = 21
 # bytes the detection must move from HBM per launch: every vertex position once (16 B),
 # every polygon's bounding circle and sorted id (36 B) and its two bucket counters (8 B),
@@ -42,12 +43,13 @@ FLOPS_PER_EDGE_PAIR = 21
 BYTES_PER_VERTEX = 16
 BYTES_PER_POLYGON = 44
 BYTES_PER_CROSSING = 24
-DETECT_KERNEL = "k_detect_poly<0>"
-# HBM bytes per k_detect_poly<0> launch on this workload when the in-run measurement below
-# cannot run: rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a copy kernel's readings, from
-# profiles/k_detect_poly_hbm_traffic_N4096.md (the command is given there)
-DETECT_TRAFFIC_CITED = 4228712
-DETECT_TRAFFIC_SOURCE = "profiles/k_detect_poly_hbm_traffic_N4096.md"
+# the detection kernels by schedule (pcp_get_detect_schedule: 0 culled enumeration, 1 whole tiles)
+DETECT_KERNELS = {0: "k_detect_poly<0>", 1: "k_detect_tile"}
+# HBM bytes per launch on this workload when the in-run measurement below cannot run:
+# rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a copy kernel's readings, from the profile named
+# (the command is given there)
+DETECT_TRAFFIC_CITED = {"k_detect_poly<0>": (4440077, "profiles/k_detect_poly_hbm_traffic_N4096.md"),
+                        "k_detect_tile": (None, "profiles/k_detect_tile_N4096.md")}
 COPY_DOUBLES = 1 << 25
 
 
@@ -458,7 +460,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cells", choices=["diameter", "edge"], default="diameter")
-    ap.add_argument("--roofline-kernel", default=DETECT_KERNEL)
+    ap.add_argument("--roofline-kernel", default=None,
+                    help="kernel to bracket with events (default: the detection kernel of the schedule in effect)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fire-seconds", type=float, default=20.0)
@@ -515,9 +518,12 @@ def main():
         s.force_eval()
     s.sync()
     nI = s.get_records(0)["numIntersections"]
-    ppairs, epairs = ctypes.c_longlong(0), ctypes.c_longlong(0)
-    s.ctx.call("pcp_detect_stats", ctypes.byref(ppairs), ctypes.byref(epairs))
+    ppairs, epairs, screened = ctypes.c_longlong(0), ctypes.c_longlong(0), ctypes.c_longlong(0)
+    s.ctx.call("pcp_detect_stats", ctypes.byref(ppairs), ctypes.byref(epairs), ctypes.byref(screened))
     tests = epairs.value
+    detect_kernel = DETECT_KERNELS[s.get_detect_schedule()]
+    if args.roofline_kernel is None:
+        args.roofline_kernel = detect_kernel
 
     s.ctx.call("pcp_profile_kernel", args.roofline_kernel.encode())
     s.sync()
@@ -536,36 +542,39 @@ def main():
     kavg_s = (ms.value / max(nl.value, 1)) * 1e-3
 
     print(f"[bench] {args.steps} evals in {el:.3f} s; {args.roofline_kernel}: {nl.value} launches, avg "
-          f"{kavg_s * 1e6:.2f} us (launch events); records {nI}; polygon pairs {ppairs.value}; edge-pair tests "
-          f"{tests}", flush=True)
+          f"{kavg_s * 1e6:.2f} us (launch events); records {nI}; polygon pairs {ppairs.value}; edge pairs "
+          f"screened {screened.value}, segment tests {tests}", flush=True)
 
-    if args.roofline_kernel == DETECT_KERNEL and nl.value > 0:
+    if args.roofline_kernel == detect_kernel and nl.value > 0:
         flops = FLOPS_PER_EDGE_PAIR * tests
         nbytes = BYTES_PER_VERTEX * s.V + BYTES_PER_POLYGON * s.N + BYTES_PER_CROSSING * nI
         gbs = nbytes / kavg_s / 1e9
-        traffic, tsrc, tdetail = DETECT_TRAFFIC_CITED, f"cited: {DETECT_TRAFFIC_SOURCE}", None
+        cited, csrc = DETECT_TRAFFIC_CITED[detect_kernel]
+        traffic, tsrc, tdetail = cited, f"cited: {csrc}", None
         if args.traffic == "measure":
             try:
-                traffic, tdetail = measure_traffic(DETECT_KERNEL, args.seed)
+                traffic, tdetail = measure_traffic(detect_kernel, args.seed)
                 traffic = int(round(traffic))
                 tsrc = ("measured in this run: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over 20 evaluations, "
                         "each scaled by k_copy's true bytes / reading")
             except Exception as e:   # profiler missing or refused: keep the cited figure
                 print(f"[bench] traffic measurement failed ({e}); using the cited figure", flush=True)
+        counters = ("profiles/k_detect_tile_N4096.md" if detect_kernel == "k_detect_tile"
+                    else "profiles/k_detect_poly_counters_N4096.md")
         roof = dict(bound="hbm", achieved=gbs, peak=HBM_PEAK_GBS, unit="GB/s", frac=gbs / HBM_PEAK_GBS,
                     traffic=traffic, traffic_source=tsrc, traffic_counters=tdetail,
-                    traffic_over_algorithmic=traffic / nbytes, kernel=DETECT_KERNEL,
+                    traffic_over_algorithmic=(traffic / nbytes) if traffic else None, kernel=detect_kernel,
                     algorithmic_bytes_per_launch=nbytes, avg_launch_us=kavg_s * 1e6,
                     fp64=dict(achieved_tflops=flops / kavg_s / 1e12, peak_tflops=FP64_PEAK_TFLOPS,
                               frac=flops / kavg_s / 1e12 / FP64_PEAK_TFLOPS, flops_per_launch=flops,
-                              note="FP64 VALU (no MFMA in the geometry): 21 FP64 operations per edge-pair test"),
-                    edge_pair_tests=tests, polygon_pairs=ppairs.value,
+                              note="FP64 VALU (no MFMA in the geometry): 21 FP64 operations per segment test"),
+                    edge_pairs_screened=screened.value, edge_pair_tests=tests, polygon_pairs=ppairs.value,
                     limiter="neither bound: the launch is one round of resident wavefronts (4096 waves, 16 per "
                             "CU) and lasts as long as its slowest wavefront's chain of dependent memory "
-                            "round trips (profiles/k_detect_poly_counters_N4096.md)")
-        print(f"[bench] roofline {DETECT_KERNEL}: {gbs:.1f} GB/s algorithmic ({100 * gbs / HBM_PEAK_GBS:.2f} % of "
-              f"HBM), traffic {traffic} B/launch = {traffic / nbytes:.2f}x algorithmic ({tsrc.split(':')[0]})",
-              flush=True)
+                            f"round trips ({counters})")
+        tr = f"{traffic} B/launch = {traffic / nbytes:.2f}x algorithmic" if traffic else "not measured"
+        print(f"[bench] roofline {detect_kernel}: {gbs:.1f} GB/s algorithmic ({100 * gbs / HBM_PEAK_GBS:.2f} % of "
+              f"HBM), traffic {tr} ({tsrc.split(':')[0]})", flush=True)
     else:
         roof = dict(kernel=args.roofline_kernel, avg_launch_us=kavg_s * 1e6, achieved=None, peak=None,
                     unit=None, frac=None, traffic=None, bound=None)

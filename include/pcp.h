@@ -342,12 +342,16 @@ int pcp_get_detect_schedule(pcp_ctx* ctx, int* mode);
  * fastest is kept.  timesUs host[11] (optional): microseconds per launch of pair 16/32/64,
  * lens 16/32/64, exterior 16/32/64, detection 0/1 (0 = not a candidate). */
 int pcp_autotune(pcp_ctx* ctx, const pcp_state* st, int reps, double* timesUs);
-/* work of the last detection: polygon pairs tested (each polygon with itself, each pair
- * of distinct polygons whose bounding circles meet once) and the edge pairs whose segment
- * test ran (n(n-3)/2 per polygon with itself; per pair of distinct polygons the product
- * of the counts of edges of each near the other's circle, or nA*nB where the cull does
- * not apply).  The last detection's position buffer must still be valid. */
-int pcp_detect_stats(pcp_ctx* ctx, long long* polygonPairs, long long* edgePairs);
+/* work of the last detection under the schedule in effect: polygon pairs tested (each polygon
+ * with itself, each pair of distinct polygons whose bounding circles meet once), the edge
+ * pairs whose segment test ran, and (screened, optional) the edge pairs considered.  Culled
+ * enumeration: n(n-3)/2 per polygon with itself; per pair of distinct polygons the product
+ * of the counts of edges of each near the other's circle, or nA*nB where the cull does not
+ * apply (screened = edgePairs).  Whole tiles: every non-adjacent pair of a polygon and every
+ * nA*nB pair are screened, those whose FP32 edge boxes meet (all of them where a polygon is
+ * not cullable) take the segment test.  The last detection's position buffer must still be
+ * valid. */
+int pcp_detect_stats(pcp_ctx* ctx, long long* polygonPairs, long long* edgePairs, long long* screened);
 /* records of the last pairing (pcp_force_eval, pcp_update_outersections) that belong to a
  * pair of distinct polygons with an odd number of crossings.  The boundary of every overlap
  * crosses the other polygon's boundary an even number of times; an odd count arises where a

@@ -91,7 +91,7 @@ SIGNATURES = {
     "pcp_profile_kernel": (_i, [_p, ctypes.c_char_p]),
     "pcp_profile_read": (_i, [_p, _p, _p]),
     "pcp_stream_copy": (_i, [_p, _p, _p, ctypes.c_longlong]),
-    "pcp_detect_stats": (_i, [_p, _p, _p]),
+    "pcp_detect_stats": (_i, [_p, _p, _p, _p]),
     "pcp_odd_crossing_records": (_i, [_p, _p]),
     "pcp_set_schedule": (_i, [_p, _i, _i, _i]),
     "pcp_get_schedule": (_i, [_p, _p, _p, _p]),

@@ -695,15 +695,17 @@ __device__ void detect_polygon(
         }
         int Bq = -1, stq = 0, nq = 0;
         double4 bc = make_double4(0.0, 0.0, 0.0, 1.0);
-        if (ci < T) {   // the candidate's circle and vertex range, loaded together
+        if (ci < T) {
             int cand = pcSorted[s0 + q];
-            double4 c4 = bounds[cand];
-            int st4 = polyStart[cand], en4 = polyStart[cand + 1];
-            if (cand > A && (big || circles_meet(bA, c4))) {
-                Bq = cand;
-                bc = c4;
-                stq = st4;
-                nq = en4 - st4;
+            if (cand > A) {   // the candidate's circle and vertex range, loaded together
+                double4 c4 = bounds[cand];
+                int st4 = polyStart[cand], en4 = polyStart[cand + 1];
+                if (big || circles_meet(bA, c4)) {
+                    Bq = cand;
+                    bc = c4;
+                    stq = st4;
+                    nq = en4 - st4;
+                }
             }
         }
         bool packed = Bq >= 0 && nA <= 64 && nq <= 64;
@@ -1044,6 +1046,10 @@ __device__ void tile_pairs(TileLds& L, int lane, int nU, int A, int stA, int nA,
             wave_lds_sync();   // the list's reads are issued before it is refilled
             nmay = 0;
         };
+        // the edges a of A this lane's edge is tested against, as a bit mask: every edge of A, or
+        // for A itself the non-adjacent ones below b (a < b - 1, and not edge 0 with edge nA - 1)
+        unsigned long long allow = 0ull;
+        if (act) allow = self ? (bits_below(b - 1) & ~((b == nA - 1) ? 1ull : 0ull)) : bits_below(nA);
         // every edge a of A against every lane's edge: the pairs whose boxes meet are collected
         for (int a0 = 0; a0 < nA; a0 += TILE_UNROLL) {
             bool may[TILE_UNROLL];
@@ -1051,10 +1057,8 @@ __device__ void tile_pairs(TileLds& L, int lane, int nU, int A, int stA, int nA,
 #pragma unroll
             for (int k = 0; k < TILE_UNROLL; ++k) {
                 int a = a0 + k;
-                bool ok = act && a < nA;
-                if (self) ok = ok && b > a && b != a + 1 && !(a == 0 && b == nA - 1);
                 float4 ab = lane_box(aBox, (a < nA) ? a : 0);   // A's edge a, from lane a
-                may[k] = ok && boxes_meet(ab, bb);
+                may[k] = a < nA && ((allow >> a) & 1ull) && boxes_meet(ab, bb);   // a < nA <= 64
                 anyMay = anyMay || may[k];
             }
             if (__ballot(anyMay) == 0ull) continue;   // wavefront-uniform: most blocks find nothing
@@ -1144,15 +1148,17 @@ __global__ __launch_bounds__(BLK) void k_detect_tile(
         bool keep = false, boxed = false;
         int cand = -1;
         int stq = 0, nq = 0;
-        if (ci < T) {   // the candidate's circle and vertex range, loaded together
+        if (ci < T) {
             cand = pcSorted[s0 + q];
-            double4 c4 = bounds[cand];
-            int st4 = polyStart[cand], en4 = polyStart[cand + 1];
-            keep = cand > A && (big || circles_meet(bA, c4));
-            boxed = cullA && cullable(c4);
-            if (keep) {
-                stq = st4;
-                nq = en4 - st4;
+            if (cand > A) {   // the candidate's circle and vertex range, loaded together
+                double4 c4 = bounds[cand];
+                int st4 = polyStart[cand], en4 = polyStart[cand + 1];
+                keep = big || circles_meet(bA, c4);
+                boxed = cullA && cullable(c4);
+                if (keep) {
+                    stq = st4;
+                    nq = en4 - st4;
+                }
             }
         }
         unsigned long long km = __ballot(keep);
@@ -1293,13 +1299,13 @@ static int launch_detect_tile(pcp_ctx* c, const double2* pos, bool cellsFromList
 }
 
 // the detection schedule in effect: the one set (pcp_set_detect_schedule / pcp_autotune), else
-// tiles for contexts of several packings of small polygons (there the kernel is bound by its
-// instruction issue, and the tile needs fewer instructions than the culled enumeration) and the
-// culled enumeration otherwise (one packing is bound by each wavefront's latency, which the
-// cull shortens; with 32-lane segments small polygons leave tile lanes idle)
+// tiles for polygons of at most 16 vertices (the box-screened tile needs fewer instructions and
+// fewer dependent steps than listing and enumerating the culled pairs: 20.0 against 21.8 us at
+// one N=4096 packing, 245 against 388 us at 32), the culled enumeration otherwise (with 32- or
+// 64-lane segments small polygons leave tile lanes idle)
 int detect_schedule(const pcp_ctx* c) {
     if (c->detSched >= 0) return (c->maxN <= 64) ? c->detSched : 0;
-    return (c->maxN <= 64 && c->P >= 4 && c->seg <= 16) ? 1 : 0;
+    return (c->maxN <= 64 && c->seg <= 16) ? 1 : 0;
 }
 
 // cellsFromList: use the cells of the last run_cells (API updateNeighbors).  ps: FIRE's
@@ -1361,9 +1367,16 @@ int detect_alloc(pcp_ctx* c) {
 // and positions with the kernel's rules: polygon pairs tested (each polygon with itself,
 // each pair of polygons once) and the edge pairs whose segment test ran (A with itself:
 // its non-adjacent pairs; A with B: the products of the edge lists left by the cull).
-int detect_stats(pcp_ctx* c, long long* polygonPairs, long long* edgePairs) {
+int detect_stats(pcp_ctx* c, long long* polygonPairs, long long* edgePairs, long long* screened) {
     PCP_HIP(hipStreamSynchronize(c->stream));
-    if (!c->lastDetectPos) { *polygonPairs = 0; *edgePairs = 0; return PCP_OK; }
+    long long scr = 0;
+    if (!c->lastDetectPos) {
+        *polygonPairs = 0;
+        *edgePairs = 0;
+        if (screened) *screened = 0;
+        return PCP_OK;
+    }
+    bool tiles = detect_schedule(c) == 1;
     std::vector<double4> bd(c->N);
     std::vector<int> sorted(c->N), packB(c->P), start((size_t)c->P * (PC_CELLS + 1));
     std::vector<double2> X(c->V);
@@ -1388,6 +1401,37 @@ int detect_stats(pcp_ctx* c, long long* polygonPairs, long long* edgePairs) {
         }
         return k;
     };
+    // the tile schedule's FP32 edge boxes about A's circle centre (edge_box, boxes_meet)
+    auto boxOf = [&](int q, int e, const double4& o) {
+        int st = c->hPolyStart[q], n = c->hPolyStart[q + 1] - st;
+        double2 p0 = X[st + e], p1 = X[st + ((e + 1 == n) ? 0 : e + 1)];
+        double Rx = wrapH(p1.x - p0.x), Ry = wrapH(p1.y - p0.y);
+        double rx = wrapH(p0.x - o.x), ry = wrapH(p0.y - o.y);
+        double x0 = std::fmin(rx, rx + Rx), x1 = std::fmax(rx, rx + Rx);
+        double y0 = std::fmin(ry, ry + Ry), y1 = std::fmax(ry, ry + Ry);
+        return std::array<float, 4>{(float)(x0 - TILE_BOX_EPS), (float)(x1 + TILE_BOX_EPS), (float)(y0 - TILE_BOX_EPS),
+                                    (float)(y1 + TILE_BOX_EPS)};
+    };
+    auto meet = [](const std::array<float, 4>& a, const std::array<float, 4>& b) {
+        return a[0] <= b[1] && b[0] <= a[1] && a[2] <= b[3] && b[2] <= a[3];
+    };
+    // edge pairs of A (edges ea) and B given the segment test by the tile: all of them where
+    // either polygon is not cullable, else those whose boxes meet
+    auto tilePairs = [&](int A, int Bq, bool self) {
+        int nA = c->hPolyStart[A + 1] - c->hPolyStart[A], nB = c->hPolyStart[Bq + 1] - c->hPolyStart[Bq];
+        long long k = 0, all = 0;
+        bool boxed = cullOk(A) && cullOk(Bq);
+        for (int a = 0; a < nA; ++a) {
+            auto ba = boxOf(A, a, bd[A]);
+            for (int b = 0; b < nB; ++b) {
+                if (self && !(b > a + 1 && !(a == 0 && b == nA - 1))) continue;
+                ++all;
+                if (!boxed || meet(ba, boxOf(Bq, b, bd[A]))) ++k;
+            }
+        }
+        scr += all;
+        return k;
+    };
     long long pp = 0, ep = 0;
     for (int p = 0; p < c->P; ++p) {
         int B = packB[p] & 0xFFFF;
@@ -1396,7 +1440,15 @@ int detect_stats(pcp_ctx* c, long long* polygonPairs, long long* edgePairs) {
         const int* st = start.data() + (size_t)p * (PC_CELLS + 1);
         for (int A = s0; A < c->hPackPoly[p + 1]; ++A) {
             long long nA = c->hPolyStart[A + 1] - c->hPolyStart[A];
-            if (nA >= 4) { ++pp; ep += nA * (nA - 3) / 2; }
+            if (nA >= 4) {
+                ++pp;
+                if (tiles) {
+                    ep += tilePairs(A, A, true);
+                } else {
+                    ep += nA * (nA - 3) / 2;
+                    scr += nA * (nA - 3) / 2;
+                }
+            }
             int ax = coord(bd[A].x, B), ay = coord(bd[A].y, B);
             int ncell = (B == 1) ? 1 : 9;
             for (int k = 0; k < ncell; ++k) {
@@ -1413,14 +1465,21 @@ int detect_stats(pcp_ctx* c, long long* polygonPairs, long long* edgePairs) {
                     }
                     ++pp;
                     long long nB = c->hPolyStart[Bq + 1] - c->hPolyStart[Bq];
-                    if (nA <= 64 && nB <= 64 && cullOk(A) && cullOk(Bq)) ep += nearCount(A, Bq) * nearCount(Bq, A);
-                    else ep += nA * nB;
+                    if (tiles) {
+                        ep += tilePairs(A, Bq, false);
+                    } else {
+                        long long k = (nA <= 64 && nB <= 64 && cullOk(A) && cullOk(Bq))
+                                          ? nearCount(A, Bq) * nearCount(Bq, A) : nA * nB;
+                        ep += k;
+                        scr += k;
+                    }
                 }
             }
         }
     }
     *polygonPairs = pp;
     *edgePairs = ep;
+    if (screened) *screened = scr;
     return PCP_OK;
 }
 

@@ -183,7 +183,7 @@ int run_detect(pcp_ctx* c, const double2* pos, int cellsFromList, const PostShak
                bool cons = false);
 int detect_schedule(const pcp_ctx* c);
 int detect_alloc(pcp_ctx* c);
-int detect_stats(pcp_ctx* c, long long* polygonPairs, long long* edgePairs);
+int detect_stats(pcp_ctx* c, long long* polygonPairs, long long* edgePairs, long long* screened);
 int seg_width(int maxN);
 int alloc_buckets(pcp_ctx* c, int Kp);
 int grow_buckets(pcp_ctx* c, int need);

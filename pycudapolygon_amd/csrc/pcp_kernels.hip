@@ -20,6 +20,8 @@
 // bit-reproducible run to run.
 #include "pcp_internal.hpp"
 #include <algorithm>
+#include <array>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -1528,8 +1530,9 @@ int pcp_stream_copy(pcp_ctx* c, double* dst, const double* src, long long n) {
     return PCP_OK;
 }
 
-int pcp_detect_stats(pcp_ctx* c, long long* polygonPairs, long long* edgePairs) {
-    return detect_stats(c, polygonPairs, edgePairs);
+int pcp_detect_stats(pcp_ctx* c, long long* polygonPairs, long long* edgePairs, long long* screened) {
+    if (!polygonPairs || !edgePairs) return fail_arg("pcp_detect_stats: polygonPairs and edgePairs are required");
+    return detect_stats(c, polygonPairs, edgePairs, screened);
 }
 int pcp_odd_crossing_records(pcp_ctx* c, int* count) {
     if (!count) return fail_arg("pcp_odd_crossing_records: count is null");
