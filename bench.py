@@ -9,7 +9,8 @@ exterior/interior forces, projection) -- the unit behind "force evals/sec".
 Positions stay resident in HBM; the timed region contains only device work.
 
 Alongside: a live roofline of one kernel (HIP events on the context's stream around
-every launch of that kernel inside the timed region), the CPU oracle timed on the
+every launch of that kernel in a second timed pass of the same evaluations, the first
+pass giving the throughput without them), the CPU oracle timed on the
 host cores on a bounded sample of the same workload, and bounded FIRE legs reporting
 packings minimized/s to max|F| <= 1e-10: the N=4096 packing (reference dtMax, scaled
 dtMax, learned warm start), 32 packings of N=4096 x 16 in one context (plain vs learned,
@@ -525,20 +526,26 @@ def main():
     if args.roofline_kernel is None:
         args.roofline_kernel = detect_kernel
 
-    s.ctx.call("pcp_profile_kernel", args.roofline_kernel.encode())
+    # throughput: the timed steps with nothing but the evaluations on the stream
     s.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         s.force_eval()
     s.sync()
     el = time.perf_counter() - t0
+    evals_per_s = args.steps / el
+    # the roofline kernel's duration: the same number of evaluations again, every launch of that
+    # kernel carrying its own start/stop events (recorded by the dispatch, on the context's
+    # stream); the events add ~5 us per evaluation, so they stay out of the throughput pass
+    s.ctx.call("pcp_profile_kernel", args.roofline_kernel.encode())
+    s.sync()
+    for _ in range(args.steps):
+        s.force_eval()
+    s.sync()
     ms = ctypes.c_double(0.0)
     nl = ctypes.c_int(0)
     s.ctx.call("pcp_profile_read", ctypes.byref(ms), ctypes.byref(nl))
     s.ctx.call("pcp_profile_kernel", None)
-    evals_per_s = args.steps / el
-    # the profiled launch carries its own start/stop events (recorded by the dispatch), so this
-    # is the kernel's duration on the context's stream
     kavg_s = (ms.value / max(nl.value, 1)) * 1e-3
 
     print(f"[bench] {args.steps} evals in {el:.3f} s; {args.roofline_kernel}: {nl.value} launches, avg "

@@ -494,7 +494,7 @@ static int fire_eval(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm
                              prm->xpbdIter, prm->xpbdTol, c->dActive, v, c->dShakeIt));
         if (overlap) {
             PCP_TRY(run_detect(c, x, 0, nullptr, cons));
-            PCP_TRY(run_records(c, area ? x : nullptr));
+            PCP_TRY(run_records(c, area ? x : nullptr, true));
         }
     } else if (overlap) {
         if (!area)   // reference order (model.cpp:416-440): detect, SHAKE, records
@@ -518,7 +518,7 @@ static int fire_eval(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm
             if (area)   // area energy: crossings of the final positions
                 PCP_TRY(run_detect(c, x, 0, nullptr, cons));
         }
-        PCP_TRY(run_records(c, area ? x : nullptr));
+        PCP_TRY(run_records(c, area ? x : nullptr, true));
     }
     PCP_TRY(run_forces(c, st->model, x, st->edgeLengths, st->areas, st->targetEdgeV, st->targetAreas,
                        st->stiffness, st->compressibility, (double2*)st->force, nullptr, 1, rigid, false,

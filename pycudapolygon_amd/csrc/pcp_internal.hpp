@@ -151,6 +151,8 @@ struct pcp_ctx {
     // detection schedule: 0 = culled edge pairs (k_detect_poly), 1 = whole edge tiles of polygon
     // pairs (k_detect_tile, polygons of at most 64 vertices), -1 = by the context's shape
     int detSched = -1;
+    // the reference pairing of the last run_records is left to the next run_forces (k_pair_exterior)
+    bool pairDeferred = false;
     // multi-step FIRE (pcp_fire_run): force backups for device-side rollback, a block-start
     // snapshot (x, v, F; per packing dt, alpha, energy, max|F| and nPos, steps, active), the
     // largest bucket count seen in the block, and the captured graph of one block of steps
@@ -188,7 +190,7 @@ int seg_width(int maxN);
 int alloc_buckets(pcp_ctx* c, int Kp);
 int grow_buckets(pcp_ctx* c, int need);
 int packing_counts(pcp_ctx* c, int p, std::vector<int>& cnt);
-int run_records(pcp_ctx* c, const double2* pairPos);
+int run_records(pcp_ctx* c, const double2* pairPos, bool deferOK = false);
 int run_forces(pcp_ctx* c, int model, const double2* pos, double* edgeLengths, double* areas,
                const double* targetEdgeV, const double* targetAreas, double stiffness,
                double compressibility, double2* force, double* energy, int geom, int project,

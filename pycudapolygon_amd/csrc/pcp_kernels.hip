@@ -1565,7 +1565,7 @@ int pcp_force_eval(pcp_ctx* c, const pcp_state* st) {
     int overlap = (mt == PCP_NORMAL || mt == PCP_HYBRID);
     if (overlap) {
         PCP_TRY(run_detect(c, pos, 0, nullptr, (st->model & PCP_CONSISTENT) != 0));
-        PCP_TRY(run_records(c, (st->model & PCP_AREA_MODES) ? pos : nullptr));
+        PCP_TRY(run_records(c, (st->model & PCP_AREA_MODES) ? pos : nullptr, true));
     }
     // the overflow check rides on the energy reduction when there is one
     bool fused = overlap && st->energy;

@@ -46,6 +46,110 @@ __device__ __forceinline__ int count_keys_below(const uint64_t* __restrict__ key
 }
 
 // ---------------------------------------------------------------- pairing
+// The pairing of record f of polygon s's si bucket (k_pair_poly): the record, its place in
+// the order (i, j, sj) of the bucket (rank), its exit record and that record's parameters
+// (the record itself when it has no partner).  mxk: the packing's largest key.
+struct PairRec {
+    uint64_t key, outer;
+    float2 tu, ut;
+    int rank;
+};
+template <int EXACT>
+__device__ __forceinline__ PairRec pair_record(int f, int cnt, long long base, int s, int pbase, int st1, int ni,
+                                               uint64_t mxk, int Kp, const int* __restrict__ polyStart,
+                                               const int* __restrict__ bcnt, const uint64_t* __restrict__ bKey,
+                                               const float2* __restrict__ bTU, const double2* __restrict__ pairPos,
+                                               int* __restrict__ scal) {
+    uint64_t key = bKey[base + f];
+    float2 tu = bTU[base + f];
+    uint64_t ord = bucket_order(key);
+    int si_l = key_si(key), sj_l = key_sj(key), i = key_i(key);
+    // rank in the bucket, and the crossings of this polygon pair held in this bucket
+    int rank = 0, nSame = 0;
+    for (int q0 = 0; q0 < cnt; q0 += 8) {
+        uint64_t v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = (q0 + k < cnt) ? bKey[base + q0 + k] : 0ull;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (q0 + k < cnt) {
+                rank += (bucket_order(v[k]) < ord) ? 1 : 0;
+                nSame += (key_sj(v[k]) == sj_l) ? 1 : 0;
+            }
+    }
+    int nOther = 0;   // the crossings of the pair held in the other polygon's bucket
+    int sjg = pbase + sj_l;
+    int st2 = polyStart[sjg], nj = polyStart[sjg + 1] - st2;
+    // exact parameter along s's edge ei of its crossing with sj's edge ej (area energy)
+    auto exact = [&](int ei, int ej, double fb) -> double {
+        int a = st1 + ei, b = st1 + ((ei + 1 == ni) ? 0 : ei + 1);
+        int c2 = st2 + ej, d = st2 + ((ej + 1 == nj) ? 0 : ej + 1);
+        return cross_param(ldpos(pairPos, a), ldpos(pairPos, b), ldpos(pairPos, c2), ldpos(pairPos, d), fb);
+    };
+    double tVal = (double)tu.y;
+    if (EXACT) tVal = exact(i, key_j(key), tVal);
+    uint64_t sij = ((uint64_t)si_l << 48) | ((uint64_t)sj_l << 32);
+    int bestDist = -1;
+    bool found = false;
+    double bestU = 0.0;
+    uint64_t bestOrd = 0, bestKey = 0;
+    float2 bestTU = make_float2(0.f, 0.f);
+    auto consider = [&](uint64_t k2, float2 tu2) {
+        int l = key_j(k2);
+        double uVal = (double)tu2.x;
+        if (EXACT && mxk >= sij) uVal = exact(l, key_i(k2), uVal);
+        int d = (l - i + ni) % ni;
+        if (l == i && tVal >= uVal) return;
+        if (bestDist == -1 || d < bestDist) { bestDist = d; found = false; }
+        if (d == bestDist && uVal == uVal) {
+            uint64_t o2 = bucket_order(k2);
+            if (!found || uVal < bestU || (uVal == bestU && o2 < bestOrd)) {
+                found = true; bestU = uVal; bestOrd = o2; bestKey = k2; bestTU = tu2;
+            }
+        }
+    };
+    if (mxk < sij) {
+        int s3 = pbase + key_si(mxk);
+        long long b3 = (long long)s3 * Kp;
+        int c3 = min(bcnt[s3], Kp);
+        for (int q = 0; q < c3; ++q)
+            if (bKey[b3 + q] == mxk) { consider(mxk, bTU[b3 + q]); break; }
+        // (no key of the packing reaches si<<48 | sj<<32, so sj's bucket holds no record of
+        // this pair: nOther stays 0)
+    } else {
+        long long b2 = (long long)sjg * Kp;
+        int c2 = min(bcnt[sjg], Kp);
+        for (int q0 = 0; q0 < c2; q0 += 8) {   // the group's loads first, then in order
+            uint64_t k8[8];
+            float2 t8[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                k8[k] = (q0 + k < c2) ? bKey[b2 + q0 + k] : 0ull;
+                t8[k] = (q0 + k < c2) ? bTU[b2 + q0 + k] : make_float2(0.f, 0.f);
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (q0 + k < c2 && key_sj(k8[k]) == si_l) {
+                    consider(k8[k], t8[k]);
+                    ++nOther;
+                }
+        }
+    }
+    // a polygon pair (not a polygon with itself) with an odd number of crossings
+    if (sjg != s && ((nSame + nOther) & 1)) atomicAdd(&scal[SC_ODD], 1);
+    // a partner whose edge indices fall outside the two polygons (possible only in the
+    // last-record case above) is treated as no partner: the reference would read
+    // vertices of other polygons there
+    if (found && (key_i(bestKey) >= nj || key_j(bestKey) >= ni)) found = false;
+    PairRec r;
+    r.key = key;
+    r.tu = tu;
+    r.rank = rank;
+    r.outer = found ? bestKey : key;
+    r.ut = found ? bestTU : tu;
+    return r;
+}
+
 // The reference's binary search starts with end = I-1, so when every key of the packing
 // lies below (si<<48 | sj<<32) the last record of the packing is the only candidate;
 // kept.  EXACT: crossing parameters recomputed from pairPos (area energy).
@@ -81,91 +185,12 @@ __global__ __launch_bounds__(64) void k_pair_poly(int N, int Kp, const int* __re
     }
     long long base = (long long)s * Kp;
     for (int f = lane; f < cnt; f += SEGX) {
-        uint64_t key = bKey[base + f];
-        float2 tu = bTU[base + f];
-        uint64_t ord = bucket_order(key);
-        int si_l = key_si(key), sj_l = key_sj(key), i = key_i(key);
-        // rank in the bucket, and the crossings of this polygon pair held in this bucket
-        int rank = 0, nSame = 0;
-        for (int q0 = 0; q0 < cnt; q0 += 8) {
-            uint64_t v[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) v[k] = (q0 + k < cnt) ? bKey[base + q0 + k] : 0ull;
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-                if (q0 + k < cnt) {
-                    rank += (bucket_order(v[k]) < ord) ? 1 : 0;
-                    nSame += (key_sj(v[k]) == sj_l) ? 1 : 0;
-                }
-        }
-        int nOther = 0;   // the crossings of the pair held in the other polygon's bucket
-        int sjg = pbase + sj_l;
-        int st2 = polyStart[sjg], nj = polyStart[sjg + 1] - st2;
-        // exact parameter along s's edge ei of its crossing with sj's edge ej (area energy)
-        auto exact = [&](int ei, int ej, double fb) -> double {
-            int a = st1 + ei, b = st1 + ((ei + 1 == ni) ? 0 : ei + 1);
-            int c2 = st2 + ej, d = st2 + ((ej + 1 == nj) ? 0 : ej + 1);
-            return cross_param(ldpos(pairPos, a), ldpos(pairPos, b), ldpos(pairPos, c2), ldpos(pairPos, d), fb);
-        };
-        double tVal = (double)tu.y;
-        if (EXACT) tVal = exact(i, key_j(key), tVal);
-        uint64_t sij = ((uint64_t)si_l << 48) | ((uint64_t)sj_l << 32);
-        int bestDist = -1;
-        bool found = false;
-        double bestU = 0.0;
-        uint64_t bestOrd = 0, bestKey = 0;
-        float2 bestTU = make_float2(0.f, 0.f);
-        auto consider = [&](uint64_t k2, float2 tu2) {
-            int l = key_j(k2);
-            double uVal = (double)tu2.x;
-            if (EXACT && mxk >= sij) uVal = exact(l, key_i(k2), uVal);
-            int d = (l - i + ni) % ni;
-            if (l == i && tVal >= uVal) return;
-            if (bestDist == -1 || d < bestDist) { bestDist = d; found = false; }
-            if (d == bestDist && uVal == uVal) {
-                uint64_t o2 = bucket_order(k2);
-                if (!found || uVal < bestU || (uVal == bestU && o2 < bestOrd)) {
-                    found = true; bestU = uVal; bestOrd = o2; bestKey = k2; bestTU = tu2;
-                }
-            }
-        };
-        if (mxk < sij) {
-            int s3 = pbase + key_si(mxk);
-            long long b3 = (long long)s3 * Kp;
-            int c3 = min(bcnt[s3], Kp);
-            for (int q = 0; q < c3; ++q)
-                if (bKey[b3 + q] == mxk) { consider(mxk, bTU[b3 + q]); break; }
-            // (no key of the packing reaches si<<48 | sj<<32, so sj's bucket holds no record of
-            // this pair: nOther stays 0)
-        } else {
-            long long b2 = (long long)sjg * Kp;
-            int c2 = min(bcnt[sjg], Kp);
-            for (int q0 = 0; q0 < c2; q0 += 8) {   // the group's loads first, then in order
-                uint64_t k8[8];
-                float2 t8[8];
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    k8[k] = (q0 + k < c2) ? bKey[b2 + q0 + k] : 0ull;
-                    t8[k] = (q0 + k < c2) ? bTU[b2 + q0 + k] : make_float2(0.f, 0.f);
-                }
-#pragma unroll
-                for (int k = 0; k < 8; ++k)
-                    if (q0 + k < c2 && key_sj(k8[k]) == si_l) {
-                        consider(k8[k], t8[k]);
-                        ++nOther;
-                    }
-            }
-        }
-        // a polygon pair (not a polygon with itself) with an odd number of crossings
-        if (sjg != s && ((nSame + nOther) & 1)) atomicAdd(&scal[SC_ODD], 1);
-        // a partner whose edge indices fall outside the two polygons (possible only in the
-        // last-record case above) is treated as no partner: the reference would read
-        // vertices of other polygons there
-        if (found && (key_i(bestKey) >= nj || key_j(bestKey) >= ni)) found = false;
-        rKey[base + rank] = key;
-        rTU[base + rank] = tu;
-        rOuter[base + rank] = found ? bestKey : key;
-        rUT[base + rank] = found ? bestTU : tu;
+        PairRec r = pair_record<EXACT>(f, cnt, base, s, pbase, st1, ni, mxk, Kp, polyStart, bcnt, bKey, bTU, pairPos,
+                                       scal);
+        rKey[base + r.rank] = r.key;
+        rTU[base + r.rank] = r.tu;
+        rOuter[base + r.rank] = r.outer;
+        rUT[base + r.rank] = r.ut;
     }
 }
 
@@ -384,154 +409,145 @@ __global__ __launch_bounds__(64) void k_lens_energy(int N, int Kp, const int* __
     }
 }
 
-// MODE 0: reference energy; 1: area energy; 2: squared lens areas; 3: squared pair areas.
-// One wavefront holds 64 / SEGX polygons, SEGX lanes each: lanes take the polygon's
-// records, then its vertices, SEGX at a time (a polygon has a few records, so a
-// wavefront per polygon would leave most lanes idle).  Every sum keeps its order.
-// GLOB: the per-polygon record staging lives in global memory (gstage, ext_lds_stride(Kp)
-// bytes per polygon slot of the launch) instead of LDS, for buckets too large for LDS; the
-// workgroup barriers order it as they order LDS (one wavefront per workgroup, one CU).
-template <int MODE, int SEGX, bool GLOB = false>
-__global__ __launch_bounds__(64) void k_exterior_poly(
-    int N, int Kp, const int* __restrict__ polyStart, const int* __restrict__ polyPack,
-    const int* __restrict__ packPoly, const double2* __restrict__ pos, const int* __restrict__ bcnt,
-    const int* __restrict__ jcnt, const uint64_t* __restrict__ jKey, const uint64_t* __restrict__ rKey,
-    const uint64_t* __restrict__ rOuter, const float2* __restrict__ rTU, const float2* __restrict__ rUT,
-    double2* __restrict__ jC, int* __restrict__ jT, double2* __restrict__ F1, double2* __restrict__ LMo,
-    double2* __restrict__ LNo, double* __restrict__ accR, const double* __restrict__ Er,
-    const double* __restrict__ laRep, const int* __restrict__ order, char* __restrict__ gstage) {
+// One record (inter, outer, float parameters tuR/utR) of polygon s for k_exterior_poly: its 8
+// exterior contributions and energy, the 4 acting on s staged in L at index idx (the record's
+// place in key order), the 4 acting on sj written at the record's rank in sj's list.  ownRow:
+// the record's row (the cycle walk of the area energies starts there).
+template <int MODE>
+__device__ __forceinline__ void exterior_record(uint64_t inter, uint64_t outer, float2 tuR, float2 utR, int idx,
+                                                long long ownRow, int s, int pbase, int st1, int n1, int Kp,
+                                                const int* __restrict__ polyStart, const double2* __restrict__ pos,
+                                                const int* __restrict__ bcnt, const int* __restrict__ jcnt,
+                                                const uint64_t* __restrict__ jKey, const uint64_t* __restrict__ rKey,
+                                                const uint64_t* __restrict__ rOuter, double2* __restrict__ jC,
+                                                int* __restrict__ jT, const double* __restrict__ Er,
+                                                const double* __restrict__ laRep, const ExtLds& L) {
     constexpr int AREA = (MODE >= 1), SQ = (MODE >= 2), PAIR = (MODE == 3);
-    constexpr int G = 64 / SEGX;
-    extern __shared__ char smem[];
-    __shared__ double leBuf[64];
-    int lane = threadIdx.x & (SEGX - 1), g = threadIdx.x / SEGX, segBase = g * SEGX;
-    ExtLds L = GLOB ? ext_lds(gstage + ((size_t)blockIdx.x * G + g) * ext_lds_stride(Kp), Kp)
-                    : ext_lds(smem + g * ext_lds_stride(Kp), Kp);
-    int slot = xcd_block(blockIdx.x, gridDim.x) * G + g;
-    bool has = slot < N;
-    int s = has ? order[slot] : 0;
-    int cnt = has ? min(bcnt[s], Kp) : 0;
-    int pk = polyPack[s];
-    int pbase = packPoly[pk];
-    int st1 = polyStart[s], n1 = has ? polyStart[s + 1] - st1 : 0;
-    long long base = (long long)s * Kp;
     const double sc = AREA ? 0.5 : 1.0;
-    for (int f = lane; f < cnt; f += SEGX) {
-        uint64_t inter = rKey[base + f], outer = rOuter[base + f];
-        int sjg = pbase + key_sj(inter);
-        int st2 = polyStart[sjg], n2 = polyStart[sjg + 1] - st2;
-        d2 inc[8];
-        int tg[8];
-        double e = 0.0;
-        d2 sp = mk(0.0, 0.0);
-        double lensA = 0.0, scale = 0.0;   // scale: lens area, or with PAIR the pair's overlap area
-        int lensN = 1;
-        bool valid = (inter != outer);
-        if (valid) {
-            int iL = key_i(inter), jL = key_j(inter), kL = key_i(outer), lL = key_j(outer);
-            int ziL = (iL + 1 == n1) ? 0 : iL + 1, zlL = (lL + 1 == n1) ? 0 : lL + 1;
-            int zjL = (jL + 1 == n2) ? 0 : jL + 1, zkL = (kL + 1 == n2) ? 0 : kL + 1;
-            d2 pi = ldpos(pos, st1 + iL), pj = ldpos(pos, st2 + jL), pkk = ldpos(pos, st2 + kL);
-            d2 pl = ldpos(pos, st1 + lL);
-            d2 pzi = ldpos(pos, st1 + ziL), pzj = ldpos(pos, st2 + zjL), pzk = ldpos(pos, st2 + zkL);
-            d2 pzl = ldpos(pos, st1 + zlL);
-            d2 r1 = wrap2(mk(pzi.x - pi.x, pzi.y - pi.y));
-            d2 r2 = wrap2(mk(pzk.x - pkk.x, pzk.y - pkk.y));
-            double t1 = (double)rTU[base + f].y, t2 = (double)rUT[base + f].y;
-            if (AREA) {   // exact crossing parameters at the current positions
-                t1 = cross_param(pi, pzi, pj, pzj, t1);
-                t2 = cross_param(pkk, pzk, pl, pzl, t2);
-            }
-            d2 fij = wrap2(mk(pi.x + t1 * r1.x, pi.y + t1 * r1.y));
-            d2 fkl = wrap2(mk(pkk.x + t2 * r2.x, pkk.y + t2 * r2.y));
-            sp = fij;
-            bool closed = false;
-            long long cr[SQ ? CYCLE_MAX : 1];
-            int cn = 1, k0 = 0;
-            if (AREA) {
-                // a record on a closed cycle of exit links (a closed lens boundary) measures
-                // from one origin per polygon pair: the first vertex of the lower polygon
-                closed = cycle_walk<SQ>(inter, outer, base + f, pbase, Kp, bcnt, rKey, rOuter, cr, &cn, &k0);
-                if (closed) sp = ldpos(pos, polyStart[min(s, sjg)]);
-            }
-            if (SQ) {
-                bool rep;
-                lensA = lens_area(closed, cr, cn, k0, Er, &lensN, &rep);
-                scale = PAIR ? pair_area(s, sjg, pbase, Kp, bcnt, rKey, laRep) : lensA;
-            }
-            double dfa[16], dfb[16];
-            if (iL == lL) {
-                e += hfun(fij, fkl, sp);
-                d2 ga, gb;
-                g12(fij, fkl, sp, ga, gb);
-                getDf(pi, pzi, pj, pzj, dfa);
-                getDf(pkk, pzk, pi, pzi, dfb);
-                inc[0] = dfdot(dfa, 0, ga);
-                inc[1] = dfdot(dfa, 2, ga);
-                inc[4] = dfdot(dfa, 4, ga);
-                inc[5] = dfdot(dfa, 6, ga);
-                inc[6] = dfdot(dfb, 0, gb);
-                inc[7] = dfdot(dfb, 2, gb);
-                inc[2] = dfdot(dfb, 4, gb);
-                inc[3] = dfdot(dfb, 6, gb);
-            } else {
-                e += hfun(fij, pzi, sp);
-                e += hfun(pl, fkl, sp);
-                d2 g1a, g2a;
-                g12(fij, pzi, sp, g1a, g2a);
-                getDf(pi, pzi, pj, pzj, dfa);
-                getDf(pkk, pzk, pl, pzl, dfb);
-                inc[0] = dfdot(dfa, 0, g1a);
-                inc[4] = dfdot(dfa, 4, g1a);
-                d2 dd = dfdot(dfa, 2, g1a);
-                inc[1] = mk(dd.x + g2a.x, dd.y + g2a.y);
-                inc[5] = dfdot(dfa, 6, g1a);
-                d2 g1b, g2b;
-                g12(pl, fkl, sp, g1b, g2b);
-                d2 de = dfdot(dfb, 4, g2b);
-                inc[2] = mk(g1b.x + de.x, g1b.y + de.y);
-                inc[6] = dfdot(dfb, 0, g2b);
-                inc[3] = dfdot(dfb, 6, g2b);
-                inc[7] = dfdot(dfb, 2, g2b);
-            }
-            tg[0] = iL; tg[1] = ziL; tg[2] = lL; tg[3] = zlL;
-            tg[4] = jL; tg[5] = zjL; tg[6] = kL; tg[7] = zkL;
-            L.il[f] = iL | (lL << 16);
+    int sjg = pbase + key_sj(inter);
+    int st2 = polyStart[sjg], n2 = polyStart[sjg + 1] - st2;
+    d2 inc[8];
+    int tg[8];
+    double e = 0.0;
+    d2 sp = mk(0.0, 0.0);
+    double lensA = 0.0, scale = 0.0;   // scale: lens area, or with PAIR the pair's overlap area
+    int lensN = 1;
+    bool valid = (inter != outer);
+    if (valid) {
+        int iL = key_i(inter), jL = key_j(inter), kL = key_i(outer), lL = key_j(outer);
+        int ziL = (iL + 1 == n1) ? 0 : iL + 1, zlL = (lL + 1 == n1) ? 0 : lL + 1;
+        int zjL = (jL + 1 == n2) ? 0 : jL + 1, zkL = (kL + 1 == n2) ? 0 : kL + 1;
+        d2 pi = ldpos(pos, st1 + iL), pj = ldpos(pos, st2 + jL), pkk = ldpos(pos, st2 + kL);
+        d2 pl = ldpos(pos, st1 + lL);
+        d2 pzi = ldpos(pos, st1 + ziL), pzj = ldpos(pos, st2 + zjL), pzk = ldpos(pos, st2 + zkL);
+        d2 pzl = ldpos(pos, st1 + zlL);
+        d2 r1 = wrap2(mk(pzi.x - pi.x, pzi.y - pi.y));
+        d2 r2 = wrap2(mk(pzk.x - pkk.x, pzk.y - pkk.y));
+        double t1 = (double)tuR.y, t2 = (double)utR.y;
+        if (AREA) {   // exact crossing parameters at the current positions
+            t1 = cross_param(pi, pzi, pj, pzj, t1);
+            t2 = cross_param(pkk, pzk, pl, pzl, t2);
+        }
+        d2 fij = wrap2(mk(pi.x + t1 * r1.x, pi.y + t1 * r1.y));
+        d2 fkl = wrap2(mk(pkk.x + t2 * r2.x, pkk.y + t2 * r2.y));
+        sp = fij;
+        bool closed = false;
+        long long cr[SQ ? CYCLE_MAX : 1];
+        int cn = 1, k0 = 0;
+        if (AREA) {
+            // a record on a closed cycle of exit links (a closed lens boundary) measures
+            // from one origin per polygon pair: the first vertex of the lower polygon
+            closed = cycle_walk<SQ>(inter, outer, ownRow, pbase, Kp, bcnt, rKey, rOuter, cr, &cn, &k0);
+            if (closed) sp = ldpos(pos, polyStart[min(s, sjg)]);
+        }
+        if (SQ) {
+            bool rep;
+            lensA = lens_area(closed, cr, cn, k0, Er, &lensN, &rep);
+            scale = PAIR ? pair_area(s, sjg, pbase, Kp, bcnt, rKey, laRep) : lensA;
+        }
+        double dfa[16], dfb[16];
+        if (iL == lL) {
+            e += hfun(fij, fkl, sp);
+            d2 ga, gb;
+            g12(fij, fkl, sp, ga, gb);
+            getDf(pi, pzi, pj, pzj, dfa);
+            getDf(pkk, pzk, pi, pzi, dfb);
+            inc[0] = dfdot(dfa, 0, ga);
+            inc[1] = dfdot(dfa, 2, ga);
+            inc[4] = dfdot(dfa, 4, ga);
+            inc[5] = dfdot(dfa, 6, ga);
+            inc[6] = dfdot(dfb, 0, gb);
+            inc[7] = dfdot(dfb, 2, gb);
+            inc[2] = dfdot(dfb, 4, gb);
+            inc[3] = dfdot(dfb, 6, gb);
         } else {
-#pragma unroll
-            for (int q = 0; q < 8; ++q) { tg[q] = -1; inc[q] = mk(0.0, 0.0); }
-            L.il[f] = -1;
+            e += hfun(fij, pzi, sp);
+            e += hfun(pl, fkl, sp);
+            d2 g1a, g2a;
+            g12(fij, pzi, sp, g1a, g2a);
+            getDf(pi, pzi, pj, pzj, dfa);
+            getDf(pkk, pzk, pl, pzl, dfb);
+            inc[0] = dfdot(dfa, 0, g1a);
+            inc[4] = dfdot(dfa, 4, g1a);
+            d2 dd = dfdot(dfa, 2, g1a);
+            inc[1] = mk(dd.x + g2a.x, dd.y + g2a.y);
+            inc[5] = dfdot(dfa, 6, g1a);
+            d2 g1b, g2b;
+            g12(pl, fkl, sp, g1b, g2b);
+            d2 de = dfdot(dfb, 4, g2b);
+            inc[2] = mk(g1b.x + de.x, g1b.y + de.y);
+            inc[6] = dfdot(dfb, 0, g2b);
+            inc[3] = dfdot(dfb, 6, g2b);
+            inc[7] = dfdot(dfb, 2, g2b);
         }
+        tg[0] = iL; tg[1] = ziL; tg[2] = lL; tg[3] = zlL;
+        tg[4] = jL; tg[5] = zjL; tg[6] = kL; tg[7] = zkL;
+        L.il[idx] = iL | (lL << 16);
+    } else {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            double cx = -inc[q].x * sc, cy = -inc[q].y * sc;
-            if (SQ) { cx *= scale; cy *= scale; }
-            inc[q] = mk(cx, cy);
-        }
+        for (int q = 0; q < 8; ++q) { tg[q] = -1; inc[q] = mk(0.0, 0.0); }
+        L.il[idx] = -1;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        double cx = -inc[q].x * sc, cy = -inc[q].y * sc;
+        if (SQ) { cx *= scale; cy *= scale; }
+        inc[q] = mk(cx, cy);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        L.c[4 * idx + q] = make_double2(inc[q].x, inc[q].y);
+        L.t[4 * idx + q] = tg[q];
+    }
+    // energy of the record: its h terms (the polygon sums them with its interior
+    // terms and halves), or with SQ its share scale*A/(2 lensN) of the lens's part of
+    // the energy (A^2/2, or with PAIR A_pair*A/2), doubled so that the same halving
+    // gives it exactly
+    L.e[idx] = SQ ? 2.0 * ((scale * lensA) / (2.0 * lensN)) : e;
+    L.la[idx] = scale;
+    L.sp[idx] = make_double2(sp.x, sp.y);
+    // sj side: at this record's rank among the records whose sj is sjg (key order)
+    long long bj = (long long)sjg * Kp;
+    int cj = min(jcnt[sjg], Kp);
+    int r2 = count_keys_below(jKey + bj, cj, [&](uint64_t k2) { return k2 < inter; });
+    if (r2 < Kp) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            L.c[4 * f + q] = make_double2(inc[q].x, inc[q].y);
-            L.t[4 * f + q] = tg[q];
-        }
-        // energy of the record: its h terms (the polygon sums them with its interior
-        // terms and halves), or with SQ its share scale*A/(2 lensN) of the lens's part of
-        // the energy (A^2/2, or with PAIR A_pair*A/2), doubled so that the same halving
-        // gives it exactly
-        L.e[f] = SQ ? 2.0 * ((scale * lensA) / (2.0 * lensN)) : e;
-        L.la[f] = scale;
-        L.sp[f] = make_double2(sp.x, sp.y);
-        // sj side: at this record's rank among the records whose sj is sjg (key order)
-        long long bj = (long long)sjg * Kp;
-        int cj = min(jcnt[sjg], Kp);
-        int r2 = count_keys_below(jKey + bj, cj, [&](uint64_t k2) { return k2 < inter; });
-        if (r2 < Kp) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                jC[(bj + r2) * 4 + q] = make_double2(inc[4 + q].x, inc[4 + q].y);
-                jT[(bj + r2) * 4 + q] = tg[4 + q];
-            }
+            jC[(bj + r2) * 4 + q] = make_double2(inc[4 + q].x, inc[4 + q].y);
+            jT[(bj + r2) * 4 + q] = tg[4 + q];
         }
     }
+}
+
+// The vertex part of k_exterior_poly for polygon s (after its records are staged in L, in key
+// order): per vertex the contributions acting on it in record order and the interior chain
+// terms; the polygon's record and interior energy into accR[s].
+template <int MODE, int SEGX>
+__device__ __forceinline__ void exterior_vertices(int lane, int segBase, bool has, int s, int cnt, int st1, int n1,
+                                                  const double2* __restrict__ pos, const ExtLds& L, double* leBuf,
+                                                  double2* __restrict__ F1, double2* __restrict__ LMo,
+                                                  double2* __restrict__ LNo, double* __restrict__ accR) {
+    constexpr int AREA = (MODE >= 1), SQ = (MODE >= 2);
     __syncthreads();
     // polygon energy, first part: record energies in record order
     double acc = 0.0;
@@ -597,6 +613,94 @@ __global__ __launch_bounds__(64) void k_exterior_poly(
         __syncthreads();
     }
     if (has && lane == 0) accR[s] = acc;
+}
+
+// MODE 0: reference energy; 1: area energy; 2: squared lens areas; 3: squared pair areas.
+// One wavefront holds 64 / SEGX polygons, SEGX lanes each: lanes take the polygon's
+// records, then its vertices, SEGX at a time (a polygon has a few records, so a
+// wavefront per polygon would leave most lanes idle).  Every sum keeps its order.
+// GLOB: the per-polygon record staging lives in global memory (gstage, ext_lds_stride(Kp)
+// bytes per polygon slot of the launch) instead of LDS, for buckets too large for LDS; the
+// workgroup barriers order it as they order LDS (one wavefront per workgroup, one CU).
+template <int MODE, int SEGX, bool GLOB = false>
+__global__ __launch_bounds__(64) void k_exterior_poly(
+    int N, int Kp, const int* __restrict__ polyStart, const int* __restrict__ polyPack,
+    const int* __restrict__ packPoly, const double2* __restrict__ pos, const int* __restrict__ bcnt,
+    const int* __restrict__ jcnt, const uint64_t* __restrict__ jKey, const uint64_t* __restrict__ rKey,
+    const uint64_t* __restrict__ rOuter, const float2* __restrict__ rTU, const float2* __restrict__ rUT,
+    double2* __restrict__ jC, int* __restrict__ jT, double2* __restrict__ F1, double2* __restrict__ LMo,
+    double2* __restrict__ LNo, double* __restrict__ accR, const double* __restrict__ Er,
+    const double* __restrict__ laRep, const int* __restrict__ order, char* __restrict__ gstage) {
+    constexpr int G = 64 / SEGX;
+    extern __shared__ char smem[];
+    __shared__ double leBuf[64];
+    int lane = threadIdx.x & (SEGX - 1), g = threadIdx.x / SEGX, segBase = g * SEGX;
+    ExtLds L = GLOB ? ext_lds(gstage + ((size_t)blockIdx.x * G + g) * ext_lds_stride(Kp), Kp)
+                    : ext_lds(smem + g * ext_lds_stride(Kp), Kp);
+    int slot = xcd_block(blockIdx.x, gridDim.x) * G + g;
+    bool has = slot < N;
+    int s = has ? order[slot] : 0;
+    int cnt = has ? min(bcnt[s], Kp) : 0;
+    int pk = polyPack[s];
+    int pbase = packPoly[pk];
+    int st1 = polyStart[s], n1 = has ? polyStart[s + 1] - st1 : 0;
+    long long base = (long long)s * Kp;
+    for (int f = lane; f < cnt; f += SEGX)
+        exterior_record<MODE>(rKey[base + f], rOuter[base + f], rTU[base + f], rUT[base + f], f, base + f, s, pbase,
+                              st1, n1, Kp, polyStart, pos, bcnt, jcnt, jKey, rKey, rOuter, jC, jT, Er, laRep, L);
+    exterior_vertices<MODE, SEGX>(lane, segBase, has, s, cnt, st1, n1, pos, L, leBuf, F1, LMo, LNo, accR);
+}
+
+// Pairing and exterior forces in one launch for the reference energy (MODE 0), where the
+// exterior terms of polygon s need only the exits of its own records: the wavefront that pairs
+// the records of s (pair_record, as k_pair_poly) stages their contributions at their ranks
+// (exterior_record, as k_exterior_poly<0>) and goes on to the vertex part, with no kernel
+// boundary and no round trip through the record rows (which are written all the same, for the
+// getters).  The results are those of the two kernels bit for bit.
+template <int SEGX>
+__global__ __launch_bounds__(64) void k_pair_exterior(
+    int N, int Kp, const int* __restrict__ polyStart, const int* __restrict__ polyPack,
+    const int* __restrict__ packPoly, const double2* __restrict__ pos, const int* __restrict__ bcnt,
+    const uint64_t* __restrict__ bKey, const float2* __restrict__ bTU,
+    const unsigned long long* __restrict__ packMaxKey, long long pmkPitch, const int* __restrict__ jcnt,
+    const uint64_t* __restrict__ jKey, uint64_t* __restrict__ rKey, float2* __restrict__ rTU,
+    uint64_t* __restrict__ rOuter, float2* __restrict__ rUT, double2* __restrict__ jC, int* __restrict__ jT,
+    double2* __restrict__ F1, double2* __restrict__ LMo, double2* __restrict__ LNo, double* __restrict__ accR,
+    const int* __restrict__ order, int* __restrict__ scal) {
+    constexpr int G = 64 / SEGX;
+    extern __shared__ char smem[];
+    __shared__ double leBuf[64];
+    int lane = threadIdx.x & (SEGX - 1), g = threadIdx.x / SEGX, segBase = g * SEGX;
+    ExtLds L = ext_lds(smem + g * ext_lds_stride(Kp), Kp);
+    int slot = xcd_block(blockIdx.x, gridDim.x) * G + g;
+    bool has = slot < N;
+    int s = has ? order[slot] : 0;
+    int cnt = has ? min(bcnt[s], Kp) : 0;
+    int pk = polyPack[s];
+    int pbase = packPoly[pk];
+    int st1 = polyStart[s], n1 = has ? polyStart[s + 1] - st1 : 0;
+    long long base = (long long)s * Kp;
+    uint64_t mxk = 0;   // the maximum of the packing's partial maxima, across the segment
+    for (int k = lane; k < PMK_SLOTS; k += SEGX) {
+        uint64_t v = packMaxKey[k * pmkPitch + pk];
+        mxk = (v > mxk) ? v : mxk;
+    }
+#pragma unroll
+    for (int o = SEGX / 2; o >= 1; o >>= 1) {
+        uint64_t y = __shfl_xor(mxk, o, SEGX);
+        mxk = (y > mxk) ? y : mxk;
+    }
+    for (int f = lane; f < cnt; f += SEGX) {
+        PairRec r = pair_record<0>(f, cnt, base, s, pbase, st1, n1, mxk, Kp, polyStart, bcnt, bKey, bTU, nullptr,
+                                   scal);
+        rKey[base + r.rank] = r.key;
+        rTU[base + r.rank] = r.tu;
+        rOuter[base + r.rank] = r.outer;
+        rUT[base + r.rank] = r.ut;
+        exterior_record<0>(r.key, r.outer, r.tu, r.ut, r.rank, base + r.rank, s, pbase, st1, n1, Kp, polyStart, pos,
+                           bcnt, jcnt, jKey, rKey, rOuter, jC, jT, nullptr, nullptr, L);
+    }
+    exterior_vertices<0, SEGX>(lane, segBase, has, s, cnt, st1, n1, pos, L, leBuf, F1, LMo, LNo, accR);
 }
 
 // ---------------------------------------------------------------- per-vertex gather
@@ -810,9 +914,16 @@ static int launch_pair(pcp_ctx* c, const double2* pairPos) {
     return prof_end(c, slot);
 }
 
-int run_records(pcp_ctx* c, const double2* pairPos) {
+// deferOK: the caller evaluates the forces next (run_forces), so the reference pairing may be
+// left to the fused launch k_pair_exterior (when a polygon's records fit the LDS stage)
+int run_records(pcp_ctx* c, const double2* pairPos, bool deferOK) {
+    c->pairDeferred = false;
     if (c->N == 0) return PCP_OK;
     if (pairPos) return launch_pair<1>(c, pairPos);
+    if (deferOK && ext_lds_stride(c->Kp) <= EXT_LDS_MAX) {
+        c->pairDeferred = true;
+        return PCP_OK;
+    }
     return launch_pair<0>(c, pairPos);
 }
 
@@ -894,6 +1005,31 @@ static int launch_exterior_m(pcp_ctx* c, const double2* pos) {
     else if (w == 16) PCP_TRY((launch_exterior_w<MODE, 16>(c, pos)));
     else if (w == 32) PCP_TRY((launch_exterior_w<MODE, 32>(c, pos)));
     else PCP_TRY((launch_exterior_w<MODE, 64>(c, pos)));
+    return prof_end(c, slot);
+}
+template <int SEGX>
+static int launch_pair_exterior_w(pcp_ctx* c, const double2* pos) {
+    constexpr int G = 64 / SEGX;
+    size_t lds = G * ext_lds_stride(c->Kp);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_exterior<SEGX>), dim3((c->N + G - 1) / G), dim3(64), lds, c->stream,
+                       c->N, c->Kp, c->dPolyStart, c->dPolyPack, c->dPackPoly, pos, c->dBcnt, c->dBKey, c->dBTU,
+                       c->dPackMaxKey, c->pmkPitch, c->dJcnt, c->dJKey, c->dRKey, c->dRTU, c->dROuter, c->dRUT,
+                       c->dJC, c->dJT, c->dF1, c->dLM, c->dLN, c->dAccR, c->dPcSorted, c->dScal);
+    PCP_LAUNCHED();
+    return PCP_OK;
+}
+// k_pair_exterior with the exterior kernel's lanes per polygon (the records fit the LDS stage)
+static int launch_pair_exterior(pcp_ctx* c, const double2* pos) {
+    int slot = -1;
+    PCP_TRY(prof_begin(c, "k_pair_exterior", &slot));
+    size_t st = ext_lds_stride(c->Kp);
+    int w = c->segExt;
+    if (w == 16 && 4 * st > EXT_LDS_MAX) w = 0;
+    if (w == 32 && 2 * st > EXT_LDS_MAX) w = 0;
+    if (w == 0) w = (4 * st <= EXT_LDS_MAX) ? 16 : ((2 * st <= EXT_LDS_MAX) ? 32 : 64);
+    if (w == 16) PCP_TRY(launch_pair_exterior_w<16>(c, pos));
+    else if (w == 32) PCP_TRY(launch_pair_exterior_w<32>(c, pos));
+    else PCP_TRY(launch_pair_exterior_w<64>(c, pos));
     return prof_end(c, slot);
 }
 static int launch_exterior(pcp_ctx* c, int mode, const double2* pos) {
@@ -1038,9 +1174,21 @@ int run_forces(pcp_ctx* c, int model, const double2* pos, double* edgeLengths, d
         return PCP_OK;
     }
     if (overlap) {
+        int mode = pair ? 3 : (sq ? 2 : (area ? 1 : 0));
+        if (c->pairDeferred) {   // the reference pairing left to this launch (run_records)
+            c->pairDeferred = false;
+            if (mode == 0) {
+                PCP_TRY(launch_pair_exterior(c, pos));
+                mode = -1;
+            } else {
+                PCP_TRY(launch_pair<0>(c, nullptr));
+            }
+        }
         if (sq) PCP_TRY(launch_lens(c, pos));
         if (pair) PCP_TRY(launch_lens_rep(c));
-        PCP_TRY(launch_exterior(c, pair ? 3 : (sq ? 2 : (area ? 1 : 0)), pos));
+        if (mode >= 0) PCP_TRY(launch_exterior(c, mode, pos));
+    } else {
+        c->pairDeferred = false;
     }
     bool proj = project && c->maxN <= 64;
     if (geom && proj)
