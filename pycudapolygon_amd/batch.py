@@ -36,14 +36,16 @@ def _with_positions(p, x):
 
 
 def minimize_batch(packings, model, fire_kwargs, maxSteps, dt0=1e-3, compact_below=0.5, min_context=8,
-                   boxSizes=None, log=None, warm=None, block=8, graph=True, autotune_reps=0):
+                   boxSizes=None, log=None, warm=None, block=8, graph=True, autotune_reps=0, small_context=16,
+                   small_block=32):
     """FIRE on `packings` until each has max|F| <= fTol or maxSteps steps of its own.
     compact_below: re-pack when fewer than this fraction of the context's packings are
     still running (and the context holds more than min_context packings).
     warm: a warmstart.WarmStart applied once to the first context before FIRE starts
     (its time is included in the result's seconds).  FIRE runs in blocks of `block` steps
     (pcp_fire_run, replayed from a HIP graph when graph is set); the batch is checked for
-    convergence and compaction between blocks.  autotune_reps > 0: the lanes-per-polygon
+    convergence and compaction between blocks (blocks of small_block steps once a context holds
+    at most small_context packings).  autotune_reps > 0: the lanes-per-polygon
     schedule of the per-record kernels is picked by pcp_autotune on the first context
     (timed over that many launches per candidate; its time is included) and kept for the
     compacted ones.  Every schedule gives the same results bit for bit."""
@@ -77,11 +79,14 @@ def minimize_batch(packings, model, fire_kwargs, maxSteps, dt0=1e-3, compact_bel
         done = np.zeros(len(idx), bool)
         local = 0
         start = int(offset[idx].max())    # every running packing has run the same number of steps
+        # a small context's steps are short (a few microseconds per kernel), so it runs longer
+        # blocks per status read: the read and the block-start snapshot cost as much as a step
+        kb = block if len(idx) > small_context else small_block
         while True:
-            K = min(block, maxSteps - start - local)
+            K = min(kb, maxSteps - start - local)
             if K <= 0:
                 break
-            st = s.fire_run(prm, K, K, graph and K == block)
+            st = s.fire_run(prm, K, K, graph and K == kb)
             local += K
             for k in range(len(idx)):
                 if not done[k] and st[k].converged:

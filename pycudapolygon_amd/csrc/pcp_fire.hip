@@ -424,6 +424,150 @@ __global__ __launch_bounds__(BLK) void k_fire_stats(int N, const int* __restrict
     }
 }
 
+// One step's FIRE bookkeeping for a context of a few small packings, one workgroup per packing:
+// the energy test (k_fire_energy), the second half-kick with the per-polygon sums
+// (a thread per polygon), the packing's sums, the dt / alpha update and
+// the convergence test (k_fire_stats, full) and the bend or rollback of every vertex
+// (k_fire_bend), with workgroup barriers where the four launches had kernel boundaries.  The same
+// functions in the same order, so the results are those of the four launches bit for bit; a
+// small context's step is bound by launch and latency, not by the work of one packing.
+constexpr int FIRE_SMALL_B = 8;
+template <bool NOKICK>
+__global__ __launch_bounds__(BLK) void k_fire_small(
+    int N, const int* __restrict__ packPoly, const int* __restrict__ packVert, const int* __restrict__ polyStart,
+    const double* __restrict__ epoly, double* __restrict__ energy, int* __restrict__ active,
+    const double* __restrict__ epre, const int* __restrict__ xpbdIt, int nMin, double dtMax, double alphaStart,
+    double fAlpha, double fInc, double fDec, double fTol, double* dt, double* alpha, int* nPos,
+    int* __restrict__ flags, int* __restrict__ steps, double* __restrict__ fs, pcp_fire_status* __restrict__ status,
+    int* __restrict__ summary, const int* __restrict__ scal, int rollbackOnDevice, int* __restrict__ blockOver,
+    double* __restrict__ red, double2* __restrict__ x, double2* __restrict__ v, double2* __restrict__ F,
+    const double2* __restrict__ xbak, const double2* __restrict__ fbak, double2* __restrict__ Fe,
+    const double2* __restrict__ febak) {
+    __shared__ double part[3 * (BLK + 1)];
+    __shared__ double L[BLK / 64];
+    int p = blockIdx.x;
+    int a0 = packPoly[p], n = packPoly[p + 1] - a0;
+    // energy = fixed-order sum of the polygon energies, then its test
+    double en = det_sum_block(epoly + a0, n, part);
+    if (threadIdx.x == 0) {
+        energy[p] = en;
+        if (xpbdIt && active[p]) flags[FL_NF * p + FL_LASTSHAKE] = xpbdIt[p];   // XPBD iterations
+        fire_control(p, active, energy, epre, alphaStart, fDec, dt, alpha, nPos, flags);
+    }
+    __syncthreads();
+    // second half-kick and per-polygon sums in vertex order (as k_fire_kick / k_fire_kick_seg), a
+    // thread per polygon, its vertices loaded FIRE_SMALL_B at a time with the loads in flight together
+    bool kickP = active[p] && flags[FL_NF * p + FL_ACCEPT];
+    double h = dt[p];
+    for (int s = a0 + threadIdx.x; s < a0 + n; s += BLK) {
+        int st = polyStart[s], nv = polyStart[s + 1] - st;
+        double a = 0.0, b = 0.0, c = 0.0, mf = 0.0;
+        for (int m0 = 0; m0 < nv; m0 += FIRE_SMALL_B) {
+            double2 fq[FIRE_SMALL_B], vq[FIRE_SMALL_B];
+#pragma unroll
+            for (int k = 0; k < FIRE_SMALL_B; ++k) {
+                bool in = m0 + k < nv;
+                fq[k] = in ? F[st + m0 + k] : make_double2(0.0, 0.0);
+                vq[k] = (in && kickP) ? v[st + m0 + k] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int k = 0; k < FIRE_SMALL_B; ++k) {
+                if (m0 + k >= nv) break;
+                double2 fi = fq[k];
+                mf = fmax(mf, fmax(fabs(fi.x), fabs(fi.y)));
+                if (kickP) {
+                    double vx = vq[k].x, vy = vq[k].y;
+                    if (!NOKICK) {
+                        vx = vq[k].x + (0.5 * h) * fi.x;
+                        vy = vq[k].y + (0.5 * h) * fi.y;
+                        v[st + m0 + k] = make_double2(vx, vy);
+                    }
+                    a += fi.x * vx;
+                    a += fi.y * vy;
+                    b += vx * vx;
+                    b += vy * vy;
+                    c += fi.x * fi.x;
+                    c += fi.y * fi.y;
+                }
+            }
+        }
+        red[s] = a;
+        red[N + s] = b;
+        red[2LL * N + s] = c;
+        red[3LL * N + s] = mf;
+    }
+    __syncthreads();
+    // the packing's sums of F.v, v.v, F.F and max |F|, dt / alpha, convergence, as k_fire_stats
+    double r[4];
+    det_reduce_block<3, true>(red + a0, N, n, part, L, r);
+    if (threadIdx.x == 0) {
+        fire_control2(p, active, r[0], r[1], r[2], nMin, dtMax, alphaStart, fAlpha, fInc, fDec, dt, alpha, nPos,
+                      flags, fs);
+        double mfp = r[3];
+        if (rollbackOnDevice && active[p] && flags[FL_NF * p + FL_ROLLBACK]) {
+            energy[p] = epre[p];
+            mfp = fs[FS_NF * p + FS_MAXFB];
+        }
+        fire_finish_p(p, active, flags, mfp, fTol, steps, dt, alpha, nPos, energy, fs, status, summary, 1, 1);
+        if (p == 0) {
+            summary[2] = scal[SC_BUCKET];
+            if (blockOver && scal[SC_BUCKET] > blockOver[0]) blockOver[0] = scal[SC_BUCKET];
+        }
+    }
+    __syncthreads();
+    // bend or rollback of every vertex of the packing, as k_fire_bend (FIRE_SMALL_B vertices per
+    // thread at a time, their loads in flight together)
+    const int* fl = flags + FL_NF * p;
+    if (fl[FL_ACTIVEBAK]) {
+        bool accept = fl[FL_ACCEPT], zero = fl[FL_ZEROV], roll = fl[FL_ROLLBACK];
+        double sc = fs[FS_NF * p + FS_SCALE], om = fs[FS_NF * p + FS_ONEMA];
+        int v0 = packVert[p], v1 = packVert[p + 1];
+        for (int i0 = v0 + threadIdx.x; i0 < v1; i0 += BLK * FIRE_SMALL_B) {
+            if (accept && !zero) {
+                double2 vq[FIRE_SMALL_B], fq[FIRE_SMALL_B];
+#pragma unroll
+                for (int k = 0; k < FIRE_SMALL_B; ++k) {
+                    int i = i0 + k * BLK;
+                    vq[k] = (i < v1) ? v[i] : make_double2(0.0, 0.0);
+                    fq[k] = (i < v1) ? F[i] : make_double2(0.0, 0.0);
+                }
+#pragma unroll
+                for (int k = 0; k < FIRE_SMALL_B; ++k) {
+                    int i = i0 + k * BLK;
+                    if (i < v1)
+                        v[i] = make_double2(om * vq[k].x + sc * fq[k].x, om * vq[k].y + sc * fq[k].y);
+                }
+            } else if (accept) {
+#pragma unroll
+                for (int k = 0; k < FIRE_SMALL_B; ++k) {
+                    int i = i0 + k * BLK;
+                    if (i < v1) v[i] = make_double2(0.0, 0.0);
+                }
+            } else if (roll) {
+                double2 xq[FIRE_SMALL_B], fq[FIRE_SMALL_B], eq[FIRE_SMALL_B];
+#pragma unroll
+                for (int k = 0; k < FIRE_SMALL_B; ++k) {
+                    int i = i0 + k * BLK;
+                    bool in = i < v1;
+                    xq[k] = in ? xbak[i] : make_double2(0.0, 0.0);
+                    fq[k] = (in && fbak) ? fbak[i] : make_double2(0.0, 0.0);
+                    eq[k] = (in && febak) ? febak[i] : make_double2(0.0, 0.0);
+                }
+#pragma unroll
+                for (int k = 0; k < FIRE_SMALL_B; ++k) {
+                    int i = i0 + k * BLK;
+                    if (i < v1) {
+                        x[i] = xq[k];
+                        v[i] = make_double2(0.0, 0.0);
+                        if (fbak) F[i] = fq[k];
+                        if (febak) Fe[i] = eq[k];
+                    }
+                }
+            }
+        }
+    }
+}
+
 // undo an attempt whose buffers overflowed: x, v, FIRE scalars, active flags, steps
 __global__ void k_fire_undo(int V, int P, const int* __restrict__ shapeId, const int* __restrict__ polyPack,
                             int* __restrict__ active, const int* __restrict__ flags, double2* __restrict__ x,
@@ -474,7 +618,30 @@ static int launch_fire_kick(pcp_ctx* c, double2* vel, const double2* force) {
     return PCP_OK;
 }
 
-static int fire_eval(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm) {
+// a context of at most FIRE_SMALL_P packings of at most FIRE_SMALL_V vertices each, polygons of
+// at most 64 vertices: its FIRE bookkeeping runs as one launch (k_fire_small)
+constexpr int FIRE_SMALL_P = 16;
+constexpr int FIRE_SMALL_V = 32768;
+static bool fire_small(const pcp_ctx* c) {
+    if (c->P > FIRE_SMALL_P || c->maxN > 64) return false;
+    for (int p = 0; p < c->P; ++p)
+        if (c->hPackVert[p + 1] - c->hPackVert[p] > FIRE_SMALL_V) return false;
+    return true;
+}
+template <bool NOKICK>
+static int launch_fire_small(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm, int block,
+                             const double2* fbak, const double2* febak) {
+PCP_LAUNCH(c, (k_fire_small<NOKICK>), dim3(c->P), dim3(BLK), c->N, c->dPackPoly, c->dPackVert, c->dPolyStart,
+               c->dEpoly, st->energy, c->dActive, c->dEpre, NOKICK ? (const int*)c->dShakeIt : (const int*)nullptr,
+               prm->nMin, prm->dtMax, prm->alphaStart, prm->fAlpha, prm->fInc, prm->fDec, prm->fTol, c->dDt,
+               c->dAlpha, c->dNPos, c->dFlags, c->dSteps, c->dFireScal, c->dStatus, c->dSummary, c->dScal, block,
+               block ? c->dBlockOver : nullptr, c->dRed, (double2*)st->pos, (double2*)st->vel,
+               (double2*)st->force, (const double2*)c->dXbak, fbak, (double2*)c->dFext, febak);
+    return PCP_OK;
+}
+
+// energyTest: end with the energy test (k_fire_energy); a small context does it in k_fire_small
+static int fire_eval(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm, bool energyTest = true) {
     double2* x = (double2*)st->pos;
     double2* v = (double2*)st->vel;
     int mt = st->model & 0xFF;
@@ -523,6 +690,7 @@ static int fire_eval(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm
     PCP_TRY(run_forces(c, st->model, x, st->edgeLengths, st->areas, st->targetEdgeV, st->targetAreas,
                        st->stiffness, st->compressibility, (double2*)st->force, nullptr, 1, rigid, false,
                        st->perimStiffness));
+    if (!energyTest) return PCP_OK;
     PCP_LAUNCH(c, k_fire_energy, dim3(c->P), dim3(BLK), c->N, c->dPackPoly, c->dEpoly, st->energy, c->dActive,
                c->dEpre, prm->alphaStart, prm->fDec, c->dDt, c->dAlpha, c->dNPos, c->dFlags,
                xpbd ? (const int*)c->dShakeIt : (const int*)nullptr);
@@ -558,6 +726,11 @@ static int fire_step_launch(pcp_ctx* c, const pcp_state* st, const pcp_fire_para
                    c->dEpre, c->dFlags, c->dFireScal, c->dShakeIt, c->dSummary, (double2*)st->pos,
                    (double2*)st->vel, (const double2*)st->force, (double2*)c->dXbak, (double2*)c->dVbak,
                    (double2*)c->dXpre, fbak, (const double2*)nullptr, (double2*)nullptr);
+    if (fire_small(c)) {   // energy test, kick, stats and bend in one launch
+        PCP_TRY(fire_eval(c, st, prm, false));
+        if (xpbd) return launch_fire_small<true>(c, st, prm, block, fbak, febak);
+        return launch_fire_small<false>(c, st, prm, block, fbak, febak);
+    }
     PCP_TRY(fire_eval(c, st, prm));
     if (xpbd) PCP_TRY(launch_fire_kick<true>(c, (double2*)st->vel, (const double2*)st->force));
     else PCP_TRY(launch_fire_kick<false>(c, (double2*)st->vel, (const double2*)st->force));
