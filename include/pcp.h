@@ -190,6 +190,13 @@ int pcp_update_positions(pcp_ctx* ctx, double* pos, const double* force, double 
 /* resetAreasCUDA model.cu:639-645 */
 int pcp_reset_areas(pcp_ctx* ctx, double* pos, const double* areas, const double* targetAreas,
                     const double* comX, const double* comY);
+/* No counterpart in the reference, whose setPhi (pyCudaPolygon.py:223-234) scales the target
+ * arrays on the host: targetAreas (device[N]) of packing p multiplied by areaFactor[p] and
+ * targetEdgeV (device[V]) by edgeFactor[p] (host[P] each; setPhi's r and sqrt(r)), on the
+ * device, so that a compression step moves no per-vertex array over PCIe.  Either target
+ * array may be NULL (then its factors are not read). */
+int pcp_scale_targets(pcp_ctx* ctx, const double* areaFactor, const double* edgeFactor, double* targetAreas,
+                      double* targetEdgeV);
 /* Shape matching, the "batched 2x2 SVD shape projection" of the compression protocol (no
  * counterpart in the reference, whose projection acts on forces only): every polygon is
  * replaced by its rest shape refShape (device[2V]: each vertex's offset from the rest

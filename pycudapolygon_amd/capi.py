@@ -72,6 +72,7 @@ SIGNATURES = {
     "pcp_update_positions": (_i, [_p, _p, _p, _d]),
     "pcp_reset_areas": (_i, [_p, _p, _p, _p, _p, _p]),
     "pcp_shape_project": (_i, [_p, _p, _p, _p]),
+    "pcp_scale_targets": (_i, [_p, _p, _p, _p, _p]),
     "pcp_max_effective_force": (_i, [_p, _p, _p, _p, _d, _p]),
     "pcp_overlap_count": (_i, [_p, _p, _i, _p, _p]),
     "pcp_overlap_count_noalloc": (_i, [_p, _p, _i, _p, _p]),

@@ -45,17 +45,9 @@ def set_phi(s, phi, ref=None):
     of rest_shapes), put its rest shape at the new target area in place (pcp_shape_project)."""
     s.sync()
     phi = np.broadcast_to(np.asarray(phi, np.float64), (s.P,))
-    ta = s.ta.get()
-    tev = s.tev.get()
-    cur = packing_fractions(s)
-    for p in range(s.P):
-        r = phi[p] / cur[p]
-        a, b = int(s.packPoly[p]), int(s.packPoly[p + 1])
-        ta[a:b] *= r
-        va, vb = s.vertex_range(p)
-        tev[va:vb] *= np.sqrt(r)
-    s.ta.upload(ta)
-    s.tev.upload(tev)
+    r = np.ascontiguousarray(phi / packing_fractions(s))
+    # areas times r and edge targets times sqrt(r), per packing, on the device
+    s.ctx.call("pcp_scale_targets", capi._ptr(r), capi._ptr(np.sqrt(r)), s.ta.ptr, s.tev.ptr)
     if ref is not None:
         s.ctx.call("pcp_shape_project", s.pos.ptr, ref.ptr, s.ta.ptr)
     else:

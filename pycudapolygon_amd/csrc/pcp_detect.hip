@@ -186,11 +186,15 @@ __device__ __forceinline__ bool circles_meet(double4 a, double4 b) {
 // reads a candidate's circle from bounds[id], in flight together with its vertex range).
 // The bounds of up to PC_REG * PP_BLK polygons are read once, all loads in flight
 // together, and kept in registers for the three passes; beyond that they are re-read.
+// active (FIRE steps that skip converged packings; else NULL): the ids of the running packings'
+// polygons also go to actOrder, packing after packing in cell order, and their number to
+// *actCount; a converged packing's workgroup leaves its cell lists as they are.
 constexpr int PC_REG = 8;
 __global__ __launch_bounds__(PP_BLK) void k_poly_cells(const int* __restrict__ packPoly,
                                                       const double4* __restrict__ bounds,
                                                       int* __restrict__ pcStart, int* __restrict__ pcSorted,
-                                                      int* __restrict__ packB) {
+                                                      int* __restrict__ packB, const int* __restrict__ active,
+                                                      int* __restrict__ actOrder, int* __restrict__ actCount) {
     __shared__ int cStart[PC_CELLS + 1];
     __shared__ int cCur[PC_CELLS];
     __shared__ double redR[PP_BLK / 64];
@@ -199,6 +203,25 @@ __global__ __launch_bounds__(PP_BLK) void k_poly_cells(const int* __restrict__ p
     int p = blockIdx.x, t = threadIdx.x, lane = t & 63, w = t >> 6;
     int s0 = packPoly[p], s1 = packPoly[p + 1], Np = s1 - s0;
     long long gbase = (long long)p * (PC_CELLS + 1);
+    int off = 0;   // the packing's first place in actOrder
+    if (active) {
+        int below = 0, all = 0;
+        for (int q = t; q < (int)gridDim.x; q += PP_BLK) {   // one workgroup per packing
+            int sz = active[q] ? packPoly[q + 1] - packPoly[q] : 0;
+            all += sz;
+            below += (q < p) ? sz : 0;
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) { below += __shfl_xor(below, o); all += __shfl_xor(all, o); }
+        if (lane == 0) { scanTmp[w] = below; redBig[w] = all; }
+        __syncthreads();
+        below = 0; all = 0;
+        for (int q = 0; q < PP_BLK / 64; ++q) { below += scanTmp[q]; all += redBig[q]; }
+        __syncthreads();   // scanTmp and redBig are reused below
+        if (p == 0 && t == 0) *actCount = all;
+        if (!active[p]) return;   // the whole workgroup
+        off = below;
+    }
     double4 bc[PC_REG];
 #pragma unroll
     for (int k = 0; k < PC_REG; ++k) {
@@ -284,19 +307,30 @@ __global__ __launch_bounds__(PP_BLK) void k_poly_cells(const int* __restrict__ p
         int s = s0 + t + k * PP_BLK;
         if (s < s1) {
             int slot = atomicAdd(&cCur[cellOf[k]], 1);
-            if (inLds) cStart[slot] = s;
-            else pcSorted[s0 + slot] = s;
+            if (inLds) {
+                cStart[slot] = s;
+            } else {
+                pcSorted[s0 + slot] = s;
+                if (active) actOrder[off + slot] = s;
+            }
         }
     }
     for (int s = s0 + t + PC_REG * PP_BLK; s < s1; s += PP_BLK) {
         double4 b = bounds[s];
         int slot = atomicAdd(&cCur[pc_coord(b.y, B) * B + pc_coord(b.x, B)], 1);
-        if (inLds) cStart[slot] = s;
-        else pcSorted[s0 + slot] = s;
+        if (inLds) {
+            cStart[slot] = s;
+        } else {
+            pcSorted[s0 + slot] = s;
+            if (active) actOrder[off + slot] = s;
+        }
     }
     if (inLds) {
         __syncthreads();
-        for (int i = t; i < Np; i += PP_BLK) pcSorted[s0 + i] = cStart[i];
+        for (int i = t; i < Np; i += PP_BLK) {
+            pcSorted[s0 + i] = cStart[i];
+            if (active) actOrder[off + i] = cStart[i];
+        }
     }
 }
 
@@ -1083,14 +1117,15 @@ __global__ __launch_bounds__(BLK) void k_detect_tile(
     const int* __restrict__ pcSorted, const int* __restrict__ packB, int Kp,
     int* __restrict__ bcnt, uint64_t* __restrict__ bKey, float2* __restrict__ bTU, int* __restrict__ jcnt,
     uint64_t* __restrict__ jKey, unsigned long long* __restrict__ packMaxKey, long long pmkPitch,
-    int* __restrict__ num, int* __restrict__ scal) {
+    int* __restrict__ num, int* __restrict__ scal, const int* __restrict__ work, const int* __restrict__ nWork) {
     __shared__ TileLds lds[BLK / 64];
     int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    int idx = xcd_block(blockIdx.x, gridDim.x) * (BLK / 64) + w;
-    if (idx >= N) return;   // whole wavefront
+    int nw, wb = work_block(nWork, BLK / 64, nw, N);
+    int idx = wb * (BLK / 64) + w;
+    if (wb < 0 || idx >= nw) return;   // whole wavefront
     TileLds& L = lds[w];
     DetOut out{Kp, bcnt, bKey, bTU, jcnt, jKey, scal};
-    int A = __builtin_amdgcn_readfirstlane(pcSorted[idx]);
+    int A = __builtin_amdgcn_readfirstlane(work[idx]);
     int stA = polyStart[A], nA = polyStart[A + 1] - stA;
     int pk = polyPack[A];
     int Bu = box[pk];
@@ -1206,13 +1241,14 @@ __global__ __launch_bounds__(BLK) void k_detect_poly(
     int* __restrict__ bcnt, uint64_t* __restrict__ bKey, float2* __restrict__ bTU, int* __restrict__ jcnt,
     uint64_t* __restrict__ jKey, unsigned long long* __restrict__ packMaxKey, long long pmkPitch,
     int* __restrict__ num,
-    int* __restrict__ scal) {
+    int* __restrict__ scal, const int* __restrict__ work, const int* __restrict__ nWork) {
     __shared__ DetLds lds[BLK / 64];
     int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    int idx = xcd_block(blockIdx.x, gridDim.x) * (BLK / 64) + w;
-    if (idx >= N) return;   // whole wavefront
+    int nw, wb = work_block(nWork, BLK / 64, nw, N);
+    int idx = wb * (BLK / 64) + w;
+    if (wb < 0 || idx >= nw) return;   // whole wavefront
     DetOut out{Kp, bcnt, bKey, bTU, jcnt, jKey, scal};
-    detect_polygon<CELLS, CONS>(__builtin_amdgcn_readfirstlane(pcSorted[idx]), lane, lds[w], polyStart, polyPack, packPoly,
+    detect_polygon<CELLS, CONS>(__builtin_amdgcn_readfirstlane(work[idx]), lane, lds[w], polyStart, polyPack, packPoly,
                           box, pos, cellOf, bounds, pcStart, pcSorted, packB, out, packMaxKey, pmkPitch,
                           num);
 }
@@ -1274,12 +1310,12 @@ static int launch_tile_t(pcp_ctx* c, const double2* pos, int grid) {
                               c->profEvents[slot].second, 0, c->N, c->dPolyStart, c->dPolyPack, c->dPackPoly,
                               c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted, c->dPackB,
                               c->Kp, c->dBcnt, c->dBKey, c->dBTU, c->dJcnt, c->dJKey, c->dPackMaxKey, c->pmkPitch,
-                              c->dNum, c->dScal);
+                              c->dNum, c->dScal, work_list(c), work_count(c));
     else
         hipLaunchKernelGGL(kern, dim3(grid), dim3(BLK), 0, c->stream, c->N, c->dPolyStart, c->dPolyPack,
                            c->dPackPoly, c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted,
                            c->dPackB, c->Kp, c->dBcnt, c->dBKey, c->dBTU, c->dJcnt, c->dJKey,
-                           c->dPackMaxKey, c->pmkPitch, c->dNum, c->dScal);
+                           c->dPackMaxKey, c->pmkPitch, c->dNum, c->dScal, work_list(c), work_count(c));
     PCP_LAUNCHED();
     return PCP_OK;
 }
@@ -1316,7 +1352,7 @@ int run_detect(pcp_ctx* c, const double2* pos, int cellsFromList, const PostShak
     c->lastDetectPos = pos;
     PCP_TRY(launch_bounds(c, pos, ps));
     PCP_LAUNCH(c, k_poly_cells, dim3(c->P), dim3(PP_BLK), c->dPackPoly, c->dBounds, c->dPcStart, c->dPcSorted,
-               c->dPackB);
+               c->dPackB, c->skipDone ? (const int*)c->dActive : nullptr, c->dActOrder, c->dActCount);
     int grid = (c->N + BLK / 64 - 1) / (BLK / 64);
     if (c->maxN <= 64 && detect_schedule(c) == 1) return launch_detect_tile(c, pos, cellsFromList != 0, cons, grid);
     if (cons) {
@@ -1324,24 +1360,24 @@ int run_detect(pcp_ctx* c, const double2* pos, int cellsFromList, const PostShak
             PCP_LAUNCH(c, (k_detect_poly<1, true>), dim3(grid), dim3(BLK), c->N, c->dPolyStart, c->dPolyPack,
                        c->dPackPoly, c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted,
                        c->dPackB, c->Kp, c->dBcnt, c->dBKey, c->dBTU, c->dJcnt, c->dJKey, c->dPackMaxKey,
-                       c->pmkPitch, c->dNum, c->dScal);
+                       c->pmkPitch, c->dNum, c->dScal, work_list(c), work_count(c));
         else
             PCP_LAUNCH(c, (k_detect_poly<0, true>), dim3(grid), dim3(BLK), c->N, c->dPolyStart, c->dPolyPack,
                        c->dPackPoly, c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted,
                        c->dPackB, c->Kp, c->dBcnt, c->dBKey, c->dBTU, c->dJcnt, c->dJKey, c->dPackMaxKey,
-                       c->pmkPitch, c->dNum, c->dScal);
+                       c->pmkPitch, c->dNum, c->dScal, work_list(c), work_count(c));
         return PCP_OK;
     }
     if (cellsFromList)
         PCP_LAUNCH(c, k_detect_poly<1>, dim3(grid), dim3(BLK), c->N, c->dPolyStart, c->dPolyPack, c->dPackPoly,
                    c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted, c->dPackB, c->Kp,
                    c->dBcnt,
-                   c->dBKey, c->dBTU, c->dJcnt, c->dJKey, c->dPackMaxKey, c->pmkPitch, c->dNum, c->dScal);
+                   c->dBKey, c->dBTU, c->dJcnt, c->dJKey, c->dPackMaxKey, c->pmkPitch, c->dNum, c->dScal, work_list(c), work_count(c));
     else
         PCP_LAUNCH(c, k_detect_poly<0>, dim3(grid), dim3(BLK), c->N, c->dPolyStart, c->dPolyPack, c->dPackPoly,
                    c->dBox, pos, c->dCellOf, c->dBounds, c->dPcStart, c->dPcSorted, c->dPackB, c->Kp,
                    c->dBcnt,
-                   c->dBKey, c->dBTU, c->dJcnt, c->dJKey, c->dPackMaxKey, c->pmkPitch, c->dNum, c->dScal);
+                   c->dBKey, c->dBTU, c->dJcnt, c->dJKey, c->dPackMaxKey, c->pmkPitch, c->dNum, c->dScal, work_list(c), work_count(c));
     return PCP_OK;
 }
 
@@ -1360,6 +1396,8 @@ int detect_alloc(pcp_ctx* c) {
     }
     PCP_TRY(dalloc(&c->dPcStart, (long long)c->P * (PC_CELLS + 1)));
     PCP_TRY(dalloc(&c->dPackB, std::max(c->P, 1)));
+    PCP_TRY(dalloc(&c->dActOrder, std::max(c->N, 1)));
+    PCP_TRY(dalloc(&c->dActCount, 1));
     return PCP_OK;
 }
 

@@ -133,4 +133,20 @@ __device__ __forceinline__ int xcd_block(int b, int G) {
     return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 
+// The polygons of a launch over a work list: list[0..n), n = *count when count is given (in
+// FIRE, the polygons of the packings still running: k_poly_cells writes the list), else all N
+// (the launch's grid).  Workgroups beyond the ceil(n / per) that the list needs (per = list
+// entries per workgroup) get -1 and leave; the others are renumbered over the XCDs as a launch
+// of exactly that many.  n is wavefront-uniform.
+__device__ __forceinline__ int work_block(const int* count, int per, int& n, int N) {
+    int G = gridDim.x;
+    n = N;
+    if (count) {
+        n = __builtin_amdgcn_readfirstlane(*count);
+        G = (n + per - 1) / per;
+        if ((int)blockIdx.x >= G) return -1;
+    }
+    return xcd_block(blockIdx.x, G);
+}
+
 }  // namespace pcp

@@ -149,9 +149,12 @@ __global__ __launch_bounds__(BLK) void k_fire_energy(int N, const int* __restric
                                                     const double* __restrict__ epoly, double* __restrict__ energy,
                                                     const int* __restrict__ active, const double* __restrict__ epre,
                                                     double alphaStart, double fDec, double* dt, double* alpha,
-                                                    int* nPos, int* flags, const int* __restrict__ xpbdIt) {
+                                                    int* nPos, int* flags, const int* __restrict__ xpbdIt,
+                                                    int keepDone) {
     __shared__ double part[BLK + 1];
     int p = blockIdx.x;
+    // keepDone: the evaluation skipped the converged packings, whose energy stays as FIRE left it
+    if (keepDone && !active[p]) return;   // the whole workgroup
     int a0 = packPoly[p], n = packPoly[p + 1] - a0;
     double e = det_sum_block(epoly + a0, n, part);
     if (threadIdx.x == 0) {
@@ -442,15 +445,15 @@ __global__ __launch_bounds__(BLK) void k_fire_small(
     int* __restrict__ summary, const int* __restrict__ scal, int rollbackOnDevice, int* __restrict__ blockOver,
     double* __restrict__ red, double2* __restrict__ x, double2* __restrict__ v, double2* __restrict__ F,
     const double2* __restrict__ xbak, const double2* __restrict__ fbak, double2* __restrict__ Fe,
-    const double2* __restrict__ febak) {
+    const double2* __restrict__ febak, int keepDone) {
     __shared__ double part[3 * (BLK + 1)];
     __shared__ double L[BLK / 64];
     int p = blockIdx.x;
     int a0 = packPoly[p], n = packPoly[p + 1] - a0;
-    // energy = fixed-order sum of the polygon energies, then its test
+    // energy = fixed-order sum of the polygon energies, then its test (keepDone: as k_fire_energy)
     double en = det_sum_block(epoly + a0, n, part);
     if (threadIdx.x == 0) {
-        energy[p] = en;
+        if (!keepDone || active[p]) energy[p] = en;
         if (xpbdIt && active[p]) flags[FL_NF * p + FL_LASTSHAKE] = xpbdIt[p];   // XPBD iterations
         fire_control(p, active, energy, epre, alphaStart, fDec, dt, alpha, nPos, flags);
     }
@@ -618,6 +621,18 @@ static int launch_fire_kick(pcp_ctx* c, double2* vel, const double2* force) {
     return PCP_OK;
 }
 
+// A context of at least SKIP_DONE_MIN_V vertices in several packings is throughput-bound: its FIRE
+// evaluations skip the packings that have converged (their polygons are left out of the
+// detection and record kernels' work list).  Below that, a step is bound by its kernels'
+// dependent-load chains, which the extra load of the list's length would lengthen.
+constexpr int SKIP_DONE_MIN_V = 1 << 17;
+static bool skip_done(const pcp_ctx* c) { return c->P > 1 && c->V >= SKIP_DONE_MIN_V; }
+struct SkipDoneScope {   // c->skipDone for the launches of one evaluation
+    pcp_ctx* c;
+    SkipDoneScope(pcp_ctx* ctx, bool on) : c(ctx) { c->skipDone = on; }
+    ~SkipDoneScope() { c->skipDone = false; }
+};
+
 // a context of at most FIRE_SMALL_P packings of at most FIRE_SMALL_V vertices each, polygons of
 // at most 64 vertices: its FIRE bookkeeping runs as one launch (k_fire_small)
 constexpr int FIRE_SMALL_P = 16;
@@ -636,7 +651,7 @@ PCP_LAUNCH(c, (k_fire_small<NOKICK>), dim3(c->P), dim3(BLK), c->N, c->dPackPoly,
                prm->nMin, prm->dtMax, prm->alphaStart, prm->fAlpha, prm->fInc, prm->fDec, prm->fTol, c->dDt,
                c->dAlpha, c->dNPos, c->dFlags, c->dSteps, c->dFireScal, c->dStatus, c->dSummary, c->dScal, block,
                block ? c->dBlockOver : nullptr, c->dRed, (double2*)st->pos, (double2*)st->vel,
-               (double2*)st->force, (const double2*)c->dXbak, fbak, (double2*)c->dFext, febak);
+               (double2*)st->force, (const double2*)c->dXbak, fbak, (double2*)c->dFext, febak, skip_done(c) ? 1 : 0);
     return PCP_OK;
 }
 
@@ -647,6 +662,8 @@ static int fire_eval(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm
     int mt = st->model & 0xFF;
     bool area = (st->model & PCP_AREA_MODES) != 0;
     bool overlap = (mt == PCP_NORMAL || mt == PCP_HYBRID);
+    // the detection's cell sort lists the running packings' polygons (models with overlap)
+    SkipDoneScope skipScope(c, overlap && skip_done(c));
     bool rigid = (mt == PCP_NORMAL);
     bool xpbd = prm->xpbdIter > 0;
     bool cons = (st->model & PCP_CONSISTENT) != 0;
@@ -693,7 +710,7 @@ static int fire_eval(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm
     if (!energyTest) return PCP_OK;
     PCP_LAUNCH(c, k_fire_energy, dim3(c->P), dim3(BLK), c->N, c->dPackPoly, c->dEpoly, st->energy, c->dActive,
                c->dEpre, prm->alphaStart, prm->fDec, c->dDt, c->dAlpha, c->dNPos, c->dFlags,
-               xpbd ? (const int*)c->dShakeIt : (const int*)nullptr);
+               xpbd ? (const int*)c->dShakeIt : (const int*)nullptr, skip_done(c) ? 1 : 0);
     return PCP_OK;
 }
 

@@ -153,6 +153,14 @@ struct pcp_ctx {
     int detSched = -1;
     // the reference pairing of the last run_records is left to the next run_forces (k_pair_exterior)
     bool pairDeferred = false;
+    // FIRE steps of a context large enough to be throughput-bound skip the packings that have
+    // converged: k_poly_cells lists the polygons of the running packings in cell order
+    // (dActOrder[0..*dActCount)) and the detection and record kernels work on that list.  A
+    // converged packing keeps the forces, energies and records of its last evaluation, which
+    // were computed at the positions it keeps.
+    bool skipDone = false;
+    int* dActOrder = nullptr;      // N
+    int* dActCount = nullptr;      // 1
     // multi-step FIRE (pcp_fire_run): force backups for device-side rollback, a block-start
     // snapshot (x, v, F; per packing dt, alpha, energy, max|F| and nPos, steps, active), the
     // largest bucket count seen in the block, and the captured graph of one block of steps
@@ -184,6 +192,10 @@ struct PostShake;   // FIRE's post-SHAKE step, done by the detection's bounds ke
 int run_detect(pcp_ctx* c, const double2* pos, int cellsFromList, const PostShake* ps = nullptr,
                bool cons = false);
 int detect_schedule(const pcp_ctx* c);
+// the polygons the detection and record kernels work on: all of them in cell order, or, in a
+// FIRE step that skips converged packings, those of the running packings (with their count)
+inline const int* work_list(const pcp_ctx* c) { return c->skipDone ? c->dActOrder : c->dPcSorted; }
+inline const int* work_count(const pcp_ctx* c) { return c->skipDone ? c->dActCount : nullptr; }
 int detect_alloc(pcp_ctx* c);
 int detect_stats(pcp_ctx* c, long long* polygonPairs, long long* edgePairs, long long* screened);
 int seg_width(int maxN);

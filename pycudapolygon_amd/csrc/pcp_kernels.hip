@@ -831,6 +831,16 @@ __global__ void k_reset_areas(long long n2, const int* __restrict__ shapeId, dou
     double p = (idx % 2) ? u + comY[s] : u + comX[s];
     pos[idx] = p - floor(p);
 }
+// the batched form of setPhi's target scaling (pyCudaPolygon.py:223-234: targetAreas *= r,
+// targetEdgeLengths *= sqrt(r)): f[p] multiplies packing p's target areas, f[P + p] its edge
+// targets.  The factors come from the host, so each product is that of the host's multiply.
+__global__ void k_scale_targets(int V, int N, int P, const int* __restrict__ shapeId,
+                                const int* __restrict__ polyPack, const double* __restrict__ f,
+                                double* __restrict__ targetAreas, double* __restrict__ targetEdgeV) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (targetEdgeV && i < V) targetEdgeV[i] *= f[P + polyPack[shapeId[i]]];
+    if (targetAreas && i < N) targetAreas[i] *= f[polyPack[i]];
+}
 // effectiveForceMagKernel kernels.cuh:625-633, max per polygon
 __global__ void k_eff_force_poly(int N, const int* __restrict__ polyStart, const double2* __restrict__ pos,
                                  const double2* __restrict__ tent, const double2* __restrict__ force,
@@ -1032,7 +1042,7 @@ static void free_topology(pcp_ctx* c) {
     dfree(c->dEpre); dfree(c->dNPos); dfree(c->dSteps); dfree(c->dActive); dfree(c->dFlags);
     dfree(c->dFireScal); dfree(c->dShakeIt); dfree(c->dCellInv); dfree(c->dCom); dfree(c->dStatusBlock);
     c->dStatus = nullptr; c->dSummary = nullptr;
-    dfree(c->dBounds); dfree(c->dPcSorted); dfree(c->dPcStart); dfree(c->dPackB); dfree(c->dSticky);
+    dfree(c->dBounds); dfree(c->dPcSorted); dfree(c->dActOrder); dfree(c->dActCount); dfree(c->dPcStart); dfree(c->dPackB); dfree(c->dSticky);
     c->haveBoxes = false;
 }
 
@@ -1323,6 +1333,22 @@ int pcp_reset_areas(pcp_ctx* c, double* pos, const double* areas, const double* 
     if (c->V == 0) return PCP_OK;
     hipLaunchKernelGGL(k_reset_areas, dim3(nblk(2LL * c->V)), dim3(BLK), 0, c->stream, 2LL * c->V,
                        c->dShapeId, pos, areas, targetAreas, comX, comY);
+    PCP_LAUNCHED();
+    return PCP_OK;
+}
+int pcp_scale_targets(pcp_ctx* c, const double* areaFactor, const double* edgeFactor, double* targetAreas,
+                      double* targetEdgeV) {
+    if ((targetAreas && !areaFactor) || (targetEdgeV && !edgeFactor))
+        return fail_arg("pcp_scale_targets: a target array without its factors");
+    if (c->N == 0 || (!targetAreas && !targetEdgeV)) return PCP_OK;
+    // the factors go to the per-packing reduction buffer (4P doubles), free between calls
+    if (targetAreas)
+        PCP_HIP(hipMemcpyAsync(c->dRedOut, areaFactor, sizeof(double) * c->P, hipMemcpyHostToDevice, c->stream));
+    if (targetEdgeV)
+        PCP_HIP(hipMemcpyAsync(c->dRedOut + c->P, edgeFactor, sizeof(double) * c->P, hipMemcpyHostToDevice,
+                               c->stream));
+    hipLaunchKernelGGL(k_scale_targets, dim3(nblk(std::max(c->V, c->N))), dim3(BLK), 0, c->stream, c->V, c->N, c->P,
+                       c->dShapeId, c->dPolyPack, c->dRedOut, targetAreas, targetEdgeV);
     PCP_LAUNCHED();
     return PCP_OK;
 }

@@ -913,7 +913,7 @@ int pcp_mpnn_guard(pcp_ctx* c, double* pos, const double* xPrev, double kappa, i
         for (int round = 0; round < 2; ++round) {
             PCP_TRY(launch_bounds(c, (const double2*)pos, nullptr));
             PCP_LAUNCH(c, k_poly_cells, dim3(c->P), dim3(PP_BLK), c->dPackPoly, c->dBounds, c->dPcStart,
-                       c->dPcSorted, c->dPackB);
+                       c->dPcSorted, c->dPackB, (const int*)nullptr, (int*)nullptr, (int*)nullptr);
             if (round == 0) {
                 PCP_LAUNCH(c, k_contained<true>, dim3(nblk(c->N)), dim3(BLK), c->N, c->dPolyPack, c->dPackPoly,
                            c->dBounds, c->dPcStart, c->dPcSorted, c->dPackB, kappa, flags, c->dGuard);

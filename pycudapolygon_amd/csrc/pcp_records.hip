@@ -163,10 +163,11 @@ __global__ __launch_bounds__(64) void k_pair_poly(int N, int Kp, const int* __re
                                                  const double2* __restrict__ pairPos, uint64_t* __restrict__ rKey,
                                                  float2* __restrict__ rTU, uint64_t* __restrict__ rOuter,
                                                  float2* __restrict__ rUT, const int* __restrict__ order,
-                                                 int* __restrict__ scal) {
+                                                 const int* __restrict__ nWork, int* __restrict__ scal) {
     int lane = threadIdx.x & (SEGX - 1);
-    int slot = xcd_block(blockIdx.x, gridDim.x) * (64 / SEGX) + threadIdx.x / SEGX;
-    if (slot >= N) return;   // whole segments: the shuffles below stay inside a segment
+    int nw, wb = work_block(nWork, 64 / SEGX, nw, N);
+    int slot = wb * (64 / SEGX) + threadIdx.x / SEGX;
+    if (wb < 0 || slot >= nw) return;   // whole segments: the shuffles below stay inside a segment
     int s = order[slot];
     int cnt = min(bcnt[s], Kp);
     if (cnt == 0) return;
@@ -323,10 +324,12 @@ __global__ __launch_bounds__(64) void k_lens_rep(int N, int Kp, const int* __res
                                                 const int* __restrict__ packPoly, const int* __restrict__ bcnt,
                                                 const uint64_t* __restrict__ rKey,
                                                 const uint64_t* __restrict__ rOuter, const double* __restrict__ Er,
-                                                double* __restrict__ laRep, const int* __restrict__ order) {
+                                                double* __restrict__ laRep, const int* __restrict__ order,
+                                                const int* __restrict__ nWork) {
     int lane = threadIdx.x & (SEGX - 1);
-    int slot = xcd_block(blockIdx.x, gridDim.x) * (64 / SEGX) + threadIdx.x / SEGX;
-    if (slot >= N) return;   // no barriers or shuffles below
+    int nw, wb = work_block(nWork, 64 / SEGX, nw, N);
+    int slot = wb * (64 / SEGX) + threadIdx.x / SEGX;
+    if (wb < 0 || slot >= nw) return;   // no barriers or shuffles below
     int s = order[slot];
     int cnt = min(bcnt[s], Kp);
     int pbase = packPoly[polyPack[s]];
@@ -357,10 +360,12 @@ __global__ __launch_bounds__(64) void k_lens_energy(int N, int Kp, const int* __
                                                    const int* __restrict__ bcnt, const uint64_t* __restrict__ rKey,
                                                    const uint64_t* __restrict__ rOuter,
                                                    const float2* __restrict__ rTU, const float2* __restrict__ rUT,
-                                                   double* __restrict__ Er, const int* __restrict__ order) {
+                                                   double* __restrict__ Er, const int* __restrict__ order,
+                                                   const int* __restrict__ nWork) {
     int lane = threadIdx.x & (SEGX - 1);
-    int slot = xcd_block(blockIdx.x, gridDim.x) * (64 / SEGX) + threadIdx.x / SEGX;
-    if (slot >= N) return;   // no barriers or shuffles below
+    int nw, wb = work_block(nWork, 64 / SEGX, nw, N);
+    int slot = wb * (64 / SEGX) + threadIdx.x / SEGX;
+    if (wb < 0 || slot >= nw) return;   // no barriers or shuffles below
     int s = order[slot];
     int cnt = min(bcnt[s], Kp);
     int pk = polyPack[s];
@@ -630,15 +635,18 @@ __global__ __launch_bounds__(64) void k_exterior_poly(
     const uint64_t* __restrict__ rOuter, const float2* __restrict__ rTU, const float2* __restrict__ rUT,
     double2* __restrict__ jC, int* __restrict__ jT, double2* __restrict__ F1, double2* __restrict__ LMo,
     double2* __restrict__ LNo, double* __restrict__ accR, const double* __restrict__ Er,
-    const double* __restrict__ laRep, const int* __restrict__ order, char* __restrict__ gstage) {
+    const double* __restrict__ laRep, const int* __restrict__ order, const int* __restrict__ nWork,
+    char* __restrict__ gstage) {
     constexpr int G = 64 / SEGX;
     extern __shared__ char smem[];
     __shared__ double leBuf[64];
     int lane = threadIdx.x & (SEGX - 1), g = threadIdx.x / SEGX, segBase = g * SEGX;
     ExtLds L = GLOB ? ext_lds(gstage + ((size_t)blockIdx.x * G + g) * ext_lds_stride(Kp), Kp)
                     : ext_lds(smem + g * ext_lds_stride(Kp), Kp);
-    int slot = xcd_block(blockIdx.x, gridDim.x) * G + g;
-    bool has = slot < N;
+    int nw, wb = work_block(nWork, G, nw, N);
+    if (wb < 0) return;   // the whole workgroup
+    int slot = wb * G + g;
+    bool has = slot < nw;
     int s = has ? order[slot] : 0;
     int cnt = has ? min(bcnt[s], Kp) : 0;
     int pk = polyPack[s];
@@ -666,14 +674,16 @@ __global__ __launch_bounds__(64) void k_pair_exterior(
     const uint64_t* __restrict__ jKey, uint64_t* __restrict__ rKey, float2* __restrict__ rTU,
     uint64_t* __restrict__ rOuter, float2* __restrict__ rUT, double2* __restrict__ jC, int* __restrict__ jT,
     double2* __restrict__ F1, double2* __restrict__ LMo, double2* __restrict__ LNo, double* __restrict__ accR,
-    const int* __restrict__ order, int* __restrict__ scal) {
+    const int* __restrict__ order, const int* __restrict__ nWork, int* __restrict__ scal) {
     constexpr int G = 64 / SEGX;
     extern __shared__ char smem[];
     __shared__ double leBuf[64];
     int lane = threadIdx.x & (SEGX - 1), g = threadIdx.x / SEGX, segBase = g * SEGX;
     ExtLds L = ext_lds(smem + g * ext_lds_stride(Kp), Kp);
-    int slot = xcd_block(blockIdx.x, gridDim.x) * G + g;
-    bool has = slot < N;
+    int nw, wb = work_block(nWork, G, nw, N);
+    if (wb < 0) return;   // the whole workgroup
+    int slot = wb * G + g;
+    bool has = slot < nw;
     int s = has ? order[slot] : 0;
     int cnt = has ? min(bcnt[s], Kp) : 0;
     int pk = polyPack[s];
@@ -716,10 +726,11 @@ __global__ __launch_bounds__(BLK) void k_gather_poly(
     const double2* __restrict__ LNi, const double* __restrict__ accR, double* __restrict__ edgeLengths,
     double* __restrict__ areas, const double* __restrict__ targetEdgeV, const double* __restrict__ targetAreas,
     double stiffness, double compressibility, double2* __restrict__ force, double* __restrict__ epoly,
-    const int* __restrict__ order, double perimK, double2* __restrict__ fext) {
+    const int* __restrict__ order, const int* __restrict__ nWork, double perimK, double2* __restrict__ fext) {
     int sub = threadIdx.x & (SEG - 1);
-    int si = (xcd_block(blockIdx.x, gridDim.x) * BLK + threadIdx.x) / SEG;
-    if (si >= N) return;   // whole segments leave together
+    int nw, wb = work_block(nWork, BLK / SEG, nw, N);
+    int si = (wb * BLK + threadIdx.x) / SEG;
+    if (wb < 0 || si >= nw) return;   // whole segments leave together
     int s = order[si];
     int st = polyStart[s], n = polyStart[s + 1] - st;
     // area in the reference's order (needed by the area spring and written when GEOM)
@@ -899,8 +910,8 @@ static int launch_pair_w(pcp_ctx* c, const double2* pairPos) {
     constexpr int G = 64 / SEGX;
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_poly<EXACT, SEGX>), dim3((c->N + G - 1) / G), dim3(64), 0, c->stream,
                        c->N, c->Kp, c->dPolyStart, c->dPolyPack, c->dPackPoly, c->dBcnt, c->dBKey, c->dBTU,
-                       c->dPackMaxKey, c->pmkPitch, pairPos, c->dRKey, c->dRTU, c->dROuter, c->dRUT, c->dPcSorted,
-                       c->dScal);
+                       c->dPackMaxKey, c->pmkPitch, pairPos, c->dRKey, c->dRTU, c->dROuter, c->dRUT, work_list(c),
+                       work_count(c), c->dScal);
     PCP_LAUNCHED();
     return PCP_OK;
 }
@@ -937,7 +948,7 @@ static int launch_gather(pcp_ctx* c, int overlap, int edgeOn, int areaOn, const 
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_gather_poly<SEG, GEOM, PROJ>), dim3(grid), dim3(BLK), 0, c->stream, c->N,
                        c->Kp, overlap, edgeOn, areaOn, c->dPolyStart, pos, c->dJcnt, c->dJC, c->dJT, c->dF1, c->dLM,
                        c->dLN, c->dAccR, edgeLengths, areas, targetEdgeV, targetAreas, stiffness, compressibility,
-                       force, c->dEpoly, c->dPcSorted, perimK, fext);
+                       force, c->dEpoly, work_list(c), work_count(c), perimK, fext);
     PCP_LAUNCHED();
     PCP_TRY(prof_end(c, slot));
     return PCP_OK;
@@ -986,7 +997,7 @@ static int launch_exterior_w(pcp_ctx* c, const double2* pos) {
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_exterior_poly<MODE, SEGX, GLOB>), dim3(grid), dim3(64), lds, c->stream,
                        c->N, c->Kp, c->dPolyStart, c->dPolyPack, c->dPackPoly, pos, c->dBcnt, c->dJcnt, c->dJKey,
                        c->dRKey, c->dROuter, c->dRTU, c->dRUT, c->dJC, c->dJT, c->dF1, c->dLM, c->dLN, c->dAccR,
-                       c->dEr, c->dLaRep, c->dPcSorted, GLOB ? c->dExtStage : (char*)nullptr);
+                       c->dEr, c->dLaRep, work_list(c), work_count(c), GLOB ? c->dExtStage : (char*)nullptr);
     PCP_LAUNCHED();
     return PCP_OK;
 }
@@ -1014,7 +1025,7 @@ static int launch_pair_exterior_w(pcp_ctx* c, const double2* pos) {
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_exterior<SEGX>), dim3((c->N + G - 1) / G), dim3(64), lds, c->stream,
                        c->N, c->Kp, c->dPolyStart, c->dPolyPack, c->dPackPoly, pos, c->dBcnt, c->dBKey, c->dBTU,
                        c->dPackMaxKey, c->pmkPitch, c->dJcnt, c->dJKey, c->dRKey, c->dRTU, c->dROuter, c->dRUT,
-                       c->dJC, c->dJT, c->dF1, c->dLM, c->dLN, c->dAccR, c->dPcSorted, c->dScal);
+                       c->dJC, c->dJT, c->dF1, c->dLM, c->dLN, c->dAccR, work_list(c), work_count(c), c->dScal);
     PCP_LAUNCHED();
     return PCP_OK;
 }
@@ -1043,7 +1054,7 @@ static int launch_lens(pcp_ctx* c, const double2* pos) {
 #define PCP_LENS(W)                                                                                           \
     PCP_LAUNCH(c, k_lens_energy<W>, dim3((c->N + 64 / W - 1) / (64 / W)), dim3(64), c->N, c->Kp, c->dPolyStart, \
                c->dPolyPack, c->dPackPoly, pos, c->dBcnt, c->dRKey, c->dROuter, c->dRTU, c->dRUT, c->dEr,       \
-               c->dPcSorted)
+               work_list(c), work_count(c))
     if (c->segLens == 64) PCP_LENS(64);
     else if (c->segLens == 32) PCP_LENS(32);
     else PCP_LENS(16);
@@ -1055,7 +1066,8 @@ static int launch_lens(pcp_ctx* c, const double2* pos) {
 static int launch_lens_rep(pcp_ctx* c) {
 #define PCP_LENS_REP(W)                                                                                     \
     PCP_LAUNCH(c, k_lens_rep<W>, dim3((c->N + 64 / W - 1) / (64 / W)), dim3(64), c->N, c->Kp, c->dPolyPack, \
-               c->dPackPoly, c->dBcnt, c->dRKey, c->dROuter, c->dEr, c->dLaRep, c->dPcSorted)
+               c->dPackPoly, c->dBcnt, c->dRKey, c->dROuter, c->dEr, c->dLaRep, work_list(c),     \
+               work_count(c))
     if (c->segLens == 64) PCP_LENS_REP(64);
     else if (c->segLens == 32) PCP_LENS_REP(32);
     else PCP_LENS_REP(16);

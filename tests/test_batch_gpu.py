@@ -49,3 +49,36 @@ def test_autotuned_batch_gives_the_same_trajectories():
     np.testing.assert_array_equal(a.energy, b.energy)
     for x, y in zip(a.positions, b.positions):
         np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize("model,n,phis,cap", [(SQUARED, 16, (0.82, 0.86), 3000),
+                                              (capi.NORMAL | 0x400, 24, (0.82, 0.86), 3000),
+                                              (capi.NORMAL, 16, (0.30, 0.86), 400)],
+                         ids=["squared-lens-16", "squared-pair-24", "reference-16"])
+def test_converged_packings_skipped_without_changing_trajectories(model, n, phis, cap):
+    """A context of 2^17 vertices or more in several packings leaves the packings that have
+    converged out of the detection and record kernels of its FIRE steps (their forces and
+    energies stay those of their last evaluation, at the positions they keep).  Two packings
+    in one context, run in graph-replayed blocks, give every packing the steps, energy and
+    positions of its own one-packing run bit for bit; the packing that converges first is
+    skipped while the other runs on.  The reference energy (whose FIRE stalls short of
+    max|F| <= 1e-10) pairs a sparse packing that converges at once with a dense one run to the
+    step cap."""
+    from pycudapolygon_amd.generate import make_packing
+    N = 131072 // (2 * n) + 1
+    # the sparse packing of the reference case: circles apart, no overlap, converged at once
+    mcf = (1.15, 0.5) if model == capi.NORMAL else (0.5, 0.5)
+    pk = [make_packing(N, n, phi, seed=70 + k, min_center_frac=mcf[k]) for k, phi in enumerate(phis)]
+    assert sum(int(p["startIndices"][-1]) for p in pk) >= 1 << 17
+    kw = dict(fTol=1e-10, dtMax=100.0)
+    print(f"[batch] 2 x N={N} x {n}, model {model:#x}, one context", flush=True)
+    both = batch.minimize_batch(pk, model, kw, cap, compact_below=0.0)
+    print(f"[batch] steps {both.steps}", flush=True)
+    assert both.contexts == [2]
+    assert (both.steps >= 0).any() and both.steps[0] != both.steps[1], both.steps   # one runs on alone
+    for k in range(2):
+        print(f"[batch] packing {k} alone", flush=True)
+        one = batch.minimize_batch([pk[k]], model, kw, cap)
+        assert one.steps[0] == both.steps[k]
+        assert one.energy[0] == both.energy[k] and one.max_force[0] == both.max_force[k]
+        np.testing.assert_array_equal(one.positions[0], both.positions[k])

@@ -63,6 +63,35 @@ def test_reset_areas_reaches_targets_about_fixed_centroids():
     assert np.abs(dc).max() <= 1e-14
 
 
+def test_scale_targets_equals_host_scaling():
+    """pcp_scale_targets multiplies each packing's target areas and edge targets by its two
+    factors on the device: bit-identical to the host's numpy products (setPhi's scaling,
+    pyCudaPolygon.py:223-234), uneven packings, a NULL array left untouched."""
+    pk = make_batch(4, 52, N_range=(30, 90), n_range=(8, 24), min_center_frac=0.5)
+    s = capi.System(pk, [box_size_for(p["diameter"]) for p in pk], model=capi.NORMAL | 0x200)
+    ta0, tev0 = s.ta.get(), s.tev.get()
+    phi = compress.packing_fractions(s)
+    r = np.ascontiguousarray((phi + np.array([0.01, 0.002, -0.003, 0.0])) / phi)
+    q = np.sqrt(r)
+    s.ctx.call("pcp_scale_targets", capi._ptr(r), capi._ptr(q), s.ta.ptr, s.tev.ptr)
+    s.sync()
+    ta_h, tev_h = ta0.copy(), tev0.copy()
+    for p in range(s.P):
+        a, b = int(s.packPoly[p]), int(s.packPoly[p + 1])
+        ta_h[a:b] *= r[p]
+        va, vb = s.vertex_range(p)
+        tev_h[va:vb] *= np.sqrt(r[p])
+    assert np.array_equal(s.ta.get(), ta_h) and np.array_equal(s.tev.get(), tev_h)
+    # edge targets only: the areas stay as they are
+    s.ctx.call("pcp_scale_targets", None, capi._ptr(q), None, s.tev.ptr)
+    s.sync()
+    assert np.array_equal(s.ta.get(), ta_h)
+    for p in range(s.P):
+        va, vb = s.vertex_range(p)
+        tev_h[va:vb] *= np.sqrt(r[p])
+    assert np.array_equal(s.tev.get(), tev_h)
+
+
 def test_short_compression_ramp_minimizes_every_step():
     pk = [make_packing(128, 16, 0.80, seed=60 + k, min_center_frac=0.5) for k in range(2)]
     steps, s = compress.ramp(pk, 0.804, 1e-3, dict(fTol=1e-10, dtMax=100.0), 4000,
