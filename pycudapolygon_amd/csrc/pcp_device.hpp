@@ -133,6 +133,15 @@ __device__ __forceinline__ int xcd_block(int b, int G) {
     return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 
+// atomicMax on a maximum that only grows during the launch (zeroed by an earlier launch): the
+// value is read first at device scope (past the XCD's L2), and a value not above what was read
+// cannot raise it, so its atomic is skipped.  Device atomics on one cache line are carried out
+// one after another (about 11 ns apart on MI355X), so one atomic per workgroup on a packing's
+// counter queued tens of thousands of them at large packings.
+__device__ __forceinline__ void max_if_above(int* a, int v) {
+    if (v > __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(a, v);
+}
+
 // The polygons of a launch over a work list: list[0..n), n = *count when count is given (in
 // FIRE, the polygons of the packings still running: k_poly_cells writes the list), else all N
 // (the launch's grid).  Workgroups beyond the ceil(n / per) that the list needs (per = list

@@ -322,7 +322,8 @@ __device__ int seg_shake(int e, int n, d2& p, double L, double A0, int maxIter, 
 // Iteration counts of the BLK / SEG polygons of a block into itOut (max per packing):
 // itOut[pk] and itOut[P + pk] are read by every polygon's packing, and one device atomic
 // per polygon on them serializes (thousands of polygons, one address); here a block that
-// holds one packing issues one atomic per counter.  pk < 0: no polygon in this slot.
+// holds one packing issues at most one atomic per counter (max_if_above).  pk < 0: no polygon
+// in this slot.
 template <int Q>
 __device__ __forceinline__ void block_pack_max(int q, bool leader, int pk, int it1, int it2, int P, int* itOut) {
     static_assert(Q <= 64, "one wavefront merges the slots");
@@ -337,12 +338,12 @@ __device__ __forceinline__ void block_pack_max(int q, bool leader, int pk, int i
         a1 = wave_max_int(a1);
         a2 = wave_max_int(a2);
         if (l == 0 && pk0 >= 0) {
-            atomicMax(&itOut[pk0], a1);
-            if (a2 >= 0) atomicMax(&itOut[P + pk0], a2);
+            max_if_above(&itOut[pk0], a1);
+            if (a2 >= 0) max_if_above(&itOut[P + pk0], a2);
         }
     } else if (l < Q && mpk >= 0) {   // packings change inside the block
-        atomicMax(&itOut[mpk], a1);
-        if (a2 >= 0) atomicMax(&itOut[P + mpk], a2);
+        max_if_above(&itOut[mpk], a1);
+        if (a2 >= 0) max_if_above(&itOut[P + mpk], a2);
     }
 }
 

@@ -93,6 +93,30 @@ def test_geometry_bit_exact(seed):
     assert s.maxEdge.get()[0] == g["maxEdgeLength"]
 
 
+@pytest.mark.parametrize("n_range", [(3, 4), (5, 8), (9, 16), (17, 32), (33, 64), (60, 80)],
+                         ids=["seg4", "seg8", "seg16", "seg32", "seg64", "thread-per-polygon"])
+def test_geometry_bit_exact_uneven(n_range):
+    """pcp_update_geometry on uneven packings (lane segments of 4..64 lanes, the thread per
+    polygon beyond 64 vertices) against the oracle's updatePolygonGeometry, packing by packing."""
+    from pycudapolygon_amd.generate import make_batch
+    pk = make_batch(3, 90 + n_range[1], N_range=(20, 60), n_range=n_range)   # centres uniform
+    s = capi.System(pk, [box_size_for(p["diameter"]) for p in pk])
+    s.geometry()
+    s.sync()
+    el, ar, cx, cy, cons, mx = (s.edgeLengths.get(), s.areas.get(), s.comX.get(), s.comY.get(),
+                                s.constraints.get(), s.maxEdge.get())
+    for k, p in enumerate(pk):
+        g = O.geometry(p["positions"], p["startIndices"])
+        a, b = s.vertex_range(k)
+        u, w = int(s.packPoly[k]), int(s.packPoly[k + 1])
+        np.testing.assert_array_equal(el[a:b], g["edgeLengths"])
+        np.testing.assert_array_equal(ar[u:w], g["areas"])
+        np.testing.assert_array_equal(cx[u:w], g["comX"])
+        np.testing.assert_array_equal(cy[u:w], g["comY"])
+        np.testing.assert_array_equal(cons[6 * a:6 * b], g["constraints"])
+        assert mx[k] == g["maxEdgeLength"]
+
+
 @pytest.mark.parametrize("seed", range(6))
 @pytest.mark.parametrize("model", [capi.NORMAL, capi.HYBRID, capi.SOFT_BODY, capi.EDGE_ONLY, capi.AREA_ONLY])
 def test_force_energy_parity(seed, model):
