@@ -99,7 +99,7 @@ def measure_traffic(kernel, seed):
     kernel (20 force evaluations in a child run) and of the plain copy kernel k_copy on 2^25
     doubles, each scaled by the copy's true bytes over its reading."""
     run = ["--steps", "20", "--warmup", "2", "--no-fire", "--no-cpu-baseline", "--traffic", "cited",
-           "--seed", str(seed)]
+           "--scale-packings", "0", "--seed", str(seed)]
     cal = ["--calibrate-copy", str(COPY_DOUBLES)]
     true_kib = 8 * COPY_DOUBLES / 1024
     out = {}
@@ -197,6 +197,59 @@ def cpu_baseline(seed, budget_s):
                           f"{threads} OpenMP threads; {one['evals']} on one thread")
     if native:
         subprocess.run(["rm", "-rf", out], check=False)
+    return res
+
+
+def eval_at_scale(args, E0, F0):
+    """The evaluation workload at the scale the chip is built for: P packings of N=4096 x 16
+    (seeds seed..seed+P-1, packing 0 the one-packing workload) in one context, each force
+    evaluation covering all of them.  Reported beside the one-packing figure (which stays the
+    headline: a FIRE run evaluates one packing after another); packing 0's energy and forces
+    must equal the one-packing context's bit for bit."""
+    from pycudapolygon_amd import capi
+    from pycudapolygon_amd.generate import make_packing, box_size_for
+    P = args.scale_packings
+    ps = [make_packing(4096, 16, 0.85, seed=args.seed + k) for k in range(P)]
+    bs = [box_size_for(q["diameter"] if args.cells == "diameter" else q["maxEdgeLength"]) for q in ps]
+    s = capi.System(ps, bs)
+    for _ in range(4):   # the first evaluations may enlarge the record buckets (reported at the sync)
+        try:
+            s.force_eval()
+            s.sync()
+        except capi.PcpError:
+            pass
+    s.force_eval()
+    s.sync()
+    same =bool(s.get_energy()[0] == E0 and np.array_equal(s.get_forces(0), F0))
+    nI = sum(s.get_records(k)["numIntersections"] for k in range(P))
+    kernel = DETECT_KERNELS[s.get_detect_schedule()]
+    K = max(10, args.steps // 20)
+    s.sync()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        s.force_eval()
+    s.sync()
+    el = time.perf_counter() - t0
+    s.ctx.call("pcp_profile_kernel", kernel.encode())
+    for _ in range(K):
+        s.force_eval()
+    s.sync()
+    ms, nl = ctypes.c_double(0.0), ctypes.c_int(0)
+    s.ctx.call("pcp_profile_read", ctypes.byref(ms), ctypes.byref(nl))
+    s.ctx.call("pcp_profile_kernel", None)
+    kavg = ms.value / max(nl.value, 1) * 1e-3
+    nbytes = BYTES_PER_VERTEX * s.V + BYTES_PER_POLYGON * s.N + BYTES_PER_CROSSING * nI
+    res = dict(packings=P, workload=f"{P} packings of the N=4096 x 16 workload (seeds {args.seed}..{args.seed + P - 1}) "
+               "in one context; one step evaluates all of them", vertices=s.V, records=int(nI), evals=K,
+               seconds=el, ms_per_context_eval=1e3 * el / K, packing_evals_per_s=P * K / el,
+               packing0_equal_to_one_packing_context=same,
+               detection=dict(kernel=kernel, avg_launch_us=kavg * 1e6, algorithmic_bytes_per_launch=nbytes,
+                              achieved_gbs=nbytes / kavg / 1e9, hbm_frac=nbytes / kavg / 1e9 / HBM_PEAK_GBS,
+                              counters="profiles/force_eval_counters_N4096_and_32xN4096.md"))
+    print(f"[bench] at scale: {P} x N=4096 in one context, {res['ms_per_context_eval']:.3f} ms per evaluation, "
+          f"{res['packing_evals_per_s']:.0f} packing evaluations/s; {kernel} {kavg * 1e6:.1f} us; packing 0 equal "
+          f"to the one-packing context: {same}", flush=True)
+    s.close()
     return res
 
 
@@ -468,6 +521,9 @@ def main():
     ap.add_argument("--fire-seconds", type=float, default=20.0)
     ap.add_argument("--n4096-batch", type=int, default=32,
                     help="packings of the N=4096 workload minimized together in one context (0: skip)")
+    ap.add_argument("--scale-packings", type=int, default=32,
+                    help="packings of the evaluation workload evaluated together in one context, reported "
+                         "beside the one-packing figure (0: skip)")
     ap.add_argument("--batch-packings", type=int, default=256)
     ap.add_argument("--compress-replicas", type=int, default=16)
     ap.add_argument("--compress-seconds", type=float, default=90.0)
@@ -586,6 +642,8 @@ def main():
         roof = dict(kernel=args.roofline_kernel, avg_launch_us=kavg_s * 1e6, achieved=None, peak=None,
                     unit=None, frac=None, traffic=None, bound=None)
 
+    scale = eval_at_scale(args, s.get_energy()[0], s.get_forces(0)) if args.scale_packings > 1 else None
+
     fire = None
     if not args.no_fire:
         fire = fire_legs(args)
@@ -604,7 +662,7 @@ def main():
                                "phi=0.85, periodic box, cell side = max polygon diameter, FP64; "
                                "step = one force evaluation", "seed": args.seed, "box_cells": B,
                    "records": int(nI)},
-        "roofline": roof, "cpu_baseline": cpu, "packings": fire,
+        "roofline": roof, "cpu_baseline": cpu, "at_scale": scale, "packings": fire,
     }
     print(json.dumps(out), flush=True)
 
