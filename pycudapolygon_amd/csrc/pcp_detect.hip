@@ -343,6 +343,32 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
+// The candidates of the 3x3 block of polygon cells as one list: lane k < ncell holds the
+// range (lo, cnt) of cell k (0 elsewhere).  The ranges are read lane by lane as wave-uniform
+// values (no LDS round trip per cell); candidate ci of the list is slot lo_k + ci - start_k of
+// the sorted ids, k the first cell whose running count exceeds ci (start_k the count of the
+// cells before k).
+__device__ __forceinline__ int cell_list_length(int cnt) {
+    int run = 0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) run += __builtin_amdgcn_readlane(cnt, k);
+    return run;
+}
+// slot of candidate ci (< the list's length) in the sorted ids
+__device__ __forceinline__ int cell_slot(int lo, int cnt, int ci) {
+    int run = 0, off = 0;
+    bool found = false;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        int l = __builtin_amdgcn_readlane(lo, k), c = __builtin_amdgcn_readlane(cnt, k);
+        bool here = !found && ci < run + c;
+        off = here ? l - run : off;
+        found = found || here;
+        run += c;
+    }
+    return ci + off;
+}
+
 // cell of an edge in the user grid as x | y << 16 (box sizes are below 2^16: at most 2e9 cells)
 __device__ __forceinline__ int cell_xy(int flat, int B) { return (flat % B) | ((flat / B) << 16); }
 __device__ __forceinline__ bool cells_adjacent(int ca, int cb, int B) {
@@ -712,21 +738,10 @@ __device__ void detect_polygon(
         edge_tile<CELLS, TILE_SELF, SRC_GLOBAL, CONS>(L, lane, stA, nA, stA, nA, sAl, sAl, Bu, s0, pos, cellOf, staged, out,
                                                 num, kmax);
     }
-    int incl = cnt;
-#pragma unroll
-    for (int o = 1; o < 16; o <<= 1) {
-        int y = __shfl_up(incl, o);
-        if (lane >= o) incl += y;
-    }
-    int T = __shfl(incl, ncell - 1);
+    int T = cell_list_length(cnt);
     for (int c0 = 0; c0 < T; c0 += 64) {
         int ci = c0 + lane;
-        int q = 0;
-        bool found = false;
-        for (int k = 0; k < ncell; ++k) {   // every lane takes part in the shuffles
-            int hiK = __shfl(incl, k), cK = __shfl(cnt, k), loK = __shfl(lo, k);
-            if (!found && ci < hiK) { q = loK + ci - (hiK - cK); found = true; }
-        }
+        int q = (ci < T) ? cell_slot(lo, cnt, ci) : 0;
         int Bq = -1, stq = 0, nq = 0;
         double4 bc = make_double4(0.0, 0.0, 0.0, 1.0);
         if (ci < T) {
@@ -1164,22 +1179,11 @@ __global__ __launch_bounds__(BLK) void k_detect_tile(
         double2 p = L.P[lane], r = L.R[lane];
         aBox = edge_box(wrap(p.x - bA.x), wrap(p.y - bA.y), mk(r.x, r.y));
     }
-    int incl = cnt;
-#pragma unroll
-    for (int o = 1; o < 16; o <<= 1) {
-        int y = __shfl_up(incl, o);
-        if (lane >= o) incl += y;
-    }
-    int T = __shfl(incl, ncell - 1);
+    int T = cell_list_length(cnt);
     bool selfPair = nA >= 4;   // a polygon of fewer vertices has no non-adjacent edge pair
     for (int c0 = 0; c0 < T || (c0 == 0 && selfPair); c0 += 64) {
         int ci = c0 + lane;
-        int q = 0;
-        bool found = false;
-        for (int k = 0; k < ncell; ++k) {   // every lane takes part in the shuffles
-            int hiK = __shfl(incl, k), cK = __shfl(cnt, k), loK = __shfl(lo, k);
-            if (!found && ci < hiK) { q = loK + ci - (hiK - cK); found = true; }
-        }
+        int q = (ci < T) ? cell_slot(lo, cnt, ci) : 0;
         bool keep = false, boxed = false;
         int cand = -1;
         int stq = 0, nq = 0;
