@@ -121,46 +121,97 @@ __global__ void k_fire_post_shake(int V, int P, int shakeIter, const int* __rest
     }
 }
 
-// energy test (model.cpp:444-470)
-__device__ void fire_control(int p, const int* __restrict__ active, const double* __restrict__ energy,
-                             const double* __restrict__ epre, double alphaStart, double fDec, double* dt,
-                             double* alpha, int* nPos, int* flags) {
-    if (!active[p]) return;
-    int* fl = flags + FL_NF * p;
-    double ePost = energy[p], ePre = epre[p];
-    if (fl[FL_FAIL]) {
-        dt[p] = dt[p] * fDec;
-        alpha[p] = alphaStart;
-        nPos[p] = 0;
-        fl[FL_ACCEPT] = 0;
+// The FIRE scalars of one packing held in registers by the thread that updates them: read
+// together at a kernel's start (one memory latency, overlapping the kernel's reductions) and
+// written back once, so a packing's bookkeeping is not a chain of dependent loads and stores.
+struct FireRegs {
+    int act, nPos, steps;
+    int fl[FL_NF];
+    double dt, alpha, energy, epre;
+    double fsScale, fsOneMa, fsMaxF, fsMaxFB;
+};
+__device__ __forceinline__ FireRegs fire_load(int p, const int* active, const int* flags, const double* dt,
+                                              const double* alpha, const int* nPos, const int* steps,
+                                              const double* energy, const double* epre, const double* fs) {
+    FireRegs S;
+    S.act = active[p];
+#pragma unroll
+    for (int k = 0; k < FL_NF; ++k) S.fl[k] = flags[FL_NF * p + k];
+    S.dt = dt[p];
+    S.alpha = alpha[p];
+    S.nPos = nPos[p];
+    S.steps = steps[p];
+    S.energy = energy[p];
+    S.epre = epre ? epre[p] : 0.0;
+    const double* f = fs + FS_NF * p;
+    S.fsScale = f[FS_SCALE];
+    S.fsOneMa = f[FS_ONEMA];
+    S.fsMaxF = f[FS_MAXF];
+    S.fsMaxFB = f[FS_MAXFB];
+    return S;
+}
+__device__ __forceinline__ void fire_store(int p, const FireRegs& S, int* active, int* flags, double* dt,
+                                           double* alpha, int* nPos, int* steps, double* energy, double* fs) {
+    active[p] = S.act;
+#pragma unroll
+    for (int k = 0; k < FL_NF; ++k) flags[FL_NF * p + k] = S.fl[k];
+    dt[p] = S.dt;
+    alpha[p] = S.alpha;
+    nPos[p] = S.nPos;
+    steps[p] = S.steps;
+    energy[p] = S.energy;
+    double* f = fs + FS_NF * p;
+    f[FS_SCALE] = S.fsScale;
+    f[FS_ONEMA] = S.fsOneMa;
+    f[FS_MAXF] = S.fsMaxF;
+}
+
+// energy test (model.cpp:444-470) on the packing's new energy S.energy
+__device__ __forceinline__ void fire_control(FireRegs& S, double alphaStart, double fDec) {
+    if (!S.act) return;
+    double ePost = S.energy, ePre = S.epre;
+    if (S.fl[FL_FAIL]) {
+        S.dt = S.dt * fDec;
+        S.alpha = alphaStart;
+        S.nPos = 0;
+        S.fl[FL_ACCEPT] = 0;
     } else if (ePost > ePre + fmax(fabs(ePre) * 1e-10, 1e-14)) {
-        fl[FL_ROLLBACK] = 1;
-        fl[FL_ACCEPT] = 0;
-        dt[p] = dt[p] * fDec;
-        alpha[p] = alphaStart;
-        nPos[p] = 0;
+        S.fl[FL_ROLLBACK] = 1;
+        S.fl[FL_ACCEPT] = 0;
+        S.dt = S.dt * fDec;
+        S.alpha = alphaStart;
+        S.nPos = 0;
     } else {
-        fl[FL_ACCEPT] = 1;
+        S.fl[FL_ACCEPT] = 1;
     }
 }
 
 // block per packing: energy = fixed-order sum of the polygon energies, then the test
 __global__ __launch_bounds__(BLK) void k_fire_energy(int N, const int* __restrict__ packPoly,
                                                     const double* __restrict__ epoly, double* __restrict__ energy,
-                                                    const int* __restrict__ active, const double* __restrict__ epre,
-                                                    double alphaStart, double fDec, double* dt, double* alpha,
-                                                    int* nPos, int* flags, const int* __restrict__ xpbdIt,
+                                                    int* __restrict__ active, const double* __restrict__ epre,
+                                                    double alphaStart, double fDec, double* __restrict__ dt,
+                                                    double* __restrict__ alpha, int* __restrict__ nPos,
+                                                    int* __restrict__ flags, int* __restrict__ steps,
+                                                    double* __restrict__ fs, const int* __restrict__ xpbdIt,
                                                     int keepDone) {
     __shared__ double part[BLK + 1];
     int p = blockIdx.x;
     // keepDone: the evaluation skipped the converged packings, whose energy stays as FIRE left it
     if (keepDone && !active[p]) return;   // the whole workgroup
     int a0 = packPoly[p], n = packPoly[p + 1] - a0;
+    FireRegs S;
+    int xit = 0;
+    if (threadIdx.x == 0) {
+        S = fire_load(p, active, flags, dt, alpha, nPos, steps, energy, epre, fs);
+        if (xpbdIt) xit = xpbdIt[p];
+    }
     double e = det_sum_block(epoly + a0, n, part);
     if (threadIdx.x == 0) {
-        energy[p] = e;
-        if (xpbdIt && active[p]) flags[FL_NF * p + FL_LASTSHAKE] = xpbdIt[p];   // XPBD iterations
-        fire_control(p, active, energy, epre, alphaStart, fDec, dt, alpha, nPos, flags);
+        S.energy = e;
+        if (xpbdIt && S.act) S.fl[FL_LASTSHAKE] = xit;   // XPBD iterations
+        fire_control(S, alphaStart, fDec);
+        fire_store(p, S, active, flags, dt, alpha, nPos, steps, energy, fs);
     }
 }
 
@@ -267,31 +318,29 @@ __global__ __launch_bounds__(BLK) void k_fire_kick_seg(int N, const int* __restr
 }
 
 // bend + P>0 logic (model.cpp:471-485)
-__device__ void fire_control2(int p, const int* __restrict__ active, double Pw, double vv, double ff,
-                              int nMin, double dtMax, double alphaStart, double fAlpha, double fInc,
-                              double fDec, double* dt, double* alpha, int* nPos, int* flags, double* fs) {
-    if (!active[p]) return;
-    int* fl = flags + FL_NF * p;
-    if (!fl[FL_ACCEPT]) return;
+__device__ __forceinline__ void fire_control2(FireRegs& S, double Pw, double vv, double ff, int nMin, double dtMax,
+                                              double alphaStart, double fAlpha, double fInc, double fDec) {
+    if (!S.act) return;
+    if (!S.fl[FL_ACCEPT]) return;
     double vnorm = sqrt(vv), fnorm = sqrt(ff);
-    double a = alpha[p];
-    fs[FS_NF * p + FS_SCALE] = (fnorm > 0.0) ? (a * vnorm) / fnorm : 0.0;
-    fs[FS_NF * p + FS_ONEMA] = 1.0 - a;
-    double h = dt[p];
+    double a = S.alpha;
+    S.fsScale = (fnorm > 0.0) ? (a * vnorm) / fnorm : 0.0;
+    S.fsOneMa = 1.0 - a;
+    double h = S.dt;
     if (Pw > 0.0) {
-        fl[FL_ZEROV] = 0;
-        int np = nPos[p] + 1;
-        nPos[p] = np;
+        S.fl[FL_ZEROV] = 0;
+        int np = S.nPos + 1;
+        S.nPos = np;
         if (np >= nMin) {
             double newDt = fmin(h * fInc, dtMax);
-            if (newDt > h) alpha[p] = alpha[p] * fAlpha;
-            dt[p] = newDt;
+            if (newDt > h) S.alpha = S.alpha * fAlpha;
+            S.dt = newDt;
         }
     } else {
-        fl[FL_ZEROV] = 1;
-        dt[p] = h * fDec;
-        alpha[p] = alphaStart;
-        nPos[p] = 0;
+        S.fl[FL_ZEROV] = 1;
+        S.dt = h * fDec;
+        S.alpha = alphaStart;
+        S.nPos = 0;
     }
 }
 
@@ -353,33 +402,29 @@ __global__ void k_fire_snapshot(int V, int P, int save, double2* __restrict__ x,
 // convergence on max |force component| (model.cpp:500), step count, status block.
 // staleRollback: a rolled-back packing's forces are not yet recomputed, so it cannot
 // converge in this pass.
-__device__ void fire_finish_p(int p, int* __restrict__ active, const int* __restrict__ flags, double mf,
-                              double fTol, int* __restrict__ steps, const double* __restrict__ dt,
-                              const double* __restrict__ alpha, const int* __restrict__ nPos,
-                              const double* __restrict__ energy, double* __restrict__ fs,
-                              pcp_fire_status* __restrict__ status, int* __restrict__ summary, int countStep,
-                              int staleRollback) {
-    const int* fl = flags + FL_NF * p;
-    if (active[p]) {
-        fs[FS_NF * p + FS_MAXF] = mf;
-        if (countStep) steps[p] = steps[p] + 1;
-        bool conv = (mf <= fTol) && !(staleRollback && fl[FL_ROLLBACK]);
-        if (conv) active[p] = 0;
-        if (fl[FL_ROLLBACK]) atomicOr(&summary[0], 1);
+__device__ __forceinline__ void fire_finish(int p, FireRegs& S, double mf, double fTol,
+                                            pcp_fire_status* __restrict__ status, int* __restrict__ summary,
+                                            int countStep, int staleRollback) {
+    if (S.act) {
+        S.fsMaxF = mf;
+        if (countStep) S.steps = S.steps + 1;
+        bool conv = (mf <= fTol) && !(staleRollback && S.fl[FL_ROLLBACK]);
+        if (conv) S.act = 0;
+        if (S.fl[FL_ROLLBACK]) atomicOr(&summary[0], 1);
         if (!conv) atomicOr(&summary[1], 1);
     }
-    pcp_fire_status s;
-    s.dt = dt[p];
-    s.alpha = alpha[p];
-    s.energy = energy[p];
-    s.maxForce = fs[FS_NF * p + FS_MAXF];
-    s.nPos = nPos[p];
-    s.steps = steps[p];
-    s.converged = active[p] ? 0 : 1;
-    s.lastShakeIters = fl[FL_LASTSHAKE];
-    s.shakeFailed = fl[FL_FAIL];
-    s.rolledBack = fl[FL_ROLLBACK];
-    status[p] = s;
+    pcp_fire_status st;
+    st.dt = S.dt;
+    st.alpha = S.alpha;
+    st.energy = S.energy;
+    st.maxForce = S.fsMaxF;
+    st.nPos = S.nPos;
+    st.steps = S.steps;
+    st.converged = S.act ? 0 : 1;
+    st.lastShakeIters = S.fl[FL_LASTSHAKE];
+    st.shakeFailed = S.fl[FL_FAIL];
+    st.rolledBack = S.fl[FL_ROLLBACK];
+    status[p] = st;
 }
 
 // Block per packing, after the kick: fixed-order sums of F.v, v.v, F.F and the max
@@ -390,7 +435,8 @@ __global__ __launch_bounds__(BLK) void k_fire_stats(int N, const int* __restrict
                                                    const double* __restrict__ red, int full, int nMin,
                                                    double dtMax, double alphaStart, double fAlpha, double fInc,
                                                    double fDec, double fTol, int* __restrict__ active,
-                                                   int* __restrict__ flags, double* dt, double* alpha, int* nPos,
+                                                   int* __restrict__ flags, double* __restrict__ dt,
+                                                   double* __restrict__ alpha, int* __restrict__ nPos,
                                                    int* __restrict__ steps, double* __restrict__ energy,
                                                    double* __restrict__ fs, pcp_fire_status* __restrict__ status,
                                                    int* __restrict__ summary, const int* __restrict__ scal,
@@ -399,6 +445,8 @@ __global__ __launch_bounds__(BLK) void k_fire_stats(int N, const int* __restrict
     __shared__ double part[3 * (BLK + 1)];
     int p = blockIdx.x;
     int a0 = packPoly[p], n = packPoly[p + 1] - a0;
+    FireRegs S;
+    if (threadIdx.x == 0) S = fire_load(p, active, flags, dt, alpha, nPos, steps, energy, epre, fs);
     double Pw = 0.0, vv = 0.0, ff = 0.0, mf;
     if (full) {   // F.v, v.v, F.F and max |F| in one pass
         double r[4];
@@ -410,16 +458,15 @@ __global__ __launch_bounds__(BLK) void k_fire_stats(int N, const int* __restrict
         mf = r[0];
     }
     if (threadIdx.x == 0) {
-        if (full)
-            fire_control2(p, active, Pw, vv, ff, nMin, dtMax, alphaStart, fAlpha, fInc, fDec, dt, alpha, nPos,
-                          flags, fs);
+        if (full) fire_control2(S, Pw, vv, ff, nMin, dtMax, alphaStart, fAlpha, fInc, fDec);
         // device-side rollback (epre given): the step's start state comes back from its
         // backups -- energy here, x, v and F in k_fire_bend -- as re-evaluating it would give
-        if (full && epre && active[p] && flags[FL_NF * p + FL_ROLLBACK]) {
-            energy[p] = epre[p];
-            mf = fs[FS_NF * p + FS_MAXFB];
+        if (full && epre && S.act && S.fl[FL_ROLLBACK]) {
+            S.energy = S.epre;
+            mf = S.fsMaxFB;
         }
-        fire_finish_p(p, active, flags, mf, fTol, steps, dt, alpha, nPos, energy, fs, status, summary, full, full);
+        fire_finish(p, S, mf, fTol, status, summary, full, full);
+        fire_store(p, S, active, flags, dt, alpha, nPos, steps, energy, fs);
         if (p == 0) {
             summary[2] = scal[SC_BUCKET];
             if (blockOver && scal[SC_BUCKET] > blockOver[0]) blockOver[0] = scal[SC_BUCKET];
@@ -440,8 +487,9 @@ __global__ __launch_bounds__(BLK) void k_fire_small(
     int N, const int* __restrict__ packPoly, const int* __restrict__ packVert, const int* __restrict__ polyStart,
     const double* __restrict__ epoly, double* __restrict__ energy, int* __restrict__ active,
     const double* __restrict__ epre, const int* __restrict__ xpbdIt, int nMin, double dtMax, double alphaStart,
-    double fAlpha, double fInc, double fDec, double fTol, double* dt, double* alpha, int* nPos,
-    int* __restrict__ flags, int* __restrict__ steps, double* __restrict__ fs, pcp_fire_status* __restrict__ status,
+    double fAlpha, double fInc, double fDec, double fTol, double* __restrict__ dt, double* __restrict__ alpha,
+    int* __restrict__ nPos, int* __restrict__ flags, int* __restrict__ steps, double* __restrict__ fs,
+    pcp_fire_status* __restrict__ status,
     int* __restrict__ summary, const int* __restrict__ scal, int rollbackOnDevice, int* __restrict__ blockOver,
     double* __restrict__ red, double2* __restrict__ x, double2* __restrict__ v, double2* __restrict__ F,
     const double2* __restrict__ xbak, const double2* __restrict__ fbak, double2* __restrict__ Fe,
@@ -450,12 +498,21 @@ __global__ __launch_bounds__(BLK) void k_fire_small(
     __shared__ double L[BLK / 64];
     int p = blockIdx.x;
     int a0 = packPoly[p], n = packPoly[p + 1] - a0;
+    // thread 0 holds the packing's FIRE scalars from here to the stats (written back before each
+    // barrier after which other threads read them)
+    FireRegs S;
+    int xit = 0;
+    if (threadIdx.x == 0) {
+        S = fire_load(p, active, flags, dt, alpha, nPos, steps, energy, epre, fs);
+        if (xpbdIt) xit = xpbdIt[p];
+    }
     // energy = fixed-order sum of the polygon energies, then its test (keepDone: as k_fire_energy)
     double en = det_sum_block(epoly + a0, n, part);
     if (threadIdx.x == 0) {
-        if (!keepDone || active[p]) energy[p] = en;
-        if (xpbdIt && active[p]) flags[FL_NF * p + FL_LASTSHAKE] = xpbdIt[p];   // XPBD iterations
-        fire_control(p, active, energy, epre, alphaStart, fDec, dt, alpha, nPos, flags);
+        if (!keepDone || S.act) S.energy = en;
+        if (xpbdIt && S.act) S.fl[FL_LASTSHAKE] = xit;   // XPBD iterations
+        fire_control(S, alphaStart, fDec);
+        fire_store(p, S, active, flags, dt, alpha, nPos, steps, energy, fs);
     }
     __syncthreads();
     // second half-kick and per-polygon sums in vertex order (as k_fire_kick / k_fire_kick_seg), a
@@ -504,14 +561,14 @@ __global__ __launch_bounds__(BLK) void k_fire_small(
     double r[4];
     det_reduce_block<3, true>(red + a0, N, n, part, L, r);
     if (threadIdx.x == 0) {
-        fire_control2(p, active, r[0], r[1], r[2], nMin, dtMax, alphaStart, fAlpha, fInc, fDec, dt, alpha, nPos,
-                      flags, fs);
+        fire_control2(S, r[0], r[1], r[2], nMin, dtMax, alphaStart, fAlpha, fInc, fDec);
         double mfp = r[3];
-        if (rollbackOnDevice && active[p] && flags[FL_NF * p + FL_ROLLBACK]) {
-            energy[p] = epre[p];
-            mfp = fs[FS_NF * p + FS_MAXFB];
+        if (rollbackOnDevice && S.act && S.fl[FL_ROLLBACK]) {
+            S.energy = S.epre;
+            mfp = S.fsMaxFB;
         }
-        fire_finish_p(p, active, flags, mfp, fTol, steps, dt, alpha, nPos, energy, fs, status, summary, 1, 1);
+        fire_finish(p, S, mfp, fTol, status, summary, 1, 1);
+        fire_store(p, S, active, flags, dt, alpha, nPos, steps, energy, fs);
         if (p == 0) {
             summary[2] = scal[SC_BUCKET];
             if (blockOver && scal[SC_BUCKET] > blockOver[0]) blockOver[0] = scal[SC_BUCKET];
@@ -709,8 +766,8 @@ static int fire_eval(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm
                        st->perimStiffness));
     if (!energyTest) return PCP_OK;
     PCP_LAUNCH(c, k_fire_energy, dim3(c->P), dim3(BLK), c->N, c->dPackPoly, c->dEpoly, st->energy, c->dActive,
-               c->dEpre, prm->alphaStart, prm->fDec, c->dDt, c->dAlpha, c->dNPos, c->dFlags,
-               xpbd ? (const int*)c->dShakeIt : (const int*)nullptr, skip_done(c) ? 1 : 0);
+               c->dEpre, prm->alphaStart, prm->fDec, c->dDt, c->dAlpha, c->dNPos, c->dFlags, c->dSteps,
+               c->dFireScal, xpbd ? (const int*)c->dShakeIt : (const int*)nullptr, skip_done(c) ? 1 : 0);
     return PCP_OK;
 }
 
