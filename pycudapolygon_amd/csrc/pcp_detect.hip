@@ -1095,10 +1095,14 @@ __device__ void tile_pairs(TileLds& L, int lane, int nU, int A, int stA, int nA,
             wave_lds_sync();   // the list's reads are issued before it is refilled
             nmay = 0;
         };
-        // the edges a of A this lane's edge is tested against, as a bit mask: every edge of A, or
-        // for A itself the non-adjacent ones below b (a < b - 1, and not edge 0 with edge nA - 1)
-        unsigned long long allow = 0ull;
-        if (act) allow = self ? (bits_below(b - 1) & ~((b == nA - 1) ? 1ull : 0ull)) : bits_below(nA);
+        // the edges a of A this lane's edge is tested against, aLo <= a < aLo + aSpan: every edge of
+        // A, or for A itself the non-adjacent ones below b (a < b - 1, and not edge 0 with edge
+        // nA - 1); one unsigned comparison per edge of A in the hot loop
+        int aLo = 0, aSpan = 0;
+        if (act) {
+            aLo = (self && b == nA - 1) ? 1 : 0;
+            aSpan = self ? max(b - 1 - aLo, 0) : nA;
+        }
         // every edge a of A against every lane's edge: the pairs whose boxes meet are collected
         for (int a0 = 0; a0 < nA; a0 += TILE_UNROLL) {
             bool may[TILE_UNROLL];
@@ -1107,7 +1111,7 @@ __device__ void tile_pairs(TileLds& L, int lane, int nU, int A, int stA, int nA,
             for (int k = 0; k < TILE_UNROLL; ++k) {
                 int a = a0 + k;
                 float4 ab = lane_box(aBox, (a < nA) ? a : 0);   // A's edge a, from lane a
-                may[k] = a < nA && ((allow >> a) & 1ull) && boxes_meet(ab, bb);   // a < nA <= 64
+                may[k] = a < nA && (unsigned)(a - aLo) < (unsigned)aSpan && boxes_meet(ab, bb);
                 anyMay = anyMay || may[k];
             }
             if (__ballot(anyMay) == 0ull) continue;   // wavefront-uniform: most blocks find nothing
