@@ -82,3 +82,45 @@ def test_converged_packings_skipped_without_changing_trajectories(model, n, phis
         assert one.steps[0] == both.steps[k]
         assert one.energy[0] == both.energy[k] and one.max_force[0] == both.max_force[k]
         np.testing.assert_array_equal(one.positions[0], both.positions[k])
+
+
+def test_packings_of_a_full_context_equal_their_own_contexts():
+    """The evaluation workload at the scale the chip is filled at: 32 packings of N=4096 x 16
+    (reference energy, uniform centres) in one context.  Every packing's records, energy and
+    forces equal those of the packing evaluated alone, bit for bit (the record rows by key,
+    the forces as IEEE bit patterns), under the schedule the context's shape selects (the
+    whole-tile detection)."""
+    from pycudapolygon_amd.generate import make_packing, box_size_for
+    P = 32
+    ps = [make_packing(4096, 16, 0.85, seed=k) for k in range(P)]
+    bs = [box_size_for(q["diameter"]) for q in ps]
+    print(f"[batch] {P} x N=4096 x 16 in one context", flush=True)
+    s = capi.System(ps, bs)
+    for _ in range(4):   # the first evaluations may enlarge the record buckets
+        try:
+            s.force_eval()
+            s.sync()
+        except capi.PcpError:
+            pass
+    s.force_eval()
+    s.sync()
+    E = s.get_energy()
+    for k in (0, 7, 19, 31):
+        print(f"[batch] packing {k} alone", flush=True)
+        one = capi.System([ps[k]], [bs[k]])
+        for _ in range(3):
+            try:
+                one.force_eval()
+                one.sync()
+            except capi.PcpError:
+                pass
+        one.force_eval()
+        one.sync()
+        assert E[k] == one.get_energy()[0]
+        assert np.array_equal(s.get_forces(k).view(np.uint64), one.get_forces(0).view(np.uint64))
+        ra, rb = s.get_records(k), one.get_records(0)
+        assert ra["numIntersections"] == rb["numIntersections"] > 0
+        for key in ("intersections", "outersections"):
+            assert np.array_equal(np.sort(np.asarray(ra[key])), np.sort(np.asarray(rb[key])))
+        one.close()
+    s.close()
