@@ -622,7 +622,7 @@ def main():
                         "each scaled by k_copy's true bytes / reading")
             except Exception as e:   # profiler missing or refused: keep the cited figure
                 print(f"[bench] traffic measurement failed ({e}); using the cited figure", flush=True)
-        counters = ("profiles/k_detect_tile_N4096.md" if detect_kernel == "k_detect_tile"
+        counters = ("profiles/force_eval_counters_N4096_and_32xN4096.md" if detect_kernel == "k_detect_tile"
                     else "profiles/k_detect_poly_counters_N4096.md")
         roof = dict(bound="hbm", achieved=gbs, peak=HBM_PEAK_GBS, unit="GB/s", frac=gbs / HBM_PEAK_GBS,
                     traffic=traffic, traffic_source=tsrc, traffic_counters=tdetail,
