@@ -187,7 +187,7 @@ __device__ __forceinline__ void fire_control(FireRegs& S, double alphaStart, dou
 }
 
 // block per packing: energy = fixed-order sum of the polygon energies, then the test
-__global__ __launch_bounds__(BLK) void k_fire_energy(int N, const int* __restrict__ packPoly,
+__global__ __launch_bounds__(REDUCE_BLK) void k_fire_energy(int N, const int* __restrict__ packPoly,
                                                     const double* __restrict__ epoly, double* __restrict__ energy,
                                                     int* __restrict__ active, const double* __restrict__ epre,
                                                     double alphaStart, double fDec, double* __restrict__ dt,
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(BLK) void k_fire_energy(int N, const int* __restric
                                                     int* __restrict__ flags, int* __restrict__ steps,
                                                     double* __restrict__ fs, const int* __restrict__ xpbdIt,
                                                     int keepDone) {
-    __shared__ double part[BLK + 1];
+    __shared__ double part[REDUCE_BLK + 1];
     int p = blockIdx.x;
     // keepDone: the evaluation skipped the converged packings, whose energy stays as FIRE left it
     if (keepDone && !active[p]) return;   // the whole workgroup
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(BLK) void k_fire_energy(int N, const int* __restric
         S = fire_load(p, active, flags, dt, alpha, nPos, steps, energy, epre, fs);
         if (xpbdIt) xit = xpbdIt[p];
     }
-    double e = det_sum_block(epoly + a0, n, part);
+    double e = det_sum_block<REDUCE_BLK>(epoly + a0, n, part);
     if (threadIdx.x == 0) {
         S.energy = e;
         if (xpbdIt && S.act) S.fl[FL_LASTSHAKE] = xit;   // XPBD iterations
@@ -431,7 +431,7 @@ __device__ __forceinline__ void fire_finish(int p, FireRegs& S, double mf, doubl
 // |force component|, then (full) the dt / alpha update (model.cpp:471-485) and the
 // convergence test, or (full = 0, after a rollback's fresh forces) the test only.
 // Block 0 also reports the largest record bucket count of the step in summary[2].
-__global__ __launch_bounds__(BLK) void k_fire_stats(int N, const int* __restrict__ packPoly,
+__global__ __launch_bounds__(REDUCE_BLK) void k_fire_stats(int N, const int* __restrict__ packPoly,
                                                    const double* __restrict__ red, int full, int nMin,
                                                    double dtMax, double alphaStart, double fAlpha, double fInc,
                                                    double fDec, double fTol, int* __restrict__ active,
@@ -441,8 +441,8 @@ __global__ __launch_bounds__(BLK) void k_fire_stats(int N, const int* __restrict
                                                    double* __restrict__ fs, pcp_fire_status* __restrict__ status,
                                                    int* __restrict__ summary, const int* __restrict__ scal,
                                                    const double* __restrict__ epre, int* __restrict__ blockOver) {
-    __shared__ double L[BLK / 64];
-    __shared__ double part[3 * (BLK + 1)];
+    __shared__ double L[REDUCE_BLK / 64];
+    __shared__ double part[3 * (REDUCE_BLK + 1)];
     int p = blockIdx.x;
     int a0 = packPoly[p], n = packPoly[p + 1] - a0;
     FireRegs S;
@@ -450,11 +450,11 @@ __global__ __launch_bounds__(BLK) void k_fire_stats(int N, const int* __restrict
     double Pw = 0.0, vv = 0.0, ff = 0.0, mf;
     if (full) {   // F.v, v.v, F.F and max |F| in one pass
         double r[4];
-        det_reduce_block<3, true>(red + a0, N, n, part, L, r);
+        det_reduce_block<3, true, REDUCE_BLK>(red + a0, N, n, part, L, r);
         Pw = r[0]; vv = r[1]; ff = r[2]; mf = r[3];
     } else {
         double r[1];
-        det_reduce_block<0, true>(red + 3LL * N + a0, N, n, part, L, r);
+        det_reduce_block<0, true, REDUCE_BLK>(red + 3LL * N + a0, N, n, part, L, r);
         mf = r[0];
     }
     if (threadIdx.x == 0) {
@@ -765,7 +765,7 @@ static int fire_eval(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm
                        st->stiffness, st->compressibility, (double2*)st->force, nullptr, 1, rigid, false,
                        st->perimStiffness));
     if (!energyTest) return PCP_OK;
-    PCP_LAUNCH(c, k_fire_energy, dim3(c->P), dim3(BLK), c->N, c->dPackPoly, c->dEpoly, st->energy, c->dActive,
+    PCP_LAUNCH(c, k_fire_energy, dim3(c->P), dim3(REDUCE_BLK), c->N, c->dPackPoly, c->dEpoly, st->energy, c->dActive,
                c->dEpre, prm->alphaStart, prm->fDec, c->dDt, c->dAlpha, c->dNPos, c->dFlags, c->dSteps,
                c->dFireScal, xpbd ? (const int*)c->dShakeIt : (const int*)nullptr, skip_done(c) ? 1 : 0);
     return PCP_OK;
@@ -808,7 +808,7 @@ static int fire_step_launch(pcp_ctx* c, const pcp_state* st, const pcp_fire_para
     PCP_TRY(fire_eval(c, st, prm));
     if (xpbd) PCP_TRY(launch_fire_kick<true>(c, (double2*)st->vel, (const double2*)st->force));
     else PCP_TRY(launch_fire_kick<false>(c, (double2*)st->vel, (const double2*)st->force));
-    PCP_LAUNCH(c, k_fire_stats, dim3(c->P), dim3(BLK), c->N, c->dPackPoly, c->dRed, 1, prm->nMin, prm->dtMax,
+    PCP_LAUNCH(c, k_fire_stats, dim3(c->P), dim3(REDUCE_BLK), c->N, c->dPackPoly, c->dRed, 1, prm->nMin, prm->dtMax,
                prm->alphaStart, prm->fAlpha, prm->fInc, prm->fDec, prm->fTol, c->dActive, c->dFlags, c->dDt, c->dAlpha,
                c->dNPos, c->dSteps, st->energy, c->dFireScal, c->dStatus, c->dSummary, c->dScal,
                block ? (const double*)c->dEpre : nullptr, block ? c->dBlockOver : nullptr);
@@ -845,7 +845,7 @@ static int fire_step_impl(pcp_ctx* c, const pcp_state* st, const pcp_fire_params
             PCP_TRY(pcp_force_eval(c, st));
             PCP_LAUNCH(c, k_max_abs_poly, dim3(nblk(c->N)), dim3(BLK), c->N, c->dPolyStart,
                        (const double2*)st->force, c->dRed + 3LL * c->N);
-            PCP_LAUNCH(c, k_fire_stats, dim3(c->P), dim3(BLK), c->N, c->dPackPoly, c->dRed, 0, prm->nMin,
+            PCP_LAUNCH(c, k_fire_stats, dim3(c->P), dim3(REDUCE_BLK), c->N, c->dPackPoly, c->dRed, 0, prm->nMin,
                        prm->dtMax, prm->alphaStart, prm->fAlpha, prm->fInc, prm->fDec, prm->fTol, c->dActive,
                        c->dFlags, c->dDt, c->dAlpha, c->dNPos, c->dSteps, st->energy, c->dFireScal, c->dStatus,
                        c->dSummary, c->dScal, (const double*)nullptr, (int*)nullptr);

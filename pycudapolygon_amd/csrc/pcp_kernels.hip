@@ -528,6 +528,8 @@ __global__ void k_sort_keys(int n, uint64_t mask, const uint64_t* __restrict__ i
 // wavefronts take DR_BATCH chunks at a time, all their loads issued first.  Result valid
 // in every thread.
 constexpr int DR_BATCH = 4;
+// block of the per-packing reduction kernels (k_det_reduce, k_fire_energy, k_fire_stats)
+constexpr int REDUCE_BLK = 1024;
 // the tree of one chunk: lane l holds v[l], v[l+64], v[l+128], v[l+192]; level 128
 // (v[l] + v[l+128], v[l+64] + v[l+192]) and level 64 inside the lane, levels 32..1 by
 // shuffles (lane t adds lane t + s); the same additions in the same order.  Sum in lane 0.
@@ -539,20 +541,24 @@ __device__ __forceinline__ double chunk_tree(double v0, double v1, double v2, do
     for (int st = 32; st >= 1; st >>= 1) x += __shfl_down(x, st);
     return x;
 }
+// NB threads per block; with NB >= 1024 a wavefront takes one chunk per pass (a 4,096-polygon
+// packing's 16 chunks are then evaluated by 16 wavefronts side by side)
+template <int NB = BLK>
 __device__ double det_sum_block(const double* __restrict__ xa, int n, double* part) {
+    constexpr int DRB = (NB >= 1024) ? 1 : DR_BATCH;
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int chunks = (n + 255) >> 8;
-    for (int c0 = w * DR_BATCH; c0 < chunks; c0 += (BLK / 64) * DR_BATCH) {
-        double v[DR_BATCH][4];
+    for (int c0 = w * DRB; c0 < chunks; c0 += (NB / 64) * DRB) {
+        double v[DRB][4];
 #pragma unroll
-        for (int u = 0; u < DR_BATCH; ++u)
+        for (int u = 0; u < DRB; ++u)
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 int q = (c0 + u) * 256 + lane + 64 * k;
                 v[u][k] = (q < n) ? xa[q] : 0.0;
             }
 #pragma unroll
-        for (int u = 0; u < DR_BATCH; ++u)
+        for (int u = 0; u < DRB; ++u)
             if (c0 + u < chunks) {   // wavefront-uniform
                 double x = chunk_tree(v[u][0], v[u][1], v[u][2], v[u][3]);
                 if (lane == 0) part[c0 + u] = x;
@@ -567,25 +573,26 @@ __device__ double det_sum_block(const double* __restrict__ xa, int n, double* pa
             double x = chunk_tree(lane < chunks ? part[lane] : 0.0, lane + 64 < chunks ? part[lane + 64] : 0.0,
                                   lane + 128 < chunks ? part[lane + 128] : 0.0,
                                   lane + 192 < chunks ? part[lane + 192] : 0.0);
-            if (lane == 0) part[BLK] = x;
+            if (lane == 0) part[NB] = x;
         }
         __syncthreads();
-        res = part[BLK];
+        res = part[NB];
     }
     __syncthreads();
     return res;
 }
 // maximum of n values (exact in any order); valid in every thread
+template <int NB = BLK>
 __device__ double max_block(const double* __restrict__ xa, int n, double* L) {
     int t = threadIdx.x;
     double m = 0.0;
-    for (int q = t; q < n; q += BLK) m = fmax(m, xa[q]);
+    for (int q = t; q < n; q += NB) m = fmax(m, xa[q]);
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o));
     if ((t & 63) == 0) L[t >> 6] = m;
     __syncthreads();
     double r = 0.0;
-    for (int q = 0; q < BLK / 64; ++q) r = fmax(r, L[q]);
+    for (int q = 0; q < NB / 64; ++q) r = fmax(r, L[q]);
     __syncthreads();
     return r;
 }
@@ -594,19 +601,20 @@ __device__ double max_block(const double* __restrict__ xa, int n, double* L) {
 // and with MX the maximum of array NS, all from one pass whose loads are issued together
 // (one memory latency instead of one per array).  part: NS * (BLK + 1) doubles, L: BLK / 64.
 // res[a] = sum of array a; res[NS] = the maximum.  Valid in every thread.
-template <int NS, bool MX>
+template <int NS, bool MX, int NB = BLK>
 __device__ void det_reduce_block(const double* __restrict__ xa, long long stride, int n, double* part, double* L,
                                  double* res) {
     constexpr int NA = NS + (MX ? 1 : 0);
+    constexpr int DRB = (NB >= 1024) ? 1 : DR_BATCH;
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int chunks = (n + 255) >> 8;
     double mx = 0.0;
-    for (int c0 = w * DR_BATCH; c0 < chunks; c0 += (BLK / 64) * DR_BATCH) {
-        double v[NA][DR_BATCH][4];
+    for (int c0 = w * DRB; c0 < chunks; c0 += (NB / 64) * DRB) {
+        double v[NA][DRB][4];
 #pragma unroll
         for (int a = 0; a < NA; ++a)
 #pragma unroll
-            for (int u = 0; u < DR_BATCH; ++u)
+            for (int u = 0; u < DRB; ++u)
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     int q = (c0 + u) * 256 + lane + 64 * k;
@@ -615,14 +623,14 @@ __device__ void det_reduce_block(const double* __restrict__ xa, long long stride
 #pragma unroll
         for (int a = 0; a < NS; ++a)
 #pragma unroll
-            for (int u = 0; u < DR_BATCH; ++u)
+            for (int u = 0; u < DRB; ++u)
                 if (c0 + u < chunks) {   // wavefront-uniform
                     double x = chunk_tree(v[a][u][0], v[a][u][1], v[a][u][2], v[a][u][3]);
-                    if (lane == 0) part[a * (BLK + 1) + c0 + u] = x;
+                    if (lane == 0) part[a * (NB + 1) + c0 + u] = x;
                 }
         if constexpr (MX) {
 #pragma unroll
-            for (int u = 0; u < DR_BATCH; ++u)
+            for (int u = 0; u < DRB; ++u)
 #pragma unroll
                 for (int k = 0; k < 4; ++k) mx = fmax(mx, v[NS][u][k]);
         }
@@ -634,19 +642,19 @@ __device__ void det_reduce_block(const double* __restrict__ xa, long long stride
     }
     __syncthreads();
     if (chunks > 1 && w < NS) {   // wavefront a: the top tree of array a
-        double* pa = part + w * (BLK + 1);
+        double* pa = part + w * (NB + 1);
         double x = chunk_tree(lane < chunks ? pa[lane] : 0.0, lane + 64 < chunks ? pa[lane + 64] : 0.0,
                               lane + 128 < chunks ? pa[lane + 128] : 0.0, lane + 192 < chunks ? pa[lane + 192] : 0.0);
-        if (lane == 0) pa[BLK] = x;
+        if (lane == 0) pa[NB] = x;
     }
-    static_assert(NS <= BLK / 64, "one wavefront per top tree");
+    static_assert(NS <= NB / 64, "one wavefront per top tree");
     __syncthreads();
 #pragma unroll
     for (int a = 0; a < NS; ++a)
-        res[a] = (chunks > 1) ? part[a * (BLK + 1) + BLK] : (chunks == 1 ? part[a * (BLK + 1)] : 0.0);
+        res[a] = (chunks > 1) ? part[a * (NB + 1) + NB] : (chunks == 1 ? part[a * (NB + 1)] : 0.0);
     if constexpr (MX) {
         double r = 0.0;
-        for (int q = 0; q < BLK / 64; ++q) r = fmax(r, L[q]);
+        for (int q = 0; q < NB / 64; ++q) r = fmax(r, L[q]);
         res[NS] = r;
     }
     __syncthreads();
@@ -655,26 +663,26 @@ __device__ void det_reduce_block(const double* __restrict__ xa, long long stride
 // per-packing reductions over polygons: nArrays arrays of N values (stride N) are summed
 // with det_sum_block; the last nMax of them are max-reduced instead.  Block per packing.
 // With stickyScal, block 0 also records whether the evaluation overflowed a record bucket.
-__global__ __launch_bounds__(BLK) void k_det_reduce(const int* __restrict__ packPoly,
+__global__ __launch_bounds__(REDUCE_BLK) void k_det_reduce(const int* __restrict__ packPoly,
                                                    const double* __restrict__ x, int N, int nArrays,
                                                    int nMax, double* __restrict__ out, int P,
                                                    const int* __restrict__ stickyScal, int Kp,
                                                    int* __restrict__ sticky) {
-    __shared__ double L[BLK / 64];
-    __shared__ double part[BLK + 1];
+    __shared__ double L[REDUCE_BLK / 64];
+    __shared__ double part[REDUCE_BLK + 1];
     int p = blockIdx.x;
     int a0 = packPoly[p], n = packPoly[p + 1] - a0;
     if (stickyScal && p == 0 && threadIdx.x == 0 && stickyScal[SC_BUCKET] > Kp) sticky[0] = 1;
     for (int a = 0; a < nArrays; ++a) {
         const double* xa = x + (long long)a * N + a0;
-        double res = (a >= nArrays - nMax) ? max_block(xa, n, L) : det_sum_block(xa, n, part);
+        double res = (a >= nArrays - nMax) ? max_block<REDUCE_BLK>(xa, n, L) : det_sum_block<REDUCE_BLK>(xa, n, part);
         if (threadIdx.x == 0) out[(long long)a * P + p] = res;
     }
 }
 
 int launch_det_reduce(pcp_ctx* c, const double* partials, int nArrays, double* outSums, int nMax, bool sticky) {
     if (c->P == 0) return PCP_OK;
-    PCP_LAUNCH(c, k_det_reduce, dim3(c->P), dim3(BLK), c->dPackPoly, partials, c->N,
+    PCP_LAUNCH(c, k_det_reduce, dim3(c->P), dim3(REDUCE_BLK), c->dPackPoly, partials, c->N,
                        nArrays, nMax, outSums, c->P, sticky ? c->dScal : nullptr, c->Kp, c->dSticky);
     return PCP_OK;
 }
