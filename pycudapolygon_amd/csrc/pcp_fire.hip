@@ -483,7 +483,7 @@ __global__ __launch_bounds__(REDUCE_BLK) void k_fire_stats(int N, const int* __r
 // small context's step is bound by launch and latency, not by the work of one packing.
 constexpr int FIRE_SMALL_B = 8;
 template <bool NOKICK>
-__global__ __launch_bounds__(BLK) void k_fire_small(
+__global__ __launch_bounds__(REDUCE_BLK) void k_fire_small(
     int N, const int* __restrict__ packPoly, const int* __restrict__ packVert, const int* __restrict__ polyStart,
     const double* __restrict__ epoly, double* __restrict__ energy, int* __restrict__ active,
     const double* __restrict__ epre, const int* __restrict__ xpbdIt, int nMin, double dtMax, double alphaStart,
@@ -494,8 +494,8 @@ __global__ __launch_bounds__(BLK) void k_fire_small(
     double* __restrict__ red, double2* __restrict__ x, double2* __restrict__ v, double2* __restrict__ F,
     const double2* __restrict__ xbak, const double2* __restrict__ fbak, double2* __restrict__ Fe,
     const double2* __restrict__ febak, int keepDone) {
-    __shared__ double part[3 * (BLK + 1)];
-    __shared__ double L[BLK / 64];
+    __shared__ double part[3 * (REDUCE_BLK + 1)];
+    __shared__ double L[REDUCE_BLK / 64];
     int p = blockIdx.x;
     int a0 = packPoly[p], n = packPoly[p + 1] - a0;
     // thread 0 holds the packing's FIRE scalars from here to the stats (written back before each
@@ -507,7 +507,7 @@ __global__ __launch_bounds__(BLK) void k_fire_small(
         if (xpbdIt) xit = xpbdIt[p];
     }
     // energy = fixed-order sum of the polygon energies, then its test (keepDone: as k_fire_energy)
-    double en = det_sum_block(epoly + a0, n, part);
+    double en = det_sum_block<REDUCE_BLK>(epoly + a0, n, part);
     if (threadIdx.x == 0) {
         if (!keepDone || S.act) S.energy = en;
         if (xpbdIt && S.act) S.fl[FL_LASTSHAKE] = xit;   // XPBD iterations
@@ -519,7 +519,7 @@ __global__ __launch_bounds__(BLK) void k_fire_small(
     // thread per polygon, its vertices loaded FIRE_SMALL_B at a time with the loads in flight together
     bool kickP = active[p] && flags[FL_NF * p + FL_ACCEPT];
     double h = dt[p];
-    for (int s = a0 + threadIdx.x; s < a0 + n; s += BLK) {
+    for (int s = a0 + threadIdx.x; s < a0 + n; s += REDUCE_BLK) {
         int st = polyStart[s], nv = polyStart[s + 1] - st;
         double a = 0.0, b = 0.0, c = 0.0, mf = 0.0;
         for (int m0 = 0; m0 < nv; m0 += FIRE_SMALL_B) {
@@ -559,7 +559,7 @@ __global__ __launch_bounds__(BLK) void k_fire_small(
     __syncthreads();
     // the packing's sums of F.v, v.v, F.F and max |F|, dt / alpha, convergence, as k_fire_stats
     double r[4];
-    det_reduce_block<3, true>(red + a0, N, n, part, L, r);
+    det_reduce_block<3, true, REDUCE_BLK>(red + a0, N, n, part, L, r);
     if (threadIdx.x == 0) {
         fire_control2(S, r[0], r[1], r[2], nMin, dtMax, alphaStart, fAlpha, fInc, fDec);
         double mfp = r[3];
@@ -582,32 +582,32 @@ __global__ __launch_bounds__(BLK) void k_fire_small(
         bool accept = fl[FL_ACCEPT], zero = fl[FL_ZEROV], roll = fl[FL_ROLLBACK];
         double sc = fs[FS_NF * p + FS_SCALE], om = fs[FS_NF * p + FS_ONEMA];
         int v0 = packVert[p], v1 = packVert[p + 1];
-        for (int i0 = v0 + threadIdx.x; i0 < v1; i0 += BLK * FIRE_SMALL_B) {
+        for (int i0 = v0 + threadIdx.x; i0 < v1; i0 += REDUCE_BLK * FIRE_SMALL_B) {
             if (accept && !zero) {
                 double2 vq[FIRE_SMALL_B], fq[FIRE_SMALL_B];
 #pragma unroll
                 for (int k = 0; k < FIRE_SMALL_B; ++k) {
-                    int i = i0 + k * BLK;
+                    int i = i0 + k * REDUCE_BLK;
                     vq[k] = (i < v1) ? v[i] : make_double2(0.0, 0.0);
                     fq[k] = (i < v1) ? F[i] : make_double2(0.0, 0.0);
                 }
 #pragma unroll
                 for (int k = 0; k < FIRE_SMALL_B; ++k) {
-                    int i = i0 + k * BLK;
+                    int i = i0 + k * REDUCE_BLK;
                     if (i < v1)
                         v[i] = make_double2(om * vq[k].x + sc * fq[k].x, om * vq[k].y + sc * fq[k].y);
                 }
             } else if (accept) {
 #pragma unroll
                 for (int k = 0; k < FIRE_SMALL_B; ++k) {
-                    int i = i0 + k * BLK;
+                    int i = i0 + k * REDUCE_BLK;
                     if (i < v1) v[i] = make_double2(0.0, 0.0);
                 }
             } else if (roll) {
                 double2 xq[FIRE_SMALL_B], fq[FIRE_SMALL_B], eq[FIRE_SMALL_B];
 #pragma unroll
                 for (int k = 0; k < FIRE_SMALL_B; ++k) {
-                    int i = i0 + k * BLK;
+                    int i = i0 + k * REDUCE_BLK;
                     bool in = i < v1;
                     xq[k] = in ? xbak[i] : make_double2(0.0, 0.0);
                     fq[k] = (in && fbak) ? fbak[i] : make_double2(0.0, 0.0);
@@ -615,7 +615,7 @@ __global__ __launch_bounds__(BLK) void k_fire_small(
                 }
 #pragma unroll
                 for (int k = 0; k < FIRE_SMALL_B; ++k) {
-                    int i = i0 + k * BLK;
+                    int i = i0 + k * REDUCE_BLK;
                     if (i < v1) {
                         x[i] = xq[k];
                         v[i] = make_double2(0.0, 0.0);
@@ -703,7 +703,7 @@ static bool fire_small(const pcp_ctx* c) {
 template <bool NOKICK>
 static int launch_fire_small(pcp_ctx* c, const pcp_state* st, const pcp_fire_params* prm, int block,
                              const double2* fbak, const double2* febak) {
-PCP_LAUNCH(c, (k_fire_small<NOKICK>), dim3(c->P), dim3(BLK), c->N, c->dPackPoly, c->dPackVert, c->dPolyStart,
+PCP_LAUNCH(c, (k_fire_small<NOKICK>), dim3(c->P), dim3(REDUCE_BLK), c->N, c->dPackPoly, c->dPackVert, c->dPolyStart,
                c->dEpoly, st->energy, c->dActive, c->dEpre, NOKICK ? (const int*)c->dShakeIt : (const int*)nullptr,
                prm->nMin, prm->dtMax, prm->alphaStart, prm->fAlpha, prm->fInc, prm->fDec, prm->fTol, c->dDt,
                c->dAlpha, c->dNPos, c->dFlags, c->dSteps, c->dFireScal, c->dStatus, c->dSummary, c->dScal, block,
